@@ -1,0 +1,111 @@
+/* ppf_mi355x.h -- C ABI of the MI355X-native per-Newton-step hot path.
+ *
+ * All pointers named d_* are device (HBM) pointers; sizes are element counts.
+ * Every function returns 0 (PPF_OK) on success or a PPF_ERR_* code, with a
+ * message in ppf_last_error(). `stream` is a hipStream_t (NULL = null stream).
+ * 3x3 blocks are 9 floats, column-major, as the reference's Mat3x3f
+ * (crates/ppf-cts-solver/src/cpp/linalg/smat.hpp).
+ *
+ * Block-sparse matrices follow the reference's FixedCSRMat semantics
+ * (crates/ppf-cts-solver/src/cpp/csrmat/csrmat.hpp:71-88): each row i stores
+ * the blocks (i, j) with j >= i in ascending j, the operator applies every
+ * stored off-diagonal block both as (i, j) and transposed as (j, i), and a
+ * separate per-row diagonal C_i is added (solver.cu:213-259).
+ */
+#ifndef PPF_MI355X_H
+#define PPF_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PPF_OK = 0,
+    PPF_ERR_ARG = 1,      /* invalid argument or shape */
+    PPF_ERR_HIP = 2,      /* HIP runtime error */
+    PPF_ERR_NODEV = 3,    /* no gfx950 device visible */
+    PPF_ERR_CAPACITY = 4, /* a fixed-capacity buffer would overflow */
+    PPF_ERR_SOLVER = 5    /* solver invariant violated (reference aborts) */
+};
+
+/* Last error message of the calling thread ("" if none). */
+const char *ppf_last_error(void);
+
+/* Library / device identity: writes the gfx arch name of device 0. */
+int ppf_device_info(char *arch, uint32_t arch_len, uint32_t *cu_count);
+
+/* ------------------------------------------------------------------------
+ * Block-Jacobi PCG (replaces solver::solve, solver.cu:2180-2321, for the
+ * block-Jacobi preconditioner; the recurrence is solver::cg_device,
+ * solver.cu:921-1383).
+ * ---------------------------------------------------------------------- */
+
+/* Outcome of one solve. status: 0 converged, 1 truncated (p^T A p inside its
+ * own fp32 round-off; success, solver.cu:1287-1303), 2 iteration cap,
+ * 3 non-finite residual, 4 p^T A p negative beyond the round-off bound,
+ * 5 r^T M^-1 r non-positive or a non-invertible diagonal block (the reference
+ * aborts with PPF_FATAL_SOLVER_INVARIANT on 4 and 5). */
+typedef struct ppf_pcg_result {
+    uint32_t iter;     /* reported iteration count (reference check schedule) */
+    float resid;       /* |r|_1 / |r0|_1 at the reported iteration */
+    int32_t status;
+    int32_t break_cause; /* 0 none, 1 pAp, 2 rz, 3 noise (solver.cu:46-57) */
+    int32_t break_iter;
+    float err0;        /* |b - A x0|_1 */
+} ppf_pcg_result;
+
+typedef struct ppf_pcg_plan ppf_pcg_plan;
+
+/* Build a solve plan for a fixed sparsity pattern: the transpose table
+ * (reference: builder.rs:1336-1345 for the fixed matrix, csrmat.cu:302-433 for
+ * the dynamic one) and all per-solve scratch. d_row_offset has n+1 entries,
+ * d_col_index nnz entries; both must stay valid for the plan's lifetime. */
+int ppf_pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *d_row_offset,
+                        const uint32_t *d_col_index, void *stream,
+                        ppf_pcg_plan **out);
+void ppf_pcg_plan_destroy(ppf_pcg_plan *plan);
+
+/* Solve M x = b to |r|_1/|r0|_1 < tol (x holds x0 on entry). d_value: nnz
+ * blocks; d_cdiag: n blocks; d_b, d_x: 3n floats. Mirrors solver::solve +
+ * solver::cg (solver.cu:1631-1675). */
+int ppf_pcg_solve(ppf_pcg_plan *plan, const float *d_value,
+                  const float *d_cdiag, const float *d_b, float *d_x, float tol,
+                  uint32_t max_iter, ppf_pcg_result *out, void *stream);
+
+/* Same solve, also writing |r|_1 after every iteration into d_err_hist
+ * (max_iter floats) for residual-curve comparisons. */
+int ppf_pcg_solve_hist(ppf_pcg_plan *plan, const float *d_value,
+                       const float *d_cdiag, const float *d_b, float *d_x,
+                       float tol, uint32_t max_iter, float *d_err_hist,
+                       ppf_pcg_result *out, void *stream);
+
+/* y = M x for the plan's pattern (solver::apply, solver.cu:213-259). */
+int ppf_pcg_spmv(ppf_pcg_plan *plan, const float *d_value, const float *d_cdiag,
+                 const float *d_x, float *d_y, void *stream);
+
+/* Time `reps` back-to-back launches of the PCG's SpMV kernel (y = M x with
+ * the x.Mx partials, the kernel of one CG iteration) with HIP events on
+ * `stream`; *avg_ms receives the mean duration of one launch. Measurement
+ * entry point, no reference counterpart. */
+int ppf_pcg_time_spmv(ppf_pcg_plan *plan, const float *d_value,
+                      const float *d_cdiag, const float *d_x, float *d_y,
+                      uint32_t reps, float *avg_ms, void *stream);
+
+/* Block-Jacobi inverse of n symmetric 3x3 blocks (solver::invert,
+ * solver.cu:357-400). *d_bad (device uint32) counts blocks the reference
+ * would trap on. */
+int ppf_bj_invert(uint32_t n, const float *d_in, float *d_out, uint32_t *d_bad,
+                  void *stream);
+
+/* Symmetric 3x3 eigendecomposition of n blocks, eigenvalues ascending
+ * (linalg::eig::symm3x3, linalg/eigsolve.hpp:195-232). */
+int ppf_symm3x3(uint32_t n, const float *d_in, float *d_val, float *d_vec,
+                void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,194 @@
+// capi.hip -- extern "C" entry points declared in include/ppf_mi355x.h.
+//
+// Plain pointers and sizes only; no torch or C++ types cross the boundary.
+#include "common.hpp"
+#include "eig3.hpp"
+#include "pcg.hpp"
+#include <cstdarg>
+#include <cstring>
+
+namespace ppf {
+
+static thread_local char g_err[1024];
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+namespace {
+
+__global__ void k_bj_invert(uint32_t n, const float *__restrict__ in,
+                            float *__restrict__ out, uint32_t *__restrict__ bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    float a[9], o[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        a[e] = in[9 * (size_t)i + e];
+    }
+    const bool ok = bj_invert(a, o);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        out[9 * (size_t)i + e] = o[e];
+    }
+    if (!ok && bad) {
+        atomicAdd(bad, 1u);
+    }
+}
+
+__global__ void k_symm3x3(uint32_t n, const float *__restrict__ in,
+                          float *__restrict__ val, float *__restrict__ vec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    m3 a;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        a.m[e] = in[9 * (size_t)i + e];
+    }
+    f3 l;
+    m3 q;
+    symm3x3(a, l, q);
+    store3(val + 3 * (size_t)i, l);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        vec[9 * (size_t)i + e] = q.m[e];
+    }
+}
+
+} // namespace
+} // namespace ppf
+
+using namespace ppf;
+
+struct ppf_pcg_plan {
+    PcgPlanImpl *impl;
+};
+
+extern "C" {
+
+const char *ppf_last_error(void) { return g_err; }
+
+int ppf_device_info(char *arch, uint32_t arch_len, uint32_t *cu_count) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no HIP device visible");
+        return PPF_ERR_NODEV;
+    }
+    hipDeviceProp_t prop;
+    PPF_HIP(hipGetDeviceProperties(&prop, 0));
+    if (arch && arch_len) {
+        snprintf(arch, arch_len, "%s", prop.gcnArchName);
+    }
+    if (cu_count) {
+        *cu_count = (uint32_t)prop.multiProcessorCount;
+    }
+    return PPF_OK;
+}
+
+int ppf_pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *d_row_offset,
+                        const uint32_t *d_col_index, void *stream,
+                        ppf_pcg_plan **out) {
+    if (!out) {
+        set_error("ppf_pcg_plan_create: out is null");
+        return PPF_ERR_ARG;
+    }
+    PcgPlanImpl *impl = nullptr;
+    int rc = pcg_plan_create(n, nnz, d_row_offset, d_col_index,
+                             (hipStream_t)stream, &impl);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    *out = new ppf_pcg_plan{impl};
+    return PPF_OK;
+}
+
+void ppf_pcg_plan_destroy(ppf_pcg_plan *plan) {
+    if (plan) {
+        pcg_plan_destroy(plan->impl);
+        delete plan;
+    }
+}
+
+int ppf_pcg_solve(ppf_pcg_plan *plan, const float *d_value,
+                  const float *d_cdiag, const float *d_b, float *d_x, float tol,
+                  uint32_t max_iter, ppf_pcg_result *out, void *stream) {
+    if (!plan) {
+        set_error("ppf_pcg_solve: plan is null");
+        return PPF_ERR_ARG;
+    }
+    return pcg_solve(plan->impl, d_value, d_cdiag, d_b, d_x, tol, max_iter,
+                     nullptr, out, (hipStream_t)stream);
+}
+
+int ppf_pcg_solve_hist(ppf_pcg_plan *plan, const float *d_value,
+                       const float *d_cdiag, const float *d_b, float *d_x,
+                       float tol, uint32_t max_iter, float *d_err_hist,
+                       ppf_pcg_result *out, void *stream) {
+    if (!plan) {
+        set_error("ppf_pcg_solve_hist: plan is null");
+        return PPF_ERR_ARG;
+    }
+    return pcg_solve(plan->impl, d_value, d_cdiag, d_b, d_x, tol, max_iter,
+                     d_err_hist, out, (hipStream_t)stream);
+}
+
+int ppf_pcg_spmv(ppf_pcg_plan *plan, const float *d_value, const float *d_cdiag,
+                 const float *d_x, float *d_y, void *stream) {
+    if (!plan || !d_value || !d_cdiag || !d_x || !d_y) {
+        set_error("ppf_pcg_spmv: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    return pcg_spmv(plan->impl, d_value, d_cdiag, d_x, d_y, (hipStream_t)stream);
+}
+
+int ppf_pcg_time_spmv(ppf_pcg_plan *plan, const float *d_value,
+                      const float *d_cdiag, const float *d_x, float *d_y,
+                      uint32_t reps, float *avg_ms, void *stream) {
+    if (!plan) {
+        set_error("ppf_pcg_time_spmv: plan is null");
+        return PPF_ERR_ARG;
+    }
+    return pcg_time_spmv(plan->impl, d_value, d_cdiag, d_x, d_y, reps, avg_ms,
+                         (hipStream_t)stream);
+}
+
+int ppf_bj_invert(uint32_t n, const float *d_in, float *d_out, uint32_t *d_bad,
+                  void *stream) {
+    if (n == 0) {
+        return PPF_OK;
+    }
+    if (!d_in || !d_out) {
+        set_error("ppf_bj_invert: null pointer");
+        return PPF_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    k_bj_invert<<<(n + 255) / 256, 256, 0, st>>>(n, d_in, d_out, d_bad);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipStreamSynchronize(st));
+    return PPF_OK;
+}
+
+int ppf_symm3x3(uint32_t n, const float *d_in, float *d_val, float *d_vec,
+                void *stream) {
+    if (n == 0) {
+        return PPF_OK;
+    }
+    if (!d_in || !d_val || !d_vec) {
+        set_error("ppf_symm3x3: null pointer");
+        return PPF_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    k_symm3x3<<<(n + 255) / 256, 256, 0, st>>>(n, d_in, d_val, d_vec);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipStreamSynchronize(st));
+    return PPF_OK;
+}
+
+} // extern "C"

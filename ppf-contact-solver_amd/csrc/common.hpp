@@ -1,0 +1,108 @@
+// common.hpp -- shared device/host helpers of the MI355X backend.
+//
+// gfx950 only: wave64, column-major 3x3 blocks (the layout of the reference's
+// Mat3x3f, crates/ppf-cts-solver/src/cpp/linalg/smat.hpp), fp32 arithmetic.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+#define PPF_HD __host__ __device__ inline
+
+namespace ppf {
+
+constexpr int kWave = 64;
+constexpr int kXcd = 8; // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+struct f3 {
+    float x, y, z;
+};
+
+PPF_HD f3 mk3(float a, float b, float c) { return f3{a, b, c}; }
+PPF_HD f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+PPF_HD f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+PPF_HD f3 operator*(float s, f3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
+PPF_HD float dot3(f3 a, f3 b) {
+    float acc = 0.0f;
+    acc = acc + a.x * b.x;
+    acc = acc + a.y * b.y;
+    acc = acc + a.z * b.z;
+    return acc;
+}
+PPF_HD f3 cross3(f3 a, f3 b) {
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z,
+               a.x * b.y - a.y * b.x);
+}
+PPF_HD f3 load3(const float *p) { return mk3(p[0], p[1], p[2]); }
+PPF_HD void store3(float *p, f3 v) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+// y = A x, column-major A, term order of the reference's UnrolledMat3x3f.
+PPF_HD f3 mv(const float *a, f3 x) {
+    return mk3(a[0] * x.x + a[3] * x.y + a[6] * x.z,
+               a[1] * x.x + a[4] * x.y + a[7] * x.z,
+               a[2] * x.x + a[5] * x.y + a[8] * x.z);
+}
+// y = A^T x.
+PPF_HD f3 mvt(const float *a, f3 x) {
+    return mk3(a[0] * x.x + a[1] * x.y + a[2] * x.z,
+               a[3] * x.x + a[4] * x.y + a[5] * x.z,
+               a[6] * x.x + a[7] * x.y + a[8] * x.z);
+}
+
+// Sum over a wave64 (butterfly; every lane ends with the total). The order of
+// operations is fixed, so the result is deterministic.
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, kWave);
+    }
+    return v;
+}
+
+// Deterministic block sum for blockDim.x == NT (multiple of 64). Every thread
+// returns the total. `lds` needs NT/64 floats.
+template <int NT> __device__ inline float block_sum(float v, float *lds) {
+    constexpr int NW = NT / kWave;
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        lds[w] = v;
+    }
+    __syncthreads();
+    float t = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        t += lds[k];
+    }
+    __syncthreads();
+    return t;
+}
+
+// Workgroup id remapped so that each XCD owns one contiguous range of logical
+// ids (the hardware deals workgroups to the 8 XCDs round-robin). Consecutive
+// rows then share one XCD's L2. Requires gridDim.x % 8 == 0.
+__device__ inline unsigned xcd_logical_block() {
+    const unsigned g = gridDim.x, b = blockIdx.x;
+    return (b % kXcd) * (g / kXcd) + b / kXcd;
+}
+
+} // namespace ppf
+
+// Error plumbing for the C ABI: every entry point returns 0 on success and a
+// nonzero code otherwise; ppf_last_error() holds the message.
+namespace ppf {
+void set_error(const char *fmt, ...);
+}
+
+#define PPF_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::ppf::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,        \
+                             hipGetErrorString(e_));                           \
+            return PPF_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)

@@ -1,0 +1,790 @@
+// pcg.hip -- block-Jacobi preconditioned CG on MI355X.
+//
+// Replaces solver::solve / solver::cg_device
+// (crates/ppf-cts-solver/src/cpp/solver/solver.cu:921-1383, 2180-2321) for the
+// block-Jacobi preconditioner. Same recurrence, same curvature guard
+// (solver.cu:57-114), same L1 relative-residual stopping rule measured against
+// the seeded residual |b - A x0|_1, and the same residual-check schedule
+// (every 4 iterations, every iteration once within 8*tol), emulated on the
+// device so that no host round trip sits inside the loop.
+//
+// MI355X layout and kernels:
+//   * the operator is the reference's split form, an upper-triangle block CSR
+//     applied both ways plus a per-row diagonal. The diagonal blocks are merged
+//     once per solve into D_i = B_ii + C_i, and the transpose of the strict
+//     upper triangle is walked through a per-column (row, slot) table, so every
+//     stored block is read from HBM once per SpMV and re-read from the L2 of
+//     the same XCD (rows are dealt to XCDs in contiguous ranges).
+//   * one iteration = 3 kernels: SpMV + p.Ap + sum|terms|; the fused x/r/z
+//     update with the r.z and |r|_1 partials; p = z + beta p. Each kernel
+//     writes one partial per workgroup; every workgroup of the next kernel
+//     reduces those partials itself in a fixed order, so the scalars are
+//     bit-identical everywhere with no atomics and no grid barrier, and every
+//     run is deterministic.
+//   * iterations are captured 8 at a time into a hipGraph; once the device
+//     bookkeeping marks the solve done, the remaining replays exit at entry.
+#include "common.hpp"
+#include "eig3.hpp"
+#include "pcg.hpp"
+#include "scan.hpp"
+#include <cstring>
+#include <new>
+
+namespace ppf {
+
+namespace {
+
+constexpr int kNT = 256;           // threads per workgroup
+constexpr int kLanes = 8;          // lanes per row in the SpMV
+constexpr int kRowsPerPass = kNT / kLanes;
+constexpr int kGraphIters = 8;     // iterations per captured graph (even)
+constexpr unsigned kNone = 0xffffffffu;
+constexpr float kNoiseSlack = 256.0f;       // solver.cu:69
+constexpr float kFloatEps = 1.19209290e-7f; // solver.cu:70
+
+// Device-resident scalars of one solve. Written by exactly one thread of one
+// kernel and read only by later kernels (stream order), except `done`, whose
+// late observation inside the kernel that sets it is harmless.
+struct Scalars {
+    float rz[2];     // r.z of the latest two iterations, by parity
+    float err0;
+    float tol;
+    uint32_t max_iter;
+    uint32_t iter;   // completed iterations
+    uint32_t next_check;
+    uint32_t stride;
+    int32_t done;
+    int32_t status;
+    int32_t breakdown;
+    int32_t cause;
+    int32_t break_iter;
+    uint32_t rep_iter;
+    float rep_resid;
+    uint32_t bad;    // non-invertible diagonal blocks
+};
+
+struct Args {
+    uint32_t n;
+    uint32_t G1, G2, G3; // workgroups of the SpMV / update / p kernels
+    uint32_t R1, R2, R3; // contiguous rows (elements for G3) per workgroup
+    const uint32_t *rp, *col, *tp, *tcol, *tslot, *dslot;
+    const float *val, *cdiag, *b;
+    float *x, *D, *P, *r, *z, *p, *Ap;
+    float *part; // [G1 pAp][G1 |terms|][G2 rz][G2 err]
+    float *err_hist;
+    Scalars *s;
+};
+
+// ---- pattern preparation (once per sparsity pattern) ----------------------
+
+__global__ void k_count_transpose(uint32_t n, const uint32_t *__restrict__ rp,
+                                  const uint32_t *__restrict__ col,
+                                  uint32_t *__restrict__ tcount,
+                                  uint32_t *__restrict__ dslot,
+                                  uint32_t *__restrict__ bad_pattern) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    uint32_t ds = kNone;
+    uint32_t prev = 0;
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        const uint32_t j = col[k];
+        if (j >= n || j < i || (k > rp[i] && j <= prev)) {
+            atomicAdd(bad_pattern, 1u); // not upper, unsorted or out of range
+            continue;
+        }
+        prev = j;
+        if (j == i) {
+            ds = k;
+        } else {
+            atomicAdd(tcount + j, 1u);
+        }
+    }
+    dslot[i] = ds;
+}
+
+__global__ void k_fill_transpose(uint32_t n, const uint32_t *__restrict__ rp,
+                                 const uint32_t *__restrict__ col,
+                                 const uint32_t *__restrict__ tp,
+                                 uint32_t *__restrict__ tfill,
+                                 uint32_t *__restrict__ tcol,
+                                 uint32_t *__restrict__ tslot) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    uint32_t prev = 0;
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        const uint32_t j = col[k];
+        // Same acceptance rule as k_count_transpose, so every write stays
+        // inside the column's counted range.
+        if (j >= n || j < i || (k > rp[i] && j <= prev)) {
+            continue;
+        }
+        prev = j;
+        if (j > i) {
+            const uint32_t pos = tp[j] + atomicAdd(tfill + j, 1u);
+            tcol[pos] = i;
+            tslot[pos] = k;
+        }
+    }
+}
+
+// row_offset must start at 0, be non-decreasing and end at nnz.
+__global__ void k_check_offsets(uint32_t n, uint32_t nnz,
+                                const uint32_t *__restrict__ rp,
+                                uint32_t *__restrict__ bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) {
+        return;
+    }
+    const uint32_t v = rp[i];
+    const bool ok = (i == 0 ? v == 0 : v >= rp[i - 1]) && (i < n || v == nnz) &&
+                    v <= nnz;
+    if (!ok) {
+        atomicAdd(bad, 1u);
+    }
+}
+
+// Order every column's mirror list by source row, as the reference's
+// transpose table (builder.rs:1336-1345), so the walk order is deterministic.
+__global__ void k_sort_transpose(uint32_t n, const uint32_t *__restrict__ tp,
+                                 uint32_t *__restrict__ tcol,
+                                 uint32_t *__restrict__ tslot) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) {
+        return;
+    }
+    const uint32_t a = tp[j], b = tp[j + 1];
+    for (uint32_t u = a + 1; u < b; ++u) {
+        const uint32_t kc = tcol[u], ks = tslot[u];
+        uint32_t v = u;
+        while (v > a && tcol[v - 1] > kc) {
+            tcol[v] = tcol[v - 1];
+            tslot[v] = tslot[v - 1];
+            --v;
+        }
+        tcol[v] = kc;
+        tslot[v] = ks;
+    }
+}
+
+// ---- per-solve setup -------------------------------------------------------
+
+// D_i = B_ii + C_i and P_i = D_i^-1 (solver.cu:2191-2195).
+__global__ __launch_bounds__(kNT) void k_diag_invert(const Args *__restrict__ A) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A->n) {
+        return;
+    }
+    const uint32_t ds = A->dslot[i];
+    float d[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        const float bii = ds != kNone ? A->val[9 * (size_t)ds + e] : 0.0f;
+        d[e] = bii + A->cdiag[9 * (size_t)i + e];
+    }
+    float inv[9];
+    const bool ok = bj_invert(d, inv);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        A->D[9 * (size_t)i + e] = d[e];
+        A->P[9 * (size_t)i + e] = inv[e];
+    }
+    if (!ok) {
+        atomicAdd(&A->s->bad, 1u);
+    }
+}
+
+// y = M v (+ per-workgroup partials of v.Mv and sum|terms| when DOT).
+// 8 lanes per row; workgroups own contiguous, XCD-grouped row ranges.
+template <bool DOT>
+__global__ __launch_bounds__(kNT) void k_spmv(const Args *__restrict__ A,
+                                              const float *__restrict__ v,
+                                              float *__restrict__ y) {
+    __shared__ float lds[kNT / kWave];
+    if (DOT && A->s->done) {
+        return;
+    }
+    const uint32_t n = A->n;
+    const uint32_t L = xcd_logical_block();
+    const uint32_t lane = threadIdx.x % kLanes, sub = threadIdx.x / kLanes;
+    const uint32_t *__restrict__ rp = A->rp;
+    const uint32_t *__restrict__ col = A->col;
+    const uint32_t *__restrict__ tp = A->tp;
+    const uint32_t *__restrict__ tcol = A->tcol;
+    const uint32_t *__restrict__ tslot = A->tslot;
+    const float *__restrict__ val = A->val;
+    const float *__restrict__ D = A->D;
+    const uint32_t lo = L * A->R1;
+    const uint32_t hi = min(n, lo + A->R1);
+    float dot = 0.0f, adot = 0.0f;
+    for (uint32_t base = lo; base < hi; base += kRowsPerPass) {
+        const uint32_t i = base + sub;
+        f3 sum = mk3(0, 0, 0), as = mk3(0, 0, 0);
+        const bool valid = i < hi;
+        if (valid) {
+            const uint32_t k1 = rp[i + 1];
+            for (uint32_t k = rp[i] + lane; k < k1; k += kLanes) {
+                const uint32_t j = col[k];
+                if (j != i) {
+                    const f3 t = mv(val + 9 * (size_t)k, load3(v + 3 * (size_t)j));
+                    sum = sum + t;
+                    as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+                }
+            }
+            const uint32_t t1 = tp[i + 1];
+            for (uint32_t k = tp[i] + lane; k < t1; k += kLanes) {
+                const f3 t = mvt(val + 9 * (size_t)tslot[k],
+                                 load3(v + 3 * (size_t)tcol[k]));
+                sum = sum + t;
+                as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+            }
+            if (lane == 0) {
+                const f3 t = mv(D + 9 * (size_t)i, load3(v + 3 * (size_t)i));
+                sum = sum + t;
+                as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+            }
+        }
+#pragma unroll
+        for (int off = kLanes / 2; off > 0; off >>= 1) {
+            sum.x += __shfl_xor(sum.x, off, kLanes);
+            sum.y += __shfl_xor(sum.y, off, kLanes);
+            sum.z += __shfl_xor(sum.z, off, kLanes);
+            if (DOT) {
+                as.x += __shfl_xor(as.x, off, kLanes);
+                as.y += __shfl_xor(as.y, off, kLanes);
+                as.z += __shfl_xor(as.z, off, kLanes);
+            }
+        }
+        if (valid && lane == 0) {
+            store3(y + 3 * (size_t)i, sum);
+            if (DOT) {
+                const f3 vi = load3(v + 3 * (size_t)i);
+                dot = dot + vi.x * sum.x;
+                dot = dot + vi.y * sum.y;
+                dot = dot + vi.z * sum.z;
+                adot = adot + fabsf(vi.x) * as.x;
+                adot = adot + fabsf(vi.y) * as.y;
+                adot = adot + fabsf(vi.z) * as.z;
+            }
+        }
+    }
+    if (DOT) {
+        dot = block_sum<kNT>(dot, lds);
+        adot = block_sum<kNT>(adot, lds);
+        if (threadIdx.x == 0) {
+            A->part[L] = dot;
+            A->part[A->G1 + L] = adot;
+        }
+    }
+}
+
+// r = b - y, z = P r, p = z, partials of r.z and |r|_1 (solver.cu:1015-1093).
+__global__ __launch_bounds__(kNT) void k_init_residual(const Args *__restrict__ A) {
+    __shared__ float lds[kNT / kWave];
+    const uint32_t n = A->n;
+    const uint32_t L = blockIdx.x;
+    const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
+    float rz = 0.0f, err = 0.0f;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kNT) {
+        f3 ri;
+        const f3 bi = load3(A->b + 3 * (size_t)i);
+        const f3 yi = load3(A->Ap + 3 * (size_t)i);
+        ri = mk3(bi.x + (-1.0f) * yi.x, bi.y + (-1.0f) * yi.y,
+                 bi.z + (-1.0f) * yi.z);
+        const f3 zi = mv(A->P + 9 * (size_t)i, ri);
+        store3(A->r + 3 * (size_t)i, ri);
+        store3(A->z + 3 * (size_t)i, zi);
+        store3(A->p + 3 * (size_t)i, zi);
+        rz = rz + ri.x * zi.x;
+        rz = rz + ri.y * zi.y;
+        rz = rz + ri.z * zi.z;
+        err = err + fabsf(ri.x);
+        err = err + fabsf(ri.y);
+        err = err + fabsf(ri.z);
+    }
+    rz = block_sum<kNT>(rz, lds);
+    err = block_sum<kNT>(err, lds);
+    if (threadIdx.x == 0) {
+        A->part[2 * A->G1 + L] = rz;
+        A->part[2 * A->G1 + A->G2 + L] = err;
+    }
+}
+
+__device__ inline float reduce_parts(const float *__restrict__ part, uint32_t cnt,
+                                     float *lds) {
+    float acc = 0.0f;
+    for (uint32_t k = threadIdx.x; k < cnt; k += kNT) {
+        acc = acc + part[k];
+    }
+    return block_sum<kNT>(acc, lds);
+}
+
+// Initial scalars and the bookkeeping state (solver.cu:1020-1031, 1170-1175).
+__global__ __launch_bounds__(kNT) void k_init_scalars(const Args *__restrict__ A) {
+    __shared__ float lds[kNT / kWave];
+    const float rz0 = reduce_parts(A->part + 2 * A->G1, A->G2, lds);
+    const float err0 = reduce_parts(A->part + 2 * A->G1 + A->G2, A->G2, lds);
+    if (threadIdx.x == 0) {
+        Scalars *s = A->s;
+        s->rz[0] = rz0;
+        s->rz[1] = rz0;
+        s->err0 = err0;
+        s->iter = 0;
+        s->next_check = 1;
+        s->stride = 4;
+        s->breakdown = 0;
+        s->cause = 0;
+        s->break_iter = 0;
+        s->done = 0;
+        s->status = -1;
+        if (s->bad) {
+            s->done = 1;
+            s->status = 5;
+        } else if (err0 == 0.0f) {
+            s->done = 1;
+            s->status = 0;
+            s->rep_iter = 1;
+            s->rep_resid = 0.0f;
+        }
+    }
+}
+
+// ---- one CG iteration ------------------------------------------------------
+
+// alpha from the SpMV partials with the curvature guard, then x += alpha p,
+// r -= alpha Ap, z = P r and the r.z / |r|_1 partials (solver.cu:431-514).
+// k = position of this iteration inside the captured graph.
+__global__ __launch_bounds__(kNT) void k_update(const Args *__restrict__ A,
+                                                uint32_t k) {
+    __shared__ float lds[kNT / kWave];
+    Scalars *s = A->s;
+    if (s->done) {
+        return;
+    }
+    const float pAp = reduce_parts(A->part, A->G1, lds);
+    const float absdot = reduce_parts(A->part + A->G1, A->G1, lds);
+    const float rz0 = s->rz[k & 1];
+    const float bound = kNoiseSlack * kFloatEps * absdot;
+    float alpha = 0.0f;
+    if (pAp > bound) {
+        alpha = rz0 / pAp;
+    } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->breakdown = 1;
+        s->cause = (pAp < -bound || pAp != pAp) ? 1 : 3;
+        s->break_iter = (int32_t)s->iter;
+    }
+    const uint32_t n = A->n;
+    const uint32_t L = blockIdx.x;
+    const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
+    float *__restrict__ x = A->x;
+    float *__restrict__ r = A->r;
+    float *__restrict__ z = A->z;
+    const float *__restrict__ p = A->p;
+    const float *__restrict__ Ap = A->Ap;
+    const float *__restrict__ P = A->P;
+    float rz = 0.0f, err = 0.0f;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kNT) {
+        const size_t o = 3 * (size_t)i;
+        const f3 pi = load3(p + o), api = load3(Ap + o);
+        f3 xi = load3(x + o), ri = load3(r + o);
+        xi = mk3(fmaf(alpha, pi.x, xi.x), fmaf(alpha, pi.y, xi.y),
+                 fmaf(alpha, pi.z, xi.z));
+        ri = mk3(fmaf(-alpha, api.x, ri.x), fmaf(-alpha, api.y, ri.y),
+                 fmaf(-alpha, api.z, ri.z));
+        const f3 zi = mv(P + 9 * (size_t)i, ri);
+        store3(x + o, xi);
+        store3(r + o, ri);
+        store3(z + o, zi);
+        rz = rz + ri.x * zi.x;
+        rz = rz + ri.y * zi.y;
+        rz = rz + ri.z * zi.z;
+        err = err + fabsf(ri.x);
+        err = err + fabsf(ri.y);
+        err = err + fabsf(ri.z);
+    }
+    rz = block_sum<kNT>(rz, lds);
+    err = block_sum<kNT>(err, lds);
+    if (threadIdx.x == 0) {
+        A->part[2 * A->G1 + L] = rz;
+        A->part[2 * A->G1 + A->G2 + L] = err;
+    }
+}
+
+// beta = rz1 / rz0 (breakdown if rz0 <= 0), p = z + beta p, and -- on one
+// thread -- the reference's residual-check schedule (solver.cu:1249-1380).
+__global__ __launch_bounds__(kNT) void k_pupdate(const Args *__restrict__ A,
+                                                 uint32_t k) {
+    __shared__ float lds[kNT / kWave];
+    Scalars *s = A->s;
+    if (s->done) {
+        return;
+    }
+    const float rz1 = reduce_parts(A->part + 2 * A->G1, A->G2, lds);
+    const float rz0 = s->rz[k & 1];
+    const bool rz_ok = rz0 > 0.0f;
+    const float beta = rz_ok ? rz1 / rz0 : 0.0f;
+    const uint32_t n3 = 3 * A->n;
+    const uint32_t L = blockIdx.x;
+    const uint32_t lo = L * A->R3, hi = min(n3, lo + A->R3);
+    float *__restrict__ p = A->p;
+    const float *__restrict__ z = A->z;
+    for (uint32_t e = lo + threadIdx.x; e < hi; e += kNT) {
+        p[e] = fmaf(beta, p[e], 1.0f * z[e]);
+    }
+    if (blockIdx.x != 0) {
+        return;
+    }
+    const float err = reduce_parts(A->part + 2 * A->G1 + A->G2, A->G2, lds);
+    if (threadIdx.x != 0) {
+        return;
+    }
+    if (!rz_ok && s->cause == 0) {
+        s->breakdown = 1;
+        s->cause = 2;
+        s->break_iter = (int32_t)s->iter;
+    }
+    s->rz[(k + 1) & 1] = rz1;
+    const uint32_t it = s->iter + 1;
+    s->iter = it;
+    if (A->err_hist) {
+        A->err_hist[it - 1] = err;
+    }
+    const bool at_cap = it >= s->max_iter;
+    if (it == s->next_check || at_cap) {
+        const float reresid = err / s->err0;
+        s->rep_iter = it;
+        s->rep_resid = reresid;
+        int32_t st = -1;
+        if (reresid < s->tol) {
+            st = 0;
+        } else if (s->breakdown) {
+            st = s->cause == 3 ? 1 : (s->cause == 2 ? 5 : 4);
+        } else if (!isfinite(reresid)) {
+            st = 3;
+        } else if (at_cap) {
+            st = 2;
+        }
+        if (st >= 0) {
+            s->status = st;
+            s->done = 1;
+        } else {
+            if (reresid < 8.0f * s->tol) {
+                s->stride = 1;
+            }
+            s->next_check = it + s->stride;
+        }
+    }
+}
+
+uint32_t round8(uint32_t v) { return (v + 7u) & ~7u; }
+uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+} // namespace
+
+// ---- plan ------------------------------------------------------------------
+
+struct PcgPlanImpl {
+    uint32_t n = 0, nnz = 0;
+    Args h{};             // host copy of the kernel arguments
+    Args *d_args = nullptr;
+    Scalars *h_scal = nullptr; // pinned
+    void *mem = nullptr;       // one device allocation for all scratch
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <class T> T *carve(char *&cur, size_t count) {
+    T *p = reinterpret_cast<T *>(cur);
+    size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+    cur += bytes;
+    return p;
+}
+
+size_t carve_bytes(size_t count, size_t elem) {
+    return (count * elem + 255) & ~size_t(255);
+}
+
+hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
+    const Args &h = pl.h;
+    k_spmv<true><<<h.G1, kNT, 0, st>>>(pl.d_args, h.p, h.Ap);
+    k_update<<<h.G2, kNT, 0, st>>>(pl.d_args, k);
+    k_pupdate<<<h.G3, kNT, 0, st>>>(pl.d_args, k);
+    return hipGetLastError();
+}
+
+} // namespace
+
+int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
+                    const uint32_t *col, hipStream_t st, PcgPlanImpl **out) {
+    if (!out || n == 0 || !rp || !col) {
+        set_error("pcg_plan_create: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    auto *pl = new (std::nothrow) PcgPlanImpl();
+    if (!pl) {
+        return PPF_ERR_ARG;
+    }
+    pl->n = n;
+    pl->nnz = nnz;
+    Args &h = pl->h;
+    h.n = n;
+    h.G1 = round8(std::min<uint32_t>(2048u, cdiv(n, kRowsPerPass)));
+    h.R1 = cdiv(cdiv(n, h.G1), kRowsPerPass) * kRowsPerPass;
+    h.G2 = std::min<uint32_t>(1024u, cdiv(n, kNT));
+    h.R2 = cdiv(cdiv(n, h.G2), kNT) * kNT;
+    h.G3 = std::min<uint32_t>(1024u, cdiv(3 * n, kNT));
+    h.R3 = cdiv(cdiv(3 * n, h.G3), kNT) * kNT;
+
+    const size_t n3 = 3 * (size_t)n;
+    const size_t scan_words = scan_scratch_words(n + 1);
+    size_t bytes = carve_bytes(n + 1, 4) * 2 + carve_bytes(nnz, 4) * 2 +
+                   carve_bytes(n, 4) + carve_bytes(9 * (size_t)n, 4) * 2 +
+                   carve_bytes(n3, 4) * 4 +
+                   carve_bytes(2 * (size_t)h.G1 + 2 * (size_t)h.G2, 4) +
+                   carve_bytes(1, sizeof(Scalars)) + carve_bytes(1, sizeof(Args)) +
+                   carve_bytes(scan_words, 4) + carve_bytes(4, 4);
+    hipError_t e = hipMalloc(&pl->mem, bytes);
+    if (e != hipSuccess) {
+        set_error("pcg_plan_create: hipMalloc(%zu): %s", bytes,
+                  hipGetErrorString(e));
+        delete pl;
+        return PPF_ERR_HIP;
+    }
+    char *cur = static_cast<char *>(pl->mem);
+    uint32_t *tp = carve<uint32_t>(cur, n + 1);
+    uint32_t *tfill = carve<uint32_t>(cur, n + 1);
+    uint32_t *tcol = carve<uint32_t>(cur, nnz);
+    uint32_t *tslot = carve<uint32_t>(cur, nnz);
+    uint32_t *dslot = carve<uint32_t>(cur, n);
+    h.D = carve<float>(cur, 9 * (size_t)n);
+    h.P = carve<float>(cur, 9 * (size_t)n);
+    h.r = carve<float>(cur, n3);
+    h.z = carve<float>(cur, n3);
+    h.p = carve<float>(cur, n3);
+    h.Ap = carve<float>(cur, n3);
+    h.part = carve<float>(cur, 2 * (size_t)h.G1 + 2 * (size_t)h.G2);
+    h.s = carve<Scalars>(cur, 1);
+    pl->d_args = carve<Args>(cur, 1);
+    uint32_t *scan_tmp = carve<uint32_t>(cur, scan_words);
+    uint32_t *flags = carve<uint32_t>(cur, 4);
+    h.rp = rp;
+    h.col = col;
+    h.tp = tp;
+    h.tcol = tcol;
+    h.tslot = tslot;
+    h.dslot = dslot;
+
+    auto fail = [&](const char *what, hipError_t err) {
+        set_error("pcg_plan_create: %s: %s", what, hipGetErrorString(err));
+        ppf::pcg_plan_destroy(pl);
+        return PPF_ERR_HIP;
+    };
+#define PLAN_HIP(call)                                                         \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return fail(#call, e_);                                            \
+    } while (0)
+
+    PLAN_HIP(hipMemsetAsync(tp, 0, (n + 1) * sizeof(uint32_t), st));
+    PLAN_HIP(hipMemsetAsync(tfill, 0, (n + 1) * sizeof(uint32_t), st));
+    PLAN_HIP(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), st));
+    const unsigned gb = cdiv(n, kNT);
+    uint32_t bad = 0;
+    k_check_offsets<<<cdiv(n + 1, kNT), kNT, 0, st>>>(n, nnz, rp, flags + 1);
+    PLAN_HIP(hipGetLastError());
+    PLAN_HIP(hipMemcpyAsync(&bad, flags + 1, sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, st));
+    PLAN_HIP(hipStreamSynchronize(st));
+    if (bad) {
+        set_error("pcg_plan_create: row_offset is not a valid CSR offset array "
+                  "(0 .. nnz, non-decreasing)");
+        ppf::pcg_plan_destroy(pl);
+        return PPF_ERR_ARG;
+    }
+    k_count_transpose<<<gb, kNT, 0, st>>>(n, rp, col, tp, dslot, flags);
+    PLAN_HIP(hipGetLastError());
+    PLAN_HIP(exclusive_scan_u32(tp, tp, n + 1, scan_tmp, st));
+    k_fill_transpose<<<gb, kNT, 0, st>>>(n, rp, col, tp, tfill, tcol, tslot);
+    PLAN_HIP(hipGetLastError());
+    k_sort_transpose<<<gb, kNT, 0, st>>>(n, tp, tcol, tslot);
+    PLAN_HIP(hipGetLastError());
+    PLAN_HIP(hipMemcpyAsync(&bad, flags, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            st));
+    PLAN_HIP(hipStreamSynchronize(st));
+    if (bad) {
+        set_error("pcg_plan_create: %u pattern entries are not strictly "
+                  "ascending upper-triangle columns < n",
+                  bad);
+        ppf::pcg_plan_destroy(pl);
+        return PPF_ERR_ARG;
+    }
+    PLAN_HIP(hipHostMalloc(reinterpret_cast<void **>(&pl->h_scal),
+                           sizeof(Scalars), hipHostMallocDefault));
+
+    // Capture kGraphIters iterations once; they read every pointer through
+    // d_args, so one graph serves every solve on this plan.
+    PLAN_HIP(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    PLAN_HIP(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
+    hipError_t le = hipSuccess;
+    for (uint32_t k = 0; k < (uint32_t)kGraphIters; ++k) {
+        hipError_t e2 = launch_iteration(*pl, k, pl->cap);
+        if (le == hipSuccess) {
+            le = e2;
+        }
+    }
+    hipError_t ce = hipStreamEndCapture(pl->cap, &g);
+    if (ce == hipSuccess && le != hipSuccess) {
+        ce = le;
+    }
+    if (ce != hipSuccess) {
+        return fail("hipStreamEndCapture", ce);
+    }
+    ce = hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ce != hipSuccess) {
+        pl->exec = nullptr;
+        return fail("hipGraphInstantiate", ce);
+    }
+#undef PLAN_HIP
+    *out = pl;
+    return PPF_OK;
+}
+
+void pcg_plan_destroy(PcgPlanImpl *pl) {
+    if (!pl) {
+        return;
+    }
+    if (pl->exec) {
+        (void)hipGraphExecDestroy(pl->exec);
+    }
+    if (pl->cap) {
+        (void)hipStreamDestroy(pl->cap);
+    }
+    if (pl->h_scal) {
+        (void)hipHostFree(pl->h_scal);
+    }
+    if (pl->mem) {
+        (void)hipFree(pl->mem);
+    }
+    delete pl;
+}
+
+int pcg_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
+             const float *x, float *y, hipStream_t st) {
+    Args &h = pl->h;
+    h.val = val;
+    h.cdiag = cdiag;
+    h.b = nullptr;
+    h.x = nullptr;
+    h.err_hist = nullptr;
+    PPF_HIP(hipMemcpyAsync(pl->d_args, &h, sizeof(Args), hipMemcpyHostToDevice,
+                           st));
+    PPF_HIP(hipMemsetAsync(&h.s->bad, 0, sizeof(uint32_t), st));
+    k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
+    PPF_HIP(hipGetLastError());
+    k_spmv<false><<<h.G1, kNT, 0, st>>>(pl->d_args, x, y);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipStreamSynchronize(st));
+    return PPF_OK;
+}
+
+int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
+                  const float *x, float *y, uint32_t reps, float *avg_ms,
+                  hipStream_t st) {
+    if (!pl || !avg_ms || reps == 0) {
+        set_error("pcg_time_spmv: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    Args &h = pl->h;
+    h.val = val;
+    h.cdiag = cdiag;
+    h.b = nullptr;
+    h.x = nullptr;
+    h.err_hist = nullptr;
+    PPF_HIP(hipMemcpyAsync(pl->d_args, &h, sizeof(Args), hipMemcpyHostToDevice,
+                           st));
+    Scalars init{};
+    PPF_HIP(hipMemcpyAsync(h.s, &init, sizeof(Scalars), hipMemcpyHostToDevice,
+                           st));
+    k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
+    PPF_HIP(hipGetLastError());
+    hipEvent_t e0, e1;
+    PPF_HIP(hipEventCreate(&e0));
+    PPF_HIP(hipEventCreate(&e1));
+    k_spmv<true><<<h.G1, kNT, 0, st>>>(pl->d_args, x, y); // warm
+    PPF_HIP(hipEventRecord(e0, st));
+    for (uint32_t r = 0; r < reps; ++r) {
+        k_spmv<true><<<h.G1, kNT, 0, st>>>(pl->d_args, x, y);
+    }
+    PPF_HIP(hipEventRecord(e1, st));
+    PPF_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    PPF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / (float)reps;
+    return PPF_OK;
+}
+
+int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
+              const float *b, float *x, float tol, uint32_t max_iter,
+              float *err_hist, ppf_pcg_result *res, hipStream_t st) {
+    if (!pl || !val || !cdiag || !b || !x || !res || max_iter == 0) {
+        set_error("pcg_solve: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    Args &h = pl->h;
+    h.val = val;
+    h.cdiag = cdiag;
+    h.b = b;
+    h.x = x;
+    h.err_hist = err_hist;
+    PPF_HIP(hipMemcpyAsync(pl->d_args, &h, sizeof(Args), hipMemcpyHostToDevice,
+                           st));
+    Scalars init{};
+    init.tol = tol;
+    init.max_iter = max_iter;
+    PPF_HIP(hipMemcpyAsync(h.s, &init, sizeof(Scalars), hipMemcpyHostToDevice,
+                           st));
+    k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
+    PPF_HIP(hipGetLastError());
+    k_spmv<false><<<h.G1, kNT, 0, st>>>(pl->d_args, x, h.Ap);
+    PPF_HIP(hipGetLastError());
+    k_init_residual<<<h.G2, kNT, 0, st>>>(pl->d_args);
+    PPF_HIP(hipGetLastError());
+    k_init_scalars<<<1, kNT, 0, st>>>(pl->d_args);
+    PPF_HIP(hipGetLastError());
+    // Replay graphs until the device bookkeeping reports done. The first
+    // status read follows one graph; the iteration cap bounds the loop.
+    const uint32_t max_launch = max_iter / kGraphIters + 2;
+    for (uint32_t g = 0;; ++g) {
+        PPF_HIP(hipMemcpyAsync(pl->h_scal, h.s, sizeof(Scalars),
+                               hipMemcpyDeviceToHost, st));
+        PPF_HIP(hipStreamSynchronize(st));
+        if (pl->h_scal->done) {
+            break;
+        }
+        if (g >= max_launch) {
+            set_error("pcg_solve: bookkeeping did not terminate");
+            return PPF_ERR_SOLVER;
+        }
+        PPF_HIP(hipGraphLaunch(pl->exec, st));
+    }
+    const Scalars &s = *pl->h_scal;
+    res->iter = s.rep_iter;
+    res->resid = s.rep_resid;
+    res->status = s.status;
+    res->break_cause = s.cause;
+    res->break_iter = s.break_iter;
+    res->err0 = s.err0;
+    return PPF_OK;
+}
+
+} // namespace ppf

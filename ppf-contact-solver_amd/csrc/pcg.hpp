@@ -1,0 +1,22 @@
+// pcg.hpp -- internal interface of the block-Jacobi PCG (pcg.hip).
+#pragma once
+#include "common.hpp"
+#include "ppf_mi355x.h"
+
+namespace ppf {
+
+struct PcgPlanImpl;
+
+int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
+                    const uint32_t *col, hipStream_t st, PcgPlanImpl **out);
+void pcg_plan_destroy(PcgPlanImpl *pl);
+int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
+              const float *b, float *x, float tol, uint32_t max_iter,
+              float *err_hist, ppf_pcg_result *res, hipStream_t st);
+int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
+                  const float *x, float *y, uint32_t reps, float *avg_ms,
+                  hipStream_t st);
+int pcg_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
+             const float *x, float *y, hipStream_t st);
+
+} // namespace ppf

@@ -1,0 +1,68 @@
+"""ctypes binding of libppf_mi355x.so (the C ABI declared in include/ppf_mi355x.h).
+
+The library is the product: there is no fallback. If it is missing or was not
+built for this machine, loading raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libppf_mi355x.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include",
+                           "ppf_mi355x.h")
+
+u32 = ctypes.c_uint32
+i32 = ctypes.c_int32
+f32 = ctypes.c_float
+vp = ctypes.c_void_p
+
+
+class PcgResult(ctypes.Structure):
+    """ppf_pcg_result (include/ppf_mi355x.h)."""
+    _fields_ = [("iter", u32), ("resid", f32), ("status", i32),
+                ("break_cause", i32), ("break_iter", i32), ("err0", f32)]
+
+
+_PROTOS = {
+    "ppf_last_error": (ctypes.c_char_p, []),
+    "ppf_device_info": (ctypes.c_int, [ctypes.c_char_p, u32, ctypes.POINTER(u32)]),
+    "ppf_pcg_plan_create": (ctypes.c_int, [u32, u32, vp, vp, vp, ctypes.POINTER(vp)]),
+    "ppf_pcg_plan_destroy": (None, [vp]),
+    "ppf_pcg_solve": (ctypes.c_int, [vp, vp, vp, vp, vp, f32, u32,
+                                     ctypes.POINTER(PcgResult), vp]),
+    "ppf_pcg_solve_hist": (ctypes.c_int, [vp, vp, vp, vp, vp, f32, u32, vp,
+                                          ctypes.POINTER(PcgResult), vp]),
+    "ppf_pcg_spmv": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+    "ppf_pcg_time_spmv": (ctypes.c_int, [vp, vp, vp, vp, vp, u32,
+                                         ctypes.POINTER(f32), vp]),
+    "ppf_bj_invert": (ctypes.c_int, [u32, vp, vp, vp, vp]),
+    "ppf_symm3x3": (ctypes.c_int, [u32, vp, vp, vp, vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises OSError if it is absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it first "
+                          "(python -c 'import __graft_entry__ as g; g.build()')")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class PPFError(RuntimeError):
+    pass
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().ppf_last_error().decode(errors="replace")
+        raise PPFError(f"{what} failed (code {rc}): {msg}")

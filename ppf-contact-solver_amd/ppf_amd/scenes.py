@@ -1,0 +1,168 @@
+"""Synthetic inputs of the hot-path workloads (numpy, host side).
+
+Grid triangulation: generate_rect_faces
+(crates/ppf-cts-core/src/datamodel/mesh.rs:181-211), alternating diagonals.
+"""
+import numpy as np
+
+
+def rect_faces(res_x, res_y):
+    """(2*(res_x-1)*(res_y-1), 3) faces of a res_x x res_y vertex grid, vertex
+    index i*res_y + j, diagonal alternating with (i % 2 == j % 2)."""
+    i, j = np.meshgrid(np.arange(res_x - 1), np.arange(res_y - 1), indexing="xy")
+    # mesh.rs iterates j outer, i inner.
+    i = i.reshape(-1).astype(np.int64)
+    j = j.reshape(-1).astype(np.int64)
+    v0 = i * res_y + j
+    v1 = v0 + 1
+    v2 = v0 + res_y
+    v3 = v2 + 1
+    same = (i % 2) == (j % 2)
+    fa = np.where(same[:, None], np.stack([v0, v1, v3], 1), np.stack([v0, v1, v2], 1))
+    fb = np.where(same[:, None], np.stack([v0, v3, v2], 1), np.stack([v1, v3, v2], 1))
+    faces = np.empty((2 * len(v0), 3), dtype=np.int64)
+    faces[0::2] = fa
+    faces[1::2] = fb
+    return faces.astype(np.uint32)
+
+
+def unique_edges(faces):
+    """Sorted unique undirected edges (E, 2) with e[0] < e[1]."""
+    f = faces.astype(np.int64)
+    e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+    e.sort(axis=1)
+    key = np.unique(e[:, 0] * (int(f.max()) + 1) + e[:, 1])
+    m = int(f.max()) + 1
+    return np.stack([key // m, key % m], 1).astype(np.uint32)
+
+
+def _colmajor(blocks):
+    """(k, 3, 3) row-major numpy blocks -> (k, 9) column-major floats."""
+    return np.ascontiguousarray(blocks.transpose(0, 2, 1).reshape(-1, 9))
+
+
+def upper_bsr(n, pairs, pair_blocks, diag_blocks):
+    """Assemble the reference's upper-triangle block CSR from strictly-upper
+    (i, j) pairs (i < j, unique) and per-row diagonal blocks, rows sorted by j.
+    Returns row_offset (n+1), col_index, value (nnz, 9) column-major."""
+    pairs = np.asarray(pairs, dtype=np.int64)
+    rows = np.concatenate([np.arange(n, dtype=np.int64), pairs[:, 0]])
+    cols = np.concatenate([np.arange(n, dtype=np.int64), pairs[:, 1]])
+    blocks = np.concatenate([diag_blocks, pair_blocks]).astype(np.float32)
+    order = np.lexsort((cols, rows))
+    rows, cols, blocks = rows[order], cols[order], blocks[order]
+    row_offset = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(row_offset, rows + 1, 1)
+    row_offset = np.cumsum(row_offset)
+    return (row_offset.astype(np.int32), cols.astype(np.int32),
+            _colmajor(blocks).astype(np.float32))
+
+
+def haar_rotations(rng, k):
+    g = rng.standard_normal((k, 3, 3))
+    q, r = np.linalg.qr(g)
+    d = np.sign(np.diagonal(r, axis1=1, axis2=2))
+    d[d == 0] = 1.0
+    return q * d[:, None, :]
+
+
+def grid_spd_system(res, seed_matrix=0, seed_rhs=1, mass=1.0, h=1.0 / 60.0,
+                    lam_lo=1e2, lam_hi=1e6):
+    """Kernel-isolation PCG system on a res x res triangulated grid.
+
+    A = mass*I + h^2 * sum_edges [[K_e, -K_e], [-K_e, K_e]], K_e = Q^T diag(l) Q,
+    l log-uniform in [lam_lo, lam_hi], Q Haar-random (seed_matrix); b ~ N(0,1)
+    (seed_rhs); x0 = 0. SPD by construction. Returns a dict of numpy arrays:
+    row_offset, col_index, value (upper blocks incl. diagonal stiffness sums),
+    cdiag (mass*I per row), b, edges.
+    """
+    n = res * res
+    edges = unique_edges(rect_faces(res, res)).astype(np.int64)
+    rng = np.random.default_rng(seed_matrix)
+    ne = len(edges)
+    lam = np.exp(rng.uniform(np.log(lam_lo), np.log(lam_hi), size=(ne, 3)))
+    q = haar_rotations(rng, ne)
+    K = np.einsum("eki,ek,ekj->eij", q, lam, q) * (h * h)
+    K = 0.5 * (K + K.transpose(0, 2, 1))
+    diag = np.zeros((n, 3, 3))
+    for c in range(2):
+        for a in range(3):
+            for b in range(3):
+                diag[:, a, b] += np.bincount(edges[:, c], weights=K[:, a, b],
+                                             minlength=n)
+    ro, ci, val = upper_bsr(n, edges, -K, diag)
+    cdiag = _colmajor(np.broadcast_to(mass * np.eye(3), (n, 3, 3))).astype(np.float32)
+    b = np.random.default_rng(seed_rhs).standard_normal(3 * n).astype(np.float32)
+    return dict(n=n, row_offset=ro, col_index=ci, value=val, cdiag=cdiag, b=b,
+                edges=edges.astype(np.uint32))
+
+
+def element_spd_system(res, seed=0, mass=1.0, scale=10.0):
+    """SPD test system whose off-diagonal blocks are NOT symmetric: per face a
+    random 9x9 G^T G element matrix (so the (i,j) and (j,i) blocks of every pair
+    are transposes, not copies), plus mass*I on the diagonal."""
+    n = res * res
+    faces = rect_faces(res, res).astype(np.int64)
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((len(faces), 6, 9))
+    E = np.einsum("fki,fkj->fij", g, g) * (scale / 9.0)
+    blocks = {}
+    for f in range(len(faces)):
+        for a in range(3):
+            for c in range(3):
+                i, j = int(faces[f, a]), int(faces[f, c])
+                if i <= j:
+                    blk = E[f, 3 * a:3 * a + 3, 3 * c:3 * c + 3]
+                    blocks[(i, j)] = blocks.get((i, j), 0) + blk
+    diag = np.zeros((n, 3, 3))
+    pairs, pblk = [], []
+    for (i, j), blk in blocks.items():
+        if i == j:
+            diag[i] = blk
+        else:
+            pairs.append((i, j))
+            pblk.append(blk)
+    ro, ci, val = upper_bsr(n, np.array(pairs), np.array(pblk), diag)
+    cdiag = _colmajor(np.broadcast_to(mass * np.eye(3), (n, 3, 3))).astype(np.float32)
+    b = rng.standard_normal(3 * n).astype(np.float32)
+    return dict(n=n, row_offset=ro, col_index=ci, value=val, cdiag=cdiag, b=b)
+
+
+def dense_from_upper(n, row_offset, col_index, value, cdiag):
+    """fp64 dense matrix of the split operator (small n only)."""
+    M = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        for k in range(row_offset[i], row_offset[i + 1]):
+            j = int(col_index[k])
+            blk = value[k].reshape(3, 3).T.astype(np.float64)  # column-major
+            M[3 * i:3 * i + 3, 3 * j:3 * j + 3] += blk
+            if j != i:
+                M[3 * j:3 * j + 3, 3 * i:3 * i + 3] += blk.T
+        M[3 * i:3 * i + 3, 3 * i:3 * i + 3] += cdiag[i].reshape(3, 3).T
+    return M
+
+
+def sparse_from_upper(n, row_offset, col_index, value, cdiag):
+    """fp64 scipy CSR of the split operator."""
+    import scipy.sparse as sp
+    rows = np.repeat(np.arange(n), np.diff(row_offset))
+    cols = col_index.astype(np.int64)
+    blk = value.reshape(-1, 3, 3).transpose(0, 2, 1).astype(np.float64)  # [r, c]
+    r_list, c_list, v_list = [], [], []
+    a, b = np.meshgrid(np.arange(3), np.arange(3), indexing="ij")
+    a, b = a.reshape(-1), b.reshape(-1)
+    r_list.append((3 * rows[:, None] + a).reshape(-1))
+    c_list.append((3 * cols[:, None] + b).reshape(-1))
+    v_list.append(blk[:, a, b].reshape(-1))
+    off = rows != cols
+    r_list.append((3 * cols[off][:, None] + b).reshape(-1))
+    c_list.append((3 * rows[off][:, None] + a).reshape(-1))
+    v_list.append(blk[off][:, a, b].reshape(-1))
+    cd = cdiag.reshape(-1, 3, 3).transpose(0, 2, 1).astype(np.float64)
+    ii = np.arange(n)
+    r_list.append((3 * ii[:, None] + a).reshape(-1))
+    c_list.append((3 * ii[:, None] + b).reshape(-1))
+    v_list.append(cd[:, a, b].reshape(-1))
+    return sp.csr_matrix((np.concatenate(v_list), (np.concatenate(r_list),
+                                                    np.concatenate(c_list))),
+                         shape=(3 * n, 3 * n))
