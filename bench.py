@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run a 256 MiB device copy (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -73,6 +75,14 @@ def main():
     from ppf_amd.solver import BlockJacobiPCG
 
     dev = torch.device("cuda:0")
+    if args.copy_calib:
+        # A plain copy of known size: 256 MiB read + 256 MiB written. Its
+        # FETCH_SIZE / WRITE_SIZE calibrate the counters of the kernels below.
+        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     s = scenes.grid_spd_system(args.res)
     n, nnz = s["n"], len(s["col_index"])
     t = {k: torch.from_numpy(np.ascontiguousarray(s[k])).to(dev).reshape(-1)
