@@ -9,7 +9,17 @@
 
 #define PPF_HD __host__ __device__ inline
 
+// Global (HBM) address space. Pointers read out of a device-side argument
+// struct are otherwise generic, which turns every access into a flat
+// instruction that also waits on the LDS/scalar counter.
+#define PPF_GLOBAL __attribute__((address_space(1)))
+
 namespace ppf {
+
+template <class T> using gptr = PPF_GLOBAL T *;
+template <class T> __device__ inline gptr<T> as_global(T *p) {
+    return (gptr<T>)p;
+}
 
 constexpr int kWave = 64;
 constexpr int kXcd = 8; // MI355X: 8 XCDs, workgroups dispatched round-robin
@@ -32,6 +42,22 @@ PPF_HD float dot3(f3 a, f3 b) {
 PPF_HD f3 cross3(f3 a, f3 b) {
     return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z,
                a.x * b.y - a.y * b.x);
+}
+__device__ inline f3 load3(gptr<const float> p) { return mk3(p[0], p[1], p[2]); }
+__device__ inline void store3(gptr<float> p, f3 v) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+__device__ inline f3 mv(gptr<const float> a, f3 x) {
+    return mk3(a[0] * x.x + a[3] * x.y + a[6] * x.z,
+               a[1] * x.x + a[4] * x.y + a[7] * x.z,
+               a[2] * x.x + a[5] * x.y + a[8] * x.z);
+}
+__device__ inline f3 mvt(gptr<const float> a, f3 x) {
+    return mk3(a[0] * x.x + a[1] * x.y + a[2] * x.z,
+               a[3] * x.x + a[4] * x.y + a[5] * x.z,
+               a[6] * x.x + a[7] * x.y + a[8] * x.z);
 }
 PPF_HD f3 load3(const float *p) { return mk3(p[0], p[1], p[2]); }
 PPF_HD void store3(float *p, f3 v) {

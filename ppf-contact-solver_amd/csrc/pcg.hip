@@ -39,6 +39,7 @@ constexpr int kLanes = 8;          // lanes per row in the SpMV
 constexpr int kRowsPerPass = kNT / kLanes;
 constexpr int kGraphIters = 8;     // iterations per captured graph (even)
 constexpr unsigned kNone = 0xffffffffu;
+constexpr unsigned kMirror = 0x80000000u; // entry slot flag: apply transposed
 constexpr float kNoiseSlack = 256.0f;       // solver.cu:69
 constexpr float kFloatEps = 1.19209290e-7f; // solver.cu:70
 
@@ -68,8 +69,10 @@ struct Args {
     uint32_t G1, G2, G3; // workgroups of the SpMV / update / p kernels
     uint32_t R1, R2, R3; // contiguous rows (elements for G3) per workgroup
     const uint32_t *rp, *col, *tp, *tcol, *tslot, *dslot;
+    const uint2 *ent;  // fixed-width entry table (ELL form), or null
+    uint32_t ell_m;    // entries per lane of the ELL kernel (row width 4*m)
     const float *val, *cdiag, *b;
-    float *x, *D, *P, *r, *z, *p, *Ap;
+    float *x, *P, *r, *z, *p, *Ap;
     float *part; // [G1 pAp][G1 |terms|][G2 rz][G2 err]
     float *err_hist;
     Scalars *s;
@@ -178,45 +181,53 @@ __global__ __launch_bounds__(kNT) void k_diag_invert(const Args *__restrict__ A)
     if (i >= A->n) {
         return;
     }
-    const uint32_t ds = A->dslot[i];
+    gptr<const float> val = as_global(A->val);
+    gptr<const float> cd = as_global(A->cdiag);
+    gptr<float> P = as_global(A->P);
+    const uint32_t ds = as_global(A->dslot)[i];
     float d[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
-        const float bii = ds != kNone ? A->val[9 * (size_t)ds + e] : 0.0f;
-        d[e] = bii + A->cdiag[9 * (size_t)i + e];
+        const float bii = ds != kNone ? val[9 * (size_t)ds + e] : 0.0f;
+        d[e] = bii + cd[9 * (size_t)i + e];
     }
     float inv[9];
     const bool ok = bj_invert(d, inv);
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
-        A->D[9 * (size_t)i + e] = d[e];
-        A->P[9 * (size_t)i + e] = inv[e];
+        P[9 * (size_t)i + e] = inv[e];
     }
     if (!ok) {
         atomicAdd(&A->s->bad, 1u);
     }
 }
 
-// y = M v (+ per-workgroup partials of v.Mv and sum|terms| when DOT).
-// 8 lanes per row; workgroups own contiguous, XCD-grouped row ranges.
+// y = M v (+ per-workgroup partials of v.Mv and sum|terms| when DOT), the
+// reference's walk form (solver.cu:812-886): stored blocks of row i (diagonal
+// included), mirrored blocks (j, i), j < i, applied transposed, then C_i.
+// 8 lanes per row: lanes 0-3 walk the stored blocks and lanes 4-7 the mirror
+// list, so both dependent load chains are in flight together; lane 7 adds
+// C_i v_i. Workgroups own contiguous, XCD-grouped row ranges.
 template <bool DOT>
 __global__ __launch_bounds__(kNT) void k_spmv(const Args *__restrict__ A,
-                                              const float *__restrict__ v,
-                                              float *__restrict__ y) {
+                                              const float *__restrict__ vin,
+                                              float *__restrict__ yout) {
     __shared__ float lds[kNT / kWave];
-    if (DOT && A->s->done) {
+    if (DOT && as_global(A->s)->done) {
         return;
     }
     const uint32_t n = A->n;
     const uint32_t L = xcd_logical_block();
     const uint32_t lane = threadIdx.x % kLanes, sub = threadIdx.x / kLanes;
-    const uint32_t *__restrict__ rp = A->rp;
-    const uint32_t *__restrict__ col = A->col;
-    const uint32_t *__restrict__ tp = A->tp;
-    const uint32_t *__restrict__ tcol = A->tcol;
-    const uint32_t *__restrict__ tslot = A->tslot;
-    const float *__restrict__ val = A->val;
-    const float *__restrict__ D = A->D;
+    const bool mirror = lane >= 4;
+    const uint32_t q = lane & 3u;
+    gptr<const uint32_t> offs = as_global(mirror ? A->tp : A->rp);
+    gptr<const uint32_t> cols = as_global(mirror ? A->tcol : A->col);
+    gptr<const uint32_t> tslot = as_global(A->tslot);
+    gptr<const float> val = as_global(A->val);
+    gptr<const float> cd = as_global(A->cdiag);
+    gptr<const float> v = as_global(vin);
+    gptr<float> y = as_global(yout);
     const uint32_t lo = L * A->R1;
     const uint32_t hi = min(n, lo + A->R1);
     float dot = 0.0f, adot = 0.0f;
@@ -224,25 +235,31 @@ __global__ __launch_bounds__(kNT) void k_spmv(const Args *__restrict__ A,
         const uint32_t i = base + sub;
         f3 sum = mk3(0, 0, 0), as = mk3(0, 0, 0);
         const bool valid = i < hi;
+        f3 vi = mk3(0, 0, 0);
         if (valid) {
-            const uint32_t k1 = rp[i + 1];
-            for (uint32_t k = rp[i] + lane; k < k1; k += kLanes) {
-                const uint32_t j = col[k];
-                if (j != i) {
-                    const f3 t = mv(val + 9 * (size_t)k, load3(v + 3 * (size_t)j));
-                    sum = sum + t;
-                    as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+            vi = load3(v + 3 * (size_t)i);
+            const uint32_t k1 = offs[i + 1];
+            for (uint32_t k = offs[i] + q; k < k1; k += 4) {
+                const uint32_t j = cols[k];
+                const uint32_t s = mirror ? tslot[k] : k;
+                gptr<const float> a = val + 9 * (size_t)s;
+                float m[9];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    m[e] = a[e];
                 }
-            }
-            const uint32_t t1 = tp[i + 1];
-            for (uint32_t k = tp[i] + lane; k < t1; k += kLanes) {
-                const f3 t = mvt(val + 9 * (size_t)tslot[k],
-                                 load3(v + 3 * (size_t)tcol[k]));
+                if (mirror) { // apply the stored block transposed
+                    float t;
+                    t = m[1], m[1] = m[3], m[3] = t;
+                    t = m[2], m[2] = m[6], m[6] = t;
+                    t = m[5], m[5] = m[7], m[7] = t;
+                }
+                const f3 t = mv(m, load3(v + 3 * (size_t)j));
                 sum = sum + t;
                 as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
             }
-            if (lane == 0) {
-                const f3 t = mv(D + 9 * (size_t)i, load3(v + 3 * (size_t)i));
+            if (lane == kLanes - 1) {
+                const f3 t = mv(cd + 9 * (size_t)i, vi);
                 sum = sum + t;
                 as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
             }
@@ -261,7 +278,6 @@ __global__ __launch_bounds__(kNT) void k_spmv(const Args *__restrict__ A,
         if (valid && lane == 0) {
             store3(y + 3 * (size_t)i, sum);
             if (DOT) {
-                const f3 vi = load3(v + 3 * (size_t)i);
                 dot = dot + vi.x * sum.x;
                 dot = dot + vi.y * sum.y;
                 dot = dot + vi.z * sum.z;
@@ -275,8 +291,149 @@ __global__ __launch_bounds__(kNT) void k_spmv(const Args *__restrict__ A,
         dot = block_sum<kNT>(dot, lds);
         adot = block_sum<kNT>(adot, lds);
         if (threadIdx.x == 0) {
-            A->part[L] = dot;
-            A->part[A->G1 + L] = adot;
+            gptr<float> part = as_global(A->part);
+            part[L] = dot;
+            part[A->G1 + L] = adot;
+        }
+    }
+}
+
+// Longest row of the split operator: stored blocks plus mirrored blocks.
+__global__ void k_row_len_max(uint32_t n, const uint32_t *__restrict__ rp,
+                              const uint32_t *__restrict__ tp,
+                              uint32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    atomicMax(out, (rp[i + 1] - rp[i]) + (tp[i + 1] - tp[i]));
+}
+
+// Per-row entry table of width W: the row's stored blocks (column, slot),
+// then its mirrored blocks (source row, slot | kMirror), padded with kNone.
+__global__ void k_build_ell(uint32_t n, uint32_t W, const uint32_t *__restrict__ rp,
+                            const uint32_t *__restrict__ col,
+                            const uint32_t *__restrict__ tp,
+                            const uint32_t *__restrict__ tcol,
+                            const uint32_t *__restrict__ tslot,
+                            uint2 *__restrict__ ent) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    uint2 *row = ent + (size_t)i * W;
+    uint32_t e = 0;
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        row[e++] = make_uint2(col[k], k);
+    }
+    for (uint32_t k = tp[i]; k < tp[i + 1]; ++k) {
+        row[e++] = make_uint2(tcol[k], tslot[k] | kMirror);
+    }
+    for (; e < W; ++e) {
+        row[e] = make_uint2(0u, kNone);
+    }
+}
+
+// ELL form of k_spmv for rows of at most 4*M entries: 4 lanes per row, lane q
+// takes entries q, q+4, ... of the row's fixed-width table, so no row-offset
+// load sits in the dependent chain and all M block loads of a lane are issued
+// before any is consumed. Padding entries read slot 0 and are discarded.
+template <bool DOT, int M>
+__global__ __launch_bounds__(kNT) void k_spmv_ell(const Args *__restrict__ A,
+                                                  const float *__restrict__ vin,
+                                                  float *__restrict__ yout) {
+    constexpr uint32_t W = 4 * M;
+    constexpr uint32_t kRows = kNT / 4;
+    __shared__ float lds[kNT / kWave];
+    if (DOT && as_global(A->s)->done) {
+        return;
+    }
+    const uint32_t n = A->n;
+    const uint32_t L = xcd_logical_block();
+    const uint32_t q = threadIdx.x & 3u, sub = threadIdx.x >> 2;
+    gptr<const uint64_t> ent = as_global(reinterpret_cast<const uint64_t *>(A->ent));
+    gptr<const float> val = as_global(A->val);
+    gptr<const float> cd = as_global(A->cdiag);
+    gptr<const float> v = as_global(vin);
+    gptr<float> y = as_global(yout);
+    const uint32_t lo = L * A->R1;
+    const uint32_t hi = min(n, lo + A->R1);
+    float dot = 0.0f, adot = 0.0f;
+    for (uint32_t base = lo; base < hi; base += kRows) {
+        const uint32_t i = base + sub;
+        const bool valid = i < hi;
+        const uint32_t ii = valid ? i : lo;
+        uint2 e[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const uint64_t w = ent[(size_t)ii * W + q + 4 * m];
+            e[m] = make_uint2((uint32_t)w, (uint32_t)(w >> 32));
+        }
+        const f3 vi = load3(v + 3 * (size_t)ii);
+        f3 sum = mk3(0, 0, 0), as = mk3(0, 0, 0);
+        float blk[M][9];
+        f3 vj[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const uint32_t slot = e[m].y == kNone ? 0u : (e[m].y & ~kMirror);
+            gptr<const float> a = val + 9 * (size_t)slot;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                blk[m][c] = a[c];
+            }
+            vj[m] = load3(v + 3 * (size_t)e[m].x);
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            float *b = blk[m];
+            if (e[m].y != kNone && (e[m].y & kMirror)) {
+                float t;
+                t = b[1], b[1] = b[3], b[3] = t;
+                t = b[2], b[2] = b[6], b[6] = t;
+                t = b[5], b[5] = b[7], b[7] = t;
+            }
+            f3 t = mv(b, vj[m]);
+            if (e[m].y == kNone) {
+                t = mk3(0, 0, 0);
+            }
+            sum = sum + t;
+            as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+        }
+        if (q == 3) {
+            const f3 t = mv(cd + 9 * (size_t)ii, vi);
+            sum = sum + t;
+            as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+        }
+#pragma unroll
+        for (int off = 2; off > 0; off >>= 1) {
+            sum.x += __shfl_xor(sum.x, off, 4);
+            sum.y += __shfl_xor(sum.y, off, 4);
+            sum.z += __shfl_xor(sum.z, off, 4);
+            if (DOT) {
+                as.x += __shfl_xor(as.x, off, 4);
+                as.y += __shfl_xor(as.y, off, 4);
+                as.z += __shfl_xor(as.z, off, 4);
+            }
+        }
+        if (valid && q == 0) {
+            store3(y + 3 * (size_t)i, sum);
+            if (DOT) {
+                dot = dot + vi.x * sum.x;
+                dot = dot + vi.y * sum.y;
+                dot = dot + vi.z * sum.z;
+                adot = adot + fabsf(vi.x) * as.x;
+                adot = adot + fabsf(vi.y) * as.y;
+                adot = adot + fabsf(vi.z) * as.z;
+            }
+        }
+    }
+    if (DOT) {
+        dot = block_sum<kNT>(dot, lds);
+        adot = block_sum<kNT>(adot, lds);
+        if (threadIdx.x == 0) {
+            gptr<float> part = as_global(A->part);
+            part[L] = dot;
+            part[A->G1 + L] = adot;
         }
     }
 }
@@ -287,17 +444,22 @@ __global__ __launch_bounds__(kNT) void k_init_residual(const Args *__restrict__ 
     const uint32_t n = A->n;
     const uint32_t L = blockIdx.x;
     const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
+    gptr<const float> b = as_global(A->b);
+    gptr<const float> yv = as_global(A->Ap);
+    gptr<const float> P = as_global(A->P);
+    gptr<float> r = as_global(A->r);
+    gptr<float> z = as_global(A->z);
+    gptr<float> p = as_global(A->p);
     float rz = 0.0f, err = 0.0f;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += kNT) {
-        f3 ri;
-        const f3 bi = load3(A->b + 3 * (size_t)i);
-        const f3 yi = load3(A->Ap + 3 * (size_t)i);
-        ri = mk3(bi.x + (-1.0f) * yi.x, bi.y + (-1.0f) * yi.y,
-                 bi.z + (-1.0f) * yi.z);
-        const f3 zi = mv(A->P + 9 * (size_t)i, ri);
-        store3(A->r + 3 * (size_t)i, ri);
-        store3(A->z + 3 * (size_t)i, zi);
-        store3(A->p + 3 * (size_t)i, zi);
+        const f3 bi = load3(b + 3 * (size_t)i);
+        const f3 yi = load3(yv + 3 * (size_t)i);
+        const f3 ri = mk3(bi.x + (-1.0f) * yi.x, bi.y + (-1.0f) * yi.y,
+                          bi.z + (-1.0f) * yi.z);
+        const f3 zi = mv(P + 9 * (size_t)i, ri);
+        store3(r + 3 * (size_t)i, ri);
+        store3(z + 3 * (size_t)i, zi);
+        store3(p + 3 * (size_t)i, zi);
         rz = rz + ri.x * zi.x;
         rz = rz + ri.y * zi.y;
         rz = rz + ri.z * zi.z;
@@ -308,12 +470,13 @@ __global__ __launch_bounds__(kNT) void k_init_residual(const Args *__restrict__ 
     rz = block_sum<kNT>(rz, lds);
     err = block_sum<kNT>(err, lds);
     if (threadIdx.x == 0) {
-        A->part[2 * A->G1 + L] = rz;
-        A->part[2 * A->G1 + A->G2 + L] = err;
+        gptr<float> part = as_global(A->part);
+        part[2 * A->G1 + L] = rz;
+        part[2 * A->G1 + A->G2 + L] = err;
     }
 }
 
-__device__ inline float reduce_parts(const float *__restrict__ part, uint32_t cnt,
+__device__ inline float reduce_parts(gptr<const float> part, uint32_t cnt,
                                      float *lds) {
     float acc = 0.0f;
     for (uint32_t k = threadIdx.x; k < cnt; k += kNT) {
@@ -325,10 +488,11 @@ __device__ inline float reduce_parts(const float *__restrict__ part, uint32_t cn
 // Initial scalars and the bookkeeping state (solver.cu:1020-1031, 1170-1175).
 __global__ __launch_bounds__(kNT) void k_init_scalars(const Args *__restrict__ A) {
     __shared__ float lds[kNT / kWave];
-    const float rz0 = reduce_parts(A->part + 2 * A->G1, A->G2, lds);
-    const float err0 = reduce_parts(A->part + 2 * A->G1 + A->G2, A->G2, lds);
+    gptr<const float> part = as_global(A->part);
+    const float rz0 = reduce_parts(part + 2 * A->G1, A->G2, lds);
+    const float err0 = reduce_parts(part + 2 * A->G1 + A->G2, A->G2, lds);
     if (threadIdx.x == 0) {
-        Scalars *s = A->s;
+        gptr<Scalars> s = as_global(A->s);
         s->rz[0] = rz0;
         s->rz[1] = rz0;
         s->err0 = err0;
@@ -360,12 +524,13 @@ __global__ __launch_bounds__(kNT) void k_init_scalars(const Args *__restrict__ A
 __global__ __launch_bounds__(kNT) void k_update(const Args *__restrict__ A,
                                                 uint32_t k) {
     __shared__ float lds[kNT / kWave];
-    Scalars *s = A->s;
+    gptr<Scalars> s = as_global(A->s);
     if (s->done) {
         return;
     }
-    const float pAp = reduce_parts(A->part, A->G1, lds);
-    const float absdot = reduce_parts(A->part + A->G1, A->G1, lds);
+    gptr<const float> part = as_global(A->part);
+    const float pAp = reduce_parts(part, A->G1, lds);
+    const float absdot = reduce_parts(part + A->G1, A->G1, lds);
     const float rz0 = s->rz[k & 1];
     const float bound = kNoiseSlack * kFloatEps * absdot;
     float alpha = 0.0f;
@@ -379,12 +544,12 @@ __global__ __launch_bounds__(kNT) void k_update(const Args *__restrict__ A,
     const uint32_t n = A->n;
     const uint32_t L = blockIdx.x;
     const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
-    float *__restrict__ x = A->x;
-    float *__restrict__ r = A->r;
-    float *__restrict__ z = A->z;
-    const float *__restrict__ p = A->p;
-    const float *__restrict__ Ap = A->Ap;
-    const float *__restrict__ P = A->P;
+    gptr<float> x = as_global(A->x);
+    gptr<float> r = as_global(A->r);
+    gptr<float> z = as_global(A->z);
+    gptr<const float> p = as_global(A->p);
+    gptr<const float> Ap = as_global(A->Ap);
+    gptr<const float> P = as_global(A->P);
     float rz = 0.0f, err = 0.0f;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += kNT) {
         const size_t o = 3 * (size_t)i;
@@ -408,8 +573,9 @@ __global__ __launch_bounds__(kNT) void k_update(const Args *__restrict__ A,
     rz = block_sum<kNT>(rz, lds);
     err = block_sum<kNT>(err, lds);
     if (threadIdx.x == 0) {
-        A->part[2 * A->G1 + L] = rz;
-        A->part[2 * A->G1 + A->G2 + L] = err;
+        gptr<float> pw = as_global(A->part);
+        pw[2 * A->G1 + L] = rz;
+        pw[2 * A->G1 + A->G2 + L] = err;
     }
 }
 
@@ -418,26 +584,27 @@ __global__ __launch_bounds__(kNT) void k_update(const Args *__restrict__ A,
 __global__ __launch_bounds__(kNT) void k_pupdate(const Args *__restrict__ A,
                                                  uint32_t k) {
     __shared__ float lds[kNT / kWave];
-    Scalars *s = A->s;
+    gptr<Scalars> s = as_global(A->s);
     if (s->done) {
         return;
     }
-    const float rz1 = reduce_parts(A->part + 2 * A->G1, A->G2, lds);
+    gptr<const float> part = as_global(A->part);
+    const float rz1 = reduce_parts(part + 2 * A->G1, A->G2, lds);
     const float rz0 = s->rz[k & 1];
     const bool rz_ok = rz0 > 0.0f;
     const float beta = rz_ok ? rz1 / rz0 : 0.0f;
     const uint32_t n3 = 3 * A->n;
     const uint32_t L = blockIdx.x;
     const uint32_t lo = L * A->R3, hi = min(n3, lo + A->R3);
-    float *__restrict__ p = A->p;
-    const float *__restrict__ z = A->z;
+    gptr<float> p = as_global(A->p);
+    gptr<const float> z = as_global(A->z);
     for (uint32_t e = lo + threadIdx.x; e < hi; e += kNT) {
         p[e] = fmaf(beta, p[e], 1.0f * z[e]);
     }
     if (blockIdx.x != 0) {
         return;
     }
-    const float err = reduce_parts(A->part + 2 * A->G1 + A->G2, A->G2, lds);
+    const float err = reduce_parts(part + 2 * A->G1 + A->G2, A->G2, lds);
     if (threadIdx.x != 0) {
         return;
     }
@@ -450,7 +617,7 @@ __global__ __launch_bounds__(kNT) void k_pupdate(const Args *__restrict__ A,
     const uint32_t it = s->iter + 1;
     s->iter = it;
     if (A->err_hist) {
-        A->err_hist[it - 1] = err;
+        as_global(A->err_hist)[it - 1] = err;
     }
     const bool at_cap = it >= s->max_iter;
     if (it == s->next_check || at_cap) {
@@ -492,6 +659,7 @@ struct PcgPlanImpl {
     Args *d_args = nullptr;
     Scalars *h_scal = nullptr; // pinned
     void *mem = nullptr;       // one device allocation for all scratch
+    void *ell = nullptr;       // ELL entry table (when rows are short)
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -509,9 +677,29 @@ size_t carve_bytes(size_t count, size_t elem) {
     return (count * elem + 255) & ~size_t(255);
 }
 
+// y = M v with the plan's SpMV form (ELL width 4*m, or the CSR walk).
+template <bool DOT>
+hipError_t launch_spmv(const PcgPlanImpl &pl, const float *v, float *y,
+                       hipStream_t st) {
+    const Args &h = pl.h;
+    switch (h.ell_m) {
+    case 1: k_spmv_ell<DOT, 1><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+    case 2: k_spmv_ell<DOT, 2><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+    case 3: k_spmv_ell<DOT, 3><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+    case 4: k_spmv_ell<DOT, 4><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+    case 6: k_spmv_ell<DOT, 6><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+    case 8: k_spmv_ell<DOT, 8><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+    default: k_spmv<DOT><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
     const Args &h = pl.h;
-    k_spmv<true><<<h.G1, kNT, 0, st>>>(pl.d_args, h.p, h.Ap);
+    hipError_t e = launch_spmv<true>(pl, h.p, h.Ap, st);
+    if (e != hipSuccess) {
+        return e;
+    }
     k_update<<<h.G2, kNT, 0, st>>>(pl.d_args, k);
     k_pupdate<<<h.G3, kNT, 0, st>>>(pl.d_args, k);
     return hipGetLastError();
@@ -521,7 +709,7 @@ hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
 
 int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
                     const uint32_t *col, hipStream_t st, PcgPlanImpl **out) {
-    if (!out || n == 0 || !rp || !col) {
+    if (!out || n == 0 || !rp || !col || nnz >= kMirror) {
         set_error("pcg_plan_create: invalid argument");
         return PPF_ERR_ARG;
     }
@@ -533,8 +721,9 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     pl->nnz = nnz;
     Args &h = pl->h;
     h.n = n;
-    h.G1 = round8(std::min<uint32_t>(2048u, cdiv(n, kRowsPerPass)));
-    h.R1 = cdiv(cdiv(n, h.G1), kRowsPerPass) * kRowsPerPass;
+    // Row ranges are multiples of 64 rows: one pass of either SpMV form.
+    h.G1 = round8(std::min<uint32_t>(2048u, cdiv(n, 64)));
+    h.R1 = cdiv(cdiv(n, h.G1), 64) * 64;
     h.G2 = std::min<uint32_t>(1024u, cdiv(n, kNT));
     h.R2 = cdiv(cdiv(n, h.G2), kNT) * kNT;
     h.G3 = std::min<uint32_t>(1024u, cdiv(3 * n, kNT));
@@ -543,7 +732,7 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     const size_t n3 = 3 * (size_t)n;
     const size_t scan_words = scan_scratch_words(n + 1);
     size_t bytes = carve_bytes(n + 1, 4) * 2 + carve_bytes(nnz, 4) * 2 +
-                   carve_bytes(n, 4) + carve_bytes(9 * (size_t)n, 4) * 2 +
+                   carve_bytes(n, 4) + carve_bytes(9 * (size_t)n, 4) +
                    carve_bytes(n3, 4) * 4 +
                    carve_bytes(2 * (size_t)h.G1 + 2 * (size_t)h.G2, 4) +
                    carve_bytes(1, sizeof(Scalars)) + carve_bytes(1, sizeof(Args)) +
@@ -561,7 +750,6 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     uint32_t *tcol = carve<uint32_t>(cur, nnz);
     uint32_t *tslot = carve<uint32_t>(cur, nnz);
     uint32_t *dslot = carve<uint32_t>(cur, n);
-    h.D = carve<float>(cur, 9 * (size_t)n);
     h.P = carve<float>(cur, 9 * (size_t)n);
     h.r = carve<float>(cur, n3);
     h.z = carve<float>(cur, n3);
@@ -624,6 +812,31 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
         ppf::pcg_plan_destroy(pl);
         return PPF_ERR_ARG;
     }
+    // Choose the SpMV form from the longest row (stored + mirrored blocks).
+    PLAN_HIP(hipMemsetAsync(flags + 2, 0, sizeof(uint32_t), st));
+    k_row_len_max<<<gb, kNT, 0, st>>>(n, rp, tp, flags + 2);
+    PLAN_HIP(hipGetLastError());
+    uint32_t maxlen = 0;
+    PLAN_HIP(hipMemcpyAsync(&maxlen, flags + 2, sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, st));
+    PLAN_HIP(hipStreamSynchronize(st));
+    h.ell_m = 0;
+    for (uint32_t m : {1u, 2u, 3u, 4u, 6u, 8u}) {
+        // Padding entries read block 0, so the ELL form needs nnz >= 1.
+        if (nnz > 0 && maxlen <= 4 * m) {
+            h.ell_m = m;
+            break;
+        }
+    }
+    h.ent = nullptr;
+    if (h.ell_m) {
+        const uint32_t W = 4 * h.ell_m;
+        PLAN_HIP(hipMalloc(&pl->ell, (size_t)n * W * sizeof(uint2)));
+        h.ent = static_cast<const uint2 *>(pl->ell);
+        k_build_ell<<<gb, kNT, 0, st>>>(n, W, rp, col, tp, tcol, tslot,
+                                        static_cast<uint2 *>(pl->ell));
+        PLAN_HIP(hipGetLastError());
+    }
     PLAN_HIP(hipHostMalloc(reinterpret_cast<void **>(&pl->h_scal),
                            sizeof(Scalars), hipHostMallocDefault));
 
@@ -673,6 +886,9 @@ void pcg_plan_destroy(PcgPlanImpl *pl) {
     if (pl->mem) {
         (void)hipFree(pl->mem);
     }
+    if (pl->ell) {
+        (void)hipFree(pl->ell);
+    }
     delete pl;
 }
 
@@ -689,8 +905,7 @@ int pcg_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
     PPF_HIP(hipMemsetAsync(&h.s->bad, 0, sizeof(uint32_t), st));
     k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
     PPF_HIP(hipGetLastError());
-    k_spmv<false><<<h.G1, kNT, 0, st>>>(pl->d_args, x, y);
-    PPF_HIP(hipGetLastError());
+    PPF_HIP(launch_spmv<false>(*pl, x, y, st));
     PPF_HIP(hipStreamSynchronize(st));
     return PPF_OK;
 }
@@ -718,10 +933,10 @@ int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
     hipEvent_t e0, e1;
     PPF_HIP(hipEventCreate(&e0));
     PPF_HIP(hipEventCreate(&e1));
-    k_spmv<true><<<h.G1, kNT, 0, st>>>(pl->d_args, x, y); // warm
+    PPF_HIP(launch_spmv<true>(*pl, x, y, st)); // warm
     PPF_HIP(hipEventRecord(e0, st));
     for (uint32_t r = 0; r < reps; ++r) {
-        k_spmv<true><<<h.G1, kNT, 0, st>>>(pl->d_args, x, y);
+        PPF_HIP(launch_spmv<true>(*pl, x, y, st));
     }
     PPF_HIP(hipEventRecord(e1, st));
     PPF_HIP(hipEventSynchronize(e1));
@@ -755,8 +970,7 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
                            st));
     k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
     PPF_HIP(hipGetLastError());
-    k_spmv<false><<<h.G1, kNT, 0, st>>>(pl->d_args, x, h.Ap);
-    PPF_HIP(hipGetLastError());
+    PPF_HIP(launch_spmv<false>(*pl, x, h.Ap, st));
     k_init_residual<<<h.G2, kNT, 0, st>>>(pl->d_args);
     PPF_HIP(hipGetLastError());
     k_init_scalars<<<1, kNT, 0, st>>>(pl->d_args);

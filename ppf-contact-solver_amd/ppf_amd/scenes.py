@@ -166,3 +166,27 @@ def sparse_from_upper(n, row_offset, col_index, value, cdiag):
     return sp.csr_matrix((np.concatenate(v_list), (np.concatenate(r_list),
                                                     np.concatenate(c_list))),
                          shape=(3 * n, 3 * n))
+
+
+def long_row_spd_system(n=400, hubs=4, fan=48, seed=0, mass=1.0):
+    """SPD system whose hub rows couple to `fan` other rows (rows far longer
+    than a mesh's), built from random rank-3 pair terms [[K, -K], [-K, K]]
+    plus a chain, so both the short-row and long-row SpMV forms are used."""
+    rng = np.random.default_rng(seed)
+    pairs = set((i, i + 1) for i in range(n - 1))
+    for h in range(hubs):
+        for j in rng.choice(n, size=fan, replace=False):
+            if j != h:
+                pairs.add((min(h, int(j)), max(h, int(j))))
+    pairs = np.array(sorted(pairs), dtype=np.int64)
+    q = haar_rotations(rng, len(pairs))
+    lam = np.exp(rng.uniform(0, 3, size=(len(pairs), 3)))
+    K = np.einsum("eki,ek,ekj->eij", q, lam, q)
+    K = 0.5 * (K + K.transpose(0, 2, 1))
+    diag = np.zeros((n, 3, 3))
+    for c in range(2):
+        np.add.at(diag, pairs[:, c], K)
+    ro, ci, val = upper_bsr(n, pairs, -K, diag)
+    cdiag = _colmajor(np.broadcast_to(mass * np.eye(3), (n, 3, 3))).astype(np.float32)
+    b = rng.standard_normal(3 * n).astype(np.float32)
+    return dict(n=n, row_offset=ro, col_index=ci, value=val, cdiag=cdiag, b=b)

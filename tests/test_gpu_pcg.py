@@ -100,10 +100,11 @@ def test_bj_invert_matches_oracle(dev):
     assert bad2 == 2  # -5 I and 0 trap in the reference
 
 
-@pytest.mark.parametrize("kind", ["grid", "element"])
+@pytest.mark.parametrize("kind", ["grid", "element", "longrows"])
 def test_spmv_matches_oracle_and_fp64(dev, kind):
-    s = (scenes.grid_spd_system(64, seed_matrix=2) if kind == "grid"
-         else scenes.element_spd_system(12, seed=3))
+    s = {"grid": lambda: scenes.grid_spd_system(64, seed_matrix=2),
+         "element": lambda: scenes.element_spd_system(12, seed=3),
+         "longrows": lambda: scenes.long_row_spd_system(seed=4)}[kind]()
     t = to_dev(s, dev)
     plan = BlockJacobiPCG(t["row_offset"], t["col_index"])
     x = np.random.default_rng(5).standard_normal(3 * s["n"]).astype(np.float32)
@@ -120,7 +121,9 @@ def test_spmv_matches_oracle_and_fp64(dev, kind):
 CASES = [("grid16", lambda: scenes.grid_spd_system(16, seed_matrix=20, seed_rhs=21)),
          ("grid96", lambda: scenes.grid_spd_system(96, seed_matrix=22, seed_rhs=23)),
          ("grid256", lambda: scenes.grid_spd_system(256, seed_matrix=24, seed_rhs=25)),
-         ("element14", lambda: scenes.element_spd_system(14, seed=26))]
+         ("element14", lambda: scenes.element_spd_system(14, seed=26)),
+         # Rows up to ~50 blocks: exercises the CSR walk instead of the ELL form.
+         ("longrows", lambda: scenes.long_row_spd_system(seed=27))]
 
 
 @pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
