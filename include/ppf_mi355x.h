@@ -104,6 +104,58 @@ int ppf_bj_invert(uint32_t n, const float *d_in, float *d_out, uint32_t *d_bad,
 int ppf_symm3x3(uint32_t n, const float *d_in, float *d_val, float *d_vec,
                 void *stream);
 
+/* ------------------------------------------------------------------------
+ * LBVH and the contact proximity narrow phase.
+ * ---------------------------------------------------------------------- */
+
+/* A contact mesh (all device pointers): positions x (3 nv floats), edges
+ * (2 ne), faces (3 nf), the reference's neighbor tables as CSR
+ * (data.mesh.neighbor: vertex->edge, vertex->face, edge->face; data.hpp:85-98),
+ * per-element contact gap and offset (VertexParam/EdgeParam/FaceParam ghat and
+ * offset, resolved per element), vflag bit0 = pinned (fix_index != 0),
+ * bit1 = collider, vbody = PD rigid body id (0 none), eflag/fflag bit0 =
+ * fixed. */
+typedef struct ppf_contact_mesh {
+    uint32_t nv, ne, nf;
+    const float *x;
+    const uint32_t *edge, *face;
+    const uint32_t *v2e_off, *v2e, *v2f_off, *v2f, *e2f_off, *e2f;
+    const float *v_ghat, *v_off, *e_ghat, *e_off, *f_ghat, *f_off;
+    const uint32_t *vflag, *vbody, *eflag, *fflag;
+} ppf_contact_mesh;
+
+/* One accepted stage-0 pair: kind 0 point-face, 1 point-edge, 2 point-point,
+ * 3 edge-edge; a = query element (vertex; edge for kind 3), b = candidate
+ * element (face, edge, vertex, edge); count = contact multiplicity. */
+typedef struct ppf_pair {
+    uint32_t kind, a, b, count;
+} ppf_pair;
+
+typedef struct ppf_narrow ppf_narrow;
+
+int ppf_narrow_create(ppf_narrow **out);
+void ppf_narrow_destroy(ppf_narrow *h);
+
+/* Build the face, edge and vertex LBVHs of the mesh at its positions
+ * (replaces lbvh::build_face_bvh / build_edge_bvh / build_vertex_bvh,
+ * lbvh/lbvh.cu:863-1026). */
+int ppf_narrow_build(ppf_narrow *h, const ppf_contact_mesh *m, void *stream);
+
+/* Copy one tree (kind 0 face, 1 edge, 2 vertex): 2*(2n-1) node words and
+ * (2n-1) boxes of 8 floats (min xyz, active, max xyz, pad). Either output may
+ * be null; *n and *root are always written. */
+int ppf_narrow_tree(ppf_narrow *h, int kind, uint32_t *d_nodes, float *d_boxes,
+                    uint32_t *n, uint32_t *root, void *stream);
+
+/* All stage-0 proximity pairs of the mesh (the detection half of
+ * contact::embed_contact_force_hessian, contact/contact.cu:1398-1787, with the
+ * acceptance rules of contact.cu:432-870), grouped by query in BVH leaf order.
+ * Writes min(*count, cap) pairs to d_out (may be null to count only). Needs
+ * ppf_narrow_build on the same positions first. */
+int ppf_narrow_proximity(ppf_narrow *h, const ppf_contact_mesh *m,
+                         ppf_pair *d_out, uint64_t cap, uint64_t *count,
+                         void *stream);
+
 #ifdef __cplusplus
 }
 #endif

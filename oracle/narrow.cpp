@@ -1,0 +1,671 @@
+// narrow.cpp -- CPU oracle: LBVH build and the contact proximity narrow phase.
+//
+// TEST INFRASTRUCTURE ONLY (the checker of the HIP path, never shipped).
+//
+// Restates in fp32 (-ffp-contract=off):
+//   * distance::point_edge_distance_coeff        contact/distance.hpp:14-26
+//   * distance::point_triangle_distance_coeff    contact/distance.hpp:46-81
+//   * distance::edge_edge_distance_coeff         contact/distance.hpp:112-192
+//   * LBVH: centroids, scene bounds, 10-bit/axis Morton codes, stable sort,
+//     Karras radix tree with index tie-break, leaf boxes inflated by
+//     0.5*ghat + offset, bottom-up box union       lbvh/lbvh.cu:108-385, 623-857
+//   * aabb::make / overlap / query (stack DFS)    contact/aabb.hpp:15-107
+//   * the stage-0 acceptance rules of the four contact kinds
+//       point-point  (index < vertex, multiplicity from incident edges/faces)
+//                                                  contact/contact.cu:432-620
+//       point-edge   (interior projection, multiplicity from edge faces)
+//                                                  contact/contact.cu:622-715
+//       point-face   (interior projection)         contact/contact.cu:717-785
+//       edge-edge    (edge < index, no shared vertex, interior closest points)
+//                                                  contact/contact.cu:787-870
+//     with the shared filters either_dyn / same_pdrd_body / both_collider and
+//     the query order of contact.cu:1622-1745 (queries in Morton order).
+#include "oracle_math.hpp"
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace {
+
+using orc::V3;
+using orc::v3;
+
+struct Box {
+    float mn[3], mx[3];
+    bool active;
+};
+
+// ---- distance coefficients -------------------------------------------------
+
+inline void pe_coeff(const V3 &p, const V3 &e0, const V3 &e1, float w[2]) {
+    V3 r = orc::sub(e1, e0);
+    float d = orc::dot(r, r);
+    if (d > 0.0f) {
+        float t = orc::dot(r, orc::sub(p, e0)) / d;
+        w[0] = 1.0f - t;
+        w[1] = t;
+    } else {
+        w[0] = w[1] = 1.0f / 2.0f;
+    }
+}
+
+inline void pt_coeff(const V3 &p, const V3 &t0, const V3 &t1, const V3 &t2,
+                     float w[3]) {
+    V3 r0 = orc::sub(t1, t0), r1 = orc::sub(t2, t0), d = orc::sub(p, t0);
+    V3 n = orc::cross(r0, r1);
+    float nn = orc::dot(n, n);
+    if (nn > 0.0f) {
+        float inv = 1.0f / nn;
+        float u = orc::dot(orc::cross(d, r1), n) * inv;
+        float v = orc::dot(orc::cross(r0, d), n) * inv;
+        w[0] = 1.0f - u - v;
+        w[1] = u;
+        w[2] = v;
+        return;
+    }
+    V3 e1 = orc::sub(t2, t1), e2 = orc::sub(t0, t2);
+    float l0 = orc::dot(r0, r0), l1 = orc::dot(e1, e1), l2 = orc::dot(e2, e2);
+    float c[2];
+    if (l0 >= l1 && l0 >= l2) {
+        pe_coeff(p, t0, t1, c);
+        w[0] = c[0], w[1] = c[1], w[2] = 0.0f;
+    } else if (l1 >= l2) {
+        pe_coeff(p, t1, t2, c);
+        w[0] = 0.0f, w[1] = c[0], w[2] = c[1];
+    } else {
+        pe_coeff(p, t2, t0, c);
+        w[0] = c[1], w[1] = 0.0f, w[2] = c[0];
+    }
+}
+
+inline float clamp_unit(float v) { return v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f; }
+
+inline void ee_coeff(const V3 &a0, const V3 &a1, const V3 &b0, const V3 &b1,
+                     float w[4]) {
+    V3 r0 = orc::sub(a1, a0), r1 = orc::sub(b1, b0), d = orc::sub(a0, b0);
+    float a = orc::dot(r0, r0), e = orc::dot(r1, r1), b = orc::dot(r0, r1);
+    float c = orc::dot(r0, d), f = orc::dot(r1, d);
+    float inv_a = a > 0.0f ? 1.0f / a : 0.0f;
+    float inv_e = e > 0.0f ? 1.0f / e : 0.0f;
+    V3 n = orc::cross(r0, r1);
+    float nn = orc::dot(n, n);
+    float inv_nn = nn > 0.0f ? 1.0f / nn : 0.0f;
+    float s_stat = clamp_unit(orc::dot(n, orc::cross(r1, d)) * inv_nn);
+    float t_stat = clamp_unit((b * s_stat + f) * inv_e);
+    float t_a0 = clamp_unit(f * inv_e);
+    float t_a1 = clamp_unit((f + b) * inv_e);
+    float s_b0 = clamp_unit(-c * inv_a);
+    float s_b1 = clamp_unit((b - c) * inv_a);
+    const float cs[5] = {s_stat, 0.0f, 1.0f, s_b0, s_b1};
+    const float ct[5] = {t_stat, t_a0, t_a1, 0.0f, 1.0f};
+    float bs = 0.0f, bt = 0.0f, best = std::numeric_limits<float>::max();
+    for (int i = 0; i < 5; ++i) {
+        V3 v = orc::sub(orc::add(d, orc::scale(cs[i], r0)), orc::scale(ct[i], r1));
+        float dist = orc::dot(v, v);
+        if (dist < best) {
+            best = dist;
+            bs = cs[i];
+            bt = ct[i];
+        }
+    }
+    w[0] = 1.0f - bs;
+    w[1] = bs;
+    w[2] = 1.0f - bt;
+    w[3] = bt;
+}
+
+inline bool interior(const float *c, int n) {
+    float mx = c[0], mn = c[0];
+    for (int i = 1; i < n; ++i) {
+        mx = std::max(mx, c[i]);
+        mn = std::min(mn, c[i]);
+    }
+    return mx < 1.0f && mn > 0.0f;
+}
+
+// ---- LBVH -----------------------------------------------------------------
+
+inline uint32_t expand_bits(uint32_t v) {
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+inline int clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
+
+struct Bvh {
+    uint32_t n = 0;
+    std::vector<uint32_t> node; // 2 per node: leaf (prim+1, 0), inner (l+1, r+1)
+    std::vector<Box> box;
+    uint32_t root = 0;
+};
+
+// lcp of sorted codes at i and j with the index tie-break of lbvh.cu:289-303.
+inline int lcp(const std::vector<uint32_t> &k, int n, int i, int j) {
+    if (j < 0 || j >= n) {
+        return -1;
+    }
+    if (k[i] == k[j]) {
+        return 32 + clz32((uint32_t)(i ^ j));
+    }
+    return clz32(k[i] ^ k[j]);
+}
+
+void karras(const std::vector<uint32_t> &k, int n, int i, int &l, int &r,
+            int &split) {
+    int d = lcp(k, n, i, i + 1) > lcp(k, n, i, i - 1) ? 1 : -1;
+    int lmin = lcp(k, n, i, i - d);
+    int lmax = 2;
+    while (lcp(k, n, i, i + lmax * d) > lmin) {
+        lmax *= 2;
+    }
+    int len = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2) {
+        if (lcp(k, n, i, i + (len + t) * d) > lmin) {
+            len += t;
+        }
+    }
+    int j = i + len * d;
+    l = std::min(i, j);
+    r = std::max(i, j);
+    int node = lcp(k, n, l, r);
+    int s = 0, t = r - l;
+    do {
+        t = (t + 1) / 2;
+        if (lcp(k, n, l, l + s + t) > node) {
+            s += t;
+        }
+    } while (t > 1);
+    split = l + s;
+}
+
+// centroids: npp points per primitive, cent[i] = sum / npp in lbvh.cu's order.
+void build_bvh(Bvh &bvh, uint32_t n, const std::vector<V3> &cent,
+               const std::vector<Box> &prim_box) {
+    bvh.n = n;
+    bvh.node.assign(2 * (size_t)std::max(1u, 2 * n - 1), 0);
+    bvh.box.assign(std::max(1u, 2 * n - 1), Box{});
+    if (n == 0) {
+        return;
+    }
+    float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = std::fmin(mn[d], cent[i][d]);
+            mx[d] = std::fmax(mx[d], cent[i][d]);
+        }
+    }
+    float sc[3];
+    for (int d = 0; d < 3; ++d) {
+        sc[d] = (mx[d] - mn[d]) > 1e-10f ? 1023.0f / (mx[d] - mn[d]) : 0.0f;
+    }
+    std::vector<uint32_t> code(n), idx(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t q[3];
+        for (int d = 0; d < 3; ++d) {
+            q[d] = (uint32_t)std::fmin(1023.0f,
+                                       std::fmax(0.0f, (cent[i][d] - mn[d]) * sc[d]));
+        }
+        code[i] = expand_bits(q[0]) | (expand_bits(q[1]) << 1) |
+                  (expand_bits(q[2]) << 2);
+        idx[i] = i;
+    }
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+    std::vector<uint32_t> sk(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        sk[i] = code[idx[i]];
+        bvh.node[2 * i] = idx[i] + 1;
+        bvh.node[2 * i + 1] = 0;
+        bvh.box[i] = prim_box[idx[i]];
+    }
+    std::vector<uint32_t> parent(2 * n - 1, 0xffffffffu);
+    for (int i = 0; i + 1 < (int)n; ++i) {
+        int l, r, s;
+        karras(sk, (int)n, i, l, r, s);
+        uint32_t lc = (l == s) ? (uint32_t)l : n + (uint32_t)s;
+        uint32_t rc = (s + 1 == r) ? (uint32_t)r : n + (uint32_t)s + 1;
+        bvh.node[2 * (n + i)] = lc + 1;
+        bvh.node[2 * (n + i) + 1] = rc + 1;
+        parent[lc] = n + i;
+        parent[rc] = n + i;
+    }
+    bvh.root = n == 1 ? 0 : n;
+    // Union of children, children first (post-order by explicit recursion).
+    std::vector<uint32_t> order;
+    std::vector<uint32_t> st{bvh.root};
+    while (!st.empty()) {
+        uint32_t v = st.back();
+        st.pop_back();
+        order.push_back(v);
+        if (bvh.node[2 * v + 1]) {
+            st.push_back(bvh.node[2 * v] - 1);
+            st.push_back(bvh.node[2 * v + 1] - 1);
+        }
+    }
+    for (size_t k = order.size(); k-- > 0;) {
+        uint32_t v = order[k];
+        if (bvh.node[2 * v + 1]) {
+            const Box &a = bvh.box[bvh.node[2 * v] - 1];
+            const Box &b = bvh.box[bvh.node[2 * v + 1] - 1];
+            Box &o = bvh.box[v];
+            for (int d = 0; d < 3; ++d) {
+                o.mn[d] = std::fmin(a.mn[d], b.mn[d]);
+                o.mx[d] = std::fmax(a.mx[d], b.mx[d]);
+            }
+            o.active = a.active && b.active;
+        }
+    }
+}
+
+inline bool overlap(const Box &a, const Box &b) {
+    if (!a.active || !b.active) {
+        return false;
+    }
+    return (a.mn[0] <= b.mx[0] && a.mx[0] >= b.mn[0]) &&
+           (a.mn[1] <= b.mx[1] && a.mx[1] >= b.mn[1]) &&
+           (a.mn[2] <= b.mx[2] && a.mx[2] >= b.mn[2]);
+}
+
+template <class F> uint32_t query(const Bvh &bvh, const Box &q, F &&leaf) {
+    if (bvh.n == 0) {
+        return 0;
+    }
+    std::vector<uint32_t> st;
+    st.reserve(128);
+    st.push_back(bvh.root);
+    uint32_t count = 0;
+    while (!st.empty()) {
+        uint32_t v = st.back();
+        st.pop_back();
+        if (!overlap(bvh.box[v], q)) {
+            continue;
+        }
+        if (bvh.node[2 * v + 1] == 0) {
+            if (leaf(bvh.node[2 * v] - 1)) {
+                ++count;
+            }
+        } else {
+            st.push_back(bvh.node[2 * v] - 1);
+            st.push_back(bvh.node[2 * v + 1] - 1);
+        }
+    }
+    return count;
+}
+
+Box make_box(const V3 *pts, int m, float margin) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        float lo = pts[0][d], hi = pts[0][d];
+        for (int k = 1; k < m; ++k) {
+            lo = std::fmin(lo, pts[k][d]);
+            hi = std::fmax(hi, pts[k][d]);
+        }
+        b.mn[d] = lo;
+        b.mx[d] = hi;
+    }
+    b.active = true;
+    if (margin != 0.0f) {
+        for (int d = 0; d < 3; ++d) {
+            b.mn[d] -= margin;
+            b.mx[d] += margin;
+        }
+    }
+    return b;
+}
+
+Box point_box(const V3 &x, float margin) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.mn[d] = x[d] - margin;
+        b.mx[d] = x[d] + margin;
+    }
+    b.active = true;
+    return b;
+}
+
+// Leaf box of lbvh.cu:623-731 with x1 == x0: min/max then -/+ margin.
+Box leaf_box(const V3 *pts, int m, float margin) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        float lo = pts[0][d], hi = pts[0][d];
+        for (int k = 1; k < m; ++k) {
+            lo = std::fmin(lo, pts[k][d]);
+            hi = std::fmax(hi, pts[k][d]);
+        }
+        b.mn[d] = std::fmin(lo, lo) - margin;
+        b.mx[d] = std::fmax(hi, hi) + margin;
+    }
+    b.active = true;
+    return b;
+}
+
+} // namespace
+
+extern "C" {
+
+// Contact mesh for the narrow phase. Neighbor tables are CSR:
+// v2e / v2f (edges / faces incident to a vertex), e2f (faces incident to an
+// edge). vflag bit0 = pinned (fix_index != 0), bit1 = collider; eflag/fflag
+// bit0 = fixed. vbody = PD rigid body id (0 = none).
+struct oracle_contact_mesh {
+    uint32_t nv, ne, nf;
+    const float *x;
+    const uint32_t *edge, *face;
+    const uint32_t *v2e_off, *v2e, *v2f_off, *v2f, *e2f_off, *e2f;
+    const float *v_ghat, *v_off, *e_ghat, *e_off, *f_ghat, *f_off;
+    const uint32_t *vflag, *vbody, *eflag, *fflag;
+};
+
+// One accepted pair: kind 0 point-face, 1 point-edge, 2 point-point,
+// 3 edge-edge; a = query element (vertex, or edge for kind 3), b = candidate
+// element, count = multiplicity.
+struct oracle_pair {
+    uint32_t kind, a, b, count;
+};
+
+// Build the face (kind 0), edge (kind 1) or vertex (kind 2) BVH of a static
+// mesh: writes 2*(2n-1) node words, (2n-1) boxes as 6 floats, and the root.
+void oracle_bvh_build(const oracle_contact_mesh *m, int kind, uint32_t *nodes,
+                      float *boxes, uint32_t *root) {
+    std::vector<V3> x(m->nv);
+    for (uint32_t i = 0; i < m->nv; ++i) {
+        x[i] = v3(m->x[3 * i], m->x[3 * i + 1], m->x[3 * i + 2]);
+    }
+    uint32_t n = kind == 0 ? m->nf : (kind == 1 ? m->ne : m->nv);
+    std::vector<V3> cent(n);
+    std::vector<Box> pb(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (kind == 0) {
+            const uint32_t *f = m->face + 3 * i;
+            V3 p[3] = {x[f[0]], x[f[1]], x[f[2]]};
+            cent[i] = v3((p[0][0] + p[1][0] + p[2][0]) / 3.0f,
+                         (p[0][1] + p[1][1] + p[2][1]) / 3.0f,
+                         (p[0][2] + p[1][2] + p[2][2]) / 3.0f);
+            pb[i] = leaf_box(p, 3, 0.5f * m->f_ghat[i] + m->f_off[i]);
+        } else if (kind == 1) {
+            const uint32_t *e = m->edge + 2 * i;
+            V3 p[2] = {x[e[0]], x[e[1]]};
+            cent[i] = v3((p[0][0] + p[1][0]) / 2.0f, (p[0][1] + p[1][1]) / 2.0f,
+                         (p[0][2] + p[1][2]) / 2.0f);
+            pb[i] = leaf_box(p, 2, 0.5f * m->e_ghat[i] + m->e_off[i]);
+        } else {
+            cent[i] = x[i];
+            pb[i] = leaf_box(&x[i], 1, 0.5f * m->v_ghat[i] + m->v_off[i]);
+        }
+    }
+    Bvh b;
+    build_bvh(b, n, cent, pb);
+    size_t nn = n ? 2 * (size_t)n - 1 : 0;
+    for (size_t k = 0; k < nn; ++k) {
+        nodes[2 * k] = b.node[2 * k];
+        nodes[2 * k + 1] = b.node[2 * k + 1];
+        for (int d = 0; d < 3; ++d) {
+            boxes[6 * k + d] = b.box[k].mn[d];
+            boxes[6 * k + 3 + d] = b.box[k].mx[d];
+        }
+    }
+    *root = b.root;
+}
+
+// Stage-0 proximity pairs of all four kinds, in the reference's query order
+// (vertex queries in vertex-BVH leaf order, PF then PE then PP; then edge
+// queries in edge-BVH leaf order). Returns the pair count; writes at most
+// `cap` pairs.
+uint64_t oracle_proximity(const oracle_contact_mesh *m, oracle_pair *out,
+                          uint64_t cap) {
+    const uint32_t nv = m->nv, ne = m->ne, nf = m->nf;
+    std::vector<V3> x(nv);
+    for (uint32_t i = 0; i < nv; ++i) {
+        x[i] = v3(m->x[3 * i], m->x[3 * i + 1], m->x[3 * i + 2]);
+    }
+    auto vdyn = [&](uint32_t v) { return (m->vflag[v] & 1u) == 0; };
+    auto vcol = [&](uint32_t v) { return (m->vflag[v] & 2u) != 0; };
+    Bvh fb, eb, vb;
+    {
+        std::vector<V3> c(nf);
+        std::vector<Box> pb(nf);
+        for (uint32_t i = 0; i < nf; ++i) {
+            const uint32_t *f = m->face + 3 * i;
+            V3 p[3] = {x[f[0]], x[f[1]], x[f[2]]};
+            c[i] = v3((p[0][0] + p[1][0] + p[2][0]) / 3.0f,
+                      (p[0][1] + p[1][1] + p[2][1]) / 3.0f,
+                      (p[0][2] + p[1][2] + p[2][2]) / 3.0f);
+            pb[i] = leaf_box(p, 3, 0.5f * m->f_ghat[i] + m->f_off[i]);
+        }
+        build_bvh(fb, nf, c, pb);
+    }
+    {
+        std::vector<V3> c(ne);
+        std::vector<Box> pb(ne);
+        for (uint32_t i = 0; i < ne; ++i) {
+            const uint32_t *e = m->edge + 2 * i;
+            V3 p[2] = {x[e[0]], x[e[1]]};
+            c[i] = v3((p[0][0] + p[1][0]) / 2.0f, (p[0][1] + p[1][1]) / 2.0f,
+                      (p[0][2] + p[1][2]) / 2.0f);
+            pb[i] = leaf_box(p, 2, 0.5f * m->e_ghat[i] + m->e_off[i]);
+        }
+        build_bvh(eb, ne, c, pb);
+    }
+    {
+        std::vector<Box> pb(nv);
+        for (uint32_t i = 0; i < nv; ++i) {
+            pb[i] = leaf_box(&x[i], 1, 0.5f * m->v_ghat[i] + m->v_off[i]);
+        }
+        build_bvh(vb, nv, x, pb);
+    }
+    // Queries in fixed chunks, each chunk's pairs in query order; chunks are
+    // concatenated in order, so the output is independent of the thread count.
+    const uint32_t kChunk = 4096;
+    const uint32_t ncv = (nv + kChunk - 1) / kChunk, nce = (ne + kChunk - 1) / kChunk;
+    std::vector<std::vector<oracle_pair>> res(ncv + nce);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long ch = 0; ch < (long)ncv; ++ch) {
+    std::vector<oracle_pair> &loc = res[ch];
+    auto emit = [&](uint32_t kind, uint32_t a, uint32_t b, uint32_t count) {
+        loc.push_back(oracle_pair{kind, a, b, count});
+    };
+    const uint32_t l0 = (uint32_t)ch * kChunk, l1 = std::min(nv, l0 + kChunk);
+    for (uint32_t leaf = l0; leaf < l1; ++leaf) {
+        const uint32_t i = vb.node[2 * leaf] - 1;
+        const V3 &p = x[i];
+        const Box q = point_box(p, 0.5f * m->v_ghat[i] + m->v_off[i]);
+        // point-face
+        query(fb, q, [&](uint32_t fi) {
+            const uint32_t *f = m->face + 3 * fi;
+            bool either_dyn = vdyn(i) || (m->fflag[fi] & 1u) == 0;
+            bool same_body = m->vbody[i] != 0 && m->vbody[i] == m->vbody[f[0]];
+            bool both_col = vcol(i) && vcol(f[0]);
+            if (f[0] == i || f[1] == i || f[2] == i || !either_dyn || same_body ||
+                both_col) {
+                return false;
+            }
+            float c[3];
+            pt_coeff(p, x[f[0]], x[f[1]], x[f[2]], c);
+            if (!interior(c, 3)) {
+                return false;
+            }
+            V3 e = orc::add(orc::add(orc::scale(c[0], orc::sub(p, x[f[0]])),
+                                     orc::scale(c[1], orc::sub(p, x[f[1]]))),
+                            orc::scale(c[2], orc::sub(p, x[f[2]])));
+            float off = m->v_off[i] + m->f_off[fi];
+            float gh = 0.5f * (m->v_ghat[i] + m->f_ghat[fi]);
+            float r = gh + off;
+            if (orc::sqnorm(e) < r * r) {
+                emit(0, i, fi, 1);
+                return true;
+            }
+            return false;
+        });
+        // point-edge
+        query(eb, q, [&](uint32_t ei) {
+            const uint32_t *ed = m->edge + 2 * ei;
+            bool either_dyn = vdyn(i) || (m->eflag[ei] & 1u) == 0;
+            bool same_body = m->vbody[i] != 0 && m->vbody[i] == m->vbody[ed[0]];
+            bool both_col = vcol(i) && vcol(ed[0]);
+            if (ed[0] == i || ed[1] == i || !either_dyn || same_body || both_col) {
+                return false;
+            }
+            float c[2];
+            pe_coeff(p, x[ed[0]], x[ed[1]], c);
+            if (!interior(c, 2)) {
+                return false;
+            }
+            V3 e = orc::add(orc::scale(c[0], orc::sub(p, x[ed[0]])),
+                            orc::scale(c[1], orc::sub(p, x[ed[1]])));
+            float off = m->v_off[i] + m->e_off[ei];
+            float gh = 0.5f * (m->v_ghat[i] + m->e_ghat[ei]);
+            float r = gh + off;
+            if (!(orc::sqnorm(e) < r * r)) {
+                return false;
+            }
+            uint32_t count = 0;
+            uint32_t f0 = m->e2f_off[ei], f1 = m->e2f_off[ei + 1];
+            if (f1 > f0) {
+                for (uint32_t k = f0; k < f1; ++k) {
+                    const uint32_t *f = m->face + 3 * m->e2f[k];
+                    if (f[0] == i || f[1] == i || f[2] == i) {
+                        ++count;
+                    } else {
+                        float cc[3];
+                        pt_coeff(p, x[f[0]], x[f[1]], x[f[2]], cc);
+                        if (!interior(cc, 3)) {
+                            ++count;
+                        }
+                    }
+                }
+            } else {
+                count = 1;
+            }
+            if (count) {
+                emit(1, i, ei, count);
+                return true;
+            }
+            return false;
+        });
+        // point-point
+        query(vb, q, [&](uint32_t j) {
+            bool either_dyn = vdyn(i) || vdyn(j);
+            bool same_body = m->vbody[i] != 0 && m->vbody[i] == m->vbody[j];
+            bool both_col = vcol(i) && vcol(j);
+            if (!(j < i) || !either_dyn || same_body || both_col) {
+                return false;
+            }
+            V3 e = orc::sub(p, x[j]);
+            float off = m->v_off[i] + m->v_off[j];
+            float gh = 0.5f * (m->v_ghat[i] + m->v_ghat[j]);
+            float r = gh + off;
+            if (!(orc::sqnorm(e) < r * r)) {
+                return false;
+            }
+            uint32_t count = 0;
+            uint32_t e0 = m->v2e_off[j], e1 = m->v2e_off[j + 1];
+            if (e1 > e0) {
+                for (uint32_t k = e0; k < e1; ++k) {
+                    const uint32_t *ed = m->edge + 2 * m->v2e[k];
+                    if (ed[0] == i || ed[1] == i) {
+                        ++count;
+                    } else {
+                        float cc[2];
+                        pe_coeff(p, x[ed[0]], x[ed[1]], cc);
+                        if (!interior(cc, 2)) {
+                            ++count;
+                        }
+                    }
+                }
+                for (uint32_t k = m->v2f_off[j]; k < m->v2f_off[j + 1]; ++k) {
+                    const uint32_t *f = m->face + 3 * m->v2f[k];
+                    if (f[0] == i || f[1] == i || f[2] == i) {
+                        ++count;
+                    } else {
+                        float cc[3];
+                        pt_coeff(p, x[f[0]], x[f[1]], x[f[2]], cc);
+                        if (!interior(cc, 3)) {
+                            ++count;
+                        }
+                    }
+                }
+            } else {
+                count = 1;
+            }
+            if (count) {
+                emit(2, i, j, count);
+                return true;
+            }
+            return false;
+        });
+    }
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long ch = 0; ch < (long)nce; ++ch) {
+    std::vector<oracle_pair> &loc = res[ncv + ch];
+    auto emit = [&](uint32_t kind, uint32_t a, uint32_t b, uint32_t count) {
+        loc.push_back(oracle_pair{kind, a, b, count});
+    };
+    const uint32_t l0 = (uint32_t)ch * kChunk, l1 = std::min(ne, l0 + kChunk);
+    for (uint32_t leaf = l0; leaf < l1; ++leaf) {
+        const uint32_t ei = eb.node[2 * leaf] - 1;
+        const uint32_t *ea = m->edge + 2 * ei;
+        V3 pts[2] = {x[ea[0]], x[ea[1]]};
+        const Box q = make_box(pts, 2, 0.5f * m->e_ghat[ei] + m->e_off[ei]);
+        query(eb, q, [&](uint32_t ej) {
+            const uint32_t *eb2 = m->edge + 2 * ej;
+            bool either_dyn = (m->eflag[ei] & 1u) == 0 || (m->eflag[ej] & 1u) == 0;
+            bool same_body = m->vbody[ea[0]] != 0 && m->vbody[ea[0]] == m->vbody[eb2[0]];
+            bool both_col = vcol(ea[0]) && vcol(eb2[0]);
+            bool shared = ea[0] == eb2[0] || ea[0] == eb2[1] || ea[1] == eb2[0] ||
+                          ea[1] == eb2[1];
+            if (!(ei < ej) || shared || !either_dyn || same_body || both_col) {
+                return false;
+            }
+            float c[4];
+            ee_coeff(x[ea[0]], x[ea[1]], x[eb2[0]], x[eb2[1]], c);
+            if (!interior(c, 4)) {
+                return false;
+            }
+            V3 x0 = orc::add(orc::scale(c[0], x[ea[0]]), orc::scale(c[1], x[ea[1]]));
+            V3 x1 = orc::add(orc::scale(c[2], x[eb2[0]]), orc::scale(c[3], x[eb2[1]]));
+            V3 e = orc::sub(x0, x1);
+            float off = m->e_off[ei] + m->e_off[ej];
+            float gh = 0.5f * (m->e_ghat[ei] + m->e_ghat[ej]);
+            float r = gh + off;
+            if (orc::sqnorm(e) < r * r) {
+                emit(3, ei, ej, 1);
+                return true;
+            }
+            return false;
+        });
+    }
+    }
+    uint64_t total = 0;
+    for (const auto &r : res) {
+        for (const oracle_pair &q : r) {
+            if (total < cap) {
+                out[total] = q;
+            }
+            ++total;
+        }
+    }
+    return total;
+}
+
+// Distance coefficients, exported for the pinning tests.
+void oracle_pe_coeff(const float *p, const float *e0, const float *e1, float *w) {
+    pe_coeff(v3(p[0], p[1], p[2]), v3(e0[0], e0[1], e0[2]),
+             v3(e1[0], e1[1], e1[2]), w);
+}
+void oracle_pt_coeff(const float *p, const float *t0, const float *t1,
+                     const float *t2, float *w) {
+    pt_coeff(v3(p[0], p[1], p[2]), v3(t0[0], t0[1], t0[2]),
+             v3(t1[0], t1[1], t1[2]), v3(t2[0], t2[1], t2[2]), w);
+}
+void oracle_ee_coeff(const float *a0, const float *a1, const float *b0,
+                     const float *b1, float *w) {
+    ee_coeff(v3(a0[0], a0[1], a0[2]), v3(a1[0], a1[1], a1[2]),
+             v3(b0[0], b0[1], b0[2]), v3(b1[0], b1[1], b1[2]), w);
+}
+}

@@ -120,3 +120,93 @@ def pcg(sys, x0=None, tol=1e-3, max_iter=10000, hist=False):
     if hist:
         out["err_hist"] = h[:res.iter]
     return out
+
+
+# ---- narrow phase -----------------------------------------------------------
+
+class ContactMesh(ctypes.Structure):
+    _fields_ = [("nv", u32), ("ne", u32), ("nf", u32), ("x", vp),
+                ("edge", vp), ("face", vp),
+                ("v2e_off", vp), ("v2e", vp), ("v2f_off", vp), ("v2f", vp),
+                ("e2f_off", vp), ("e2f", vp),
+                ("v_ghat", vp), ("v_off", vp), ("e_ghat", vp), ("e_off", vp),
+                ("f_ghat", vp), ("f_off", vp),
+                ("vflag", vp), ("vbody", vp), ("eflag", vp), ("fflag", vp)]
+
+
+_FLOAT_FIELDS = {"x", "v_ghat", "v_off", "e_ghat", "e_off", "f_ghat", "f_off"}
+
+PAIR_DTYPE = np.dtype([("kind", np.uint32), ("a", np.uint32), ("b", np.uint32),
+                       ("count", np.uint32)])
+
+
+def _mesh_struct(m):
+    keep = {}
+    fields = {}
+    for name, _ in ContactMesh._fields_:
+        if name in ("nv", "ne", "nf"):
+            fields[name] = int(m[name])
+            continue
+        dt = np.float32 if name in _FLOAT_FIELDS else np.uint32
+        a = np.ascontiguousarray(m[name], dtype=dt)
+        keep[name] = a
+        fields[name] = a.ctypes.data_as(vp)
+    return ContactMesh(**fields), keep
+
+
+def _load_narrow():
+    lib = load()
+    if not hasattr(lib, "_narrow_ready"):
+        lib.oracle_proximity.argtypes = [ctypes.POINTER(ContactMesh), vp,
+                                         ctypes.c_uint64]
+        lib.oracle_proximity.restype = ctypes.c_uint64
+        lib.oracle_bvh_build.argtypes = [ctypes.POINTER(ContactMesh), ctypes.c_int,
+                                         vp, vp, ctypes.POINTER(u32)]
+        for fn in ("oracle_pe_coeff", "oracle_pt_coeff", "oracle_ee_coeff"):
+            getattr(lib, fn).restype = None
+        lib._narrow_ready = True
+    return lib
+
+
+def proximity(m):
+    """Stage-0 proximity pairs (structured array kind, a, b, count)."""
+    lib = _load_narrow()
+    s, keep = _mesh_struct(m)
+    cap = 1 << 16
+    while True:
+        out = np.zeros(cap, PAIR_DTYPE)
+        n = lib.oracle_proximity(ctypes.byref(s), out.ctypes.data_as(vp), cap)
+        if n <= cap:
+            return out[:n]
+        cap = int(n)
+
+
+def bvh_build(m, kind):
+    """(nodes (2n-1, 2) uint32, boxes (2n-1, 6) float32, root)."""
+    lib = _load_narrow()
+    s, keep = _mesh_struct(m)
+    n = [m["nf"], m["ne"], m["nv"]][kind]
+    nn = max(1, 2 * n - 1)
+    nodes = np.zeros((nn, 2), np.uint32)
+    boxes = np.zeros((nn, 6), np.float32)
+    root = u32()
+    lib.oracle_bvh_build(ctypes.byref(s), kind, nodes.ctypes.data_as(vp),
+                         boxes.ctypes.data_as(vp), ctypes.byref(root))
+    return nodes, boxes, root.value
+
+
+def coeff(kind, pts, lib_name="oracle"):
+    """Distance coefficients for a batch: kind 'pe' (p,e0,e1), 'pt' (p,t0,t1,t2),
+    'ee' (a0,a1,b0,b1); pts shape (k, m, 3). lib_name 'ref' uses the
+    reference-header build."""
+    lib = _load_narrow() if lib_name == "oracle" else load_ref()
+    fn = getattr(lib, f"{lib_name}_{kind}_coeff")
+    fn.restype = None
+    pts = _f32(pts)
+    k, mm = pts.shape[0], pts.shape[1]
+    nout = {"pe": 2, "pt": 3, "ee": 4}[kind]
+    out = np.zeros((k, nout), np.float32)
+    for i in range(k):
+        args = [pts[i, j].ctypes.data_as(vp) for j in range(mm)]
+        fn(*args, out[i].ctypes.data_as(vp))
+    return out

@@ -3,6 +3,7 @@
 // Plain pointers and sizes only; no torch or C++ types cross the boundary.
 #include "common.hpp"
 #include "eig3.hpp"
+#include "narrow.hpp"
 #include "pcg.hpp"
 #include <cstdarg>
 #include <cstring>
@@ -69,6 +70,10 @@ using namespace ppf;
 
 struct ppf_pcg_plan {
     PcgPlanImpl *impl;
+};
+
+struct ppf_narrow {
+    Narrow impl;
 };
 
 extern "C" {
@@ -189,6 +194,67 @@ int ppf_symm3x3(uint32_t n, const float *d_in, float *d_val, float *d_vec,
     PPF_HIP(hipGetLastError());
     PPF_HIP(hipStreamSynchronize(st));
     return PPF_OK;
+}
+
+int ppf_narrow_create(ppf_narrow **out) {
+    if (!out) {
+        set_error("ppf_narrow_create: out is null");
+        return PPF_ERR_ARG;
+    }
+    *out = new ppf_narrow();
+    return PPF_OK;
+}
+
+void ppf_narrow_destroy(ppf_narrow *h) { delete h; }
+
+int ppf_narrow_build(ppf_narrow *h, const ppf_contact_mesh *m, void *stream) {
+    if (!h || !m || !m->x) {
+        set_error("ppf_narrow_build: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.build_trees(*m, (hipStream_t)stream);
+}
+
+int ppf_narrow_tree(ppf_narrow *h, int kind, uint32_t *d_nodes, float *d_boxes,
+                    uint32_t *n, uint32_t *root, void *stream) {
+    if (!h || kind < 0 || kind > 2 || !n || !root) {
+        set_error("ppf_narrow_tree: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    Lbvh &t = h->impl.tree(kind);
+    const BvhView v = t.view();
+    *n = v.n;
+    *root = v.root;
+    if (v.n == 0) {
+        return PPF_OK;
+    }
+    const size_t nodes = 2 * (size_t)v.n - 1;
+    hipStream_t st = (hipStream_t)stream;
+    if (d_nodes) {
+        PPF_HIP(hipMemcpyAsync(d_nodes, v.node, 2 * nodes * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, st));
+    }
+    if (d_boxes) {
+        PPF_HIP(hipMemcpyAsync(d_boxes, v.box, nodes * sizeof(Aabb),
+                               hipMemcpyDeviceToDevice, st));
+    }
+    PPF_HIP(hipStreamSynchronize(st));
+    return PPF_OK;
+}
+
+int ppf_narrow_proximity(ppf_narrow *h, const ppf_contact_mesh *m,
+                         ppf_pair *d_out, uint64_t cap, uint64_t *count,
+                         void *stream) {
+    if (!h || !m || !count) {
+        set_error("ppf_narrow_proximity: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    if (h->impl.tree(0).view().n != m->nf || h->impl.tree(1).view().n != m->ne ||
+        h->impl.tree(2).view().n != m->nv) {
+        set_error("ppf_narrow_proximity: trees were built for another mesh");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.proximity(*m, d_out, cap, count, (hipStream_t)stream);
 }
 
 } // extern "C"

@@ -1,0 +1,348 @@
+// lbvh.hip -- LBVH build and refit on MI355X.
+//
+// Tree semantics follow lbvh/lbvh.cu:108-385 (Morton quantization, Karras
+// split with the index tie-break) and the swept leaf boxes of
+// lbvh/lbvh.cu:623-731; the inner boxes are unions of the children. The
+// per-level launches and host round trips of the reference are replaced by a
+// single bottom-up kernel with per-node arrival counters: the second child to
+// arrive writes the union and continues upward.
+#include "lbvh.hpp"
+#include "ppf_mi355x.h"
+#include "sort.hpp"
+
+namespace ppf {
+
+namespace {
+
+constexpr int kNT = 256;
+constexpr uint32_t kNoParent = 0xffffffffu;
+
+__device__ inline uint32_t prim_vertex(const PrimSet &p, uint32_t i, int k) {
+    return p.elem ? p.elem[(size_t)i * p.arity + k] : i;
+}
+
+__device__ inline uint32_t expand_bits(uint32_t v) {
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Centroids (lbvh.cu:120-167 term order) and per-block min/max partials.
+__global__ __launch_bounds__(kNT) void k_centroid(PrimSet p, const float *__restrict__ x,
+                                                  float *__restrict__ cent,
+                                                  float *__restrict__ part) {
+    __shared__ float s[6][kNT];
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    float c[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    if (i < p.n) {
+        const f3 a = load3(x + 3 * (size_t)prim_vertex(p, i, 0));
+        f3 m;
+        if (p.arity == 3) {
+            const f3 b = load3(x + 3 * (size_t)prim_vertex(p, i, 1));
+            const f3 d = load3(x + 3 * (size_t)prim_vertex(p, i, 2));
+            m = mk3((a.x + b.x + d.x) / 3.0f, (a.y + b.y + d.y) / 3.0f,
+                    (a.z + b.z + d.z) / 3.0f);
+        } else if (p.arity == 2) {
+            const f3 b = load3(x + 3 * (size_t)prim_vertex(p, i, 1));
+            m = mk3((a.x + b.x) / 2.0f, (a.y + b.y) / 2.0f, (a.z + b.z) / 2.0f);
+        } else {
+            m = a;
+        }
+        store3(cent + 3 * (size_t)i, m);
+        c[0] = mx[0] = m.x;
+        c[1] = mx[1] = m.y;
+        c[2] = mx[2] = m.z;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        s[d][threadIdx.x] = c[d];
+        s[3 + d][threadIdx.x] = mx[d];
+    }
+    __syncthreads();
+    for (int w = kNT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                s[d][threadIdx.x] = fminf(s[d][threadIdx.x], s[d][threadIdx.x + w]);
+                s[3 + d][threadIdx.x] =
+                    fmaxf(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) {
+        part[6 * (size_t)blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+    }
+}
+
+__global__ __launch_bounds__(kNT) void k_bounds(uint32_t nb, float *__restrict__ part,
+                                                float *__restrict__ bounds) {
+    __shared__ float s[6][kNT];
+    float v[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
+    for (uint32_t b = threadIdx.x; b < nb; b += kNT) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            v[d] = fminf(v[d], part[6 * (size_t)b + d]);
+            v[3 + d] = fmaxf(v[3 + d], part[6 * (size_t)b + 3 + d]);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        s[d][threadIdx.x] = v[d];
+    }
+    __syncthreads();
+    for (int w = kNT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                s[d][threadIdx.x] = fminf(s[d][threadIdx.x], s[d][threadIdx.x + w]);
+                s[3 + d][threadIdx.x] =
+                    fmaxf(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) {
+        bounds[threadIdx.x] = s[threadIdx.x][0];
+    }
+}
+
+// 30-bit Morton codes (lbvh.cu:253-279).
+__global__ __launch_bounds__(kNT) void k_morton(uint32_t n, const float *__restrict__ cent,
+                                                const float *__restrict__ bounds,
+                                                uint32_t *__restrict__ code,
+                                                uint32_t *__restrict__ idx) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    uint32_t q[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float lo = bounds[d], hi = bounds[3 + d];
+        const float sc = (hi - lo) > 1e-10f ? 1023.0f / (hi - lo) : 0.0f;
+        q[d] = (uint32_t)fminf(1023.0f, fmaxf(0.0f, (cent[3 * (size_t)i + d] - lo) * sc));
+    }
+    code[i] = expand_bits(q[0]) | (expand_bits(q[1]) << 1) | (expand_bits(q[2]) << 2);
+    idx[i] = i;
+}
+
+__device__ inline int lcp(const uint32_t *__restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) {
+        return -1;
+    }
+    const uint32_t a = k[i], b = k[j];
+    if (a == b) {
+        return 32 + __clz((uint32_t)(i ^ j));
+    }
+    return __clz(a ^ b);
+}
+
+// Karras radix tree over the sorted codes (lbvh.cu:305-385), parents inline.
+__global__ __launch_bounds__(kNT) void k_karras(uint32_t n, const uint32_t *__restrict__ k,
+                                                const uint32_t *__restrict__ idx,
+                                                uint32_t *__restrict__ node,
+                                                uint32_t *__restrict__ parent) {
+    const int i = blockIdx.x * kNT + threadIdx.x;
+    const int nn = (int)n;
+    if (i < nn) {
+        node[2 * (size_t)i] = idx[i] + 1;
+        node[2 * (size_t)i + 1] = 0;
+    }
+    if (i == 0) {
+        parent[nn == 1 ? 0 : n] = kNoParent;
+    }
+    if (i >= nn - 1) {
+        return;
+    }
+    const int d = lcp(k, nn, i, i + 1) > lcp(k, nn, i, i - 1) ? 1 : -1;
+    const int lmin = lcp(k, nn, i, i - d);
+    int lmax = 2;
+    while (lcp(k, nn, i, i + lmax * d) > lmin) {
+        lmax *= 2;
+    }
+    int len = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2) {
+        if (lcp(k, nn, i, i + (len + t) * d) > lmin) {
+            len += t;
+        }
+    }
+    const int j = i + len * d;
+    const int l = min(i, j), r = max(i, j);
+    const int nodelcp = lcp(k, nn, l, r);
+    int s = 0, t = r - l;
+    do {
+        t = (t + 1) / 2;
+        if (lcp(k, nn, l, l + s + t) > nodelcp) {
+            s += t;
+        }
+    } while (t > 1);
+    const int split = l + s;
+    const uint32_t lc = (l == split) ? (uint32_t)l : n + (uint32_t)split;
+    const uint32_t rc = (split + 1 == r) ? (uint32_t)r : n + (uint32_t)split + 1;
+    node[2 * ((size_t)n + i)] = lc + 1;
+    node[2 * ((size_t)n + i) + 1] = rc + 1;
+    parent[lc] = n + i;
+    parent[rc] = n + i;
+}
+
+// Swept leaf box (lbvh.cu:623-731), then the union walk to the root.
+__global__ __launch_bounds__(kNT) void k_refit(PrimSet p, const float *__restrict__ x0,
+                                               const float *__restrict__ x1, float t,
+                                               const uint32_t *__restrict__ node,
+                                               const uint32_t *__restrict__ parent,
+                                               uint32_t *__restrict__ visit,
+                                               uint32_t root, Aabb *__restrict__ box) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= p.n) {
+        return;
+    }
+    const uint32_t prim = node[2 * (size_t)i] - 1;
+    const float margin = 0.5f * p.ghat[prim] + p.offset[prim];
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    for (int k = 0; k < p.arity; ++k) {
+        const uint32_t v = prim_vertex(p, prim, k);
+        const f3 y0 = load3(x0 + 3 * (size_t)v);
+        f3 z = y0;
+        if (x1) {
+            const f3 y1 = load3(x1 + 3 * (size_t)v);
+            z = mk3(t * (y1.x - y0.x) + y0.x, t * (y1.y - y0.y) + y0.y,
+                    t * (y1.z - y0.z) + y0.z);
+        }
+        const float a[3] = {y0.x, y0.y, y0.z}, b[3] = {z.x, z.y, z.z};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo0[d] = k ? fminf(lo0[d], a[d]) : a[d];
+            hi0[d] = k ? fmaxf(hi0[d], a[d]) : a[d];
+            lo1[d] = k ? fminf(lo1[d], b[d]) : b[d];
+            hi1[d] = k ? fmaxf(hi1[d], b[d]) : b[d];
+        }
+    }
+    Aabb bx;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        bx.mn[d] = fminf(lo0[d], lo1[d]) - margin;
+        bx.mx[d] = fmaxf(hi0[d], hi1[d]) + margin;
+    }
+    bx.active = 1.0f;
+    bx.pad = 0.0f;
+    box[i] = bx;
+    uint32_t cur = i;
+    while (cur != root) {
+        const uint32_t par = parent[cur];
+        __threadfence();
+        if (atomicAdd(visit + (par - p.n), 1u) == 0) {
+            return; // the sibling's walk finishes this node
+        }
+        __threadfence();
+        const Aabb a = box[node[2 * (size_t)par] - 1];
+        const Aabb b = box[node[2 * (size_t)par + 1] - 1];
+        Aabb u;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            u.mn[d] = fminf(a.mn[d], b.mn[d]);
+            u.mx[d] = fmaxf(a.mx[d], b.mx[d]);
+        }
+        u.active = (a.active != 0.0f && b.active != 0.0f) ? 1.0f : 0.0f;
+        u.pad = 0.0f;
+        box[par] = u;
+        cur = par;
+    }
+}
+
+uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+} // namespace
+
+Lbvh::~Lbvh() { release(); }
+
+void Lbvh::release() {
+    for (void *p : {(void *)node_, (void *)parent_, (void *)visit_, (void *)code_,
+                    (void *)idx_, (void *)scratch_, (void *)cent_, (void *)part_,
+                    (void *)box_}) {
+        if (p) {
+            (void)hipFree(p);
+        }
+    }
+    node_ = parent_ = visit_ = code_ = idx_ = scratch_ = nullptr;
+    cent_ = part_ = nullptr;
+    box_ = nullptr;
+    cap_ = 0;
+}
+
+int Lbvh::reserve(uint32_t n) {
+    if (n <= cap_) {
+        return PPF_OK;
+    }
+    release();
+    const size_t nodes = 2 * (size_t)n;
+    const size_t nb = cdiv(n, kNT);
+    PPF_HIP(hipMalloc(&node_, 2 * nodes * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&parent_, nodes * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&visit_, nodes * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&code_, n * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&idx_, n * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&scratch_, radix_sort_scratch_words(n) * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&cent_, 3 * (size_t)n * sizeof(float)));
+    PPF_HIP(hipMalloc(&part_, (6 * nb + 6) * sizeof(float)));
+    PPF_HIP(hipMalloc(&box_, nodes * sizeof(Aabb)));
+    cap_ = n;
+    return PPF_OK;
+}
+
+int Lbvh::build(const PrimSet &p, const float *x0, const float *x1, float t,
+                hipStream_t s) {
+    int rc = reserve(p.n);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    n_ = p.n;
+    root_ = n_ == 1 ? 0 : n_;
+    if (n_ == 0) {
+        return PPF_OK;
+    }
+    const uint32_t nb = cdiv(n_, kNT);
+    float *bounds = part_ + 6 * (size_t)nb;
+    k_centroid<<<nb, kNT, 0, s>>>(p, x0, cent_, part_);
+    PPF_HIP(hipGetLastError());
+    k_bounds<<<1, kNT, 0, s>>>(nb, part_, bounds);
+    PPF_HIP(hipGetLastError());
+    k_morton<<<nb, kNT, 0, s>>>(n_, cent_, bounds, code_, idx_);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(radix_sort_pairs(code_, idx_, n_, 30, scratch_, s));
+    k_karras<<<nb, kNT, 0, s>>>(n_, code_, idx_, node_, parent_);
+    PPF_HIP(hipGetLastError());
+    return refit(p, x0, x1, t, s);
+}
+
+int Lbvh::refit(const PrimSet &p, const float *x0, const float *x1, float t,
+                hipStream_t s) {
+    if (p.n != n_) {
+        set_error("Lbvh::refit: primitive count %u differs from the build (%u)",
+                  p.n, n_);
+        return PPF_ERR_ARG;
+    }
+    if (n_ == 0) {
+        return PPF_OK;
+    }
+    if (n_ > 1) {
+        PPF_HIP(hipMemsetAsync(visit_, 0, (n_ - 1) * sizeof(uint32_t), s));
+    }
+    k_refit<<<cdiv(n_, kNT), kNT, 0, s>>>(p, x0, x1, t, node_, parent_, visit_,
+                                          root_, box_);
+    PPF_HIP(hipGetLastError());
+    return PPF_OK;
+}
+
+BvhView Lbvh::view() const {
+    BvhView v;
+    v.n = n_;
+    v.root = root_;
+    v.node = node_;
+    v.box = box_;
+    return v;
+}
+
+} // namespace ppf

@@ -1,0 +1,64 @@
+// lbvh.hpp -- linear BVH over faces, edges or vertices (device).
+//
+// Same tree as the reference's LBVH (lbvh/lbvh.cu:108-385): centroids, scene
+// bounds of the centroids, 10-bit-per-axis Morton codes, a stable sort of
+// (code, index), and the Karras radix tree with the index tie-break on equal
+// codes. Node i < n is leaf i (prim + 1, 0); node n + i is internal node i
+// (left + 1, right + 1); the root is node n (leaf 0 when n == 1). Boxes are
+// refit bottom-up with per-node arrival counters instead of per-level launches.
+#pragma once
+#include "common.hpp"
+
+namespace ppf {
+
+// Box of one node: min xyz, active flag, max xyz, pad (32 B, two 16 B loads).
+struct alignas(32) Aabb {
+    float mn[3];
+    float active;
+    float mx[3];
+    float pad;
+};
+
+struct BvhView {
+    uint32_t n = 0;            // leaves
+    uint32_t root = 0;
+    const uint32_t *node = nullptr; // 2 words per node
+    const Aabb *box = nullptr;
+};
+
+// Primitive set a tree is built over: `arity` vertex indices per primitive
+// (3 faces, 2 edges, 1 vertices; index = element for vertices), margin
+// 0.5*ghat + offset per primitive.
+struct PrimSet {
+    uint32_t n = 0;
+    int arity = 1;
+    const uint32_t *elem = nullptr;  // n * arity vertex indices (null: identity)
+    const float *ghat = nullptr;     // per primitive
+    const float *offset = nullptr;   // per primitive
+};
+
+class Lbvh {
+  public:
+    ~Lbvh();
+    // Allocates for up to n primitives (grows when needed).
+    int reserve(uint32_t n);
+    // Full build from positions x0 (swept to x0 + t (x1 - x0) when x1 is set).
+    int build(const PrimSet &p, const float *x0, const float *x1, float t,
+              hipStream_t s);
+    // Refit the leaf boxes and the inner boxes, same topology.
+    int refit(const PrimSet &p, const float *x0, const float *x1, float t,
+              hipStream_t s);
+    BvhView view() const;
+    uint32_t *nodes() { return node_; }
+    Aabb *boxes() { return box_; }
+
+  private:
+    uint32_t cap_ = 0, n_ = 0, root_ = 0;
+    uint32_t *node_ = nullptr, *parent_ = nullptr, *visit_ = nullptr;
+    uint32_t *code_ = nullptr, *idx_ = nullptr, *scratch_ = nullptr;
+    float *cent_ = nullptr, *part_ = nullptr;
+    Aabb *box_ = nullptr;
+    void release();
+};
+
+} // namespace ppf

@@ -37,6 +37,13 @@ _PROTOS = {
                                          ctypes.POINTER(f32), vp]),
     "ppf_bj_invert": (ctypes.c_int, [u32, vp, vp, vp, vp]),
     "ppf_symm3x3": (ctypes.c_int, [u32, vp, vp, vp, vp]),
+    "ppf_narrow_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
+    "ppf_narrow_destroy": (None, [vp]),
+    "ppf_narrow_build": (ctypes.c_int, [vp, vp, vp]),
+    "ppf_narrow_tree": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, ctypes.POINTER(u32),
+                                       ctypes.POINTER(u32), vp]),
+    "ppf_narrow_proximity": (ctypes.c_int, [vp, vp, vp, ctypes.c_uint64,
+                                            ctypes.POINTER(ctypes.c_uint64), vp]),
 }
 
 _lib = None

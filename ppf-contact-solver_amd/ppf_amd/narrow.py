@@ -1,0 +1,130 @@
+"""LBVH + contact proximity narrow phase over torch device buffers.
+
+Mirrors the detection half of contact::embed_contact_force_hessian
+(crates/ppf-cts-solver/src/cpp/contact/contact.cu:1398-1787) and the LBVH
+builds of lbvh/lbvh.cu:863-1026. All compute is in libppf_mi355x.so.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+vp = ctypes.c_void_p
+u32 = ctypes.c_uint32
+
+FLOAT_FIELDS = ("x", "v_ghat", "v_off", "e_ghat", "e_off", "f_ghat", "f_off")
+INDEX_FIELDS = ("edge", "face", "v2e_off", "v2e", "v2f_off", "v2f", "e2f_off",
+                "e2f", "vflag", "vbody", "eflag", "fflag")
+PAIR_DTYPE = np.dtype([("kind", np.uint32), ("a", np.uint32), ("b", np.uint32),
+                       ("count", np.uint32)])
+
+
+class ContactMeshC(ctypes.Structure):
+    """ppf_contact_mesh (include/ppf_mi355x.h)."""
+    _fields_ = [("nv", u32), ("ne", u32), ("nf", u32), ("x", vp),
+                ("edge", vp), ("face", vp),
+                ("v2e_off", vp), ("v2e", vp), ("v2f_off", vp), ("v2f", vp),
+                ("e2f_off", vp), ("e2f", vp),
+                ("v_ghat", vp), ("v_off", vp), ("e_ghat", vp), ("e_off", vp),
+                ("f_ghat", vp), ("f_off", vp),
+                ("vflag", vp), ("vbody", vp), ("eflag", vp), ("fflag", vp)]
+
+
+def mesh_to_device(m, dev):
+    """Upload a contact-mesh dict (numpy) as contiguous device tensors."""
+    out = {"nv": int(m["nv"]), "ne": int(m["ne"]), "nf": int(m["nf"])}
+    for k in FLOAT_FIELDS:
+        out[k] = torch.from_numpy(np.ascontiguousarray(m[k], np.float32)).reshape(-1).to(dev)
+    for k in INDEX_FIELDS:
+        a = np.ascontiguousarray(m[k], np.uint32).view(np.int32)
+        out[k] = torch.from_numpy(a).reshape(-1).to(dev)
+    return out
+
+
+def mesh_struct(t):
+    fields = {"nv": t["nv"], "ne": t["ne"], "nf": t["nf"]}
+    for k in FLOAT_FIELDS + INDEX_FIELDS:
+        ten = t[k]
+        if not ten.is_cuda or not ten.is_contiguous():
+            raise ValueError(f"{k}: expected a contiguous cuda tensor")
+        fields[k] = ten.data_ptr() if ten.numel() else None
+    return ContactMeshC(**fields)
+
+
+def _stream():
+    return vp(torch.cuda.current_stream().cuda_stream)
+
+
+class NarrowPhase:
+    def __init__(self):
+        self.lib = _lib.load()
+        h = vp()
+        check(self.lib.ppf_narrow_create(ctypes.byref(h)), "ppf_narrow_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.ppf_narrow_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build(self, t):
+        s = mesh_struct(t)
+        check(self.lib.ppf_narrow_build(self._h, ctypes.byref(s), _stream()),
+              "ppf_narrow_build")
+
+    def tree(self, kind):
+        n, root = u32(), u32()
+        check(self.lib.ppf_narrow_tree(self._h, kind, None, None, ctypes.byref(n),
+                                       ctypes.byref(root), _stream()), "ppf_narrow_tree")
+        nn = max(1, 2 * n.value - 1)
+        dev = torch.device("cuda")
+        nodes = torch.zeros(2 * nn, dtype=torch.int32, device=dev)
+        boxes = torch.zeros(8 * nn, dtype=torch.float32, device=dev)
+        check(self.lib.ppf_narrow_tree(self._h, kind, vp(nodes.data_ptr()),
+                                       vp(boxes.data_ptr()), ctypes.byref(n),
+                                       ctypes.byref(root), _stream()), "ppf_narrow_tree")
+        return (nodes.cpu().numpy().view(np.uint32).reshape(-1, 2),
+                boxes.cpu().numpy().reshape(-1, 8), root.value)
+
+    def count(self, t):
+        s = mesh_struct(t)
+        c = ctypes.c_uint64()
+        check(self.lib.ppf_narrow_proximity(self._h, ctypes.byref(s), None, 0,
+                                            ctypes.byref(c), _stream()),
+              "ppf_narrow_proximity")
+        return c.value
+
+    def proximity(self, t, out=None):
+        """Pairs as a device tensor (k, 4) int32 (kind, a, b, count)."""
+        s = mesh_struct(t)
+        c = ctypes.c_uint64()
+        if out is None:
+            check(self.lib.ppf_narrow_proximity(self._h, ctypes.byref(s), None, 0,
+                                                ctypes.byref(c), _stream()),
+                  "ppf_narrow_proximity")
+            out = torch.empty((max(1, c.value), 4), dtype=torch.int32,
+                              device=t["x"].device)
+        cap = out.shape[0]
+        check(self.lib.ppf_narrow_proximity(self._h, ctypes.byref(s),
+                                            vp(out.data_ptr()), cap, ctypes.byref(c),
+                                            _stream()), "ppf_narrow_proximity")
+        if c.value > cap:
+            raise _lib.PPFError(f"pair buffer too small: {c.value} > {cap}")
+        return out[:c.value]
+
+
+def pairs_to_numpy(t):
+    a = t.cpu().numpy().astype(np.uint32)
+    out = np.zeros(len(a), PAIR_DTYPE)
+    for i, k in enumerate(("kind", "a", "b", "count")):
+        out[k] = a[:, i]
+    return out

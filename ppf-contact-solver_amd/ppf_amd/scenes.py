@@ -190,3 +190,84 @@ def long_row_spd_system(n=400, hubs=4, fan=48, seed=0, mass=1.0):
     cdiag = _colmajor(np.broadcast_to(mass * np.eye(3), (n, 3, 3))).astype(np.float32)
     b = rng.standard_normal(3 * n).astype(np.float32)
     return dict(n=n, row_offset=ro, col_index=ci, value=val, cdiag=cdiag, b=b)
+
+
+# ---- contact meshes --------------------------------------------------------
+
+def mesh_topology(nv, faces):
+    """Edges (sorted unique, e[0] < e[1]) and the CSR neighbor tables the
+    narrow phase reads: vertex->edges, vertex->faces, edge->faces (the
+    reference's data.mesh.neighbor.vertex.edge / .face and .edge.face)."""
+    faces = np.asarray(faces, dtype=np.int64)
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+    e.sort(axis=1)
+    key = e[:, 0] * nv + e[:, 1]
+    ukey, inv = np.unique(key, return_inverse=True)
+    edges = np.stack([ukey // nv, ukey % nv], 1)
+    ne, nf = len(edges), len(faces)
+
+    def csr(rows, cols, nrow):
+        order = np.lexsort((cols, rows))
+        off = np.zeros(nrow + 1, np.int64)
+        np.add.at(off, rows + 1, 1)
+        return np.cumsum(off).astype(np.uint32), cols[order].astype(np.uint32)
+
+    v2e_off, v2e = csr(np.concatenate([edges[:, 0], edges[:, 1]]),
+                       np.concatenate([np.arange(ne), np.arange(ne)]), nv)
+    v2f_off, v2f = csr(faces.reshape(-1), np.repeat(np.arange(nf), 3), nv)
+    fe = inv.reshape(3, nf).T  # the three edges of every face
+    e2f_off, e2f = csr(fe.reshape(-1), np.repeat(np.arange(nf), 3), ne)
+    return dict(edge=edges.astype(np.uint32), v2e_off=v2e_off, v2e=v2e,
+                v2f_off=v2f_off, v2f=v2f, e2f_off=e2f_off, e2f=e2f)
+
+
+def contact_mesh(x, faces, ghat=1e-3, offset=0.0):
+    """Contact mesh dict for the narrow phase (uniform contact parameters)."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+    faces = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+    nv = len(x)
+    topo = mesh_topology(nv, faces)
+    ne, nf = len(topo["edge"]), len(faces)
+    m = dict(nv=nv, ne=ne, nf=nf, x=x, face=faces, **topo)
+    for k, n in (("v", nv), ("e", ne), ("f", nf)):
+        m[k + "_ghat"] = np.full(n, ghat, np.float32)
+        m[k + "_off"] = np.full(n, offset, np.float32)
+    m["vflag"] = np.zeros(nv, np.uint32)
+    m["vbody"] = np.zeros(nv, np.uint32)
+    m["eflag"] = np.zeros(ne, np.uint32)
+    m["fflag"] = np.zeros(nf, np.uint32)
+    return m
+
+
+def random_triangles(nf, seed=2, jitter=5e-3):
+    """Independent triangles: centroid ~ U[0,1]^3, every vertex = centroid +
+    U[-jitter, jitter]^3 (the broad/narrow-phase kernel-isolation shape)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, 1.0, size=(nf, 1, 3))
+    x = (c + rng.uniform(-jitter, jitter, size=(nf, 3, 3))).reshape(-1, 3)
+    faces = np.arange(3 * nf, dtype=np.uint32).reshape(nf, 3)
+    return x.astype(np.float32), faces
+
+
+def stacked_sheets(res=32, layers=2, gap=6e-4, noise=2e-4, shift=(0.31, 0.17),
+                   seed=3, size=0.1, shifts=None):
+    """`layers` square sheets (res x res vertices, side `size`) stacked `gap`
+    apart in z with N(0, noise) vertex noise and odd layers shifted by `shift`
+    cells (or layer k by shifts[k]): dense self-proximity with shared mesh
+    topology, so the multiplicity rules of the point-edge / point-point kinds
+    are exercised (nearly aligned vertices give high point-point counts)."""
+    rng = np.random.default_rng(seed)
+    f0 = rect_faces(res, res).astype(np.int64)
+    h = size / (res - 1)
+    i, j = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    xs, faces = [], []
+    for layer in range(layers):
+        sx, sy = (shifts[layer] if shifts is not None
+                  else (shift if layer % 2 else (0.0, 0.0)))
+        p = np.stack([(i.reshape(-1) + sx) * h, (j.reshape(-1) + sy) * h,
+                      np.full(res * res, layer * gap)], 1)
+        p[:, 2] += rng.normal(0.0, noise, res * res)
+        xs.append(p)
+        faces.append(f0 + layer * res * res)
+    return (np.concatenate(xs).astype(np.float32),
+            np.concatenate(faces).astype(np.uint32))

@@ -1,0 +1,215 @@
+"""CPU oracle: distance coefficients, LBVH and the proximity narrow phase.
+
+Pins: bit-exact against the reference's own contact/distance.hpp compiled
+host-side (oracle/_ref); fp64 geometric properties of the coefficients; LBVH
+structural invariants; BVH-query completeness against an all-pairs scan; and
+known-answer configurations for the acceptance rules of contact.cu:432-870.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ppf_amd import scenes
+
+
+def rand_pts(rng, k, m, scale=1.0):
+    return (rng.standard_normal((k, m, 3)) * scale).astype(np.float32)
+
+
+def degenerate_cases(m):
+    rng = np.random.default_rng(9)
+    c = []
+    base = rand_pts(rng, 1, m)[0]
+    c.append(np.repeat(base[:1], m, 0))                  # all coincident
+    if m >= 3:
+        t = base.copy()
+        t[2] = t[1]                                       # collapsed edge
+        c.append(t)
+        t = base.copy()
+        t[-1] = 0.5 * (t[-2] + t[-3])                     # collinear
+        c.append(t.astype(np.float32))
+    if m == 4:
+        t = base.copy()
+        t[3] = t[2] + (t[1] - t[0])                       # parallel edges
+        c.append(t.astype(np.float32))
+    return np.stack(c).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind,m", [("pe", 3), ("pt", 4), ("ee", 4)])
+def test_coefficients_bit_exact_vs_reference_build(kind, m):
+    if oracle.load_ref() is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    rng = np.random.default_rng(1)
+    pts = np.concatenate([rand_pts(rng, 3000, m), rand_pts(rng, 1000, m, 1e-3),
+                          degenerate_cases(m)])
+    a = oracle.coeff(kind, pts)
+    b = oracle.coeff(kind, pts, lib_name="ref")
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_coefficient_geometry_fp64():
+    rng = np.random.default_rng(2)
+    pts = rand_pts(rng, 500, 4).astype(np.float64)
+    # point-edge: residual orthogonal to the edge.
+    w = oracle.coeff("pe", pts[:, :3]).astype(np.float64)
+    p, e0, e1 = pts[:, 0], pts[:, 1], pts[:, 2]
+    r = p - (w[:, :1] * e0 + w[:, 1:2] * e1)
+    assert np.allclose(w.sum(1), 1.0, atol=1e-6)
+    assert np.abs((r * (e1 - e0)).sum(1)).max() < 1e-4
+    # point-triangle projection: residual normal to the plane.
+    w = oracle.coeff("pt", pts).astype(np.float64)
+    p, t0, t1, t2 = pts[:, 0], pts[:, 1], pts[:, 2], pts[:, 3]
+    r = p - (w[:, :1] * t0 + w[:, 1:2] * t1 + w[:, 2:3] * t2)
+    assert np.allclose(w.sum(1), 1.0, atol=1e-5)
+    assert np.abs((r * (t1 - t0)).sum(1)).max() < 1e-4
+    assert np.abs((r * (t2 - t0)).sum(1)).max() < 1e-4
+    # edge-edge: realized distance equals the fp64 minimum over a fine grid
+    # refined around the best sample.
+    w = oracle.coeff("ee", pts).astype(np.float64)
+    a0, a1, b0, b1 = pts[:, 0], pts[:, 1], pts[:, 2], pts[:, 3]
+    got = np.linalg.norm(w[:, :1] * a0 + w[:, 1:2] * a1 - w[:, 2:3] * b0 -
+                         w[:, 3:4] * b1, axis=1)
+    s = np.linspace(0, 1, 201)
+    S, T = np.meshgrid(s, s, indexing="ij")
+    for k in range(0, 500, 25):
+        A = a0[k] + S[..., None] * (a1[k] - a0[k])
+        B = b0[k] + T[..., None] * (b1[k] - b0[k])
+        best = np.linalg.norm(A - B, axis=-1).min()
+        assert got[k] <= best + 1e-5
+
+
+def _leaf_ranges_ok(nodes, n, root):
+    """Every internal node covers a contiguous leaf range; leaves once."""
+    seen = np.zeros(n, int)
+
+    def walk(v):
+        if nodes[v, 1] == 0:
+            seen[v] += 1
+            return v, v
+        l0, l1 = walk(nodes[v, 0] - 1)
+        r0, r1 = walk(nodes[v, 1] - 1)
+        assert l1 + 1 == r0
+        return l0, r1
+    import sys
+    sys.setrecursionlimit(100000)
+    lo, hi = walk(root)
+    return lo == 0 and hi == n - 1 and np.all(seen == 1)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_lbvh_invariants(kind):
+    x, f = scenes.random_triangles(3000, seed=4)
+    m = scenes.contact_mesh(x, f, ghat=1e-3, offset=2e-4)
+    nodes, boxes, root = oracle.bvh_build(m, kind)
+    n = [m["nf"], m["ne"], m["nv"]][kind]
+    assert root == n
+    prims = nodes[:n, 0] - 1
+    assert np.array_equal(np.sort(prims), np.arange(n))
+    assert np.all(nodes[:n, 1] == 0)
+    assert _leaf_ranges_ok(nodes, n, root)
+    # Leaf boxes: the primitive's bounds inflated by 0.5 ghat + offset.
+    xx = m["x"].astype(np.float64)
+    elems = [m["face"], m["edge"], np.arange(m["nv"])[:, None]][kind]
+    pts = xx[elems[prims].astype(np.int64)]
+    lo, hi = pts.min(1), pts.max(1)
+    margin = 0.5e-3 + 2e-4
+    assert np.allclose(boxes[:n, :3], lo - margin, atol=1e-6)
+    assert np.allclose(boxes[:n, 3:], hi + margin, atol=1e-6)
+    # Inner boxes are the unions of their children.
+    inner = np.arange(n, 2 * n - 1)
+    l, r = nodes[inner, 0] - 1, nodes[inner, 1] - 1
+    assert np.array_equal(boxes[inner, :3], np.minimum(boxes[l, :3], boxes[r, :3]))
+    assert np.array_equal(boxes[inner, 3:], np.maximum(boxes[l, 3:], boxes[r, 3:]))
+
+
+def test_query_completeness_against_all_pairs_scan():
+    # Small soup: every pair within reach must be found by the tree query. An
+    # all-pairs distance scan in fp64 gives the point-face pairs whose interior
+    # projection lies within ghat (well away from the threshold).
+    x, f = scenes.random_triangles(1500, seed=5, jitter=0.05)
+    x = (x * 0.2).astype(np.float32)
+    m = scenes.contact_mesh(x, f, ghat=2e-3)
+    pairs = oracle.proximity(m)
+    got = {(int(p["a"]), int(p["b"])) for p in pairs if p["kind"] == 0}
+    xx = x.astype(np.float64)
+    found_any = 0
+    for fi in range(len(f)):
+        t0, t1, t2 = xx[f[fi, 0]], xx[f[fi, 1]], xx[f[fi, 2]]
+        n = np.cross(t1 - t0, t2 - t0)
+        nn = n @ n
+        d = xx - t0
+        u = np.cross(d, t2 - t0) @ n / nn
+        v = np.cross(t1 - t0, d) @ n / nn
+        w0 = 1 - u - v
+        inside = (np.minimum(np.minimum(w0, u), v) > 1e-3) & \
+                 (np.maximum(np.maximum(w0, u), v) < 1 - 1e-3)
+        dist = np.abs(d @ n) / np.sqrt(nn)
+        sure = inside & (dist < 0.9 * 2e-3)
+        sure[f[fi]] = False
+        for vi in np.nonzero(sure)[0]:
+            found_any += 1
+            assert (int(vi), fi) in got, (vi, fi)
+        never = inside & (dist > 1.1 * 2e-3)
+        for vi in np.nonzero(never)[0]:
+            assert (int(vi), fi) not in got
+    assert found_any > 30
+
+
+def test_known_answer_configurations():
+    g = 1e-3
+    # A vertex over the middle of a triangle at 0.5 g is a point-face pair; at
+    # 1.5 g it is not. Its projection onto no edge is interior-and-close, so no
+    # point-edge pair; the point-point distances exceed the reach.
+    tri = np.array([[0, 0, 0], [0.01, 0, 0], [0, 0.01, 0]], np.float32)
+    for h, expect in ((0.5 * g, 1), (1.5 * g, 0)):
+        x = np.concatenate([tri, [[0.003, 0.003, h]]]).astype(np.float32)
+        m = scenes.contact_mesh(x, np.array([[0, 1, 2]], np.uint32), ghat=g)
+        # the lone vertex needs a face of its own to be a mesh vertex
+        p = oracle.proximity(m)
+        pf = [q for q in p if q["kind"] == 0 and q["a"] == 3]
+        assert len(pf) == expect
+    # A vertex beside the hypotenuse, outside the triangle's projection but
+    # over the edge's interior: a point-edge pair whose multiplicity counts the
+    # edge's single face (projection outside it), so count = 1.
+    x = np.concatenate([tri, [[0.0054, 0.0054, 0.3 * g]]]).astype(np.float32)
+    m = scenes.contact_mesh(x, np.array([[0, 1, 2]], np.uint32), ghat=g)
+    p = oracle.proximity(m)
+    pe = [q for q in p if q["kind"] == 1 and q["a"] == 3]
+    assert len(pe) == 1 and pe[0]["count"] == 1
+    e = m["edge"][pe[0]["b"]]
+    assert set(e.tolist()) == {1, 2}
+    # Pinned (both sides fixed) and same rigid body pairs are filtered out.
+    x = np.concatenate([tri, [[0.003, 0.003, 0.5 * g]]]).astype(np.float32)
+    m = scenes.contact_mesh(x, np.array([[0, 1, 2]], np.uint32), ghat=g)
+    m["vflag"][:] = 1
+    m["fflag"][:] = 1
+    assert len([q for q in oracle.proximity(m) if q["a"] == 3]) == 0
+    m = scenes.contact_mesh(x, np.array([[0, 1, 2]], np.uint32), ghat=g)
+    m["vbody"][:] = 7
+    assert len([q for q in oracle.proximity(m) if q["a"] == 3]) == 0
+    # Two crossing edges 0.5 g apart: one edge-edge pair (lower index first).
+    x = np.array([[-1, 0, 0], [1, 0, 0], [0, -1, 0.5 * g], [0, 1, 0.5 * g],
+                  [5, 5, 5], [6, 5, 5]], np.float32) * np.float32(0.01)
+    x[2:4, 2] = 0.5 * g
+    faces = np.array([[0, 1, 4], [2, 3, 5]], np.uint32)
+    m = scenes.contact_mesh(x, faces, ghat=g)
+    ee = [q for q in oracle.proximity(m) if q["kind"] == 3]
+    assert len(ee) == 1
+    ea, eb = m["edge"][ee[0]["a"]], m["edge"][ee[0]["b"]]
+    assert {tuple(sorted(ea)), tuple(sorted(eb))} == {(0, 1), (2, 3)}
+    assert ee[0]["a"] < ee[0]["b"]
+
+
+def test_stacked_sheets_have_every_kind_and_multiplicity():
+    x, f = scenes.stacked_sheets(24, layers=3, gap=4e-4, noise=5e-5,
+                                 shifts=[(0, 0), (0.31, 0.17), (0.02, 0.03)])
+    m = scenes.contact_mesh(x, f, ghat=1e-3)
+    p = oracle.proximity(m)
+    kinds = np.bincount(p["kind"], minlength=4)
+    assert np.all(kinds > 0), kinds
+    assert p["count"].max() > 1
+    # Same thread-count independence as the rest of the oracle.
+    oracle.set_threads(4)
+    q = oracle.proximity(m)
+    oracle.set_threads(1)
+    assert np.array_equal(p, q)
