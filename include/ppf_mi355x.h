@@ -124,9 +124,11 @@ typedef struct ppf_contact_mesh {
     const uint32_t *vflag, *vbody, *eflag, *fflag;
 } ppf_contact_mesh;
 
-/* One accepted stage-0 pair: kind 0 point-face, 1 point-edge, 2 point-point,
- * 3 edge-edge; a = query element (vertex; edge for kind 3), b = candidate
- * element (face, edge, vertex, edge); count = contact multiplicity. */
+/* One accepted stage-0 pair: kind (low byte) 0 point-face, 1 point-edge,
+ * 2 point-point, 3 edge-edge, with bit 8 set when the pair carries no friction
+ * (a neighbor element of the candidate contains the query vertex,
+ * contact.cu:498-531, 685-695); a = query element (vertex; edge for kind 3),
+ * b = candidate element (face, edge, vertex, edge); count = multiplicity. */
 typedef struct ppf_pair {
     uint32_t kind, a, b, count;
 } ppf_pair;
@@ -155,6 +157,47 @@ int ppf_narrow_tree(ppf_narrow *h, int kind, uint32_t *d_nodes, float *d_boxes,
 int ppf_narrow_proximity(ppf_narrow *h, const ppf_contact_mesh *m,
                          ppf_pair *d_out, uint64_t cap, uint64_t *count,
                          void *stream);
+
+/* ------------------------------------------------------------------------
+ * Contact barrier + friction force and Hessian assembly.
+ * ---------------------------------------------------------------------- */
+
+/* Per-solve contact material: the step's start positions x0 (vertex.curr,
+ * 3 nv), per-vertex mass (VertexProp.mass), per-element friction coefficients,
+ * FrictionMode (0 min, 1 max, 2 mean; data.hpp:79), ParamSet.friction_eps, and
+ * the elastic-only upper block CSR the stiffness reads its local Hessian from
+ * (the tmp_fixed snapshot of main.cu:806; may be null = no elastic term). */
+typedef struct ppf_contact_params {
+    const float *x0;
+    const float *mass;
+    const float *v_fric, *e_fric, *f_fric;
+    int32_t friction_mode;
+    float friction_eps;
+    const uint32_t *el_rp, *el_col;
+    const float *el_val;
+} ppf_contact_params;
+
+typedef struct ppf_contact ppf_contact;
+
+int ppf_contact_create(ppf_contact **out);
+void ppf_contact_destroy(ppf_contact *h);
+
+/* Assemble the contact energy gradient (d_grad, 3 nv floats, overwritten) and
+ * the upper-triangle 3x3 contact blocks of the pairs (replaces the stage-1
+ * fill of contact::embed_contact_force_hessian, contact/contact.cu:262-354 and
+ * 1398-1787, with barrier::compute_stiffness, barrier/barrier.cu:49-86, the
+ * cubic barrier and Friction). Blocks are summed per (row, col) in pair order;
+ * *nnz receives their count. m->x holds the Newton iterate (eval_x). */
+int ppf_contact_assemble(ppf_contact *h, const ppf_contact_mesh *m,
+                         const ppf_contact_params *p, const ppf_pair *d_pairs,
+                         uint64_t npairs, float *d_grad, uint64_t *nnz,
+                         void *stream);
+
+/* The blocks of the last assemble, owned by h: row offsets (nv+1), row and
+ * column per block (sorted by row, then column), 9 floats per block. */
+int ppf_contact_blocks(ppf_contact *h, const uint32_t **d_row_offset,
+                       const uint32_t **d_rows, const uint32_t **d_cols,
+                       const float **d_vals, uint64_t *nnz);
 
 #ifdef __cplusplus
 }

@@ -362,8 +362,10 @@ struct oracle_contact_mesh {
 };
 
 // One accepted pair: kind 0 point-face, 1 point-edge, 2 point-point,
-// 3 edge-edge; a = query element (vertex, or edge for kind 3), b = candidate
-// element, count = multiplicity.
+// 3 edge-edge, plus bit 8 set when friction is excluded (a neighbor element of
+// the candidate contains the query vertex, contact.cu:498-531, 685-695);
+// a = query element (vertex, or edge for kind 3), b = candidate element,
+// count = multiplicity.
 struct oracle_pair {
     uint32_t kind, a, b, count;
 };
@@ -523,13 +525,14 @@ uint64_t oracle_proximity(const oracle_contact_mesh *m, oracle_pair *out,
             if (!(orc::sqnorm(e) < r * r)) {
                 return false;
             }
-            uint32_t count = 0;
+            uint32_t count = 0, nofric = 0;
             uint32_t f0 = m->e2f_off[ei], f1 = m->e2f_off[ei + 1];
             if (f1 > f0) {
                 for (uint32_t k = f0; k < f1; ++k) {
                     const uint32_t *f = m->face + 3 * m->e2f[k];
                     if (f[0] == i || f[1] == i || f[2] == i) {
                         ++count;
+                        nofric = 1;
                     } else {
                         float cc[3];
                         pt_coeff(p, x[f[0]], x[f[1]], x[f[2]], cc);
@@ -542,7 +545,7 @@ uint64_t oracle_proximity(const oracle_contact_mesh *m, oracle_pair *out,
                 count = 1;
             }
             if (count) {
-                emit(1, i, ei, count);
+                emit(1 | (nofric << 8), i, ei, count);
                 return true;
             }
             return false;
@@ -562,13 +565,14 @@ uint64_t oracle_proximity(const oracle_contact_mesh *m, oracle_pair *out,
             if (!(orc::sqnorm(e) < r * r)) {
                 return false;
             }
-            uint32_t count = 0;
+            uint32_t count = 0, nofric = 0;
             uint32_t e0 = m->v2e_off[j], e1 = m->v2e_off[j + 1];
             if (e1 > e0) {
                 for (uint32_t k = e0; k < e1; ++k) {
                     const uint32_t *ed = m->edge + 2 * m->v2e[k];
                     if (ed[0] == i || ed[1] == i) {
                         ++count;
+                        nofric = 1;
                     } else {
                         float cc[2];
                         pe_coeff(p, x[ed[0]], x[ed[1]], cc);
@@ -581,6 +585,7 @@ uint64_t oracle_proximity(const oracle_contact_mesh *m, oracle_pair *out,
                     const uint32_t *f = m->face + 3 * m->v2f[k];
                     if (f[0] == i || f[1] == i || f[2] == i) {
                         ++count;
+                        nofric = 1;
                     } else {
                         float cc[3];
                         pt_coeff(p, x[f[0]], x[f[1]], x[f[2]], cc);
@@ -593,7 +598,7 @@ uint64_t oracle_proximity(const oracle_contact_mesh *m, oracle_pair *out,
                 count = 1;
             }
             if (count) {
-                emit(2, i, j, count);
+                emit(2 | (nofric << 8), i, j, count);
                 return true;
             }
             return false;

@@ -210,3 +210,48 @@ def coeff(kind, pts, lib_name="oracle"):
         args = [pts[i, j].ctypes.data_as(vp) for j in range(mm)]
         fn(*args, out[i].ctypes.data_as(vp))
     return out
+
+
+# ---- contact assembly -------------------------------------------------------
+
+class ContactParams(ctypes.Structure):
+    _fields_ = [("x0", vp), ("mass", vp), ("v_fric", vp), ("e_fric", vp),
+                ("f_fric", vp), ("friction_mode", ctypes.c_int),
+                ("friction_eps", ctypes.c_float), ("el_rp", vp), ("el_col", vp),
+                ("el_val", vp)]
+
+
+def contact_assemble(m, x0, mass, fric, pairs, friction_mode=0, friction_eps=1e-5,
+                     elastic=None):
+    """(grad (3nv,), rows, cols, vals (nnz, 9)) in the oracle's order."""
+    lib = load()
+    if not hasattr(lib, "_asm_ready"):
+        lib.oracle_contact_assemble.argtypes = [ctypes.POINTER(ContactMesh),
+                                                ctypes.POINTER(ContactParams), vp,
+                                                ctypes.c_uint64, vp, vp, vp, vp,
+                                                ctypes.c_uint64]
+        lib.oracle_contact_assemble.restype = ctypes.c_uint64
+        lib._asm_ready = True
+    s, keep = _mesh_struct(m)
+    x0 = _f32(x0).reshape(-1)
+    mass = _f32(mass)
+    fv, fe, ff = _f32(fric["v"]), _f32(fric["e"]), _f32(fric["f"])
+    el = elastic or (None, None, None)
+    el_rp = _u32(el[0]) if el[0] is not None else None
+    el_col = _u32(el[1]) if el[1] is not None else None
+    el_val = _f32(el[2]).reshape(-1) if el[2] is not None else None
+    p = ContactParams(x0=_p(x0), mass=_p(mass), v_fric=_p(fv), e_fric=_p(fe),
+                      f_fric=_p(ff), friction_mode=friction_mode,
+                      friction_eps=friction_eps,
+                      el_rp=_p(el_rp) if el_rp is not None else None,
+                      el_col=_p(el_col) if el_col is not None else None,
+                      el_val=_p(el_val) if el_val is not None else None)
+    pr = np.ascontiguousarray(pairs).view(np.uint32).reshape(-1, 4).copy()
+    grad = np.zeros(3 * int(m["nv"]), np.float32)
+    cap = 16 * max(1, len(pr))
+    rows = np.zeros(cap, np.uint32)
+    cols = np.zeros(cap, np.uint32)
+    vals = np.zeros((cap, 9), np.float32)
+    n = lib.oracle_contact_assemble(ctypes.byref(s), ctypes.byref(p), _p(pr), len(pr),
+                                    _p(grad), _p(rows), _p(cols), _p(vals), cap)
+    return grad, rows[:n], cols[:n], vals[:n]

@@ -3,6 +3,7 @@
 // Plain pointers and sizes only; no torch or C++ types cross the boundary.
 #include "common.hpp"
 #include "eig3.hpp"
+#include "assemble.hpp"
 #include "narrow.hpp"
 #include "pcg.hpp"
 #include <cstdarg>
@@ -74,6 +75,10 @@ struct ppf_pcg_plan {
 
 struct ppf_narrow {
     Narrow impl;
+};
+
+struct ppf_contact {
+    ContactAssembler impl;
 };
 
 extern "C" {
@@ -255,6 +260,49 @@ int ppf_narrow_proximity(ppf_narrow *h, const ppf_contact_mesh *m,
         return PPF_ERR_ARG;
     }
     return h->impl.proximity(*m, d_out, cap, count, (hipStream_t)stream);
+}
+
+int ppf_contact_create(ppf_contact **out) {
+    if (!out) {
+        set_error("ppf_contact_create: out is null");
+        return PPF_ERR_ARG;
+    }
+    *out = new ppf_contact();
+    return PPF_OK;
+}
+
+void ppf_contact_destroy(ppf_contact *h) { delete h; }
+
+int ppf_contact_assemble(ppf_contact *h, const ppf_contact_mesh *m,
+                         const ppf_contact_params *p, const ppf_pair *d_pairs,
+                         uint64_t npairs, float *d_grad, uint64_t *nnz,
+                         void *stream) {
+    if (!h || !m || !p || !d_grad || !nnz || (npairs && !d_pairs) || !p->x0 ||
+        !p->mass || !p->v_fric || !p->e_fric || !p->f_fric) {
+        set_error("ppf_contact_assemble: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    int rc = h->impl.assemble(*m, *p, d_pairs, npairs, d_grad, (hipStream_t)stream);
+    *nnz = h->impl.nnz();
+    if (rc == PPF_OK) {
+        PPF_HIP(hipStreamSynchronize((hipStream_t)stream));
+    }
+    return rc;
+}
+
+int ppf_contact_blocks(ppf_contact *h, const uint32_t **d_row_offset,
+                       const uint32_t **d_rows, const uint32_t **d_cols,
+                       const float **d_vals, uint64_t *nnz) {
+    if (!h) {
+        set_error("ppf_contact_blocks: null handle");
+        return PPF_ERR_ARG;
+    }
+    if (d_row_offset) *d_row_offset = h->impl.row_offset();
+    if (d_rows) *d_rows = h->impl.rows();
+    if (d_cols) *d_cols = h->impl.cols();
+    if (d_vals) *d_vals = h->impl.values();
+    if (nnz) *nnz = h->impl.nnz();
+    return PPF_OK;
 }
 
 } // extern "C"

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
         if (!(dot3(e, e) < r * r)) {
             return false;
         }
-        uint32_t count = 0;
+        uint32_t count = 0, nofric = 0;
         const uint32_t k0 = m.e2f_off[ei], k1 = m.e2f_off[ei + 1];
         if (k1 > k0) {
             for (uint32_t k = k0; k < k1; ++k) {
@@ -184,6 +184,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
                                g2 = m.face[3 * (size_t)f + 2];
                 if (g0 == i || g1 == i || g2 == i) {
                     ++count;
+                    nofric = 1;
                 } else {
                     float cc[3];
                     pt_coeff(p, X(m, g0), X(m, g1), X(m, g2), cc);
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
             count = 1;
         }
         if (count) {
-            emit<FILL>(c, slot, 1, i, ei, count);
+            emit<FILL>(c, slot, 1u | (nofric << 8), i, ei, count);
             return true;
         }
         return false;
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
         if (!(dot3(e, e) < r * r)) {
             return false;
         }
-        uint32_t count = 0;
+        uint32_t count = 0, nofric = 0;
         const uint32_t k0 = m.v2e_off[j], k1 = m.v2e_off[j + 1];
         if (k1 > k0) {
             for (uint32_t k = k0; k < k1; ++k) {
@@ -224,6 +225,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
                 const uint32_t a0 = m.edge[2 * (size_t)ed], a1 = m.edge[2 * (size_t)ed + 1];
                 if (a0 == i || a1 == i) {
                     ++count;
+                    nofric = 1;
                 } else {
                     float cc[2];
                     pe_coeff(p, X(m, a0), X(m, a1), cc);
@@ -238,6 +240,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
                                g2 = m.face[3 * (size_t)f + 2];
                 if (g0 == i || g1 == i || g2 == i) {
                     ++count;
+                    nofric = 1;
                 } else {
                     float cc[3];
                     pt_coeff(p, X(m, g0), X(m, g1), X(m, g2), cc);
@@ -250,7 +253,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
             count = 1;
         }
         if (count) {
-            emit<FILL>(c, slot, 2, i, j, count);
+            emit<FILL>(c, slot, 2u | (nofric << 8), i, j, count);
             return true;
         }
         return false;

@@ -44,6 +44,13 @@ _PROTOS = {
                                        ctypes.POINTER(u32), vp]),
     "ppf_narrow_proximity": (ctypes.c_int, [vp, vp, vp, ctypes.c_uint64,
                                             ctypes.POINTER(ctypes.c_uint64), vp]),
+    "ppf_contact_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
+    "ppf_contact_destroy": (None, [vp]),
+    "ppf_contact_assemble": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_uint64, vp,
+                                            ctypes.POINTER(ctypes.c_uint64), vp]),
+    "ppf_contact_blocks": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                          ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 _lib = None

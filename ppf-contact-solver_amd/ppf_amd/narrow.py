@@ -128,3 +128,90 @@ def pairs_to_numpy(t):
     for i, k in enumerate(("kind", "a", "b", "count")):
         out[k] = a[:, i]
     return out
+
+
+class ContactParamsC(ctypes.Structure):
+    """ppf_contact_params (include/ppf_mi355x.h)."""
+    _fields_ = [("x0", vp), ("mass", vp), ("v_fric", vp), ("e_fric", vp),
+                ("f_fric", vp), ("friction_mode", ctypes.c_int32),
+                ("friction_eps", ctypes.c_float), ("el_rp", vp), ("el_col", vp),
+                ("el_val", vp)]
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+class ContactAssembler:
+    """Contact gradient + upper-triangle 3x3 blocks of a pair list."""
+
+    def __init__(self):
+        self.lib = _lib.load()
+        h = vp()
+        check(self.lib.ppf_contact_create(ctypes.byref(h)), "ppf_contact_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.ppf_contact_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def assemble(self, t, x0, mass, fric, pairs, friction_mode=0, friction_eps=1e-5,
+                 elastic=None):
+        """t: device contact mesh (x = Newton iterate); x0, mass: device
+        tensors; fric: dict v/e/f -> device tensors; pairs: (k, 4) int32 device
+        tensor; elastic: (row_offset, col, val) device tensors or None.
+        Returns (grad, row_offset, rows, cols, vals) as device tensors (the
+        block arrays are copies)."""
+        s = mesh_struct(t)
+        el = elastic or (None, None, None)
+        p = ContactParamsC(x0=_ptr(x0), mass=_ptr(mass), v_fric=_ptr(fric["v"]),
+                           e_fric=_ptr(fric["e"]), f_fric=_ptr(fric["f"]),
+                           friction_mode=friction_mode, friction_eps=friction_eps,
+                           el_rp=_ptr(el[0]), el_col=_ptr(el[1]), el_val=_ptr(el[2]))
+        grad = torch.empty(3 * t["nv"], dtype=torch.float32, device=t["x"].device)
+        nnz = ctypes.c_uint64()
+        check(self.lib.ppf_contact_assemble(self._h, ctypes.byref(s), ctypes.byref(p),
+                                            vp(pairs.data_ptr()), pairs.shape[0],
+                                            vp(grad.data_ptr()), ctypes.byref(nnz),
+                                            _stream()), "ppf_contact_assemble")
+        rp, rows, cols, vals = vp(), vp(), vp(), vp()
+        check(self.lib.ppf_contact_blocks(self._h, ctypes.byref(rp), ctypes.byref(rows),
+                                          ctypes.byref(cols), ctypes.byref(vals),
+                                          ctypes.byref(nnz)), "ppf_contact_blocks")
+        n = nnz.value
+        dev = t["x"].device
+        out_rp = _copy_dev(rp.value, t["nv"] + 1, torch.int32, dev)
+        out_rows = _copy_dev(rows.value, n, torch.int32, dev)
+        out_cols = _copy_dev(cols.value, n, torch.int32, dev)
+        out_vals = _copy_dev(vals.value, 9 * n, torch.float32, dev)
+        return grad, out_rp, out_rows, out_cols, out_vals
+
+
+def _copy_dev(ptr, count, dtype, dev):
+    """Copy `count` elements from a library-owned device pointer."""
+    out = torch.empty(max(1, count), dtype=dtype, device=dev)
+    if count and ptr:
+        nbytes = count * out.element_size()
+        rc = _hip().hipMemcpy(vp(out.data_ptr()), vp(ptr), ctypes.c_size_t(nbytes), 3)
+        if rc != 0:
+            raise _lib.PPFError(f"hipMemcpy failed ({rc})")
+    return out[:count]
+
+
+_HIP = None
+
+
+def _hip():
+    global _HIP
+    if _HIP is None:
+        _HIP = ctypes.CDLL("libamdhip64.so")
+        _HIP.hipMemcpy.restype = ctypes.c_int
+        _HIP.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    return _HIP

@@ -271,3 +271,52 @@ def stacked_sheets(res=32, layers=2, gap=6e-4, noise=2e-4, shift=(0.31, 0.17),
         faces.append(f0 + layer * res * res)
     return (np.concatenate(xs).astype(np.float32),
             np.concatenate(faces).astype(np.uint32))
+
+
+def face_element_bsr(nv, faces, seed=0, scale=1.0):
+    """Random PSD per-face 9x9 element matrices G^T G assembled into the
+    reference's upper block CSR (stand-in for an elastic Hessian: its
+    off-diagonal blocks are unsymmetric, so a lookup that forgets to transpose
+    (i > j) is caught)."""
+    faces = np.asarray(faces, dtype=np.int64)
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((len(faces), 5, 9))
+    E = np.einsum("fki,fkj->fij", g, g) * (scale / 9.0)
+    keys, blks = [], []
+    for a in range(3):
+        for c in range(3):
+            i, j = faces[:, a], faces[:, c]
+            sel = i <= j
+            keys.append(np.stack([i[sel], j[sel]], 1))
+            blks.append(E[sel, 3 * a:3 * a + 3, 3 * c:3 * c + 3])
+    keys = np.concatenate(keys)
+    blks = np.concatenate(blks)
+    key = keys[:, 0] * nv + keys[:, 1]
+    uk, inv = np.unique(key, return_inverse=True)
+    acc = np.zeros((len(uk), 3, 3))
+    np.add.at(acc, inv, blks)
+    rows, cols = uk // nv, uk % nv
+    diag = np.zeros((nv, 3, 3))
+    on = rows == cols
+    diag[rows[on]] = acc[on]
+    ro, ci, val = upper_bsr(nv, np.stack([rows[~on], cols[~on]], 1), acc[~on], diag)
+    return ro, ci, val
+
+
+def contact_scenario(res=24, seed=0, slip=2e-5):
+    """Stacked-sheet contact mesh plus everything the contact assembly reads:
+    start positions x0 = x - slip noise (both friction branches occur),
+    masses (a few zero, as pinned vertices carry), friction per element, and a
+    random elastic upper block CSR on the face pattern."""
+    x, f = stacked_sheets(res, layers=3, gap=4e-4, noise=5e-5,
+                          shifts=[(0, 0), (0.31, 0.17), (0.02, 0.03)], seed=seed)
+    m = contact_mesh(x, f, ghat=1e-3)
+    rng = np.random.default_rng(seed + 100)
+    x0 = (x + rng.normal(0.0, slip, x.shape)).astype(np.float32)
+    mass = rng.uniform(1e-5, 1e-4, m["nv"]).astype(np.float32)
+    mass[rng.random(m["nv"]) < 0.05] = 0.0
+    fric = {"v": rng.uniform(0.0, 0.5, m["nv"]).astype(np.float32),
+            "e": rng.uniform(0.0, 0.5, m["ne"]).astype(np.float32),
+            "f": rng.uniform(0.0, 0.5, m["nf"]).astype(np.float32)}
+    elastic = face_element_bsr(m["nv"], f, seed=seed + 7, scale=1.0)
+    return m, x0, mass, fric, elastic
