@@ -199,6 +199,57 @@ int ppf_contact_blocks(ppf_contact *h, const uint32_t **d_row_offset,
                        const uint32_t **d_rows, const uint32_t **d_cols,
                        const float **d_vals, uint64_t *nnz);
 
+/* ------------------------------------------------------------------------
+ * CCD step filter (contact line search).
+ * ---------------------------------------------------------------------- */
+
+/* Static collision mesh (data.constraint.mesh, data.hpp:225-240): positions,
+ * edges, faces and per-element contact gap / offset, all device pointers.
+ * nv == 0 (or a null pointer to it) means no collision mesh. */
+typedef struct ppf_static_mesh {
+    uint32_t nv, ne, nf;
+    const float *x;
+    const uint32_t *edge, *face;
+    const float *v_ghat, *v_off, *e_ghat, *e_off, *f_ghat, *f_off;
+} ppf_static_mesh;
+
+/* Line-search inputs besides the mesh (whose x is the step's start x0): the
+ * target positions x1 (3 nv), VertexProp / EdgeProp / FaceProp mass (the
+ * collision-mesh sweeps skip massless elements), ParamSet.line_search_max_t
+ * and ParamSet.ccd_eps. */
+typedef struct ppf_ls_params {
+    const float *x1;
+    const float *vmass, *emass, *fmass;
+    float max_t;
+    float ccd_eps;
+} ppf_ls_params;
+
+typedef struct ppf_ccd ppf_ccd;
+
+int ppf_ccd_create(ppf_ccd **out);
+void ppf_ccd_destroy(ppf_ccd *h);
+
+/* Largest certified fraction of the step x0 -> x1 (replaces
+ * contact::line_search, contact/contact.cu:2472-2740, with the ACCD of
+ * contact/accd.hpp:64-696): the minimum time of impact over all point-face,
+ * point-point, edge-edge and collision-mesh pairs whose swept boxes overlap,
+ * divided by max_t; *toi = 1 when nothing is hit. *overlap is set when a pair
+ * starts inside its offset. rebuild = 0 refits the swept trees of the last
+ * call on the same mesh sizes instead of rebuilding them. */
+int ppf_line_search(ppf_ccd *h, const ppf_contact_mesh *m, const ppf_static_mesh *sm,
+                    const ppf_ls_params *p, int rebuild, float *toi, int *overlap,
+                    void *stream);
+
+/* ACCD of n independent pairs of one kind (accd::point_triangle_ccd,
+ * point_edge_ccd, point_point_ccd, edge_edge_ccd; contact/accd.hpp:547-696):
+ * kind 0 point-triangle (p, t0, t1, t2), 1 point-edge (p, e0, e1),
+ * 2 point-point, 3 edge-edge (a0, a1, b0, b1). d_x0 / d_x1 hold 3 * columns
+ * floats per pair, d_offset / d_ghat one float per pair. Writes the toi per
+ * pair to d_toi and the overlap flag (0/1) to d_overlap (may be null). */
+int ppf_ccd_batch(int kind, uint32_t n, const float *d_x0, const float *d_x1,
+                  const float *d_offset, const float *d_ghat, float max_t,
+                  float ccd_eps, float *d_toi, int32_t *d_overlap, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -255,3 +255,71 @@ def contact_assemble(m, x0, mass, fric, pairs, friction_mode=0, friction_eps=1e-
     n = lib.oracle_contact_assemble(ctypes.byref(s), ctypes.byref(p), _p(pr), len(pr),
                                     _p(grad), _p(rows), _p(cols), _p(vals), cap)
     return grad, rows[:n], cols[:n], vals[:n]
+
+
+# ---- CCD / line search ------------------------------------------------------
+
+class StaticMesh(ctypes.Structure):
+    _fields_ = [("nv", u32), ("ne", u32), ("nf", u32), ("x", vp), ("edge", vp),
+                ("face", vp), ("v_ghat", vp), ("v_off", vp), ("e_ghat", vp),
+                ("e_off", vp), ("f_ghat", vp), ("f_off", vp)]
+
+
+class LsParams(ctypes.Structure):
+    _fields_ = [("x1", vp), ("vmass", vp), ("emass", vp), ("fmass", vp),
+                ("max_t", f32), ("ccd_eps", f32)]
+
+
+def _load_ccd():
+    lib = load()
+    if not hasattr(lib, "_ccd_ready"):
+        lib.oracle_ccd.argtypes = [ctypes.c_int, vp, vp, f32, f32, f32, f32,
+                                   ctypes.POINTER(ctypes.c_int)]
+        lib.oracle_ccd.restype = f32
+        lib.oracle_line_search.argtypes = [ctypes.POINTER(ContactMesh),
+                                           ctypes.POINTER(StaticMesh),
+                                           ctypes.POINTER(LsParams),
+                                           ctypes.POINTER(ctypes.c_int)]
+        lib.oracle_line_search.restype = f32
+        lib._ccd_ready = True
+    return lib
+
+
+def ccd(kind, x0, x1, offset, ghat, max_t=1.25, ccd_eps=1e-7):
+    """Single-pair ACCD: kind 0 point-triangle, 1 point-edge, 2 point-point,
+    3 edge-edge; x0/x1 (columns, 3). Returns (toi, overlap)."""
+    lib = _load_ccd()
+    a, b = _f32(x0).reshape(-1), _f32(x1).reshape(-1)
+    ov = ctypes.c_int()
+    t = lib.oracle_ccd(kind, _p(a), _p(b), offset, ghat, max_t, ccd_eps,
+                       ctypes.byref(ov))
+    return float(t), bool(ov.value)
+
+
+def static_mesh_struct(sm):
+    keep = {}
+    if sm is None:
+        return StaticMesh(nv=0, ne=0, nf=0), keep
+    fields = {"nv": int(sm["nv"]), "ne": int(sm["ne"]), "nf": int(sm["nf"])}
+    for name in ("x", "edge", "face", "v_ghat", "v_off", "e_ghat", "e_off",
+                 "f_ghat", "f_off"):
+        dt = np.uint32 if name in ("edge", "face") else np.float32
+        a = np.ascontiguousarray(sm[name], dtype=dt)
+        keep[name] = a
+        fields[name] = _p(a)
+    return StaticMesh(**fields), keep
+
+
+def line_search(m, x1, vmass, emass, fmass, static=None, max_t=1.25, ccd_eps=1e-7):
+    """contact::line_search over the mesh from m['x'] to x1: (fraction, overlap)."""
+    lib = _load_ccd()
+    s, keep = _mesh_struct(m)
+    ss, keep2 = static_mesh_struct(static)
+    x1 = _f32(x1).reshape(-1)
+    vm, em, fm = _f32(vmass), _f32(emass), _f32(fmass)
+    p = LsParams(x1=_p(x1), vmass=_p(vm), emass=_p(em), fmass=_p(fm), max_t=max_t,
+                 ccd_eps=ccd_eps)
+    ov = ctypes.c_int()
+    t = lib.oracle_line_search(ctypes.byref(s), ctypes.byref(ss), ctypes.byref(p),
+                               ctypes.byref(ov))
+    return float(t), bool(ov.value)

@@ -4,6 +4,7 @@
 #include "common.hpp"
 #include "eig3.hpp"
 #include "assemble.hpp"
+#include "linesearch.hpp"
 #include "narrow.hpp"
 #include "pcg.hpp"
 #include <cstdarg>
@@ -79,6 +80,10 @@ struct ppf_narrow {
 
 struct ppf_contact {
     ContactAssembler impl;
+};
+
+struct ppf_ccd {
+    LineSearch impl;
 };
 
 extern "C" {
@@ -303,6 +308,49 @@ int ppf_contact_blocks(ppf_contact *h, const uint32_t **d_row_offset,
     if (d_vals) *d_vals = h->impl.values();
     if (nnz) *nnz = h->impl.nnz();
     return PPF_OK;
+}
+
+int ppf_ccd_create(ppf_ccd **out) {
+    if (!out) {
+        set_error("ppf_ccd_create: out is null");
+        return PPF_ERR_ARG;
+    }
+    *out = new ppf_ccd();
+    return PPF_OK;
+}
+
+void ppf_ccd_destroy(ppf_ccd *h) { delete h; }
+
+int ppf_line_search(ppf_ccd *h, const ppf_contact_mesh *m, const ppf_static_mesh *sm,
+                    const ppf_ls_params *p, int rebuild, float *toi, int *overlap,
+                    void *stream) {
+    if (!h || !m || !p || !toi || !overlap || !m->x || !p->x1 || !p->vmass ||
+        !p->emass || !p->fmass || !(p->max_t > 0.0f)) {
+        set_error("ppf_line_search: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    if (sm && sm->nv && (!sm->x || (sm->ne && !sm->edge) || (sm->nf && !sm->face))) {
+        set_error("ppf_line_search: static mesh arrays missing");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.run(*m, sm, *p, rebuild != 0, toi, overlap, (hipStream_t)stream);
+}
+
+int ppf_ccd_batch(int kind, uint32_t n, const float *d_x0, const float *d_x1,
+                  const float *d_offset, const float *d_ghat, float max_t,
+                  float ccd_eps, float *d_toi, int32_t *d_overlap, void *stream) {
+    if (kind < 0 || kind > 3 || (n && (!d_x0 || !d_x1 || !d_offset || !d_ghat || !d_toi)) ||
+        !(max_t > 0.0f)) {
+        set_error("ppf_ccd_batch: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int rc = ccd_batch(kind, n, d_x0, d_x1, d_offset, d_ghat, max_t, ccd_eps, d_toi,
+                       d_overlap, st);
+    if (rc == PPF_OK) {
+        PPF_HIP(hipStreamSynchronize(st));
+    }
+    return rc;
 }
 
 } // extern "C"

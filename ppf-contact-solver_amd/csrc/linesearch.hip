@@ -1,0 +1,449 @@
+// linesearch.hip -- contact CCD line search on MI355X.
+//
+// Replaces contact::line_search (contact/contact.cu:2472-2740) with the CCD
+// functors of contact.cu:1956-2316: per dynamic vertex the point-face
+// (own faces: the vertex against the opposite edge), point-point and
+// vertex-vs-static-face sweeps; per static vertex the face sweeps; the
+// vertex-face minimum then seeds every edge's sweep box for the edge-edge
+// (shared vertex: two point-edge sweeps) and edge-vs-static-edge sweeps.
+// Boxes are the reference's swept boxes; the minima are combined with
+// integer atomics on the float bits (all values are non-negative), which is
+// exact and order independent.
+#include "ccd.hpp"
+#include "linesearch.hpp"
+
+namespace ppf {
+
+namespace {
+
+constexpr int kQT = 128;
+constexpr int kStack = 64;
+
+struct Box6 {
+    float mn[3], mx[3];
+};
+
+__device__ inline bool overlaps(const Aabb &a, const Box6 &q) {
+    return a.active != 0.0f && (a.mn[0] <= q.mx[0] && a.mx[0] >= q.mn[0]) &&
+           (a.mn[1] <= q.mx[1] && a.mx[1] >= q.mn[1]) &&
+           (a.mn[2] <= q.mx[2] && a.mx[2] >= q.mn[2]);
+}
+
+__device__ inline Aabb load_box(const Aabb *b, uint32_t i) {
+    const float4 *p = reinterpret_cast<const float4 *>(b + i);
+    const float4 lo = p[0], hi = p[1];
+    Aabb r;
+    r.mn[0] = lo.x, r.mn[1] = lo.y, r.mn[2] = lo.z, r.active = lo.w;
+    r.mx[0] = hi.x, r.mx[1] = hi.y, r.mx[2] = hi.z, r.pad = hi.w;
+    return r;
+}
+
+template <class F>
+__device__ void traverse(const BvhView &bvh, const Box6 &q, uint32_t *stk,
+                         uint32_t *overflow, F &&leaf) {
+    if (bvh.n == 0 || !overlaps(load_box(bvh.box, bvh.root), q)) {
+        return;
+    }
+    uint32_t sp = 0;
+    stk[(sp++) * kQT] = bvh.root;
+    while (sp) {
+        const uint32_t v = stk[(--sp) * kQT];
+        const uint32_t c1 = bvh.node[2 * (size_t)v + 1];
+        if (c1 == 0) {
+            leaf(bvh.node[2 * (size_t)v] - 1);
+            continue;
+        }
+        const uint32_t l = bvh.node[2 * (size_t)v] - 1, r = c1 - 1;
+        const bool ol = overlaps(load_box(bvh.box, l), q);
+        const bool orr = overlaps(load_box(bvh.box, r), q);
+        if (sp + 2 > (uint32_t)kStack) {
+            atomicOr(overflow, 1u);
+            return;
+        }
+        if (ol) {
+            stk[(sp++) * kQT] = l;
+        }
+        if (orr) {
+            stk[(sp++) * kQT] = r;
+        }
+    }
+}
+
+struct LsCtx {
+    ppf_contact_mesh m;
+    ppf_static_mesh sm;
+    ppf_ls_params lp;
+    BvhView fb, eb, vb, sfb, seb;
+    uint32_t *out; // [0] t_vf bits, [1] t_ee bits, [2] overlap, [3] stack overflow
+};
+
+__device__ inline f3 X(const float *a, uint32_t v) { return load3(a + 3 * (size_t)v); }
+
+__device__ inline f3 XS(const LsCtx &c, uint32_t v) {
+    const f3 a = X(c.m.x, v), b = X(c.lp.x1, v);
+    const float t = c.lp.max_t;
+    return mk3(t * (b.x - a.x) + a.x, t * (b.y - a.y) + a.y, t * (b.z - a.z) + a.z);
+}
+
+__device__ inline Box6 seg_box(f3 a, f3 b, float margin) {
+    Box6 q{{fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)},
+           {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}};
+    if (margin != 0.0f) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            q.mn[d] -= margin;
+            q.mx[d] += margin;
+        }
+    }
+    return q;
+}
+
+__device__ inline void note(const LsCtx &c, bool ovl, float r, float &best) {
+    if (ovl) {
+        atomicOr(c.out + 2, 1u);
+    }
+    if (r < c.lp.max_t) {
+        best = fminf(best, r);
+    }
+}
+
+__global__ __launch_bounds__(kQT) void k_ls_vertex(LsCtx c) {
+    __shared__ uint32_t stk_all[kStack * kQT];
+    const uint32_t leaf = blockIdx.x * kQT + threadIdx.x;
+    if (leaf >= c.vb.n) {
+        return;
+    }
+    uint32_t *stk = stk_all + threadIdx.x;
+    const ppf_contact_mesh &m = c.m;
+    const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
+    const uint32_t i = c.vb.node[2 * (size_t)leaf] - 1;
+    const Box6 q = seg_box(X(m.x, i), XS(c, i), 0.5f * m.v_ghat[i] + m.v_off[i]);
+    const bool idyn = (m.vflag[i] & 1u) == 0, icol = (m.vflag[i] & 2u) != 0;
+    float best = max_t;
+    // point-face (contact.cu:2110-2164)
+    traverse(c.fb, q, stk, c.out + 3, [&](uint32_t fi) {
+        const uint32_t f[3] = {m.face[3 * (size_t)fi], m.face[3 * (size_t)fi + 1],
+                               m.face[3 * (size_t)fi + 2]};
+        const bool either = idyn || (m.fflag[fi] & 1u) == 0;
+        const bool same = m.vbody[i] != 0 && m.vbody[i] == m.vbody[f[0]];
+        const bool both = icol && (m.vflag[f[0]] & 2u) != 0;
+        if (!either || same || both) {
+            return;
+        }
+        int dup = -1;
+        for (int k = 0; k < 3; ++k) {
+            if (f[k] == i) {
+                dup = k;
+                break;
+            }
+        }
+        bool ovl = false;
+        float r;
+        if (dup < 0) {
+            const f3 a[4] = {X(m.x, i), X(m.x, f[0]), X(m.x, f[1]), X(m.x, f[2])};
+            const f3 b[4] = {X(c.lp.x1, i), X(c.lp.x1, f[0]), X(c.lp.x1, f[1]),
+                             X(c.lp.x1, f[2])};
+            r = ccd<4, 1>(a, b, m.v_off[i] + m.f_off[fi],
+                          0.5f * (m.v_ghat[i] + m.f_ghat[fi]), max_t, eps, ovl);
+        } else {
+            const uint32_t p = f[dup], j = f[(dup + 1) % 3], k = f[(dup + 2) % 3];
+            const f3 a[3] = {X(m.x, p), X(m.x, j), X(m.x, k)};
+            const f3 b[3] = {X(c.lp.x1, p), X(c.lp.x1, j), X(c.lp.x1, k)};
+            r = ccd<3, 1>(a, b, 2.0f * m.v_off[i], m.v_ghat[i], max_t, eps, ovl);
+        }
+        note(c, ovl, r, best);
+    });
+    // point-point (contact.cu:2266-2316)
+    traverse(c.vb, q, stk, c.out + 3, [&](uint32_t j) {
+        const bool either = idyn || (m.vflag[j] & 1u) == 0;
+        const bool same = m.vbody[i] != 0 && m.vbody[i] == m.vbody[j];
+        const bool both = icol && (m.vflag[j] & 2u) != 0;
+        if (!(j < i) || !either || same || both) {
+            return;
+        }
+        const f3 a[2] = {X(m.x, i), X(m.x, j)};
+        const f3 b[2] = {X(c.lp.x1, i), X(c.lp.x1, j)};
+        bool ovl = false;
+        const float r = ccd<2, 1>(a, b, m.v_off[i] + m.v_off[j],
+                                  0.5f * (m.v_ghat[i] + m.v_ghat[j]), max_t, eps, ovl);
+        note(c, ovl, r, best);
+    });
+    // dynamic vertex vs static face (contact.cu:1956-1994)
+    if (c.sm.nf && idyn) {
+        traverse(c.sfb, q, stk, c.out + 3, [&](uint32_t fi) {
+            if (c.lp.vmass[i] == 0.0f) {
+                return;
+            }
+            const uint32_t f0 = c.sm.face[3 * (size_t)fi], f1 = c.sm.face[3 * (size_t)fi + 1],
+                           f2 = c.sm.face[3 * (size_t)fi + 2];
+            const f3 t0 = X(c.sm.x, f0), t1 = X(c.sm.x, f1), t2 = X(c.sm.x, f2);
+            const f3 a[4] = {X(m.x, i), t0, t1, t2};
+            const f3 b[4] = {X(c.lp.x1, i), t0, t1, t2};
+            bool ovl = false;
+            const float r = ccd<4, 1>(a, b, m.v_off[i] + c.sm.f_off[fi],
+                                      0.5f * (m.v_ghat[i] + c.sm.f_ghat[fi]), max_t,
+                                      eps, ovl);
+            note(c, ovl, r, best);
+        });
+    }
+    if (best < max_t) {
+        atomicMin(c.out + 0, __float_as_uint(best));
+    }
+}
+
+// static vertex vs dynamic face (contact.cu:1996-2034)
+__global__ __launch_bounds__(kQT) void k_ls_static_vertex(LsCtx c) {
+    __shared__ uint32_t stk_all[kStack * kQT];
+    const uint32_t i = blockIdx.x * kQT + threadIdx.x;
+    if (i >= c.sm.nv) {
+        return;
+    }
+    uint32_t *stk = stk_all + threadIdx.x;
+    const ppf_contact_mesh &m = c.m;
+    const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
+    const f3 p = X(c.sm.x, i);
+    const float qm = 0.5f * c.sm.v_ghat[i] + c.sm.v_off[i];
+    const Box6 q{{p.x - qm, p.y - qm, p.z - qm}, {p.x + qm, p.y + qm, p.z + qm}};
+    float best = max_t;
+    traverse(c.fb, q, stk, c.out + 3, [&](uint32_t fi) {
+        if ((m.fflag[fi] & 1u) != 0 || !(c.lp.fmass[fi] > 0.0f)) {
+            return;
+        }
+        const uint32_t f0 = m.face[3 * (size_t)fi], f1 = m.face[3 * (size_t)fi + 1],
+                       f2 = m.face[3 * (size_t)fi + 2];
+        const f3 a[4] = {p, X(m.x, f0), X(m.x, f1), X(m.x, f2)};
+        const f3 b[4] = {p, X(c.lp.x1, f0), X(c.lp.x1, f1), X(c.lp.x1, f2)};
+        bool ovl = false;
+        const float r = ccd<4, 1>(a, b, m.f_off[fi] + c.sm.v_off[i],
+                                  0.5f * (m.f_ghat[fi] + c.sm.v_ghat[i]), max_t, eps,
+                                  ovl);
+        note(c, ovl, r, best);
+    });
+    if (best < max_t) {
+        atomicMin(c.out + 0, __float_as_uint(best));
+    }
+}
+
+// edge-edge and edge vs static edge, seeded by the vertex-face minimum.
+__global__ __launch_bounds__(kQT) void k_ls_edge(LsCtx c) {
+    __shared__ uint32_t stk_all[kStack * kQT];
+    const uint32_t leaf = blockIdx.x * kQT + threadIdx.x;
+    if (leaf >= c.eb.n) {
+        return;
+    }
+    uint32_t *stk = stk_all + threadIdx.x;
+    const ppf_contact_mesh &m = c.m;
+    const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
+    const float seed = __uint_as_float(c.out[0]);
+    const uint32_t ei = c.eb.node[2 * (size_t)leaf] - 1;
+    const uint32_t a0 = m.edge[2 * (size_t)ei], a1 = m.edge[2 * (size_t)ei + 1];
+    const float ext = 0.5f * m.e_ghat[ei] + m.e_off[ei];
+    const f3 p0 = X(m.x, a0), p1 = X(m.x, a1), q0 = X(c.lp.x1, a0), q1 = X(c.lp.x1, a1);
+    const f3 s0 = mk3(seed * (q0.x - p0.x) + p0.x, seed * (q0.y - p0.y) + p0.y,
+                      seed * (q0.z - p0.z) + p0.z);
+    const f3 s1 = mk3(seed * (q1.x - p1.x) + p1.x, seed * (q1.y - p1.y) + p1.y,
+                      seed * (q1.z - p1.z) + p1.z);
+    const Box6 b0 = seg_box(p0, p1, ext), b1 = seg_box(s0, s1, ext);
+    Box6 q;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        q.mn[d] = fminf(b0.mn[d], b1.mn[d]);
+        q.mx[d] = fmaxf(b0.mx[d], b1.mx[d]);
+    }
+    const bool edyn = (m.eflag[ei] & 1u) == 0;
+    const bool acol = (m.vflag[a0] & 2u) != 0;
+    float best = seed;
+    traverse(c.eb, q, stk, c.out + 3, [&](uint32_t ej) {
+        const uint32_t b0i = m.edge[2 * (size_t)ej], b1i = m.edge[2 * (size_t)ej + 1];
+        const bool either = edyn || (m.eflag[ej] & 1u) == 0;
+        const bool same = m.vbody[a0] != 0 && m.vbody[a0] == m.vbody[b0i];
+        const bool both = acol && (m.vflag[b0i] & 2u) != 0;
+        if (!(ei < ej) || !either || same || both) {
+            return;
+        }
+        const float off = m.e_off[ei] + m.e_off[ej];
+        const float gh = 0.5f * (m.e_ghat[ei] + m.e_ghat[ej]);
+        const uint32_t ea[2] = {a0, a1}, eb[2] = {b0i, b1i};
+        float r = max_t;
+        bool ovl = false;
+        const bool shared = a0 == b0i || a0 == b1i || a1 == b0i || a1 == b1i;
+        if (!shared) {
+            const f3 a[4] = {p0, p1, X(m.x, b0i), X(m.x, b1i)};
+            const f3 b[4] = {q0, q1, X(c.lp.x1, b0i), X(c.lp.x1, b1i)};
+            r = ccd<4, 2>(a, b, off, gh, max_t, eps, ovl);
+        } else {
+            const int ij[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+            for (int k = 0; k < 4; ++k) {
+                const int ii = ij[k][0], jj = ij[k][1];
+                if (ea[ii] == eb[jj]) {
+                    const uint32_t i0 = ea[ii], i1 = ea[1 - ii], i2 = eb[1 - jj];
+                    const f3 x0a[3] = {X(m.x, i1), X(m.x, i0), X(m.x, i2)};
+                    const f3 x1a[3] = {X(c.lp.x1, i1), X(c.lp.x1, i0), X(c.lp.x1, i2)};
+                    const f3 x0b[3] = {X(m.x, i2), X(m.x, i0), X(m.x, i1)};
+                    const f3 x1b[3] = {X(c.lp.x1, i2), X(c.lp.x1, i0), X(c.lp.x1, i1)};
+                    const float r0 = ccd<3, 1>(x0a, x1a, off, gh, max_t, eps, ovl);
+                    const float r1 = ccd<3, 1>(x0b, x1b, off, gh, max_t, eps, ovl);
+                    r = fminf(r0, r1);
+                    break;
+                }
+            }
+        }
+        note(c, ovl, r, best);
+    });
+    if (c.sm.ne && edyn && c.lp.emass[ei] > 0.0f) {
+        traverse(c.seb, q, stk, c.out + 3, [&](uint32_t sj) {
+            const uint32_t s0i = c.sm.edge[2 * (size_t)sj], s1i = c.sm.edge[2 * (size_t)sj + 1];
+            const f3 w0 = X(c.sm.x, s0i), w1 = X(c.sm.x, s1i);
+            const f3 a[4] = {p0, p1, w0, w1};
+            const f3 b[4] = {q0, q1, w0, w1};
+            bool ovl = false;
+            const float r = ccd<4, 2>(a, b, m.e_off[ei] + c.sm.e_off[sj],
+                                      0.5f * (m.e_ghat[ei] + c.sm.e_ghat[sj]), max_t,
+                                      eps, ovl);
+            note(c, ovl, r, best);
+        });
+    }
+    if (best < seed) {
+        atomicMin(c.out + 1, __float_as_uint(best));
+    }
+}
+
+template <int C, int SPLIT>
+__global__ __launch_bounds__(256) void k_ccd_batch(uint32_t n, const float *__restrict__ x0,
+                                                   const float *__restrict__ x1,
+                                                   const float *__restrict__ offset,
+                                                   const float *__restrict__ ghat,
+                                                   float max_t, float eps,
+                                                   float *__restrict__ toi,
+                                                   int32_t *__restrict__ overlap) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    f3 a[C], b[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        a[k] = load3(x0 + 3 * ((size_t)i * C + k));
+        b[k] = load3(x1 + 3 * ((size_t)i * C + k));
+    }
+    bool ovl = false;
+    toi[i] = ccd<C, SPLIT>(a, b, offset[i], ghat[i], max_t, eps, ovl);
+    if (overlap) {
+        overlap[i] = ovl ? 1 : 0;
+    }
+}
+
+uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+} // namespace
+
+int ccd_batch(int kind, uint32_t n, const float *x0, const float *x1, const float *offset,
+              const float *ghat, float max_t, float eps, float *toi, int32_t *overlap,
+              hipStream_t s) {
+    if (n == 0) {
+        return PPF_OK;
+    }
+    const dim3 g(cdiv(n, 256)), b(256);
+    switch (kind) {
+    case 0:
+        k_ccd_batch<4, 1><<<g, b, 0, s>>>(n, x0, x1, offset, ghat, max_t, eps, toi, overlap);
+        break;
+    case 1:
+        k_ccd_batch<3, 1><<<g, b, 0, s>>>(n, x0, x1, offset, ghat, max_t, eps, toi, overlap);
+        break;
+    case 2:
+        k_ccd_batch<2, 1><<<g, b, 0, s>>>(n, x0, x1, offset, ghat, max_t, eps, toi, overlap);
+        break;
+    default:
+        k_ccd_batch<4, 2><<<g, b, 0, s>>>(n, x0, x1, offset, ghat, max_t, eps, toi, overlap);
+        break;
+    }
+    PPF_HIP(hipGetLastError());
+    return PPF_OK;
+}
+
+LineSearch::~LineSearch() {
+    if (out_) (void)hipFree(out_);
+    if (h_out_) (void)hipHostFree(h_out_);
+}
+
+int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
+                    const ppf_ls_params &lp, bool rebuild, float *toi, int *overlap,
+                    hipStream_t s) {
+    if (!out_) {
+        PPF_HIP(hipMalloc(&out_, 4 * sizeof(uint32_t)));
+        PPF_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_out_), 4 * sizeof(uint32_t),
+                              hipHostMallocDefault));
+    }
+    ppf_static_mesh none{};
+    const ppf_static_mesh &S = sm ? *sm : none;
+    PrimSet f, e, v;
+    f.n = m.nf, f.arity = 3, f.elem = m.face, f.ghat = m.f_ghat, f.offset = m.f_off;
+    e.n = m.ne, e.arity = 2, e.elem = m.edge, e.ghat = m.e_ghat, e.offset = m.e_off;
+    v.n = m.nv, v.arity = 1, v.elem = nullptr, v.ghat = m.v_ghat, v.offset = m.v_off;
+    const bool fresh = rebuild || face_.view().n != m.nf || edge_.view().n != m.ne ||
+                       vert_.view().n != m.nv;
+    int rc;
+    if (fresh) {
+        rc = face_.build(f, m.x, lp.x1, lp.max_t, s);
+        if (rc == PPF_OK) rc = edge_.build(e, m.x, lp.x1, lp.max_t, s);
+        if (rc == PPF_OK) rc = vert_.build(v, m.x, lp.x1, lp.max_t, s);
+    } else {
+        rc = face_.refit(f, m.x, lp.x1, lp.max_t, s);
+        if (rc == PPF_OK) rc = edge_.refit(e, m.x, lp.x1, lp.max_t, s);
+        if (rc == PPF_OK) rc = vert_.refit(v, m.x, lp.x1, lp.max_t, s);
+    }
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    if (S.nv && (rebuild || sface_.view().n != S.nf || sedge_.view().n != S.ne ||
+                 static_x_ != S.x)) {
+        PrimSet sf, se;
+        sf.n = S.nf, sf.arity = 3, sf.elem = S.face, sf.ghat = S.f_ghat, sf.offset = S.f_off;
+        se.n = S.ne, se.arity = 2, se.elem = S.edge, se.ghat = S.e_ghat, se.offset = S.e_off;
+        rc = sface_.build(sf, S.x, nullptr, 0.0f, s);
+        if (rc == PPF_OK) rc = sedge_.build(se, S.x, nullptr, 0.0f, s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        static_x_ = S.x;
+    }
+    const uint32_t init[4] = {__builtin_bit_cast(uint32_t, lp.max_t),
+                              __builtin_bit_cast(uint32_t, lp.max_t), 0u, 0u};
+    PPF_HIP(hipMemcpyAsync(out_, init, sizeof(init), hipMemcpyHostToDevice, s));
+    LsCtx c;
+    c.m = m;
+    c.sm = S;
+    c.lp = lp;
+    c.fb = face_.view();
+    c.eb = edge_.view();
+    c.vb = vert_.view();
+    c.sfb = S.nv ? sface_.view() : BvhView{};
+    c.seb = S.nv ? sedge_.view() : BvhView{};
+    c.out = out_;
+    if (m.nv) {
+        k_ls_vertex<<<cdiv(m.nv, kQT), kQT, 0, s>>>(c);
+        PPF_HIP(hipGetLastError());
+    }
+    if (S.nv && m.nf) {
+        k_ls_static_vertex<<<cdiv(S.nv, kQT), kQT, 0, s>>>(c);
+        PPF_HIP(hipGetLastError());
+    }
+    if (m.ne) {
+        k_ls_edge<<<cdiv(m.ne, kQT), kQT, 0, s>>>(c);
+        PPF_HIP(hipGetLastError());
+    }
+    PPF_HIP(hipMemcpyAsync(h_out_, out_, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipStreamSynchronize(s));
+    if (h_out_[3]) {
+        set_error("line search: a BVH traversal exceeded its %d-entry stack", kStack);
+        return PPF_ERR_CAPACITY;
+    }
+    const float tvf = __builtin_bit_cast(float, h_out_[0]);
+    const float tee = __builtin_bit_cast(float, h_out_[1]);
+    *toi = fminf(tvf, tee) / lp.max_t;
+    *overlap = h_out_[2] ? 1 : 0;
+    return PPF_OK;
+}
+
+} // namespace ppf

@@ -51,6 +51,13 @@ _PROTOS = {
     "ppf_contact_blocks": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.POINTER(ctypes.c_uint64)]),
+    "ppf_ccd_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
+    "ppf_ccd_destroy": (None, [vp]),
+    "ppf_line_search": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int,
+                                       ctypes.POINTER(f32), ctypes.POINTER(ctypes.c_int),
+                                       vp]),
+    "ppf_ccd_batch": (ctypes.c_int, [ctypes.c_int, u32, vp, vp, vp, vp, f32, f32, vp,
+                                     vp, vp]),
 }
 
 _lib = None

@@ -320,3 +320,120 @@ def contact_scenario(res=24, seed=0, slip=2e-5):
             "f": rng.uniform(0.0, 0.5, m["nf"]).astype(np.float32)}
     elastic = face_element_bsr(m["nv"], f, seed=seed + 7, scale=1.0)
     return m, x0, mass, fric, elastic
+
+
+def static_mesh(x, faces, ghat=1e-3, offset=0.0):
+    """Static collision mesh dict (data.constraint.mesh layout): positions,
+    unique edges, faces and uniform per-element contact gap / offset."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+    faces = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+    edge = unique_edges(faces).astype(np.uint32)
+    nv, ne, nf = len(x), len(edge), len(faces)
+    sm = dict(nv=nv, ne=ne, nf=nf, x=x, edge=edge, face=faces)
+    for k, n in (("v", nv), ("e", ne), ("f", nf)):
+        sm[k + "_ghat"] = np.full(n, ghat, np.float32)
+        sm[k + "_off"] = np.full(n, offset, np.float32)
+    return sm
+
+
+def falling_scene(res=24, layers=2, seed=0, drop=3e-3, ground_res=None,
+                  flags=True, offset=5e-5, ground_gap=1e-3):
+    """Line-search scene: `layers` noisy sheets stacked 1.5e-3 apart whose
+    target positions x1 fall by `drop` (upper layers faster, so sheets pass
+    through each other) with per-vertex jitter, above a static ground grid
+    placed `ground_gap` below the lowest sheet. With `flags`, a few vertices are
+    pinned, a few marked collider and one sheet is a rigid body, so every
+    pair filter is exercised. Returns (mesh, x1, vmass, emass, fmass,
+    static mesh)."""
+    rng = np.random.default_rng(seed)
+    x, f = stacked_sheets(res, layers=layers, gap=1.5e-3, noise=1e-4,
+                          shifts=[(0.13 * k, 0.29 * k) for k in range(layers)],
+                          seed=seed + 1)
+    m = contact_mesh(x, f, ghat=1e-3, offset=offset)
+    nv, ne, nf = m["nv"], m["ne"], m["nf"]
+    per = res * res
+    x1 = x.copy()
+    for k in range(layers):
+        x1[k * per:(k + 1) * per, 2] -= drop * (1.0 + 0.6 * k)
+    x1 += rng.normal(0.0, 2e-4, x1.shape).astype(np.float32)
+    if flags:
+        pin = rng.random(nv) < 0.03
+        m["vflag"][pin] |= 1
+        m["vflag"][rng.random(nv) < 0.03] |= 2
+        if layers > 2:
+            m["vbody"][2 * per:3 * per] = 1
+        x1[pin] = x[pin]
+        e = m["edge"]
+        m["eflag"] = ((m["vflag"][e[:, 0]] & 1) & (m["vflag"][e[:, 1]] & 1)).astype(np.uint32)
+        fc = m["face"]
+        m["fflag"] = ((m["vflag"][fc[:, 0]] & 1) & (m["vflag"][fc[:, 1]] & 1)
+                      & (m["vflag"][fc[:, 2]] & 1)).astype(np.uint32)
+    vmass = rng.uniform(1e-5, 1e-4, nv).astype(np.float32)
+    vmass[rng.random(nv) < 0.02] = 0.0
+    emass = rng.uniform(1e-5, 1e-4, ne).astype(np.float32)
+    fmass = rng.uniform(1e-5, 1e-4, nf).astype(np.float32)
+    gr = ground_res or max(4, res // 2)
+    gx, gf = stacked_sheets(gr, layers=1, gap=0.0, noise=0.0, seed=seed + 2,
+                            size=0.12)
+    gx = gx + np.float32([-0.01, -0.01, float(x[:, 2].min()) - ground_gap])
+    sm = static_mesh(gx, gf, ghat=1e-3, offset=offset)
+    return m, x1.astype(np.float32), vmass, emass, fmass, sm
+
+
+def ccd_pairs(kind, n, seed=0, scale=0.01):
+    """n random primitive pairs of one CCD kind (0 point-triangle, 1
+    point-edge, 2 point-point, 3 edge-edge) whose first primitive is driven
+    toward (and often through) the second, so many sweeps end in contact.
+    Returns x0, x1 of shape (n, columns, 3) float32."""
+    cols = {0: 4, 1: 3, 2: 2, 3: 4}[kind]
+    split = 2 if kind == 3 else 1
+    rng = np.random.default_rng(seed)
+    x0 = rng.uniform(-1, 1, (n, cols, 3)) * scale
+    v = rng.uniform(-1, 1, (n, cols, 3)) * (scale * (2e-3 if kind in (1, 2) else 0.4))
+    pull = x0[:, split:].mean(1) - x0[:, :split].mean(1)
+    v[:, :split] += rng.uniform(0.5, 2.0, (n, 1, 1)) * pull[:, None, :]
+    return x0.astype(np.float32), (x0 + v).astype(np.float32)
+
+
+def static_feature_scene(feature, res=8, offset=5e-5):
+    """A flat sheet (rotated 30 degrees about z, so no edge is axis aligned)
+    falling 5e-3 at uniform speed onto one static feature that decides the
+    step through one collision-mesh pair kind:
+      'spike' -- a tetrahedron tip 1e-3 below the sheet inside a face
+                 (static vertex vs dynamic face);
+      'ridge' -- a steep roof whose ridge edge runs 1e-3 below the sheet
+                 across its edges (dynamic edge vs static edge);
+      'floor' -- one large triangle 1e-3 below (dynamic vertex vs static
+                 face).
+    Returns (mesh, x1, vmass, emass, fmass, static mesh)."""
+    x, f = stacked_sheets(res, layers=1, gap=0.0, noise=0.0, seed=0, size=0.1)
+    c, s_ = np.cos(np.pi / 6), np.sin(np.pi / 6)
+    rot = np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1]])
+    x = (x.astype(np.float64) - [0.05, 0.05, 0]) @ rot.T
+    x = x.astype(np.float32)
+    m = contact_mesh(x, f, ghat=1e-3, offset=offset)
+    x1 = (x - np.float32([0, 0, 5e-3])).astype(np.float32)
+    if feature == "spike":
+        # tip under the centroid of a face near the middle
+        fc = x[f].mean(1)
+        k = int(np.argmin(np.linalg.norm(fc[:, :2] - [0.003, 0.002], axis=1)))
+        tip = [fc[k, 0], fc[k, 1], -1e-3]
+        base = [[tip[0] - 0.02, tip[1] - 0.02, -3e-2], [tip[0] + 0.02, tip[1] - 0.02, -3e-2],
+                [tip[0], tip[1] + 0.02, -3e-2]]
+        sx = np.array([tip] + base, np.float32)
+        sf = np.array([[0, 1, 2], [0, 2, 3], [0, 3, 1]], np.uint32)
+    elif feature == "ridge":
+        sx = np.array([[-0.08, 0.0013, -1e-3], [0.08, 0.0013, -1e-3],
+                       [-0.08, -0.0037, -2e-2], [0.08, -0.0037, -2e-2],
+                       [-0.08, 0.0063, -2e-2], [0.08, 0.0063, -2e-2]], np.float32)
+        sf = np.array([[0, 1, 2], [1, 3, 2], [0, 4, 1], [1, 4, 5]], np.uint32)
+    elif feature == "floor":
+        # one large triangle: its edges and vertices are far from the sheet
+        sx = np.array([[-1.0, -1.0, -1e-3], [1.0, -1.0, -1e-3], [0.0, 1.0, -1e-3]],
+                      np.float32)
+        sf = np.array([[0, 1, 2]], np.uint32)
+    else:
+        raise ValueError(feature)
+    sm = static_mesh(sx, sf, ghat=1e-3, offset=offset)
+    ones = lambda n: np.ones(n, np.float32)
+    return m, x1, ones(m["nv"]), ones(m["ne"]), ones(m["nf"]), sm

@@ -1,0 +1,294 @@
+"""CPU oracle: additive CCD and the contact line search.
+
+The reference's accd.hpp is not host-buildable here (it relies on CUDA's
+global device math), so the oracle's ACCD is pinned by mathematics instead:
+  * conservativeness -- no pair comes within its offset over the certified
+    sweep, checked with independent fp64 closest-point distances;
+  * closed forms -- head-on approaches stop just short of the first contact;
+  * zero and separating motion certify the whole sweep (max_t);
+  * an overlapping start returns 0 and raises the overlap flag;
+  * the mesh line search equals the minimum over all pairs (no tree).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ppf_amd import scenes
+
+
+def seg_seg_dist(p1, q1, p2, q2):
+    """Exact fp64 segment-segment distance (Ericson, Real-Time Collision
+    Detection, 5.1.9), degenerate segments included."""
+    d1, d2, r = q1 - p1, q2 - p2, p1 - p2
+    a, e, f = d1 @ d1, d2 @ d2, d2 @ r
+    if a <= 1e-30 and e <= 1e-30:
+        return np.linalg.norm(r)
+    if a <= 1e-30:
+        s, t = 0.0, np.clip(f / e, 0, 1)
+    else:
+        c = d1 @ r
+        if e <= 1e-30:
+            t, s = 0.0, np.clip(-c / a, 0, 1)
+        else:
+            b = d1 @ d2
+            den = a * e - b * b
+            s = np.clip((b * f - c * e) / den, 0, 1) if den > 1e-30 else 0.0
+            t = (b * s + f) / e
+            if t < 0:
+                t, s = 0.0, np.clip(-c / a, 0, 1)
+            elif t > 1:
+                t, s = 1.0, np.clip((b - c) / a, 0, 1)
+    return np.linalg.norm((p1 + d1 * s) - (p2 + d2 * t))
+
+
+def point_tri_dist(p, t0, t1, t2):
+    # Exact fp64 point-triangle distance (plane projection or the edges).
+    n = np.cross(t1 - t0, t2 - t0)
+    nn = n @ n
+    u = np.cross(p - t0, t2 - t0) @ n / nn
+    v = np.cross(t1 - t0, p - t0) @ n / nn
+    if u >= 0 and v >= 0 and u + v <= 1:
+        return abs((p - t0) @ n) / np.sqrt(nn)
+
+    def pe(a, b):
+        t = np.clip((p - a) @ (b - a) / ((b - a) @ (b - a)), 0, 1)
+        return np.linalg.norm(p - (a + t * (b - a)))
+    return min(pe(t0, t1), pe(t1, t2), pe(t2, t0))
+
+
+def min_dist_over(kind, x0, x1, t_end, samples=300):
+    x0 = np.asarray(x0, np.float64)
+    x1 = np.asarray(x1, np.float64)
+    best = np.inf
+    for t in np.linspace(0.0, t_end, samples):
+        x = x0 + t * (x1 - x0)
+        if kind == 0:
+            d = point_tri_dist(x[0], x[1], x[2], x[3])
+        elif kind == 1:
+            d = seg_seg_dist(x[0], x[0], x[1], x[2])
+        elif kind == 2:
+            d = np.linalg.norm(x[0] - x[1])
+        else:
+            d = seg_seg_dist(x[0], x[1], x[2], x[3])
+        best = min(best, d)
+    return best
+
+
+def random_pair(rng, kind):
+    """Random primitives whose first one is driven toward (and often through)
+    the second, so many sweeps end in contact."""
+    cols = {0: 4, 1: 3, 2: 2, 3: 4}[kind]
+    split = 2 if kind == 3 else 1
+    x0 = rng.uniform(-1, 1, (cols, 3)) * 0.01
+    v = rng.uniform(-1, 1, (cols, 3)) * (2e-5 if kind == 2 else 0.004)
+    pull = x0[split:].mean(0) - x0[:split].mean(0)
+    v[:split] += rng.uniform(0.5, 2.0) * pull
+    return x0.astype(np.float32), (x0 + v).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_ccd_is_conservative(kind):
+    rng = np.random.default_rng(kind)
+    offset, ghat = 1e-4, 1e-3
+    hits = 0
+    for _ in range(150):
+        x0, x1 = random_pair(rng, kind)
+        if min_dist_over(kind, x0, x0, 0.0, 1) <= offset:
+            continue
+        t, ov = oracle.ccd(kind, x0, x1, offset, ghat)
+        assert 0.0 <= t <= 1.25
+        assert not ov
+        if t < 1.25:
+            hits += 1
+        # Nothing closer than the offset over the certified sweep.
+        assert min_dist_over(kind, x0, x1, t) > offset * (1 - 1e-3)
+    assert hits > 5
+
+
+def test_ccd_head_on_closed_form():
+    # A point falling straight onto a triangle's interior from height D with
+    # displacement V per unit step touches the offset at t* = (D - off) / V.
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float64) * 0.01
+    offset, ghat = 2e-4, 1e-3
+    for D, V in ((5e-3, 8e-3), (2e-3, 1e-2), (1e-3, 4e-3)):
+        x0 = np.vstack([[0.003, 0.003, D], tri]).astype(np.float32)
+        x1 = x0.copy()
+        x1[0, 2] -= V
+        t, ov = oracle.ccd(0, x0, x1, offset, ghat)
+        tstar = (D - offset) / V
+        assert t < tstar
+        # It parks within a small multiple of the clearance floor.
+        gap_at_t = D - t * V - offset
+        assert 0 < gap_at_t <= 3e-2 * ghat + 1e-6, (t, tstar, gap_at_t)
+
+
+def test_ccd_trivial_motions_and_overlap():
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32) * 0.01
+    x0 = np.vstack([[0.003, 0.003, 2e-3], tri]).astype(np.float32)
+    # zero relative motion: whole sweep
+    t, _ = oracle.ccd(0, x0, x0 + np.float32(0.5), 1e-4, 1e-3)
+    assert t == 1.25
+    # separating motion: whole sweep
+    x1 = x0.copy()
+    x1[0, 2] += 1e-2
+    t, _ = oracle.ccd(0, x0, x1, 1e-4, 1e-3)
+    assert t == 1.25
+    # starting inside the offset: 0 and the overlap flag
+    t, ov = oracle.ccd(0, x0, x1, 5e-3, 1e-3)
+    assert t == 0.0 and ov
+
+
+def test_line_search_equals_all_pairs_minimum():
+    x, f = scenes.stacked_sheets(8, layers=2, gap=1.5e-3, noise=1e-4, seed=4)
+    m = scenes.contact_mesh(x, f, ghat=1e-3, offset=5e-5)
+    rng = np.random.default_rng(5)
+    x1 = x.copy()
+    x1[len(x) // 2:, 2] -= 2.5e-3                 # upper sheet falls through
+    x1 += rng.normal(0, 2e-4, x1.shape).astype(np.float32)
+    nv, ne, nf = m["nv"], m["ne"], m["nf"]
+    ones = lambda n: np.ones(n, np.float32)
+    t, ov = oracle.line_search(m, x1, ones(nv), ones(ne), ones(nf))
+    assert not ov and 0.0 < t < 1.0
+    # Brute force over every pair with the single-pair routine.
+    best = 1.25
+    X0, X1 = x.astype(np.float32), x1.astype(np.float32)
+    E, F = m["edge"], m["face"]
+    off = 5e-5
+    for i in range(nv):
+        for fi in range(nf):
+            fv = F[fi]
+            if i in fv:
+                k = int(np.nonzero(fv == i)[0][0])
+                j, l = fv[(k + 1) % 3], fv[(k + 2) % 3]
+                r, _ = oracle.ccd(1, X0[[i, j, l]], X1[[i, j, l]], 2 * off, 1e-3)
+            else:
+                r, _ = oracle.ccd(0, X0[[i, *fv]], X1[[i, *fv]], 2 * off, 1e-3)
+            best = min(best, r)
+        for j in range(i):
+            r, _ = oracle.ccd(2, X0[[i, j]], X1[[i, j]], 2 * off, 1e-3)
+            best = min(best, r)
+    for a in range(ne):
+        for b in range(a + 1, ne):
+            ea, eb = E[a], E[b]
+            shared = set(ea.tolist()) & set(eb.tolist())
+            if not shared:
+                r, _ = oracle.ccd(3, X0[[*ea, *eb]], X1[[*ea, *eb]], 2 * off, 1e-3)
+            else:
+                s = shared.pop()
+                i1 = ea[0] if ea[1] == s else ea[1]
+                i2 = eb[0] if eb[1] == s else eb[1]
+                r0, _ = oracle.ccd(1, X0[[i1, s, i2]], X1[[i1, s, i2]], 2 * off, 1e-3)
+                r1, _ = oracle.ccd(1, X0[[i2, s, i1]], X1[[i2, s, i1]], 2 * off, 1e-3)
+                r = min(r0, r1)
+            best = min(best, r)
+    assert t == np.float32(best) / np.float32(1.25)
+
+
+def _brute_dynamic(X0, X1, m, off, gh, best):
+    E, F = m["edge"], m["face"]
+    for i in range(m["nv"]):
+        for fi in range(m["nf"]):
+            fv = F[fi]
+            if i in fv:
+                k = int(np.nonzero(fv == i)[0][0])
+                j, l = fv[(k + 1) % 3], fv[(k + 2) % 3]
+                r, _ = oracle.ccd(1, X0[[i, j, l]], X1[[i, j, l]], off, gh)
+            else:
+                r, _ = oracle.ccd(0, X0[[i, *fv]], X1[[i, *fv]], off, gh)
+            best = min(best, r)
+        for j in range(i):
+            r, _ = oracle.ccd(2, X0[[i, j]], X1[[i, j]], off, gh)
+            best = min(best, r)
+    for a in range(m["ne"]):
+        for b in range(a + 1, m["ne"]):
+            ea, eb = E[a], E[b]
+            shared = set(ea.tolist()) & set(eb.tolist())
+            if not shared:
+                r, _ = oracle.ccd(3, X0[[*ea, *eb]], X1[[*ea, *eb]], off, gh)
+            else:
+                s = shared.pop()
+                i1 = ea[0] if ea[1] == s else ea[1]
+                i2 = eb[0] if eb[1] == s else eb[1]
+                r0, _ = oracle.ccd(1, X0[[i1, s, i2]], X1[[i1, s, i2]], off, gh)
+                r1, _ = oracle.ccd(1, X0[[i2, s, i1]], X1[[i2, s, i1]], off, gh)
+                r = min(r0, r1)
+            best = min(best, r)
+    return best
+
+
+def test_line_search_static_mesh_equals_all_pairs_minimum():
+    # One sheet falling through a static ground: the collision-mesh sweeps
+    # (vertex vs static face, static vertex vs face, edge vs static edge)
+    # decide the step; the result is the minimum over every pair.
+    m, x1, _, _, _, sm = scenes.falling_scene(res=5, layers=1, seed=3, drop=4e-3,
+                                              ground_res=4, flags=False)
+    nv, ne, nf = m["nv"], m["ne"], m["nf"]
+    ones = lambda n: np.ones(n, np.float32)
+    t, ov = oracle.line_search(m, x1, ones(nv), ones(ne), ones(nf), static=sm)
+    assert not ov and 0.0 < t < 1.0
+    X0, X1 = m["x"], x1
+    off, gh = 2 * 5e-5, 1e-3
+    best = _brute_dynamic(X0, X1, m, off, gh, 1.25)
+    S = sm["x"]
+    for i in range(nv):
+        for fv in sm["face"]:
+            a = np.vstack([X0[i], S[fv]])
+            b = np.vstack([X1[i], S[fv]])
+            best = min(best, oracle.ccd(0, a, b, off, gh)[0])
+    for sv in S:
+        for fv in m["face"]:
+            a = np.vstack([sv, X0[fv]])
+            b = np.vstack([sv, X1[fv]])
+            best = min(best, oracle.ccd(0, a, b, off, gh)[0])
+    for ea in m["edge"]:
+        for se in sm["edge"]:
+            a = np.vstack([X0[ea], S[se]])
+            b = np.vstack([X1[ea], S[se]])
+            best = min(best, oracle.ccd(3, a, b, off, gh)[0])
+    assert t == np.float32(best) / np.float32(1.25)
+    # Without the ground the step goes further.
+    t_free, _ = oracle.line_search(m, x1, ones(nv), ones(ne), ones(nf))
+    assert t_free > t
+
+
+def test_line_search_pair_filters():
+    # Two sheets swapping places collide; each filter alone removes every
+    # pair between them (same rigid body, both colliders, both fixed), and
+    # massless elements are skipped against the collision mesh.
+    x, f = scenes.stacked_sheets(6, layers=2, gap=1.5e-3, noise=1e-4, seed=9)
+    base = scenes.contact_mesh(x, f, ghat=1e-3, offset=5e-5)
+    x1 = x.copy()
+    half = len(x) // 2
+    x1[:half, 2] += 3e-3
+    x1[half:, 2] -= 3e-3
+    nv, ne, nf = base["nv"], base["ne"], base["nf"]
+    ones = lambda n: np.ones(n, np.float32)
+    t, _ = oracle.line_search(base, x1, ones(nv), ones(ne), ones(nf))
+    assert t < 0.6
+
+    def clear(**kw):
+        m = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in base.items()}
+        for k, v in kw.items():
+            m[k] = v
+        # Within one sheet nothing moves relative to itself, so the step is
+        # only limited by pairs across the sheets.
+        return oracle.line_search(m, x1, ones(nv), ones(ne), ones(nf))[0]
+
+    assert clear(vbody=np.full(nv, 7, np.uint32)) == 1.0
+    assert clear(vflag=np.full(nv, 2, np.uint32)) == 1.0
+    assert clear(vflag=np.full(nv, 1, np.uint32), eflag=np.ones(ne, np.uint32),
+                 fflag=np.ones(nf, np.uint32)) == 1.0
+    # One sheet fixed: the other still collides with it.
+    vf = np.zeros(nv, np.uint32)
+    vf[:half] = 1
+    assert clear(vflag=vf) < 1.0
+    # Massless vertices do not sweep against the collision mesh.
+    m, x1g, _, _, _, sm = scenes.falling_scene(res=5, layers=1, seed=3, drop=4e-3,
+                                               ground_res=4, flags=False)
+    n1, e1, f1 = m["nv"], m["ne"], m["nf"]
+    t_all, _ = oracle.line_search(m, x1g, ones(n1), ones(e1), ones(f1), static=sm)
+    t_none, _ = oracle.line_search(m, x1g, np.zeros(n1, np.float32),
+                                   np.zeros(e1, np.float32), np.zeros(f1, np.float32),
+                                   static=sm)
+    t_free, _ = oracle.line_search(m, x1g, ones(n1), ones(e1), ones(f1))
+    assert t_all < t_none == t_free
