@@ -158,6 +158,31 @@ int ppf_narrow_proximity(ppf_narrow *h, const ppf_contact_mesh *m,
                          ppf_pair *d_out, uint64_t cap, uint64_t *count,
                          void *stream);
 
+/* One intersecting pair (IntersectionRecord, data.hpp:741-748): type 0
+ * face-edge, 1 edge-edge, 2 collision-mesh face vs edge, 3 point-point;
+ * elem0 / elem1 as the reference records them; positions packs elem0's
+ * vertices then elem1's. */
+typedef struct ppf_isect_record {
+    uint32_t type, elem0, elem1, num_verts0, num_verts1;
+    float positions[15];
+} ppf_isect_record;
+
+struct ppf_static_mesh;
+
+/* Post-step intersection check at m->x (replaces contact::check_intersection,
+ * contact/contact.cu:3056-3175): edges crossing faces or the collision mesh
+ * (sm may be null), edge pairs and surface-vertex pairs (the first
+ * surface_vert_count vertices) closer than their summed offsets. Element
+ * masses are VertexProp / EdgeProp / FaceProp mass. *count receives the number
+ * of intersecting pairs (0 = clear); the first min(*count, cap) records are
+ * copied to the host array `records` (may be null). Needs ppf_narrow_build at
+ * the same positions first. */
+int ppf_intersection_check(ppf_narrow *h, const ppf_contact_mesh *m,
+                           const struct ppf_static_mesh *sm, const float *d_vmass,
+                           const float *d_emass, const float *d_fmass,
+                           uint32_t surface_vert_count, ppf_isect_record *records,
+                           uint32_t cap, uint32_t *count, void *stream);
+
 /* ------------------------------------------------------------------------
  * Contact barrier + friction force and Hessian assembly.
  * ---------------------------------------------------------------------- */

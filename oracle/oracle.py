@@ -323,3 +323,34 @@ def line_search(m, x1, vmass, emass, fmass, static=None, max_t=1.25, ccd_eps=1e-
     t = lib.oracle_line_search(ctypes.byref(s), ctypes.byref(ss), ctypes.byref(p),
                                ctypes.byref(ov))
     return float(t), bool(ov.value)
+
+
+# ---- intersection check -----------------------------------------------------
+
+ISECT_DTYPE = np.dtype([("type", np.uint32), ("elem0", np.uint32), ("elem1", np.uint32),
+                        ("num_verts0", np.uint32), ("num_verts1", np.uint32),
+                        ("positions", np.float32, (15,))])
+
+
+def _load_isect():
+    lib = load()
+    if not hasattr(lib, "_isect_ready"):
+        lib.oracle_intersection_check.argtypes = [ctypes.POINTER(ContactMesh),
+                                                  ctypes.POINTER(StaticMesh), vp, vp, vp,
+                                                  u32, vp, u32]
+        lib.oracle_intersection_check.restype = u32
+        lib._isect_ready = True
+    return lib
+
+
+def intersection_check(m, vmass, emass, fmass, static=None, surface=None, cap=256):
+    """contact::check_intersection at m['x']: (count, records[:min(count, cap)])."""
+    lib = _load_isect()
+    s, keep = _mesh_struct(m)
+    ss, keep2 = static_mesh_struct(static)
+    vm, em, fm = _f32(vmass), _f32(emass), _f32(fmass)
+    rec = np.zeros(max(1, cap), ISECT_DTYPE)
+    nsurf = int(m["nv"]) if surface is None else int(surface)
+    n = lib.oracle_intersection_check(ctypes.byref(s), ctypes.byref(ss), _p(vm), _p(em),
+                                      _p(fm), nsurf, rec.ctypes.data_as(vp), cap)
+    return int(n), rec[:min(int(n), cap)]

@@ -1,0 +1,96 @@
+// bvh_query.hpp -- box queries over an LBVH (device).
+//
+// aabb::overlap / aabb::query of contact/aabb.hpp:70-107: depth-first from the
+// root, inactive boxes never overlap. One thread per query; the per-thread
+// stack lives in LDS, strided by the workgroup size so a wave's stack slots
+// fall in distinct banks. Children are tested before they are pushed, so the
+// stack only ever holds overlapping nodes. A query that would push past the
+// stack flags `overflow` and stops (the caller reports a capacity error).
+#pragma once
+#include "lbvh.hpp"
+
+namespace ppf {
+
+constexpr int kQueryThreads = 128; // threads per query workgroup
+constexpr int kQueryStack = 64;    // stack entries per thread
+
+struct QBox {
+    float mn[3], mx[3];
+};
+
+__device__ inline bool overlaps(const Aabb &a, const QBox &q) {
+    return a.active != 0.0f && (a.mn[0] <= q.mx[0] && a.mx[0] >= q.mn[0]) &&
+           (a.mn[1] <= q.mx[1] && a.mx[1] >= q.mn[1]) &&
+           (a.mn[2] <= q.mx[2] && a.mx[2] >= q.mn[2]);
+}
+
+__device__ inline Aabb load_box(const Aabb *b, uint32_t i) {
+    const float4 *p = reinterpret_cast<const float4 *>(b + i);
+    const float4 lo = p[0];
+    const float4 hi = p[1];
+    Aabb r;
+    r.mn[0] = lo.x, r.mn[1] = lo.y, r.mn[2] = lo.z, r.active = lo.w;
+    r.mx[0] = hi.x, r.mx[1] = hi.y, r.mx[2] = hi.z, r.pad = hi.w;
+    return r;
+}
+
+// Box of the points a, b (min / max), then -/+ margin when it is non-zero
+// (aabb::make, contact/aabb.hpp:45-60).
+__device__ inline QBox seg_box(f3 a, f3 b, float margin) {
+    QBox q{{fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)},
+           {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}};
+    if (margin != 0.0f) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            q.mn[d] -= margin;
+            q.mx[d] += margin;
+        }
+    }
+    return q;
+}
+
+// Point +/- margin (aabb::make, contact/aabb.hpp:62-69).
+__device__ inline QBox point_box(f3 p, float margin) {
+    return QBox{{p.x - margin, p.y - margin, p.z - margin},
+                {p.x + margin, p.y + margin, p.z + margin}};
+}
+
+// Visits every leaf whose box overlaps q; `leaf(prim)` returns whether the
+// candidate was accepted. Returns the accepted count. `stk` is this thread's
+// first LDS slot (stride kQueryThreads).
+template <class F>
+__device__ uint32_t traverse(const BvhView &bvh, const QBox &q, uint32_t *stk,
+                             uint32_t *overflow, F &&leaf) {
+    if (bvh.n == 0 || !overlaps(load_box(bvh.box, bvh.root), q)) {
+        return 0;
+    }
+    uint32_t count = 0;
+    uint32_t sp = 0;
+    stk[(sp++) * kQueryThreads] = bvh.root;
+    while (sp) {
+        const uint32_t v = stk[(--sp) * kQueryThreads];
+        const uint32_t c1 = bvh.node[2 * (size_t)v + 1];
+        if (c1 == 0) {
+            if (leaf(bvh.node[2 * (size_t)v] - 1)) {
+                ++count;
+            }
+            continue;
+        }
+        const uint32_t l = bvh.node[2 * (size_t)v] - 1, r = c1 - 1;
+        const bool ol = overlaps(load_box(bvh.box, l), q);
+        const bool orr = overlaps(load_box(bvh.box, r), q);
+        if (sp + 2 > (uint32_t)kQueryStack) {
+            atomicOr(overflow, 1u);
+            return count;
+        }
+        if (ol) {
+            stk[(sp++) * kQueryThreads] = l;
+        }
+        if (orr) {
+            stk[(sp++) * kQueryThreads] = r;
+        }
+    }
+    return count;
+}
+
+} // namespace ppf

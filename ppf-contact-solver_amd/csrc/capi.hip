@@ -3,6 +3,7 @@
 // Plain pointers and sizes only; no torch or C++ types cross the boundary.
 #include "common.hpp"
 #include "eig3.hpp"
+#include "isect.hpp"
 #include "assemble.hpp"
 #include "linesearch.hpp"
 #include "narrow.hpp"
@@ -76,6 +77,7 @@ struct ppf_pcg_plan {
 
 struct ppf_narrow {
     Narrow impl;
+    IntersectionCheck isect;
 };
 
 struct ppf_contact {
@@ -265,6 +267,29 @@ int ppf_narrow_proximity(ppf_narrow *h, const ppf_contact_mesh *m,
         return PPF_ERR_ARG;
     }
     return h->impl.proximity(*m, d_out, cap, count, (hipStream_t)stream);
+}
+
+int ppf_intersection_check(ppf_narrow *h, const ppf_contact_mesh *m,
+                           const struct ppf_static_mesh *sm, const float *d_vmass,
+                           const float *d_emass, const float *d_fmass,
+                           uint32_t surface_vert_count, ppf_isect_record *records,
+                           uint32_t cap, uint32_t *count, void *stream) {
+    if (!h || !m || !m->x || !count || !d_vmass || !d_emass || !d_fmass) {
+        set_error("ppf_intersection_check: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    if (h->impl.tree(0).view().n != m->nf || h->impl.tree(1).view().n != m->ne ||
+        h->impl.tree(2).view().n != m->nv) {
+        set_error("ppf_intersection_check: trees were built for another mesh");
+        return PPF_ERR_ARG;
+    }
+    if (sm && sm->nv && (!sm->x || (sm->nf && !sm->face))) {
+        set_error("ppf_intersection_check: static mesh arrays missing");
+        return PPF_ERR_ARG;
+    }
+    return h->isect.run(h->impl.tree(0).view(), h->impl.tree(1).view(),
+                        h->impl.tree(2).view(), *m, sm, d_vmass, d_emass, d_fmass,
+                        surface_vert_count, records, cap, count, (hipStream_t)stream);
 }
 
 int ppf_contact_create(ppf_contact **out) {

@@ -9,6 +9,7 @@
 // Boxes are the reference's swept boxes; the minima are combined with
 // integer atomics on the float bits (all values are non-negative), which is
 // exact and order independent.
+#include "bvh_query.hpp"
 #include "ccd.hpp"
 #include "linesearch.hpp"
 
@@ -16,58 +17,9 @@ namespace ppf {
 
 namespace {
 
-constexpr int kQT = 128;
-constexpr int kStack = 64;
-
-struct Box6 {
-    float mn[3], mx[3];
-};
-
-__device__ inline bool overlaps(const Aabb &a, const Box6 &q) {
-    return a.active != 0.0f && (a.mn[0] <= q.mx[0] && a.mx[0] >= q.mn[0]) &&
-           (a.mn[1] <= q.mx[1] && a.mx[1] >= q.mn[1]) &&
-           (a.mn[2] <= q.mx[2] && a.mx[2] >= q.mn[2]);
-}
-
-__device__ inline Aabb load_box(const Aabb *b, uint32_t i) {
-    const float4 *p = reinterpret_cast<const float4 *>(b + i);
-    const float4 lo = p[0], hi = p[1];
-    Aabb r;
-    r.mn[0] = lo.x, r.mn[1] = lo.y, r.mn[2] = lo.z, r.active = lo.w;
-    r.mx[0] = hi.x, r.mx[1] = hi.y, r.mx[2] = hi.z, r.pad = hi.w;
-    return r;
-}
-
-template <class F>
-__device__ void traverse(const BvhView &bvh, const Box6 &q, uint32_t *stk,
-                         uint32_t *overflow, F &&leaf) {
-    if (bvh.n == 0 || !overlaps(load_box(bvh.box, bvh.root), q)) {
-        return;
-    }
-    uint32_t sp = 0;
-    stk[(sp++) * kQT] = bvh.root;
-    while (sp) {
-        const uint32_t v = stk[(--sp) * kQT];
-        const uint32_t c1 = bvh.node[2 * (size_t)v + 1];
-        if (c1 == 0) {
-            leaf(bvh.node[2 * (size_t)v] - 1);
-            continue;
-        }
-        const uint32_t l = bvh.node[2 * (size_t)v] - 1, r = c1 - 1;
-        const bool ol = overlaps(load_box(bvh.box, l), q);
-        const bool orr = overlaps(load_box(bvh.box, r), q);
-        if (sp + 2 > (uint32_t)kStack) {
-            atomicOr(overflow, 1u);
-            return;
-        }
-        if (ol) {
-            stk[(sp++) * kQT] = l;
-        }
-        if (orr) {
-            stk[(sp++) * kQT] = r;
-        }
-    }
-}
+constexpr int kQT = kQueryThreads;
+constexpr int kStack = kQueryStack;
+using Box6 = QBox;
 
 struct LsCtx {
     ppf_contact_mesh m;
@@ -83,19 +35,6 @@ __device__ inline f3 XS(const LsCtx &c, uint32_t v) {
     const f3 a = X(c.m.x, v), b = X(c.lp.x1, v);
     const float t = c.lp.max_t;
     return mk3(t * (b.x - a.x) + a.x, t * (b.y - a.y) + a.y, t * (b.z - a.z) + a.z);
-}
-
-__device__ inline Box6 seg_box(f3 a, f3 b, float margin) {
-    Box6 q{{fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)},
-           {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}};
-    if (margin != 0.0f) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            q.mn[d] -= margin;
-            q.mx[d] += margin;
-        }
-    }
-    return q;
 }
 
 __device__ inline void note(const LsCtx &c, bool ovl, float r, float &best) {
@@ -128,7 +67,7 @@ __global__ __launch_bounds__(kQT) void k_ls_vertex(LsCtx c) {
         const bool same = m.vbody[i] != 0 && m.vbody[i] == m.vbody[f[0]];
         const bool both = icol && (m.vflag[f[0]] & 2u) != 0;
         if (!either || same || both) {
-            return;
+            return false;
         }
         int dup = -1;
         for (int k = 0; k < 3; ++k) {
@@ -152,6 +91,7 @@ __global__ __launch_bounds__(kQT) void k_ls_vertex(LsCtx c) {
             r = ccd<3, 1>(a, b, 2.0f * m.v_off[i], m.v_ghat[i], max_t, eps, ovl);
         }
         note(c, ovl, r, best);
+        return true;
     });
     // point-point (contact.cu:2266-2316)
     traverse(c.vb, q, stk, c.out + 3, [&](uint32_t j) {
@@ -159,7 +99,7 @@ __global__ __launch_bounds__(kQT) void k_ls_vertex(LsCtx c) {
         const bool same = m.vbody[i] != 0 && m.vbody[i] == m.vbody[j];
         const bool both = icol && (m.vflag[j] & 2u) != 0;
         if (!(j < i) || !either || same || both) {
-            return;
+            return false;
         }
         const f3 a[2] = {X(m.x, i), X(m.x, j)};
         const f3 b[2] = {X(c.lp.x1, i), X(c.lp.x1, j)};
@@ -167,12 +107,13 @@ __global__ __launch_bounds__(kQT) void k_ls_vertex(LsCtx c) {
         const float r = ccd<2, 1>(a, b, m.v_off[i] + m.v_off[j],
                                   0.5f * (m.v_ghat[i] + m.v_ghat[j]), max_t, eps, ovl);
         note(c, ovl, r, best);
+        return true;
     });
     // dynamic vertex vs static face (contact.cu:1956-1994)
     if (c.sm.nf && idyn) {
         traverse(c.sfb, q, stk, c.out + 3, [&](uint32_t fi) {
             if (c.lp.vmass[i] == 0.0f) {
-                return;
+                return false;
             }
             const uint32_t f0 = c.sm.face[3 * (size_t)fi], f1 = c.sm.face[3 * (size_t)fi + 1],
                            f2 = c.sm.face[3 * (size_t)fi + 2];
@@ -184,6 +125,7 @@ __global__ __launch_bounds__(kQT) void k_ls_vertex(LsCtx c) {
                                       0.5f * (m.v_ghat[i] + c.sm.f_ghat[fi]), max_t,
                                       eps, ovl);
             note(c, ovl, r, best);
+            return true;
         });
     }
     if (best < max_t) {
@@ -207,7 +149,7 @@ __global__ __launch_bounds__(kQT) void k_ls_static_vertex(LsCtx c) {
     float best = max_t;
     traverse(c.fb, q, stk, c.out + 3, [&](uint32_t fi) {
         if ((m.fflag[fi] & 1u) != 0 || !(c.lp.fmass[fi] > 0.0f)) {
-            return;
+            return false;
         }
         const uint32_t f0 = m.face[3 * (size_t)fi], f1 = m.face[3 * (size_t)fi + 1],
                        f2 = m.face[3 * (size_t)fi + 2];
@@ -218,6 +160,7 @@ __global__ __launch_bounds__(kQT) void k_ls_static_vertex(LsCtx c) {
                                   0.5f * (m.f_ghat[fi] + c.sm.v_ghat[i]), max_t, eps,
                                   ovl);
         note(c, ovl, r, best);
+        return true;
     });
     if (best < max_t) {
         atomicMin(c.out + 0, __float_as_uint(best));
@@ -259,7 +202,7 @@ __global__ __launch_bounds__(kQT) void k_ls_edge(LsCtx c) {
         const bool same = m.vbody[a0] != 0 && m.vbody[a0] == m.vbody[b0i];
         const bool both = acol && (m.vflag[b0i] & 2u) != 0;
         if (!(ei < ej) || !either || same || both) {
-            return;
+            return false;
         }
         const float off = m.e_off[ei] + m.e_off[ej];
         const float gh = 0.5f * (m.e_ghat[ei] + m.e_ghat[ej]);
@@ -289,6 +232,7 @@ __global__ __launch_bounds__(kQT) void k_ls_edge(LsCtx c) {
             }
         }
         note(c, ovl, r, best);
+        return true;
     });
     if (c.sm.ne && edyn && c.lp.emass[ei] > 0.0f) {
         traverse(c.seb, q, stk, c.out + 3, [&](uint32_t sj) {
@@ -301,6 +245,7 @@ __global__ __launch_bounds__(kQT) void k_ls_edge(LsCtx c) {
                                       0.5f * (m.e_ghat[ei] + c.sm.e_ghat[sj]), max_t,
                                       eps, ovl);
             note(c, ovl, r, best);
+            return true;
         });
     }
     if (best < seed) {

@@ -8,6 +8,7 @@
 // in BVH leaf (Morton) order, one thread each, with the traversal stack in LDS.
 // Children are tested before they are pushed. Two passes -- count, exclusive
 // scan, fill -- give every query a contiguous, deterministic output range.
+#include "bvh_query.hpp"
 #include "distance.hpp"
 #include "narrow.hpp"
 #include "scan.hpp"
@@ -16,68 +17,9 @@ namespace ppf {
 
 namespace {
 
-constexpr int kQT = 128;     // threads per query workgroup
-constexpr int kStack = 64;   // per-thread traversal stack (LDS)
-
-struct Q {
-    float mn[3], mx[3];
-};
-
-__device__ inline bool overlaps(const Aabb &a, const Q &q) {
-    return a.active != 0.0f && (a.mn[0] <= q.mx[0] && a.mx[0] >= q.mn[0]) &&
-           (a.mn[1] <= q.mx[1] && a.mx[1] >= q.mn[1]) &&
-           (a.mn[2] <= q.mx[2] && a.mx[2] >= q.mn[2]);
-}
-
-__device__ inline Aabb load_box(const Aabb *b, uint32_t i) {
-    const float4 *p = reinterpret_cast<const float4 *>(b + i);
-    const float4 lo = p[0];
-    const float4 hi = p[1];
-    Aabb r;
-    r.mn[0] = lo.x, r.mn[1] = lo.y, r.mn[2] = lo.z, r.active = lo.w;
-    r.mx[0] = hi.x, r.mx[1] = hi.y, r.mx[2] = hi.z, r.pad = hi.w;
-    return r;
-}
-
-// Depth-first traversal; `leaf(prim)` returns whether the pair was accepted.
-// Returns the accepted count, or flags an overflow of the fixed stack.
-template <class F>
-__device__ uint32_t traverse(const BvhView &bvh, const Q &q, uint32_t *stk,
-                             uint32_t *overflow, F &&leaf) {
-    if (bvh.n == 0) {
-        return 0;
-    }
-    uint32_t count = 0;
-    if (!overlaps(load_box(bvh.box, bvh.root), q)) {
-        return 0;
-    }
-    uint32_t sp = 0;
-    stk[(sp++) * kQT] = bvh.root;
-    while (sp) {
-        const uint32_t v = stk[(--sp) * kQT];
-        const uint32_t c1 = bvh.node[2 * (size_t)v + 1];
-        if (c1 == 0) {
-            if (leaf(bvh.node[2 * (size_t)v] - 1)) {
-                ++count;
-            }
-            continue;
-        }
-        const uint32_t l = bvh.node[2 * (size_t)v] - 1, r = c1 - 1;
-        const bool ol = overlaps(load_box(bvh.box, l), q);
-        const bool orr = overlaps(load_box(bvh.box, r), q);
-        if (sp + 2 > (uint32_t)kStack) {
-            atomicOr(overflow, 1u);
-            return count;
-        }
-        if (ol) {
-            stk[(sp++) * kQT] = l;
-        }
-        if (orr) {
-            stk[(sp++) * kQT] = r;
-        }
-    }
-    return count;
-}
+constexpr int kQT = kQueryThreads;
+constexpr int kStack = kQueryStack;
+using Q = QBox;
 
 struct Ctx {
     ppf_contact_mesh m;

@@ -51,6 +51,8 @@ _PROTOS = {
     "ppf_contact_blocks": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.POINTER(ctypes.c_uint64)]),
+    "ppf_intersection_check": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, u32,
+                                              ctypes.POINTER(u32), vp]),
     "ppf_ccd_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
     "ppf_ccd_destroy": (None, [vp]),
     "ppf_line_search": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int,

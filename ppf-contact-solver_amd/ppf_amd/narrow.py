@@ -20,6 +20,10 @@ INDEX_FIELDS = ("edge", "face", "v2e_off", "v2e", "v2f_off", "v2f", "e2f_off",
                 "e2f", "vflag", "vbody", "eflag", "fflag")
 PAIR_DTYPE = np.dtype([("kind", np.uint32), ("a", np.uint32), ("b", np.uint32),
                        ("count", np.uint32)])
+# ppf_isect_record (include/ppf_mi355x.h)
+ISECT_DTYPE = np.dtype([("type", np.uint32), ("elem0", np.uint32), ("elem1", np.uint32),
+                        ("num_verts0", np.uint32), ("num_verts1", np.uint32),
+                        ("positions", np.float32, (15,))])
 
 
 class ContactMeshC(ctypes.Structure):
@@ -120,6 +124,26 @@ class NarrowPhase:
         if c.value > cap:
             raise _lib.PPFError(f"pair buffer too small: {c.value} > {cap}")
         return out[:c.value]
+
+    def intersection_check(self, t, vmass, emass, fmass, static=None, surface=None,
+                           cap=256):
+        """Intersection check at t['x'] (build() at the same positions first):
+        (count of intersecting pairs, records[:min(count, cap)])."""
+        from .ccd import static_struct
+        s = mesh_struct(t)
+        ss = static_struct(static)
+        for name, ten in (("vmass", vmass), ("emass", emass), ("fmass", fmass)):
+            if not ten.is_cuda or not ten.is_contiguous() or ten.dtype != torch.float32:
+                raise ValueError(f"{name}: expected a contiguous float32 cuda tensor")
+        rec = np.zeros(max(1, cap), ISECT_DTYPE)
+        n = u32()
+        nsurf = t["nv"] if surface is None else int(surface)
+        check(self.lib.ppf_intersection_check(
+            self._h, ctypes.byref(s), ctypes.byref(ss), vp(vmass.data_ptr()),
+            vp(emass.data_ptr()), vp(fmass.data_ptr()), nsurf,
+            rec.ctypes.data_as(vp), cap, ctypes.byref(n), _stream()),
+            "ppf_intersection_check")
+        return n.value, rec[:min(n.value, cap)]
 
 
 def pairs_to_numpy(t):
