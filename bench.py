@@ -135,7 +135,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_spmv<true>",
+                     "traffic": traffic,
+                     "kernel": "pcg spmv (k_spmv_ell<true,3> on this grid)",
                      "kernel_ms": round(spmv_ms, 5),
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": None,
