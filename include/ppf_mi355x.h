@@ -275,6 +275,61 @@ int ppf_ccd_batch(int kind, uint32_t n, const float *d_x0, const float *d_x1,
                   const float *d_offset, const float *d_ghat, float max_t,
                   float ccd_eps, float *d_toi, int32_t *d_overlap, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Inertia, air drag, membrane and bending (elastic) assembly.
+ * ---------------------------------------------------------------------- */
+
+/* A shell (all device pointers). Positions: x = the Newton iterate (eval_x),
+ * x0 = vertex.curr, target = the inertial target y. Faces and hinges as
+ * data.mesh.mesh (hinge = 4 vertices in stored order; the bending term works
+ * on (h2, h1, h0, h3), dihedral_angle.hpp:15-17). v2f_off / v2f: vertex->face
+ * CSR in the order the vertex normal sums (data.mesh.neighbor.vertex.face).
+ * Per vertex: VertexProp mass and area, vflag bit0 = pinned (fix_index != 0).
+ * Per face: FaceProp mass, inv_rest2x2 (4 floats, column-major), FaceParam mu
+ * and lambda, fflag bit0 = excluded (fixed, collider or rest-excluded). Per
+ * hinge: HingeProp rest angle, length and area, the resolved bend
+ * (hinge_bend_directional applied), hflag bit0 = excluded. */
+typedef struct ppf_shell {
+    uint32_t nv, nf, nh;
+    const float *x, *x0, *target;
+    const uint32_t *face, *hinge, *v2f_off, *v2f;
+    const float *vmass, *varea;
+    const uint32_t *vflag;
+    const float *fmass, *inv_rest, *f_mu, *f_lambda;
+    const uint32_t *fflag;
+    const float *h_rest, *h_length, *h_area, *h_bend;
+    const uint32_t *hflag;
+} ppf_shell;
+
+/* dt and the ParamSet air terms (air_density, air_friction, wind). */
+typedef struct ppf_elastic_params {
+    float dt, air_density, air_friction;
+    float wind[3];
+} ppf_elastic_params;
+
+typedef struct ppf_elastic ppf_elastic;
+
+/* Build the fixed upper-block pattern of the shell (every face and hinge
+ * vertex pair plus every diagonal; the shell stencils of builder.rs:1244-1355)
+ * and the tables that sum element blocks into it. Only nv/nf/nh, face and
+ * hinge are read. */
+int ppf_elastic_create(const ppf_shell *topology, void *stream, ppf_elastic **out);
+void ppf_elastic_destroy(ppf_elastic *h);
+
+/* The pattern, owned by h: row offsets (nv+1) and column indices (nnz). */
+int ppf_elastic_pattern(ppf_elastic *h, const uint32_t **d_row_offset,
+                        const uint32_t **d_col_index, uint32_t *nnz);
+
+/* Elastic + inertia force and Hessian at sh->x (replaces
+ * energy::embed_momentum_force_hessian and the shell part of
+ * energy::embed_elastic_force_hessian, energy/energy.cu:182-553, 700-794):
+ * d_force (3 nv) and d_cdiag (9 nv, the per-vertex diagonal) are overwritten,
+ * d_value (nnz blocks of the pattern) receives the face and hinge blocks.
+ * Contributions are summed per block in a fixed order (faces, then hinges, by
+ * index). sh must have the topology of create. */
+int ppf_elastic_assemble(ppf_elastic *h, const ppf_shell *sh, const ppf_elastic_params *p,
+                         float *d_force, float *d_cdiag, float *d_value, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

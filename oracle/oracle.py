@@ -354,3 +354,117 @@ def intersection_check(m, vmass, emass, fmass, static=None, surface=None, cap=25
     n = lib.oracle_intersection_check(ctypes.byref(s), ctypes.byref(ss), _p(vm), _p(em),
                                       _p(fm), nsurf, rec.ctypes.data_as(vp), cap)
     return int(n), rec[:min(int(n), cap)]
+
+
+# ---- elastic / inertia assembly ---------------------------------------------
+
+class Shell(ctypes.Structure):
+    _fields_ = [("nv", u32), ("nf", u32), ("nh", u32), ("x", vp), ("x0", vp),
+                ("target", vp), ("face", vp), ("hinge", vp), ("v2f_off", vp), ("v2f", vp),
+                ("vmass", vp), ("varea", vp), ("vflag", vp), ("fmass", vp),
+                ("inv_rest", vp), ("f_mu", vp), ("f_lambda", vp), ("fflag", vp),
+                ("h_rest", vp), ("h_length", vp), ("h_area", vp), ("h_bend", vp),
+                ("hflag", vp)]
+
+
+class ElasticParams(ctypes.Structure):
+    _fields_ = [("dt", f32), ("air_density", f32), ("air_friction", f32),
+                ("wind", f32 * 3)]
+
+
+_SHELL_U32 = {"face", "hinge", "v2f_off", "v2f", "vflag", "fflag", "hflag"}
+
+
+def shell_struct(sh):
+    keep, fields = {}, {}
+    for name, _ in Shell._fields_:
+        if name in ("nv", "nf", "nh"):
+            fields[name] = int(sh[name])
+            continue
+        dt = np.uint32 if name in _SHELL_U32 else np.float32
+        a = np.ascontiguousarray(sh[name], dtype=dt).reshape(-1)
+        if a.size == 0:
+            a = np.zeros(1, dt)
+        keep[name] = a
+        fields[name] = a.ctypes.data_as(vp)
+    return Shell(**fields), keep
+
+
+def _load_elastic():
+    lib = load()
+    if not hasattr(lib, "_elastic_ready"):
+        lib.oracle_elastic_local.argtypes = [ctypes.c_int, vp, f32, vp, vp]
+        lib.oracle_elastic_local.restype = None
+        lib.oracle_dihedral_angle.argtypes = [vp]
+        lib.oracle_dihedral_angle.restype = f32
+        lib.oracle_elastic_assemble.argtypes = [ctypes.POINTER(Shell),
+                                                ctypes.POINTER(ElasticParams), vp, vp, vp,
+                                                vp, vp]
+        lib.oracle_elastic_assemble.restype = u32
+        lib._elastic_ready = True
+    return lib
+
+
+def elastic_local(kind, inp, s0):
+    """kind 0 stretch / 1 shear (F (3,2)) -> (grad (3,2), hess (6,6)); kind 2
+    hinge (4 remapped points (4,3), s0 = rest angle) -> (force (4,3), hess (12,12))."""
+    lib = _load_elastic()
+    if kind in (0, 1):
+        a = np.asfortranarray(np.asarray(inp, np.float32).reshape(3, 2)).ravel(order="F")
+        g = np.zeros(6, np.float32)
+        h = np.zeros(36, np.float32)
+        lib.oracle_elastic_local(kind, _p(a), s0, _p(g), _p(h))
+        return g.reshape(2, 3).T, h.reshape(6, 6).T
+    a = _f32(inp).reshape(-1)
+    g = np.zeros(12, np.float32)
+    h = np.zeros(144, np.float32)
+    lib.oracle_elastic_local(kind, _p(a), s0, _p(g), _p(h))
+    return g.reshape(4, 3), h.reshape(12, 12)
+
+
+def ref_elastic_local(kind, inp, s0):
+    """The reference's own BaraffWitkin / face_compute_force_hessian (built
+    from the reference headers into oracle/_ref); kind 2 takes the hinge's four
+    points in STORED order and returns the force in the remapped order."""
+    lib = load_ref()
+    if not hasattr(lib, "_elastic_ready"):
+        lib.ref_bw.argtypes = [ctypes.c_int, vp, f32, vp, vp]
+        lib.ref_bw.restype = None
+        lib.ref_hinge.argtypes = [vp, f32, vp, vp]
+        lib.ref_hinge.restype = None
+        lib._elastic_ready = True
+    if kind in (0, 1):
+        a = np.asarray(inp, np.float32).reshape(3, 2).ravel(order="F").copy()
+        g = np.zeros(6, np.float32)
+        h = np.zeros(36, np.float32)
+        lib.ref_bw(kind, _p(a), s0, _p(g), _p(h))
+        return g.reshape(2, 3).T, h.reshape(6, 6).T
+    a = _f32(inp).reshape(-1).copy()
+    g = np.zeros(12, np.float32)
+    h = np.zeros(144, np.float32)
+    lib.ref_hinge(_p(a), s0, _p(g), _p(h))
+    return g.reshape(4, 3), h.reshape(12, 12)
+
+
+def dihedral_angle(pts):
+    lib = _load_elastic()
+    a = _f32(pts).reshape(-1)
+    return float(lib.oracle_dihedral_angle(_p(a)))
+
+
+def elastic_assemble(sh, rp, ci, dt, air_density=1e-3, air_friction=0.2, wind=(0, 0, 0)):
+    """Elastic + inertia assembly over the upper pattern (rp, ci):
+    (force (nv,3), cdiag (nv,3,3) column-major blocks, val (nnz, 9), missing)."""
+    lib = _load_elastic()
+    s, keep = shell_struct(sh)
+    p = ElasticParams(dt=dt, air_density=air_density, air_friction=air_friction,
+                      wind=(f32 * 3)(*wind))
+    rp = _u32(rp)
+    ci = _u32(ci)
+    nv = int(sh["nv"])
+    force = np.zeros(3 * nv, np.float32)
+    cdiag = np.zeros(9 * nv, np.float32)
+    val = np.zeros(9 * max(1, len(ci)), np.float32)
+    miss = lib.oracle_elastic_assemble(ctypes.byref(s), ctypes.byref(p), _p(rp), _p(ci),
+                                       _p(force), _p(cdiag), _p(val))
+    return force, cdiag, val[:9 * len(ci)].reshape(-1, 9), int(miss)

@@ -3,11 +3,9 @@
 // tree (see ../Makefile, target `ref`). This file holds only wrappers; the
 // arithmetic is the reference's. TEST INFRASTRUCTURE ONLY: it pins the oracle.
 #include "contact/distance.hpp"
+#include "energy/model/baraffwitkin.hpp"
+#include "energy/model/dihedral_angle.hpp"
 #include "linalg/eigsolve.hpp"
-
-// common.hpp declares the host logger the Rust side normally provides; the
-// wrapped functions never log.
-extern "C" void print_rust(const char *) {}
 
 namespace {
 Vec3f v3(const float *p) { return Vec3f(p[0], p[1], p[2]); }
@@ -57,5 +55,43 @@ void ref_ee_coeff(const float *a0, const float *a1, const float *b0,
     for (int i = 0; i < 4; ++i) {
         w[i] = c[i];
     }
+}
+
+// BaraffWitkin::stretch_gradient / stretch_hessian (kind 0) and
+// shear_gradient / shear_hessian (kind 1), energy/model/baraffwitkin.hpp:27-226.
+// F: 6 floats column-major; grad 6, hess 36 column-major.
+void ref_bw(int kind, const float *f, float s, float *grad, float *hess) {
+    Mat3x2f F;
+    for (int c = 0; c < 2; ++c)
+        for (int r = 0; r < 3; ++r) F(r, c) = f[r + 3 * c];
+    Mat3x2f g = kind == 0 ? BaraffWitkin::stretch_gradient(F, s)
+                          : BaraffWitkin::shear_gradient(F, s);
+    Mat6x6f h = kind == 0 ? BaraffWitkin::stretch_hessian(F, s)
+                          : BaraffWitkin::shear_hessian(F, s);
+    for (int c = 0; c < 2; ++c)
+        for (int r = 0; r < 3; ++r) grad[r + 3 * c] = g(r, c);
+    for (int c = 0; c < 6; ++c)
+        for (int r = 0; r < 6; ++r) hess[r + 6 * c] = h(r, c);
+}
+
+// dihedral_angle::face_compute_force_hessian (energy/model/dihedral_angle.hpp:
+// 130-355) on the hinge (0, 1, 2, 3) over four points p (12 floats, stored
+// order). Writes the force (12, remapped vertex order) and the 12x12 Hessian
+// row-major.
+void ref_hinge(const float *p, float rest, float *force, float *hess) {
+    Vec3f pts[4];
+    for (int k = 0; k < 4; ++k) pts[k] = v3(p + 3 * k);
+    Vec<Vec3f> vertex;
+    vertex.data = pts;
+    vertex.size = 4;
+    vertex.allocated = 4;
+    Vec4u hinge(0, 1, 2, 3);
+    Mat3x4f f;
+    Mat12x12f H;
+    dihedral_angle::face_compute_force_hessian(vertex, hinge, rest, f, H);
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 3; ++r) force[3 * c + r] = f(r, c);
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) hess[12 * i + j] = H(i, j);
 }
 }

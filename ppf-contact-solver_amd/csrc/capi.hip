@@ -3,6 +3,7 @@
 // Plain pointers and sizes only; no torch or C++ types cross the boundary.
 #include "common.hpp"
 #include "eig3.hpp"
+#include "elastic.hpp"
 #include "isect.hpp"
 #include "assemble.hpp"
 #include "linesearch.hpp"
@@ -86,6 +87,10 @@ struct ppf_contact {
 
 struct ppf_ccd {
     LineSearch impl;
+};
+
+struct ppf_elastic {
+    ElasticAssembler impl;
 };
 
 extern "C" {
@@ -372,6 +377,58 @@ int ppf_ccd_batch(int kind, uint32_t n, const float *d_x0, const float *d_x1,
     hipStream_t st = (hipStream_t)stream;
     int rc = ccd_batch(kind, n, d_x0, d_x1, d_offset, d_ghat, max_t, ccd_eps, d_toi,
                        d_overlap, st);
+    if (rc == PPF_OK) {
+        PPF_HIP(hipStreamSynchronize(st));
+    }
+    return rc;
+}
+
+int ppf_elastic_create(const ppf_shell *topology, void *stream, ppf_elastic **out) {
+    if (!topology || !out || (topology->nf && !topology->face) ||
+        (topology->nh && !topology->hinge)) {
+        set_error("ppf_elastic_create: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    ppf_elastic *h = new ppf_elastic();
+    int rc = h->impl.create(*topology, (hipStream_t)stream);
+    if (rc != PPF_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return PPF_OK;
+}
+
+void ppf_elastic_destroy(ppf_elastic *h) { delete h; }
+
+int ppf_elastic_pattern(ppf_elastic *h, const uint32_t **d_row_offset,
+                        const uint32_t **d_col_index, uint32_t *nnz) {
+    if (!h) {
+        set_error("ppf_elastic_pattern: null handle");
+        return PPF_ERR_ARG;
+    }
+    if (d_row_offset) *d_row_offset = h->impl.row_offset();
+    if (d_col_index) *d_col_index = h->impl.col_index();
+    if (nnz) *nnz = h->impl.nnz();
+    return PPF_OK;
+}
+
+int ppf_elastic_assemble(ppf_elastic *h, const ppf_shell *sh, const ppf_elastic_params *p,
+                         float *d_force, float *d_cdiag, float *d_value, void *stream) {
+    if (!h || !sh || !p || !d_force || !d_cdiag || (h->impl.nnz() && !d_value) || !sh->x ||
+        !sh->x0 || !sh->target || !sh->vmass || !sh->varea || !sh->vflag || !sh->v2f_off ||
+        (sh->nf && (!sh->fmass || !sh->inv_rest || !sh->f_mu || !sh->f_lambda || !sh->fflag)) ||
+        (sh->nh && (!sh->h_rest || !sh->h_length || !sh->h_area || !sh->h_bend || !sh->hflag)) ||
+        !(p->dt > 0.0f)) {
+        set_error("ppf_elastic_assemble: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    if (sh->nv != h->impl.nv() || sh->nf != h->impl.nf() || sh->nh != h->impl.nh()) {
+        set_error("ppf_elastic_assemble: shell sizes differ from the pattern's");
+        return PPF_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int rc = h->impl.assemble(*sh, *p, d_force, d_cdiag, d_value, st);
     if (rc == PPF_OK) {
         PPF_HIP(hipStreamSynchronize(st));
     }

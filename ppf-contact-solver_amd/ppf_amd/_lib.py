@@ -53,6 +53,11 @@ _PROTOS = {
                                           ctypes.POINTER(ctypes.c_uint64)]),
     "ppf_intersection_check": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, u32,
                                               ctypes.POINTER(u32), vp]),
+    "ppf_elastic_create": (ctypes.c_int, [vp, vp, ctypes.POINTER(vp)]),
+    "ppf_elastic_destroy": (None, [vp]),
+    "ppf_elastic_pattern": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                           ctypes.POINTER(u32)]),
+    "ppf_elastic_assemble": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "ppf_ccd_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
     "ppf_ccd_destroy": (None, [vp]),
     "ppf_line_search": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int,

@@ -437,3 +437,126 @@ def static_feature_scene(feature, res=8, offset=5e-5):
     sm = static_mesh(sx, sf, ghat=1e-3, offset=offset)
     ones = lambda n: np.ones(n, np.float32)
     return m, x1, ones(m["nv"]), ones(m["ne"]), ones(m["nf"]), sm
+
+
+def shell_hinges(faces):
+    """Interior edges shared by two faces as hinges (e0, e1, flap0, flap1): the
+    stored layout whose remap (h2, h1, h0, h3) puts the shared edge in the
+    middle. Sorted by edge."""
+    f = np.asarray(faces, np.int64)
+    owner = {}
+    hinges = []
+    for fi, tri in enumerate(f):
+        for k in range(3):
+            a, b, c = tri[k], tri[(k + 1) % 3], tri[(k + 2) % 3]
+            key = (min(a, b), max(a, b))
+            if key in owner:
+                gi, gc = owner.pop(key)
+                hinges.append((key[0], key[1], gc, c))
+            else:
+                owner[key] = (fi, c)
+    h = np.array(sorted(hinges), np.uint32).reshape(-1, 4)
+    return h
+
+
+def vertex_face_csr(nv, faces):
+    f = np.asarray(faces, np.int64)
+    cnt = np.bincount(f.reshape(-1), minlength=nv)
+    off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint32)
+    order = np.argsort(f.reshape(-1), kind="stable")
+    v2f = (order // 3).astype(np.uint32)
+    return off, v2f
+
+
+def _dihedral64(p0, p1, p2, p3):
+    """Signed dihedral angle of the remapped hinge (X0..X3) in fp64 (the
+    atan2 form of dihedral_angle.hpp:86-108)."""
+    n1 = np.cross(p1 - p0, p2 - p0)
+    n2 = np.cross(p2 - p3, p1 - p3)
+    e = p1 - p2
+    s = np.einsum("...i,...i->...", np.cross(n2, n1), e) / np.linalg.norm(e, axis=-1)
+    c = np.einsum("...i,...i->...", n1, n2)
+    return np.arctan2(s, c)
+
+
+def shell_scene(res=16, size=1.0, density=0.2, young=1e4, poisson=0.25, bend=10.0,
+                dt=1.0 / 60.0, seed=0, bump=0.05, stretch=0.02, pins=0, gravity=-9.8):
+    """A square cloth (res x res vertices, side `size`, the alternating
+    triangulation of rect_faces) with everything the elastic assembly reads:
+    rest geometry (lumped vertex mass / area, face mass, inv_rest2x2, hinge
+    rest angle / length / area), the current state x0 = rest + a smooth bump,
+    a perturbed Newton iterate x, and the inertial target x0 + dt^2 g. `pins`
+    vertices (the first ones) are pinned. Lame parameters from
+    young / poisson as builder.rs:1739-1743."""
+    rng = np.random.default_rng(seed)
+    faces = rect_faces(res, res).astype(np.uint32)
+    i, j = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    h = size / (res - 1)
+    rest = np.stack([i.reshape(-1) * h, j.reshape(-1) * h, np.zeros(res * res)], 1)
+    nv, nf = len(rest), len(faces)
+    # rest-frame quantities (fp64, then fp32 as the builder stores them)
+    P = rest[faces.astype(np.int64)]
+    e1, e2 = P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]
+    area = 0.5 * np.linalg.norm(np.cross(e1, e2), axis=1)
+    l1 = np.linalg.norm(e1, axis=1)
+    u = e1 / l1[:, None]
+    w = np.cross(np.cross(e1, e2), e1)
+    w /= np.linalg.norm(w, axis=1)[:, None]
+    D = np.stack([np.stack([l1, np.einsum("ij,ij->i", e2, u)], 1),
+                  np.stack([np.zeros(nf), np.einsum("ij,ij->i", e2, w)], 1)], 1)
+    inv = np.linalg.inv(D)                              # (nf, 2, 2) row-major
+    inv_rest = inv.transpose(0, 2, 1).reshape(nf, 4)    # column-major
+    fmass = density * area
+    varea = np.zeros(nv)
+    np.add.at(varea, faces.reshape(-1).astype(np.int64), np.repeat(area / 3.0, 3))
+    vmass = density * varea
+    hinge = shell_hinges(faces)
+    H = hinge.astype(np.int64)
+    hl = np.linalg.norm(rest[H[:, 1]] - rest[H[:, 0]], axis=1)
+    # combined rest area of the two faces of each hinge
+    fa = {}
+    for fi, tri in enumerate(faces.astype(np.int64)):
+        for k in range(3):
+            key = (min(tri[k], tri[(k + 1) % 3]), max(tri[k], tri[(k + 1) % 3]))
+            fa[key] = fa.get(key, 0.0) + area[fi]
+    harea = np.array([fa[(min(a, b), max(a, b))] for a, b in H[:, :2]])
+    mu = young / (2.0 * (1.0 + poisson))
+    lam = young * poisson / ((1.0 + poisson) * (1.0 - 2.0 * poisson))
+    # state: a smooth bump, rest angles of the rest (flat) shape
+    x0 = rest.copy()
+    x0[:, 2] = bump * np.sin(np.pi * rest[:, 0] / size) * np.sin(np.pi * rest[:, 1] / size)
+    x = x0 + rng.normal(0.0, stretch * h, x0.shape)
+    target = x0 + np.array([0.0, 0.0, gravity]) * dt * dt
+    hrest = _dihedral64(rest[H[:, 2]], rest[H[:, 1]], rest[H[:, 0]], rest[H[:, 3]])
+    v2f_off, v2f = vertex_face_csr(nv, faces)
+    vflag = np.zeros(nv, np.uint32)
+    vflag[:pins] = 1
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    return dict(nv=nv, nf=nf, nh=len(hinge), x=f32(x), x0=f32(x0), target=f32(target),
+                face=faces, hinge=hinge, v2f_off=v2f_off, v2f=v2f,
+                vmass=f32(vmass), varea=f32(varea), vflag=vflag,
+                fmass=f32(fmass), inv_rest=f32(inv_rest), f_mu=f32(np.full(nf, mu)),
+                f_lambda=f32(np.full(nf, lam)), fflag=np.zeros(nf, np.uint32),
+                h_rest=f32(hrest), h_length=f32(hl), h_area=f32(harea),
+                h_bend=f32(np.full(len(hinge), bend)),
+                hflag=np.zeros(len(hinge), np.uint32), dt=dt)
+
+
+def upper_pattern(nv, faces, hinges):
+    """Upper-triangle block pattern (row offsets, columns) of every face and
+    hinge vertex pair plus every diagonal (numpy, for checking)."""
+    pairs = [np.stack([np.arange(nv), np.arange(nv)], 1)]
+    f = np.asarray(faces, np.int64)
+    for a in range(3):
+        for b in range(3):
+            pairs.append(np.stack([f[:, a], f[:, b]], 1))
+    hh = np.asarray(hinges, np.int64).reshape(-1, 4)
+    for a in range(4):
+        for b in range(4):
+            pairs.append(np.stack([hh[:, a], hh[:, b]], 1))
+    p = np.concatenate(pairs)
+    p = p[p[:, 0] <= p[:, 1]]
+    key = np.unique(p[:, 0] * nv + p[:, 1])
+    rows, cols = key // nv, key % nv
+    ro = np.searchsorted(rows, np.arange(nv + 1)).astype(np.uint32)
+    return ro, cols.astype(np.uint32)
