@@ -128,7 +128,10 @@ typedef struct ppf_contact_mesh {
  * 2 point-point, 3 edge-edge, with bit 8 set when the pair carries no friction
  * (a neighbor element of the candidate contains the query vertex,
  * contact.cu:498-531, 685-695); a = query element (vertex; edge for kind 3),
- * b = candidate element (face, edge, vertex, edge); count = multiplicity. */
+ * b = candidate element (face, edge, vertex, edge); count = multiplicity.
+ * Collision-mesh pairs (contact.cu:899-1152): kind 4 dynamic vertex a vs
+ * static face b, 5 static vertex a vs dynamic face b, 6 dynamic edge a vs
+ * static edge b; count 1. */
 typedef struct ppf_pair {
     uint32_t kind, a, b, count;
 } ppf_pair;
@@ -157,6 +160,20 @@ int ppf_narrow_tree(ppf_narrow *h, int kind, uint32_t *d_nodes, float *d_boxes,
 int ppf_narrow_proximity(ppf_narrow *h, const ppf_contact_mesh *m,
                          ppf_pair *d_out, uint64_t cap, uint64_t *count,
                          void *stream);
+
+struct ppf_static_mesh;
+
+/* All collision-mesh proximity pairs (kinds 4, 5, 6; the detection of
+ * contact::embed_constraint_force_hessian, contact/contact.cu:1789-1954, with
+ * the acceptance tests of contact.cu:899-1152): per unpinned vertex of
+ * positive mass against the static faces, per static vertex against the
+ * faces that are not fixed and have mass, per edge that is not fixed and has
+ * mass against the static edges; grouped by query. Element masses are
+ * VertexProp / EdgeProp / FaceProp mass. Needs ppf_narrow_build at m->x. */
+int ppf_narrow_static(ppf_narrow *h, const ppf_contact_mesh *m,
+                      const struct ppf_static_mesh *sm, const float *d_vmass,
+                      const float *d_emass, const float *d_fmass, ppf_pair *d_out,
+                      uint64_t cap, uint64_t *count, void *stream);
 
 /* One intersecting pair (IntersectionRecord, data.hpp:741-748): type 0
  * face-edge, 1 edge-edge, 2 collision-mesh face vs edge, 3 point-point;
@@ -200,6 +217,9 @@ typedef struct ppf_contact_params {
     float friction_eps;
     const uint32_t *el_rp, *el_col;
     const float *el_val;
+    /* the collision mesh the kind 4-6 pairs refer to (host pointer to a struct
+     * of device pointers; may be null when there are no such pairs) */
+    const struct ppf_static_mesh *sm;
 } ppf_contact_params;
 
 typedef struct ppf_contact ppf_contact;
@@ -236,6 +256,9 @@ typedef struct ppf_static_mesh {
     const float *x;
     const uint32_t *edge, *face;
     const float *v_ghat, *v_off, *e_ghat, *e_off, *f_ghat, *f_off;
+    /* per-element friction (the collision mesh's Vertex/Edge/FaceParam
+     * friction); read by the contact assembly only */
+    const float *v_fric, *e_fric, *f_fric;
 } ppf_static_mesh;
 
 /* Line-search inputs besides the mesh (whose x is the step's start x0): the

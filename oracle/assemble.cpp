@@ -20,6 +20,7 @@
 // and force rows summed in pair order, then in the (ii, jj) / ii loop order of
 // atomic_embed_hessian / atomic_embed_force (contact.cu:175-190,
 // utility/utility.hpp:63-72), starting from zero.
+#include "oracle_distance.hpp"
 #include "oracle_math.hpp"
 #include <cmath>
 #include <cstdint>
@@ -31,79 +32,9 @@ namespace {
 
 using orc::V3;
 using orc::v3;
-
-void pe_coeff(const V3 &p, const V3 &e0, const V3 &e1, float w[2]) {
-    V3 r = orc::sub(e1, e0);
-    float d = orc::dot(r, r);
-    if (d > 0.0f) {
-        float t = orc::dot(r, orc::sub(p, e0)) / d;
-        w[0] = 1.0f - t;
-        w[1] = t;
-    } else {
-        w[0] = w[1] = 1.0f / 2.0f;
-    }
-}
-
-void pt_coeff(const V3 &p, const V3 &t0, const V3 &t1, const V3 &t2, float w[3]) {
-    V3 r0 = orc::sub(t1, t0), r1 = orc::sub(t2, t0), d = orc::sub(p, t0);
-    V3 n = orc::cross(r0, r1);
-    float nn = orc::dot(n, n);
-    if (nn > 0.0f) {
-        float inv = 1.0f / nn;
-        float u = orc::dot(orc::cross(d, r1), n) * inv;
-        float v = orc::dot(orc::cross(r0, d), n) * inv;
-        w[0] = 1.0f - u - v;
-        w[1] = u;
-        w[2] = v;
-        return;
-    }
-    V3 e1 = orc::sub(t2, t1), e2 = orc::sub(t0, t2);
-    float l0 = orc::dot(r0, r0), l1 = orc::dot(e1, e1), l2 = orc::dot(e2, e2);
-    float c[2];
-    if (l0 >= l1 && l0 >= l2) {
-        pe_coeff(p, t0, t1, c);
-        w[0] = c[0], w[1] = c[1], w[2] = 0.0f;
-    } else if (l1 >= l2) {
-        pe_coeff(p, t1, t2, c);
-        w[0] = 0.0f, w[1] = c[0], w[2] = c[1];
-    } else {
-        pe_coeff(p, t2, t0, c);
-        w[0] = c[1], w[1] = 0.0f, w[2] = c[0];
-    }
-}
-
-float clamp_unit(float v) { return v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f; }
-
-void ee_coeff(const V3 &a0, const V3 &a1, const V3 &b0, const V3 &b1, float w[4]) {
-    V3 r0 = orc::sub(a1, a0), r1 = orc::sub(b1, b0), d = orc::sub(a0, b0);
-    float a = orc::dot(r0, r0), e = orc::dot(r1, r1), b = orc::dot(r0, r1);
-    float c = orc::dot(r0, d), f = orc::dot(r1, d);
-    float inv_a = a > 0.0f ? 1.0f / a : 0.0f;
-    float inv_e = e > 0.0f ? 1.0f / e : 0.0f;
-    V3 n = orc::cross(r0, r1);
-    float nn = orc::dot(n, n);
-    float inv_nn = nn > 0.0f ? 1.0f / nn : 0.0f;
-    float s_stat = clamp_unit(orc::dot(n, orc::cross(r1, d)) * inv_nn);
-    float t_stat = clamp_unit((b * s_stat + f) * inv_e);
-    const float cs[5] = {s_stat, 0.0f, 1.0f, clamp_unit(-c * inv_a),
-                         clamp_unit((b - c) * inv_a)};
-    const float ct[5] = {t_stat, clamp_unit(f * inv_e), clamp_unit((f + b) * inv_e),
-                         0.0f, 1.0f};
-    float bs = 0.0f, bt = 0.0f, best = 3.40282347e38f;
-    for (int i = 0; i < 5; ++i) {
-        V3 v = orc::sub(orc::add(d, orc::scale(cs[i], r0)), orc::scale(ct[i], r1));
-        float dist = orc::dot(v, v);
-        if (dist < best) {
-            best = dist;
-            bs = cs[i];
-            bt = ct[i];
-        }
-    }
-    w[0] = 1.0f - bs;
-    w[1] = bs;
-    w[2] = 1.0f - bt;
-    w[3] = bt;
-}
+using orc::ee_coeff;
+using orc::pe_coeff;
+using orc::pt_coeff;
 
 // Elastic block (i, j) of an upper block CSR, transposed when i > j.
 void lookup(uint32_t i, uint32_t j, const uint32_t *rp, const uint32_t *col,
@@ -157,6 +88,80 @@ float combine(float a, float b, int mode) {
     return std::fmin(a, b);
 }
 
+
+// Friction (energy/model/friction.hpp:19-81) added to f and H.
+void add_friction(const V3 &nrm, const V3 &dx, float mu, float eps, V3 &f, float H[9]) {
+    const float contact = -orc::dot(nrm, f);
+    const float ndx = orc::dot(nrm, dx);
+    const V3 u = orc::sub(dx, v3(nrm[0] * ndx, nrm[1] * ndx, nrm[2] * ndx));
+    const float un = std::sqrt(orc::sqnorm(u));
+    const bool kinetic = un > eps;
+    const V3 sd = kinetic ? v3(u[0] / un, u[1] / un, u[2] / un) : v3(0, 0, 0);
+    const float lam = mu > 0.0f ? mu * contact / std::fmax(eps, un) : 0.0f;
+    f = orc::add(f, orc::scale(lam, u));
+    float fh[9];
+    if (kinetic) {
+        const V3 wd = orc::cross(nrm, sd);
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) fh[r + 3 * c] = lam * (wd[r] * wd[c]);
+    } else {
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) {
+                const float id = r == c ? 1.0f : 0.0f;
+                fh[r + 3 * c] = lam * (id - nrm[r] * nrm[c]);
+            }
+    }
+    for (int k = 0; k < 9; ++k) {
+        H[k] = H[k] + fh[k];
+    }
+}
+
+// push::gradient / push::hessian scaled by k (energy/model/push.hpp:26-46).
+void push_terms(const V3 &e, const V3 &n, float eps, float k, V3 &f, float H[9]) {
+    const float d = orc::dot(e, n);
+    V3 g = v3(0, 0, 0);
+    float cv = 0.0f;
+    if (d < 0.0f) {
+        g = orc::scale(-(d * d) / eps, n);
+        cv = -2.0f * d / eps;
+    }
+    f = orc::scale(k, g);
+    const V3 cn = orc::scale(cv, n);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) H[r + 3 * c] = k * (cn[r] * n[c]);
+}
+
+// (K ne).ne / |ne|^2 over N vertices, K = elastic blocks + mass / gap^2 on
+// the diagonal blocks, ne = c_i n (contact.cu:1010-1028, 1092-1106).
+float ext_stiffness(int N, const uint32_t *idx, const float *c, const V3 &n, float gap2,
+                    const float *mass, const uint32_t *rp, const uint32_t *col,
+                    const float *val) {
+    float ne[12], Kn[12];
+    for (int j = 0; j < 3 * N; ++j) {
+        ne[j] = c[j / 3] * n[j % 3];
+        Kn[j] = 0.0f;
+    }
+    // row-major 3N x 3N product accumulated over columns in order
+    for (int col_ = 0; col_ < 3 * N; ++col_) {
+        const int jj = col_ / 3, cc = col_ % 3;
+        for (int row = 0; row < 3 * N; ++row) {
+            const int ii = row / 3, r = row % 3;
+            float blk[9];
+            lookup(idx[ii], idx[jj], rp, col, val, blk);
+            float v = blk[r + 3 * cc];
+            if (ii == jj) {
+                v = v + (r == cc ? mass[idx[ii]] / gap2 : 0.0f);
+            } else {
+                v = v + 0.0f;
+            }
+            Kn[row] = Kn[row] + v * ne[col_];
+        }
+    }
+    float num = 0.0f, den = 0.0f;
+    for (int j = 0; j < 3 * N; ++j) num = num + Kn[j] * ne[j];
+    for (int j = 0; j < 3 * N; ++j) den = den + ne[j] * ne[j];
+    return num / den;
+}
 } // namespace
 
 extern "C" {
@@ -176,6 +181,16 @@ struct oracle_pair_in {
 };
 
 // Contact material and solver constants for the assembly.
+// Collision mesh (the same layout as oracle_static_mesh of ccd.cpp, with the
+// per-element friction appended).
+struct oracle_static_view {
+    uint32_t nv, ne, nf;
+    const float *x;
+    const uint32_t *edge, *face;
+    const float *v_ghat, *v_off, *e_ghat, *e_off, *f_ghat, *f_off;
+    const float *v_fric, *e_fric, *f_fric;
+};
+
 struct oracle_contact_params {
     const float *x0;     // vertex.curr, 3 nv (the step's start positions)
     const float *mass;   // per vertex
@@ -184,6 +199,7 @@ struct oracle_contact_params {
     float friction_eps;
     const uint32_t *el_rp, *el_col; // elastic-only upper block CSR (may be null)
     const float *el_val;
+    const oracle_static_view *sm;   // collision mesh of kind 4-6 pairs (may be null)
 };
 
 // Assemble the contact gradient (grad, 3 nv, written) and the upper-triangle
@@ -209,6 +225,112 @@ uint64_t oracle_contact_assemble(const oracle_mesh_view *m,
         uint32_t idx[4];
         float w[4];
         int N;
+        if (kind >= 4) {
+            const oracle_static_view &sm = *cp->sm;
+            auto SX = [&](uint32_t v) { return X(sm.x, v); };
+            V3 f;
+            float H[9];
+            if (kind == 4) {
+                const V3 p = X(m->x, a);
+                const V3 q = orc::sub(X(cp->x0, a), p);
+                const uint32_t *fc = sm.face + 3 * (size_t)b;
+                const V3 t0 = orc::sub(SX(fc[0]), p), t1 = orc::sub(SX(fc[1]), p),
+                         t2 = orc::sub(SX(fc[2]), p);
+                float c[3];
+                orc::pt_coeff_unclassified(v3(0, 0, 0), t0, t1, t2, c);
+                const V3 y = orc::add(orc::add(orc::scale(c[0], t0), orc::scale(c[1], t1)),
+                                      orc::scale(c[2], t2));
+                const V3 e = v3(-y[0], -y[1], -y[2]);
+                const float off = m->v_off[a] + sm.f_off[b];
+                const float gh = 0.5f * (m->v_ghat[a] + sm.f_ghat[b]);
+                const float mu = combine(cp->v_fric[a], sm.f_fric[b], cp->friction_mode);
+                const V3 n = orc::normalized(e);
+                const V3 proj = orc::add(y, orc::scale(off + gh, n));
+                const float g = std::sqrt(orc::sqnorm(e)) - off;
+                const float gap2 = g * g;
+                float blk[9];
+                lookup(a, a, cp->el_rp, cp->el_col, cp->el_val, blk);
+                const V3 Hn = v3((blk[0] * n[0] + blk[3] * n[1]) + blk[6] * n[2],
+                                 (blk[1] * n[0] + blk[4] * n[1]) + blk[7] * n[2],
+                                 (blk[2] * n[0] + blk[5] * n[1]) + blk[8] * n[2]);
+                const float k = orc::dot(n, Hn) + cp->mass[a] / gap2;
+                push_terms(v3(-proj[0], -proj[1], -proj[2]), n, gh, k, f, H);
+                add_friction(n, v3(-q[0], -q[1], -q[2]), mu, cp->friction_eps, f, H);
+                N = 1;
+                idx[0] = a;
+                w[0] = 1.0f;
+            } else if (kind == 5) {
+                const V3 yv = SX(a);
+                const uint32_t *fc = m->face + 3 * (size_t)b;
+                const V3 t0 = orc::sub(X(m->x, fc[0]), yv), t1 = orc::sub(X(m->x, fc[1]), yv),
+                         t2 = orc::sub(X(m->x, fc[2]), yv);
+                const V3 s0 = orc::sub(X(cp->x0, fc[0]), yv),
+                         s1 = orc::sub(X(cp->x0, fc[1]), yv),
+                         s2 = orc::sub(X(cp->x0, fc[2]), yv);
+                float c[3];
+                orc::pt_coeff_unclassified(v3(0, 0, 0), t0, t1, t2, c);
+                const V3 p = orc::add(orc::add(orc::scale(c[0], t0), orc::scale(c[1], t1)),
+                                      orc::scale(c[2], t2));
+                const V3 q = orc::add(orc::add(orc::scale(c[0], s0), orc::scale(c[1], s1)),
+                                      orc::scale(c[2], s2));
+                const float off = m->f_off[b] + sm.v_off[a];
+                const float gh = 0.5f * (m->f_ghat[b] + sm.v_ghat[a]);
+                const float mu = combine(cp->f_fric[b], sm.v_fric[a], cp->friction_mode);
+                const V3 n = orc::normalized(p);
+                const float g = std::sqrt(orc::sqnorm(p)) - off;
+                const float gap2 = g * g;
+                const uint32_t fi[3] = {fc[0], fc[1], fc[2]};
+                const float k = ext_stiffness(3, fi, c, n, gap2, cp->mass, cp->el_rp,
+                                              cp->el_col, cp->el_val);
+                const V3 proj = orc::scale(off + gh, n);
+                push_terms(orc::sub(p, proj), n, gh, k, f, H);
+                add_friction(n, orc::sub(p, q), mu, cp->friction_eps, f, H);
+                N = 3;
+                idx[0] = fc[0], idx[1] = fc[1], idx[2] = fc[2];
+                w[0] = c[0], w[1] = c[1], w[2] = c[2];
+            } else {
+                const uint32_t *ea = m->edge + 2 * (size_t)a;
+                const uint32_t *eb = sm.edge + 2 * (size_t)b;
+                const V3 p0 = X(m->x, ea[0]), p1 = X(m->x, ea[1]);
+                const V3 q0 = SX(eb[0]), q1 = SX(eb[1]);
+                const V3 b0 = X(cp->x0, ea[0]), b1 = X(cp->x0, ea[1]);
+                float c[4];
+                ee_coeff(p0, p1, q0, q1, c);
+                const V3 x = orc::add(orc::scale(c[0], p0), orc::scale(c[1], p1));
+                const V3 y = orc::add(orc::scale(c[2], q0), orc::scale(c[3], q1));
+                const V3 z = orc::add(orc::scale(c[0], b0), orc::scale(c[1], b1));
+                const V3 e = orc::sub(x, y);
+                const float off = m->e_off[a] + sm.e_off[b];
+                const float gh = 0.5f * (m->e_ghat[a] + sm.e_ghat[b]);
+                const float mu = combine(cp->e_fric[a], sm.e_fric[b], cp->friction_mode);
+                const V3 n = orc::normalized(e);
+                const V3 proj = orc::scale(off + gh, n);
+                const float g = std::sqrt(orc::sqnorm(e)) - off;
+                const float gap2 = g * g;
+                const uint32_t ei[2] = {ea[0], ea[1]};
+                const float k = ext_stiffness(2, ei, c, n, gap2, cp->mass, cp->el_rp,
+                                              cp->el_col, cp->el_val);
+                push_terms(orc::sub(e, proj), n, gh, k, f, H);
+                add_friction(n, orc::sub(x, z), mu, cp->friction_eps, f, H);
+                N = 2;
+                idx[0] = ea[0], idx[1] = ea[1];
+                w[0] = c[0], w[1] = c[1];
+            }
+            const float cnt = (float)pairs[pi].count;
+            for (int ii = 0; ii < N; ++ii)
+                for (int cc = 0; cc < 3; ++cc) grad[3 * (size_t)idx[ii] + cc] += cnt * (w[ii] * f[cc]);
+            for (int ii = 0; ii < N; ++ii)
+                for (int jj = 0; jj < N; ++jj) {
+                    const uint32_t i = idx[ii], j = idx[jj];
+                    if (i <= j) {
+                        std::vector<float> &acc = blocks[{i, j}];
+                        if (acc.empty()) acc.assign(9, 0.0f);
+                        const float s = w[ii] * w[jj];
+                        for (int k = 0; k < 9; ++k) acc[k] = acc[k] + cnt * (s * H[k]);
+                    }
+                }
+            continue;
+        }
         float gh, off, mu;
         if (kind == 0) {
             const uint32_t *f = m->face + 3 * (size_t)b;
@@ -324,35 +446,7 @@ uint64_t oracle_contact_assemble(const oracle_mesh_view *m,
             }
         }
         if (!no_fric) {
-            // Friction (friction.hpp:19-81).
-            const float contact = -orc::dot(nrm, f);
-            const float ndx = orc::dot(nrm, dx);
-            const V3 u = orc::sub(dx, v3(nrm[0] * ndx, nrm[1] * ndx, nrm[2] * ndx));
-            const float un = std::sqrt(orc::sqnorm(u));
-            const bool kinetic = un > cp->friction_eps;
-            const V3 sd = kinetic ? v3(u[0] / un, u[1] / un, u[2] / un) : v3(0, 0, 0);
-            const float lam = mu > 0.0f ? mu * contact / std::fmax(cp->friction_eps, un)
-                                        : 0.0f;
-            f = orc::add(f, orc::scale(lam, u));
-            float fh[9];
-            if (kinetic) {
-                const V3 wd = orc::cross(nrm, sd);
-                for (int c = 0; c < 3; ++c) {
-                    for (int r = 0; r < 3; ++r) {
-                        fh[r + 3 * c] = lam * (wd[r] * wd[c]);
-                    }
-                }
-            } else {
-                for (int c = 0; c < 3; ++c) {
-                    for (int r = 0; r < 3; ++r) {
-                        const float id = r == c ? 1.0f : 0.0f;
-                        fh[r + 3 * c] = lam * (id - nrm[r] * nrm[c]);
-                    }
-                }
-            }
-            for (int k = 0; k < 9; ++k) {
-                H[k] = H[k] + fh[k];
-            }
+            add_friction(nrm, dx, mu, cp->friction_eps, f, H);
         }
         const float cnt = (float)pairs[pi].count;
         for (int ii = 0; ii < N; ++ii) {

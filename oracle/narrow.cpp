@@ -384,3 +384,110 @@ void oracle_ee_coeff(const float *a0, const float *a1, const float *b0,
              v3(b0[0], b0[1], b0[2]), v3(b1[0], b1[1], b1[2]), w);
 }
 }
+
+extern "C" {
+
+// Collision mesh for the static pairs (prefix layout of ppf_static_mesh).
+struct oracle_static_view_n {
+    uint32_t nv, ne, nf;
+    const float *x;
+    const uint32_t *edge, *face;
+    const float *v_ghat, *v_off, *e_ghat, *e_off, *f_ghat, *f_off;
+};
+
+// Collision-mesh proximity pairs (contact.cu:899-1152, 1789-1954): kind 4 per
+// unpinned vertex of positive mass vs static faces, kind 5 per static vertex
+// vs faces that are not fixed and have mass, kind 6 per edge that is not
+// fixed and has mass vs static edges; candidates from boxes of margin
+// 0.5 * ghat + offset on both sides, accepted when the squared distance is
+// below (offset + ghat)^2. Queries in index order. Returns the pair count;
+// writes at most `cap`.
+uint64_t oracle_static_proximity(const oracle_contact_mesh *m, const oracle_static_view_n *sm,
+                                 const float *vmass, const float *emass, const float *fmass,
+                                 oracle_pair *out, uint64_t cap) {
+    const uint32_t nv = m->nv, ne = m->ne, nf = m->nf;
+    std::vector<V3> x(nv), sx(sm->nv);
+    for (uint32_t i = 0; i < nv; ++i) x[i] = v3(m->x[3 * i], m->x[3 * i + 1], m->x[3 * i + 2]);
+    for (uint32_t i = 0; i < sm->nv; ++i)
+        sx[i] = v3(sm->x[3 * i], sm->x[3 * i + 1], sm->x[3 * i + 2]);
+    auto tree = [](Bvh &t, uint32_t n, int k, const uint32_t *el, const float *gh,
+                   const float *off, const std::vector<V3> &pos) {
+        std::vector<V3> c(n);
+        std::vector<Box> pb(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            V3 p[3];
+            for (int a = 0; a < k; ++a) p[a] = pos[el[(size_t)k * i + a]];
+            c[i] = p[0];
+            pb[i] = leaf_box(p, k, 0.5f * gh[i] + off[i]);
+        }
+        build_bvh(t, n, c, pb);
+    };
+    Bvh fb, sfb, seb;
+    tree(fb, nf, 3, m->face, m->f_ghat, m->f_off, x);
+    tree(sfb, sm->nf, 3, sm->face, sm->f_ghat, sm->f_off, sx);
+    tree(seb, sm->ne, 2, sm->edge, sm->e_ghat, sm->e_off, sx);
+    uint64_t n = 0;
+    auto emit = [&](uint32_t kind, uint32_t a, uint32_t b) {
+        if (n < cap) out[n] = oracle_pair{kind, a, b, 1};
+        ++n;
+    };
+    const V3 zero = v3(0, 0, 0);
+    for (uint32_t i = 0; i < nv; ++i) {
+        if ((m->vflag[i] & 1u) != 0 || !(vmass[i] > 0.0f)) continue;
+        const V3 p = x[i];
+        query(sfb, point_box(p, 0.5f * m->v_ghat[i] + m->v_off[i]), [&](uint32_t fi) {
+            const uint32_t *f = sm->face + 3 * (size_t)fi;
+            const V3 t0 = orc::sub(sx[f[0]], p), t1 = orc::sub(sx[f[1]], p),
+                     t2 = orc::sub(sx[f[2]], p);
+            float c[3];
+            orc::pt_coeff_unclassified(zero, t0, t1, t2, c);
+            const V3 y = orc::add(orc::add(orc::scale(c[0], t0), orc::scale(c[1], t1)),
+                                  orc::scale(c[2], t2));
+            const float r = (m->v_off[i] + sm->f_off[fi]) + 0.5f * (m->v_ghat[i] + sm->f_ghat[fi]);
+            if (orc::dot(y, y) < r * r) {
+                emit(4, i, fi);
+                return true;
+            }
+            return false;
+        });
+    }
+    for (uint32_t sv = 0; sv < sm->nv; ++sv) {
+        const V3 y = sx[sv];
+        query(fb, point_box(y, 0.5f * sm->v_ghat[sv] + sm->v_off[sv]), [&](uint32_t fi) {
+            if ((m->fflag[fi] & 1u) != 0 || !(fmass[fi] > 0.0f)) return false;
+            const uint32_t *f = m->face + 3 * (size_t)fi;
+            const V3 t0 = orc::sub(x[f[0]], y), t1 = orc::sub(x[f[1]], y),
+                     t2 = orc::sub(x[f[2]], y);
+            float c[3];
+            orc::pt_coeff_unclassified(zero, t0, t1, t2, c);
+            const V3 e = orc::add(orc::add(orc::scale(c[0], t0), orc::scale(c[1], t1)),
+                                  orc::scale(c[2], t2));
+            const float r = (m->f_off[fi] + sm->v_off[sv]) + 0.5f * (m->f_ghat[fi] + sm->v_ghat[sv]);
+            if (orc::dot(e, e) < r * r) {
+                emit(5, sv, fi);
+                return true;
+            }
+            return false;
+        });
+    }
+    for (uint32_t ei = 0; ei < ne; ++ei) {
+        if ((m->eflag[ei] & 1u) != 0 || !(emass[ei] > 0.0f)) continue;
+        const V3 p0 = x[m->edge[2 * ei]], p1 = x[m->edge[2 * ei + 1]];
+        const V3 seg[2] = {p0, p1};
+        query(seb, orc::make_box(seg, 2, 0.5f * m->e_ghat[ei] + m->e_off[ei]), [&](uint32_t sj) {
+            const V3 q0 = sx[sm->edge[2 * sj]], q1 = sx[sm->edge[2 * sj + 1]];
+            float c[4];
+            orc::ee_coeff(p0, p1, q0, q1, c);
+            const V3 e = orc::sub(orc::add(orc::scale(c[0], p0), orc::scale(c[1], p1)),
+                                  orc::add(orc::scale(c[2], q0), orc::scale(c[3], q1)));
+            const float r = (m->e_off[ei] + sm->e_off[sj]) + 0.5f * (m->e_ghat[ei] + sm->e_ghat[sj]);
+            if (orc::dot(e, e) < r * r) {
+                emit(6, ei, sj);
+                return true;
+            }
+            return false;
+        });
+    }
+    return n;
+}
+}

@@ -218,11 +218,11 @@ class ContactParams(ctypes.Structure):
     _fields_ = [("x0", vp), ("mass", vp), ("v_fric", vp), ("e_fric", vp),
                 ("f_fric", vp), ("friction_mode", ctypes.c_int),
                 ("friction_eps", ctypes.c_float), ("el_rp", vp), ("el_col", vp),
-                ("el_val", vp)]
+                ("el_val", vp), ("sm", vp)]
 
 
 def contact_assemble(m, x0, mass, fric, pairs, friction_mode=0, friction_eps=1e-5,
-                     elastic=None):
+                     elastic=None, static=None):
     """(grad (3nv,), rows, cols, vals (nnz, 9)) in the oracle's order."""
     lib = load()
     if not hasattr(lib, "_asm_ready"):
@@ -240,12 +240,14 @@ def contact_assemble(m, x0, mass, fric, pairs, friction_mode=0, friction_eps=1e-
     el_rp = _u32(el[0]) if el[0] is not None else None
     el_col = _u32(el[1]) if el[1] is not None else None
     el_val = _f32(el[2]).reshape(-1) if el[2] is not None else None
+    ss, keep2 = static_mesh_struct(static)
     p = ContactParams(x0=_p(x0), mass=_p(mass), v_fric=_p(fv), e_fric=_p(fe),
                       f_fric=_p(ff), friction_mode=friction_mode,
                       friction_eps=friction_eps,
                       el_rp=_p(el_rp) if el_rp is not None else None,
                       el_col=_p(el_col) if el_col is not None else None,
-                      el_val=_p(el_val) if el_val is not None else None)
+                      el_val=_p(el_val) if el_val is not None else None,
+                      sm=ctypes.addressof(ss) if static is not None else None)
     pr = np.ascontiguousarray(pairs).view(np.uint32).reshape(-1, 4).copy()
     grad = np.zeros(3 * int(m["nv"]), np.float32)
     cap = 16 * max(1, len(pr))
@@ -262,7 +264,8 @@ def contact_assemble(m, x0, mass, fric, pairs, friction_mode=0, friction_eps=1e-
 class StaticMesh(ctypes.Structure):
     _fields_ = [("nv", u32), ("ne", u32), ("nf", u32), ("x", vp), ("edge", vp),
                 ("face", vp), ("v_ghat", vp), ("v_off", vp), ("e_ghat", vp),
-                ("e_off", vp), ("f_ghat", vp), ("f_off", vp)]
+                ("e_off", vp), ("f_ghat", vp), ("f_off", vp), ("v_fric", vp),
+                ("e_fric", vp), ("f_fric", vp)]
 
 
 class LsParams(ctypes.Structure):
@@ -302,9 +305,13 @@ def static_mesh_struct(sm):
         return StaticMesh(nv=0, ne=0, nf=0), keep
     fields = {"nv": int(sm["nv"]), "ne": int(sm["ne"]), "nf": int(sm["nf"])}
     for name in ("x", "edge", "face", "v_ghat", "v_off", "e_ghat", "e_off",
-                 "f_ghat", "f_off"):
+                 "f_ghat", "f_off", "v_fric", "e_fric", "f_fric"):
+        if name not in sm:
+            continue
         dt = np.uint32 if name in ("edge", "face") else np.float32
         a = np.ascontiguousarray(sm[name], dtype=dt)
+        if a.size == 0:
+            a = np.zeros(1, dt)
         keep[name] = a
         fields[name] = _p(a)
     return StaticMesh(**fields), keep
@@ -468,3 +475,25 @@ def elastic_assemble(sh, rp, ci, dt, air_density=1e-3, air_friction=0.2, wind=(0
     miss = lib.oracle_elastic_assemble(ctypes.byref(s), ctypes.byref(p), _p(rp), _p(ci),
                                        _p(force), _p(cdiag), _p(val))
     return force, cdiag, val[:9 * len(ci)].reshape(-1, 9), int(miss)
+
+
+def static_proximity(m, sm, vmass, emass, fmass):
+    """Collision-mesh proximity pairs (kinds 4-6), queries in index order."""
+    lib = _load_narrow()
+    if not hasattr(lib, "_static_ready"):
+        lib.oracle_static_proximity.argtypes = [ctypes.POINTER(ContactMesh),
+                                                ctypes.POINTER(StaticMesh), vp, vp, vp, vp,
+                                                ctypes.c_uint64]
+        lib.oracle_static_proximity.restype = ctypes.c_uint64
+        lib._static_ready = True
+    s, keep = _mesh_struct(m)
+    ss, keep2 = static_mesh_struct(sm)
+    vm, em, fm = _f32(vmass), _f32(emass), _f32(fmass)
+    cap = 1 << 14
+    while True:
+        out = np.zeros(cap, PAIR_DTYPE)
+        n = lib.oracle_static_proximity(ctypes.byref(s), ctypes.byref(ss), _p(vm), _p(em),
+                                        _p(fm), out.ctypes.data_as(vp), cap)
+        if n <= cap:
+            return out[:n]
+        cap = int(n)

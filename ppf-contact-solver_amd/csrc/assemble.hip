@@ -13,6 +13,7 @@
 // count * w_i * w_j * H with one symmetric 3x3 H, so a record stores only the
 // pair and the two positions.
 #include "assemble.hpp"
+#include "ccd.hpp"
 #include "distance.hpp"
 #include "scan.hpp"
 #include "sort.hpp"
@@ -86,8 +87,191 @@ __device__ inline void lookup(uint32_t i, uint32_t j, const uint32_t *rp,
     }
 }
 
+
+// Friction (energy/model/friction.hpp:19-81) on the pair force f and Hessian
+// H: normal force -n.f, tangential displacement of dx, kinetic above eps.
+__device__ inline void add_friction(f3 nrm, f3 dx, float mu, float eps, f3 &f, float H[9]) {
+    const float contact = -dot3(nrm, f);
+    const float ndx = dot3(nrm, dx);
+    const f3 u = dx - mk3(nrm.x * ndx, nrm.y * ndx, nrm.z * ndx);
+    const float un = sqrtf(dot3(u, u));
+    const bool kinetic = un > eps;
+    const f3 sd = kinetic ? mk3(u.x / un, u.y / un, u.z / un) : mk3(0, 0, 0);
+    const float lam = mu > 0.0f ? mu * contact / fmaxf(eps, un) : 0.0f;
+    f = f + lam * u;
+    const float na[3] = {nrm.x, nrm.y, nrm.z};
+    if (kinetic) {
+        const f3 wd = cross3(nrm, sd);
+        const float wa[3] = {wd.x, wd.y, wd.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                H[r + 3 * c] = H[r + 3 * c] + lam * (wa[r] * wa[c]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const float id = r == c ? 1.0f : 0.0f;
+                H[r + 3 * c] = H[r + 3 * c] + lam * (id - na[r] * na[c]);
+            }
+        }
+    }
+}
+
+// Push barrier of a collision-mesh pair (energy/model/push.hpp:26-46):
+// f = k * (-d^2 / eps) n and H = k * ((-2 d / eps) n) n^T for d = e.n < 0.
+__device__ inline void push_terms(f3 e, f3 n, float eps, float k, f3 &f, float H[9]) {
+    const float d = dot3(e, n);
+    f3 g = mk3(0.0f, 0.0f, 0.0f);
+    float cv = 0.0f;
+    if (d < 0.0f) {
+        g = (-(d * d) / eps) * n;
+        cv = -2.0f * d / eps;
+    }
+    f = k * g;
+    const f3 cn = cv * n;
+    const float ca[3] = {cn.x, cn.y, cn.z}, na[3] = {n.x, n.y, n.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            H[r + 3 * c] = k * (ca[r] * na[c]);
+        }
+    }
+}
+
+__device__ inline f3 normalized(f3 e) {
+    const float n = sqrtf(dot3(e, e));
+    return n > 0.0f ? mk3(e.x / n, e.y / n, e.z / n) : e;
+}
+
+// Stiffness (K ne).ne / |ne|^2 of the N vertices idx with weights c along n,
+// K = elastic blocks + mass / gap^2 on the diagonal blocks.
+template <int N>
+__device__ float ext_stiffness(const ppf_contact_params &cp, const uint32_t *idx,
+                               const float *c, f3 n, float gap2) {
+    const float na[3] = {n.x, n.y, n.z};
+    float ne[3 * N];
+#pragma unroll
+    for (int j = 0; j < 3 * N; ++j) ne[j] = c[j / 3] * na[j % 3];
+    float Kn[3 * N];
+#pragma unroll
+    for (int j = 0; j < 3 * N; ++j) Kn[j] = 0.0f;
+    for (int jj = 0; jj < N; ++jj) {
+        for (int ii = 0; ii < N; ++ii) {
+            float blk[9];
+            lookup(idx[ii], idx[jj], cp.el_rp, cp.el_col, cp.el_val, blk);
+            const float sd = ii == jj ? cp.mass[idx[ii]] / gap2 : 0.0f;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                blk[e] = blk[e] + ((e % 4 == 0) ? sd : 0.0f);
+            }
+            // block (ii, jj) occupies rows 3ii.., cols 3jj..; K ne accumulates
+            // over the 3N columns in order
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) Kn[3 * ii + r] = Kn[3 * ii + r] + blk[r + 3 * cc] * ne[3 * jj + cc];
+        }
+    }
+    float num = 0.0f, den = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 3 * N; ++j) num = num + Kn[j] * ne[j];
+#pragma unroll
+    for (int j = 0; j < 3 * N; ++j) den = den + ne[j] * ne[j];
+    return num / den;
+}
+
+// Collision-mesh pairs (contact.cu:899-1152): kind 4 dynamic vertex a vs
+// static face b, 5 static vertex a vs dynamic face b, 6 dynamic edge a vs
+// static edge b.
+__device__ void eval_static(const ppf_contact_mesh &m, const ppf_contact_params &cp,
+                            const ppf_static_mesh &sm, uint32_t kind, uint32_t a, uint32_t b,
+                            uint32_t idx[4], float w[4], int &N, f3 &f, float H[9]) {
+    const f3 zero = mk3(0.0f, 0.0f, 0.0f);
+    if (kind == 4) {
+        const f3 p = X(m.x, a);
+        const f3 q = X(cp.x0, a) - p;
+        const uint32_t f0 = sm.face[3 * (size_t)b], f1 = sm.face[3 * (size_t)b + 1],
+                       f2 = sm.face[3 * (size_t)b + 2];
+        const f3 t0 = X(sm.x, f0) - p, t1 = X(sm.x, f1) - p, t2 = X(sm.x, f2) - p;
+        float c[3];
+        pt_coeff_unclassified(zero, t0, t1, t2, c);
+        const f3 y = (c[0] * t0 + c[1] * t1) + c[2] * t2;
+        const f3 e = mk3(-y.x, -y.y, -y.z);
+        const float off = m.v_off[a] + sm.f_off[b];
+        const float gh = 0.5f * (m.v_ghat[a] + sm.f_ghat[b]);
+        const float mu = combine(cp.v_fric[a], sm.f_fric[b], cp.friction_mode);
+        const f3 n = normalized(e);
+        const f3 proj = y + (off + gh) * n;
+        const float g = sqrtf(dot3(e, e)) - off;
+        const float gap2 = g * g;
+        float blk[9];
+        lookup(a, a, cp.el_rp, cp.el_col, cp.el_val, blk);
+        const float k = dot3(n, mv(blk, n)) + cp.mass[a] / gap2;
+        push_terms(mk3(-proj.x, -proj.y, -proj.z), n, gh, k, f, H);
+        add_friction(n, mk3(-q.x, -q.y, -q.z), mu, cp.friction_eps, f, H);
+        N = 1;
+        idx[0] = a;
+        w[0] = 1.0f;
+    } else if (kind == 5) {
+        const f3 yv = X(sm.x, a);
+        const uint32_t fc[3] = {m.face[3 * (size_t)b], m.face[3 * (size_t)b + 1],
+                                m.face[3 * (size_t)b + 2]};
+        const f3 t0 = X(m.x, fc[0]) - yv, t1 = X(m.x, fc[1]) - yv, t2 = X(m.x, fc[2]) - yv;
+        const f3 s0 = X(cp.x0, fc[0]) - yv, s1 = X(cp.x0, fc[1]) - yv,
+                 s2 = X(cp.x0, fc[2]) - yv;
+        float c[3];
+        pt_coeff_unclassified(zero, t0, t1, t2, c);
+        const f3 p = (c[0] * t0 + c[1] * t1) + c[2] * t2;
+        const f3 q = (c[0] * s0 + c[1] * s1) + c[2] * s2;
+        const float off = m.f_off[b] + sm.v_off[a];
+        const float gh = 0.5f * (m.f_ghat[b] + sm.v_ghat[a]);
+        const float mu = combine(cp.f_fric[b], sm.v_fric[a], cp.friction_mode);
+        const f3 n = normalized(p);
+        const float g = sqrtf(dot3(p, p)) - off;
+        const float gap2 = g * g;
+        const float k = ext_stiffness<3>(cp, fc, c, n, gap2);
+        const f3 proj = (off + gh) * n;
+        push_terms(p - proj, n, gh, k, f, H);
+        add_friction(n, p - q, mu, cp.friction_eps, f, H);
+        N = 3;
+        idx[0] = fc[0], idx[1] = fc[1], idx[2] = fc[2];
+        w[0] = c[0], w[1] = c[1], w[2] = c[2];
+    } else {
+        const uint32_t e2[2] = {m.edge[2 * (size_t)a], m.edge[2 * (size_t)a + 1]};
+        const f3 p0 = X(m.x, e2[0]), p1 = X(m.x, e2[1]);
+        const f3 q0 = X(sm.x, sm.edge[2 * (size_t)b]), q1 = X(sm.x, sm.edge[2 * (size_t)b + 1]);
+        const f3 b0 = X(cp.x0, e2[0]), b1 = X(cp.x0, e2[1]);
+        float c[4];
+        ee_coeff(p0, p1, q0, q1, c);
+        const f3 x = c[0] * p0 + c[1] * p1;
+        const f3 y = c[2] * q0 + c[3] * q1;
+        const f3 z = c[0] * b0 + c[1] * b1;
+        const f3 e = x - y;
+        const float off = m.e_off[a] + sm.e_off[b];
+        const float gh = 0.5f * (m.e_ghat[a] + sm.e_ghat[b]);
+        const float mu = combine(cp.e_fric[a], sm.e_fric[b], cp.friction_mode);
+        const f3 n = normalized(e);
+        const f3 proj = (off + gh) * n;
+        const float g = sqrtf(dot3(e, e)) - off;
+        const float gap2 = g * g;
+        const float k = ext_stiffness<2>(cp, e2, c, n, gap2);
+        push_terms(e - proj, n, gh, k, f, H);
+        add_friction(n, x - z, mu, cp.friction_eps, f, H);
+        N = 2;
+        idx[0] = e2[0], idx[1] = e2[1];
+        w[0] = c[0], w[1] = c[1];
+    }
+}
+
 __global__ __launch_bounds__(kNT) void k_pair_eval(ppf_contact_mesh m,
                                                    ppf_contact_params cp,
+                                                   ppf_static_mesh sm,
                                                    const ppf_pair *__restrict__ pairs,
                                                    uint64_t npairs,
                                                    PairEval *__restrict__ out) {
@@ -102,6 +286,26 @@ __global__ __launch_bounds__(kNT) void k_pair_eval(ppf_contact_mesh m,
     uint32_t idx[4] = {0, 0, 0, 0};
     float w[4] = {0, 0, 0, 0};
     int N;
+    if (kind >= 4) {
+        f3 f;
+        float H[9];
+        eval_static(m, cp, sm, kind, a, b, idx, w, N, f, H);
+        PairEval pe;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            pe.idx[k] = k < N ? idx[k] : kInvalid;
+            pe.w[k] = w[k];
+        }
+        pe.f[0] = f.x, pe.f[1] = f.y, pe.f[2] = f.z;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            pe.H[k] = H[k];
+        }
+        pe.cnt = (float)pr.count;
+        pe.n = (uint32_t)N;
+        out[pi] = pe;
+        return;
+    }
     float gh, off, mu;
     if (kind == 0) {
         const uint32_t f0 = m.face[3 * (size_t)b], f1 = m.face[3 * (size_t)b + 1],
@@ -240,36 +444,7 @@ __global__ __launch_bounds__(kNT) void k_pair_eval(ppf_contact_mesh m,
         }
     }
     if (!no_fric) {
-        // Friction (friction.hpp:19-81).
-        const float contact = -dot3(nrm, f);
-        const float ndx = dot3(nrm, dx);
-        const f3 u = dx - mk3(nrm.x * ndx, nrm.y * ndx, nrm.z * ndx);
-        const float un = sqrtf(dot3(u, u));
-        const bool kinetic = un > cp.friction_eps;
-        const f3 sd = kinetic ? mk3(u.x / un, u.y / un, u.z / un) : mk3(0, 0, 0);
-        const float lam = mu > 0.0f ? mu * contact / fmaxf(cp.friction_eps, un) : 0.0f;
-        f = f + lam * u;
-        const float na[3] = {nrm.x, nrm.y, nrm.z};
-        if (kinetic) {
-            const f3 wd = cross3(nrm, sd);
-            const float wa[3] = {wd.x, wd.y, wd.z};
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    H[r + 3 * c] = H[r + 3 * c] + lam * (wa[r] * wa[c]);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const float id = r == c ? 1.0f : 0.0f;
-                    H[r + 3 * c] = H[r + 3 * c] + lam * (id - na[r] * na[c]);
-                }
-            }
-        }
+        add_friction(nrm, dx, mu, cp.friction_eps, f, H);
     }
     PairEval pe;
 #pragma unroll
@@ -492,7 +667,11 @@ int ContactAssembler::assemble(const ppf_contact_mesh &m, const ppf_contact_para
     uint32_t *sortw = vstart + (nv + 1);
     uint32_t *scanw = sortw + radix_sort_scratch_words(nh);
 
-    k_pair_eval<<<cdiv64(npairs, kNT), kNT, 0, s>>>(m, cp, pairs, npairs, pe_);
+    ppf_static_mesh sm{};
+    if (cp.sm) {
+        sm = *cp.sm;
+    }
+    k_pair_eval<<<cdiv64(npairs, kNT), kNT, 0, s>>>(m, cp, sm, pairs, npairs, pe_);
     PPF_HIP(hipGetLastError());
     k_records<<<cdiv64(npairs, kNT), kNT, 0, s>>>(pe_, npairs, fkey, fval, hrow,
                                                   hcol, hval);

@@ -274,6 +274,31 @@ int ppf_narrow_proximity(ppf_narrow *h, const ppf_contact_mesh *m,
     return h->impl.proximity(*m, d_out, cap, count, (hipStream_t)stream);
 }
 
+int ppf_narrow_static(ppf_narrow *h, const ppf_contact_mesh *m,
+                      const struct ppf_static_mesh *sm, const float *d_vmass,
+                      const float *d_emass, const float *d_fmass, ppf_pair *d_out,
+                      uint64_t cap, uint64_t *count, void *stream) {
+    if (!h || !m || !count || !d_vmass || !d_emass || !d_fmass) {
+        set_error("ppf_narrow_static: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    if (h->impl.tree(0).view().n != m->nf || h->impl.tree(1).view().n != m->ne ||
+        h->impl.tree(2).view().n != m->nv) {
+        set_error("ppf_narrow_static: trees were built for another mesh");
+        return PPF_ERR_ARG;
+    }
+    if (!sm || sm->nv == 0) {
+        *count = 0;
+        return PPF_OK;
+    }
+    if (!sm->x || (sm->nf && !sm->face) || (sm->ne && !sm->edge)) {
+        set_error("ppf_narrow_static: static mesh arrays missing");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.static_proximity(*m, *sm, d_vmass, d_emass, d_fmass, d_out, cap, count,
+                                    (hipStream_t)stream);
+}
+
 int ppf_intersection_check(ppf_narrow *h, const ppf_contact_mesh *m,
                            const struct ppf_static_mesh *sm, const float *d_vmass,
                            const float *d_emass, const float *d_fmass,
@@ -315,6 +340,12 @@ int ppf_contact_assemble(ppf_contact *h, const ppf_contact_mesh *m,
     if (!h || !m || !p || !d_grad || !nnz || (npairs && !d_pairs) || !p->x0 ||
         !p->mass || !p->v_fric || !p->e_fric || !p->f_fric) {
         set_error("ppf_contact_assemble: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    if (p->sm && p->sm->nv &&
+        (!p->sm->x || !p->sm->v_fric || (p->sm->ne && (!p->sm->edge || !p->sm->e_fric)) ||
+         (p->sm->nf && (!p->sm->face || !p->sm->f_fric)))) {
+        set_error("ppf_contact_assemble: collision mesh arrays missing");
         return PPF_ERR_ARG;
     }
     int rc = h->impl.assemble(*m, *p, d_pairs, npairs, d_grad, (hipStream_t)stream);

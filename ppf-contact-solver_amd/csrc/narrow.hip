@@ -9,6 +9,7 @@
 // Children are tested before they are pushed. Two passes -- count, exclusive
 // scan, fill -- give every query a contiguous, deterministic output range.
 #include "bvh_query.hpp"
+#include "ccd.hpp"
 #include "distance.hpp"
 #include "narrow.hpp"
 #include "scan.hpp"
@@ -262,6 +263,149 @@ __global__ __launch_bounds__(kQT) void k_edge_queries(Ctx c) {
     }
 }
 
+// ---- collision-mesh pairs (contact.cu:899-1152, 1789-1954) -------------------
+
+struct SCtx {
+    ppf_contact_mesh m;
+    ppf_static_mesh sm;
+    const float *vmass, *emass, *fmass;
+    BvhView fb, eb, vb, sfb, seb;
+    uint32_t *cnt; // per query: vertex leaves, static vertices, edge leaves
+    const uint32_t *ofs;
+    ppf_pair *out;
+    uint64_t cap;
+    uint32_t *overflow;
+};
+
+template <bool FILL>
+__device__ inline void emit_s(const SCtx &c, uint64_t &slot, uint32_t kind, uint32_t a,
+                              uint32_t b) {
+    if (FILL) {
+        if (slot < c.cap) {
+            c.out[slot] = ppf_pair{kind, a, b, 1};
+        } else {
+            atomicOr(c.overflow + 1, 1u);
+        }
+        ++slot;
+    }
+}
+
+__device__ inline f3 SX(const ppf_static_mesh &sm, uint32_t v) {
+    return load3(sm.x + 3 * (size_t)v);
+}
+
+// Dynamic vertex vs static face (kind 4), one thread per vertex leaf.
+template <bool FILL> __global__ __launch_bounds__(kQT) void k_m2c(SCtx c) {
+    __shared__ uint32_t stk_all[kStack * kQT];
+    const uint32_t qpos = blockIdx.x * kQT + threadIdx.x;
+    if (qpos >= c.vb.n) {
+        return;
+    }
+    uint32_t *stk = stk_all + threadIdx.x;
+    const ppf_contact_mesh &m = c.m;
+    const uint32_t i = c.vb.node[2 * (size_t)qpos] - 1;
+    uint64_t slot = FILL ? (uint64_t)c.ofs[qpos] : 0;
+    uint32_t total = 0;
+    if ((m.vflag[i] & 1u) == 0 && c.vmass[i] > 0.0f) {
+        const f3 p = X(m, i);
+        const QBox q = point_box(p, 0.5f * m.v_ghat[i] + m.v_off[i]);
+        total = traverse(c.sfb, q, stk, c.overflow, [&](uint32_t fi) {
+            const uint32_t f0 = c.sm.face[3 * (size_t)fi], f1 = c.sm.face[3 * (size_t)fi + 1],
+                           f2 = c.sm.face[3 * (size_t)fi + 2];
+            const f3 t0 = SX(c.sm, f0) - p, t1 = SX(c.sm, f1) - p, t2 = SX(c.sm, f2) - p;
+            float w[3];
+            pt_coeff_unclassified(mk3(0.0f, 0.0f, 0.0f), t0, t1, t2, w);
+            const f3 y = (w[0] * t0 + w[1] * t1) + w[2] * t2;
+            const float off = m.v_off[i] + c.sm.f_off[fi];
+            const float gh = 0.5f * (m.v_ghat[i] + c.sm.f_ghat[fi]);
+            const float r = off + gh;
+            if (dot3(y, y) < r * r) {
+                emit_s<FILL>(c, slot, 4, i, fi);
+                return true;
+            }
+            return false;
+        });
+    }
+    if (!FILL) {
+        c.cnt[qpos] = total;
+    }
+}
+
+// Static vertex vs dynamic face (kind 5), one thread per static vertex.
+template <bool FILL> __global__ __launch_bounds__(kQT) void k_c2m(SCtx c) {
+    __shared__ uint32_t stk_all[kStack * kQT];
+    const uint32_t sv = blockIdx.x * kQT + threadIdx.x;
+    if (sv >= c.sm.nv) {
+        return;
+    }
+    uint32_t *stk = stk_all + threadIdx.x;
+    const ppf_contact_mesh &m = c.m;
+    const uint32_t qpos = c.vb.n + sv;
+    uint64_t slot = FILL ? (uint64_t)c.ofs[qpos] : 0;
+    const f3 y = SX(c.sm, sv);
+    const QBox q = point_box(y, 0.5f * c.sm.v_ghat[sv] + c.sm.v_off[sv]);
+    const uint32_t total = traverse(c.fb, q, stk, c.overflow, [&](uint32_t fi) {
+        if ((m.fflag[fi] & 1u) != 0 || !(c.fmass[fi] > 0.0f)) {
+            return false;
+        }
+        const uint32_t f0 = m.face[3 * (size_t)fi], f1 = m.face[3 * (size_t)fi + 1],
+                       f2 = m.face[3 * (size_t)fi + 2];
+        const f3 t0 = X(m, f0) - y, t1 = X(m, f1) - y, t2 = X(m, f2) - y;
+        float w[3];
+        pt_coeff_unclassified(mk3(0.0f, 0.0f, 0.0f), t0, t1, t2, w);
+        const f3 e = (w[0] * t0 + w[1] * t1) + w[2] * t2;
+        const float off = m.f_off[fi] + c.sm.v_off[sv];
+        const float gh = 0.5f * (m.f_ghat[fi] + c.sm.v_ghat[sv]);
+        const float r = off + gh;
+        if (dot3(e, e) < r * r) {
+            emit_s<FILL>(c, slot, 5, sv, fi);
+            return true;
+        }
+        return false;
+    });
+    if (!FILL) {
+        c.cnt[qpos] = total;
+    }
+}
+
+// Dynamic edge vs static edge (kind 6), one thread per edge leaf.
+template <bool FILL> __global__ __launch_bounds__(kQT) void k_cm_ee(SCtx c) {
+    __shared__ uint32_t stk_all[kStack * kQT];
+    const uint32_t lp = blockIdx.x * kQT + threadIdx.x;
+    if (lp >= c.eb.n) {
+        return;
+    }
+    uint32_t *stk = stk_all + threadIdx.x;
+    const ppf_contact_mesh &m = c.m;
+    const uint32_t qpos = c.vb.n + c.sm.nv + lp;
+    const uint32_t ei = c.eb.node[2 * (size_t)lp] - 1;
+    uint64_t slot = FILL ? (uint64_t)c.ofs[qpos] : 0;
+    uint32_t total = 0;
+    if ((m.eflag[ei] & 1u) == 0 && c.emass[ei] > 0.0f) {
+        const uint32_t a0 = m.edge[2 * (size_t)ei], a1 = m.edge[2 * (size_t)ei + 1];
+        const f3 p0 = X(m, a0), p1 = X(m, a1);
+        const QBox q = seg_box(p0, p1, 0.5f * m.e_ghat[ei] + m.e_off[ei]);
+        total = traverse(c.seb, q, stk, c.overflow, [&](uint32_t sj) {
+            const uint32_t b0 = c.sm.edge[2 * (size_t)sj], b1 = c.sm.edge[2 * (size_t)sj + 1];
+            const f3 q0 = SX(c.sm, b0), q1 = SX(c.sm, b1);
+            float w[4];
+            ee_coeff(p0, p1, q0, q1, w);
+            const f3 e = (w[0] * p0 + w[1] * p1) - (w[2] * q0 + w[3] * q1);
+            const float off = m.e_off[ei] + c.sm.e_off[sj];
+            const float gh = 0.5f * (m.e_ghat[ei] + c.sm.e_ghat[sj]);
+            const float r = off + gh;
+            if (dot3(e, e) < r * r) {
+                emit_s<FILL>(c, slot, 6, ei, sj);
+                return true;
+            }
+            return false;
+        });
+    }
+    if (!FILL) {
+        c.cnt[qpos] = total;
+    }
+}
+
 uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 } // namespace
@@ -357,6 +501,102 @@ int Narrow::proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,
     PPF_HIP(hipStreamSynchronize(s));
     if (h_flags_[0]) {
         set_error("proximity: a BVH traversal exceeded its %d-entry stack", kStack);
+        return PPF_ERR_CAPACITY;
+    }
+    return PPF_OK;
+}
+
+int Narrow::build_static(const ppf_static_mesh &sm, hipStream_t s) {
+    if (sm.nv == 0) {
+        return PPF_OK;
+    }
+    if (static_x_ == sm.x && sface_.view().n == sm.nf && sedge_.view().n == sm.ne &&
+        static_nv_ == sm.nv) {
+        return PPF_OK;
+    }
+    PrimSet f, e;
+    f.n = sm.nf, f.arity = 3, f.elem = sm.face, f.ghat = sm.f_ghat, f.offset = sm.f_off;
+    e.n = sm.ne, e.arity = 2, e.elem = sm.edge, e.ghat = sm.e_ghat, e.offset = sm.e_off;
+    int rc = sface_.build(f, sm.x, nullptr, 0.0f, s);
+    if (rc == PPF_OK) {
+        rc = sedge_.build(e, sm.x, nullptr, 0.0f, s);
+    }
+    if (rc == PPF_OK) {
+        static_x_ = sm.x;
+        static_nv_ = sm.nv;
+    }
+    return rc;
+}
+
+int Narrow::static_proximity(const ppf_contact_mesh &m, const ppf_static_mesh &sm,
+                             const float *vmass, const float *emass, const float *fmass,
+                             ppf_pair *out, uint64_t cap, uint64_t *count, hipStream_t s) {
+    *count = 0;
+    if (sm.nv == 0) {
+        return PPF_OK;
+    }
+    int rc = build_static(sm, s);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    const uint32_t nq = m.nv + sm.nv + m.ne;
+    rc = reserve(nq);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    PPF_HIP(hipMemsetAsync(flags_, 0, 4 * sizeof(uint32_t), s));
+    SCtx c;
+    c.m = m;
+    c.sm = sm;
+    c.vmass = vmass;
+    c.emass = emass;
+    c.fmass = fmass;
+    c.fb = face_.view();
+    c.eb = edge_.view();
+    c.vb = vert_.view();
+    c.sfb = sface_.view();
+    c.seb = sedge_.view();
+    c.cnt = cnt_;
+    c.ofs = ofs_;
+    c.out = out;
+    c.cap = cap;
+    c.overflow = flags_;
+    auto launch = [&](bool fill) -> int {
+        if (m.nv) {
+            if (fill) k_m2c<true><<<cdiv(m.nv, kQT), kQT, 0, s>>>(c);
+            else k_m2c<false><<<cdiv(m.nv, kQT), kQT, 0, s>>>(c);
+            PPF_HIP(hipGetLastError());
+        }
+        if (fill) k_c2m<true><<<cdiv(sm.nv, kQT), kQT, 0, s>>>(c);
+        else k_c2m<false><<<cdiv(sm.nv, kQT), kQT, 0, s>>>(c);
+        PPF_HIP(hipGetLastError());
+        if (m.ne) {
+            if (fill) k_cm_ee<true><<<cdiv(m.ne, kQT), kQT, 0, s>>>(c);
+            else k_cm_ee<false><<<cdiv(m.ne, kQT), kQT, 0, s>>>(c);
+            PPF_HIP(hipGetLastError());
+        }
+        return PPF_OK;
+    };
+    rc = launch(false);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    PPF_HIP(hipMemsetAsync(cnt_ + nq, 0, sizeof(uint32_t), s));
+    PPF_HIP(exclusive_scan_u32(cnt_, ofs_, nq + 1, scratch_, s));
+    uint32_t total = 0;
+    PPF_HIP(hipMemcpyAsync(&total, ofs_ + nq, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipStreamSynchronize(s));
+    *count = total;
+    if (out && cap) {
+        rc = launch(true);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+    }
+    PPF_HIP(hipMemcpyAsync(h_flags_, flags_, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipStreamSynchronize(s));
+    if (h_flags_[0]) {
+        set_error("static proximity: a BVH traversal exceeded its %d-entry stack", kStack);
         return PPF_ERR_CAPACITY;
     }
     return PPF_OK;

@@ -16,10 +16,18 @@ class Narrow {
     int proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,
                   uint64_t *count, hipStream_t s);
     Lbvh &tree(int kind) { return kind == 0 ? face_ : (kind == 1 ? edge_ : vert_); }
+    // Collision-mesh pairs (kinds 4, 5, 6) at the positions of the last
+    // build_trees; the static trees are rebuilt when the static mesh changes.
+    int static_proximity(const ppf_contact_mesh &m, const ppf_static_mesh &sm,
+                         const float *vmass, const float *emass, const float *fmass,
+                         ppf_pair *out, uint64_t cap, uint64_t *count, hipStream_t s);
 
   private:
     int reserve(uint32_t queries);
-    Lbvh face_, edge_, vert_;
+    int build_static(const ppf_static_mesh &sm, hipStream_t s);
+    Lbvh face_, edge_, vert_, sface_, sedge_;
+    const float *static_x_ = nullptr;
+    uint32_t static_nv_ = 0;
     uint32_t qcap_ = 0;
     uint32_t *cnt_ = nullptr, *ofs_ = nullptr, *scratch_ = nullptr;
     uint32_t *flags_ = nullptr, *h_flags_ = nullptr;

@@ -51,6 +51,8 @@ _PROTOS = {
     "ppf_contact_blocks": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.POINTER(ctypes.c_uint64)]),
+    "ppf_narrow_static": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, ctypes.c_uint64,
+                                         ctypes.POINTER(ctypes.c_uint64), vp]),
     "ppf_intersection_check": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, u32,
                                               ctypes.POINTER(u32), vp]),
     "ppf_elastic_create": (ctypes.c_int, [vp, vp, ctypes.POINTER(vp)]),

@@ -17,6 +17,7 @@ u32 = ctypes.c_uint32
 f32 = ctypes.c_float
 
 STATIC_FLOAT = ("x", "v_ghat", "v_off", "e_ghat", "e_off", "f_ghat", "f_off")
+STATIC_FRIC = ("v_fric", "e_fric", "f_fric")
 STATIC_INDEX = ("edge", "face")
 KIND_COLUMNS = {0: 4, 1: 3, 2: 2, 3: 4}
 
@@ -25,7 +26,8 @@ class StaticMeshC(ctypes.Structure):
     """ppf_static_mesh (include/ppf_mi355x.h)."""
     _fields_ = [("nv", u32), ("ne", u32), ("nf", u32), ("x", vp), ("edge", vp),
                 ("face", vp), ("v_ghat", vp), ("v_off", vp), ("e_ghat", vp),
-                ("e_off", vp), ("f_ghat", vp), ("f_off", vp)]
+                ("e_off", vp), ("f_ghat", vp), ("f_off", vp), ("v_fric", vp),
+                ("e_fric", vp), ("f_fric", vp)]
 
 
 class LsParamsC(ctypes.Structure):
@@ -40,6 +42,9 @@ def static_to_device(sm, dev):
     out = {"nv": int(sm["nv"]), "ne": int(sm["ne"]), "nf": int(sm["nf"])}
     for k in STATIC_FLOAT:
         out[k] = torch.from_numpy(np.ascontiguousarray(sm[k], np.float32)).reshape(-1).to(dev)
+    for k in STATIC_FRIC:
+        if k in sm:
+            out[k] = torch.from_numpy(np.ascontiguousarray(sm[k], np.float32)).reshape(-1).to(dev)
     for k in STATIC_INDEX:
         a = np.ascontiguousarray(sm[k], np.uint32).view(np.int32)
         out[k] = torch.from_numpy(a).reshape(-1).to(dev)
@@ -50,7 +55,9 @@ def static_struct(t):
     if t is None:
         return StaticMeshC(nv=0, ne=0, nf=0)
     fields = {"nv": t["nv"], "ne": t["ne"], "nf": t["nf"]}
-    for k in STATIC_FLOAT + STATIC_INDEX:
+    for k in STATIC_FLOAT + STATIC_INDEX + STATIC_FRIC:
+        if k not in t:
+            continue
         ten = t[k]
         if not ten.is_cuda or not ten.is_contiguous():
             raise ValueError(f"{k}: expected a contiguous cuda tensor")

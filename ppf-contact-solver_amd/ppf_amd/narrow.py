@@ -125,6 +125,22 @@ class NarrowPhase:
             raise _lib.PPFError(f"pair buffer too small: {c.value} > {cap}")
         return out[:c.value]
 
+    def static_pairs(self, t, static, vmass, emass, fmass):
+        """Collision-mesh pairs (kinds 4-6) as a (k, 4) int32 device tensor."""
+        from .ccd import static_struct
+        s = mesh_struct(t)
+        ss = static_struct(static)
+        c = ctypes.c_uint64()
+        args = (vp(vmass.data_ptr()), vp(emass.data_ptr()), vp(fmass.data_ptr()))
+        check(self.lib.ppf_narrow_static(self._h, ctypes.byref(s), ctypes.byref(ss), *args,
+                                         None, 0, ctypes.byref(c), _stream()),
+              "ppf_narrow_static")
+        out = torch.empty((max(1, c.value), 4), dtype=torch.int32, device=t["x"].device)
+        check(self.lib.ppf_narrow_static(self._h, ctypes.byref(s), ctypes.byref(ss), *args,
+                                         vp(out.data_ptr()), out.shape[0], ctypes.byref(c),
+                                         _stream()), "ppf_narrow_static")
+        return out[:c.value]
+
     def intersection_check(self, t, vmass, emass, fmass, static=None, surface=None,
                            cap=256):
         """Intersection check at t['x'] (build() at the same positions first):
@@ -159,7 +175,7 @@ class ContactParamsC(ctypes.Structure):
     _fields_ = [("x0", vp), ("mass", vp), ("v_fric", vp), ("e_fric", vp),
                 ("f_fric", vp), ("friction_mode", ctypes.c_int32),
                 ("friction_eps", ctypes.c_float), ("el_rp", vp), ("el_col", vp),
-                ("el_val", vp)]
+                ("el_val", vp), ("sm", vp)]
 
 
 def _ptr(t):
@@ -187,18 +203,22 @@ class ContactAssembler:
             pass
 
     def assemble(self, t, x0, mass, fric, pairs, friction_mode=0, friction_eps=1e-5,
-                 elastic=None):
+                 elastic=None, static=None):
         """t: device contact mesh (x = Newton iterate); x0, mass: device
         tensors; fric: dict v/e/f -> device tensors; pairs: (k, 4) int32 device
-        tensor; elastic: (row_offset, col, val) device tensors or None.
+        tensor; elastic: (row_offset, col, val) device tensors or None;
+        static: device collision mesh (with v/e/f_fric) for kind 4-6 pairs.
         Returns (grad, row_offset, rows, cols, vals) as device tensors (the
         block arrays are copies)."""
+        from .ccd import static_struct
         s = mesh_struct(t)
         el = elastic or (None, None, None)
+        ss = static_struct(static) if static is not None else None
         p = ContactParamsC(x0=_ptr(x0), mass=_ptr(mass), v_fric=_ptr(fric["v"]),
                            e_fric=_ptr(fric["e"]), f_fric=_ptr(fric["f"]),
                            friction_mode=friction_mode, friction_eps=friction_eps,
-                           el_rp=_ptr(el[0]), el_col=_ptr(el[1]), el_val=_ptr(el[2]))
+                           el_rp=_ptr(el[0]), el_col=_ptr(el[1]), el_val=_ptr(el[2]),
+                           sm=ctypes.addressof(ss) if ss is not None else None)
         grad = torch.empty(3 * t["nv"], dtype=torch.float32, device=t["x"].device)
         nnz = ctypes.c_uint64()
         check(self.lib.ppf_contact_assemble(self._h, ctypes.byref(s), ctypes.byref(p),

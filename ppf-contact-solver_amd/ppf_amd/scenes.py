@@ -28,7 +28,9 @@ def rect_faces(res_x, res_y):
 
 def unique_edges(faces):
     """Sorted unique undirected edges (E, 2) with e[0] < e[1]."""
-    f = faces.astype(np.int64)
+    f = np.asarray(faces).astype(np.int64).reshape(-1, 3)
+    if f.size == 0:
+        return np.zeros((0, 2), np.uint32)
     e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
     e.sort(axis=1)
     key = np.unique(e[:, 0] * (int(f.max()) + 1) + e[:, 1])
