@@ -353,6 +353,74 @@ int ppf_elastic_pattern(ppf_elastic *h, const uint32_t **d_row_offset,
 int ppf_elastic_assemble(ppf_elastic *h, const ppf_shell *sh, const ppf_elastic_params *p,
                          float *d_force, float *d_cdiag, float *d_value, void *stream);
 
+/* ------------------------------------------------------------------------
+ * One time step: the Newton loop of main_helper::advance.
+ * ---------------------------------------------------------------------- */
+
+/* A scene (device pointers). mesh.x is vertex.curr and prev is vertex.prev:
+ * both are updated in place by a successful advance. The shell carries the
+ * elastic data (its x, x0 and target fields are ignored: the stepper owns the
+ * iterate); shell.vmass is also the contact mass and the pinned flag is
+ * mesh.vflag bit0. collision.nv == 0 means no collision mesh. fix_pos holds
+ * the target position of every pinned vertex (3 nv; may be null when no
+ * vertex is pinned); emass is EdgeProp mass; *_fric the per-element
+ * friction. */
+typedef struct ppf_scene {
+    ppf_contact_mesh mesh;
+    ppf_shell shell;
+    ppf_static_mesh collision;
+    float *prev;
+    const float *emass;
+    const float *v_fric, *e_fric, *f_fric;
+    const float *fix_pos;
+} ppf_scene;
+
+/* The ParamSet fields the step reads (data.hpp:515-568). */
+typedef struct ppf_step_params {
+    float dt, prev_dt;
+    float gravity[3];
+    float cg_tol;
+    uint32_t cg_max_iter;
+    float target_toi, line_search_max_t, ccd_eps, max_dx;
+    float friction_eps;
+    int32_t friction_mode;
+    uint32_t min_newton_steps, max_newton_steps;
+    float air_density, air_friction;
+    float wind[3];
+    int32_t disable_contact;
+    int32_t check_intersection;
+} ppf_step_params;
+
+/* Outcome of one step (StepResult, data.hpp:570-595). status: 0 success,
+ * 1 PCG failed, 2 CCD failed (toi <= FLT_EPSILON), 3 a contact pair starts
+ * inside its offset, 4 intersection after the step, 5 Newton stalled
+ * (max_newton_steps). Times are HIP-event milliseconds on the stream. */
+typedef struct ppf_step_result {
+    int32_t status;
+    uint32_t newton_steps;
+    uint32_t pcg_iter_total, pcg_iter_max;
+    uint32_t pairs_max;
+    uint32_t intersections;
+    float toi_advanced;
+    float dt;
+    float ms_elastic, ms_contact, ms_solve, ms_line_search, ms_total;
+} ppf_step_result;
+
+typedef struct ppf_stepper ppf_stepper;
+
+int ppf_stepper_create(const ppf_scene *scene, void *stream, ppf_stepper **out);
+void ppf_stepper_destroy(ppf_stepper *h);
+
+/* Advance the scene by one step (replaces main_helper::advance,
+ * main/main.cu:267-1649, for shells with fix pins, self contact and a static
+ * collision mesh): inertial target, Newton iterations (elastic + contact
+ * assembly, Dirichlet elimination of pinned vertices, block-Jacobi PCG, CCD
+ * line search, toi accumulation to target_toi, then one error-reduction
+ * iteration at the shortened dt), post-step intersection check, commit of
+ * prev / curr. */
+int ppf_stepper_advance(ppf_stepper *h, const ppf_scene *scene, const ppf_step_params *p,
+                        ppf_step_result *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -497,3 +497,70 @@ def static_proximity(m, sm, vmass, emass, fmass):
         if n <= cap:
             return out[:n]
         cap = int(n)
+
+
+# ---- one time step ------------------------------------------------------------
+
+class OracleScene(ctypes.Structure):
+    _fields_ = [("mesh", ContactMesh), ("shell", Shell), ("collision", StaticMesh),
+                ("prev", vp), ("emass", vp), ("v_fric", vp), ("e_fric", vp), ("f_fric", vp),
+                ("fix_pos", vp)]
+
+
+class OracleStepParams(ctypes.Structure):
+    _fields_ = [("dt", f32), ("prev_dt", f32), ("gravity", f32 * 3), ("cg_tol", f32),
+                ("cg_max_iter", u32), ("target_toi", f32), ("line_search_max_t", f32),
+                ("ccd_eps", f32), ("max_dx", f32), ("friction_eps", f32),
+                ("friction_mode", ctypes.c_int32), ("min_newton_steps", u32),
+                ("max_newton_steps", u32), ("air_density", f32), ("air_friction", f32),
+                ("wind", f32 * 3), ("disable_contact", ctypes.c_int32),
+                ("check_intersection", ctypes.c_int32)]
+
+
+class OracleStepResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("newton_steps", u32), ("pcg_iter_total", u32),
+                ("pcg_iter_max", u32), ("pairs_max", u32), ("intersections", u32),
+                ("toi_advanced", f32), ("dt", f32)]
+
+
+class OracleStepper:
+    """The oracle's time stepping of a scene dict (scenes.drape_scene layout);
+    scene['mesh']['x'] and scene['prev'] are advanced in place."""
+
+    def __init__(self, sc):
+        lib = load()
+        if not hasattr(lib, "_step_ready"):
+            lib.oracle_step.argtypes = [ctypes.POINTER(OracleScene),
+                                        ctypes.POINTER(OracleStepParams),
+                                        ctypes.POINTER(OracleStepResult)]
+            lib.oracle_step.restype = ctypes.c_int
+            lib._step_ready = True
+        self.lib = lib
+        self.sc = sc
+        sc["mesh"]["x"] = np.ascontiguousarray(sc["mesh"]["x"], np.float32)
+        sc["prev"] = np.ascontiguousarray(sc["prev"], np.float32)
+        self.m, self.keep_m = _mesh_struct(sc["mesh"])
+        # _mesh_struct copies x; point it at the live array instead
+        self.m.x = _p(sc["mesh"]["x"])
+        shell = dict(sc["shell"])
+        for k in ("x", "x0", "target"):
+            shell.setdefault(k, np.zeros(1, np.float32))
+        self.s, self.keep_s = shell_struct(shell)
+        self.st, self.keep_st = static_mesh_struct(sc.get("static"))
+        self.keep = [_f32(sc["emass"]), _f32(sc["fric"]["v"]), _f32(sc["fric"]["e"]),
+                     _f32(sc["fric"]["f"])]
+        self.fix = _f32(sc["fix_pos"]).reshape(-1) if sc.get("fix_pos") is not None else None
+        self.scene = OracleScene(mesh=self.m, shell=self.s, collision=self.st,
+                                 prev=_p(sc["prev"]), emass=_p(self.keep[0]),
+                                 v_fric=_p(self.keep[1]), e_fric=_p(self.keep[2]),
+                                 f_fric=_p(self.keep[3]),
+                                 fix_pos=_p(self.fix) if self.fix is not None else None)
+
+    def advance(self, params):
+        kw = dict(params)
+        kw["gravity"] = (f32 * 3)(*params["gravity"])
+        kw["wind"] = (f32 * 3)(*params["wind"])
+        p = OracleStepParams(**kw)
+        r = OracleStepResult()
+        self.lib.oracle_step(ctypes.byref(self.scene), ctypes.byref(p), ctypes.byref(r))
+        return {k: getattr(r, k) for k, _ in OracleStepResult._fields_}

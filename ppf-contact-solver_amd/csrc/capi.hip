@@ -8,6 +8,7 @@
 #include "assemble.hpp"
 #include "linesearch.hpp"
 #include "narrow.hpp"
+#include "newton.hpp"
 #include "pcg.hpp"
 #include <cstdarg>
 #include <cstring>
@@ -91,6 +92,10 @@ struct ppf_ccd {
 
 struct ppf_elastic {
     ElasticAssembler impl;
+};
+
+struct ppf_stepper {
+    Stepper impl;
 };
 
 extern "C" {
@@ -464,6 +469,34 @@ int ppf_elastic_assemble(ppf_elastic *h, const ppf_shell *sh, const ppf_elastic_
         PPF_HIP(hipStreamSynchronize(st));
     }
     return rc;
+}
+
+int ppf_stepper_create(const ppf_scene *scene, void *stream, ppf_stepper **out) {
+    if (!scene || !out || !scene->mesh.x || !scene->prev || !scene->emass ||
+        !scene->v_fric || !scene->e_fric || !scene->f_fric) {
+        set_error("ppf_stepper_create: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    ppf_stepper *h = new ppf_stepper();
+    int rc = h->impl.create(*scene, (hipStream_t)stream);
+    if (rc != PPF_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return PPF_OK;
+}
+
+void ppf_stepper_destroy(ppf_stepper *h) { delete h; }
+
+int ppf_stepper_advance(ppf_stepper *h, const ppf_scene *scene, const ppf_step_params *p,
+                        ppf_step_result *out, void *stream) {
+    if (!h || !scene || !p || !out || !(p->dt > 0.0f) || !(p->line_search_max_t > 0.0f) ||
+        p->cg_max_iter == 0) {
+        set_error("ppf_stepper_advance: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.advance(*scene, *p, out, (hipStream_t)stream);
 }
 
 } // extern "C"

@@ -1,0 +1,616 @@
+// newton.hip -- one time step of the contact solver on MI355X.
+//
+// The Newton loop of main_helper::advance (main/main.cu:267-1649) for
+// shells with fix pins, self contact and a static collision mesh:
+//   target  = x + (x - x_prev) dt / dt_prev + dt^2 g (fix position if pinned)
+//                                                        main.cu:491-511
+//   loop: inertia + elastic assembly; contact pairs at the iterate, contact
+//         and collision-mesh barrier + friction with the elastic blocks as
+//         the stiffness reference; Dirichlet elimination of pinned vertices
+//         (main.cu:954-1049); block-Jacobi PCG on the merged upper matrix;
+//         step clamped by max_dx; CCD line search from the previous iterate
+//         (line_search_max_t sweep); toi accumulated until target_toi, then
+//         one error-reduction iteration at dt * toi_advanced
+//                                                        main.cu:683-1514
+//   commit: intersection check, prev <- curr, curr <- iterate
+//                                                        main.cu:1515-1600
+// Every stage is a kernel sequence of this library; the host only reads the
+// scalars that steer the loop (pair counts, toi, PCG status).
+#include "newton.hpp"
+#include "scan.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace ppf {
+
+namespace {
+
+constexpr int kNT = 256;
+
+uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+__device__ inline f3 X(const float *p, uint32_t i) { return load3(p + 3 * (size_t)i); }
+
+// Inertial target (main.cu:491-511).
+__global__ void k_target(uint32_t n, const float *__restrict__ curr,
+                         const float *__restrict__ prev, const uint32_t *__restrict__ vflag,
+                         const float *__restrict__ fix, float dt, float prev_dt, f3 g,
+                         float *__restrict__ target) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    if ((vflag[i] & 1u) && fix) {
+        store3(target + 3 * (size_t)i, X(fix, i));
+        return;
+    }
+    const f3 x1 = X(curr, i), x0 = X(prev, i);
+    const float tr = dt / prev_dt, h2 = dt * dt;
+    const f3 d = x1 - x0;
+    const f3 y = mk3(d.x * tr, d.y * tr, d.z * tr) + h2 * g;
+    store3(target + 3 * (size_t)i, x1 + y);
+}
+
+// dx = eval_x - target on pinned vertices, 0 elsewhere (main.cu:788-801).
+__global__ void k_dx_init(uint32_t n, const uint32_t *__restrict__ mask,
+                          const float *__restrict__ ex, const float *__restrict__ target,
+                          float *__restrict__ dx) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const f3 d = mask[i] ? X(ex, i) - X(target, i) : mk3(0.0f, 0.0f, 0.0f);
+    store3(dx + 3 * (size_t)i, d);
+}
+
+__global__ void k_mask(uint32_t n, const uint32_t *__restrict__ vflag, const float *fix,
+                       uint32_t *__restrict__ mask) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i < n) {
+        mask[i] = (fix && (vflag[i] & 1u)) ? 1u : 0u;
+    }
+}
+
+__global__ void k_add(uint64_t n, const float *__restrict__ a, const float *__restrict__ b,
+                      float *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x;
+    if (i < n) {
+        out[i] = a[i] + b[i];
+    }
+}
+
+// Union of two upper block CSRs (A = elastic pattern, B = contact blocks):
+// per-row counts, then the merged columns and values (A + B where both).
+__global__ void k_merge_count(uint32_t n, const uint32_t *__restrict__ rpa,
+                              const uint32_t *__restrict__ cia,
+                              const uint32_t *__restrict__ rpb,
+                              const uint32_t *__restrict__ cib, uint32_t *__restrict__ cnt) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    uint32_t a = rpa[i], ae = rpa[i + 1], b = rpb[i], be = rpb[i + 1], c = 0;
+    while (a < ae || b < be) {
+        if (b >= be || (a < ae && cia[a] < cib[b])) {
+            ++a;
+        } else if (a >= ae || cib[b] < cia[a]) {
+            ++b;
+        } else {
+            ++a;
+            ++b;
+        }
+        ++c;
+    }
+    cnt[i] = c;
+}
+
+__global__ void k_merge_fill(uint32_t n, const uint32_t *__restrict__ rpa,
+                             const uint32_t *__restrict__ cia, const float *__restrict__ va,
+                             const uint32_t *__restrict__ rpb, const uint32_t *__restrict__ cib,
+                             const float *__restrict__ vb, const uint32_t *__restrict__ rpm,
+                             uint32_t *__restrict__ cim, float *__restrict__ vm) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    uint32_t a = rpa[i], ae = rpa[i + 1], b = rpb[i], be = rpb[i + 1], o = rpm[i];
+    while (a < ae || b < be) {
+        float *dst = vm + 9 * (size_t)o;
+        if (b >= be || (a < ae && cia[a] < cib[b])) {
+            cim[o] = cia[a];
+            for (int e = 0; e < 9; ++e) dst[e] = va[9 * (size_t)a + e];
+            ++a;
+        } else if (a >= ae || cib[b] < cia[a]) {
+            cim[o] = cib[b];
+            for (int e = 0; e < 9; ++e) dst[e] = vb[9 * (size_t)b + e];
+            ++b;
+        } else {
+            cim[o] = cia[a];
+            for (int e = 0; e < 9; ++e) dst[e] = va[9 * (size_t)a + e] + vb[9 * (size_t)b + e];
+            ++a;
+            ++b;
+        }
+        ++o;
+    }
+}
+
+// Dirichlet elimination (main.cu:954-1049): move the coupling of free rows to
+// pinned columns into the right-hand side, zero every block touching a pinned
+// vertex. The lifts into other rows use float atomics, as the reference.
+__global__ void k_dirichlet_lift(uint32_t n, const uint32_t *__restrict__ mask,
+                                 const uint32_t *__restrict__ rp,
+                                 const uint32_t *__restrict__ ci, float *__restrict__ val,
+                                 const float *__restrict__ ex, const float *__restrict__ target,
+                                 float *__restrict__ force) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const bool mi = mask[i] != 0u;
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        const uint32_t j = ci[k];
+        const bool mj = mask[j] != 0u;
+        float *b = val + 9 * (size_t)k;
+        if (mi != mj) {
+            if (mj) {
+                const f3 p = X(ex, j) - X(target, j);
+                const float c0 = b[0] * p.x + b[3] * p.y + b[6] * p.z;
+                const float c1 = b[1] * p.x + b[4] * p.y + b[7] * p.z;
+                const float c2 = b[2] * p.x + b[5] * p.y + b[8] * p.z;
+                atomicAdd(force + 3 * (size_t)i, -c0);
+                atomicAdd(force + 3 * (size_t)i + 1, -c1);
+                atomicAdd(force + 3 * (size_t)i + 2, -c2);
+            } else {
+                const f3 p = X(ex, i) - X(target, i);
+                const float c0 = b[0] * p.x + b[1] * p.y + b[2] * p.z;
+                const float c1 = b[3] * p.x + b[4] * p.y + b[5] * p.z;
+                const float c2 = b[6] * p.x + b[7] * p.y + b[8] * p.z;
+                atomicAdd(force + 3 * (size_t)j, -c0);
+                atomicAdd(force + 3 * (size_t)j + 1, -c1);
+                atomicAdd(force + 3 * (size_t)j + 2, -c2);
+            }
+        }
+        if (mi || mj) {
+            for (int e = 0; e < 9; ++e) b[e] = 0.0f;
+        }
+    }
+}
+
+__global__ void k_dirichlet_diag(uint32_t n, const uint32_t *__restrict__ mask,
+                                 const float *__restrict__ ex,
+                                 const float *__restrict__ target, float *__restrict__ cdiag,
+                                 float *__restrict__ force) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= n || !mask[i]) {
+        return;
+    }
+    float *c = cdiag + 9 * (size_t)i;
+    for (int e = 0; e < 9; ++e) c[e] = (e % 4 == 0) ? 1.0f : 0.0f;
+    store3(force + 3 * (size_t)i, X(ex, i) - X(target, i));
+}
+
+// max_i |dx_i| as float bits (non-negative, so the integer order is the float
+// order).
+__global__ void k_max_norm(uint32_t n, const float *__restrict__ dx, uint32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    float v = 0.0f;
+    if (i < n) {
+        const f3 d = X(dx, i);
+        v = sqrtf(dot3(d, d));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        v = fmaxf(v, __shfl_xor(v, o));
+    }
+    if ((threadIdx.x & 63) == 0 && v > 0.0f) {
+        atomicMax(out, __float_as_uint(v));
+    }
+}
+
+// target <- eval_x; eval_x <- eval_x - recale * dx (main.cu:1177-1187).
+__global__ void k_step(uint64_t n3, float recale, const float *__restrict__ dx,
+                       float *__restrict__ ex, float *__restrict__ target) {
+    const uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x;
+    if (i < n3) {
+        const float x = ex[i];
+        target[i] = x;
+        ex[i] = x - recale * dx[i];
+    }
+}
+
+// eval_x <- target + toi (eval_x - target) (main.cu:1322-1329).
+__global__ void k_shorten(uint64_t n3, float toi, const float *__restrict__ target,
+                          float *__restrict__ ex) {
+    const uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x;
+    if (i < n3) {
+        const float t = target[i];
+        ex[i] = t + toi * (ex[i] - t);
+    }
+}
+
+} // namespace
+
+Stepper::~Stepper() { release(); }
+
+void Stepper::release() {
+    for (void *p : {(void *)ex_, (void *)target_, (void *)dx_, (void *)force_, (void *)cdiag_,
+                    (void *)eval_, (void *)cgrad_, (void *)mask_, (void *)scal_,
+                    (void *)pairs_, (void *)mrp_, (void *)mci_, (void *)mcnt_,
+                    (void *)scan_, (void *)mval_}) {
+        if (p) {
+            (void)hipFree(p);
+        }
+    }
+    if (h_scal_) {
+        (void)hipHostFree(h_scal_);
+    }
+    for (hipEvent_t &e : ev_) {
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    }
+    ex_ = target_ = dx_ = force_ = cdiag_ = eval_ = cgrad_ = nullptr;
+    mask_ = scal_ = h_scal_ = nullptr;
+    pairs_ = nullptr;
+    mrp_ = mci_ = mcnt_ = scan_ = nullptr;
+    mval_ = nullptr;
+    pair_cap_ = mcap_ = 0;
+}
+
+int Stepper::create(const ppf_scene &sc, hipStream_t s) {
+    release();
+    nv_ = sc.mesh.nv, ne_ = sc.mesh.ne, nf_ = sc.mesh.nf, nh_ = sc.shell.nh;
+    if (sc.shell.nv != nv_ || sc.shell.nf != nf_) {
+        set_error("stepper: shell and contact mesh sizes differ");
+        return PPF_ERR_ARG;
+    }
+    int rc = elastic_.create(sc.shell, s);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    const size_t n3 = 3 * (size_t)nv_;
+    PPF_HIP(hipMalloc(&ex_, n3 * sizeof(float)));
+    PPF_HIP(hipMalloc(&target_, n3 * sizeof(float)));
+    PPF_HIP(hipMalloc(&dx_, n3 * sizeof(float)));
+    PPF_HIP(hipMalloc(&force_, n3 * sizeof(float)));
+    PPF_HIP(hipMalloc(&cgrad_, n3 * sizeof(float)));
+    PPF_HIP(hipMalloc(&cdiag_, 9 * (size_t)nv_ * sizeof(float)));
+    PPF_HIP(hipMalloc(&eval_, 9 * ((size_t)elastic_.nnz() + 1) * sizeof(float)));
+    PPF_HIP(hipMalloc(&mask_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&scal_, 4 * sizeof(uint32_t)));
+    PPF_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_scal_), 4 * sizeof(uint32_t),
+                          hipHostMallocDefault));
+    PPF_HIP(hipMalloc(&mrp_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&mcnt_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&scan_, scan_scratch_words(nv_ + 1) * sizeof(uint32_t)));
+    for (hipEvent_t &e : ev_) {
+        PPF_HIP(hipEventCreate(&e));
+    }
+    k_mask<<<cdiv(nv_, kNT), kNT, 0, s>>>(nv_, sc.mesh.vflag, sc.fix_pos, mask_);
+    PPF_HIP(hipGetLastError());
+    // Pinned vertices are fixed for the scene: count them once (Dirichlet
+    // elimination runs only when there are any).
+    npinned_ = 0;
+    if (sc.fix_pos && nv_) {
+        std::vector<uint32_t> h(nv_);
+        PPF_HIP(hipMemcpyAsync(h.data(), mask_, (size_t)nv_ * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, s));
+        PPF_HIP(hipStreamSynchronize(s));
+        for (uint32_t v : h) {
+            npinned_ += v;
+        }
+    }
+    return PPF_OK;
+}
+
+int Stepper::reserve_pairs(uint64_t n) {
+    if (n <= pair_cap_) {
+        return PPF_OK;
+    }
+    if (pairs_) {
+        PPF_HIP(hipFree(pairs_));
+        pairs_ = nullptr;
+    }
+    const uint64_t cap = n + n / 2 + 1024;
+    PPF_HIP(hipMalloc(&pairs_, cap * sizeof(ppf_pair)));
+    pair_cap_ = cap;
+    return PPF_OK;
+}
+
+int Stepper::reserve_merged(uint64_t nnz) {
+    if (nnz <= mcap_) {
+        return PPF_OK;
+    }
+    if (mci_) (void)hipFree(mci_);
+    if (mval_) (void)hipFree(mval_);
+    mci_ = nullptr;
+    mval_ = nullptr;
+    const uint64_t cap = nnz + nnz / 4 + 1024;
+    PPF_HIP(hipMalloc(&mci_, cap * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&mval_, 9 * cap * sizeof(float)));
+    mcap_ = cap;
+    return PPF_OK;
+}
+
+int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, float dt,
+                              bool first_iteration, ppf_step_result *out, float *toi_out,
+                              float *recale_out, hipStream_t s) {
+    const uint32_t n = nv_;
+    const size_t n3 = 3 * (size_t)n;
+    // Inertia + elastic at the iterate.
+    PPF_HIP(hipEventRecord(ev_[0], s));
+    ppf_shell sh = sc.shell;
+    sh.x = ex_;
+    sh.x0 = sc.mesh.x;
+    sh.target = target_;
+    ppf_elastic_params ep;
+    ep.dt = dt;
+    ep.air_density = p.air_density;
+    ep.air_friction = p.air_friction;
+    ep.wind[0] = p.wind[0], ep.wind[1] = p.wind[1], ep.wind[2] = p.wind[2];
+    k_dx_init<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, ex_, target_, dx_);
+    PPF_HIP(hipGetLastError());
+    int rc = elastic_.assemble(sh, ep, force_, cdiag_, eval_, s);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    PPF_HIP(hipEventRecord(ev_[1], s));
+    // Contact pairs and their force / blocks (tmp_fixed = elastic blocks).
+    ppf_contact_mesh m = sc.mesh;
+    m.x = ex_;
+    uint64_t npairs = 0;
+    if (!p.disable_contact) {
+        rc = narrow_.build_trees(m, s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        uint64_t nself = 0, nstat = 0;
+        rc = narrow_.proximity(m, pairs_, pair_cap_, &nself, s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        if (nself > pair_cap_) {
+            rc = reserve_pairs(nself);
+            if (rc != PPF_OK) {
+                return rc;
+            }
+            rc = narrow_.proximity(m, pairs_, pair_cap_, &nself, s);
+            if (rc != PPF_OK) {
+                return rc;
+            }
+        }
+        if (sc.collision.nv) {
+            rc = narrow_.static_proximity(m, sc.collision, sc.shell.vmass, sc.emass,
+                                          sc.shell.fmass, pairs_ + nself,
+                                          pair_cap_ - nself, &nstat, s);
+            if (rc != PPF_OK) {
+                return rc;
+            }
+            if (nself + nstat > pair_cap_) {
+                rc = reserve_pairs(nself + nstat);
+                if (rc != PPF_OK) {
+                    return rc;
+                }
+                rc = narrow_.proximity(m, pairs_, pair_cap_, &nself, s);
+                if (rc == PPF_OK) {
+                    rc = narrow_.static_proximity(m, sc.collision, sc.shell.vmass, sc.emass,
+                                                  sc.shell.fmass, pairs_ + nself,
+                                                  pair_cap_ - nself, &nstat, s);
+                }
+                if (rc != PPF_OK) {
+                    return rc;
+                }
+            }
+        }
+        npairs = nself + nstat;
+    }
+    if (npairs > out->pairs_max) {
+        out->pairs_max = (uint32_t)npairs;
+    }
+    ppf_contact_params cp;
+    cp.x0 = sc.mesh.x;
+    cp.mass = sc.shell.vmass;
+    cp.v_fric = sc.v_fric;
+    cp.e_fric = sc.e_fric;
+    cp.f_fric = sc.f_fric;
+    cp.friction_mode = p.friction_mode;
+    cp.friction_eps = p.friction_eps;
+    cp.el_rp = elastic_.row_offset();
+    cp.el_col = elastic_.col_index();
+    cp.el_val = eval_;
+    cp.sm = sc.collision.nv ? &sc.collision : nullptr;
+    rc = contact_.assemble(m, cp, pairs_, npairs, cgrad_, s);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    k_add<<<cdiv(n3, kNT), kNT, 0, s>>>(n3, force_, cgrad_, force_);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipEventRecord(ev_[2], s));
+    // Merged matrix, Dirichlet elimination, PCG.
+    k_merge_count<<<cdiv(n, kNT), kNT, 0, s>>>(n, elastic_.row_offset(), elastic_.col_index(),
+                                                contact_.row_offset(), contact_.cols(), mcnt_);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipMemsetAsync(mcnt_ + n, 0, sizeof(uint32_t), s));
+    PPF_HIP(exclusive_scan_u32(mcnt_, mrp_, n + 1, scan_, s));
+    uint32_t nnz = 0;
+    PPF_HIP(hipMemcpyAsync(&nnz, mrp_ + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipStreamSynchronize(s));
+    rc = reserve_merged(nnz);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    k_merge_fill<<<cdiv(n, kNT), kNT, 0, s>>>(n, elastic_.row_offset(), elastic_.col_index(),
+                                               eval_, contact_.row_offset(), contact_.cols(),
+                                               contact_.values(), mrp_, mci_, mval_);
+    PPF_HIP(hipGetLastError());
+    if (npinned_) {
+        k_dirichlet_lift<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, mrp_, mci_, mval_, ex_,
+                                                      target_, force_);
+        PPF_HIP(hipGetLastError());
+        k_dirichlet_diag<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, ex_, target_, cdiag_, force_);
+        PPF_HIP(hipGetLastError());
+    }
+    PcgPlanImpl *plan = nullptr;
+    rc = pcg_plan_create(n, nnz, mrp_, mci_, s, &plan);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    ppf_pcg_result pr{};
+    rc = pcg_solve(plan, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr, &pr, s);
+    pcg_plan_destroy(plan);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    out->pcg_iter_total += pr.iter;
+    if (pr.iter > out->pcg_iter_max) {
+        out->pcg_iter_max = pr.iter;
+    }
+    if (pr.status != 0 && pr.status != 1) {
+        out->status = 1;
+        return PPF_OK;
+    }
+    PPF_HIP(hipEventRecord(ev_[3], s));
+    // Clamp the step (max_dx), move, CCD line search, shorten.
+    PPF_HIP(hipMemsetAsync(scal_, 0, sizeof(uint32_t), s));
+    k_max_norm<<<cdiv(n, kNT), kNT, 0, s>>>(n, dx_, scal_);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipMemcpyAsync(h_scal_, scal_, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipStreamSynchronize(s));
+    const float max_dx = __builtin_bit_cast(float, h_scal_[0]);
+    const float recale = fminf(1.0f, p.max_dx / max_dx);
+    k_step<<<cdiv(n3, kNT), kNT, 0, s>>>(n3, recale, dx_, ex_, target_);
+    PPF_HIP(hipGetLastError());
+    float toi = 1.0f;
+    if (!p.disable_contact) {
+        ppf_contact_mesh lm = sc.mesh;
+        lm.x = target_;
+        ppf_ls_params lp;
+        lp.x1 = ex_;
+        lp.vmass = sc.shell.vmass;
+        lp.emass = sc.emass;
+        lp.fmass = sc.shell.fmass;
+        lp.max_t = p.line_search_max_t;
+        lp.ccd_eps = p.ccd_eps;
+        int overlap = 0;
+        float t = 1.0f;
+        rc = ls_.run(lm, sc.collision.nv ? &sc.collision : nullptr, lp, first_iteration, &t,
+                     &overlap, s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        toi = fminf(toi, t);
+        if (overlap) {
+            out->status = 3;
+            return PPF_OK;
+        }
+    }
+    PPF_HIP(hipEventRecord(ev_[4], s));
+    PPF_HIP(hipEventSynchronize(ev_[4]));
+    float ms;
+    PPF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+    out->ms_elastic += ms;
+    PPF_HIP(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
+    out->ms_contact += ms;
+    PPF_HIP(hipEventElapsedTime(&ms, ev_[2], ev_[3]));
+    out->ms_solve += ms;
+    PPF_HIP(hipEventElapsedTime(&ms, ev_[3], ev_[4]));
+    out->ms_line_search += ms;
+    if (toi <= FLT_EPSILON) {
+        out->status = 2;
+        return PPF_OK;
+    }
+    k_shorten<<<cdiv(n3, kNT), kNT, 0, s>>>(n3, toi, target_, ex_);
+    PPF_HIP(hipGetLastError());
+    *toi_out = toi;
+    *recale_out = recale;
+    return PPF_OK;
+}
+
+int Stepper::advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_result *out,
+                     hipStream_t s) {
+    *out = ppf_step_result{};
+    const uint32_t n = nv_;
+    const size_t n3 = 3 * (size_t)n;
+    PPF_HIP(hipEventRecord(ev_[5], s));
+    const f3 g = mk3(p.gravity[0], p.gravity[1], p.gravity[2]);
+    float dt = p.dt;
+    const float prev_dt = p.prev_dt > 0.0f ? p.prev_dt : p.dt;
+    k_target<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.x, sc.prev, sc.mesh.vflag, sc.fix_pos, dt,
+                                           prev_dt, g, target_);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipMemcpyAsync(ex_, sc.mesh.x, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    double toi_advanced = 0.0;
+    uint32_t step = 1;
+    bool final_step = false;
+    while (true) {
+        if (!final_step && p.max_newton_steps > 0 && step >= p.max_newton_steps) {
+            out->status = 5;
+            break;
+        }
+        if (final_step) {
+            dt *= (float)toi_advanced;
+            k_target<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.x, sc.prev, sc.mesh.vflag,
+                                                   sc.fix_pos, dt, prev_dt, g, target_);
+            PPF_HIP(hipGetLastError());
+        }
+        float toi = 1.0f, recale = 1.0f;
+        int rc = newton_iteration(sc, p, dt, out->newton_steps == 0, out, &toi, &recale, s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        out->newton_steps += 1;
+        if (out->status != 0) {
+            break;
+        }
+        if (!final_step) {
+            toi_advanced += std::max(0.0, 1.0 - toi_advanced) * (double)(recale * toi);
+        }
+        if (final_step) {
+            break;
+        } else if (toi_advanced >= p.target_toi && step >= p.min_newton_steps) {
+            final_step = true;
+        } else {
+            ++step;
+            k_target<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.x, sc.prev, sc.mesh.vflag,
+                                                   sc.fix_pos, dt, prev_dt, g, target_);
+            PPF_HIP(hipGetLastError());
+        }
+    }
+    out->toi_advanced = (float)toi_advanced;
+    out->dt = dt;
+    if (out->status == 0 && p.check_intersection) {
+        ppf_contact_mesh m = sc.mesh;
+        m.x = ex_;
+        int rc = narrow_.build_trees(m, s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        uint32_t count = 0;
+        rc = isect_.run(narrow_.tree(0).view(), narrow_.tree(1).view(), narrow_.tree(2).view(),
+                        m, sc.collision.nv ? &sc.collision : nullptr, sc.shell.vmass,
+                        sc.emass, sc.shell.fmass, n, nullptr, 0, &count, s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        out->intersections = count;
+        if (count) {
+            out->status = 4;
+        }
+    }
+    if (out->status == 0) {
+        // prev <- curr, curr <- iterate
+        PPF_HIP(hipMemcpyAsync(sc.prev, sc.mesh.x, n3 * sizeof(float), hipMemcpyDeviceToDevice,
+                               s));
+        PPF_HIP(hipMemcpyAsync(const_cast<float *>(sc.mesh.x), ex_, n3 * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
+    }
+    PPF_HIP(hipEventRecord(ev_[6], s));
+    PPF_HIP(hipEventSynchronize(ev_[6]));
+    PPF_HIP(hipEventElapsedTime(&out->ms_total, ev_[5], ev_[6]));
+    return PPF_OK;
+}
+
+} // namespace ppf

@@ -1,0 +1,47 @@
+// newton.hpp -- one time step of the contact solver (internal interface).
+#pragma once
+#include "assemble.hpp"
+#include "elastic.hpp"
+#include "isect.hpp"
+#include "linesearch.hpp"
+#include "narrow.hpp"
+#include "pcg.hpp"
+
+namespace ppf {
+
+// Owns every per-step buffer and stage of the Newton loop of one scene; the
+// scene's arrays stay with the caller.
+class Stepper {
+  public:
+    ~Stepper();
+    int create(const ppf_scene &sc, hipStream_t s);
+    int advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_result *out,
+                hipStream_t s);
+    const float *eval_x() const { return ex_; }
+
+  private:
+    int newton_iteration(const ppf_scene &sc, const ppf_step_params &p, float dt,
+                         bool first_iteration, ppf_step_result *out, float *toi_out,
+                         float *recale_out, hipStream_t s);
+    int reserve_pairs(uint64_t n);
+    int reserve_merged(uint64_t nnz);
+    void release();
+    uint32_t nv_ = 0, ne_ = 0, nf_ = 0, nh_ = 0;
+    ElasticAssembler elastic_;
+    Narrow narrow_;
+    ContactAssembler contact_;
+    LineSearch ls_;
+    IntersectionCheck isect_;
+    float *ex_ = nullptr, *target_ = nullptr, *dx_ = nullptr, *force_ = nullptr;
+    float *cdiag_ = nullptr, *eval_ = nullptr, *cgrad_ = nullptr;
+    uint32_t *mask_ = nullptr, *scal_ = nullptr, *h_scal_ = nullptr;
+    ppf_pair *pairs_ = nullptr;
+    uint64_t pair_cap_ = 0;
+    uint32_t *mrp_ = nullptr, *mci_ = nullptr, *mcnt_ = nullptr, *scan_ = nullptr;
+    float *mval_ = nullptr;
+    uint64_t mcap_ = 0;
+    uint32_t npinned_ = 0;
+    hipEvent_t ev_[8] = {};
+};
+
+} // namespace ppf

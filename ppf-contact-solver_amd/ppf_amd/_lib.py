@@ -60,6 +60,9 @@ _PROTOS = {
     "ppf_elastic_pattern": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                            ctypes.POINTER(u32)]),
     "ppf_elastic_assemble": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "ppf_stepper_create": (ctypes.c_int, [vp, vp, ctypes.POINTER(vp)]),
+    "ppf_stepper_destroy": (None, [vp]),
+    "ppf_stepper_advance": (ctypes.c_int, [vp, vp, vp, vp, vp]),
     "ppf_ccd_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
     "ppf_ccd_destroy": (None, [vp]),
     "ppf_line_search": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int,

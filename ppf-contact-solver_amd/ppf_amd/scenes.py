@@ -441,23 +441,29 @@ def static_feature_scene(feature, res=8, offset=5e-5):
     return m, x1, ones(m["nv"]), ones(m["ne"]), ones(m["nf"]), sm
 
 
-def shell_hinges(faces):
+def shell_hinges(faces, with_faces=False):
     """Interior edges shared by two faces as hinges (e0, e1, flap0, flap1): the
     stored layout whose remap (h2, h1, h0, h3) puts the shared edge in the
-    middle. Sorted by edge."""
-    f = np.asarray(faces, np.int64)
-    owner = {}
-    hinges = []
-    for fi, tri in enumerate(f):
-        for k in range(3):
-            a, b, c = tri[k], tri[(k + 1) % 3], tri[(k + 2) % 3]
-            key = (min(a, b), max(a, b))
-            if key in owner:
-                gi, gc = owner.pop(key)
-                hinges.append((key[0], key[1], gc, c))
-            else:
-                owner[key] = (fi, c)
-    h = np.array(sorted(hinges), np.uint32).reshape(-1, 4)
+    middle. Sorted by edge; flap0 belongs to the lower-index face. With
+    `with_faces`, also the two face indices of every hinge."""
+    f = np.asarray(faces, np.int64).reshape(-1, 3)
+    nf = len(f)
+    if nf == 0:
+        h = np.zeros((0, 4), np.uint32)
+        return (h, np.zeros((0, 2), np.int64)) if with_faces else h
+    a = f.reshape(-1)
+    b = f[:, [1, 2, 0]].reshape(-1)
+    c = f[:, [2, 0, 1]].reshape(-1)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    n = int(f.max()) + 1
+    key = lo * n + hi
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    dup = np.nonzero(ks[1:] == ks[:-1])[0]
+    i0, i1 = order[dup], order[dup + 1]
+    h = np.stack([lo[i0], hi[i0], c[i0], c[i1]], 1).astype(np.uint32)
+    if with_faces:
+        return h, np.stack([i0 // 3, i1 // 3], 1)
     return h
 
 
@@ -481,28 +487,22 @@ def _dihedral64(p0, p1, p2, p3):
     return np.arctan2(s, c)
 
 
-def shell_scene(res=16, size=1.0, density=0.2, young=1e4, poisson=0.25, bend=10.0,
-                dt=1.0 / 60.0, seed=0, bump=0.05, stretch=0.02, pins=0, gravity=-9.8):
-    """A square cloth (res x res vertices, side `size`, the alternating
-    triangulation of rect_faces) with everything the elastic assembly reads:
-    rest geometry (lumped vertex mass / area, face mass, inv_rest2x2, hinge
-    rest angle / length / area), the current state x0 = rest + a smooth bump,
-    a perturbed Newton iterate x, and the inertial target x0 + dt^2 g. `pins`
-    vertices (the first ones) are pinned. Lame parameters from
-    young / poisson as builder.rs:1739-1743."""
-    rng = np.random.default_rng(seed)
-    faces = rect_faces(res, res).astype(np.uint32)
-    i, j = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
-    h = size / (res - 1)
-    rest = np.stack([i.reshape(-1) * h, j.reshape(-1) * h, np.zeros(res * res)], 1)
+def shell_rest(rest, faces, density=0.2, young=1e4, poisson=0.25, bend=10.0):
+    """Rest-state shell data the elastic assembly reads (builder.rs:157-272,
+    339-644, mesh.rs hinges): lumped vertex mass / area, face mass,
+    inv_rest2x2 (column-major) in each face's own 2D frame, hinges with rest
+    dihedral angle / edge length / combined area, Lame parameters from young /
+    poisson (builder.rs:1739-1743)."""
+    rest = np.asarray(rest, np.float64).reshape(-1, 3)
+    faces = np.asarray(faces, np.uint32).reshape(-1, 3)
     nv, nf = len(rest), len(faces)
-    # rest-frame quantities (fp64, then fp32 as the builder stores them)
     P = rest[faces.astype(np.int64)]
     e1, e2 = P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]
-    area = 0.5 * np.linalg.norm(np.cross(e1, e2), axis=1)
+    cr = np.cross(e1, e2)
+    area = 0.5 * np.linalg.norm(cr, axis=1)
     l1 = np.linalg.norm(e1, axis=1)
     u = e1 / l1[:, None]
-    w = np.cross(np.cross(e1, e2), e1)
+    w = np.cross(cr, e1)
     w /= np.linalg.norm(w, axis=1)[:, None]
     D = np.stack([np.stack([l1, np.einsum("ij,ij->i", e2, u)], 1),
                   np.stack([np.zeros(nf), np.einsum("ij,ij->i", e2, w)], 1)], 1)
@@ -512,36 +512,45 @@ def shell_scene(res=16, size=1.0, density=0.2, young=1e4, poisson=0.25, bend=10.
     varea = np.zeros(nv)
     np.add.at(varea, faces.reshape(-1).astype(np.int64), np.repeat(area / 3.0, 3))
     vmass = density * varea
-    hinge = shell_hinges(faces)
+    hinge, hf = shell_hinges(faces, with_faces=True)
     H = hinge.astype(np.int64)
     hl = np.linalg.norm(rest[H[:, 1]] - rest[H[:, 0]], axis=1)
-    # combined rest area of the two faces of each hinge
-    fa = {}
-    for fi, tri in enumerate(faces.astype(np.int64)):
-        for k in range(3):
-            key = (min(tri[k], tri[(k + 1) % 3]), max(tri[k], tri[(k + 1) % 3]))
-            fa[key] = fa.get(key, 0.0) + area[fi]
-    harea = np.array([fa[(min(a, b), max(a, b))] for a, b in H[:, :2]])
+    harea = area[hf[:, 0]] + area[hf[:, 1]] if len(H) else np.zeros(0)
     mu = young / (2.0 * (1.0 + poisson))
     lam = young * poisson / ((1.0 + poisson) * (1.0 - 2.0 * poisson))
-    # state: a smooth bump, rest angles of the rest (flat) shape
+    hrest = (_dihedral64(rest[H[:, 2]], rest[H[:, 1]], rest[H[:, 0]], rest[H[:, 3]])
+             if len(H) else np.zeros(0))
+    v2f_off, v2f = vertex_face_csr(nv, faces)
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    return dict(nv=nv, nf=nf, nh=len(hinge), face=faces, hinge=hinge, v2f_off=v2f_off,
+                v2f=v2f, vmass=f32(vmass), varea=f32(varea), vflag=np.zeros(nv, np.uint32),
+                fmass=f32(fmass), inv_rest=f32(inv_rest), f_mu=f32(np.full(nf, mu)),
+                f_lambda=f32(np.full(nf, lam)), fflag=np.zeros(nf, np.uint32),
+                h_rest=f32(hrest), h_length=f32(hl), h_area=f32(harea),
+                h_bend=f32(np.full(len(hinge), bend)), hflag=np.zeros(len(hinge), np.uint32))
+
+
+def shell_scene(res=16, size=1.0, density=0.2, young=1e4, poisson=0.25, bend=10.0,
+                dt=1.0 / 60.0, seed=0, bump=0.05, stretch=0.02, pins=0, gravity=-9.8):
+    """A square cloth (res x res vertices, side `size`, the alternating
+    triangulation of rect_faces) with everything the elastic assembly reads
+    (shell_rest), the current state x0 = rest + a smooth bump, a perturbed
+    Newton iterate x, and the inertial target x0 + dt^2 g. `pins` vertices
+    (the first ones) are pinned."""
+    rng = np.random.default_rng(seed)
+    faces = rect_faces(res, res).astype(np.uint32)
+    i, j = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    h = size / (res - 1)
+    rest = np.stack([i.reshape(-1) * h, j.reshape(-1) * h, np.zeros(res * res)], 1)
+    sh = shell_rest(rest, faces, density, young, poisson, bend)
     x0 = rest.copy()
     x0[:, 2] = bump * np.sin(np.pi * rest[:, 0] / size) * np.sin(np.pi * rest[:, 1] / size)
     x = x0 + rng.normal(0.0, stretch * h, x0.shape)
     target = x0 + np.array([0.0, 0.0, gravity]) * dt * dt
-    hrest = _dihedral64(rest[H[:, 2]], rest[H[:, 1]], rest[H[:, 0]], rest[H[:, 3]])
-    v2f_off, v2f = vertex_face_csr(nv, faces)
-    vflag = np.zeros(nv, np.uint32)
-    vflag[:pins] = 1
+    sh["vflag"][:pins] = 1
     f32 = lambda a: np.ascontiguousarray(a, np.float32)
-    return dict(nv=nv, nf=nf, nh=len(hinge), x=f32(x), x0=f32(x0), target=f32(target),
-                face=faces, hinge=hinge, v2f_off=v2f_off, v2f=v2f,
-                vmass=f32(vmass), varea=f32(varea), vflag=vflag,
-                fmass=f32(fmass), inv_rest=f32(inv_rest), f_mu=f32(np.full(nf, mu)),
-                f_lambda=f32(np.full(nf, lam)), fflag=np.zeros(nf, np.uint32),
-                h_rest=f32(hrest), h_length=f32(hl), h_area=f32(harea),
-                h_bend=f32(np.full(len(hinge), bend)),
-                hflag=np.zeros(len(hinge), np.uint32), dt=dt)
+    sh.update(x=f32(x), x0=f32(x0), target=f32(target), dt=dt)
+    return sh
 
 
 def upper_pattern(nv, faces, hinges):
@@ -562,3 +571,80 @@ def upper_pattern(nv, faces, hinges):
     rows, cols = key // nv, key % nv
     ro = np.searchsorted(rows, np.arange(nv + 1)).astype(np.uint32)
     return ro, cols.astype(np.uint32)
+
+
+def icosphere(radius=0.3, subdiv=3):
+    """Icosphere: the icosahedron with every face split in four `subdiv`
+    times and the new vertices projected to the sphere (subdiv 3: 642
+    vertices, 1,280 faces)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+         (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+         (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+         (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = list(f)
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return (np.array(verts) * radius).astype(np.float32), np.array(faces, np.uint32)
+
+
+def drape_scene(res=64, size=1.0, height=0.05, radius=0.3, subdiv=3, ghat=1e-3, offset=0.0,
+                friction=0.5, density=0.2, young=1e4, poisson=0.25, bend=10.0, pins=None):
+    """Square cloth (res x res vertices, side `size`, in the xz plane, y up)
+    centred `height` above a static icosphere of `radius` at the origin: the
+    plain-method drape configuration (64 x 64, edge 1/63 m, 1,280-triangle
+    sphere, gap 1e-3, friction 0.5, density 0.2, Young 1e4, Poisson 0.25, bend
+    10). `pins`: indices of pinned vertices (held in place). Returns a dict of
+    numpy arrays: mesh (contact mesh, x = current positions), shell, static,
+    prev, emass, fric, fix_pos."""
+    faces = rect_faces(res, res).astype(np.uint32)
+    i, j = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    h = size / (res - 1)
+    rest = np.stack([i.reshape(-1) * h - size / 2, np.zeros(res * res),
+                     j.reshape(-1) * h - size / 2], 1)
+    x = rest + np.array([0.0, radius + height, 0.0])
+    shell = shell_rest(rest, faces, density, young, poisson, bend)
+    mesh = contact_mesh(x.astype(np.float32), faces, ghat=ghat, offset=offset)
+    sx, sf = icosphere(radius, subdiv)
+    static = static_mesh(sx, sf, ghat=ghat, offset=offset)
+    for k, n in (("v", static["nv"]), ("e", static["ne"]), ("f", static["nf"])):
+        static[k + "_fric"] = np.full(n, friction, np.float32)
+    e = mesh["edge"].astype(np.int64)
+    emass = (0.5 * (shell["vmass"][e[:, 0]] + shell["vmass"][e[:, 1]])).astype(np.float32)
+    fric = {"v": np.full(mesh["nv"], friction, np.float32),
+            "e": np.full(mesh["ne"], friction, np.float32),
+            "f": np.full(mesh["nf"], friction, np.float32)}
+    fix_pos = None
+    if pins is not None and len(pins):
+        mesh["vflag"][np.asarray(pins)] |= 1
+        shell["vflag"] = mesh["vflag"].copy()
+        fix_pos = mesh["x"].copy()
+    return dict(mesh=mesh, shell=shell, static=static, prev=mesh["x"].copy(), emass=emass,
+                fric=fric, fix_pos=fix_pos)
+
+
+def step_params(dt=1.0 / 60.0, **kw):
+    """ParamSet defaults the step reads (crates/ppf-cts-core/src/datamodel/
+    params.rs:215-280), with gravity (0, -9.8, 0)."""
+    p = dict(dt=dt, prev_dt=dt, gravity=(0.0, -9.8, 0.0), cg_tol=1e-3, cg_max_iter=10000,
+             target_toi=0.25, line_search_max_t=1.25, ccd_eps=1e-7, max_dx=1.0,
+             friction_eps=1e-5, friction_mode=0, min_newton_steps=0, max_newton_steps=2048,
+             air_density=1e-3, air_friction=0.2, wind=(0.0, 0.0, 0.0), disable_contact=0,
+             check_intersection=1)
+    p.update(kw)
+    return p

@@ -1,0 +1,123 @@
+"""One time step of the contact solver over torch device buffers.
+
+Mirrors main_helper::advance (crates/ppf-cts-solver/src/cpp/main/main.cu:
+267-1649) for shells with fix pins, self contact and a static collision
+mesh. All compute is in libppf_mi355x.so.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+from .ccd import StaticMeshC, static_struct, static_to_device
+from .elastic import SHELL_F32, SHELL_U32, ShellC
+from .narrow import ContactMeshC, _ptr, _stream, mesh_struct, mesh_to_device
+
+vp = ctypes.c_void_p
+u32 = ctypes.c_uint32
+f32 = ctypes.c_float
+i32 = ctypes.c_int32
+
+
+class SceneC(ctypes.Structure):
+    """ppf_scene (include/ppf_mi355x.h)."""
+    _fields_ = [("mesh", ContactMeshC), ("shell", ShellC), ("collision", StaticMeshC),
+                ("prev", vp), ("emass", vp), ("v_fric", vp), ("e_fric", vp), ("f_fric", vp),
+                ("fix_pos", vp)]
+
+
+class StepParamsC(ctypes.Structure):
+    """ppf_step_params (include/ppf_mi355x.h)."""
+    _fields_ = [("dt", f32), ("prev_dt", f32), ("gravity", f32 * 3), ("cg_tol", f32),
+                ("cg_max_iter", u32), ("target_toi", f32), ("line_search_max_t", f32),
+                ("ccd_eps", f32), ("max_dx", f32), ("friction_eps", f32),
+                ("friction_mode", i32), ("min_newton_steps", u32), ("max_newton_steps", u32),
+                ("air_density", f32), ("air_friction", f32), ("wind", f32 * 3),
+                ("disable_contact", i32), ("check_intersection", i32)]
+
+
+class StepResultC(ctypes.Structure):
+    """ppf_step_result (include/ppf_mi355x.h)."""
+    _fields_ = [("status", i32), ("newton_steps", u32), ("pcg_iter_total", u32),
+                ("pcg_iter_max", u32), ("pairs_max", u32), ("intersections", u32),
+                ("toi_advanced", f32), ("dt", f32), ("ms_elastic", f32), ("ms_contact", f32),
+                ("ms_solve", f32), ("ms_line_search", f32), ("ms_total", f32)]
+
+
+def params_struct(p):
+    kw = dict(p)
+    kw["gravity"] = (f32 * 3)(*p["gravity"])
+    kw["wind"] = (f32 * 3)(*p["wind"])
+    return StepParamsC(**kw)
+
+
+def result_dict(r):
+    return {k: getattr(r, k) for k, _ in StepResultC._fields_}
+
+
+def scene_to_device(sc, dev):
+    """Upload a scene dict (scenes.drape_scene layout) as device tensors."""
+    t = {"mesh": mesh_to_device(sc["mesh"], dev)}
+    sh = {k: int(sc["shell"][k]) for k in ("nv", "nf", "nh")}
+    for k in SHELL_F32:
+        if k in ("x", "x0", "target"):
+            continue
+        sh[k] = torch.from_numpy(np.ascontiguousarray(sc["shell"][k], np.float32).reshape(-1)).to(dev)
+    for k in SHELL_U32:
+        a = np.ascontiguousarray(sc["shell"][k], np.uint32).reshape(-1).view(np.int32)
+        sh[k] = torch.from_numpy(a).to(dev)
+    t["shell"] = sh
+    t["static"] = static_to_device(sc["static"], dev) if sc.get("static") else None
+    f = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32).reshape(-1)).to(dev)
+    t["prev"] = f(sc["prev"])
+    t["emass"] = f(sc["emass"])
+    t["fric"] = {k: f(v) for k, v in sc["fric"].items()}
+    t["fix_pos"] = f(sc["fix_pos"]) if sc.get("fix_pos") is not None else None
+    return t
+
+
+def scene_struct(t):
+    sh = t["shell"]
+    fields = {k: sh[k] for k in ("nv", "nf", "nh")}
+    for k in SHELL_F32 + SHELL_U32:
+        if k in sh:
+            fields[k] = sh[k].data_ptr() if sh[k].numel() else None
+    st = static_struct(t["static"]) if t["static"] is not None else StaticMeshC(nv=0, ne=0, nf=0)
+    return SceneC(mesh=mesh_struct(t["mesh"]), shell=ShellC(**fields), collision=st,
+                  prev=_ptr(t["prev"]), emass=_ptr(t["emass"]), v_fric=_ptr(t["fric"]["v"]),
+                  e_fric=_ptr(t["fric"]["e"]), f_fric=_ptr(t["fric"]["f"]),
+                  fix_pos=_ptr(t["fix_pos"]))
+
+
+class Stepper:
+    """Time stepping of a device scene (mesh['x'] and prev are advanced in
+    place)."""
+
+    def __init__(self, t):
+        self.lib = _lib.load()
+        self.t = t
+        self._sc = scene_struct(t)
+        h = vp()
+        check(self.lib.ppf_stepper_create(ctypes.byref(self._sc), _stream(), ctypes.byref(h)),
+              "ppf_stepper_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.ppf_stepper_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def advance(self, params):
+        p = params_struct(params)
+        r = StepResultC()
+        check(self.lib.ppf_stepper_advance(self._h, ctypes.byref(self._sc), ctypes.byref(p),
+                                           ctypes.byref(r), _stream()), "ppf_stepper_advance")
+        return result_dict(r)
