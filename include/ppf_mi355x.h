@@ -67,6 +67,13 @@ int ppf_pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *d_row_offset,
                         ppf_pcg_plan **out);
 void ppf_pcg_plan_destroy(ppf_pcg_plan *plan);
 
+/* Point an existing plan at another pattern with the same n (the contact
+ * matrix changes pattern every Newton iteration; reference: the per-iteration
+ * rebuild of the dynamic matrix in csrmat.cu:302-433). Scratch grows when the
+ * new pattern is larger; the previous pattern's arrays may be freed after. */
+int ppf_pcg_plan_repattern(ppf_pcg_plan *plan, uint32_t nnz, const uint32_t *d_row_offset,
+                           const uint32_t *d_col_index, void *stream);
+
 /* Solve M x = b to |r|_1/|r0|_1 < tol (x holds x0 on entry). d_value: nnz
  * blocks; d_cdiag: n blocks; d_b, d_x: 3n floats. Mirrors solver::solve +
  * solver::cg (solver.cu:1631-1675). */
@@ -114,7 +121,10 @@ int ppf_symm3x3(uint32_t n, const float *d_in, float *d_val, float *d_vec,
  * per-element contact gap and offset (VertexParam/EdgeParam/FaceParam ghat and
  * offset, resolved per element), vflag bit0 = pinned (fix_index != 0),
  * bit1 = collider, vbody = PD rigid body id (0 none), eflag/fflag bit0 =
- * fixed. */
+ * fixed. vflag / eflag bit2 = outside the element's collision window: the
+ * queries the vertex (or edge) issues find nothing, while it can still be
+ * found by others (the inactive query boxes of contact.cu:1637, 1728,
+ * 1868, 1940, 2563, 2591, 2688). */
 typedef struct ppf_contact_mesh {
     uint32_t nv, ne, nf;
     const float *x;
@@ -364,7 +374,11 @@ int ppf_elastic_assemble(ppf_elastic *h, const ppf_shell *sh, const ppf_elastic_
  * mesh.vflag bit0. collision.nv == 0 means no collision mesh. fix_pos holds
  * the target position of every pinned vertex (3 nv; may be null when no
  * vertex is pinned); emass is EdgeProp mass; *_fric the per-element
- * friction. */
+ * friction. fix_delta (3 nv, may be null) is each pin's displacement over the
+ * step (FixPair::step_delta: zero for a static pin): when the Newton loop
+ * shortens the step to toi_advanced < 1, fix_pos is rewound in place to
+ * fix_pos - (1 - toi_advanced) * fix_delta before the final iteration
+ * (rewind_kinematic_fix, main.cu:513-545). */
 typedef struct ppf_scene {
     ppf_contact_mesh mesh;
     ppf_shell shell;
@@ -372,7 +386,8 @@ typedef struct ppf_scene {
     float *prev;
     const float *emass;
     const float *v_fric, *e_fric, *f_fric;
-    const float *fix_pos;
+    float *fix_pos;
+    const float *fix_delta;
 } ppf_scene;
 
 /* The ParamSet fields the step reads (data.hpp:515-568). */
@@ -394,7 +409,9 @@ typedef struct ppf_step_params {
 /* Outcome of one step (StepResult, data.hpp:570-595). status: 0 success,
  * 1 PCG failed, 2 CCD failed (toi <= FLT_EPSILON), 3 a contact pair starts
  * inside its offset, 4 intersection after the step, 5 Newton stalled
- * (max_newton_steps). Times are HIP-event milliseconds on the stream. */
+ * (max_newton_steps). The step is committed (prev <- curr, curr <- the new
+ * positions) on 0 and on 4, as main.cu:1515-1629 commits before the host sees
+ * the intersection flag. Times are HIP-event milliseconds on the stream. */
 typedef struct ppf_step_result {
     int32_t status;
     uint32_t newton_steps;
@@ -420,6 +437,23 @@ void ppf_stepper_destroy(ppf_stepper *h);
  * prev / curr. */
 int ppf_stepper_advance(ppf_stepper *h, const ppf_scene *scene, const ppf_step_params *p,
                         ppf_step_result *out, void *stream);
+
+/* Re-read the pin set after the scene's mesh.vflag bit0 or fix_pos changed
+ * (update_constraint, main.cu:1988-2077). */
+int ppf_stepper_update_pins(ppf_stepper *h, const ppf_scene *scene, void *stream);
+
+/* Records of the stepper's last post-step intersection check (the first
+ * min(cap, 256) of them; contact::check_intersection, contact.cu:3056-3175).
+ * Returns the number of intersections that check found. */
+uint32_t ppf_stepper_intersection_records(ppf_stepper *h, ppf_isect_record *out,
+                                          uint32_t cap);
+
+/* Upper 3x3 blocks of the last contact matrix the stepper assembled (rows =
+ * vertices; self contact and collision mesh, before Dirichlet elimination).
+ * Writes *nnz; fills whichever of d_row_offset (nv + 1), d_col_index (nnz),
+ * d_value (9 nnz) is non-null. */
+int ppf_stepper_contact_pattern(ppf_stepper *h, uint32_t *nnz, uint32_t *d_row_offset,
+                                uint32_t *d_col_index, float *d_value, void *stream);
 
 #ifdef __cplusplus
 }

@@ -434,7 +434,8 @@ float oracle_line_search(const oracle_ls_mesh *m, const oracle_static_mesh *sm,
     for (uint32_t i = 0; i < nv; ++i) {
         const float ext = 0.5f * m->v_ghat[i] + m->v_off[i];
         V3 seg[2] = {x0[i], xs[i]};
-        const Box q = orc::make_box(seg, 2, ext);
+        Box q = orc::make_box(seg, 2, ext);
+        q.active = (m->vflag[i] & 4u) == 0; // collision window (contact.cu:2563)
         // point-face (contact.cu:2110-2164)
         orc::query(fb, q, [&](uint32_t fi) {
             const uint32_t *f = m->face + 3 * fi;
@@ -539,7 +540,7 @@ float oracle_line_search(const oracle_ls_mesh *m, const oracle_static_mesh *sm,
             q.mn[d] = std::min(b0.mn[d], b1.mn[d]);
             q.mx[d] = std::max(b0.mx[d], b1.mx[d]);
         }
-        q.active = true;
+        q.active = (m->eflag[ei] & 4u) == 0; // collision window (contact.cu:2688)
         float toi = seed;
         // edge-edge (contact.cu:2166-2264)
         orc::query(eb, q, [&](uint32_t ej) {

@@ -185,7 +185,8 @@ uint64_t oracle_proximity(const oracle_contact_mesh *m, oracle_pair *out,
     for (uint32_t leaf = l0; leaf < l1; ++leaf) {
         const uint32_t i = vb.node[2 * leaf] - 1;
         const V3 &p = x[i];
-        const Box q = point_box(p, 0.5f * m->v_ghat[i] + m->v_off[i]);
+        Box q = point_box(p, 0.5f * m->v_ghat[i] + m->v_off[i]);
+        q.active = (m->vflag[i] & 4u) == 0; // collision window (contact.cu:1637)
         // point-face
         query(fb, q, [&](uint32_t fi) {
             const uint32_t *f = m->face + 3 * fi;
@@ -326,7 +327,8 @@ uint64_t oracle_proximity(const oracle_contact_mesh *m, oracle_pair *out,
         const uint32_t ei = eb.node[2 * leaf] - 1;
         const uint32_t *ea = m->edge + 2 * ei;
         V3 pts[2] = {x[ea[0]], x[ea[1]]};
-        const Box q = make_box(pts, 2, 0.5f * m->e_ghat[ei] + m->e_off[ei]);
+        Box q = make_box(pts, 2, 0.5f * m->e_ghat[ei] + m->e_off[ei]);
+        q.active = (m->eflag[ei] & 4u) == 0; // collision window (contact.cu:1728)
         query(eb, q, [&](uint32_t ej) {
             const uint32_t *eb2 = m->edge + 2 * ej;
             bool either_dyn = (m->eflag[ei] & 1u) == 0 || (m->eflag[ej] & 1u) == 0;
@@ -433,7 +435,7 @@ uint64_t oracle_static_proximity(const oracle_contact_mesh *m, const oracle_stat
     };
     const V3 zero = v3(0, 0, 0);
     for (uint32_t i = 0; i < nv; ++i) {
-        if ((m->vflag[i] & 1u) != 0 || !(vmass[i] > 0.0f)) continue;
+        if ((m->vflag[i] & 5u) != 0 || !(vmass[i] > 0.0f)) continue;
         const V3 p = x[i];
         query(sfb, point_box(p, 0.5f * m->v_ghat[i] + m->v_off[i]), [&](uint32_t fi) {
             const uint32_t *f = sm->face + 3 * (size_t)fi;
@@ -471,7 +473,7 @@ uint64_t oracle_static_proximity(const oracle_contact_mesh *m, const oracle_stat
         });
     }
     for (uint32_t ei = 0; ei < ne; ++ei) {
-        if ((m->eflag[ei] & 1u) != 0 || !(emass[ei] > 0.0f)) continue;
+        if ((m->eflag[ei] & 5u) != 0 || !(emass[ei] > 0.0f)) continue;
         const V3 p0 = x[m->edge[2 * ei]], p1 = x[m->edge[2 * ei + 1]];
         const V3 seg[2] = {p0, p1};
         query(seb, orc::make_box(seg, 2, 0.5f * m->e_ghat[ei] + m->e_off[ei]), [&](uint32_t sj) {

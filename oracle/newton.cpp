@@ -109,7 +109,8 @@ struct oracle_scene {
     float *prev;
     const float *emass;
     const float *v_fric, *e_fric, *f_fric;
-    const float *fix_pos;
+    float *fix_pos;
+    const float *fix_delta;
 };
 
 struct oracle_step_params {
@@ -201,6 +202,15 @@ int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_resul
             break;
         }
         if (final_step) {
+            // rewind_kinematic_fix (main.cu:531-545): pins back to the
+            // integrated fraction of their step, in place
+            if (sc->fix_pos && sc->fix_delta && toi_advanced < 1.0) {
+                const float back = 1.0f - (float)toi_advanced;
+                for (uint32_t i = 0; i < n; ++i)
+                    if (sc->mesh.vflag[i] & 1u)
+                        for (int d = 0; d < 3; ++d)
+                            sc->fix_pos[3 * i + d] -= back * sc->fix_delta[3 * i + d];
+            }
             dt *= (float)toi_advanced;
             compute_target(dt);
         }
@@ -360,7 +370,8 @@ int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_resul
             sc->shell.fmass, n, nullptr, 0);
         if (out->intersections) out->status = 4;
     }
-    if (out->status == 0) {
+    // committed also when the check finds intersections (main.cu:1515-1629)
+    if (out->status == 0 || out->status == 4) {
         std::memcpy(sc->prev, curr, n3 * sizeof(float));
         std::memcpy(curr, ex.data(), n3 * sizeof(float));
     }

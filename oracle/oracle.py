@@ -504,7 +504,7 @@ def static_proximity(m, sm, vmass, emass, fmass):
 class OracleScene(ctypes.Structure):
     _fields_ = [("mesh", ContactMesh), ("shell", Shell), ("collision", StaticMesh),
                 ("prev", vp), ("emass", vp), ("v_fric", vp), ("e_fric", vp), ("f_fric", vp),
-                ("fix_pos", vp)]
+                ("fix_pos", vp), ("fix_delta", vp)]
 
 
 class OracleStepParams(ctypes.Structure):
@@ -549,12 +549,18 @@ class OracleStepper:
         self.st, self.keep_st = static_mesh_struct(sc.get("static"))
         self.keep = [_f32(sc["emass"]), _f32(sc["fric"]["v"]), _f32(sc["fric"]["e"]),
                      _f32(sc["fric"]["f"])]
-        self.fix = _f32(sc["fix_pos"]).reshape(-1) if sc.get("fix_pos") is not None else None
+        # fix_pos is rewound in place by the step: keep it in the scene dict
+        if sc.get("fix_pos") is not None:
+            sc["fix_pos"] = np.ascontiguousarray(sc["fix_pos"], np.float32)
+        self.fix = sc["fix_pos"].reshape(-1) if sc.get("fix_pos") is not None else None
+        self.delta = (_f32(sc["fix_delta"]).reshape(-1)
+                      if sc.get("fix_delta") is not None else None)
         self.scene = OracleScene(mesh=self.m, shell=self.s, collision=self.st,
                                  prev=_p(sc["prev"]), emass=_p(self.keep[0]),
                                  v_fric=_p(self.keep[1]), e_fric=_p(self.keep[2]),
                                  f_fric=_p(self.keep[3]),
-                                 fix_pos=_p(self.fix) if self.fix is not None else None)
+                                 fix_pos=_p(self.fix) if self.fix is not None else None,
+                                 fix_delta=_p(self.delta) if self.delta is not None else None)
 
     def advance(self, params):
         kw = dict(params)

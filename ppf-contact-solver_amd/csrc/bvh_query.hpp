@@ -49,6 +49,13 @@ __device__ inline QBox seg_box(f3 a, f3 b, float margin) {
     return q;
 }
 
+// A box that overlaps nothing: the query of an element outside its
+// collision window (an inactive query box, contact.cu:1637, 1728, 2563).
+__device__ inline QBox empty_box() {
+    const float inf = __builtin_inff();
+    return QBox{{inf, inf, inf}, {-inf, -inf, -inf}};
+}
+
 // Point +/- margin (aabb::make, contact/aabb.hpp:62-69).
 __device__ inline QBox point_box(f3 p, float margin) {
     return QBox{{p.x - margin, p.y - margin, p.z - margin},

@@ -143,6 +143,16 @@ void ppf_pcg_plan_destroy(ppf_pcg_plan *plan) {
     }
 }
 
+int ppf_pcg_plan_repattern(ppf_pcg_plan *plan, uint32_t nnz, const uint32_t *d_row_offset,
+                           const uint32_t *d_col_index, void *stream) {
+    if (!plan) {
+        set_error("ppf_pcg_plan_repattern: plan is null");
+        return PPF_ERR_ARG;
+    }
+    return pcg_plan_repattern(plan->impl, nnz, d_row_offset, d_col_index,
+                              (hipStream_t)stream);
+}
+
 int ppf_pcg_solve(ppf_pcg_plan *plan, const float *d_value,
                   const float *d_cdiag, const float *d_b, float *d_x, float tol,
                   uint32_t max_iter, ppf_pcg_result *out, void *stream) {
@@ -497,6 +507,49 @@ int ppf_stepper_advance(ppf_stepper *h, const ppf_scene *scene, const ppf_step_p
         return PPF_ERR_ARG;
     }
     return h->impl.advance(*scene, *p, out, (hipStream_t)stream);
+}
+
+int ppf_stepper_update_pins(ppf_stepper *h, const ppf_scene *scene, void *stream) {
+    if (!h || !scene) {
+        set_error("ppf_stepper_update_pins: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.refresh_pins(*scene, (hipStream_t)stream);
+}
+
+uint32_t ppf_stepper_intersection_records(ppf_stepper *h, ppf_isect_record *out,
+                                          uint32_t cap) {
+    return h ? h->impl.records(out, cap) : 0;
+}
+
+int ppf_stepper_contact_pattern(ppf_stepper *h, uint32_t *nnz, uint32_t *d_row_offset,
+                                uint32_t *d_col_index, float *d_value, void *stream) {
+    if (!h || !nnz) {
+        set_error("ppf_stepper_contact_pattern: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    const ContactAssembler &c = h->impl.contact();
+    const hipStream_t s = (hipStream_t)stream;
+    const uint64_t k = c.row_offset() ? c.nnz() : 0;
+    *nnz = (uint32_t)k;
+    const uint32_t nv = h->impl.nv();
+    if (d_row_offset) {
+        if (c.row_offset()) {
+            PPF_HIP(hipMemcpyAsync(d_row_offset, c.row_offset(), (nv + 1) * sizeof(uint32_t),
+                                   hipMemcpyDeviceToDevice, s));
+        } else {
+            PPF_HIP(hipMemsetAsync(d_row_offset, 0, (nv + 1) * sizeof(uint32_t), s));
+        }
+    }
+    if (k && d_col_index) {
+        PPF_HIP(hipMemcpyAsync(d_col_index, c.cols(), k * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, s));
+    }
+    if (k && d_value) {
+        PPF_HIP(hipMemcpyAsync(d_value, c.values(), 9 * k * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
+    }
+    return PPF_OK;
 }
 
 } // extern "C"

@@ -56,7 +56,9 @@ __global__ __launch_bounds__(kQT) void k_ls_vertex(LsCtx c) {
     const ppf_contact_mesh &m = c.m;
     const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
     const uint32_t i = c.vb.node[2 * (size_t)leaf] - 1;
-    const Box6 q = seg_box(X(m.x, i), XS(c, i), 0.5f * m.v_ghat[i] + m.v_off[i]);
+    const Box6 q = (m.vflag[i] & 4u)
+                       ? empty_box()
+                       : seg_box(X(m.x, i), XS(c, i), 0.5f * m.v_ghat[i] + m.v_off[i]);
     const bool idyn = (m.vflag[i] & 1u) == 0, icol = (m.vflag[i] & 2u) != 0;
     float best = max_t;
     // point-face (contact.cu:2110-2164)
@@ -192,6 +194,9 @@ __global__ __launch_bounds__(kQT) void k_ls_edge(LsCtx c) {
     for (int d = 0; d < 3; ++d) {
         q.mn[d] = fminf(b0.mn[d], b1.mn[d]);
         q.mx[d] = fmaxf(b0.mx[d], b1.mx[d]);
+    }
+    if (m.eflag[ei] & 4u) {
+        q = empty_box();
     }
     const bool edyn = (m.eflag[ei] & 1u) == 0;
     const bool acol = (m.vflag[a0] & 2u) != 0;

@@ -67,7 +67,8 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
     const uint32_t i = c.vb.node[2 * (size_t)leaf_pos] - 1;
     const f3 p = X(m, i);
     const float qm = 0.5f * m.v_ghat[i] + m.v_off[i];
-    const Q q{{p.x - qm, p.y - qm, p.z - qm}, {p.x + qm, p.y + qm, p.z + qm}};
+    const Q q = (m.vflag[i] & 4u) ? empty_box()
+                                  : Q{{p.x - qm, p.y - qm, p.z - qm}, {p.x + qm, p.y + qm, p.z + qm}};
     uint64_t slot = FILL ? (uint64_t)c.ofs[leaf_pos] : 0;
     uint32_t total = 0;
     // point-face (contact.cu:717-785)
@@ -229,6 +230,9 @@ __global__ __launch_bounds__(kQT) void k_edge_queries(Ctx c) {
             q.mx[d] += qm;
         }
     }
+    if (m.eflag[ei] & 4u) {
+        q = empty_box();
+    }
     const uint32_t qpos = c.vb.n + leaf_pos;
     uint64_t slot = FILL ? (uint64_t)c.ofs[qpos] : 0;
     const uint32_t total = traverse(c.eb, q, stk, c.overflow, [&](uint32_t ej) {
@@ -306,7 +310,7 @@ template <bool FILL> __global__ __launch_bounds__(kQT) void k_m2c(SCtx c) {
     const uint32_t i = c.vb.node[2 * (size_t)qpos] - 1;
     uint64_t slot = FILL ? (uint64_t)c.ofs[qpos] : 0;
     uint32_t total = 0;
-    if ((m.vflag[i] & 1u) == 0 && c.vmass[i] > 0.0f) {
+    if ((m.vflag[i] & 5u) == 0 && c.vmass[i] > 0.0f) {
         const f3 p = X(m, i);
         const QBox q = point_box(p, 0.5f * m.v_ghat[i] + m.v_off[i]);
         total = traverse(c.sfb, q, stk, c.overflow, [&](uint32_t fi) {
@@ -381,7 +385,7 @@ template <bool FILL> __global__ __launch_bounds__(kQT) void k_cm_ee(SCtx c) {
     const uint32_t ei = c.eb.node[2 * (size_t)lp] - 1;
     uint64_t slot = FILL ? (uint64_t)c.ofs[qpos] : 0;
     uint32_t total = 0;
-    if ((m.eflag[ei] & 1u) == 0 && c.emass[ei] > 0.0f) {
+    if ((m.eflag[ei] & 5u) == 0 && c.emass[ei] > 0.0f) {
         const uint32_t a0 = m.edge[2 * (size_t)ei], a1 = m.edge[2 * (size_t)ei + 1];
         const f3 p0 = X(m, a0), p1 = X(m, a1);
         const QBox q = seg_box(p0, p1, 0.5f * m.e_ghat[ei] + m.e_off[ei]);
@@ -449,6 +453,24 @@ int Narrow::build_trees(const ppf_contact_mesh &m, hipStream_t s) {
     }
     if (rc == PPF_OK) {
         rc = vert_.build(v, m.x, nullptr, 0.0f, s);
+    }
+    return rc;
+}
+
+int Narrow::refit_trees(const ppf_contact_mesh &m, hipStream_t s) {
+    if (face_.view().n != m.nf || edge_.view().n != m.ne || vert_.view().n != m.nv) {
+        return build_trees(m, s);
+    }
+    PrimSet f, e, v;
+    f.n = m.nf, f.arity = 3, f.elem = m.face, f.ghat = m.f_ghat, f.offset = m.f_off;
+    e.n = m.ne, e.arity = 2, e.elem = m.edge, e.ghat = m.e_ghat, e.offset = m.e_off;
+    v.n = m.nv, v.arity = 1, v.elem = nullptr, v.ghat = m.v_ghat, v.offset = m.v_off;
+    int rc = face_.refit(f, m.x, nullptr, 0.0f, s);
+    if (rc == PPF_OK) {
+        rc = edge_.refit(e, m.x, nullptr, 0.0f, s);
+    }
+    if (rc == PPF_OK) {
+        rc = vert_.refit(v, m.x, nullptr, 0.0f, s);
     }
     return rc;
 }

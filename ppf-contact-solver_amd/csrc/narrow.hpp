@@ -11,6 +11,9 @@ class Narrow {
   public:
     ~Narrow();
     int build_trees(const ppf_contact_mesh &m, hipStream_t s);
+    // Refit the boxes of the last build at new positions (same topology); a
+    // size change rebuilds.
+    int refit_trees(const ppf_contact_mesh &m, hipStream_t s);
     // All stage-0 pairs; writes min(count, cap) pairs to `out` (may be null to
     // count only). Returns PPF_ERR_CAPACITY if a traversal stack overflowed.
     int proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,

@@ -54,6 +54,18 @@ __global__ void k_target(uint32_t n, const float *__restrict__ curr,
     store3(target + 3 * (size_t)i, x1 + y);
 }
 
+// Walk pins back to the fraction of their step the loop integrated
+// (main.cu:531-545): fix -= back * delta (a static pin's delta is zero).
+__global__ void k_rewind(uint32_t n, const uint32_t *__restrict__ vflag, float *__restrict__ fix,
+                         const float *__restrict__ delta, float back) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= n || !(vflag[i] & 1u)) {
+        return;
+    }
+    const f3 d = X(delta, i);
+    store3(fix + 3 * (size_t)i, X(fix, i) - mk3(back * d.x, back * d.y, back * d.z));
+}
+
 // dx = eval_x - target on pinned vertices, 0 elsewhere (main.cu:788-801).
 __global__ void k_dx_init(uint32_t n, const uint32_t *__restrict__ mask,
                           const float *__restrict__ ex, const float *__restrict__ target,
@@ -246,6 +258,10 @@ void Stepper::release() {
     if (h_scal_) {
         (void)hipHostFree(h_scal_);
     }
+    if (plan_) {
+        pcg_plan_destroy(plan_);
+        plan_ = nullptr;
+    }
     for (hipEvent_t &e : ev_) {
         if (e) {
             (void)hipEventDestroy(e);
@@ -289,10 +305,14 @@ int Stepper::create(const ppf_scene &sc, hipStream_t s) {
     for (hipEvent_t &e : ev_) {
         PPF_HIP(hipEventCreate(&e));
     }
+    return refresh_pins(sc, s);
+}
+
+int Stepper::refresh_pins(const ppf_scene &sc, hipStream_t s) {
     k_mask<<<cdiv(nv_, kNT), kNT, 0, s>>>(nv_, sc.mesh.vflag, sc.fix_pos, mask_);
     PPF_HIP(hipGetLastError());
-    // Pinned vertices are fixed for the scene: count them once (Dirichlet
-    // elimination runs only when there are any).
+    // Count the pinned vertices once per pin set (Dirichlet elimination runs
+    // only when there are any).
     npinned_ = 0;
     if (sc.fix_pos && nv_) {
         std::vector<uint32_t> h(nv_);
@@ -363,7 +383,9 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
     m.x = ex_;
     uint64_t npairs = 0;
     if (!p.disable_contact) {
-        rc = narrow_.build_trees(m, s);
+        // Trees are built once per step and refit per iteration (the pair set
+        // depends only on the boxes containing the primitives).
+        rc = first_iteration ? narrow_.build_trees(m, s) : narrow_.refit_trees(m, s);
         if (rc != PPF_OK) {
             return rc;
         }
@@ -453,14 +475,14 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         k_dirichlet_diag<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, ex_, target_, cdiag_, force_);
         PPF_HIP(hipGetLastError());
     }
-    PcgPlanImpl *plan = nullptr;
-    rc = pcg_plan_create(n, nnz, mrp_, mci_, s, &plan);
+    // One solver plan per scene: later iterations only swap the pattern in.
+    rc = plan_ ? pcg_plan_repattern(plan_, nnz, mrp_, mci_, s)
+               : pcg_plan_create(n, nnz, mrp_, mci_, s, &plan_);
     if (rc != PPF_OK) {
         return rc;
     }
     ppf_pcg_result pr{};
-    rc = pcg_solve(plan, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr, &pr, s);
-    pcg_plan_destroy(plan);
+    rc = pcg_solve(plan_, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr, &pr, s);
     if (rc != PPF_OK) {
         return rc;
     }
@@ -529,9 +551,18 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
     return PPF_OK;
 }
 
+uint32_t Stepper::records(ppf_isect_record *out, uint32_t cap) const {
+    const uint32_t keep = std::min(std::min(cap, kMaxRecords), records_count_);
+    for (uint32_t k = 0; out && k < keep; ++k) {
+        out[k] = records_[k];
+    }
+    return records_count_;
+}
+
 int Stepper::advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_result *out,
                      hipStream_t s) {
     *out = ppf_step_result{};
+    records_count_ = 0;
     const uint32_t n = nv_;
     const size_t n3 = 3 * (size_t)n;
     PPF_HIP(hipEventRecord(ev_[5], s));
@@ -551,6 +582,11 @@ int Stepper::advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_res
             break;
         }
         if (final_step) {
+            if (sc.fix_pos && sc.fix_delta && toi_advanced < 1.0) {
+                k_rewind<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.vflag, sc.fix_pos, sc.fix_delta,
+                                                      1.0f - (float)toi_advanced);
+                PPF_HIP(hipGetLastError());
+            }
             dt *= (float)toi_advanced;
             k_target<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.x, sc.prev, sc.mesh.vflag,
                                                    sc.fix_pos, dt, prev_dt, g, target_);
@@ -591,16 +627,17 @@ int Stepper::advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_res
         uint32_t count = 0;
         rc = isect_.run(narrow_.tree(0).view(), narrow_.tree(1).view(), narrow_.tree(2).view(),
                         m, sc.collision.nv ? &sc.collision : nullptr, sc.shell.vmass,
-                        sc.emass, sc.shell.fmass, n, nullptr, 0, &count, s);
+                        sc.emass, sc.shell.fmass, n, records_, kMaxRecords, &count, s);
         if (rc != PPF_OK) {
             return rc;
         }
+        records_count_ = count;
         out->intersections = count;
         if (count) {
             out->status = 4;
         }
     }
-    if (out->status == 0) {
+    if (out->status == 0 || out->status == 4) {
         // prev <- curr, curr <- iterate
         PPF_HIP(hipMemcpyAsync(sc.prev, sc.mesh.x, n3 * sizeof(float), hipMemcpyDeviceToDevice,
                                s));

@@ -15,9 +15,16 @@ class Stepper {
   public:
     ~Stepper();
     int create(const ppf_scene &sc, hipStream_t s);
+    // Re-read the pin set (mesh.vflag bit0, fix_pos) after it changed.
+    int refresh_pins(const ppf_scene &sc, hipStream_t s);
     int advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_result *out,
                 hipStream_t s);
     const float *eval_x() const { return ex_; }
+    uint32_t nv() const { return nv_; }
+    static constexpr uint32_t kMaxRecords = 256;
+    // Records of the last post-step intersection check.
+    uint32_t records(ppf_isect_record *out, uint32_t cap) const;
+    const ContactAssembler &contact() const { return contact_; }
 
   private:
     int newton_iteration(const ppf_scene &sc, const ppf_step_params &p, float dt,
@@ -41,6 +48,9 @@ class Stepper {
     float *mval_ = nullptr;
     uint64_t mcap_ = 0;
     uint32_t npinned_ = 0;
+    PcgPlanImpl *plan_ = nullptr;
+    ppf_isect_record records_[kMaxRecords] = {};
+    uint32_t records_count_ = 0;
     hipEvent_t ev_[8] = {};
 };
 

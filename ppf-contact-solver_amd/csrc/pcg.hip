@@ -658,8 +658,14 @@ struct PcgPlanImpl {
     Args h{};             // host copy of the kernel arguments
     Args *d_args = nullptr;
     Scalars *h_scal = nullptr; // pinned
-    void *mem = nullptr;       // one device allocation for all scratch
+    void *mem = nullptr;       // one device allocation for the n-sized scratch
+    uint32_t *tp = nullptr, *tfill = nullptr, *dslot = nullptr;
+    uint32_t *scan_tmp = nullptr, *flags = nullptr;
+    uint32_t *pat = nullptr;   // transpose columns | slots, grown with nnz
+    size_t pat_cap = 0;
     void *ell = nullptr;       // ELL entry table (when rows are short)
+    size_t ell_cap = 0;        // entries
+    uint32_t graph_ell_m = 0xffffffffu; // ELL width the graph was captured for
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -707,119 +713,71 @@ hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
 
 } // namespace
 
-int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
-                    const uint32_t *col, hipStream_t st, PcgPlanImpl **out) {
-    if (!out || n == 0 || !rp || !col || nnz >= kMirror) {
-        set_error("pcg_plan_create: invalid argument");
-        return PPF_ERR_ARG;
-    }
-    auto *pl = new (std::nothrow) PcgPlanImpl();
-    if (!pl) {
-        return PPF_ERR_ARG;
-    }
-    pl->n = n;
-    pl->nnz = nnz;
-    Args &h = pl->h;
-    h.n = n;
-    // Row ranges are multiples of 64 rows: one pass of either SpMV form.
-    h.G1 = round8(std::min<uint32_t>(2048u, cdiv(n, 64)));
-    h.R1 = cdiv(cdiv(n, h.G1), 64) * 64;
-    h.G2 = std::min<uint32_t>(1024u, cdiv(n, kNT));
-    h.R2 = cdiv(cdiv(n, h.G2), kNT) * kNT;
-    h.G3 = std::min<uint32_t>(1024u, cdiv(3 * n, kNT));
-    h.R3 = cdiv(cdiv(3 * n, h.G3), kNT) * kNT;
+namespace {
 
-    const size_t n3 = 3 * (size_t)n;
-    const size_t scan_words = scan_scratch_words(n + 1);
-    size_t bytes = carve_bytes(n + 1, 4) * 2 + carve_bytes(nnz, 4) * 2 +
-                   carve_bytes(n, 4) + carve_bytes(9 * (size_t)n, 4) +
-                   carve_bytes(n3, 4) * 4 +
-                   carve_bytes(2 * (size_t)h.G1 + 2 * (size_t)h.G2, 4) +
-                   carve_bytes(1, sizeof(Scalars)) + carve_bytes(1, sizeof(Args)) +
-                   carve_bytes(scan_words, 4) + carve_bytes(4, 4);
-    hipError_t e = hipMalloc(&pl->mem, bytes);
-    if (e != hipSuccess) {
-        set_error("pcg_plan_create: hipMalloc(%zu): %s", bytes,
-                  hipGetErrorString(e));
-        delete pl;
-        return PPF_ERR_HIP;
+// Transpose table, row-length choice of the SpMV form and the ELL table for
+// a pattern. Buffers sized by nnz (or n * row width) grow when needed.
+int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                hipStream_t st) {
+    const uint32_t n = pl->n;
+    Args &h = pl->h;
+    if (nnz >= kMirror) {
+        set_error("pcg: %u stored blocks exceed the entry index range", nnz);
+        return PPF_ERR_ARG;
     }
-    char *cur = static_cast<char *>(pl->mem);
-    uint32_t *tp = carve<uint32_t>(cur, n + 1);
-    uint32_t *tfill = carve<uint32_t>(cur, n + 1);
-    uint32_t *tcol = carve<uint32_t>(cur, nnz);
-    uint32_t *tslot = carve<uint32_t>(cur, nnz);
-    uint32_t *dslot = carve<uint32_t>(cur, n);
-    h.P = carve<float>(cur, 9 * (size_t)n);
-    h.r = carve<float>(cur, n3);
-    h.z = carve<float>(cur, n3);
-    h.p = carve<float>(cur, n3);
-    h.Ap = carve<float>(cur, n3);
-    h.part = carve<float>(cur, 2 * (size_t)h.G1 + 2 * (size_t)h.G2);
-    h.s = carve<Scalars>(cur, 1);
-    pl->d_args = carve<Args>(cur, 1);
-    uint32_t *scan_tmp = carve<uint32_t>(cur, scan_words);
-    uint32_t *flags = carve<uint32_t>(cur, 4);
+    if (2 * (size_t)nnz > pl->pat_cap) {
+        if (pl->pat) {
+            PPF_HIP(hipFree(pl->pat));
+            pl->pat = nullptr;
+        }
+        const size_t cap = 2 * ((size_t)nnz + nnz / 4 + 256);
+        PPF_HIP(hipMalloc(&pl->pat, cap * sizeof(uint32_t)));
+        pl->pat_cap = cap;
+    }
+    uint32_t *tcol = pl->pat, *tslot = pl->pat + pl->pat_cap / 2;
+    pl->nnz = nnz;
     h.rp = rp;
     h.col = col;
-    h.tp = tp;
+    h.tp = pl->tp;
     h.tcol = tcol;
     h.tslot = tslot;
-    h.dslot = dslot;
-
-    auto fail = [&](const char *what, hipError_t err) {
-        set_error("pcg_plan_create: %s: %s", what, hipGetErrorString(err));
-        ppf::pcg_plan_destroy(pl);
-        return PPF_ERR_HIP;
-    };
-#define PLAN_HIP(call)                                                         \
-    do {                                                                       \
-        hipError_t e_ = (call);                                                \
-        if (e_ != hipSuccess)                                                  \
-            return fail(#call, e_);                                            \
-    } while (0)
-
-    PLAN_HIP(hipMemsetAsync(tp, 0, (n + 1) * sizeof(uint32_t), st));
-    PLAN_HIP(hipMemsetAsync(tfill, 0, (n + 1) * sizeof(uint32_t), st));
-    PLAN_HIP(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), st));
+    h.dslot = pl->dslot;
+    uint32_t *flags = pl->flags;
+    PPF_HIP(hipMemsetAsync(pl->tp, 0, (n + 1) * sizeof(uint32_t), st));
+    PPF_HIP(hipMemsetAsync(pl->tfill, 0, (n + 1) * sizeof(uint32_t), st));
+    PPF_HIP(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), st));
     const unsigned gb = cdiv(n, kNT);
     uint32_t bad = 0;
     k_check_offsets<<<cdiv(n + 1, kNT), kNT, 0, st>>>(n, nnz, rp, flags + 1);
-    PLAN_HIP(hipGetLastError());
-    PLAN_HIP(hipMemcpyAsync(&bad, flags + 1, sizeof(uint32_t),
-                            hipMemcpyDeviceToHost, st));
-    PLAN_HIP(hipStreamSynchronize(st));
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipMemcpyAsync(&bad, flags + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PPF_HIP(hipStreamSynchronize(st));
     if (bad) {
-        set_error("pcg_plan_create: row_offset is not a valid CSR offset array "
-                  "(0 .. nnz, non-decreasing)");
-        ppf::pcg_plan_destroy(pl);
+        set_error("pcg: row_offset is not a valid CSR offset array (0 .. nnz, non-decreasing)");
         return PPF_ERR_ARG;
     }
-    k_count_transpose<<<gb, kNT, 0, st>>>(n, rp, col, tp, dslot, flags);
-    PLAN_HIP(hipGetLastError());
-    PLAN_HIP(exclusive_scan_u32(tp, tp, n + 1, scan_tmp, st));
-    k_fill_transpose<<<gb, kNT, 0, st>>>(n, rp, col, tp, tfill, tcol, tslot);
-    PLAN_HIP(hipGetLastError());
-    k_sort_transpose<<<gb, kNT, 0, st>>>(n, tp, tcol, tslot);
-    PLAN_HIP(hipGetLastError());
-    PLAN_HIP(hipMemcpyAsync(&bad, flags, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                            st));
-    PLAN_HIP(hipStreamSynchronize(st));
+    k_count_transpose<<<gb, kNT, 0, st>>>(n, rp, col, pl->tp, pl->dslot, flags);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(exclusive_scan_u32(pl->tp, pl->tp, n + 1, pl->scan_tmp, st));
+    k_fill_transpose<<<gb, kNT, 0, st>>>(n, rp, col, pl->tp, pl->tfill, tcol, tslot);
+    PPF_HIP(hipGetLastError());
+    k_sort_transpose<<<gb, kNT, 0, st>>>(n, pl->tp, tcol, tslot);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipMemcpyAsync(&bad, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PPF_HIP(hipStreamSynchronize(st));
     if (bad) {
-        set_error("pcg_plan_create: %u pattern entries are not strictly "
-                  "ascending upper-triangle columns < n",
+        set_error("pcg: %u pattern entries are not strictly ascending upper-triangle "
+                  "columns < n",
                   bad);
-        ppf::pcg_plan_destroy(pl);
         return PPF_ERR_ARG;
     }
     // Choose the SpMV form from the longest row (stored + mirrored blocks).
-    PLAN_HIP(hipMemsetAsync(flags + 2, 0, sizeof(uint32_t), st));
-    k_row_len_max<<<gb, kNT, 0, st>>>(n, rp, tp, flags + 2);
-    PLAN_HIP(hipGetLastError());
+    PPF_HIP(hipMemsetAsync(flags + 2, 0, sizeof(uint32_t), st));
+    k_row_len_max<<<gb, kNT, 0, st>>>(n, rp, pl->tp, flags + 2);
+    PPF_HIP(hipGetLastError());
     uint32_t maxlen = 0;
-    PLAN_HIP(hipMemcpyAsync(&maxlen, flags + 2, sizeof(uint32_t),
-                            hipMemcpyDeviceToHost, st));
-    PLAN_HIP(hipStreamSynchronize(st));
+    PPF_HIP(hipMemcpyAsync(&maxlen, flags + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PPF_HIP(hipStreamSynchronize(st));
     h.ell_m = 0;
     for (uint32_t m : {1u, 2u, 3u, 4u, 6u, 8u}) {
         // Padding entries read block 0, so the ELL form needs nnz >= 1.
@@ -831,20 +789,34 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     h.ent = nullptr;
     if (h.ell_m) {
         const uint32_t W = 4 * h.ell_m;
-        PLAN_HIP(hipMalloc(&pl->ell, (size_t)n * W * sizeof(uint2)));
+        if ((size_t)n * W > pl->ell_cap) {
+            if (pl->ell) {
+                PPF_HIP(hipFree(pl->ell));
+                pl->ell = nullptr;
+            }
+            PPF_HIP(hipMalloc(&pl->ell, (size_t)n * W * sizeof(uint2)));
+            pl->ell_cap = (size_t)n * W;
+        }
         h.ent = static_cast<const uint2 *>(pl->ell);
-        k_build_ell<<<gb, kNT, 0, st>>>(n, W, rp, col, tp, tcol, tslot,
+        k_build_ell<<<gb, kNT, 0, st>>>(n, W, rp, col, pl->tp, tcol, tslot,
                                         static_cast<uint2 *>(pl->ell));
-        PLAN_HIP(hipGetLastError());
+        PPF_HIP(hipGetLastError());
     }
-    PLAN_HIP(hipHostMalloc(reinterpret_cast<void **>(&pl->h_scal),
-                           sizeof(Scalars), hipHostMallocDefault));
+    return PPF_OK;
+}
 
-    // Capture kGraphIters iterations once; they read every pointer through
-    // d_args, so one graph serves every solve on this plan.
-    PLAN_HIP(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
+// Capture kGraphIters iterations; they read every pointer through d_args, so
+// the graph serves every solve and pattern of the plan with the same SpMV form.
+int capture_graph(PcgPlanImpl *pl) {
+    if (pl->exec) {
+        (void)hipGraphExecDestroy(pl->exec);
+        pl->exec = nullptr;
+    }
+    if (!pl->cap) {
+        PPF_HIP(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
+    }
     hipGraph_t g = nullptr;
-    PLAN_HIP(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
+    PPF_HIP(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
     hipError_t le = hipSuccess;
     for (uint32_t k = 0; k < (uint32_t)kGraphIters; ++k) {
         hipError_t e2 = launch_iteration(*pl, k, pl->cap);
@@ -857,17 +829,100 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
         ce = le;
     }
     if (ce != hipSuccess) {
-        return fail("hipStreamEndCapture", ce);
+        set_error("pcg: graph capture: %s", hipGetErrorString(ce));
+        return PPF_ERR_HIP;
     }
     ce = hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ce != hipSuccess) {
         pl->exec = nullptr;
-        return fail("hipGraphInstantiate", ce);
+        set_error("pcg: hipGraphInstantiate: %s", hipGetErrorString(ce));
+        return PPF_ERR_HIP;
     }
-#undef PLAN_HIP
+    pl->graph_ell_m = pl->h.ell_m;
+    return PPF_OK;
+}
+
+} // namespace
+
+int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
+                    const uint32_t *col, hipStream_t st, PcgPlanImpl **out) {
+    if (!out || n == 0 || !rp || !col || nnz >= kMirror) {
+        set_error("pcg_plan_create: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    auto *pl = new (std::nothrow) PcgPlanImpl();
+    if (!pl) {
+        return PPF_ERR_ARG;
+    }
+    pl->n = n;
+    Args &h = pl->h;
+    h.n = n;
+    // Row ranges are multiples of 64 rows: one pass of either SpMV form.
+    h.G1 = round8(std::min<uint32_t>(2048u, cdiv(n, 64)));
+    h.R1 = cdiv(cdiv(n, h.G1), 64) * 64;
+    h.G2 = std::min<uint32_t>(1024u, cdiv(n, kNT));
+    h.R2 = cdiv(cdiv(n, h.G2), kNT) * kNT;
+    h.G3 = std::min<uint32_t>(1024u, cdiv(3 * n, kNT));
+    h.R3 = cdiv(cdiv(3 * n, h.G3), kNT) * kNT;
+
+    const size_t n3 = 3 * (size_t)n;
+    const size_t scan_words = scan_scratch_words(n + 1);
+    size_t bytes = carve_bytes(n + 1, 4) * 2 + carve_bytes(n, 4) +
+                   carve_bytes(9 * (size_t)n, 4) + carve_bytes(n3, 4) * 4 +
+                   carve_bytes(2 * (size_t)h.G1 + 2 * (size_t)h.G2, 4) +
+                   carve_bytes(1, sizeof(Scalars)) + carve_bytes(1, sizeof(Args)) +
+                   carve_bytes(scan_words, 4) + carve_bytes(4, 4);
+    hipError_t e = hipMalloc(&pl->mem, bytes);
+    if (e != hipSuccess) {
+        set_error("pcg_plan_create: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        delete pl;
+        return PPF_ERR_HIP;
+    }
+    char *cur = static_cast<char *>(pl->mem);
+    pl->tp = carve<uint32_t>(cur, n + 1);
+    pl->tfill = carve<uint32_t>(cur, n + 1);
+    pl->dslot = carve<uint32_t>(cur, n);
+    h.P = carve<float>(cur, 9 * (size_t)n);
+    h.r = carve<float>(cur, n3);
+    h.z = carve<float>(cur, n3);
+    h.p = carve<float>(cur, n3);
+    h.Ap = carve<float>(cur, n3);
+    h.part = carve<float>(cur, 2 * (size_t)h.G1 + 2 * (size_t)h.G2);
+    h.s = carve<Scalars>(cur, 1);
+    pl->d_args = carve<Args>(cur, 1);
+    pl->scan_tmp = carve<uint32_t>(cur, scan_words);
+    pl->flags = carve<uint32_t>(cur, 4);
+    e = hipHostMalloc(reinterpret_cast<void **>(&pl->h_scal), sizeof(Scalars),
+                      hipHostMallocDefault);
+    if (e != hipSuccess) {
+        set_error("pcg_plan_create: hipHostMalloc: %s", hipGetErrorString(e));
+        ppf::pcg_plan_destroy(pl);
+        return PPF_ERR_HIP;
+    }
+    int rc = set_pattern(pl, nnz, rp, col, st);
+    if (rc == PPF_OK) {
+        rc = capture_graph(pl);
+    }
+    if (rc != PPF_OK) {
+        ppf::pcg_plan_destroy(pl);
+        return rc;
+    }
     *out = pl;
     return PPF_OK;
+}
+
+int pcg_plan_repattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                       hipStream_t st) {
+    if (!pl || !rp || !col) {
+        set_error("pcg_plan_repattern: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    int rc = set_pattern(pl, nnz, rp, col, st);
+    if (rc == PPF_OK && pl->h.ell_m != pl->graph_ell_m) {
+        rc = capture_graph(pl);
+    }
+    return rc;
 }
 
 void pcg_plan_destroy(PcgPlanImpl *pl) {
@@ -885,6 +940,9 @@ void pcg_plan_destroy(PcgPlanImpl *pl) {
     }
     if (pl->mem) {
         (void)hipFree(pl->mem);
+    }
+    if (pl->pat) {
+        (void)hipFree(pl->pat);
     }
     if (pl->ell) {
         (void)hipFree(pl->ell);

@@ -10,6 +10,10 @@ struct PcgPlanImpl;
 int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
                     const uint32_t *col, hipStream_t st, PcgPlanImpl **out);
 void pcg_plan_destroy(PcgPlanImpl *pl);
+// Point the plan at a new pattern of the same n (buffers grow as needed; the
+// iteration graph is re-captured only when the SpMV form changes).
+int pcg_plan_repattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                       hipStream_t st);
 int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
               const float *b, float *x, float tol, uint32_t max_iter,
               float *err_hist, ppf_pcg_result *res, hipStream_t st);

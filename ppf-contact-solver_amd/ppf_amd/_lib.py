@@ -28,6 +28,7 @@ _PROTOS = {
     "ppf_device_info": (ctypes.c_int, [ctypes.c_char_p, u32, ctypes.POINTER(u32)]),
     "ppf_pcg_plan_create": (ctypes.c_int, [u32, u32, vp, vp, vp, ctypes.POINTER(vp)]),
     "ppf_pcg_plan_destroy": (None, [vp]),
+    "ppf_pcg_plan_repattern": (ctypes.c_int, [vp, u32, vp, vp, vp]),
     "ppf_pcg_solve": (ctypes.c_int, [vp, vp, vp, vp, vp, f32, u32,
                                      ctypes.POINTER(PcgResult), vp]),
     "ppf_pcg_solve_hist": (ctypes.c_int, [vp, vp, vp, vp, vp, f32, u32, vp,
@@ -63,6 +64,9 @@ _PROTOS = {
     "ppf_stepper_create": (ctypes.c_int, [vp, vp, ctypes.POINTER(vp)]),
     "ppf_stepper_destroy": (None, [vp]),
     "ppf_stepper_advance": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "ppf_stepper_update_pins": (ctypes.c_int, [vp, vp, vp]),
+    "ppf_stepper_intersection_records": (u32, [vp, vp, u32]),
+    "ppf_stepper_contact_pattern": (ctypes.c_int, [vp, ctypes.POINTER(u32), vp, vp, vp, vp]),
     "ppf_ccd_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
     "ppf_ccd_destroy": (None, [vp]),
     "ppf_line_search": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int,

@@ -648,3 +648,81 @@ def step_params(dt=1.0 / 60.0, **kw):
              check_intersection=1)
     p.update(kw)
     return p
+
+
+def cylinder(radius=0.2, length=1.2, n_around=64, n_along=24):
+    """Closed-side cylinder surface along z (no caps), centred at the origin:
+    n_along rings of n_around vertices, alternating diagonals."""
+    th = 2 * np.pi * np.arange(n_around) / n_around
+    z = np.linspace(-length / 2, length / 2, n_along)
+    x = np.stack([np.tile(radius * np.cos(th), n_along), np.tile(radius * np.sin(th), n_along),
+                  np.repeat(z, n_around)], 1)
+    faces = []
+    for j in range(n_along - 1):
+        for i in range(n_around):
+            a = j * n_around + i
+            b = j * n_around + (i + 1) % n_around
+            c = a + n_around
+            d = b + n_around
+            faces += [(a, b, d), (a, d, c)] if (i + j) % 2 else [(a, b, c), (b, d, c)]
+    return x.astype(np.float32), np.array(faces, np.uint32)
+
+
+def stacked_drape_scene(res=256, layers=8, gap=2e-3, noise=1e-4, size=1.0, radius=0.2,
+                        clearance=5e-3, seed=3, ghat=1e-3, friction=0.5, density=0.2,
+                        young=1e4, poisson=0.25, bend=10.0, n_around=96, n_along=32):
+    """Stacked-layer drape: `layers` square sheets (res x res, side `size`,
+    xz plane) stacked `gap` apart with N(0, noise) vertex noise, the lowest
+    `clearance` above a horizontal cylinder of `radius` along z. The cylinder
+    is part of the dynamic mesh with every vertex pinned (Dirichlet), its
+    faces and hinges excluded from the elastic terms, so it can be driven
+    kinematically (rotation: `spin_fix_positions`). Same dict layout as
+    drape_scene, plus 'cyl' (indices of the cylinder vertices) and 'cyl_rest'."""
+    rng = np.random.default_rng(seed)
+    f0 = rect_faces(res, res).astype(np.int64)
+    i, j = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    h = size / (res - 1)
+    xs, fs, rests = [], [], []
+    for k in range(layers):
+        p = np.stack([i.reshape(-1) * h - size / 2, np.zeros(res * res),
+                      j.reshape(-1) * h - size / 2], 1)
+        rests.append(p.copy())
+        p[:, 1] = radius + clearance + k * gap + rng.normal(0.0, noise, res * res)
+        xs.append(p)
+        fs.append(f0 + k * res * res)
+    cx, cf = cylinder(radius, 1.2 * size, n_around, n_along)
+    ns = layers * res * res
+    x = np.concatenate(xs + [cx.astype(np.float64)])
+    rest = np.concatenate(rests + [cx.astype(np.float64)])
+    faces = np.concatenate(fs + [cf.astype(np.int64) + ns]).astype(np.uint32)
+    nfs = layers * len(f0)
+    shell = shell_rest(rest, faces, density, young, poisson, bend)
+    mesh = contact_mesh(x.astype(np.float32), faces, ghat=ghat, offset=0.0)
+    cyl = np.arange(ns, len(x))
+    mesh["vflag"][cyl] |= 1
+    shell["vflag"] = mesh["vflag"].copy()
+    shell["fflag"][nfs:] = 1
+    hv = shell["hinge"].astype(np.int64)
+    shell["hflag"][(hv >= ns).any(1)] = 1
+    mesh["fflag"][nfs:] = 1
+    e = mesh["edge"].astype(np.int64)
+    mesh["eflag"][(e >= ns).all(1)] = 1
+    emass = (0.5 * (shell["vmass"][e[:, 0]] + shell["vmass"][e[:, 1]])).astype(np.float32)
+    fric = {"v": np.full(mesh["nv"], friction, np.float32),
+            "e": np.full(mesh["ne"], friction, np.float32),
+            "f": np.full(mesh["nf"], friction, np.float32)}
+    static = None
+    return dict(mesh=mesh, shell=shell, static=static, prev=mesh["x"].copy(), emass=emass,
+                fric=fric, fix_pos=mesh["x"].copy(), cyl=cyl, cyl_rest=cx)
+
+
+def spin_fix_positions(sc, t, omega=2.0):
+    """Fix positions of the cylinder rotated by omega * t about z (all other
+    entries keep their values)."""
+    c, s = np.cos(omega * t), np.sin(omega * t)
+    r = sc["cyl_rest"].astype(np.float64)
+    fix = sc["fix_pos"]
+    fix[sc["cyl"], 0] = c * r[:, 0] - s * r[:, 1]
+    fix[sc["cyl"], 1] = s * r[:, 0] + c * r[:, 1]
+    fix[sc["cyl"], 2] = r[:, 2]
+    return fix

@@ -46,6 +46,17 @@ class BlockJacobiPCG:
                                            ctypes.byref(h)), "ppf_pcg_plan_create")
         self._h = h
 
+    def repattern(self, row_offset, col_index):
+        """Switch to another pattern with the same number of rows."""
+        nnz = col_index.numel()
+        _need(row_offset, torch.int32, self.n + 1, "row_offset")
+        _need(col_index, torch.int32, nnz, "col_index")
+        check(self.lib.ppf_pcg_plan_repattern(self._h, nnz, _ptr(row_offset),
+                                              _ptr(col_index), _stream()),
+              "ppf_pcg_plan_repattern")
+        self.nnz = nnz
+        self._keep = (row_offset, col_index)
+
     def close(self):
         if getattr(self, "_h", None):
             self.lib.ppf_pcg_plan_destroy(self._h)

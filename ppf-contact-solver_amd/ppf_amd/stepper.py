@@ -25,7 +25,7 @@ class SceneC(ctypes.Structure):
     """ppf_scene (include/ppf_mi355x.h)."""
     _fields_ = [("mesh", ContactMeshC), ("shell", ShellC), ("collision", StaticMeshC),
                 ("prev", vp), ("emass", vp), ("v_fric", vp), ("e_fric", vp), ("f_fric", vp),
-                ("fix_pos", vp)]
+                ("fix_pos", vp), ("fix_delta", vp)]
 
 
 class StepParamsC(ctypes.Structure):
@@ -75,6 +75,7 @@ def scene_to_device(sc, dev):
     t["emass"] = f(sc["emass"])
     t["fric"] = {k: f(v) for k, v in sc["fric"].items()}
     t["fix_pos"] = f(sc["fix_pos"]) if sc.get("fix_pos") is not None else None
+    t["fix_delta"] = f(sc["fix_delta"]) if sc.get("fix_delta") is not None else None
     return t
 
 
@@ -88,7 +89,8 @@ def scene_struct(t):
     return SceneC(mesh=mesh_struct(t["mesh"]), shell=ShellC(**fields), collision=st,
                   prev=_ptr(t["prev"]), emass=_ptr(t["emass"]), v_fric=_ptr(t["fric"]["v"]),
                   e_fric=_ptr(t["fric"]["e"]), f_fric=_ptr(t["fric"]["f"]),
-                  fix_pos=_ptr(t["fix_pos"]))
+                  fix_pos=_ptr(t["fix_pos"]),
+                  fix_delta=_ptr(t.get("fix_delta")))
 
 
 class Stepper:

@@ -184,6 +184,30 @@ def test_pcg_edge_cases(dev):
     assert g["status"] in (4, 5) and o["status"] in (4, 5)
 
 
+def test_plan_repattern_matches_fresh_plan(dev):
+    """One plan switched between patterns of the same n -- a chain (short
+    rows), medium fans (a wider ELL form), long fans (CSR form, a larger
+    pattern) and back -- solves each bit-identically to a plan built for it."""
+    order = [scenes.long_row_spd_system(hubs=0, seed=60),
+             scenes.long_row_spd_system(hubs=3, fan=10, seed=61),
+             scenes.long_row_spd_system(hubs=6, fan=60, seed=62),
+             scenes.long_row_spd_system(hubs=0, seed=63)]
+    t0 = to_dev(order[0], dev)
+    plan = BlockJacobiPCG(t0["row_offset"], t0["col_index"])
+    for k, s in enumerate(order):
+        t = to_dev(s, dev)
+        if k:
+            plan.repattern(t["row_offset"], t["col_index"])
+        x = torch.zeros_like(t["b"])
+        r = plan.solve(t["value"], t["cdiag"], t["b"], x)
+        fresh = solve_gpu(s, dev, hist=False)
+        print(f"pattern {k}: nnz {len(s['col_index'])} iter {r.iter}/{fresh['iter']}")
+        assert r.status == fresh["status"] == 0
+        assert r.iter == fresh["iter"]
+        assert np.array_equal(x.cpu().numpy().view(np.uint32), fresh["x"].view(np.uint32))
+    plan.close()
+
+
 def test_plan_rejects_malformed_pattern(dev):
     from ppf_amd._lib import PPFError
     ro = torch.tensor([0, 2, 3], dtype=torch.int32, device=dev)
