@@ -292,3 +292,26 @@ def test_line_search_pair_filters():
                                    static=sm)
     t_free, _ = oracle.line_search(m, x1g, ones(n1), ones(e1), ones(f1))
     assert t_all < t_none == t_free
+
+
+def test_line_search_collision_window_gate():
+    # Every vertex and edge query outside its collision window (flag bit2):
+    # no self pair is tested, so the falling sheet is not stopped; with only
+    # the upper sheet's queries gated, the lower sheet's queries (which find
+    # the upper sheet's faces and edges) still stop it.
+    x, f = scenes.stacked_sheets(8, layers=2, gap=1.5e-3, noise=1e-4, seed=4)
+    m = scenes.contact_mesh(x, f, ghat=1e-3, offset=5e-5)
+    x1 = x.copy()
+    x1[len(x) // 2:, 2] -= 2.5e-3
+    nv, ne, nf = m["nv"], m["ne"], m["nf"]
+    ones = lambda n: np.ones(n, np.float32)
+    t0, _ = oracle.line_search(m, x1, ones(nv), ones(ne), ones(nf))
+    g = dict(m, vflag=m["vflag"] | 4, eflag=m["eflag"] | 4)
+    t_all, ov = oracle.line_search(g, x1, ones(nv), ones(ne), ones(nf))
+    assert t0 < 1.0 and t_all == 1.0 and not ov
+    upper_v = np.arange(nv) >= nv // 2
+    upper_e = upper_v[m["edge"][:, 0]]
+    h = dict(m, vflag=m["vflag"] | np.where(upper_v, 4, 0).astype(np.uint32),
+             eflag=m["eflag"] | np.where(upper_e, 4, 0).astype(np.uint32))
+    t_half, _ = oracle.line_search(h, x1, ones(nv), ones(ne), ones(nf))
+    assert t0 <= t_half < 1.0

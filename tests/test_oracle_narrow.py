@@ -213,3 +213,27 @@ def test_stacked_sheets_have_every_kind_and_multiplicity():
     q = oracle.proximity(m)
     oracle.set_threads(1)
     assert np.array_equal(p, q)
+
+
+def test_collision_window_gates_only_the_query_side():
+    # vflag / eflag bit2 (outside the collision window): the element's own
+    # queries find nothing, while other queries still find it
+    # (contact.cu:1637, 1728: the query box is made inactive, the tree is not).
+    x, f = scenes.stacked_sheets(res=10, layers=2, gap=6e-4, seed=3)
+    m = scenes.contact_mesh(x, f, ghat=1e-3)
+    base = oracle.proximity(m)
+    assert len(base) > 0
+    rng = np.random.default_rng(0)
+    off_v = rng.random(m["nv"]) < 0.3
+    off_e = rng.random(m["ne"]) < 0.3
+    g = dict(m)
+    g["vflag"] = m["vflag"] | np.where(off_v, 4, 0).astype(np.uint32)
+    g["eflag"] = m["eflag"] | np.where(off_e, 4, 0).astype(np.uint32)
+    gated = oracle.proximity(g)
+    key = lambda p: (int(p["kind"]), int(p["a"]), int(p["b"]), int(p["count"]))
+    query_off = lambda p: off_e[p["a"]] if (p["kind"] & 255) == 3 else off_v[p["a"]]
+    expect = {key(p) for p in base if not query_off(p)}
+    assert {key(p) for p in gated} == expect
+    # inactive elements still appear as the found side of other queries
+    assert any(off_e[p["b"]] for p in gated if (p["kind"] & 255) == 3)
+    assert len(gated) < len(base)

@@ -49,3 +49,37 @@ def test_pins_stay_and_no_penetration():
         dist = np.linalg.norm(x, axis=1)
         assert dist.min() > 0.3 - 1e-4, (k, dist.min())
     assert shortened > 0        # contact shortened some steps
+
+
+def test_kinematic_pins_rewind_to_the_integrated_fraction():
+    # Pins slide along x; each step they are aimed at the step's end. When the
+    # line search shortens the step to toi_advanced < 1, the pins are walked
+    # back to position - (1 - toi_advanced) * step_delta before the final
+    # iteration (main.cu:513-545), so they end there exactly.
+    res = 10
+    pins = [0, res - 1]
+    sc = scenes.drape_scene(res=res, height=0.004, pins=pins)
+    sc["fix_delta"] = np.zeros_like(sc["fix_pos"])
+    st = oracle.OracleStepper(sc)
+    p = scenes.step_params()
+    v = np.float32(0.05 * p["dt"])
+    rewound = 0
+    for k in range(16):
+        start = sc["mesh"]["x"][pins].copy()
+        aim = start + np.array([v, 0, 0], np.float32)
+        sc["fix_pos"][pins] = aim
+        sc["fix_delta"][pins] = [v, 0, 0]
+        st.delta[:] = sc["fix_delta"].reshape(-1)
+        r = st.advance(p)
+        assert r["status"] == 0, (k, r)
+        p["prev_dt"] = r["dt"]
+        x = sc["mesh"]["x"][pins]
+        ta = np.float32(r["toi_advanced"])
+        if ta < 1.0:
+            back = np.float32(1.0) - ta
+            expect = aim - back * np.array([v, 0, 0], np.float32)
+            rewound += 1
+        else:
+            expect = aim
+        assert np.array_equal(x, expect.astype(np.float32)), (k, x, expect)
+    assert rewound > 0
