@@ -442,6 +442,16 @@ int ppf_stepper_advance(ppf_stepper *h, const ppf_scene *scene, const ppf_step_p
  * (update_constraint, main.cu:1988-2077). */
 int ppf_stepper_update_pins(ppf_stepper *h, const ppf_scene *scene, void *stream);
 
+/* The last Newton system the stepper solved: n rows, nnz stored upper blocks
+ * of the merged matrix after Dirichlet elimination (main.cu:954-1049), the
+ * per-row diagonal and the right-hand side -- the (Hessian, residual) pair a
+ * linear-system dump records (dump_linsys.hpp:6-16). Writes *n and *nnz and
+ * copies into whichever of d_row_offset (n + 1), d_col_index (nnz), d_value
+ * (9 nnz), d_cdiag (9 n), d_rhs (3 n) is non-null. */
+int ppf_stepper_system(ppf_stepper *h, uint32_t *n, uint32_t *nnz, uint32_t *d_row_offset,
+                       uint32_t *d_col_index, float *d_value, float *d_cdiag, float *d_rhs,
+                       void *stream);
+
 /* Records of the stepper's last post-step intersection check (the first
  * min(cap, 256) of them; contact::check_intersection, contact.cu:3056-3175).
  * Returns the number of intersections that check found. */

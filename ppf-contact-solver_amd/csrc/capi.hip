@@ -517,6 +517,37 @@ int ppf_stepper_update_pins(ppf_stepper *h, const ppf_scene *scene, void *stream
     return h->impl.refresh_pins(*scene, (hipStream_t)stream);
 }
 
+int ppf_stepper_system(ppf_stepper *h, uint32_t *n, uint32_t *nnz, uint32_t *d_row_offset,
+                       uint32_t *d_col_index, float *d_value, float *d_cdiag, float *d_rhs,
+                       void *stream) {
+    if (!h || !n || !nnz) {
+        set_error("ppf_stepper_system: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    const uint32_t *rp, *ci;
+    const float *val, *cdiag, *rhs;
+    h->impl.system(nnz, &rp, &ci, &val, &cdiag, &rhs);
+    *n = h->impl.nv();
+    if (*nnz == 0) {
+        set_error("ppf_stepper_system: no system solved yet");
+        return PPF_ERR_ARG;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t nr = *n, nz = *nnz;
+    auto cp = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (dst) {
+            PPF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+        }
+        return PPF_OK;
+    };
+    if (cp(d_row_offset, rp, (nr + 1) * 4) || cp(d_col_index, ci, nz * 4) ||
+        cp(d_value, val, 9 * nz * 4) || cp(d_cdiag, cdiag, 9 * nr * 4) ||
+        cp(d_rhs, rhs, 3 * nr * 4)) {
+        return PPF_ERR_HIP;
+    }
+    return PPF_OK;
+}
+
 uint32_t ppf_stepper_intersection_records(ppf_stepper *h, ppf_isect_record *out,
                                           uint32_t cap) {
     return h ? h->impl.records(out, cap) : 0;

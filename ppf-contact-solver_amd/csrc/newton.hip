@@ -464,6 +464,7 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
     if (rc != PPF_OK) {
         return rc;
     }
+    sys_nnz_ = nnz;
     k_merge_fill<<<cdiv(n, kNT), kNT, 0, s>>>(n, elastic_.row_offset(), elastic_.col_index(),
                                                eval_, contact_.row_offset(), contact_.cols(),
                                                contact_.values(), mrp_, mci_, mval_);

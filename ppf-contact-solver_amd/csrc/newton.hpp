@@ -25,6 +25,12 @@ class Stepper {
     // Records of the last post-step intersection check.
     uint32_t records(ppf_isect_record *out, uint32_t cap) const;
     const ContactAssembler &contact() const { return contact_; }
+    // The last Newton system solved (merged upper blocks after Dirichlet
+    // elimination, per-row diagonal, right-hand side).
+    void system(uint32_t *nnz, const uint32_t **rp, const uint32_t **ci, const float **val,
+                const float **cdiag, const float **rhs) const {
+        *nnz = sys_nnz_, *rp = mrp_, *ci = mci_, *val = mval_, *cdiag = cdiag_, *rhs = force_;
+    }
 
   private:
     int newton_iteration(const ppf_scene &sc, const ppf_step_params &p, float dt,
@@ -49,6 +55,7 @@ class Stepper {
     uint64_t mcap_ = 0;
     uint32_t npinned_ = 0;
     PcgPlanImpl *plan_ = nullptr;
+    uint32_t sys_nnz_ = 0;
     ppf_isect_record records_[kMaxRecords] = {};
     uint32_t records_count_ = 0;
     hipEvent_t ev_[8] = {};

@@ -117,6 +117,27 @@ class Stepper:
         except Exception:
             pass
 
+    def system(self):
+        """Copy of the last Newton system solved: dict of device tensors
+        row_offset (n+1, int32), col_index (nnz, int32), value (9 nnz),
+        cdiag (9 n), b (3 n)."""
+        n, nnz = u32(), u32()
+        check(self.lib.ppf_stepper_system(self._h, ctypes.byref(n), ctypes.byref(nnz), None,
+                                          None, None, None, None, _stream()),
+              "ppf_stepper_system")
+        dev = self.t["mesh"]["x"].device
+        n, nnz = n.value, nnz.value
+        out = {"row_offset": torch.empty(n + 1, dtype=torch.int32, device=dev),
+               "col_index": torch.empty(nnz, dtype=torch.int32, device=dev),
+               "value": torch.empty(9 * nnz, dtype=torch.float32, device=dev),
+               "cdiag": torch.empty(9 * n, dtype=torch.float32, device=dev),
+               "b": torch.empty(3 * n, dtype=torch.float32, device=dev)}
+        check(self.lib.ppf_stepper_system(self._h, ctypes.byref(u32()), ctypes.byref(u32()),
+                                          *(_ptr(out[k]) for k in ("row_offset", "col_index",
+                                                                   "value", "cdiag", "b")),
+                                          _stream()), "ppf_stepper_system")
+        return out
+
     def advance(self, params):
         p = params_struct(params)
         r = StepResultC()

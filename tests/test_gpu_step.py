@@ -87,3 +87,39 @@ def test_drape_c1_resolution(dev):
     # sphere so contact starts within the first steps.
     sc = scenes.drape_scene(res=64, height=0.002)
     _compare(dev, sc, 6, scenes.step_params())
+
+
+def test_last_newton_system_is_a_solvable_dirichlet_system(dev):
+    from ppf_amd.solver import BlockJacobiPCG
+    res = 12
+    pins = [0, res - 1]
+    sc = scenes.drape_scene(res=res, height=0.004, pins=pins)
+    t = scene_to_device(sc, dev)
+    gpu = Stepper(t)
+    p = scenes.step_params()
+    for _ in range(3):
+        r = gpu.advance(p)
+        assert r["status"] == 0
+        p["prev_dt"] = r["dt"]
+    s = gpu.system()
+    ro = s["row_offset"].cpu().numpy()
+    ci = s["col_index"].cpu().numpy()
+    n = res * res
+    assert len(ro) == n + 1 and ro[0] == 0 and np.all(np.diff(ro) >= 0) and ro[-1] == len(ci)
+    # upper storage: columns ascending and >= the row
+    rows = np.repeat(np.arange(n), np.diff(ro))
+    assert np.all(ci >= rows)
+    val = s["value"].cpu().numpy().reshape(-1, 9)
+    cd = s["cdiag"].cpu().numpy().reshape(-1, 9)
+    # pinned rows and columns are eliminated: identity diagonal, no coupling
+    for v in pins:
+        touch = (rows == v) | (ci == v)
+        assert np.all(val[touch & (ci != rows)] == 0.0)
+        diag = val[(rows == v) & (ci == v)].sum(0) + cd[v]
+        assert np.allclose(diag, np.eye(3).reshape(9))
+    plan = BlockJacobiPCG(s["row_offset"], s["col_index"])
+    x = torch.zeros_like(s["b"])
+    res_ = plan.solve(s["value"], s["cdiag"], s["b"], x)
+    assert res_.status == 0
+    plan.close()
+    gpu.close()
