@@ -1,18 +1,24 @@
 """Benchmark of the per-Newton-step hot path on one MI355X.
 
-Workload: the kernel-isolation PCG shape -- a block-sparse SPD system on a
-1024x1024 triangulated vertex grid (1,048,576 rows of 3x3 fp32 blocks, about
-7 blocks per row, 4 stored in the upper triangle), each edge block
-h^2 * Q^T diag(l) Q with l log-uniform in [1e2, 1e6], plus mass * I; random
-unit-normal right-hand side. One step = one block-Jacobi PCG solve to
-|r|_1/|r0|_1 < 1e-3 from x0 = 0, the linear solve of a Newton step. Inputs
-are resident in HBM before timing starts.
+Default workload (--workload drape-stack): the stacked-layer contact scene --
+8 cloth sheets of 256 x 256 vertices (1,040,400 triangles), stacked 2 mm
+apart with seeded 1e-4 m vertex noise, falling onto a cylinder of radius
+0.2 m spinning at 2 rad/s (its vertices pinned and driven kinematically),
+fp32, contact gap 1e-3 m, friction 0.5, dt = 1/60 s. W untimed time steps,
+then K timed time steps; every Newton step of those K steps is timed (elastic
+and contact assembly, block-Jacobi PCG to 1e-3 L1 relative residual, CCD
+line search). value = wall milliseconds per Newton step; the PCG iterations
+per solve are reported beside it. Inputs are resident in HBM before timing.
 
-Prints one JSON line. roofline: the SpMV kernel (the dominant kernel of an
-iteration), algorithmic bytes n*(40*s + 28) per launch with s stored upper
-blocks per row, over its mean launch duration from HIP events on the stream
-it runs on. cpu_baseline: the oracle (reference-algorithm CPU restatement,
-OpenMP) solving the same system on this host's cores.
+--workload pcg-grid: the kernel-isolation PCG system (1024 x 1024 grid, ~1M
+rows of 3x3 blocks); one step = one block-Jacobi PCG solve.
+
+roofline: the PCG SpMV (the kernel every Newton step runs most: once per PCG
+iteration), on the last Newton system of the timed steps; algorithmic bytes
+n * (40 s + 28) per launch (s stored upper blocks per row) over the mean
+launch duration from HIP events on the plan's stream. cpu_baseline: the
+oracle (the CPU restatement of the reference algorithm, OpenMP) on a bounded
+sample of the same workload, on this host's cores.
 """
 import argparse
 import json
@@ -29,93 +35,39 @@ METRIC = ("ms/Newton step + PCG iters to 1e-3 rel residual, learned precond "
 HBM_PEAK_GBS = 8000.0
 
 
-def traffic_from_profiles():
-    """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "pcg_spmv_grid1024.md")
+def traffic_from_profiles(name):
+    """HBM bytes per SpMV launch from a committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(sys_np, tol):
+def oracle_module():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = min(16, os.cpu_count() or 1)
     oracle.set_threads(threads)
-    t0 = time.perf_counter()
-    r = oracle.pcg(sys_np, tol=tol)
-    dt = time.perf_counter() - t0
-    return {"value": dt * 1e3, "unit": "ms/solve", "cores": threads, "kind": "port",
-            "sample": f"one full solve of the same 1,048,576-row system "
-                      f"({r['iter']} iterations, status {r['status']}); "
-                      "reference-algorithm CPU restatement, not the reference binary"}
+    return oracle, threads
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--res", type=int, default=1024)
-    ap.add_argument("--tol", type=float, default=1e-3)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calib", action="store_true",
-                    help="first run a 256 MiB device copy (PMC calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(run with --gpus 1)", file=sys.stderr)
-        sys.exit(2)
-
-    import numpy as np
-    import torch
-
-    from ppf_amd import scenes
-    from ppf_amd.solver import BlockJacobiPCG
-
-    dev = torch.device("cuda:0")
-    if args.copy_calib:
-        # A plain copy of known size: 256 MiB read + 256 MiB written. Its
-        # FETCH_SIZE / WRITE_SIZE calibrate the counters of the kernels below.
-        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
-    s = scenes.grid_spd_system(args.res)
-    n, nnz = s["n"], len(s["col_index"])
-    t = {k: torch.from_numpy(np.ascontiguousarray(s[k])).to(dev).reshape(-1)
-         for k in ("row_offset", "col_index", "value", "cdiag", "b")}
-    plan = BlockJacobiPCG(t["row_offset"], t["col_index"])
-    x = torch.zeros_like(t["b"])
-
-    def step():
-        x.zero_()
-        return plan.solve(t["value"], t["cdiag"], t["b"], x, tol=args.tol)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    iters = []
-    for _ in range(args.steps):
-        res = step()
-        iters.append(res.iter)
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    assert res.status == 0, f"PCG did not converge: status {res.status}"
-
-    spmv_ms = plan.time_spmv(t["value"], t["cdiag"], t["b"], reps=50)
+def spmv_roofline(plan, value, cdiag, b, n, nnz, kernel, traffic):
+    spmv_ms = plan.time_spmv(value, cdiag, b, reps=50)
     s_blocks = nnz / n
     alg_bytes = n * (40.0 * s_blocks + 28.0)
     achieved = alg_bytes / (spmv_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": kernel, "kernel_ms": round(spmv_ms, 5),
+            "alg_bytes_per_launch": alg_bytes, "rows": n, "stored_blocks": nnz}
 
+
+def head(args, value, ms, workload, extra):
     out = {
         "metric": METRIC,
-        "value": round(ms, 4),
-        "unit": "ms/step",
+        "value": round(value, 4),
+        "unit": "ms/Newton step" if args.workload == "drape-stack" else "ms/solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -125,25 +77,185 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "pcg_iters": int(np.median(iters)),
         "precond": "block-jacobi",
-        "config": {"workload": f"kernel-isolation PCG: {args.res}x{args.res} "
-                               f"triangulated grid, {n} rows of 3x3 fp32 blocks "
-                               f"({nnz} stored upper blocks), block-Jacobi to "
-                               f"{args.tol:g} L1 relative residual",
-                   "rows": n, "stored_blocks": nnz, "tol": args.tol},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "kernel": "pcg spmv (k_spmv_ell<true,3> on this grid)",
-                     "kernel_ms": round(spmv_ms, 5),
-                     "alg_bytes_per_launch": alg_bytes},
-        "cpu_baseline": None,
+        "config": workload,
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s, args.tol)
+    out.update(extra)
+    return out
+
+
+def run_drape(args, dev):
+    import numpy as np
+    import torch
+
+    from ppf_amd import scenes
+    from ppf_amd.solver import BlockJacobiPCG
+    from ppf_amd.stepper import Stepper, scene_to_device
+
+    sc = scenes.stacked_drape_scene(res=args.res, layers=args.layers)
+    ns = args.layers * args.res * args.res
+    t = scene_to_device(sc, dev)
+    t["fix_delta"] = torch.zeros_like(t["mesh"]["x"])
+    gpu = Stepper(t)
+    p = scenes.step_params()
+    clock = [0.0]
+
+    def one_step():
+        # the cylinder aimed at the step's end, with its displacement over the
+        # step (the kinematic pin's step_delta)
+        now = scenes.spin_fix_positions(sc, clock[0])[ns:].copy()
+        aim = scenes.spin_fix_positions(sc, clock[0] + p["dt"])[ns:].copy()
+        t["fix_pos"].view(-1, 3)[ns:] = torch.from_numpy(aim).to(dev)
+        t["fix_delta"].view(-1, 3)[ns:] = torch.from_numpy(aim - now).to(dev)
+        r = gpu.advance(p)
+        if r["status"] != 0:
+            raise RuntimeError(f"step failed: {r}")
+        p["prev_dt"] = r["dt"]
+        clock[0] += r["dt"]
+        return r
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    newton = pcg = pairs = 0
+    phase = {"elastic": 0.0, "contact": 0.0, "solve": 0.0, "line_search": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = one_step()
+        newton += r["newton_steps"]
+        pcg += r["pcg_iter_total"]
+        pairs = max(pairs, r["pairs_max"])
+        for k in phase:
+            phase[k] += r["ms_" + k]
+    torch.cuda.synchronize()
+    total_ms = (time.perf_counter() - t0) * 1e3
+    per_newton = total_ms / newton
+    sysd = gpu.system()
+    n, nnz = sysd["row_offset"].numel() - 1, sysd["col_index"].numel()
+    plan = BlockJacobiPCG(sysd["row_offset"], sysd["col_index"])
+    roof = spmv_roofline(plan, sysd["value"], sysd["cdiag"], sysd["b"], n, nnz,
+                         "pcg spmv on the last Newton system of the timed steps",
+                         traffic_from_profiles("pcg_spmv_drape_stack.md"))
     plan.close()
+    gpu.close()
+    workload = {"workload": f"stacked-layer drape: {args.layers} sheets of {args.res}x{args.res} "
+                            "vertices 2 mm apart (noise 1e-4 m) over a cylinder r = 0.2 m "
+                            "spinning at 2 rad/s, gap 1e-3 m, friction 0.5, dt 1/60 s; time "
+                            f"steps {args.warmup + 1}..{args.warmup + args.steps} of the drape",
+                "vertices": int(sc["mesh"]["nv"]), "faces": int(sc["mesh"]["nf"]),
+                "time_steps": args.steps, "newton_steps": newton,
+                "tol": 1e-3}
+    extra = {"pcg_iters": round(pcg / newton, 1),
+             "phase_ms_per_newton_step": {k: round(v / newton, 3) for k, v in phase.items()},
+             "max_contact_pairs": pairs,
+             "roofline": roof, "cpu_baseline": None}
+    out = head(args, per_newton, per_newton, workload, extra)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_drape(args)
+    return out
+
+
+def cpu_drape(args):
+    """The oracle's first time step of the same scene (a bounded sample)."""
+    from ppf_amd import scenes
+    oracle, threads = oracle_module()
+    sc = scenes.stacked_drape_scene(res=args.res, layers=args.layers)
+    st = oracle.OracleStepper(sc)
+    p = scenes.step_params()
+    scenes.spin_fix_positions(sc, p["dt"])
+    t0 = time.perf_counter()
+    r = st.advance(p)
+    dt = time.perf_counter() - t0
+    return {"value": round(dt * 1e3 / max(1, r["newton_steps"]), 1), "unit": "ms/Newton step",
+            "cores": threads, "kind": "port",
+            "sample": f"the first time step of the same scene ({r['newton_steps']} Newton "
+                      f"steps, {r['pcg_iter_total']} PCG iterations, {dt:.1f} s); "
+                      "reference-algorithm CPU restatement, not the reference binary"}
+
+
+def run_grid(args, dev):
+    import numpy as np
+    import torch
+
+    from ppf_amd import scenes
+    from ppf_amd.solver import BlockJacobiPCG
+
+    res = args.res if args.res != 256 else 1024
+    s = scenes.grid_spd_system(res)
+    n, nnz = s["n"], len(s["col_index"])
+    t = {k: torch.from_numpy(np.ascontiguousarray(s[k])).to(dev).reshape(-1)
+         for k in ("row_offset", "col_index", "value", "cdiag", "b")}
+    plan = BlockJacobiPCG(t["row_offset"], t["col_index"])
+    x = torch.zeros_like(t["b"])
+
+    def step():
+        x.zero_()
+        return plan.solve(t["value"], t["cdiag"], t["b"], x, tol=1e-3)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = []
+    for _ in range(args.steps):
+        r = step()
+        iters.append(r.iter)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    if r.status != 0:
+        raise RuntimeError(f"PCG did not converge: status {r.status}")
+    roof = spmv_roofline(plan, t["value"], t["cdiag"], t["b"], n, nnz,
+                         "pcg spmv (k_spmv_ell<true,3> on this grid)",
+                         traffic_from_profiles("pcg_spmv_grid1024.md"))
+    plan.close()
+    workload = {"workload": f"kernel-isolation PCG: {res}x{res} triangulated grid, {n} rows "
+                            f"of 3x3 fp32 blocks ({nnz} stored upper blocks), block-Jacobi "
+                            "to 1e-3 L1 relative residual",
+                "rows": n, "stored_blocks": nnz, "tol": 1e-3}
+    out = head(args, ms, ms, workload, {"pcg_iters": int(np.median(iters)), "roofline": roof,
+                                        "cpu_baseline": None})
+    if not args.no_cpu:
+        oracle, threads = oracle_module()
+        t0 = time.perf_counter()
+        ro = oracle.pcg(s, tol=1e-3)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(dt * 1e3, 1), "unit": "ms/solve",
+                               "cores": threads, "kind": "port",
+                               "sample": f"one full solve of the same system ({ro['iter']} "
+                                         "iterations); reference-algorithm CPU restatement, "
+                                         "not the reference binary"}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--workload", choices=("drape-stack", "pcg-grid"), default="drape-stack")
+    ap.add_argument("--res", type=int, default=256)
+    ap.add_argument("--layers", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run a 256 MiB device copy (PMC calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    dev = torch.device("cuda:0")
+    if args.copy_calib:
+        # A plain copy of known size: 256 MiB read + 256 MiB written. Its
+        # FETCH_SIZE / WRITE_SIZE calibrate the counters of the kernels after.
+        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    out = run_drape(args, dev) if args.workload == "drape-stack" else run_grid(args, dev)
     print(json.dumps(out))
 
 

@@ -84,9 +84,55 @@ def test_drape_pinned_corners(dev):
 
 def test_drape_c1_resolution(dev):
     # The plain-method drape at its full 64 x 64 resolution, close to the
-    # sphere so contact starts within the first steps.
+    # sphere so contact starts within the first steps. Its contact systems
+    # take 500+ PCG iterations; at the default tolerance (1e-3) the two fp32
+    # solvers, which sum in different orders, land on different points of the
+    # tolerance ball (measured: 1.7e-5 of the coordinate scale at step 4, 2%
+    # of the step). At cg_tol 1e-4 both solutions are close enough to the
+    # exact Newton step for the 1e-5 bar.
     sc = scenes.drape_scene(res=64, height=0.002)
-    _compare(dev, sc, 6, scenes.step_params())
+    _compare(dev, sc, 6, scenes.step_params(cg_tol=1e-4))
+
+
+def test_newton_solve_error_vs_fp64(dev):
+    """The fp32 PCG solution of a contact Newton system against an fp64
+    direct solve of the same system: stated, and bounded by what the L1
+    residual rule allows (error <= condition * tolerance)."""
+    import scipy.sparse.linalg as sla
+    from ppf_amd.solver import BlockJacobiPCG
+    sc = scenes.drape_scene(res=32, height=0.002)
+    t = scene_to_device(sc, dev)
+    gpu = Stepper(t)
+    p = scenes.step_params()
+    for _ in range(4):
+        r = gpu.advance(p)
+        assert r["status"] == 0
+        p["prev_dt"] = r["dt"]
+    s = gpu.system()
+    n = s["row_offset"].numel() - 1
+    host = {k: v.cpu().numpy() for k, v in s.items()}
+    M = scenes.sparse_from_upper(n, host["row_offset"], host["col_index"], host["value"],
+                                 host["cdiag"]).tocsc()
+    b = host["b"].astype(np.float64)
+    exact = sla.spsolve(M, b)
+    out = []
+    for tol in (1e-3, 1e-5):
+        plan = BlockJacobiPCG(s["row_offset"], s["col_index"])
+        x = torch.zeros_like(s["b"])
+        res = plan.solve(s["value"], s["cdiag"], s["b"], x, tol=tol)
+        plan.close()
+        xg = x.cpu().numpy().astype(np.float64)
+        err = np.linalg.norm(xg - exact) / np.linalg.norm(exact)
+        ob = oracle.pcg({**host, "n": n}, tol=tol)
+        oerr = np.linalg.norm(ob["x"] - exact) / np.linalg.norm(exact)
+        resid = np.abs(b - M @ xg).sum() / np.abs(b).sum()
+        print(f"tol {tol:g}: gpu iter {res.iter} rel err vs fp64 {err:.3e} (L1 resid "
+              f"{resid:.2e}); oracle iter {ob['iter']} rel err {oerr:.3e}")
+        assert res.status == 0 and resid < 1.5 * tol
+        out.append(err)
+    # a tighter tolerance moves the fp32 solution towards the fp64 one
+    assert out[1] < out[0] and out[1] < 1e-3
+    gpu.close()
 
 
 def test_last_newton_system_is_a_solvable_dirichlet_system(dev):
