@@ -27,6 +27,7 @@
 #include "eig3.hpp"
 #include "pcg.hpp"
 #include "scan.hpp"
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -334,6 +335,135 @@ __global__ void k_build_ell(uint32_t n, uint32_t W, const uint32_t *__restrict__
     }
 }
 
+// Split form of k_spmv_ell: the same entry list per row (the row's stored
+// slots, then its mirrored slots) and the same lane assignment, so the sums
+// are bit-identical, but the entries come from the CSR pattern and the
+// transpose table themselves instead of a padded copy: 4 B per stored entry
+// and 8 B per mirrored one instead of 8 B per padded entry.
+template <bool DOT, int M>
+__global__ __launch_bounds__(kNT) void k_spmv_split(const Args *__restrict__ A,
+                                                    const float *__restrict__ vin,
+                                                    float *__restrict__ yout) {
+    constexpr uint32_t kRows = kNT / 4;
+    __shared__ float lds[kNT / kWave];
+    if (DOT && as_global(A->s)->done) {
+        return;
+    }
+    const uint32_t n = A->n;
+    const uint32_t L = xcd_logical_block();
+    const uint32_t q = threadIdx.x & 3u, sub = threadIdx.x >> 2;
+    gptr<const uint32_t> rp = as_global(A->rp);
+    gptr<const uint32_t> col = as_global(A->col);
+    gptr<const uint32_t> tp = as_global(A->tp);
+    gptr<const uint32_t> tcol = as_global(A->tcol);
+    gptr<const uint32_t> tslot = as_global(A->tslot);
+    gptr<const float> val = as_global(A->val);
+    gptr<const float> cd = as_global(A->cdiag);
+    gptr<const float> v = as_global(vin);
+    gptr<float> y = as_global(yout);
+    const uint32_t lo = L * A->R1;
+    const uint32_t hi = min(n, lo + A->R1);
+    float dot = 0.0f, adot = 0.0f;
+    // row bounds of the next chunk are loaded while this chunk is in flight
+    uint32_t nb[4];
+    {
+        const uint32_t i0 = lo + sub < hi ? lo + sub : lo;
+        nb[0] = rp[i0], nb[1] = rp[i0 + 1], nb[2] = tp[i0], nb[3] = tp[i0 + 1];
+    }
+    for (uint32_t base = lo; base < hi; base += kRows) {
+        const uint32_t i = base + sub;
+        const bool valid = i < hi;
+        const uint32_t ii = valid ? i : lo;
+        const uint32_t r0 = nb[0], ns = nb[1] - nb[0], t0 = nb[2], nm = nb[3] - nb[2];
+        if (base + kRows < hi) {
+            const uint32_t in = base + kRows + sub;
+            const uint32_t iin = in < hi ? in : lo;
+            nb[0] = rp[iin], nb[1] = rp[iin + 1], nb[2] = tp[iin], nb[3] = tp[iin + 1];
+        }
+        uint32_t cj[M], sl[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const uint32_t k = q + 4 * m;
+            if (k < ns) {
+                cj[m] = col[r0 + k];
+                sl[m] = r0 + k;
+            } else if (k < ns + nm) {
+                cj[m] = tcol[t0 + k - ns];
+                sl[m] = tslot[t0 + k - ns] | kMirror;
+            } else {
+                cj[m] = 0;
+                sl[m] = kNone;
+            }
+        }
+        const f3 vi = load3(v + 3 * (size_t)ii);
+        f3 sum = mk3(0, 0, 0), as = mk3(0, 0, 0);
+        float blk[M][9];
+        f3 vj[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const uint32_t slot = sl[m] == kNone ? 0u : (sl[m] & ~kMirror);
+            gptr<const float> a = val + 9 * (size_t)slot;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                blk[m][c] = a[c];
+            }
+            vj[m] = load3(v + 3 * (size_t)cj[m]);
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            float *b = blk[m];
+            if (sl[m] != kNone && (sl[m] & kMirror)) {
+                float t;
+                t = b[1], b[1] = b[3], b[3] = t;
+                t = b[2], b[2] = b[6], b[6] = t;
+                t = b[5], b[5] = b[7], b[7] = t;
+            }
+            f3 t = mv(b, vj[m]);
+            if (sl[m] == kNone) {
+                t = mk3(0, 0, 0);
+            }
+            sum = sum + t;
+            as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+        }
+        if (q == 3) {
+            const f3 t = mv(cd + 9 * (size_t)ii, vi);
+            sum = sum + t;
+            as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+        }
+#pragma unroll
+        for (int off = 2; off > 0; off >>= 1) {
+            sum.x += __shfl_xor(sum.x, off, 4);
+            sum.y += __shfl_xor(sum.y, off, 4);
+            sum.z += __shfl_xor(sum.z, off, 4);
+            if (DOT) {
+                as.x += __shfl_xor(as.x, off, 4);
+                as.y += __shfl_xor(as.y, off, 4);
+                as.z += __shfl_xor(as.z, off, 4);
+            }
+        }
+        if (valid && q == 0) {
+            store3(y + 3 * (size_t)i, sum);
+            if (DOT) {
+                dot = dot + vi.x * sum.x;
+                dot = dot + vi.y * sum.y;
+                dot = dot + vi.z * sum.z;
+                adot = adot + fabsf(vi.x) * as.x;
+                adot = adot + fabsf(vi.y) * as.y;
+                adot = adot + fabsf(vi.z) * as.z;
+            }
+        }
+    }
+    if (DOT) {
+        dot = block_sum<kNT>(dot, lds);
+        adot = block_sum<kNT>(adot, lds);
+        if (threadIdx.x == 0) {
+            gptr<float> part = as_global(A->part);
+            part[L] = dot;
+            part[A->G1 + L] = adot;
+        }
+    }
+}
+
 // ELL form of k_spmv for rows of at most 4*M entries: 4 lanes per row, lane q
 // takes entries q, q+4, ... of the row's fixed-width table, so no row-offset
 // load sits in the dependent chain and all M block loads of a lane are issued
@@ -359,6 +489,16 @@ __global__ __launch_bounds__(kNT) void k_spmv_ell(const Args *__restrict__ A,
     const uint32_t lo = L * A->R1;
     const uint32_t hi = min(n, lo + A->R1);
     float dot = 0.0f, adot = 0.0f;
+    // The entries of the next chunk of rows are loaded while this chunk's
+    // blocks are in flight: one dependent round trip per chunk instead of two.
+    uint64_t wnext[M];
+    {
+        const uint32_t i0 = lo + sub < hi ? lo + sub : lo;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            wnext[m] = lo < hi ? ent[(size_t)i0 * W + q + 4 * m] : 0;
+        }
+    }
     for (uint32_t base = lo; base < hi; base += kRows) {
         const uint32_t i = base + sub;
         const bool valid = i < hi;
@@ -366,8 +506,15 @@ __global__ __launch_bounds__(kNT) void k_spmv_ell(const Args *__restrict__ A,
         uint2 e[M];
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-            const uint64_t w = ent[(size_t)ii * W + q + 4 * m];
-            e[m] = make_uint2((uint32_t)w, (uint32_t)(w >> 32));
+            e[m] = make_uint2((uint32_t)wnext[m], (uint32_t)(wnext[m] >> 32));
+        }
+        if (base + kRows < hi) {
+            const uint32_t in = base + kRows + sub;
+            const uint32_t iin = in < hi ? in : lo;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                wnext[m] = ent[(size_t)iin * W + q + 4 * m];
+            }
         }
         const f3 vi = load3(v + 3 * (size_t)ii);
         f3 sum = mk3(0, 0, 0), as = mk3(0, 0, 0);
@@ -666,6 +813,7 @@ struct PcgPlanImpl {
     void *ell = nullptr;       // ELL entry table (when rows are short)
     size_t ell_cap = 0;        // entries
     uint32_t graph_ell_m = 0xffffffffu; // ELL width the graph was captured for
+    bool split = false;        // short rows read the pattern itself (no ELL table)
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -688,6 +836,18 @@ template <bool DOT>
 hipError_t launch_spmv(const PcgPlanImpl &pl, const float *v, float *y,
                        hipStream_t st) {
     const Args &h = pl.h;
+    if (pl.split) {
+        switch (h.ell_m) {
+        case 1: k_spmv_split<DOT, 1><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+        case 2: k_spmv_split<DOT, 2><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+        case 3: k_spmv_split<DOT, 3><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+        case 4: k_spmv_split<DOT, 4><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+        case 6: k_spmv_split<DOT, 6><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+        case 8: k_spmv_split<DOT, 8><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+        default: k_spmv<DOT><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+        }
+        return hipGetLastError();
+    }
     switch (h.ell_m) {
     case 1: k_spmv_ell<DOT, 1><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
     case 2: k_spmv_ell<DOT, 2><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
@@ -787,7 +947,7 @@ int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_
         }
     }
     h.ent = nullptr;
-    if (h.ell_m) {
+    if (h.ell_m && !pl->split) {
         const uint32_t W = 4 * h.ell_m;
         if ((size_t)n * W > pl->ell_cap) {
             if (pl->ell) {
@@ -856,6 +1016,10 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
         return PPF_ERR_ARG;
     }
     pl->n = n;
+    {
+        const char *f = getenv("PPF_SPMV_FORM");
+        pl->split = f && !strcmp(f, "split");
+    }
     Args &h = pl->h;
     h.n = n;
     // Row ranges are multiples of 64 rows: one pass of either SpMV form.

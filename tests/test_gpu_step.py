@@ -85,13 +85,17 @@ def test_drape_pinned_corners(dev):
 def test_drape_c1_resolution(dev):
     # The plain-method drape at its full 64 x 64 resolution, close to the
     # sphere so contact starts within the first steps. Its contact systems
-    # take 500+ PCG iterations; at the default tolerance (1e-3) the two fp32
-    # solvers, which sum in different orders, land on different points of the
-    # tolerance ball (measured: 1.7e-5 of the coordinate scale at step 4, 2%
-    # of the step). At cg_tol 1e-4 both solutions are close enough to the
-    # exact Newton step for the 1e-5 bar.
+    # take 500-900 PCG iterations. At the default tolerance (1e-3), and still
+    # at 1e-4, the two fp32 solvers, which sum in different orders, land on
+    # different points of the tolerance ball (measured: 1.7e-5 and 1.8e-5 of
+    # the coordinate scale, about 2.5% of the step's displacement, on steps
+    # needing ~900 iterations), which is the solve's own uncertainty: against
+    # an fp64 re-solve both fp32 solutions carry the same error (see
+    # test_newton_solve_error_vs_fp64). At cg_tol 1e-5 the linear solve is no
+    # longer the limit and the whole step (assembly, contact, CCD, toi
+    # bookkeeping) must meet the 1e-5 bar.
     sc = scenes.drape_scene(res=64, height=0.002)
-    _compare(dev, sc, 6, scenes.step_params(cg_tol=1e-4))
+    _compare(dev, sc, 6, scenes.step_params(cg_tol=1e-5))
 
 
 def test_newton_solve_error_vs_fp64(dev):
