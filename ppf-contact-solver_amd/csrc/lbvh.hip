@@ -188,13 +188,11 @@ __global__ __launch_bounds__(kNT) void k_karras(uint32_t n, const uint32_t *__re
     parent[rc] = n + i;
 }
 
-// Swept leaf box (lbvh.cu:623-731), then the union walk to the root.
-__global__ __launch_bounds__(kNT) void k_refit(PrimSet p, const float *__restrict__ x0,
-                                               const float *__restrict__ x1, float t,
-                                               const uint32_t *__restrict__ node,
-                                               const uint32_t *__restrict__ parent,
-                                               uint32_t *__restrict__ visit,
-                                               uint32_t root, Aabb *__restrict__ box) {
+// Swept leaf box (lbvh.cu:623-731).
+__global__ __launch_bounds__(kNT) void k_leaf_boxes(PrimSet p, const float *__restrict__ x0,
+                                                    const float *__restrict__ x1, float t,
+                                                    const uint32_t *__restrict__ node,
+                                                    Aabb *__restrict__ box) {
     const uint32_t i = blockIdx.x * kNT + threadIdx.x;
     if (i >= p.n) {
         return;
@@ -229,27 +227,111 @@ __global__ __launch_bounds__(kNT) void k_refit(PrimSet p, const float *__restric
     bx.active = 1.0f;
     bx.pad = 0.0f;
     box[i] = bx;
-    uint32_t cur = i;
-    while (cur != root) {
-        const uint32_t par = parent[cur];
-        __threadfence();
-        if (atomicAdd(visit + (par - p.n), 1u) == 0) {
-            return; // the sibling's walk finishes this node
-        }
-        __threadfence();
-        const Aabb a = box[node[2 * (size_t)par] - 1];
-        const Aabb b = box[node[2 * (size_t)par + 1] - 1];
-        Aabb u;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            u.mn[d] = fminf(a.mn[d], b.mn[d]);
-            u.mx[d] = fmaxf(a.mx[d], b.mx[d]);
-        }
-        u.active = (a.active != 0.0f && b.active != 0.0f) ? 1.0f : 0.0f;
-        u.pad = 0.0f;
-        box[par] = u;
-        cur = par;
+}
+
+// Depth of every internal node (steps to the root along the parent links),
+// the deepest level and a per-depth count.
+__global__ __launch_bounds__(kNT) void k_depth(uint32_t n, uint32_t root,
+                                               const uint32_t *__restrict__ parent,
+                                               uint32_t *__restrict__ depth,
+                                               uint32_t *__restrict__ maxd) {
+    const uint32_t k = blockIdx.x * kNT + threadIdx.x;
+    if (k >= n - 1) {
+        return;
     }
+    uint32_t cur = n + k, d = 0;
+    while (cur != root && d < n) {
+        cur = parent[cur];
+        ++d;
+    }
+    depth[k] = d;
+    atomicMax(maxd, d);
+}
+
+// Per-depth counts. Most nodes of a workgroup share a few depths, so they
+// are counted in LDS first (depths below kLdsDepth) and each workgroup adds
+// one total per depth: a global atomic per node would serialize on the few
+// hot counters.
+constexpr uint32_t kLdsDepth = 256;
+
+__global__ __launch_bounds__(kNT) void k_depth_count(uint32_t m, const uint32_t *__restrict__ depth,
+                                                     uint32_t *__restrict__ count) {
+    __shared__ uint32_t h[kLdsDepth];
+    for (uint32_t d = threadIdx.x; d < kLdsDepth; d += kNT) {
+        h[d] = 0;
+    }
+    __syncthreads();
+    const uint32_t k = blockIdx.x * kNT + threadIdx.x;
+    if (k < m) {
+        const uint32_t d = depth[k];
+        if (d < kLdsDepth) {
+            atomicAdd(h + d, 1u);
+        } else {
+            atomicAdd(count + d, 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < kLdsDepth; d += kNT) {
+        if (h[d]) {
+            atomicAdd(count + d, h[d]);
+        }
+    }
+}
+
+// Internal nodes grouped by depth (order within a depth is irrelevant: a
+// node's box depends only on its children's). Each workgroup reserves one
+// range per depth and ranks its nodes inside it in LDS.
+__global__ __launch_bounds__(kNT) void k_depth_scatter(uint32_t m, uint32_t n,
+                                                       const uint32_t *__restrict__ depth,
+                                                       uint32_t *__restrict__ fill,
+                                                       uint32_t *__restrict__ level) {
+    __shared__ uint32_t h[kLdsDepth], base[kLdsDepth];
+    for (uint32_t d = threadIdx.x; d < kLdsDepth; d += kNT) {
+        h[d] = 0;
+    }
+    __syncthreads();
+    const uint32_t k = blockIdx.x * kNT + threadIdx.x;
+    uint32_t d = 0, rank = 0;
+    if (k < m) {
+        d = depth[k];
+        if (d < kLdsDepth) {
+            rank = atomicAdd(h + d, 1u);
+        } else {
+            level[atomicAdd(fill + d, 1u)] = n + k;
+        }
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < kLdsDepth; e += kNT) {
+        base[e] = h[e] ? atomicAdd(fill + e, h[e]) : 0u;
+    }
+    __syncthreads();
+    if (k < m && d < kLdsDepth) {
+        level[base[d] + rank] = n + k;
+    }
+}
+
+// Boxes of the internal nodes of one depth from their children's (the
+// deeper levels are done by the launches before this one).
+__global__ __launch_bounds__(kNT) void k_refit_level(const uint32_t *__restrict__ level,
+                                                     uint32_t count,
+                                                     const uint32_t *__restrict__ node,
+                                                     Aabb *__restrict__ box) {
+    const uint32_t k = blockIdx.x * kNT + threadIdx.x;
+    if (k >= count) {
+        return;
+    }
+    const uint32_t par = level[k];
+    const Aabb a = box[node[2 * (size_t)par] - 1];
+    const Aabb b = box[node[2 * (size_t)par + 1] - 1];
+    Aabb u;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        u.mn[d] = fminf(a.mn[d], b.mn[d]);
+        u.mx[d] = fmaxf(a.mx[d], b.mx[d]);
+    }
+    u.active = (a.active != 0.0f && b.active != 0.0f) ? 1.0f : 0.0f;
+    u.pad = 0.0f;
+    box[par] = u;
 }
 
 uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
@@ -259,14 +341,15 @@ uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 Lbvh::~Lbvh() { release(); }
 
 void Lbvh::release() {
-    for (void *p : {(void *)node_, (void *)parent_, (void *)visit_, (void *)code_,
+    for (void *p : {(void *)node_, (void *)parent_, (void *)depth_, (void *)level_,
+                    (void *)code_,
                     (void *)idx_, (void *)scratch_, (void *)cent_, (void *)part_,
                     (void *)box_}) {
         if (p) {
             (void)hipFree(p);
         }
     }
-    node_ = parent_ = visit_ = code_ = idx_ = scratch_ = nullptr;
+    node_ = parent_ = depth_ = level_ = code_ = idx_ = scratch_ = nullptr;
     cent_ = part_ = nullptr;
     box_ = nullptr;
     cap_ = 0;
@@ -281,7 +364,8 @@ int Lbvh::reserve(uint32_t n) {
     const size_t nb = cdiv(n, kNT);
     PPF_HIP(hipMalloc(&node_, 2 * nodes * sizeof(uint32_t)));
     PPF_HIP(hipMalloc(&parent_, nodes * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&visit_, nodes * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&depth_, n * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&level_, n * sizeof(uint32_t)));
     PPF_HIP(hipMalloc(&code_, n * sizeof(uint32_t)));
     PPF_HIP(hipMalloc(&idx_, n * sizeof(uint32_t)));
     PPF_HIP(hipMalloc(&scratch_, radix_sort_scratch_words(n) * sizeof(uint32_t)));
@@ -314,7 +398,53 @@ int Lbvh::build(const PrimSet &p, const float *x0, const float *x1, float t,
     PPF_HIP(radix_sort_pairs(code_, idx_, n_, 30, scratch_, s));
     k_karras<<<nb, kNT, 0, s>>>(n_, code_, idx_, node_, parent_);
     PPF_HIP(hipGetLastError());
+    rc = levels(s);
+    if (rc != PPF_OK) {
+        return rc;
+    }
     return refit(p, x0, x1, t, s);
+}
+
+// Internal nodes grouped by depth (lbvh.cu:479-616 computes the same levels):
+// the order refit visits them in. The sort keys and indices are free by now
+// and hold the per-depth counts and fill cursors.
+int Lbvh::levels(hipStream_t s) {
+    loff_.assign(1, 0);
+    if (n_ < 2) {
+        return PPF_OK;
+    }
+    const uint32_t m = n_ - 1;
+    uint32_t *maxd = scratch_;
+    PPF_HIP(hipMemsetAsync(maxd, 0, sizeof(uint32_t), s));
+    k_depth<<<cdiv(m, kNT), kNT, 0, s>>>(n_, root_, parent_, depth_, maxd);
+    PPF_HIP(hipGetLastError());
+    uint32_t hmax = 0;
+    PPF_HIP(hipMemcpyAsync(&hmax, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipStreamSynchronize(s));
+    if (hmax >= n_) {
+        set_error("Lbvh: parent links do not reach the root");
+        return PPF_ERR_SOLVER;
+    }
+    const uint32_t levels = hmax + 1;
+    uint32_t *count = code_, *fill = idx_;
+    PPF_HIP(hipMemsetAsync(count, 0, levels * sizeof(uint32_t), s));
+    k_depth_count<<<cdiv(m, kNT), kNT, 0, s>>>(m, depth_, count);
+    PPF_HIP(hipGetLastError());
+    std::vector<uint32_t> h(levels);
+    PPF_HIP(hipMemcpyAsync(h.data(), count, levels * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           s));
+    PPF_HIP(hipStreamSynchronize(s));
+    loff_.assign(levels + 1, 0);
+    for (uint32_t d = 0; d < levels; ++d) {
+        loff_[d + 1] = loff_[d] + h[d];
+    }
+    PPF_HIP(hipMemcpyAsync(fill, loff_.data(), levels * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, s));
+    k_depth_scatter<<<cdiv(m, kNT), kNT, 0, s>>>(m, n_, depth_, fill, level_);
+    PPF_HIP(hipGetLastError());
+    // loff_ must stay intact until the copy above has read it
+    PPF_HIP(hipStreamSynchronize(s));
+    return PPF_OK;
 }
 
 int Lbvh::refit(const PrimSet &p, const float *x0, const float *x1, float t,
@@ -327,12 +457,17 @@ int Lbvh::refit(const PrimSet &p, const float *x0, const float *x1, float t,
     if (n_ == 0) {
         return PPF_OK;
     }
-    if (n_ > 1) {
-        PPF_HIP(hipMemsetAsync(visit_, 0, (n_ - 1) * sizeof(uint32_t), s));
-    }
-    k_refit<<<cdiv(n_, kNT), kNT, 0, s>>>(p, x0, x1, t, node_, parent_, visit_,
-                                          root_, box_);
+    k_leaf_boxes<<<cdiv(n_, kNT), kNT, 0, s>>>(p, x0, x1, t, node_, box_);
     PPF_HIP(hipGetLastError());
+    // Inner boxes level by level from the deepest: a kernel boundary orders
+    // each level after its children, with no fences or arrival counters.
+    for (size_t d = loff_.size() - 1; d-- > 0;) {
+        const uint32_t cnt = loff_[d + 1] - loff_[d];
+        if (cnt) {
+            k_refit_level<<<cdiv(cnt, kNT), kNT, 0, s>>>(level_ + loff_[d], cnt, node_, box_);
+            PPF_HIP(hipGetLastError());
+        }
+    }
     return PPF_OK;
 }
 

@@ -5,8 +5,10 @@
 // (code, index), and the Karras radix tree with the index tie-break on equal
 // codes. Node i < n is leaf i (prim + 1, 0); node n + i is internal node i
 // (left + 1, right + 1); the root is node n (leaf 0 when n == 1). Boxes are
-// refit bottom-up with per-node arrival counters instead of per-level launches.
+// refit bottom-up one depth level per launch (levels computed once per build).
 #pragma once
+#include <vector>
+
 #include "common.hpp"
 
 namespace ppf {
@@ -54,11 +56,13 @@ class Lbvh {
 
   private:
     uint32_t cap_ = 0, n_ = 0, root_ = 0;
-    uint32_t *node_ = nullptr, *parent_ = nullptr, *visit_ = nullptr;
+    uint32_t *node_ = nullptr, *parent_ = nullptr, *depth_ = nullptr, *level_ = nullptr;
+    std::vector<uint32_t> loff_{0}; // level offsets into level_ (host copy)
     uint32_t *code_ = nullptr, *idx_ = nullptr, *scratch_ = nullptr;
     float *cent_ = nullptr, *part_ = nullptr;
     Aabb *box_ = nullptr;
     void release();
+    int levels(hipStream_t s);
 };
 
 } // namespace ppf

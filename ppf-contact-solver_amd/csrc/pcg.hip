@@ -336,10 +336,12 @@ __global__ void k_build_ell(uint32_t n, uint32_t W, const uint32_t *__restrict__
 }
 
 // Split form of k_spmv_ell: the same entry list per row (the row's stored
-// slots, then its mirrored slots) and the same lane assignment, so the sums
-// are bit-identical, but the entries come from the CSR pattern and the
-// transpose table themselves instead of a padded copy: 4 B per stored entry
-// and 8 B per mirrored one instead of 8 B per padded entry.
+// slots, then its mirrored slots) and the same lane assignment (lane q sums
+// entries q, q+4, q+8, ... in that order), so the sums are bit-identical to
+// the ELL form, but the entries come from the CSR pattern and the transpose
+// table themselves instead of a padded copy (4 B per stored entry, 8 B per
+// mirrored one) and rows of any length are taken in passes of 4*M entries,
+// all loads of a pass issued before any is used.
 template <bool DOT, int M>
 __global__ __launch_bounds__(kNT) void k_spmv_split(const Args *__restrict__ A,
                                                     const float *__restrict__ vin,
@@ -380,50 +382,52 @@ __global__ __launch_bounds__(kNT) void k_spmv_split(const Args *__restrict__ A,
             const uint32_t iin = in < hi ? in : lo;
             nb[0] = rp[iin], nb[1] = rp[iin + 1], nb[2] = tp[iin], nb[3] = tp[iin + 1];
         }
-        uint32_t cj[M], sl[M];
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const uint32_t k = q + 4 * m;
-            if (k < ns) {
-                cj[m] = col[r0 + k];
-                sl[m] = r0 + k;
-            } else if (k < ns + nm) {
-                cj[m] = tcol[t0 + k - ns];
-                sl[m] = tslot[t0 + k - ns] | kMirror;
-            } else {
-                cj[m] = 0;
-                sl[m] = kNone;
-            }
-        }
         const f3 vi = load3(v + 3 * (size_t)ii);
         f3 sum = mk3(0, 0, 0), as = mk3(0, 0, 0);
-        float blk[M][9];
-        f3 vj[M];
+        for (uint32_t k0 = 0; k0 < ns + nm; k0 += 4 * M) {
+            uint32_t cj[M], sl[M];
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const uint32_t slot = sl[m] == kNone ? 0u : (sl[m] & ~kMirror);
-            gptr<const float> a = val + 9 * (size_t)slot;
+            for (int m = 0; m < M; ++m) {
+                const uint32_t k = k0 + q + 4 * m;
+                if (k < ns) {
+                    cj[m] = col[r0 + k];
+                    sl[m] = r0 + k;
+                } else if (k < ns + nm) {
+                    cj[m] = tcol[t0 + k - ns];
+                    sl[m] = tslot[t0 + k - ns] | kMirror;
+                } else {
+                    cj[m] = 0;
+                    sl[m] = kNone;
+                }
+            }
+            float blk[M][9];
+            f3 vj[M];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                blk[m][c] = a[c];
-            }
-            vj[m] = load3(v + 3 * (size_t)cj[m]);
-        }
+            for (int m = 0; m < M; ++m) {
+                const uint32_t slot = sl[m] == kNone ? 0u : (sl[m] & ~kMirror);
+                gptr<const float> a = val + 9 * (size_t)slot;
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            float *b = blk[m];
-            if (sl[m] != kNone && (sl[m] & kMirror)) {
-                float t;
-                t = b[1], b[1] = b[3], b[3] = t;
-                t = b[2], b[2] = b[6], b[6] = t;
-                t = b[5], b[5] = b[7], b[7] = t;
+                for (int c = 0; c < 9; ++c) {
+                    blk[m][c] = a[c];
+                }
+                vj[m] = load3(v + 3 * (size_t)cj[m]);
             }
-            f3 t = mv(b, vj[m]);
-            if (sl[m] == kNone) {
-                t = mk3(0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                float *b = blk[m];
+                if (sl[m] != kNone && (sl[m] & kMirror)) {
+                    float t;
+                    t = b[1], b[1] = b[3], b[3] = t;
+                    t = b[2], b[2] = b[6], b[6] = t;
+                    t = b[5], b[5] = b[7], b[7] = t;
+                }
+                f3 t = mv(b, vj[m]);
+                if (sl[m] == kNone) {
+                    t = mk3(0, 0, 0);
+                }
+                sum = sum + t;
+                as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
             }
-            sum = sum + t;
-            as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
         }
         if (q == 3) {
             const f3 t = mv(cd + 9 * (size_t)ii, vi);
@@ -812,8 +816,9 @@ struct PcgPlanImpl {
     size_t pat_cap = 0;
     void *ell = nullptr;       // ELL entry table (when rows are short)
     size_t ell_cap = 0;        // entries
-    uint32_t graph_ell_m = 0xffffffffu; // ELL width the graph was captured for
+    uint32_t graph_key = 0xffffffffu; // SpMV form the graph was captured for
     bool split = false;        // short rows read the pattern itself (no ELL table)
+    uint32_t split_m = 4;      // entries per lane per pass of the split form
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -831,20 +836,19 @@ size_t carve_bytes(size_t count, size_t elem) {
     return (count * elem + 255) & ~size_t(255);
 }
 
+// The SpMV kernel a plan launches (its graph is captured for one).
+uint32_t spmv_key(const PcgPlanImpl &pl) { return pl.split ? 100u + pl.split_m : pl.h.ell_m; }
+
 // y = M v with the plan's SpMV form (ELL width 4*m, or the CSR walk).
 template <bool DOT>
 hipError_t launch_spmv(const PcgPlanImpl &pl, const float *v, float *y,
                        hipStream_t st) {
     const Args &h = pl.h;
     if (pl.split) {
-        switch (h.ell_m) {
-        case 1: k_spmv_split<DOT, 1><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+        switch (pl.split_m) {
         case 2: k_spmv_split<DOT, 2><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
         case 3: k_spmv_split<DOT, 3><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
-        case 4: k_spmv_split<DOT, 4><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
-        case 6: k_spmv_split<DOT, 6><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
-        case 8: k_spmv_split<DOT, 8><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
-        default: k_spmv<DOT><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+        default: k_spmv_split<DOT, 4><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
         }
         return hipGetLastError();
     }
@@ -946,6 +950,13 @@ int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_
             break;
         }
     }
+    // Split form: a pass covers the average row (stored + mirrored entries,
+    // 2 nnz - n with every diagonal stored), longer rows take more passes.
+    {
+        const double avg = n ? (2.0 * nnz - n) / n : 0.0;
+        const uint32_t m = (uint32_t)((avg + 1.0 + 3.999) / 4.0);
+        pl->split_m = m < 2 ? 2 : (m > 4 ? 4 : m);
+    }
     h.ent = nullptr;
     if (h.ell_m && !pl->split) {
         const uint32_t W = 4 * h.ell_m;
@@ -999,7 +1010,7 @@ int capture_graph(PcgPlanImpl *pl) {
         set_error("pcg: hipGraphInstantiate: %s", hipGetErrorString(ce));
         return PPF_ERR_HIP;
     }
-    pl->graph_ell_m = pl->h.ell_m;
+    pl->graph_key = spmv_key(*pl);
     return PPF_OK;
 }
 
@@ -1083,7 +1094,7 @@ int pcg_plan_repattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
         return PPF_ERR_ARG;
     }
     int rc = set_pattern(pl, nnz, rp, col, st);
-    if (rc == PPF_OK && pl->h.ell_m != pl->graph_ell_m) {
+    if (rc == PPF_OK && spmv_key(*pl) != pl->graph_key) {
         rc = capture_graph(pl);
     }
     return rc;
