@@ -81,6 +81,54 @@ int ppf_pcg_solve(ppf_pcg_plan *plan, const float *d_value,
                   const float *d_cdiag, const float *d_b, float *d_x, float tol,
                   uint32_t max_iter, ppf_pcg_result *out, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Learned preconditioner. No counterpart in the reference, whose block
+ * Jacobi (solver.cu:357-400) is the starting point (zero output heads) and
+ * the fallback. A message-passing network over the system's 3x3-block graph
+ * (node features from the diagonal and right-hand side, edge features from
+ * the normalised blocks L_i^-1 A_ij L_j^-T, A_ii = L_i L_i^T; 3 layers of
+ * width 64 on bf16 MFMA with fp32 accumulation) predicts a block FSAI factor
+ * Ghat on A's pattern; M^-1 = G^T G with G = Ghat L^-1 is symmetric positive
+ * definite for any weights. Trained on the log Kaporin number of
+ * Ghat Ahat Ghat^T with Rademacher probes.
+ * ---------------------------------------------------------------------- */
+
+typedef struct ppf_lpc ppf_lpc;
+
+/* Number of fp32 weights (row-major [in][out]: W0 16x64, c0; per layer
+ * We 80x64, ce, Wn 128x64, cn; Wo 64x6, co; Wg 144x9, cg). */
+uint32_t ppf_lpc_weight_count(void);
+int ppf_lpc_create(ppf_lpc **out);
+void ppf_lpc_destroy(ppf_lpc *h);
+int ppf_lpc_set_weights(ppf_lpc *h, const float *host_weights, uint32_t count, void *stream);
+int ppf_lpc_get_weights(ppf_lpc *h, float *host_weights, uint32_t count, void *stream);
+
+/* Graph, features, network and factor for the system (plan's pattern,
+ * d_value, d_cdiag, right-hand side d_b). */
+int ppf_lpc_factor(ppf_lpc *h, ppf_pcg_plan *plan, const float *d_value,
+                   const float *d_cdiag, const float *d_b, void *stream);
+
+/* Device copies of the last factor's head outputs t (n x 6), u (nnz x 9, by
+ * slot; zero on diagonal slots) and of G: diagonal blocks (n x 9) and lower
+ * blocks by slot (nnz x 9, G_ji for the slot (i, j)). Any may be null. */
+int ppf_lpc_outputs(ppf_lpc *h, float *d_t, float *d_u, float *d_gd, float *d_gl,
+                    void *stream);
+
+/* Loss and weight gradient (host, ppf_lpc_weight_count floats) for the
+ * probes d_probes (nprobe x n x 3), after ppf_lpc_factor on the same system. */
+int ppf_lpc_gradient(ppf_lpc *h, ppf_pcg_plan *plan, const float *d_value,
+                     const float *d_cdiag, const float *d_probes, uint32_t nprobe,
+                     float *loss, float *host_grad, void *stream);
+
+/* One Adam step (beta 0.9 / 0.999) on the last gradient. */
+int ppf_lpc_adam(ppf_lpc *h, float lr, uint32_t step, void *stream);
+
+/* PCG with M^-1 = G^T G from the last ppf_lpc_factor on this system; same
+ * stopping rule and result as ppf_pcg_solve. */
+int ppf_pcg_solve_lpc(ppf_pcg_plan *plan, ppf_lpc *h, const float *d_value,
+                      const float *d_cdiag, const float *d_b, float *d_x, float tol,
+                      uint32_t max_iter, ppf_pcg_result *out, void *stream);
+
 /* Same solve, also writing |r|_1 after every iteration into d_err_hist
  * (max_iter floats) for residual-curve comparisons. */
 int ppf_pcg_solve_hist(ppf_pcg_plan *plan, const float *d_value,

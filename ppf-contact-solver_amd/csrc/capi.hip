@@ -6,6 +6,7 @@
 #include "elastic.hpp"
 #include "isect.hpp"
 #include "assemble.hpp"
+#include "learned.hpp"
 #include "linesearch.hpp"
 #include "narrow.hpp"
 #include "newton.hpp"
@@ -98,6 +99,10 @@ struct ppf_stepper {
     Stepper impl;
 };
 
+struct ppf_lpc {
+    Lpc impl;
+};
+
 extern "C" {
 
 const char *ppf_last_error(void) { return g_err; }
@@ -151,6 +156,101 @@ int ppf_pcg_plan_repattern(ppf_pcg_plan *plan, uint32_t nnz, const uint32_t *d_r
     }
     return pcg_plan_repattern(plan->impl, nnz, d_row_offset, d_col_index,
                               (hipStream_t)stream);
+}
+
+uint32_t ppf_lpc_weight_count(void) { return LpcLayout::count; }
+
+int ppf_lpc_create(ppf_lpc **out) {
+    if (!out) {
+        set_error("ppf_lpc_create: out is null");
+        return PPF_ERR_ARG;
+    }
+    *out = new ppf_lpc();
+    return PPF_OK;
+}
+
+void ppf_lpc_destroy(ppf_lpc *h) { delete h; }
+
+int ppf_lpc_set_weights(ppf_lpc *h, const float *host_weights, uint32_t count, void *stream) {
+    if (!h || !host_weights) {
+        set_error("ppf_lpc_set_weights: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.set_weights(host_weights, count, (hipStream_t)stream);
+}
+
+int ppf_lpc_get_weights(ppf_lpc *h, float *host_weights, uint32_t count, void *stream) {
+    if (!h || !host_weights) {
+        set_error("ppf_lpc_get_weights: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.get_weights(host_weights, count, (hipStream_t)stream);
+}
+
+int ppf_lpc_factor(ppf_lpc *h, ppf_pcg_plan *plan, const float *d_value,
+                   const float *d_cdiag, const float *d_b, void *stream) {
+    if (!h || !plan || !d_value || !d_cdiag || !d_b) {
+        set_error("ppf_lpc_factor: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    int rc = h->impl.prepare(pcg_pattern(plan->impl), d_value, d_cdiag, d_b, s);
+    return rc == PPF_OK ? h->impl.forward(s) : rc;
+}
+
+int ppf_lpc_outputs(ppf_lpc *h, float *d_t, float *d_u, float *d_gd, float *d_gl,
+                    void *stream) {
+    if (!h || !h->impl.Gd()) {
+        set_error("ppf_lpc_outputs: no factor computed");
+        return PPF_ERR_ARG;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t n = h->impl.n(), z = h->impl.nnz();
+    if (d_t) PPF_HIP(hipMemcpyAsync(d_t, h->impl.t(), 6 * n * 4, hipMemcpyDeviceToDevice, s));
+    if (d_u) PPF_HIP(hipMemcpyAsync(d_u, h->impl.u(), 9 * z * 4, hipMemcpyDeviceToDevice, s));
+    if (d_gd) PPF_HIP(hipMemcpyAsync(d_gd, h->impl.Gd(), 9 * n * 4, hipMemcpyDeviceToDevice, s));
+    if (d_gl) PPF_HIP(hipMemcpyAsync(d_gl, h->impl.Gl(), 9 * z * 4, hipMemcpyDeviceToDevice, s));
+    return PPF_OK;
+}
+
+int ppf_lpc_gradient(ppf_lpc *h, ppf_pcg_plan *plan, const float *d_value,
+                     const float *d_cdiag, const float *d_probes, uint32_t nprobe,
+                     float *loss, float *host_grad, void *stream) {
+    if (!h || !plan || !d_value || !d_cdiag || !d_probes || !loss || !h->impl.Gd()) {
+        set_error("ppf_lpc_gradient: invalid argument (or no factor computed)");
+        return PPF_ERR_ARG;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    int rc = h->impl.gradient(plan->impl, d_value, d_cdiag, d_probes, nprobe, loss, s);
+    if (rc == PPF_OK && host_grad) {
+        PPF_HIP(hipMemcpyAsync(host_grad, h->impl.grad(), LpcLayout::count * sizeof(float),
+                               hipMemcpyDeviceToHost, s));
+        PPF_HIP(hipStreamSynchronize(s));
+    }
+    return rc;
+}
+
+int ppf_lpc_adam(ppf_lpc *h, float lr, uint32_t step, void *stream) {
+    if (!h || step == 0) {
+        set_error("ppf_lpc_adam: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.adam(lr, step, (hipStream_t)stream);
+}
+
+int ppf_pcg_solve_lpc(ppf_pcg_plan *plan, ppf_lpc *h, const float *d_value,
+                      const float *d_cdiag, const float *d_b, float *d_x, float tol,
+                      uint32_t max_iter, ppf_pcg_result *out, void *stream) {
+    if (!plan || !h || !h->impl.Gd() || !out) {
+        set_error("ppf_pcg_solve_lpc: invalid argument (or no factor computed)");
+        return PPF_ERR_ARG;
+    }
+    if (h->impl.n() != pcg_pattern(plan->impl).n || h->impl.nnz() != pcg_pattern(plan->impl).nnz) {
+        set_error("ppf_pcg_solve_lpc: the factor was computed for another system");
+        return PPF_ERR_ARG;
+    }
+    return pcg_solve(plan->impl, d_value, d_cdiag, d_b, d_x, tol, max_iter, nullptr, out,
+                     (hipStream_t)stream, h->impl.Gd(), h->impl.Gl());
 }
 
 int ppf_pcg_solve(ppf_pcg_plan *plan, const float *d_value,

@@ -77,6 +77,11 @@ struct Args {
     float *part; // [G1 pAp][G1 |terms|][G2 rz][G2 err]
     float *err_hist;
     Scalars *s;
+    // Factor preconditioner M^-1 = G^T G (null: block Jacobi): Gd the
+    // diagonal blocks of G (n), Gl[s] = G_ji for every stored slot s = (i, j),
+    // i < j (the lower block mirrored by the slot); gy holds G r.
+    const float *Gd, *Gl;
+    float *gy;
 };
 
 // ---- pattern preparation (once per sparsity pattern) ----------------------
@@ -384,7 +389,9 @@ __global__ __launch_bounds__(kNT) void k_spmv_split(const Args *__restrict__ A,
         }
         const f3 vi = load3(v + 3 * (size_t)ii);
         f3 sum = mk3(0, 0, 0), as = mk3(0, 0, 0);
-        for (uint32_t k0 = 0; k0 < ns + nm; k0 += 4 * M) {
+        // the first pass is unconditional (straight-line loads for the common
+        // row); rows longer than 4 M entries take further passes
+        for (uint32_t k0 = 0; k0 == 0 || k0 < ns + nm; k0 += 4 * M) {
             uint32_t cj[M], sl[M];
 #pragma unroll
             for (int m = 0; m < M; ++m) {
@@ -590,6 +597,7 @@ __global__ __launch_bounds__(kNT) void k_spmv_ell(const Args *__restrict__ A,
 }
 
 // r = b - y, z = P r, p = z, partials of r.z and |r|_1 (solver.cu:1015-1093).
+template <bool FAC>
 __global__ __launch_bounds__(kNT) void k_init_residual(const Args *__restrict__ A) {
     __shared__ float lds[kNT / kWave];
     const uint32_t n = A->n;
@@ -607,22 +615,28 @@ __global__ __launch_bounds__(kNT) void k_init_residual(const Args *__restrict__ 
         const f3 yi = load3(yv + 3 * (size_t)i);
         const f3 ri = mk3(bi.x + (-1.0f) * yi.x, bi.y + (-1.0f) * yi.y,
                           bi.z + (-1.0f) * yi.z);
-        const f3 zi = mv(P + 9 * (size_t)i, ri);
         store3(r + 3 * (size_t)i, ri);
-        store3(z + 3 * (size_t)i, zi);
-        store3(p + 3 * (size_t)i, zi);
-        rz = rz + ri.x * zi.x;
-        rz = rz + ri.y * zi.y;
-        rz = rz + ri.z * zi.z;
+        if (!FAC) {
+            const f3 zi = mv(P + 9 * (size_t)i, ri);
+            store3(z + 3 * (size_t)i, zi);
+            store3(p + 3 * (size_t)i, zi);
+            rz = rz + ri.x * zi.x;
+            rz = rz + ri.y * zi.y;
+            rz = rz + ri.z * zi.z;
+        }
         err = err + fabsf(ri.x);
         err = err + fabsf(ri.y);
         err = err + fabsf(ri.z);
     }
-    rz = block_sum<kNT>(rz, lds);
+    if (!FAC) {
+        rz = block_sum<kNT>(rz, lds);
+    }
     err = block_sum<kNT>(err, lds);
     if (threadIdx.x == 0) {
         gptr<float> part = as_global(A->part);
-        part[2 * A->G1 + L] = rz;
+        if (!FAC) {
+            part[2 * A->G1 + L] = rz;
+        }
         part[2 * A->G1 + A->G2 + L] = err;
     }
 }
@@ -672,6 +686,7 @@ __global__ __launch_bounds__(kNT) void k_init_scalars(const Args *__restrict__ A
 // alpha from the SpMV partials with the curvature guard, then x += alpha p,
 // r -= alpha Ap, z = P r and the r.z / |r|_1 partials (solver.cu:431-514).
 // k = position of this iteration inside the captured graph.
+template <bool FAC>
 __global__ __launch_bounds__(kNT) void k_update(const Args *__restrict__ A,
                                                 uint32_t k) {
     __shared__ float lds[kNT / kWave];
@@ -710,23 +725,89 @@ __global__ __launch_bounds__(kNT) void k_update(const Args *__restrict__ A,
                  fmaf(alpha, pi.z, xi.z));
         ri = mk3(fmaf(-alpha, api.x, ri.x), fmaf(-alpha, api.y, ri.y),
                  fmaf(-alpha, api.z, ri.z));
-        const f3 zi = mv(P + 9 * (size_t)i, ri);
         store3(x + o, xi);
         store3(r + o, ri);
-        store3(z + o, zi);
-        rz = rz + ri.x * zi.x;
-        rz = rz + ri.y * zi.y;
-        rz = rz + ri.z * zi.z;
+        if (!FAC) { // the factor form applies z = G^T G r in two kernels after
+            const f3 zi = mv(P + 9 * (size_t)i, ri);
+            store3(z + o, zi);
+            rz = rz + ri.x * zi.x;
+            rz = rz + ri.y * zi.y;
+            rz = rz + ri.z * zi.z;
+        }
         err = err + fabsf(ri.x);
         err = err + fabsf(ri.y);
         err = err + fabsf(ri.z);
     }
-    rz = block_sum<kNT>(rz, lds);
+    if (!FAC) {
+        rz = block_sum<kNT>(rz, lds);
+    }
     err = block_sum<kNT>(err, lds);
     if (threadIdx.x == 0) {
         gptr<float> pw = as_global(A->part);
-        pw[2 * A->G1 + L] = rz;
+        if (!FAC) {
+            pw[2 * A->G1 + L] = rz;
+        }
         pw[2 * A->G1 + A->G2 + L] = err;
+    }
+}
+
+// gy = G r: row j takes its diagonal block and, through its mirrored
+// entries (slot s = (i, j), i < j), the lower blocks G_ji = Gl[s].
+__global__ __launch_bounds__(kNT) void k_fac_g(const Args *__restrict__ A, int init) {
+    if (!init && as_global(A->s)->done) {
+        return;
+    }
+    const uint32_t n = A->n;
+    const uint32_t j = blockIdx.x * kNT + threadIdx.x;
+    if (j >= n) {
+        return;
+    }
+    gptr<const float> Gd = as_global(A->Gd), Gl = as_global(A->Gl), r = as_global(A->r);
+    gptr<const uint32_t> tp = as_global(A->tp), tcol = as_global(A->tcol),
+                         tslot = as_global(A->tslot);
+    f3 y = mv(Gd + 9 * (size_t)j, load3(r + 3 * (size_t)j));
+    for (uint32_t k = tp[j]; k < tp[j + 1]; ++k) {
+        y = y + mv(Gl + 9 * (size_t)tslot[k], load3(r + 3 * (size_t)tcol[k]));
+    }
+    store3(as_global(A->gy) + 3 * (size_t)j, y);
+}
+
+// z = G^T gy: row i takes its diagonal block transposed and, through its
+// stored off-diagonal slots s = (i, j), G_ji^T = Gl[s]^T; then the r.z
+// partials (and p = z when init).
+__global__ __launch_bounds__(kNT) void k_fac_gt(const Args *__restrict__ A, int init) {
+    __shared__ float lds[kNT / kWave];
+    if (!init && as_global(A->s)->done) {
+        return;
+    }
+    const uint32_t n = A->n;
+    const uint32_t L = blockIdx.x;
+    const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
+    gptr<const float> Gd = as_global(A->Gd), Gl = as_global(A->Gl), gy = as_global(A->gy);
+    gptr<const float> r = as_global(A->r);
+    gptr<const uint32_t> rp = as_global(A->rp), col = as_global(A->col);
+    gptr<float> z = as_global(A->z), p = as_global(A->p);
+    float rz = 0.0f;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kNT) {
+        f3 zi = mvt(Gd + 9 * (size_t)i, load3(gy + 3 * (size_t)i));
+        for (uint32_t s = rp[i]; s < rp[i + 1]; ++s) {
+            const uint32_t j = col[s];
+            if (j != i) {
+                zi = zi + mvt(Gl + 9 * (size_t)s, load3(gy + 3 * (size_t)j));
+            }
+        }
+        store3(z + 3 * (size_t)i, zi);
+        if (init) {
+            store3(p + 3 * (size_t)i, zi);
+        }
+        const f3 ri = load3(r + 3 * (size_t)i);
+        rz = rz + ri.x * zi.x;
+        rz = rz + ri.y * zi.y;
+        rz = rz + ri.z * zi.z;
+    }
+    rz = block_sum<kNT>(rz, lds);
+    if (threadIdx.x == 0) {
+        as_global(A->part)[2 * A->G1 + L] = rz;
     }
 }
 
@@ -837,7 +918,9 @@ size_t carve_bytes(size_t count, size_t elem) {
 }
 
 // The SpMV kernel a plan launches (its graph is captured for one).
-uint32_t spmv_key(const PcgPlanImpl &pl) { return pl.split ? 100u + pl.split_m : pl.h.ell_m; }
+uint32_t spmv_key(const PcgPlanImpl &pl) {
+    return (pl.h.Gd ? 1000u : 0u) + (pl.split ? 100u + pl.split_m : pl.h.ell_m);
+}
 
 // y = M v with the plan's SpMV form (ELL width 4*m, or the CSR walk).
 template <bool DOT>
@@ -870,7 +953,13 @@ hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
     if (e != hipSuccess) {
         return e;
     }
-    k_update<<<h.G2, kNT, 0, st>>>(pl.d_args, k);
+    if (h.Gd) {
+        k_update<true><<<h.G2, kNT, 0, st>>>(pl.d_args, k);
+        k_fac_g<<<cdiv(h.n, kNT), kNT, 0, st>>>(pl.d_args, 0);
+        k_fac_gt<<<h.G2, kNT, 0, st>>>(pl.d_args, 0);
+    } else {
+        k_update<false><<<h.G2, kNT, 0, st>>>(pl.d_args, k);
+    }
     k_pupdate<<<h.G3, kNT, 0, st>>>(pl.d_args, k);
     return hipGetLastError();
 }
@@ -1044,7 +1133,7 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     const size_t n3 = 3 * (size_t)n;
     const size_t scan_words = scan_scratch_words(n + 1);
     size_t bytes = carve_bytes(n + 1, 4) * 2 + carve_bytes(n, 4) +
-                   carve_bytes(9 * (size_t)n, 4) + carve_bytes(n3, 4) * 4 +
+                   carve_bytes(9 * (size_t)n, 4) + carve_bytes(n3, 4) * 5 +
                    carve_bytes(2 * (size_t)h.G1 + 2 * (size_t)h.G2, 4) +
                    carve_bytes(1, sizeof(Scalars)) + carve_bytes(1, sizeof(Args)) +
                    carve_bytes(scan_words, 4) + carve_bytes(4, 4);
@@ -1063,6 +1152,7 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     h.z = carve<float>(cur, n3);
     h.p = carve<float>(cur, n3);
     h.Ap = carve<float>(cur, n3);
+    h.gy = carve<float>(cur, n3);
     h.part = carve<float>(cur, 2 * (size_t)h.G1 + 2 * (size_t)h.G2);
     h.s = carve<Scalars>(cur, 1);
     pl->d_args = carve<Args>(cur, 1);
@@ -1085,6 +1175,11 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     }
     *out = pl;
     return PPF_OK;
+}
+
+PcgPattern pcg_pattern(const PcgPlanImpl *pl) {
+    const Args &h = pl->h;
+    return PcgPattern{pl->n, pl->nnz, h.rp, h.col, h.tp, h.tcol, h.tslot, h.dslot};
 }
 
 int pcg_plan_repattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
@@ -1183,8 +1278,9 @@ int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
 
 int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
               const float *b, float *x, float tol, uint32_t max_iter,
-              float *err_hist, ppf_pcg_result *res, hipStream_t st) {
-    if (!pl || !val || !cdiag || !b || !x || !res || max_iter == 0) {
+              float *err_hist, ppf_pcg_result *res, hipStream_t st, const float *Gd,
+              const float *Gl) {
+    if (!pl || !val || !cdiag || !b || !x || !res || max_iter == 0 || (!Gd != !Gl)) {
         set_error("pcg_solve: invalid argument");
         return PPF_ERR_ARG;
     }
@@ -1194,6 +1290,15 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
     h.b = b;
     h.x = x;
     h.err_hist = err_hist;
+    h.Gd = Gd;
+    h.Gl = Gl;
+    if (spmv_key(*pl) != pl->graph_key) {
+        // the iteration graph holds the preconditioner's kernels
+        int rc = capture_graph(pl);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+    }
     PPF_HIP(hipMemcpyAsync(pl->d_args, &h, sizeof(Args), hipMemcpyHostToDevice,
                            st));
     Scalars init{};
@@ -1201,10 +1306,18 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
     init.max_iter = max_iter;
     PPF_HIP(hipMemcpyAsync(h.s, &init, sizeof(Scalars), hipMemcpyHostToDevice,
                            st));
-    k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
-    PPF_HIP(hipGetLastError());
+    if (!Gd) {
+        k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
+        PPF_HIP(hipGetLastError());
+    }
     PPF_HIP(launch_spmv<false>(*pl, x, h.Ap, st));
-    k_init_residual<<<h.G2, kNT, 0, st>>>(pl->d_args);
+    if (Gd) {
+        k_init_residual<true><<<h.G2, kNT, 0, st>>>(pl->d_args);
+        k_fac_g<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args, 1);
+        k_fac_gt<<<h.G2, kNT, 0, st>>>(pl->d_args, 1);
+    } else {
+        k_init_residual<false><<<h.G2, kNT, 0, st>>>(pl->d_args);
+    }
     PPF_HIP(hipGetLastError());
     k_init_scalars<<<1, kNT, 0, st>>>(pl->d_args);
     PPF_HIP(hipGetLastError());

@@ -14,9 +14,20 @@ void pcg_plan_destroy(PcgPlanImpl *pl);
 // iteration graph is re-captured only when the SpMV form changes).
 int pcg_plan_repattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
                        hipStream_t st);
+// Gd / Gl (both or neither): the factor preconditioner M^-1 = G^T G with
+// diagonal blocks Gd (n) and the lower blocks by slot (Gl[s] = G_ji for the
+// stored slot s = (i, j), i < j); null: block Jacobi.
 int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
               const float *b, float *x, float tol, uint32_t max_iter,
-              float *err_hist, ppf_pcg_result *res, hipStream_t st);
+              float *err_hist, ppf_pcg_result *res, hipStream_t st,
+              const float *Gd = nullptr, const float *Gl = nullptr);
+// Pattern arrays of the plan (device): row offsets, columns, transpose table
+// (per row the mirrored entries: source row and slot), diagonal slot per row.
+struct PcgPattern {
+    uint32_t n, nnz;
+    const uint32_t *rp, *col, *tp, *tcol, *tslot, *dslot;
+};
+PcgPattern pcg_pattern(const PcgPlanImpl *pl);
 int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
                   const float *x, float *y, uint32_t reps, float *avg_ms,
                   hipStream_t st);

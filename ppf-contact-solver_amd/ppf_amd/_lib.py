@@ -29,6 +29,17 @@ _PROTOS = {
     "ppf_pcg_plan_create": (ctypes.c_int, [u32, u32, vp, vp, vp, ctypes.POINTER(vp)]),
     "ppf_pcg_plan_destroy": (None, [vp]),
     "ppf_pcg_plan_repattern": (ctypes.c_int, [vp, u32, vp, vp, vp]),
+    "ppf_lpc_weight_count": (u32, []),
+    "ppf_lpc_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
+    "ppf_lpc_destroy": (None, [vp]),
+    "ppf_lpc_set_weights": (ctypes.c_int, [vp, vp, u32, vp]),
+    "ppf_lpc_get_weights": (ctypes.c_int, [vp, vp, u32, vp]),
+    "ppf_lpc_factor": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+    "ppf_lpc_outputs": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+    "ppf_lpc_gradient": (ctypes.c_int, [vp, vp, vp, vp, vp, u32, ctypes.POINTER(f32), vp, vp]),
+    "ppf_lpc_adam": (ctypes.c_int, [vp, f32, u32, vp]),
+    "ppf_pcg_solve_lpc": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, f32, u32,
+                                         ctypes.POINTER(PcgResult), vp]),
     "ppf_pcg_solve": (ctypes.c_int, [vp, vp, vp, vp, vp, f32, u32,
                                      ctypes.POINTER(PcgResult), vp]),
     "ppf_pcg_solve_hist": (ctypes.c_int, [vp, vp, vp, vp, vp, f32, u32, vp,
