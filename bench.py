@@ -7,8 +7,12 @@ apart with seeded 1e-4 m vertex noise, falling onto a cylinder of radius
 fp32, contact gap 1e-3 m, friction 0.5, dt = 1/60 s. W untimed time steps,
 then K timed time steps; every Newton step of those K steps is timed (elastic
 and contact assembly, block-Jacobi PCG to 1e-3 L1 relative residual, CCD
-line search). value = wall milliseconds per Newton step; the PCG iterations
-per solve are reported beside it. Inputs are resident in HBM before timing.
+line search). value = wall milliseconds per Newton step with the
+reference's block-Jacobi preconditioner; the PCG iterations per solve are
+reported beside it. "learned": the learned preconditioner, trained first on
+Newton systems of the same scene family (8 x 64x64 sheets, other seeds), on
+the same time steps and on the same last Newton system. Inputs are resident
+in HBM before timing.
 
 --workload pcg-grid: the kernel-isolation PCG system (1024 x 1024 grid, ~1M
 rows of 3x3 blocks); one step = one block-Jacobi PCG solve.
@@ -131,6 +135,9 @@ def run_drape(args, dev):
     total_ms = (time.perf_counter() - t0) * 1e3
     per_newton = total_ms / newton
     sysd = gpu.system()
+    learned = None
+    if args.precond == "both":
+        learned = run_learned(args, dev, sysd)
     n, nnz = sysd["row_offset"].numel() - 1, sysd["col_index"].numel()
     plan = BlockJacobiPCG(sysd["row_offset"], sysd["col_index"])
     roof = spmv_roofline(plan, sysd["value"], sysd["cdiag"], sysd["b"], n, nnz,
@@ -146,6 +153,7 @@ def run_drape(args, dev):
                 "time_steps": args.steps, "newton_steps": newton,
                 "tol": 1e-3}
     extra = {"pcg_iters": round(pcg / newton, 1),
+             "learned": learned,
              "phase_ms_per_newton_step": {k: round(v / newton, 3) for k, v in phase.items()},
              "max_contact_pairs": pairs,
              "roofline": roof, "cpu_baseline": None}
@@ -153,6 +161,82 @@ def run_drape(args, dev):
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_drape(args)
     return out
+
+
+def run_learned(args, dev, last_system):
+    """The learned preconditioner: trained on Newton systems of the same
+    scene family at 64 x 64 per sheet on other seeds (the block-Jacobi solver
+    generates them), then (a) the last Newton system of the timed block-Jacobi
+    steps solved both ways, (b) the same time steps re-run with the learned
+    factor in every Newton solve."""
+    import numpy as np
+    import torch
+
+    from ppf_amd import lpc_train, scenes
+    from ppf_amd.stepper import Stepper, scene_to_device
+
+    t0 = time.perf_counter()
+    systems = lpc_train.collect_drape_systems(dev, res=64, layers=args.layers, seeds=(0, 1, 2),
+                                              steps=12)
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lpc, hist = lpc_train.train(systems, steps=args.lpc_steps, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    same = lpc_train.compare(lpc, [last_system])[0]
+    del systems
+    # the timed steps again, learned factor in every Newton solve
+    sc = scenes.stacked_drape_scene(res=args.res, layers=args.layers)
+    ns = args.layers * args.res * args.res
+    t = scene_to_device(sc, dev)
+    t["fix_delta"] = torch.zeros_like(t["mesh"]["x"])
+    gpu = Stepper(t)
+    gpu.set_preconditioner(lpc)
+    p = scenes.step_params()
+    clock = [0.0]
+
+    def one_step():
+        now = scenes.spin_fix_positions(sc, clock[0])[ns:].copy()
+        aim = scenes.spin_fix_positions(sc, clock[0] + p["dt"])[ns:].copy()
+        t["fix_pos"].view(-1, 3)[ns:] = torch.from_numpy(aim).to(dev)
+        t["fix_delta"].view(-1, 3)[ns:] = torch.from_numpy(aim - now).to(dev)
+        r = gpu.advance(p)
+        if r["status"] != 0:
+            raise RuntimeError(f"learned step failed: {r}")
+        p["prev_dt"] = r["dt"]
+        clock[0] += r["dt"]
+        return r
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    newton = pcg = fallbacks = 0
+    solve_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = one_step()
+        newton += r["newton_steps"]
+        pcg += r["pcg_iter_total"]
+        fallbacks += r["lpc_fallbacks"]
+        solve_ms += r["ms_solve"]
+    torch.cuda.synchronize()
+    total = (time.perf_counter() - t0) * 1e3
+    gpu.close()
+    lpc.close()
+    return {"ms_per_newton_step": round(total / newton, 4),
+            "pcg_iters": round(pcg / newton, 1),
+            "solve_ms_per_newton_step": round(solve_ms / newton, 3),
+            "newton_steps": newton, "block_jacobi_fallbacks": fallbacks,
+            "same_system": {"block_jacobi_iters": same["bj_iter"],
+                            "learned_iters": same["lpc_iter"],
+                            "block_jacobi_solve_ms": round(same["bj_ms"], 3),
+                            "learned_solve_ms": round(same["lpc_ms"], 3)},
+            "training": {"systems": "the last Newton system of each of 12 time steps, "
+                                    "stacked drape 8 x 64x64, seeds 0-2",
+                         "adam_steps": args.lpc_steps,
+                         "loss_first_last": [round(float(np.mean(hist[:20])), 4),
+                                             round(float(np.mean(hist[-20:])), 4)],
+                         "collect_s": round(t_collect, 2), "train_s": round(t_train, 2)}}
 
 
 def cpu_drape(args):
@@ -236,6 +320,9 @@ def main():
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--layers", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precond", choices=("bj", "both"), default="both",
+                    help="both: also train and time the learned preconditioner")
+    ap.add_argument("--lpc-steps", type=int, default=400)
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a 256 MiB device copy (PMC calibration)")
     args = ap.parse_args()

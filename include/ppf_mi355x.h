@@ -469,6 +469,7 @@ typedef struct ppf_step_result {
     float toi_advanced;
     float dt;
     float ms_elastic, ms_contact, ms_solve, ms_line_search, ms_total;
+    uint32_t lpc_fallbacks; /* solves the learned factor handed to block Jacobi */
 } ppf_step_result;
 
 typedef struct ppf_stepper ppf_stepper;
@@ -485,6 +486,12 @@ void ppf_stepper_destroy(ppf_stepper *h);
  * prev / curr. */
 int ppf_stepper_advance(ppf_stepper *h, const ppf_scene *scene, const ppf_step_params *p,
                         ppf_step_result *out, void *stream);
+
+/* Precondition the stepper's Newton solves with the learned factor of h
+ * (recomputed for every Newton system; ms_solve includes it), falling back
+ * to block Jacobi for a solve that does not reach the tolerance. NULL
+ * restores block Jacobi. The handle must outlive its use. */
+int ppf_stepper_set_preconditioner(ppf_stepper *s, ppf_lpc *h);
 
 /* Re-read the pin set after the scene's mesh.vflag bit0 or fix_pos changed
  * (update_constraint, main.cu:1988-2077). */

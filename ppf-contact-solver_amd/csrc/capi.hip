@@ -609,6 +609,15 @@ int ppf_stepper_advance(ppf_stepper *h, const ppf_scene *scene, const ppf_step_p
     return h->impl.advance(*scene, *p, out, (hipStream_t)stream);
 }
 
+int ppf_stepper_set_preconditioner(ppf_stepper *s, ppf_lpc *h) {
+    if (!s) {
+        set_error("ppf_stepper_set_preconditioner: stepper is null");
+        return PPF_ERR_ARG;
+    }
+    s->impl.set_lpc(h ? &h->impl : nullptr);
+    return PPF_OK;
+}
+
 int ppf_stepper_update_pins(ppf_stepper *h, const ppf_scene *scene, void *stream) {
     if (!h || !scene) {
         set_error("ppf_stepper_update_pins: invalid argument");

@@ -483,9 +483,31 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         return rc;
     }
     ppf_pcg_result pr{};
-    rc = pcg_solve(plan_, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr, &pr, s);
-    if (rc != PPF_OK) {
-        return rc;
+    bool solved = false;
+    if (lpc_) {
+        // learned factor; block Jacobi when it cannot be formed or the solve
+        // does not reach the tolerance
+        rc = lpc_->prepare(pcg_pattern(plan_), mval_, cdiag_, force_, s);
+        if (rc == PPF_OK) {
+            rc = lpc_->forward(s);
+        }
+        if (rc == PPF_OK) {
+            rc = pcg_solve(plan_, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr,
+                           &pr, s, lpc_->Gd(), lpc_->Gl());
+        }
+        solved = rc == PPF_OK && (pr.status == 0 || pr.status == 1);
+        if (!solved) {
+            ++out->lpc_fallbacks;
+            k_dx_init<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, ex_, target_, dx_);
+            PPF_HIP(hipGetLastError());
+        }
+    }
+    if (!solved) {
+        rc = pcg_solve(plan_, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr, &pr,
+                       s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
     }
     out->pcg_iter_total += pr.iter;
     if (pr.iter > out->pcg_iter_max) {

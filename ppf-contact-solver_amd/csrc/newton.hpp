@@ -3,6 +3,7 @@
 #include "assemble.hpp"
 #include "elastic.hpp"
 #include "isect.hpp"
+#include "learned.hpp"
 #include "linesearch.hpp"
 #include "narrow.hpp"
 #include "pcg.hpp"
@@ -21,6 +22,8 @@ class Stepper {
                 hipStream_t s);
     const float *eval_x() const { return ex_; }
     uint32_t nv() const { return nv_; }
+    // Learned preconditioner for the Newton solves (null: block Jacobi).
+    void set_lpc(Lpc *lpc) { lpc_ = lpc; }
     static constexpr uint32_t kMaxRecords = 256;
     // Records of the last post-step intersection check.
     uint32_t records(ppf_isect_record *out, uint32_t cap) const;
@@ -55,6 +58,7 @@ class Stepper {
     uint64_t mcap_ = 0;
     uint32_t npinned_ = 0;
     PcgPlanImpl *plan_ = nullptr;
+    Lpc *lpc_ = nullptr;
     uint32_t sys_nnz_ = 0;
     ppf_isect_record records_[kMaxRecords] = {};
     uint32_t records_count_ = 0;

@@ -1039,12 +1039,16 @@ int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_
             break;
         }
     }
-    // Split form: a pass covers the average row (stored + mirrored entries,
-    // 2 nnz - n with every diagonal stored), longer rows take more passes.
+    // SpMV form: the padded ELL table when every row fits it with at most
+    // twice the average row (measured fastest on mesh-like patterns), else
+    // the split form,
+    // whose pass covers the average row (stored + mirrored entries, 2 nnz - n
+    // with every diagonal stored) and takes long rows in further passes.
     {
         const double avg = n ? (2.0 * nnz - n) / n : 0.0;
         const uint32_t m = (uint32_t)((avg + 1.0 + 3.999) / 4.0);
         pl->split_m = m < 2 ? 2 : (m > 4 ? 4 : m);
+        pl->split = !(h.ell_m && 4.0 * h.ell_m <= 2.0 * avg);
     }
     h.ent = nullptr;
     if (h.ell_m && !pl->split) {
@@ -1116,10 +1120,6 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
         return PPF_ERR_ARG;
     }
     pl->n = n;
-    {
-        const char *f = getenv("PPF_SPMV_FORM");
-        pl->split = f && !strcmp(f, "split");
-    }
     Args &h = pl->h;
     h.n = n;
     // Row ranges are multiples of 64 rows: one pass of either SpMV form.

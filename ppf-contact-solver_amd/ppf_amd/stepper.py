@@ -43,7 +43,8 @@ class StepResultC(ctypes.Structure):
     _fields_ = [("status", i32), ("newton_steps", u32), ("pcg_iter_total", u32),
                 ("pcg_iter_max", u32), ("pairs_max", u32), ("intersections", u32),
                 ("toi_advanced", f32), ("dt", f32), ("ms_elastic", f32), ("ms_contact", f32),
-                ("ms_solve", f32), ("ms_line_search", f32), ("ms_total", f32)]
+                ("ms_solve", f32), ("ms_line_search", f32), ("ms_total", f32),
+                ("lpc_fallbacks", u32)]
 
 
 def params_struct(p):
@@ -116,6 +117,12 @@ class Stepper:
             self.close()
         except Exception:
             pass
+
+    def set_preconditioner(self, lpc):
+        """Use a LearnedPreconditioner for the Newton solves (None: block Jacobi)."""
+        self._lpc = lpc
+        check(self.lib.ppf_stepper_set_preconditioner(self._h, lpc._h if lpc else None),
+              "ppf_stepper_set_preconditioner")
 
     def system(self):
         """Copy of the last Newton system solved: dict of device tensors
