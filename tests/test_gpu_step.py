@@ -173,3 +173,50 @@ def test_last_newton_system_is_a_solvable_dirichlet_system(dev):
     assert res_.status == 0
     plan.close()
     gpu.close()
+
+
+def test_stacked_sheets_on_spinning_cylinder(dev):
+    # the stacked-layer scene at 2 x 16 x 16, cylinder pinned and driven
+    # kinematically (positions aimed at each step's end, step deltas for the
+    # rewind of shortened steps): per step against the oracle
+    sc = scenes.stacked_drape_scene(res=16, layers=2, n_around=24, n_along=8)
+    ns = 2 * 16 * 16
+    sc["fix_delta"] = np.zeros_like(sc["fix_pos"])
+    sco = copy.deepcopy(sc)
+    t = scene_to_device(sc, dev)
+    gpu = Stepper(t)
+    orc = oracle.OracleStepper(sco)
+    # 100-450 PCG iterations per step: a tight solve keeps the comparison off
+    # the tolerance ball (see test_drape_c1_resolution)
+    p = scenes.step_params(cg_tol=1e-5)
+    clock = 0.0
+    for k in range(8):
+        now = scenes.spin_fix_positions(sc, clock)[ns:].copy()
+        aim = scenes.spin_fix_positions(sc, clock + p["dt"])[ns:].copy()
+        t["fix_pos"].view(-1, 3)[ns:] = torch.from_numpy(aim).to(dev)
+        t["fix_delta"].view(-1, 3)[ns:] = torch.from_numpy(aim - now).to(dev)
+        sco["fix_pos"][ns:] = aim
+        sco["fix_delta"][ns:] = aim - now
+        orc.delta[:] = sco["fix_delta"].reshape(-1)
+        sco["mesh"]["x"][:] = t["mesh"]["x"].cpu().numpy().reshape(-1, 3)
+        sco["prev"][:] = t["prev"].cpu().numpy().reshape(-1, 3)
+        xg0 = sco["mesh"]["x"].copy()
+        rg = gpu.advance(p)
+        ro = orc.advance(p)
+        xn = t["mesh"]["x"].cpu().numpy().reshape(-1, 3)
+        xo = sco["mesh"]["x"]
+        err = np.abs(xn - xo).max() / np.abs(xo).max()
+        print(f"step {k}: newton {rg['newton_steps']}/{ro['newton_steps']} pcg "
+              f"{rg['pcg_iter_total']}/{ro['pcg_iter_total']} pairs {rg['pairs_max']}/"
+              f"{ro['pairs_max']} toi {rg['toi_advanced']:.4f}/{ro['toi_advanced']:.4f} "
+              f"pos err {err:.2e} (disp {np.abs(xo - xg0).max():.2e})")
+        assert rg["status"] == ro["status"] == 0
+        assert rg["newton_steps"] == ro["newton_steps"]
+        assert abs(rg["dt"] - ro["dt"]) <= 1e-3 * ro["dt"]
+        assert err < 1e-5
+        # the cylinder ends on its (possibly rewound) path
+        fx = t["fix_pos"].cpu().numpy().reshape(-1, 3)[ns:]
+        assert np.array_equal(xn[ns:], fx)
+        p["prev_dt"] = rg["dt"]
+        clock += rg["dt"]
+    gpu.close()
