@@ -250,7 +250,7 @@ int ppf_pcg_solve_lpc(ppf_pcg_plan *plan, ppf_lpc *h, const float *d_value,
         return PPF_ERR_ARG;
     }
     return pcg_solve(plan->impl, d_value, d_cdiag, d_b, d_x, tol, max_iter, nullptr, out,
-                     (hipStream_t)stream, h->impl.Gd(), h->impl.Gl());
+                     (hipStream_t)stream, h->impl.Gd16(), h->impl.Gl16());
 }
 
 int ppf_pcg_solve(ppf_pcg_plan *plan, const float *d_value,

@@ -432,8 +432,16 @@ __global__ void k_heads(PcgPattern pt, const uint16_t *h3, const uint16_t *f, co
 __constant__ int kLowR[6] = {0, 1, 1, 2, 2, 2};
 __constant__ int kLowC[6] = {0, 0, 1, 0, 1, 2};
 
+__device__ inline void pack_g16(const float *g, uint16_t *o) {
+    for (int e = 0; e < 9; ++e) {
+        o[e] = to_bf16(g[e]);
+    }
+    o[9] = 0;
+}
+
 __global__ void k_factor(PcgPattern pt, const float *t, const float *u, const float *linv,
-                         const uint32_t *srow, float *ghd, float *ghl, float *Gd, float *Gl) {
+                         const uint32_t *srow, float *ghd, float *ghl, float *Gd, float *Gl,
+                         uint16_t *Gd16, uint16_t *Gl16) {
     const uint32_t k = blockIdx.x * kNT + threadIdx.x;
     if (k < pt.n) {
         float g[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -442,12 +450,14 @@ __global__ void k_factor(PcgPattern pt, const float *t, const float *u, const fl
         }
         for (int o = 0; o < 9; ++o) ghd[9 * (size_t)k + o] = g[o];
         mm3(g, linv + 9 * (size_t)k, Gd + 9 * (size_t)k);
+        pack_g16(Gd + 9 * (size_t)k, Gd16 + 10 * (size_t)k);
     }
     if (k < pt.nnz) {
         float g[9];
         for (int o = 0; o < 9; ++o) g[o] = 0.5f * tanhf(u[9 * (size_t)k + o]);
         for (int o = 0; o < 9; ++o) ghl[9 * (size_t)k + o] = g[o];
         mm3(g, linv + 9 * (size_t)srow[k], Gl + 9 * (size_t)k);
+        pack_g16(Gl + 9 * (size_t)k, Gl16 + 10 * (size_t)k);
     }
 }
 
@@ -816,7 +826,7 @@ int Lpc::reserve(uint32_t n, uint32_t nnz, uint32_t E) {
                    b(kH * EE, 2) * kLpcLayers + b(kH * N, 2) * kLpcLayers + b(6 * N, 4) +
                    b(9 * Z, 4) * 3 + b(9 * N, 4) * 3 + b(kH * N, 4) + b(9 * Z, 4) + b(16, 4) +
                    b(kH * rows, 2) * 2 + b(2 * kH * N, 4) + b(kH * EE, 4) + b(part_words, 4) +
-                   b(scan_scratch_words(N + 1), 4);
+                   b(scan_scratch_words(N + 1), 4) + b(10 * N, 2) + b(10 * Z, 2);
     PPF_HIP(hipMalloc(&mem_, bytes));
     mem_bytes_ = bytes;
     char *cur = static_cast<char *>(mem_);
@@ -841,6 +851,8 @@ int Lpc::reserve(uint32_t n, uint32_t nnz, uint32_t E) {
     Gl_ = carve<float>(cur, 9 * Z);
     ghd_ = carve<float>(cur, 9 * N);
     Gd_ = carve<float>(cur, 9 * N);
+    Gd16_ = carve<uint16_t>(cur, 10 * N);
+    Gl16_ = carve<uint16_t>(cur, 10 * Z);
     dgd_ = carve<float>(cur, 9 * N);
     dh_ = carve<float>(cur, kH * N);
     dgl_ = carve<float>(cur, 9 * Z);
@@ -943,7 +955,8 @@ int Lpc::forward(hipStream_t s) {
     const uint32_t m = std::max(n, nnz_);
     k_heads<<<cdiv(m, kNT), kNT, 0, s>>>(pt_, h_[kLpcLayers], f_, srow_, semir_, w_, t_, u_);
     PPF_HIP(hipGetLastError());
-    k_factor<<<cdiv(m, kNT), kNT, 0, s>>>(pt_, t_, u_, linv_, srow_, ghd_, ghl_, Gd_, Gl_);
+    k_factor<<<cdiv(m, kNT), kNT, 0, s>>>(pt_, t_, u_, linv_, srow_, ghd_, ghl_, Gd_, Gl_,
+                                          Gd16_, Gl16_);
     PPF_HIP(hipGetLastError());
     factored_ = true;
     return PPF_OK;

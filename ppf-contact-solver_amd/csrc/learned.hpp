@@ -51,6 +51,9 @@ class Lpc {
     // null until forward has run on the current system
     const float *Gd() const { return factored_ ? Gd_ : nullptr; }
     const float *Gl() const { return factored_ ? Gl_ : nullptr; }
+    // the same blocks in bf16 10-entry records (what the PCG applies)
+    const uint16_t *Gd16() const { return factored_ ? Gd16_ : nullptr; }
+    const uint16_t *Gl16() const { return factored_ ? Gl16_ : nullptr; }
     const float *t() const { return t_; }
     const float *u() const { return u_; }
     const float *grad() const { return grad_; }
@@ -81,6 +84,7 @@ class Lpc {
     uint16_t *h_[kLpcLayers + 1] = {}, *m_[kLpcLayers] = {}, *M_[kLpcLayers] = {};
     float *t_ = nullptr, *u_ = nullptr, *ghd_ = nullptr, *ghl_ = nullptr;
     float *Gd_ = nullptr, *Gl_ = nullptr;
+    uint16_t *Gd16_ = nullptr, *Gl16_ = nullptr;
     // backward scratch
     float *dh_ = nullptr, *dgd_ = nullptr, *dgl_ = nullptr, *scal_ = nullptr;
     uint16_t *dZ_ = nullptr, *dY_ = nullptr, *dXn_ = nullptr, *dXe_ = nullptr;

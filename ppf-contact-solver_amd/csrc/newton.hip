@@ -493,7 +493,7 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         }
         if (rc == PPF_OK) {
             rc = pcg_solve(plan_, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr,
-                           &pr, s, lpc_->Gd(), lpc_->Gl());
+                           &pr, s, lpc_->Gd16(), lpc_->Gl16());
         }
         solved = rc == PPF_OK && (pr.status == 0 || pr.status == 1);
         if (!solved) {

@@ -79,8 +79,9 @@ struct Args {
     Scalars *s;
     // Factor preconditioner M^-1 = G^T G (null: block Jacobi): Gd the
     // diagonal blocks of G (n), Gl[s] = G_ji for every stored slot s = (i, j),
-    // i < j (the lower block mirrored by the slot); gy holds G r.
-    const float *Gd, *Gl;
+    // i < j (the lower block mirrored by the slot), both bf16 column-major in
+    // 10-entry (20 B) records; gy holds G r.
+    const uint16_t *Gd, *Gl;
     float *gy;
 };
 
@@ -751,30 +752,65 @@ __global__ __launch_bounds__(kNT) void k_update(const Args *__restrict__ A,
     }
 }
 
-// gy = G r: row j takes its diagonal block and, through its mirrored
-// entries (slot s = (i, j), i < j), the lower blocks G_ji = Gl[s].
+// A bf16 factor block (10-entry record, column-major) times a vector, or
+// its transpose times the vector.
+__device__ inline void load_g16(gptr<const uint16_t> g, size_t blk, float *m) {
+    gptr<const uint32_t> w = reinterpret_cast<gptr<const uint32_t>>(g + 10 * blk);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const uint32_t v = w[k];
+        m[2 * k] = __uint_as_float(v << 16);
+        if (2 * k + 1 < 9) {
+            m[2 * k + 1] = __uint_as_float(v & 0xffff0000u);
+        }
+    }
+}
+
+// gy = G r, 4 lanes per row: row j's entries are its diagonal block, then
+// its mirrored entries (slot s = (i, j), i < j: G_ji = Gl[s] times r_i).
 __global__ __launch_bounds__(kNT) void k_fac_g(const Args *__restrict__ A, int init) {
     if (!init && as_global(A->s)->done) {
         return;
     }
     const uint32_t n = A->n;
-    const uint32_t j = blockIdx.x * kNT + threadIdx.x;
-    if (j >= n) {
-        return;
-    }
-    gptr<const float> Gd = as_global(A->Gd), Gl = as_global(A->Gl), r = as_global(A->r);
+    const uint32_t q = threadIdx.x & 3u;
+    const uint32_t j = blockIdx.x * (kNT / 4) + (threadIdx.x >> 2);
+    const bool valid = j < n;
+    const uint32_t jj = valid ? j : 0;
+    gptr<const uint16_t> Gd = as_global(A->Gd), Gl = as_global(A->Gl);
+    gptr<const float> r = as_global(A->r);
     gptr<const uint32_t> tp = as_global(A->tp), tcol = as_global(A->tcol),
                          tslot = as_global(A->tslot);
-    f3 y = mv(Gd + 9 * (size_t)j, load3(r + 3 * (size_t)j));
-    for (uint32_t k = tp[j]; k < tp[j + 1]; ++k) {
-        y = y + mv(Gl + 9 * (size_t)tslot[k], load3(r + 3 * (size_t)tcol[k]));
+    const uint32_t t0 = tp[jj], cnt = 1 + tp[jj + 1] - t0;
+    f3 y = mk3(0, 0, 0);
+    for (uint32_t k = q; k < cnt; k += 4) {
+        float m[9];
+        uint32_t src;
+        if (k == 0) {
+            load_g16(Gd, jj, m);
+            src = jj;
+        } else {
+            load_g16(Gl, tslot[t0 + k - 1], m);
+            src = tcol[t0 + k - 1];
+        }
+        const f3 v = load3(r + 3 * (size_t)src);
+        y = y + mk3(m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+                    m[2] * v.x + m[5] * v.y + m[8] * v.z);
     }
-    store3(as_global(A->gy) + 3 * (size_t)j, y);
+#pragma unroll
+    for (int off = 2; off > 0; off >>= 1) {
+        y.x += __shfl_xor(y.x, off, 4);
+        y.y += __shfl_xor(y.y, off, 4);
+        y.z += __shfl_xor(y.z, off, 4);
+    }
+    if (valid && q == 0) {
+        store3(as_global(A->gy) + 3 * (size_t)j, y);
+    }
 }
 
-// z = G^T gy: row i takes its diagonal block transposed and, through its
-// stored off-diagonal slots s = (i, j), G_ji^T = Gl[s]^T; then the r.z
-// partials (and p = z when init).
+// z = G^T gy, 4 lanes per row: row i's entries are its diagonal block
+// (transposed) and its stored off-diagonal slots s = (i, j): G_ji^T gy_j.
+// Then the r.z partials (and p = z when init).
 __global__ __launch_bounds__(kNT) void k_fac_gt(const Args *__restrict__ A, int init) {
     __shared__ float lds[kNT / kWave];
     if (!init && as_global(A->s)->done) {
@@ -783,27 +819,53 @@ __global__ __launch_bounds__(kNT) void k_fac_gt(const Args *__restrict__ A, int 
     const uint32_t n = A->n;
     const uint32_t L = blockIdx.x;
     const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
-    gptr<const float> Gd = as_global(A->Gd), Gl = as_global(A->Gl), gy = as_global(A->gy);
-    gptr<const float> r = as_global(A->r);
+    const uint32_t q = threadIdx.x & 3u, sub = threadIdx.x >> 2;
+    gptr<const uint16_t> Gd = as_global(A->Gd), Gl = as_global(A->Gl);
+    gptr<const float> gy = as_global(A->gy), r = as_global(A->r);
     gptr<const uint32_t> rp = as_global(A->rp), col = as_global(A->col);
     gptr<float> z = as_global(A->z), p = as_global(A->p);
     float rz = 0.0f;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kNT) {
-        f3 zi = mvt(Gd + 9 * (size_t)i, load3(gy + 3 * (size_t)i));
-        for (uint32_t s = rp[i]; s < rp[i + 1]; ++s) {
-            const uint32_t j = col[s];
-            if (j != i) {
-                zi = zi + mvt(Gl + 9 * (size_t)s, load3(gy + 3 * (size_t)j));
+    for (uint32_t base = lo; base < hi; base += kNT / 4) {
+        const uint32_t i = base + sub;
+        const bool valid = i < hi;
+        const uint32_t ii = valid ? i : lo;
+        const uint32_t r0 = rp[ii], cnt = 1 + rp[ii + 1] - r0;
+        f3 zi = mk3(0, 0, 0);
+        for (uint32_t k = q; k < cnt; k += 4) {
+            float m[9];
+            uint32_t src;
+            if (k == 0) {
+                load_g16(Gd, ii, m);
+                src = ii;
+            } else {
+                const uint32_t s = r0 + k - 1;
+                src = col[s];
+                if (src == ii) {
+                    continue; // the stored diagonal slot carries no factor block
+                }
+                load_g16(Gl, s, m);
             }
+            const f3 v = load3(gy + 3 * (size_t)src);
+            zi = zi + mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z,
+                          m[3] * v.x + m[4] * v.y + m[5] * v.z,
+                          m[6] * v.x + m[7] * v.y + m[8] * v.z);
         }
-        store3(z + 3 * (size_t)i, zi);
-        if (init) {
-            store3(p + 3 * (size_t)i, zi);
+#pragma unroll
+        for (int off = 2; off > 0; off >>= 1) {
+            zi.x += __shfl_xor(zi.x, off, 4);
+            zi.y += __shfl_xor(zi.y, off, 4);
+            zi.z += __shfl_xor(zi.z, off, 4);
         }
-        const f3 ri = load3(r + 3 * (size_t)i);
-        rz = rz + ri.x * zi.x;
-        rz = rz + ri.y * zi.y;
-        rz = rz + ri.z * zi.z;
+        if (valid && q == 0) {
+            store3(z + 3 * (size_t)i, zi);
+            if (init) {
+                store3(p + 3 * (size_t)i, zi);
+            }
+            const f3 ri = load3(r + 3 * (size_t)i);
+            rz = rz + ri.x * zi.x;
+            rz = rz + ri.y * zi.y;
+            rz = rz + ri.z * zi.z;
+        }
     }
     rz = block_sum<kNT>(rz, lds);
     if (threadIdx.x == 0) {
@@ -955,7 +1017,7 @@ hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
     }
     if (h.Gd) {
         k_update<true><<<h.G2, kNT, 0, st>>>(pl.d_args, k);
-        k_fac_g<<<cdiv(h.n, kNT), kNT, 0, st>>>(pl.d_args, 0);
+        k_fac_g<<<cdiv(h.n, kNT / 4), kNT, 0, st>>>(pl.d_args, 0);
         k_fac_gt<<<h.G2, kNT, 0, st>>>(pl.d_args, 0);
     } else {
         k_update<false><<<h.G2, kNT, 0, st>>>(pl.d_args, k);
@@ -1278,8 +1340,8 @@ int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
 
 int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
               const float *b, float *x, float tol, uint32_t max_iter,
-              float *err_hist, ppf_pcg_result *res, hipStream_t st, const float *Gd,
-              const float *Gl) {
+              float *err_hist, ppf_pcg_result *res, hipStream_t st, const uint16_t *Gd,
+              const uint16_t *Gl) {
     if (!pl || !val || !cdiag || !b || !x || !res || max_iter == 0 || (!Gd != !Gl)) {
         set_error("pcg_solve: invalid argument");
         return PPF_ERR_ARG;
@@ -1313,7 +1375,7 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
     PPF_HIP(launch_spmv<false>(*pl, x, h.Ap, st));
     if (Gd) {
         k_init_residual<true><<<h.G2, kNT, 0, st>>>(pl->d_args);
-        k_fac_g<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args, 1);
+        k_fac_g<<<cdiv(pl->n, kNT / 4), kNT, 0, st>>>(pl->d_args, 1);
         k_fac_gt<<<h.G2, kNT, 0, st>>>(pl->d_args, 1);
     } else {
         k_init_residual<false><<<h.G2, kNT, 0, st>>>(pl->d_args);

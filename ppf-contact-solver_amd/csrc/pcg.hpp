@@ -16,11 +16,12 @@ int pcg_plan_repattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
                        hipStream_t st);
 // Gd / Gl (both or neither): the factor preconditioner M^-1 = G^T G with
 // diagonal blocks Gd (n) and the lower blocks by slot (Gl[s] = G_ji for the
-// stored slot s = (i, j), i < j); null: block Jacobi.
+// stored slot s = (i, j), i < j), bf16 column-major in 10-entry records;
+// null: block Jacobi.
 int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
               const float *b, float *x, float tol, uint32_t max_iter,
               float *err_hist, ppf_pcg_result *res, hipStream_t st,
-              const float *Gd = nullptr, const float *Gl = nullptr);
+              const uint16_t *Gd = nullptr, const uint16_t *Gl = nullptr);
 // Pattern arrays of the plan (device): row offsets, columns, transpose table
 // (per row the mirrored entries: source row and slot), diagonal slot per row.
 struct PcgPattern {
