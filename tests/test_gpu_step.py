@@ -186,8 +186,13 @@ def test_stacked_sheets_on_spinning_cylinder(dev):
     t = scene_to_device(sc, dev)
     gpu = Stepper(t)
     orc = oracle.OracleStepper(sco)
-    # 100-450 PCG iterations per step: a tight solve keeps the comparison off
-    # the tolerance ball (see test_drape_c1_resolution)
+    # 200-1200 PCG iterations per step with 400+ active contacts. Even at
+    # cg_tol 1e-5 the two fp32 solvers (different summation orders) stop at
+    # different points of the tolerance ball of these stiff systems: measured
+    # per-step differences up to 2.9e-5 of the coordinate scale (step 7, 360
+    # vs 362 iterations). The bound is 1e-4 for positions, with the discrete
+    # outcomes (status, Newton count, dt) exact and the accepted toi within
+    # 1e-3 relative.
     p = scenes.step_params(cg_tol=1e-5)
     clock = 0.0
     for k in range(8):
@@ -213,10 +218,12 @@ def test_stacked_sheets_on_spinning_cylinder(dev):
         assert rg["status"] == ro["status"] == 0
         assert rg["newton_steps"] == ro["newton_steps"]
         assert abs(rg["dt"] - ro["dt"]) <= 1e-3 * ro["dt"]
-        assert err < 1e-5
-        # the cylinder ends on its (possibly rewound) path
+        assert abs(rg["toi_advanced"] - ro["toi_advanced"]) <= 1e-3 * ro["toi_advanced"]
+        assert err < 1e-4
+        # the rewound pin targets agree, and the cylinder follows the oracle's
         fx = t["fix_pos"].cpu().numpy().reshape(-1, 3)[ns:]
-        assert np.array_equal(xn[ns:], fx)
+        assert np.abs(fx - sco["fix_pos"][ns:]).max() <= 1e-6 * np.abs(fx).max()
+        assert np.abs(xn[ns:] - xo[ns:]).max() <= 1e-6 * np.abs(xo).max()
         p["prev_dt"] = rg["dt"]
         clock += rg["dt"]
     gpu.close()
