@@ -100,4 +100,91 @@ __device__ uint32_t traverse(const BvhView &bvh, const QBox &q, uint32_t *stk,
     return count;
 }
 
+// ---- wave-cooperative traversal --------------------------------------------
+//
+// For queries whose per-candidate work is long and data dependent (the CCD
+// sweeps): a workgroup is one wave of kWaveQueries queries. Every lane walks
+// its own query one node per round; a leaf that `accept(prim)` keeps enters a
+// queue in LDS as (query id, prim). Whenever the queue holds a wave's worth of
+// entries, and when the walks are over, every lane takes one entry and runs
+// `work(qid, prim)`, so the long work runs with (nearly) all lanes active
+// instead of with the few lanes that happen to sit on a leaf.
+
+constexpr int kWaveQueries = 64;        // one wave per workgroup
+constexpr int kWaveQueue = 2 * kWaveQueries;
+
+struct WaveScratch {
+    uint32_t stk[kQueryStack * kWaveQueries];
+    uint32_t qid[kWaveQueue], prim[kWaveQueue];
+};
+
+__device__ inline uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// `active`: this lane has a query (lanes past the end pass false). All lanes
+// of the workgroup must call it (the rounds are wave uniform).
+template <class A, class W>
+__device__ void traverse_wave(const BvhView &bvh, const QBox &q, bool active, uint32_t qid,
+                              WaveScratch &ws, uint32_t *overflow, A &&accept, W &&work) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t *stk = ws.stk + lane;
+    uint32_t sp = 0;
+    if (active && bvh.n != 0 && overlaps(load_box(bvh.box, bvh.root), q)) {
+        stk[0] = bvh.root;
+        sp = 1;
+    }
+    uint32_t queued = 0; // wave uniform
+    while (true) {
+        uint32_t cand = 0xffffffffu;
+        if (sp) {
+            const uint32_t v = stk[(--sp) * kWaveQueries];
+            const uint32_t c1 = bvh.node[2 * (size_t)v + 1];
+            if (c1 == 0) {
+                const uint32_t prim = bvh.node[2 * (size_t)v] - 1;
+                if (accept(prim)) {
+                    cand = prim;
+                }
+            } else {
+                const uint32_t l = bvh.node[2 * (size_t)v] - 1, r = c1 - 1;
+                const bool ol = overlaps(load_box(bvh.box, l), q);
+                const bool orr = overlaps(load_box(bvh.box, r), q);
+                if (sp + 2 > (uint32_t)kQueryStack) {
+                    atomicOr(overflow, 1u);
+                    sp = 0;
+                } else {
+                    if (ol) {
+                        stk[(sp++) * kWaveQueries] = l;
+                    }
+                    if (orr) {
+                        stk[(sp++) * kWaveQueries] = r;
+                    }
+                }
+            }
+        }
+        const uint64_t m = __ballot(cand != 0xffffffffu);
+        if (cand != 0xffffffffu) {
+            const uint32_t at = queued + lanes_below(m);
+            ws.qid[at] = qid;
+            ws.prim[at] = cand;
+        }
+        queued += (uint32_t)__popcll(m);
+        const bool walking = __ballot(sp != 0) != 0;
+        if (queued >= (uint32_t)kWaveQueries || (!walking && queued)) {
+            __syncthreads();
+            const uint32_t take = min(queued, (uint32_t)kWaveQueries);
+            if (lane < take) {
+                const uint32_t at = queued - 1 - lane;
+                work(ws.qid[at], ws.prim[at]);
+            }
+            queued -= take;
+            __syncthreads();
+        }
+        if (!walking && !queued) {
+            break;
+        }
+    }
+}
+
 } // namespace ppf

@@ -249,8 +249,9 @@ int ppf_pcg_solve_lpc(ppf_pcg_plan *plan, ppf_lpc *h, const float *d_value,
         set_error("ppf_pcg_solve_lpc: the factor was computed for another system");
         return PPF_ERR_ARG;
     }
+    const PcgFactor fac = h->impl.factor();
     return pcg_solve(plan->impl, d_value, d_cdiag, d_b, d_x, tol, max_iter, nullptr, out,
-                     (hipStream_t)stream, h->impl.Gd16(), h->impl.Gl16());
+                     (hipStream_t)stream, &fac);
 }
 
 int ppf_pcg_solve(ppf_pcg_plan *plan, const float *d_value,

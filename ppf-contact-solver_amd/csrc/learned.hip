@@ -461,6 +461,29 @@ __global__ void k_factor(PcgPattern pt, const float *t, const float *u, const fl
     }
 }
 
+// Rows of G in order for the PCG's G r pass: row j at j + tp[j] holds G_jj,
+// then G_ji for its mirrored entries in transpose-table order (4 lanes/row).
+__global__ void k_row_records(PcgPattern pt, const uint16_t *Gd16, const uint16_t *Gl16,
+                              uint16_t *Gr16) {
+    const uint32_t j = blockIdx.x * (kNT / 4) + (threadIdx.x >> 2);
+    const uint32_t q = threadIdx.x & 3u;
+    if (j >= pt.n) {
+        return;
+    }
+    const uint32_t t0 = pt.tp[j], cnt = 1 + pt.tp[j + 1] - t0;
+    const size_t g0 = (size_t)j + t0;
+    for (uint32_t k = q; k < cnt; k += 4) {
+        const uint16_t *src = k == 0 ? Gd16 + 10 * (size_t)j
+                                     : Gl16 + 10 * (size_t)pt.tslot[t0 + k - 1];
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(src);
+        uint32_t *o = reinterpret_cast<uint32_t *>(Gr16 + 10 * (g0 + k));
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            o[e] = w[e];
+        }
+    }
+}
+
 // ---- loss and its gradient with respect to Ghat -------------------------
 
 // y = Ghat^T z for every probe: y_a = Ghat_aa^T z_a + sum over row a's stored
@@ -826,7 +849,8 @@ int Lpc::reserve(uint32_t n, uint32_t nnz, uint32_t E) {
                    b(kH * EE, 2) * kLpcLayers + b(kH * N, 2) * kLpcLayers + b(6 * N, 4) +
                    b(9 * Z, 4) * 3 + b(9 * N, 4) * 3 + b(kH * N, 4) + b(9 * Z, 4) + b(16, 4) +
                    b(kH * rows, 2) * 2 + b(2 * kH * N, 4) + b(kH * EE, 4) + b(part_words, 4) +
-                   b(scan_scratch_words(N + 1), 4) + b(10 * N, 2) + b(10 * Z, 2);
+                   b(scan_scratch_words(N + 1), 4) + b(10 * N, 2) + b(10 * Z, 2) +
+                   b(10 * ((size_t)N + Z), 2);
     PPF_HIP(hipMalloc(&mem_, bytes));
     mem_bytes_ = bytes;
     char *cur = static_cast<char *>(mem_);
@@ -853,6 +877,7 @@ int Lpc::reserve(uint32_t n, uint32_t nnz, uint32_t E) {
     Gd_ = carve<float>(cur, 9 * N);
     Gd16_ = carve<uint16_t>(cur, 10 * N);
     Gl16_ = carve<uint16_t>(cur, 10 * Z);
+    Gr16_ = carve<uint16_t>(cur, 10 * ((size_t)N + Z));
     dgd_ = carve<float>(cur, 9 * N);
     dh_ = carve<float>(cur, kH * N);
     dgl_ = carve<float>(cur, 9 * Z);
@@ -957,6 +982,8 @@ int Lpc::forward(hipStream_t s) {
     PPF_HIP(hipGetLastError());
     k_factor<<<cdiv(m, kNT), kNT, 0, s>>>(pt_, t_, u_, linv_, srow_, ghd_, ghl_, Gd_, Gl_,
                                           Gd16_, Gl16_);
+    PPF_HIP(hipGetLastError());
+    k_row_records<<<cdiv(n, kNT / 4), kNT, 0, s>>>(pt_, Gd16_, Gl16_, Gr16_);
     PPF_HIP(hipGetLastError());
     factored_ = true;
     return PPF_OK;

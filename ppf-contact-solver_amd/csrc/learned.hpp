@@ -51,9 +51,11 @@ class Lpc {
     // null until forward has run on the current system
     const float *Gd() const { return factored_ ? Gd_ : nullptr; }
     const float *Gl() const { return factored_ ? Gl_ : nullptr; }
-    // the same blocks in bf16 10-entry records (what the PCG applies)
-    const uint16_t *Gd16() const { return factored_ ? Gd16_ : nullptr; }
-    const uint16_t *Gl16() const { return factored_ ? Gl16_ : nullptr; }
+    // the same blocks in bf16 10-entry records (what the PCG applies), plus
+    // the rows of G in order; null members until forward has run
+    PcgFactor factor() const {
+        return factored_ ? PcgFactor{Gd16_, Gl16_, Gr16_} : PcgFactor{nullptr, nullptr, nullptr};
+    }
     const float *t() const { return t_; }
     const float *u() const { return u_; }
     const float *grad() const { return grad_; }
@@ -84,7 +86,7 @@ class Lpc {
     uint16_t *h_[kLpcLayers + 1] = {}, *m_[kLpcLayers] = {}, *M_[kLpcLayers] = {};
     float *t_ = nullptr, *u_ = nullptr, *ghd_ = nullptr, *ghl_ = nullptr;
     float *Gd_ = nullptr, *Gl_ = nullptr;
-    uint16_t *Gd16_ = nullptr, *Gl16_ = nullptr;
+    uint16_t *Gd16_ = nullptr, *Gl16_ = nullptr, *Gr16_ = nullptr;
     // backward scratch
     float *dh_ = nullptr, *dgd_ = nullptr, *dgl_ = nullptr, *scal_ = nullptr;
     uint16_t *dZ_ = nullptr, *dY_ = nullptr, *dXn_ = nullptr, *dXe_ = nullptr;

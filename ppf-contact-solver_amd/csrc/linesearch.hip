@@ -17,7 +17,6 @@ namespace ppf {
 
 namespace {
 
-constexpr int kQT = kQueryThreads;
 constexpr int kStack = kQueryStack;
 using Box6 = QBox;
 
@@ -46,141 +45,148 @@ __device__ inline void note(const LsCtx &c, bool ovl, float r, float &best) {
     }
 }
 
-__global__ __launch_bounds__(kQT) void k_ls_vertex(LsCtx c) {
-    __shared__ uint32_t stk_all[kStack * kQT];
-    const uint32_t leaf = blockIdx.x * kQT + threadIdx.x;
-    if (leaf >= c.vb.n) {
-        return;
+constexpr int kWQ = kWaveQueries;
+
+// Smallest value over the wave, then one atomic (values are non-negative
+// floats, so their bits order like unsigned integers).
+__device__ inline void wave_min_out(uint32_t *out, float best, float above) {
+    float v = best;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v = fminf(v, __shfl_xor(v, o));
     }
-    uint32_t *stk = stk_all + threadIdx.x;
+    if (threadIdx.x == 0 && v < above) {
+        atomicMin(out, __float_as_uint(v));
+    }
+}
+
+// Per dynamic vertex: point-face (contact.cu:2110-2164), point-point
+// (contact.cu:2266-2316) and vertex vs static face (contact.cu:1956-1994).
+// The filters run on the query's lane while it walks; the sweeps run from the
+// wave queue.
+__global__ __launch_bounds__(kWQ) void k_ls_vertex(LsCtx c) {
+    __shared__ WaveScratch ws;
+    const uint32_t leaf = blockIdx.x * kWQ + threadIdx.x;
+    const bool active = leaf < c.vb.n;
     const ppf_contact_mesh &m = c.m;
     const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
-    const uint32_t i = c.vb.node[2 * (size_t)leaf] - 1;
-    const Box6 q = (m.vflag[i] & 4u)
-                       ? empty_box()
-                       : seg_box(X(m.x, i), XS(c, i), 0.5f * m.v_ghat[i] + m.v_off[i]);
-    const bool idyn = (m.vflag[i] & 1u) == 0, icol = (m.vflag[i] & 2u) != 0;
+    const uint32_t i = active ? c.vb.node[2 * (size_t)leaf] - 1 : 0;
+    const uint32_t vf = active ? m.vflag[i] : 4u;
+    const Box6 q = (vf & 4u) ? empty_box()
+                             : seg_box(X(m.x, i), XS(c, i), 0.5f * m.v_ghat[i] + m.v_off[i]);
+    const bool idyn = (vf & 1u) == 0, icol = (vf & 2u) != 0;
+    const uint32_t ibody = active ? m.vbody[i] : 0u;
     float best = max_t;
-    // point-face (contact.cu:2110-2164)
-    traverse(c.fb, q, stk, c.out + 3, [&](uint32_t fi) {
-        const uint32_t f[3] = {m.face[3 * (size_t)fi], m.face[3 * (size_t)fi + 1],
-                               m.face[3 * (size_t)fi + 2]};
-        const bool either = idyn || (m.fflag[fi] & 1u) == 0;
-        const bool same = m.vbody[i] != 0 && m.vbody[i] == m.vbody[f[0]];
-        const bool both = icol && (m.vflag[f[0]] & 2u) != 0;
-        if (!either || same || both) {
-            return false;
-        }
-        int dup = -1;
-        for (int k = 0; k < 3; ++k) {
-            if (f[k] == i) {
-                dup = k;
-                break;
+    traverse_wave(
+        c.fb, q, active, i, ws, c.out + 3,
+        [&](uint32_t fi) {
+            const uint32_t f0 = m.face[3 * (size_t)fi];
+            const bool either = idyn || (m.fflag[fi] & 1u) == 0;
+            const bool same = ibody != 0 && ibody == m.vbody[f0];
+            const bool both = icol && (m.vflag[f0] & 2u) != 0;
+            return either && !same && !both;
+        },
+        [&](uint32_t v, uint32_t fi) {
+            const uint32_t f[3] = {m.face[3 * (size_t)fi], m.face[3 * (size_t)fi + 1],
+                                   m.face[3 * (size_t)fi + 2]};
+            int dup = -1;
+            for (int k = 0; k < 3; ++k) {
+                if (f[k] == v) {
+                    dup = k;
+                    break;
+                }
             }
-        }
-        bool ovl = false;
-        float r;
-        if (dup < 0) {
-            const f3 a[4] = {X(m.x, i), X(m.x, f[0]), X(m.x, f[1]), X(m.x, f[2])};
-            const f3 b[4] = {X(c.lp.x1, i), X(c.lp.x1, f[0]), X(c.lp.x1, f[1]),
-                             X(c.lp.x1, f[2])};
-            r = ccd<4, 1>(a, b, m.v_off[i] + m.f_off[fi],
-                          0.5f * (m.v_ghat[i] + m.f_ghat[fi]), max_t, eps, ovl);
-        } else {
-            const uint32_t p = f[dup], j = f[(dup + 1) % 3], k = f[(dup + 2) % 3];
-            const f3 a[3] = {X(m.x, p), X(m.x, j), X(m.x, k)};
-            const f3 b[3] = {X(c.lp.x1, p), X(c.lp.x1, j), X(c.lp.x1, k)};
-            r = ccd<3, 1>(a, b, 2.0f * m.v_off[i], m.v_ghat[i], max_t, eps, ovl);
-        }
-        note(c, ovl, r, best);
-        return true;
-    });
-    // point-point (contact.cu:2266-2316)
-    traverse(c.vb, q, stk, c.out + 3, [&](uint32_t j) {
-        const bool either = idyn || (m.vflag[j] & 1u) == 0;
-        const bool same = m.vbody[i] != 0 && m.vbody[i] == m.vbody[j];
-        const bool both = icol && (m.vflag[j] & 2u) != 0;
-        if (!(j < i) || !either || same || both) {
-            return false;
-        }
-        const f3 a[2] = {X(m.x, i), X(m.x, j)};
-        const f3 b[2] = {X(c.lp.x1, i), X(c.lp.x1, j)};
-        bool ovl = false;
-        const float r = ccd<2, 1>(a, b, m.v_off[i] + m.v_off[j],
-                                  0.5f * (m.v_ghat[i] + m.v_ghat[j]), max_t, eps, ovl);
-        note(c, ovl, r, best);
-        return true;
-    });
-    // dynamic vertex vs static face (contact.cu:1956-1994)
-    if (c.sm.nf && idyn) {
-        traverse(c.sfb, q, stk, c.out + 3, [&](uint32_t fi) {
-            if (c.lp.vmass[i] == 0.0f) {
-                return false;
-            }
-            const uint32_t f0 = c.sm.face[3 * (size_t)fi], f1 = c.sm.face[3 * (size_t)fi + 1],
-                           f2 = c.sm.face[3 * (size_t)fi + 2];
-            const f3 t0 = X(c.sm.x, f0), t1 = X(c.sm.x, f1), t2 = X(c.sm.x, f2);
-            const f3 a[4] = {X(m.x, i), t0, t1, t2};
-            const f3 b[4] = {X(c.lp.x1, i), t0, t1, t2};
             bool ovl = false;
-            const float r = ccd<4, 1>(a, b, m.v_off[i] + c.sm.f_off[fi],
-                                      0.5f * (m.v_ghat[i] + c.sm.f_ghat[fi]), max_t,
-                                      eps, ovl);
+            float r;
+            if (dup < 0) {
+                const f3 a[4] = {X(m.x, v), X(m.x, f[0]), X(m.x, f[1]), X(m.x, f[2])};
+                const f3 b[4] = {X(c.lp.x1, v), X(c.lp.x1, f[0]), X(c.lp.x1, f[1]),
+                                 X(c.lp.x1, f[2])};
+                r = ccd<4, 1>(a, b, m.v_off[v] + m.f_off[fi],
+                              0.5f * (m.v_ghat[v] + m.f_ghat[fi]), max_t, eps, ovl);
+            } else {
+                const uint32_t p = f[dup], j = f[(dup + 1) % 3], k = f[(dup + 2) % 3];
+                const f3 a[3] = {X(m.x, p), X(m.x, j), X(m.x, k)};
+                const f3 b[3] = {X(c.lp.x1, p), X(c.lp.x1, j), X(c.lp.x1, k)};
+                r = ccd<3, 1>(a, b, 2.0f * m.v_off[v], m.v_ghat[v], max_t, eps, ovl);
+            }
             note(c, ovl, r, best);
-            return true;
         });
+    traverse_wave(
+        c.vb, q, active, i, ws, c.out + 3,
+        [&](uint32_t j) {
+            const bool either = idyn || (m.vflag[j] & 1u) == 0;
+            const bool same = ibody != 0 && ibody == m.vbody[j];
+            const bool both = icol && (m.vflag[j] & 2u) != 0;
+            return j < i && either && !same && !both;
+        },
+        [&](uint32_t v, uint32_t j) {
+            const f3 a[2] = {X(m.x, v), X(m.x, j)};
+            const f3 b[2] = {X(c.lp.x1, v), X(c.lp.x1, j)};
+            bool ovl = false;
+            const float r = ccd<2, 1>(a, b, m.v_off[v] + m.v_off[j],
+                                      0.5f * (m.v_ghat[v] + m.v_ghat[j]), max_t, eps, ovl);
+            note(c, ovl, r, best);
+        });
+    if (c.sm.nf) {
+        const bool go = active && idyn && c.lp.vmass[i] != 0.0f;
+        traverse_wave(
+            c.sfb, q, go, i, ws, c.out + 3, [&](uint32_t) { return true; },
+            [&](uint32_t v, uint32_t fi) {
+                const uint32_t f0 = c.sm.face[3 * (size_t)fi], f1 = c.sm.face[3 * (size_t)fi + 1],
+                               f2 = c.sm.face[3 * (size_t)fi + 2];
+                const f3 t0 = X(c.sm.x, f0), t1 = X(c.sm.x, f1), t2 = X(c.sm.x, f2);
+                const f3 a[4] = {X(m.x, v), t0, t1, t2};
+                const f3 b[4] = {X(c.lp.x1, v), t0, t1, t2};
+                bool ovl = false;
+                const float r = ccd<4, 1>(a, b, m.v_off[v] + c.sm.f_off[fi],
+                                          0.5f * (m.v_ghat[v] + c.sm.f_ghat[fi]), max_t, eps,
+                                          ovl);
+                note(c, ovl, r, best);
+            });
     }
-    if (best < max_t) {
-        atomicMin(c.out + 0, __float_as_uint(best));
-    }
+    wave_min_out(c.out + 0, best, max_t);
 }
 
 // static vertex vs dynamic face (contact.cu:1996-2034)
-__global__ __launch_bounds__(kQT) void k_ls_static_vertex(LsCtx c) {
-    __shared__ uint32_t stk_all[kStack * kQT];
-    const uint32_t i = blockIdx.x * kQT + threadIdx.x;
-    if (i >= c.sm.nv) {
-        return;
-    }
-    uint32_t *stk = stk_all + threadIdx.x;
+__global__ __launch_bounds__(kWQ) void k_ls_static_vertex(LsCtx c) {
+    __shared__ WaveScratch ws;
+    const uint32_t i = blockIdx.x * kWQ + threadIdx.x;
+    const bool active = i < c.sm.nv;
     const ppf_contact_mesh &m = c.m;
     const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
-    const f3 p = X(c.sm.x, i);
-    const float qm = 0.5f * c.sm.v_ghat[i] + c.sm.v_off[i];
+    const uint32_t ii = active ? i : 0;
+    const f3 p = X(c.sm.x, ii);
+    const float qm = 0.5f * c.sm.v_ghat[ii] + c.sm.v_off[ii];
     const Box6 q{{p.x - qm, p.y - qm, p.z - qm}, {p.x + qm, p.y + qm, p.z + qm}};
     float best = max_t;
-    traverse(c.fb, q, stk, c.out + 3, [&](uint32_t fi) {
-        if ((m.fflag[fi] & 1u) != 0 || !(c.lp.fmass[fi] > 0.0f)) {
-            return false;
-        }
-        const uint32_t f0 = m.face[3 * (size_t)fi], f1 = m.face[3 * (size_t)fi + 1],
-                       f2 = m.face[3 * (size_t)fi + 2];
-        const f3 a[4] = {p, X(m.x, f0), X(m.x, f1), X(m.x, f2)};
-        const f3 b[4] = {p, X(c.lp.x1, f0), X(c.lp.x1, f1), X(c.lp.x1, f2)};
-        bool ovl = false;
-        const float r = ccd<4, 1>(a, b, m.f_off[fi] + c.sm.v_off[i],
-                                  0.5f * (m.f_ghat[fi] + c.sm.v_ghat[i]), max_t, eps,
-                                  ovl);
-        note(c, ovl, r, best);
-        return true;
-    });
-    if (best < max_t) {
-        atomicMin(c.out + 0, __float_as_uint(best));
-    }
+    traverse_wave(
+        c.fb, q, active, ii, ws, c.out + 3,
+        [&](uint32_t fi) { return (m.fflag[fi] & 1u) == 0 && c.lp.fmass[fi] > 0.0f; },
+        [&](uint32_t v, uint32_t fi) {
+            const f3 pv = X(c.sm.x, v);
+            const uint32_t f0 = m.face[3 * (size_t)fi], f1 = m.face[3 * (size_t)fi + 1],
+                           f2 = m.face[3 * (size_t)fi + 2];
+            const f3 a[4] = {pv, X(m.x, f0), X(m.x, f1), X(m.x, f2)};
+            const f3 b[4] = {pv, X(c.lp.x1, f0), X(c.lp.x1, f1), X(c.lp.x1, f2)};
+            bool ovl = false;
+            const float r = ccd<4, 1>(a, b, m.f_off[fi] + c.sm.v_off[v],
+                                      0.5f * (m.f_ghat[fi] + c.sm.v_ghat[v]), max_t, eps,
+                                      ovl);
+            note(c, ovl, r, best);
+        });
+    wave_min_out(c.out + 0, best, max_t);
 }
 
 // edge-edge and edge vs static edge, seeded by the vertex-face minimum.
-__global__ __launch_bounds__(kQT) void k_ls_edge(LsCtx c) {
-    __shared__ uint32_t stk_all[kStack * kQT];
-    const uint32_t leaf = blockIdx.x * kQT + threadIdx.x;
-    if (leaf >= c.eb.n) {
-        return;
-    }
-    uint32_t *stk = stk_all + threadIdx.x;
+__global__ __launch_bounds__(kWQ) void k_ls_edge(LsCtx c) {
+    __shared__ WaveScratch ws;
+    const uint32_t leaf = blockIdx.x * kWQ + threadIdx.x;
+    const bool active = leaf < c.eb.n;
     const ppf_contact_mesh &m = c.m;
     const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
     const float seed = __uint_as_float(c.out[0]);
-    const uint32_t ei = c.eb.node[2 * (size_t)leaf] - 1;
+    const uint32_t ei = active ? c.eb.node[2 * (size_t)leaf] - 1 : 0;
     const uint32_t a0 = m.edge[2 * (size_t)ei], a1 = m.edge[2 * (size_t)ei + 1];
     const float ext = 0.5f * m.e_ghat[ei] + m.e_off[ei];
     const f3 p0 = X(m.x, a0), p1 = X(m.x, a1), q0 = X(c.lp.x1, a0), q1 = X(c.lp.x1, a1);
@@ -195,67 +201,75 @@ __global__ __launch_bounds__(kQT) void k_ls_edge(LsCtx c) {
         q.mn[d] = fminf(b0.mn[d], b1.mn[d]);
         q.mx[d] = fmaxf(b0.mx[d], b1.mx[d]);
     }
-    if (m.eflag[ei] & 4u) {
+    const uint32_t ef = active ? m.eflag[ei] : 4u;
+    if (ef & 4u) {
         q = empty_box();
     }
-    const bool edyn = (m.eflag[ei] & 1u) == 0;
+    const bool edyn = (ef & 1u) == 0;
     const bool acol = (m.vflag[a0] & 2u) != 0;
+    const uint32_t abody = m.vbody[a0];
     float best = seed;
-    traverse(c.eb, q, stk, c.out + 3, [&](uint32_t ej) {
-        const uint32_t b0i = m.edge[2 * (size_t)ej], b1i = m.edge[2 * (size_t)ej + 1];
-        const bool either = edyn || (m.eflag[ej] & 1u) == 0;
-        const bool same = m.vbody[a0] != 0 && m.vbody[a0] == m.vbody[b0i];
-        const bool both = acol && (m.vflag[b0i] & 2u) != 0;
-        if (!(ei < ej) || !either || same || both) {
-            return false;
-        }
-        const float off = m.e_off[ei] + m.e_off[ej];
-        const float gh = 0.5f * (m.e_ghat[ei] + m.e_ghat[ej]);
-        const uint32_t ea[2] = {a0, a1}, eb[2] = {b0i, b1i};
-        float r = max_t;
-        bool ovl = false;
-        const bool shared = a0 == b0i || a0 == b1i || a1 == b0i || a1 == b1i;
-        if (!shared) {
-            const f3 a[4] = {p0, p1, X(m.x, b0i), X(m.x, b1i)};
-            const f3 b[4] = {q0, q1, X(c.lp.x1, b0i), X(c.lp.x1, b1i)};
-            r = ccd<4, 2>(a, b, off, gh, max_t, eps, ovl);
-        } else {
-            const int ij[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
-            for (int k = 0; k < 4; ++k) {
-                const int ii = ij[k][0], jj = ij[k][1];
-                if (ea[ii] == eb[jj]) {
-                    const uint32_t i0 = ea[ii], i1 = ea[1 - ii], i2 = eb[1 - jj];
-                    const f3 x0a[3] = {X(m.x, i1), X(m.x, i0), X(m.x, i2)};
-                    const f3 x1a[3] = {X(c.lp.x1, i1), X(c.lp.x1, i0), X(c.lp.x1, i2)};
-                    const f3 x0b[3] = {X(m.x, i2), X(m.x, i0), X(m.x, i1)};
-                    const f3 x1b[3] = {X(c.lp.x1, i2), X(c.lp.x1, i0), X(c.lp.x1, i1)};
-                    const float r0 = ccd<3, 1>(x0a, x1a, off, gh, max_t, eps, ovl);
-                    const float r1 = ccd<3, 1>(x0b, x1b, off, gh, max_t, eps, ovl);
-                    r = fminf(r0, r1);
-                    break;
+    traverse_wave(
+        c.eb, q, active, ei, ws, c.out + 3,
+        [&](uint32_t ej) {
+            const uint32_t b0i = m.edge[2 * (size_t)ej];
+            const bool either = edyn || (m.eflag[ej] & 1u) == 0;
+            const bool same = abody != 0 && abody == m.vbody[b0i];
+            const bool both = acol && (m.vflag[b0i] & 2u) != 0;
+            return ei < ej && either && !same && !both;
+        },
+        [&](uint32_t e, uint32_t ej) {
+            const uint32_t e0 = m.edge[2 * (size_t)e], e1 = m.edge[2 * (size_t)e + 1];
+            const uint32_t b0i = m.edge[2 * (size_t)ej], b1i = m.edge[2 * (size_t)ej + 1];
+            const float off = m.e_off[e] + m.e_off[ej];
+            const float gh = 0.5f * (m.e_ghat[e] + m.e_ghat[ej]);
+            const uint32_t ea[2] = {e0, e1}, eb[2] = {b0i, b1i};
+            float r = max_t;
+            bool ovl = false;
+            const bool shared = e0 == b0i || e0 == b1i || e1 == b0i || e1 == b1i;
+            if (!shared) {
+                const f3 a[4] = {X(m.x, e0), X(m.x, e1), X(m.x, b0i), X(m.x, b1i)};
+                const f3 b[4] = {X(c.lp.x1, e0), X(c.lp.x1, e1), X(c.lp.x1, b0i),
+                                 X(c.lp.x1, b1i)};
+                r = ccd<4, 2>(a, b, off, gh, max_t, eps, ovl);
+            } else {
+                const int ij[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+                for (int k = 0; k < 4; ++k) {
+                    const int ii = ij[k][0], jj = ij[k][1];
+                    if (ea[ii] == eb[jj]) {
+                        const uint32_t i0 = ea[ii], i1 = ea[1 - ii], i2 = eb[1 - jj];
+                        const f3 x0a[3] = {X(m.x, i1), X(m.x, i0), X(m.x, i2)};
+                        const f3 x1a[3] = {X(c.lp.x1, i1), X(c.lp.x1, i0), X(c.lp.x1, i2)};
+                        const f3 x0b[3] = {X(m.x, i2), X(m.x, i0), X(m.x, i1)};
+                        const f3 x1b[3] = {X(c.lp.x1, i2), X(c.lp.x1, i0), X(c.lp.x1, i1)};
+                        const float r0 = ccd<3, 1>(x0a, x1a, off, gh, max_t, eps, ovl);
+                        const float r1 = ccd<3, 1>(x0b, x1b, off, gh, max_t, eps, ovl);
+                        r = fminf(r0, r1);
+                        break;
+                    }
                 }
             }
-        }
-        note(c, ovl, r, best);
-        return true;
-    });
-    if (c.sm.ne && edyn && c.lp.emass[ei] > 0.0f) {
-        traverse(c.seb, q, stk, c.out + 3, [&](uint32_t sj) {
-            const uint32_t s0i = c.sm.edge[2 * (size_t)sj], s1i = c.sm.edge[2 * (size_t)sj + 1];
-            const f3 w0 = X(c.sm.x, s0i), w1 = X(c.sm.x, s1i);
-            const f3 a[4] = {p0, p1, w0, w1};
-            const f3 b[4] = {q0, q1, w0, w1};
-            bool ovl = false;
-            const float r = ccd<4, 2>(a, b, m.e_off[ei] + c.sm.e_off[sj],
-                                      0.5f * (m.e_ghat[ei] + c.sm.e_ghat[sj]), max_t,
-                                      eps, ovl);
             note(c, ovl, r, best);
-            return true;
         });
+    if (c.sm.ne) {
+        const bool go = active && edyn && c.lp.emass[ei] > 0.0f;
+        traverse_wave(
+            c.seb, q, go, ei, ws, c.out + 3, [&](uint32_t) { return true; },
+            [&](uint32_t e, uint32_t sj) {
+                const uint32_t e0 = m.edge[2 * (size_t)e], e1 = m.edge[2 * (size_t)e + 1];
+                const uint32_t s0i = c.sm.edge[2 * (size_t)sj],
+                               s1i = c.sm.edge[2 * (size_t)sj + 1];
+                const f3 w0 = X(c.sm.x, s0i), w1 = X(c.sm.x, s1i);
+                const f3 a[4] = {X(m.x, e0), X(m.x, e1), w0, w1};
+                const f3 b[4] = {X(c.lp.x1, e0), X(c.lp.x1, e1), w0, w1};
+                bool ovl = false;
+                const float r = ccd<4, 2>(a, b, m.e_off[e] + c.sm.e_off[sj],
+                                          0.5f * (m.e_ghat[e] + c.sm.e_ghat[sj]), max_t,
+                                          eps, ovl);
+                note(c, ovl, r, best);
+            });
     }
-    if (best < seed) {
-        atomicMin(c.out + 1, __float_as_uint(best));
-    }
+    wave_min_out(c.out + 1, best, seed);
 }
 
 template <int C, int SPLIT>
@@ -372,15 +386,15 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
     c.seb = S.nv ? sedge_.view() : BvhView{};
     c.out = out_;
     if (m.nv) {
-        k_ls_vertex<<<cdiv(m.nv, kQT), kQT, 0, s>>>(c);
+        k_ls_vertex<<<cdiv(m.nv, kWQ), kWQ, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     if (S.nv && m.nf) {
-        k_ls_static_vertex<<<cdiv(S.nv, kQT), kQT, 0, s>>>(c);
+        k_ls_static_vertex<<<cdiv(S.nv, kWQ), kWQ, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     if (m.ne) {
-        k_ls_edge<<<cdiv(m.ne, kQT), kQT, 0, s>>>(c);
+        k_ls_edge<<<cdiv(m.ne, kWQ), kWQ, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     PPF_HIP(hipMemcpyAsync(h_out_, out_, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));

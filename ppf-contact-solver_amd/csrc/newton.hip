@@ -492,8 +492,9 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
             rc = lpc_->forward(s);
         }
         if (rc == PPF_OK) {
+            const PcgFactor fac = lpc_->factor();
             rc = pcg_solve(plan_, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr,
-                           &pr, s, lpc_->Gd16(), lpc_->Gl16());
+                           &pr, s, &fac);
         }
         solved = rc == PPF_OK && (pr.status == 0 || pr.status == 1);
         if (!solved) {

@@ -81,7 +81,7 @@ struct Args {
     // diagonal blocks of G (n), Gl[s] = G_ji for every stored slot s = (i, j),
     // i < j (the lower block mirrored by the slot), both bf16 column-major in
     // 10-entry (20 B) records; gy holds G r.
-    const uint16_t *Gd, *Gl;
+    const uint16_t *Gd, *Gl, *Gr; // Gr: the rows of G in order (row j at j + tp[j])
     float *gy;
 };
 
@@ -766,52 +766,74 @@ __device__ inline void load_g16(gptr<const uint16_t> g, size_t blk, float *m) {
     }
 }
 
-// gy = G r, 4 lanes per row: row j's entries are its diagonal block, then
-// its mirrored entries (slot s = (i, j), i < j: G_ji = Gl[s] times r_i).
+// The factor passes share the SpMV's shape: P lanes per row, lane q taking
+// entries q, q + P, ... of the row, M of them per pass with every load of the
+// pass issued before any is used (one dependent round trip per pass), the
+// lanes' sums combined by shuffles.
+constexpr uint32_t kFacP = 2, kFacM = 4;
+
+// gy = G r: row j's entries are G_jj, then G_ji for its mirrored entries
+// (slot s = (i, j), i < j), read in order from the row-ordered copy Gr.
 __global__ __launch_bounds__(kNT) void k_fac_g(const Args *__restrict__ A, int init) {
+    constexpr uint32_t P = kFacP, M = kFacM;
     if (!init && as_global(A->s)->done) {
         return;
     }
     const uint32_t n = A->n;
-    const uint32_t q = threadIdx.x & 3u;
-    const uint32_t j = blockIdx.x * (kNT / 4) + (threadIdx.x >> 2);
+    const uint32_t q = threadIdx.x % P;
+    const uint32_t j = blockIdx.x * (kNT / P) + threadIdx.x / P;
     const bool valid = j < n;
     const uint32_t jj = valid ? j : 0;
-    gptr<const uint16_t> Gd = as_global(A->Gd), Gl = as_global(A->Gl);
+    gptr<const uint16_t> Gr = as_global(A->Gr);
     gptr<const float> r = as_global(A->r);
-    gptr<const uint32_t> tp = as_global(A->tp), tcol = as_global(A->tcol),
-                         tslot = as_global(A->tslot);
+    gptr<const uint32_t> tp = as_global(A->tp), tcol = as_global(A->tcol);
     const uint32_t t0 = tp[jj], cnt = 1 + tp[jj + 1] - t0;
+    const size_t g0 = (size_t)jj + t0;
     f3 y = mk3(0, 0, 0);
-    for (uint32_t k = q; k < cnt; k += 4) {
-        float m[9];
-        uint32_t src;
-        if (k == 0) {
-            load_g16(Gd, jj, m);
-            src = jj;
-        } else {
-            load_g16(Gl, tslot[t0 + k - 1], m);
-            src = tcol[t0 + k - 1];
+    for (uint32_t k0 = 0; k0 < cnt; k0 += P * M) {
+        uint32_t src[M];
+        bool ok[M];
+#pragma unroll
+        for (uint32_t m = 0; m < M; ++m) {
+            const uint32_t k = k0 + q + P * m;
+            ok[m] = k < cnt;
+            src[m] = (k == 0 || !ok[m]) ? jj : tcol[t0 + k - 1];
         }
-        const f3 v = load3(r + 3 * (size_t)src);
-        y = y + mk3(m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
-                    m[2] * v.x + m[5] * v.y + m[8] * v.z);
+        float g[M][9];
+        f3 v[M];
+#pragma unroll
+        for (uint32_t m = 0; m < M; ++m) {
+            const uint32_t k = k0 + q + P * m;
+            load_g16(Gr, ok[m] ? g0 + k : g0, g[m]);
+            v[m] = load3(r + 3 * (size_t)src[m]);
+        }
+#pragma unroll
+        for (uint32_t m = 0; m < M; ++m) {
+            if (ok[m]) {
+                const float *b = g[m];
+                const f3 w = v[m];
+                y = y + mk3(b[0] * w.x + b[3] * w.y + b[6] * w.z,
+                            b[1] * w.x + b[4] * w.y + b[7] * w.z,
+                            b[2] * w.x + b[5] * w.y + b[8] * w.z);
+            }
+        }
     }
 #pragma unroll
-    for (int off = 2; off > 0; off >>= 1) {
-        y.x += __shfl_xor(y.x, off, 4);
-        y.y += __shfl_xor(y.y, off, 4);
-        y.z += __shfl_xor(y.z, off, 4);
+    for (uint32_t off = P / 2; off > 0; off >>= 1) {
+        y.x += __shfl_xor(y.x, off, P);
+        y.y += __shfl_xor(y.y, off, P);
+        y.z += __shfl_xor(y.z, off, P);
     }
     if (valid && q == 0) {
         store3(as_global(A->gy) + 3 * (size_t)j, y);
     }
 }
 
-// z = G^T gy, 4 lanes per row: row i's entries are its diagonal block
-// (transposed) and its stored off-diagonal slots s = (i, j): G_ji^T gy_j.
-// Then the r.z partials (and p = z when init).
+// z = G^T gy: row i's entries are G_ii (transposed) and, for its stored
+// off-diagonal slots s = (i, j), G_ji^T gy_j (the stored diagonal slot
+// carries no factor block). Then the r.z partials (and p = z when init).
 __global__ __launch_bounds__(kNT) void k_fac_gt(const Args *__restrict__ A, int init) {
+    constexpr uint32_t P = kFacP, M = kFacM, kRows = kNT / kFacP;
     __shared__ float lds[kNT / kWave];
     if (!init && as_global(A->s)->done) {
         return;
@@ -819,42 +841,67 @@ __global__ __launch_bounds__(kNT) void k_fac_gt(const Args *__restrict__ A, int 
     const uint32_t n = A->n;
     const uint32_t L = blockIdx.x;
     const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
-    const uint32_t q = threadIdx.x & 3u, sub = threadIdx.x >> 2;
+    const uint32_t q = threadIdx.x % P, sub = threadIdx.x / P;
     gptr<const uint16_t> Gd = as_global(A->Gd), Gl = as_global(A->Gl);
     gptr<const float> gy = as_global(A->gy), r = as_global(A->r);
     gptr<const uint32_t> rp = as_global(A->rp), col = as_global(A->col);
     gptr<float> z = as_global(A->z), p = as_global(A->p);
     float rz = 0.0f;
-    for (uint32_t base = lo; base < hi; base += kNT / 4) {
+    // row bounds of the next chunk are loaded while this chunk is in flight
+    uint32_t nb[2] = {0u, 0u};
+    if (lo < hi) {
+        const uint32_t i0 = lo + sub < hi ? lo + sub : lo;
+        nb[0] = rp[i0], nb[1] = rp[i0 + 1];
+    }
+    for (uint32_t base = lo; base < hi; base += kRows) {
         const uint32_t i = base + sub;
         const bool valid = i < hi;
         const uint32_t ii = valid ? i : lo;
-        const uint32_t r0 = rp[ii], cnt = 1 + rp[ii + 1] - r0;
+        const uint32_t r0 = nb[0], cnt = 1 + nb[1] - r0;
+        if (base + kRows < hi) {
+            const uint32_t in = base + kRows + sub;
+            const uint32_t iin = in < hi ? in : lo;
+            nb[0] = rp[iin], nb[1] = rp[iin + 1];
+        }
         f3 zi = mk3(0, 0, 0);
-        for (uint32_t k = q; k < cnt; k += 4) {
-            float m[9];
-            uint32_t src;
-            if (k == 0) {
-                load_g16(Gd, ii, m);
-                src = ii;
-            } else {
-                const uint32_t s = r0 + k - 1;
-                src = col[s];
-                if (src == ii) {
-                    continue; // the stored diagonal slot carries no factor block
-                }
-                load_g16(Gl, s, m);
+        for (uint32_t k0 = 0; k0 < cnt; k0 += P * M) {
+            uint32_t src[M];
+            bool ok[M];
+#pragma unroll
+            for (uint32_t m = 0; m < M; ++m) {
+                const uint32_t k = k0 + q + P * m;
+                ok[m] = k < cnt;
+                src[m] = (k == 0 || !ok[m]) ? ii : col[r0 + k - 1];
+                ok[m] = ok[m] && (k == 0 || src[m] != ii);
             }
-            const f3 v = load3(gy + 3 * (size_t)src);
-            zi = zi + mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z,
-                          m[3] * v.x + m[4] * v.y + m[5] * v.z,
-                          m[6] * v.x + m[7] * v.y + m[8] * v.z);
+            float g[M][9];
+            f3 v[M];
+#pragma unroll
+            for (uint32_t m = 0; m < M; ++m) {
+                const uint32_t k = k0 + q + P * m;
+                if (k == 0 || !ok[m]) {
+                    load_g16(Gd, ii, g[m]);
+                } else {
+                    load_g16(Gl, r0 + k - 1, g[m]);
+                }
+                v[m] = load3(gy + 3 * (size_t)src[m]);
+            }
+#pragma unroll
+            for (uint32_t m = 0; m < M; ++m) {
+                if (ok[m]) {
+                    const float *b = g[m];
+                    const f3 w = v[m];
+                    zi = zi + mk3(b[0] * w.x + b[1] * w.y + b[2] * w.z,
+                                  b[3] * w.x + b[4] * w.y + b[5] * w.z,
+                                  b[6] * w.x + b[7] * w.y + b[8] * w.z);
+                }
+            }
         }
 #pragma unroll
-        for (int off = 2; off > 0; off >>= 1) {
-            zi.x += __shfl_xor(zi.x, off, 4);
-            zi.y += __shfl_xor(zi.y, off, 4);
-            zi.z += __shfl_xor(zi.z, off, 4);
+        for (uint32_t off = P / 2; off > 0; off >>= 1) {
+            zi.x += __shfl_xor(zi.x, off, P);
+            zi.y += __shfl_xor(zi.y, off, P);
+            zi.z += __shfl_xor(zi.z, off, P);
         }
         if (valid && q == 0) {
             store3(z + 3 * (size_t)i, zi);
@@ -941,6 +988,20 @@ __global__ __launch_bounds__(kNT) void k_pupdate(const Args *__restrict__ A,
 }
 
 uint32_t round8(uint32_t v) { return (v + 7u) & ~7u; }
+
+// Workgroups of kNT threads of `kernel` the current device holds at once
+// (CUs x resident workgroups per CU), or `fallback` if the runtime cannot say.
+template <class K> uint32_t resident_blocks(K kernel, uint32_t fallback) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kNT, 0) != hipSuccess ||
+        cus <= 0 || per_cu <= 0) {
+        return fallback;
+    }
+    return (uint32_t)cus * (uint32_t)per_cu;
+}
 uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 } // namespace
@@ -1017,7 +1078,7 @@ hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
     }
     if (h.Gd) {
         k_update<true><<<h.G2, kNT, 0, st>>>(pl.d_args, k);
-        k_fac_g<<<cdiv(h.n, kNT / 4), kNT, 0, st>>>(pl.d_args, 0);
+        k_fac_g<<<cdiv(h.n, kNT / kFacP), kNT, 0, st>>>(pl.d_args, 0);
         k_fac_gt<<<h.G2, kNT, 0, st>>>(pl.d_args, 0);
     } else {
         k_update<false><<<h.G2, kNT, 0, st>>>(pl.d_args, k);
@@ -1184,13 +1245,17 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     pl->n = n;
     Args &h = pl->h;
     h.n = n;
-    // Row ranges are multiples of 64 rows: one pass of either SpMV form.
-    h.G1 = round8(std::min<uint32_t>(2048u, cdiv(n, 64)));
+    // One round of resident workgroups per kernel (a second, partly filled
+    // round costs as much as the first): the SpMV's register use allows 5
+    // workgroups per CU, the streaming kernels 8. Row ranges are multiples of
+    // 64 rows: one pass of the SpMV, one wave of the streaming kernels.
+    h.G1 = round8(std::min(resident_blocks(k_spmv_split<true, 4>, 1280u), cdiv(n, 64)));
     h.R1 = cdiv(cdiv(n, h.G1), 64) * 64;
-    h.G2 = std::min<uint32_t>(1024u, cdiv(n, kNT));
-    h.R2 = cdiv(cdiv(n, h.G2), kNT) * kNT;
-    h.G3 = std::min<uint32_t>(1024u, cdiv(3 * n, kNT));
-    h.R3 = cdiv(cdiv(3 * n, h.G3), kNT) * kNT;
+    const uint32_t g2 = resident_blocks(k_update<false>, 2048u);
+    h.G2 = std::min(g2, cdiv(n, 64));
+    h.R2 = cdiv(cdiv(n, h.G2), 64) * 64;
+    h.G3 = std::min(g2, cdiv(3 * n, 64));
+    h.R3 = cdiv(cdiv(3 * n, h.G3), 64) * 64;
 
     const size_t n3 = 3 * (size_t)n;
     const size_t scan_words = scan_scratch_words(n + 1);
@@ -1340,9 +1405,9 @@ int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
 
 int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
               const float *b, float *x, float tol, uint32_t max_iter,
-              float *err_hist, ppf_pcg_result *res, hipStream_t st, const uint16_t *Gd,
-              const uint16_t *Gl) {
-    if (!pl || !val || !cdiag || !b || !x || !res || max_iter == 0 || (!Gd != !Gl)) {
+              float *err_hist, ppf_pcg_result *res, hipStream_t st, const PcgFactor *fac) {
+    if (!pl || !val || !cdiag || !b || !x || !res || max_iter == 0 ||
+        (fac && (!fac->Gd || !fac->Gl || !fac->Gr))) {
         set_error("pcg_solve: invalid argument");
         return PPF_ERR_ARG;
     }
@@ -1352,8 +1417,10 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
     h.b = b;
     h.x = x;
     h.err_hist = err_hist;
-    h.Gd = Gd;
-    h.Gl = Gl;
+    h.Gd = fac ? fac->Gd : nullptr;
+    h.Gl = fac ? fac->Gl : nullptr;
+    h.Gr = fac ? fac->Gr : nullptr;
+    const bool Gd = fac != nullptr;
     if (spmv_key(*pl) != pl->graph_key) {
         // the iteration graph holds the preconditioner's kernels
         int rc = capture_graph(pl);
@@ -1375,7 +1442,7 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
     PPF_HIP(launch_spmv<false>(*pl, x, h.Ap, st));
     if (Gd) {
         k_init_residual<true><<<h.G2, kNT, 0, st>>>(pl->d_args);
-        k_fac_g<<<cdiv(pl->n, kNT / 4), kNT, 0, st>>>(pl->d_args, 1);
+        k_fac_g<<<cdiv(pl->n, kNT / kFacP), kNT, 0, st>>>(pl->d_args, 1);
         k_fac_gt<<<h.G2, kNT, 0, st>>>(pl->d_args, 1);
     } else {
         k_init_residual<false><<<h.G2, kNT, 0, st>>>(pl->d_args);

@@ -14,14 +14,18 @@ void pcg_plan_destroy(PcgPlanImpl *pl);
 // iteration graph is re-captured only when the SpMV form changes).
 int pcg_plan_repattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
                        hipStream_t st);
-// Gd / Gl (both or neither): the factor preconditioner M^-1 = G^T G with
-// diagonal blocks Gd (n) and the lower blocks by slot (Gl[s] = G_ji for the
-// stored slot s = (i, j), i < j), bf16 column-major in 10-entry records;
-// null: block Jacobi.
+// The factor preconditioner M^-1 = G^T G (null: block Jacobi), bf16
+// column-major blocks in 10-entry records: Gd the diagonal blocks (n), Gl the
+// lower blocks by slot (Gl[s] = G_ji for the stored slot s = (i, j), i < j),
+// Gr the rows of G in order (row j at j + tp[j]: G_jj, then G_ji for the
+// row's mirrored entries in transpose-table order).
+struct PcgFactor {
+    const uint16_t *Gd, *Gl, *Gr;
+};
 int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
               const float *b, float *x, float tol, uint32_t max_iter,
               float *err_hist, ppf_pcg_result *res, hipStream_t st,
-              const uint16_t *Gd = nullptr, const uint16_t *Gl = nullptr);
+              const PcgFactor *fac = nullptr);
 // Pattern arrays of the plan (device): row offsets, columns, transpose table
 // (per row the mirrored entries: source row and slot), diagonal slot per row.
 struct PcgPattern {
