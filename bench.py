@@ -322,7 +322,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precond", choices=("bj", "both"), default="both",
                     help="both: also train and time the learned preconditioner")
-    ap.add_argument("--lpc-steps", type=int, default=400)
+    ap.add_argument("--lpc-steps", type=int, default=1600)
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a 256 MiB device copy (PMC calibration)")
     args = ap.parse_args()
