@@ -140,6 +140,38 @@ int ppf_pcg_solve_hist(ppf_pcg_plan *plan, const float *d_value,
 int ppf_pcg_spmv(ppf_pcg_plan *plan, const float *d_value, const float *d_cdiag,
                  const float *d_x, float *d_y, void *stream);
 
+/* Batched PCG: one plan over many independent systems ("scenes") stacked
+ * block-diagonally: scene c owns rows h_scene_row[c] .. h_scene_row[c + 1]
+ * (host array of nscene + 1 rising row indices, 0 .. n) and its blocks couple
+ * only its own rows (checked). Each kernel launch serves every scene; each
+ * scene keeps its own alpha, beta, stopping rule and result, so a scene's
+ * iterates do not depend on the others'. Work is cut into tiles of whole
+ * rows of one scene. No reference counterpart: the reference solves one
+ * scene per process (solver::solve, solver.cu:2180-2321, once per scene). */
+int ppf_pcg_plan_create_batch(uint32_t n, uint32_t nnz, const uint32_t *d_row_offset,
+                              const uint32_t *d_col_index, uint32_t nscene,
+                              const uint32_t *h_scene_row, void *stream, ppf_pcg_plan **out);
+
+/* Rows per tile of the SpMV (spmv_rows) and of the vector kernels
+ * (vector_rows). 0, 0 on a one-scene plan: one resident round of workgroups
+ * over the whole system (the default). */
+int ppf_pcg_plan_set_tiles(ppf_pcg_plan *plan, uint32_t spmv_rows, uint32_t vector_rows,
+                           void *stream);
+
+/* Autotuner: time the same capped solve (24 iterations) of this system
+ * under candidate tile sizes (SpMV 128..2048 rows, vector kernels 256..2048)
+ * and keep the fastest. h_tile_rows[2] receives the chosen sizes and *h_ms
+ * (may be null) the chosen candidate's time. */
+int ppf_pcg_plan_autotune(ppf_pcg_plan *plan, const float *d_value, const float *d_cdiag,
+                          const float *d_b, float tol, uint32_t *h_tile_rows, float *h_ms,
+                          void *stream);
+
+/* Solve every scene of a batch plan: h_out receives nscene results. The
+ * stopping rule and statuses are ppf_pcg_solve's, per scene. */
+int ppf_pcg_solve_batch(ppf_pcg_plan *plan, const float *d_value, const float *d_cdiag,
+                        const float *d_b, float *d_x, float tol, uint32_t max_iter,
+                        ppf_pcg_result *h_out, void *stream);
+
 /* Time `reps` back-to-back launches of the PCG's SpMV kernel (y = M x with
  * the x.Mx partials, the kernel of one CG iteration) with HIP events on
  * `stream`; *avg_ms receives the mean duration of one launch. Measurement

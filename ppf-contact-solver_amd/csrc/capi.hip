@@ -254,11 +254,76 @@ int ppf_pcg_solve_lpc(ppf_pcg_plan *plan, ppf_lpc *h, const float *d_value,
                      (hipStream_t)stream, &fac);
 }
 
+namespace {
+
+// The one-result entry points take one-scene plans only.
+bool one_scene(const ppf_pcg_plan *plan, const char *who) {
+    if (!plan) {
+        set_error("%s: plan is null", who);
+        return false;
+    }
+    if (pcg_plan_scenes(plan->impl) != 1) {
+        set_error("%s: the plan holds %u scenes (ppf_pcg_solve_batch solves them)", who,
+                  pcg_plan_scenes(plan->impl));
+        return false;
+    }
+    return true;
+}
+
+} // namespace
+
+int ppf_pcg_plan_create_batch(uint32_t n, uint32_t nnz, const uint32_t *d_row_offset,
+                              const uint32_t *d_col_index, uint32_t nscene,
+                              const uint32_t *h_scene_row, void *stream, ppf_pcg_plan **out) {
+    if (!out || !h_scene_row || nscene == 0) {
+        set_error("ppf_pcg_plan_create_batch: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    PcgPlanImpl *impl = nullptr;
+    int rc = pcg_plan_create_scenes(n, nnz, d_row_offset, d_col_index, nscene, h_scene_row,
+                                    (hipStream_t)stream, &impl);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    *out = new ppf_pcg_plan{impl};
+    return PPF_OK;
+}
+
+int ppf_pcg_plan_set_tiles(ppf_pcg_plan *plan, uint32_t spmv_rows, uint32_t vector_rows,
+                           void *stream) {
+    if (!plan) {
+        set_error("ppf_pcg_plan_set_tiles: plan is null");
+        return PPF_ERR_ARG;
+    }
+    return pcg_plan_set_tiles(plan->impl, spmv_rows, vector_rows, (hipStream_t)stream);
+}
+
+int ppf_pcg_plan_autotune(ppf_pcg_plan *plan, const float *d_value, const float *d_cdiag,
+                          const float *d_b, float tol, uint32_t *h_tile_rows, float *h_ms,
+                          void *stream) {
+    if (!plan || !h_tile_rows) {
+        set_error("ppf_pcg_plan_autotune: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    return pcg_plan_autotune(plan->impl, d_value, d_cdiag, d_b, tol, h_tile_rows,
+                             h_tile_rows + 1, h_ms, (hipStream_t)stream);
+}
+
+int ppf_pcg_solve_batch(ppf_pcg_plan *plan, const float *d_value, const float *d_cdiag,
+                        const float *d_b, float *d_x, float tol, uint32_t max_iter,
+                        ppf_pcg_result *h_out, void *stream) {
+    if (!plan) {
+        set_error("ppf_pcg_solve_batch: plan is null");
+        return PPF_ERR_ARG;
+    }
+    return pcg_solve(plan->impl, d_value, d_cdiag, d_b, d_x, tol, max_iter, nullptr, h_out,
+                     (hipStream_t)stream);
+}
+
 int ppf_pcg_solve(ppf_pcg_plan *plan, const float *d_value,
                   const float *d_cdiag, const float *d_b, float *d_x, float tol,
                   uint32_t max_iter, ppf_pcg_result *out, void *stream) {
-    if (!plan) {
-        set_error("ppf_pcg_solve: plan is null");
+    if (!one_scene(plan, "ppf_pcg_solve")) {
         return PPF_ERR_ARG;
     }
     return pcg_solve(plan->impl, d_value, d_cdiag, d_b, d_x, tol, max_iter,
@@ -269,8 +334,7 @@ int ppf_pcg_solve_hist(ppf_pcg_plan *plan, const float *d_value,
                        const float *d_cdiag, const float *d_b, float *d_x,
                        float tol, uint32_t max_iter, float *d_err_hist,
                        ppf_pcg_result *out, void *stream) {
-    if (!plan) {
-        set_error("ppf_pcg_solve_hist: plan is null");
+    if (!one_scene(plan, "ppf_pcg_solve_hist")) {
         return PPF_ERR_ARG;
     }
     return pcg_solve(plan->impl, d_value, d_cdiag, d_b, d_x, tol, max_iter,

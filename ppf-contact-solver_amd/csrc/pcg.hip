@@ -27,17 +27,17 @@
 #include "eig3.hpp"
 #include "pcg.hpp"
 #include "scan.hpp"
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <vector>
 
 namespace ppf {
 
 namespace {
 
 constexpr int kNT = 256;           // threads per workgroup
-constexpr int kLanes = 8;          // lanes per row in the SpMV
-constexpr int kRowsPerPass = kNT / kLanes;
 constexpr int kGraphIters = 8;     // iterations per captured graph (even)
 constexpr unsigned kNone = 0xffffffffu;
 constexpr unsigned kMirror = 0x80000000u; // entry slot flag: apply transposed
@@ -65,18 +65,27 @@ struct Scalars {
     uint32_t bad;    // non-invertible diagonal blocks
 };
 
+// Work of one workgroup: rows [lo, hi) of one scene (a plain solve is one
+// scene); `first` marks the scene's first tile, whose thread 0 keeps the
+// scene's bookkeeping.
+struct Tile {
+    uint32_t lo, hi, scene, first;
+};
+
 struct Args {
     uint32_t n;
-    uint32_t G1, G2, G3; // workgroups of the SpMV / update / p kernels
-    uint32_t R1, R2, R3; // contiguous rows (elements for G3) per workgroup
+    uint32_t G1, G2;     // tiles (workgroups) of the SpMV / of the vector kernels
+    uint32_t nscene;
+    const Tile *tile1, *tile2; // [G1], [G2]; scene ranges contiguous
+    const uint32_t *sc1, *sc2; // per scene its first tile1 / tile2 ([nscene + 1])
     const uint32_t *rp, *col, *tp, *tcol, *tslot, *dslot;
     const uint2 *ent;  // fixed-width entry table (ELL form), or null
     uint32_t ell_m;    // entries per lane of the ELL kernel (row width 4*m)
     const float *val, *cdiag, *b;
     float *x, *P, *r, *z, *p, *Ap;
-    float *part; // [G1 pAp][G1 |terms|][G2 rz][G2 err]
-    float *err_hist;
-    Scalars *s;
+    float *part; // [G1 pAp][G1 |terms|][G2 rz][G2 err], one per tile
+    float *err_hist; // (one-scene solves)
+    Scalars *s;      // [nscene]
     // Factor preconditioner M^-1 = G^T G (null: block Jacobi): Gd the
     // diagonal blocks of G (n), Gl[s] = G_ji for every stored slot s = (i, j),
     // i < j (the lower block mirrored by the slot), both bf16 column-major in
@@ -84,6 +93,11 @@ struct Args {
     const uint16_t *Gd, *Gl, *Gr; // Gr: the rows of G in order (row j at j + tp[j])
     float *gy;
 };
+
+__device__ inline Tile load_tile(const Tile *t, uint32_t L) {
+    const uint4 v = reinterpret_cast<const uint4 *>(t)[L];
+    return Tile{v.x, v.y, v.z, v.w};
+}
 
 // ---- pattern preparation (once per sparsity pattern) ----------------------
 
@@ -184,123 +198,26 @@ __global__ void k_sort_transpose(uint32_t n, const uint32_t *__restrict__ tp,
 
 // D_i = B_ii + C_i and P_i = D_i^-1 (solver.cu:2191-2195).
 __global__ __launch_bounds__(kNT) void k_diag_invert(const Args *__restrict__ A) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A->n) {
-        return;
-    }
+    const Tile T = load_tile(A->tile2, blockIdx.x);
     gptr<const float> val = as_global(A->val);
     gptr<const float> cd = as_global(A->cdiag);
     gptr<float> P = as_global(A->P);
-    const uint32_t ds = as_global(A->dslot)[i];
-    float d[9];
+    for (uint32_t i = T.lo + threadIdx.x; i < T.hi; i += kNT) {
+        const uint32_t ds = as_global(A->dslot)[i];
+        float d[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) {
-        const float bii = ds != kNone ? val[9 * (size_t)ds + e] : 0.0f;
-        d[e] = bii + cd[9 * (size_t)i + e];
-    }
-    float inv[9];
-    const bool ok = bj_invert(d, inv);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) {
-        P[9 * (size_t)i + e] = inv[e];
-    }
-    if (!ok) {
-        atomicAdd(&A->s->bad, 1u);
-    }
-}
-
-// y = M v (+ per-workgroup partials of v.Mv and sum|terms| when DOT), the
-// reference's walk form (solver.cu:812-886): stored blocks of row i (diagonal
-// included), mirrored blocks (j, i), j < i, applied transposed, then C_i.
-// 8 lanes per row: lanes 0-3 walk the stored blocks and lanes 4-7 the mirror
-// list, so both dependent load chains are in flight together; lane 7 adds
-// C_i v_i. Workgroups own contiguous, XCD-grouped row ranges.
-template <bool DOT>
-__global__ __launch_bounds__(kNT) void k_spmv(const Args *__restrict__ A,
-                                              const float *__restrict__ vin,
-                                              float *__restrict__ yout) {
-    __shared__ float lds[kNT / kWave];
-    if (DOT && as_global(A->s)->done) {
-        return;
-    }
-    const uint32_t n = A->n;
-    const uint32_t L = xcd_logical_block();
-    const uint32_t lane = threadIdx.x % kLanes, sub = threadIdx.x / kLanes;
-    const bool mirror = lane >= 4;
-    const uint32_t q = lane & 3u;
-    gptr<const uint32_t> offs = as_global(mirror ? A->tp : A->rp);
-    gptr<const uint32_t> cols = as_global(mirror ? A->tcol : A->col);
-    gptr<const uint32_t> tslot = as_global(A->tslot);
-    gptr<const float> val = as_global(A->val);
-    gptr<const float> cd = as_global(A->cdiag);
-    gptr<const float> v = as_global(vin);
-    gptr<float> y = as_global(yout);
-    const uint32_t lo = L * A->R1;
-    const uint32_t hi = min(n, lo + A->R1);
-    float dot = 0.0f, adot = 0.0f;
-    for (uint32_t base = lo; base < hi; base += kRowsPerPass) {
-        const uint32_t i = base + sub;
-        f3 sum = mk3(0, 0, 0), as = mk3(0, 0, 0);
-        const bool valid = i < hi;
-        f3 vi = mk3(0, 0, 0);
-        if (valid) {
-            vi = load3(v + 3 * (size_t)i);
-            const uint32_t k1 = offs[i + 1];
-            for (uint32_t k = offs[i] + q; k < k1; k += 4) {
-                const uint32_t j = cols[k];
-                const uint32_t s = mirror ? tslot[k] : k;
-                gptr<const float> a = val + 9 * (size_t)s;
-                float m[9];
-#pragma unroll
-                for (int e = 0; e < 9; ++e) {
-                    m[e] = a[e];
-                }
-                if (mirror) { // apply the stored block transposed
-                    float t;
-                    t = m[1], m[1] = m[3], m[3] = t;
-                    t = m[2], m[2] = m[6], m[6] = t;
-                    t = m[5], m[5] = m[7], m[7] = t;
-                }
-                const f3 t = mv(m, load3(v + 3 * (size_t)j));
-                sum = sum + t;
-                as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
-            }
-            if (lane == kLanes - 1) {
-                const f3 t = mv(cd + 9 * (size_t)i, vi);
-                sum = sum + t;
-                as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
-            }
+        for (int e = 0; e < 9; ++e) {
+            const float bii = ds != kNone ? val[9 * (size_t)ds + e] : 0.0f;
+            d[e] = bii + cd[9 * (size_t)i + e];
         }
+        float inv[9];
+        const bool ok = bj_invert(d, inv);
 #pragma unroll
-        for (int off = kLanes / 2; off > 0; off >>= 1) {
-            sum.x += __shfl_xor(sum.x, off, kLanes);
-            sum.y += __shfl_xor(sum.y, off, kLanes);
-            sum.z += __shfl_xor(sum.z, off, kLanes);
-            if (DOT) {
-                as.x += __shfl_xor(as.x, off, kLanes);
-                as.y += __shfl_xor(as.y, off, kLanes);
-                as.z += __shfl_xor(as.z, off, kLanes);
-            }
+        for (int e = 0; e < 9; ++e) {
+            P[9 * (size_t)i + e] = inv[e];
         }
-        if (valid && lane == 0) {
-            store3(y + 3 * (size_t)i, sum);
-            if (DOT) {
-                dot = dot + vi.x * sum.x;
-                dot = dot + vi.y * sum.y;
-                dot = dot + vi.z * sum.z;
-                adot = adot + fabsf(vi.x) * as.x;
-                adot = adot + fabsf(vi.y) * as.y;
-                adot = adot + fabsf(vi.z) * as.z;
-            }
-        }
-    }
-    if (DOT) {
-        dot = block_sum<kNT>(dot, lds);
-        adot = block_sum<kNT>(adot, lds);
-        if (threadIdx.x == 0) {
-            gptr<float> part = as_global(A->part);
-            part[L] = dot;
-            part[A->G1 + L] = adot;
+        if (!ok) {
+            atomicAdd(&A->s[T.scene].bad, 1u);
         }
     }
 }
@@ -354,11 +271,11 @@ __global__ __launch_bounds__(kNT) void k_spmv_split(const Args *__restrict__ A,
                                                     float *__restrict__ yout) {
     constexpr uint32_t kRows = kNT / 4;
     __shared__ float lds[kNT / kWave];
-    if (DOT && as_global(A->s)->done) {
+    const uint32_t L = xcd_logical_block();
+    const Tile T = load_tile(A->tile1, L);
+    if (DOT && as_global(A->s + T.scene)->done) {
         return;
     }
-    const uint32_t n = A->n;
-    const uint32_t L = xcd_logical_block();
     const uint32_t q = threadIdx.x & 3u, sub = threadIdx.x >> 2;
     gptr<const uint32_t> rp = as_global(A->rp);
     gptr<const uint32_t> col = as_global(A->col);
@@ -369,8 +286,7 @@ __global__ __launch_bounds__(kNT) void k_spmv_split(const Args *__restrict__ A,
     gptr<const float> cd = as_global(A->cdiag);
     gptr<const float> v = as_global(vin);
     gptr<float> y = as_global(yout);
-    const uint32_t lo = L * A->R1;
-    const uint32_t hi = min(n, lo + A->R1);
+    const uint32_t lo = T.lo, hi = T.hi;
     float dot = 0.0f, adot = 0.0f;
     // row bounds of the next chunk are loaded while this chunk is in flight
     uint32_t nb[4];
@@ -487,19 +403,18 @@ __global__ __launch_bounds__(kNT) void k_spmv_ell(const Args *__restrict__ A,
     constexpr uint32_t W = 4 * M;
     constexpr uint32_t kRows = kNT / 4;
     __shared__ float lds[kNT / kWave];
-    if (DOT && as_global(A->s)->done) {
+    const uint32_t L = xcd_logical_block();
+    const Tile T = load_tile(A->tile1, L);
+    if (DOT && as_global(A->s + T.scene)->done) {
         return;
     }
-    const uint32_t n = A->n;
-    const uint32_t L = xcd_logical_block();
     const uint32_t q = threadIdx.x & 3u, sub = threadIdx.x >> 2;
     gptr<const uint64_t> ent = as_global(reinterpret_cast<const uint64_t *>(A->ent));
     gptr<const float> val = as_global(A->val);
     gptr<const float> cd = as_global(A->cdiag);
     gptr<const float> v = as_global(vin);
     gptr<float> y = as_global(yout);
-    const uint32_t lo = L * A->R1;
-    const uint32_t hi = min(n, lo + A->R1);
+    const uint32_t lo = T.lo, hi = T.hi;
     float dot = 0.0f, adot = 0.0f;
     // The entries of the next chunk of rows are loaded while this chunk's
     // blocks are in flight: one dependent round trip per chunk instead of two.
@@ -601,9 +516,9 @@ __global__ __launch_bounds__(kNT) void k_spmv_ell(const Args *__restrict__ A,
 template <bool FAC>
 __global__ __launch_bounds__(kNT) void k_init_residual(const Args *__restrict__ A) {
     __shared__ float lds[kNT / kWave];
-    const uint32_t n = A->n;
     const uint32_t L = blockIdx.x;
-    const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
+    const Tile T = load_tile(A->tile2, L);
+    const uint32_t lo = T.lo, hi = T.hi;
     gptr<const float> b = as_global(A->b);
     gptr<const float> yv = as_global(A->Ap);
     gptr<const float> P = as_global(A->P);
@@ -652,13 +567,16 @@ __device__ inline float reduce_parts(gptr<const float> part, uint32_t cnt,
 }
 
 // Initial scalars and the bookkeeping state (solver.cu:1020-1031, 1170-1175).
+// One workgroup per scene.
 __global__ __launch_bounds__(kNT) void k_init_scalars(const Args *__restrict__ A) {
     __shared__ float lds[kNT / kWave];
+    const uint32_t sc = blockIdx.x;
+    const uint32_t t0 = A->sc2[sc], nt = A->sc2[sc + 1] - t0;
     gptr<const float> part = as_global(A->part);
-    const float rz0 = reduce_parts(part + 2 * A->G1, A->G2, lds);
-    const float err0 = reduce_parts(part + 2 * A->G1 + A->G2, A->G2, lds);
+    const float rz0 = reduce_parts(part + 2 * A->G1 + t0, nt, lds);
+    const float err0 = reduce_parts(part + 2 * A->G1 + A->G2 + t0, nt, lds);
     if (threadIdx.x == 0) {
-        gptr<Scalars> s = as_global(A->s);
+        gptr<Scalars> s = as_global(A->s + sc);
         s->rz[0] = rz0;
         s->rz[1] = rz0;
         s->err0 = err0;
@@ -691,26 +609,27 @@ template <bool FAC>
 __global__ __launch_bounds__(kNT) void k_update(const Args *__restrict__ A,
                                                 uint32_t k) {
     __shared__ float lds[kNT / kWave];
-    gptr<Scalars> s = as_global(A->s);
+    const uint32_t L = blockIdx.x;
+    const Tile T = load_tile(A->tile2, L);
+    gptr<Scalars> s = as_global(A->s + T.scene);
     if (s->done) {
         return;
     }
     gptr<const float> part = as_global(A->part);
-    const float pAp = reduce_parts(part, A->G1, lds);
-    const float absdot = reduce_parts(part + A->G1, A->G1, lds);
+    const uint32_t t0 = A->sc1[T.scene], nt = A->sc1[T.scene + 1] - t0;
+    const float pAp = reduce_parts(part + t0, nt, lds);
+    const float absdot = reduce_parts(part + A->G1 + t0, nt, lds);
     const float rz0 = s->rz[k & 1];
     const float bound = kNoiseSlack * kFloatEps * absdot;
     float alpha = 0.0f;
     if (pAp > bound) {
         alpha = rz0 / pAp;
-    } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+    } else if (T.first && threadIdx.x == 0) {
         s->breakdown = 1;
         s->cause = (pAp < -bound || pAp != pAp) ? 1 : 3;
         s->break_iter = (int32_t)s->iter;
     }
-    const uint32_t n = A->n;
-    const uint32_t L = blockIdx.x;
-    const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
+    const uint32_t lo = T.lo, hi = T.hi;
     gptr<float> x = as_global(A->x);
     gptr<float> r = as_global(A->r);
     gptr<float> z = as_global(A->z);
@@ -835,12 +754,12 @@ __global__ __launch_bounds__(kNT) void k_fac_g(const Args *__restrict__ A, int i
 __global__ __launch_bounds__(kNT) void k_fac_gt(const Args *__restrict__ A, int init) {
     constexpr uint32_t P = kFacP, M = kFacM, kRows = kNT / kFacP;
     __shared__ float lds[kNT / kWave];
-    if (!init && as_global(A->s)->done) {
+    const uint32_t L = blockIdx.x;
+    const Tile T = load_tile(A->tile2, L);
+    if (!init && as_global(A->s + T.scene)->done) {
         return;
     }
-    const uint32_t n = A->n;
-    const uint32_t L = blockIdx.x;
-    const uint32_t lo = L * A->R2, hi = min(n, lo + A->R2);
+    const uint32_t lo = T.lo, hi = T.hi;
     const uint32_t q = threadIdx.x % P, sub = threadIdx.x / P;
     gptr<const uint16_t> Gd = as_global(A->Gd), Gl = as_global(A->Gl);
     gptr<const float> gy = as_global(A->gy), r = as_global(A->r);
@@ -925,27 +844,27 @@ __global__ __launch_bounds__(kNT) void k_fac_gt(const Args *__restrict__ A, int 
 __global__ __launch_bounds__(kNT) void k_pupdate(const Args *__restrict__ A,
                                                  uint32_t k) {
     __shared__ float lds[kNT / kWave];
-    gptr<Scalars> s = as_global(A->s);
+    const Tile T = load_tile(A->tile2, blockIdx.x);
+    gptr<Scalars> s = as_global(A->s + T.scene);
     if (s->done) {
         return;
     }
     gptr<const float> part = as_global(A->part);
-    const float rz1 = reduce_parts(part + 2 * A->G1, A->G2, lds);
+    const uint32_t t0 = A->sc2[T.scene], nt = A->sc2[T.scene + 1] - t0;
+    const float rz1 = reduce_parts(part + 2 * A->G1 + t0, nt, lds);
     const float rz0 = s->rz[k & 1];
     const bool rz_ok = rz0 > 0.0f;
     const float beta = rz_ok ? rz1 / rz0 : 0.0f;
-    const uint32_t n3 = 3 * A->n;
-    const uint32_t L = blockIdx.x;
-    const uint32_t lo = L * A->R3, hi = min(n3, lo + A->R3);
+    const uint32_t lo = 3 * T.lo, hi = 3 * T.hi;
     gptr<float> p = as_global(A->p);
     gptr<const float> z = as_global(A->z);
     for (uint32_t e = lo + threadIdx.x; e < hi; e += kNT) {
         p[e] = fmaf(beta, p[e], 1.0f * z[e]);
     }
-    if (blockIdx.x != 0) {
+    if (!T.first) {
         return;
     }
-    const float err = reduce_parts(part + 2 * A->G1 + A->G2, A->G2, lds);
+    const float err = reduce_parts(part + 2 * A->G1 + A->G2 + t0, nt, lds);
     if (threadIdx.x != 0) {
         return;
     }
@@ -1025,6 +944,13 @@ struct PcgPlanImpl {
     uint32_t split_m = 4;      // entries per lane per pass of the split form
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
+    // scenes (a plain plan is one scene) and their tiles
+    uint32_t nscene = 1;
+    std::vector<uint32_t> scene_row; // [nscene + 1] first row of each scene
+    uint32_t tile_r1 = 0, tile_r2 = 0; // rows per tile; 0: one resident round
+    bool compacted = false;    // tiles cover only the scenes still iterating
+    void *tmem = nullptr;      // scalars | scene tile starts | tiles | partials
+    size_t tmem_cap = 0;
 };
 
 namespace {
@@ -1064,8 +990,7 @@ hipError_t launch_spmv(const PcgPlanImpl &pl, const float *v, float *y,
     case 3: k_spmv_ell<DOT, 3><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
     case 4: k_spmv_ell<DOT, 4><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
     case 6: k_spmv_ell<DOT, 6><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
-    case 8: k_spmv_ell<DOT, 8><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
-    default: k_spmv<DOT><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
+    default: k_spmv_ell<DOT, 8><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
     }
     return hipGetLastError();
 }
@@ -1083,7 +1008,7 @@ hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
     } else {
         k_update<false><<<h.G2, kNT, 0, st>>>(pl.d_args, k);
     }
-    k_pupdate<<<h.G3, kNT, 0, st>>>(pl.d_args, k);
+    k_pupdate<<<h.G2, kNT, 0, st>>>(pl.d_args, k);
     return hipGetLastError();
 }
 
@@ -1192,6 +1117,102 @@ int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_
     return PPF_OK;
 }
 
+// Tiles of the SpMV (tile1) and of the vector kernels (tile2). Rows per tile
+// r1 / r2 == 0: one resident round over the whole system (one scene), rows
+// in multiples of 64. Otherwise every scene is cut into tiles of at most
+// r1 / r2 rows (an empty scene gets one empty tile); tile1 is padded to a
+// multiple of 8 (the XCD mapping) with empty tiles of the last scene. Each
+// scene's partial sums are then those of its own contiguous tile range.
+// `active` (per scene, or null for all): scenes left out get no tiles (their
+// solves are over). The scalars keep their place across calls.
+int set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st,
+              const std::vector<uint8_t> *active = nullptr) {
+    const uint32_t n = pl->n, ns = pl->nscene;
+    Args &h = pl->h;
+    std::vector<Tile> t1, t2;
+    std::vector<uint32_t> sc1(ns + 1), sc2(ns + 1);
+    auto cut = [&](std::vector<Tile> &t, std::vector<uint32_t> &sc, uint32_t rows) {
+        for (uint32_t c = 0; c < ns; ++c) {
+            const uint32_t a = pl->scene_row[c], b = pl->scene_row[c + 1];
+            sc[c] = (uint32_t)t.size();
+            if (active && !(*active)[c]) {
+                continue;
+            }
+            if (a == b) {
+                t.push_back(Tile{a, a, c, 1u});
+            }
+            for (uint32_t lo = a; lo < b; lo += rows) {
+                t.push_back(Tile{lo, std::min(b, lo + rows), c, lo == a ? 1u : 0u});
+            }
+        }
+    };
+    if (r1 == 0 || r2 == 0) {
+        if (ns != 1) {
+            set_error("pcg: a plan of %u scenes needs explicit tile sizes", ns);
+            return PPF_ERR_ARG;
+        }
+        // One round of resident workgroups per kernel (a second, partly
+        // filled round costs as much as the first): the SpMV's register use
+        // allows 5 workgroups per CU, the vector kernels 8.
+        const uint32_t g1 =
+            round8(std::min(resident_blocks(k_spmv_split<true, 4>, 1280u), cdiv(n, 64)));
+        const uint32_t g2 = std::min(resident_blocks(k_update<false>, 2048u), cdiv(n, 64));
+        r1 = cdiv(cdiv(n, g1), 64) * 64;
+        r2 = cdiv(cdiv(n, g2), 64) * 64;
+        cut(t1, sc1, r1);
+        cut(t2, sc2, r2);
+        while (t1.size() < g1) {
+            t1.push_back(Tile{n, n, 0u, 0u});
+        }
+        pl->tile_r1 = pl->tile_r2 = 0;
+    } else {
+        cut(t1, sc1, r1);
+        cut(t2, sc2, r2);
+        pl->tile_r1 = r1, pl->tile_r2 = r2;
+    }
+    pl->compacted = active != nullptr;
+    while (t1.size() % 8) {
+        t1.push_back(Tile{n, n, ns - 1, 0u});
+    }
+    sc1[ns] = (uint32_t)t1.size();
+    sc2[ns] = (uint32_t)t2.size();
+    h.G1 = (uint32_t)t1.size();
+    h.G2 = (uint32_t)t2.size();
+    h.nscene = ns;
+    const size_t bytes = carve_bytes(ns, sizeof(Scalars)) + 2 * carve_bytes(ns + 1, 4) +
+                         carve_bytes(t1.size(), sizeof(Tile)) +
+                         carve_bytes(t2.size(), sizeof(Tile)) +
+                         carve_bytes(2 * t1.size() + 2 * t2.size(), 4);
+    if (bytes > pl->tmem_cap) {
+        if (active) {
+            set_error("pcg: compacted tiles outgrow the full tiling");
+            return PPF_ERR_ARG;
+        }
+        if (pl->tmem) {
+            PPF_HIP(hipFree(pl->tmem));
+            pl->tmem = nullptr;
+            pl->tmem_cap = 0;
+        }
+        PPF_HIP(hipMalloc(&pl->tmem, bytes));
+        pl->tmem_cap = bytes;
+    }
+    char *cur = static_cast<char *>(pl->tmem);
+    h.s = carve<Scalars>(cur, ns);
+    uint32_t *dsc1 = carve<uint32_t>(cur, ns + 1);
+    uint32_t *dsc2 = carve<uint32_t>(cur, ns + 1);
+    Tile *d1 = carve<Tile>(cur, t1.size());
+    Tile *d2 = carve<Tile>(cur, t2.size());
+    h.tile1 = d1, h.tile2 = d2, h.sc1 = dsc1, h.sc2 = dsc2;
+    h.part = carve<float>(cur, 2 * t1.size() + 2 * t2.size());
+    PPF_HIP(hipMemcpyAsync(d1, t1.data(), t1.size() * sizeof(Tile), hipMemcpyHostToDevice, st));
+    PPF_HIP(hipMemcpyAsync(d2, t2.data(), t2.size() * sizeof(Tile), hipMemcpyHostToDevice, st));
+    PPF_HIP(hipMemcpyAsync(dsc1, sc1.data(), sc1.size() * 4, hipMemcpyHostToDevice, st));
+    PPF_HIP(hipMemcpyAsync(dsc2, sc2.data(), sc2.size() * 4, hipMemcpyHostToDevice, st));
+    PPF_HIP(hipStreamSynchronize(st)); // the host vectors end here
+    pl->graph_key = 0xffffffffu;      // grids changed: capture again
+    return PPF_OK;
+}
+
 // Capture kGraphIters iterations; they read every pointer through d_args, so
 // the graph serves every solve and pattern of the plan with the same SpMV form.
 int capture_graph(PcgPlanImpl *pl) {
@@ -1232,38 +1253,74 @@ int capture_graph(PcgPlanImpl *pl) {
 
 } // namespace
 
-int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
-                    const uint32_t *col, hipStream_t st, PcgPlanImpl **out) {
-    if (!out || n == 0 || !rp || !col || nnz >= kMirror) {
+namespace {
+
+// Every column of a row of scene c must lie in scene c's rows.
+__global__ void k_check_scenes(uint32_t n, uint32_t nscene, const uint32_t *__restrict__ srow,
+                               const uint32_t *__restrict__ rp,
+                               const uint32_t *__restrict__ col, uint32_t *__restrict__ bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    uint32_t lo = 0, hi = nscene; // last scene whose first row <= i
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (srow[mid] <= i) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const uint32_t a = srow[lo], b = srow[lo + 1];
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        if (col[k] < a || col[k] >= b) {
+            atomicAdd(bad, 1u);
+        }
+    }
+}
+
+} // namespace
+
+int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                           uint32_t nscene, const uint32_t *scene_row, hipStream_t st,
+                           PcgPlanImpl **out) {
+    if (!out || n == 0 || !rp || !col || nnz >= kMirror || nscene == 0 ||
+        (nscene > 1 && !scene_row)) {
         set_error("pcg_plan_create: invalid argument");
         return PPF_ERR_ARG;
+    }
+    if (nscene > 1) {
+        bool ok = scene_row[0] == 0 && scene_row[nscene] == n;
+        for (uint32_t c = 0; ok && c < nscene; ++c) {
+            ok = scene_row[c] <= scene_row[c + 1];
+        }
+        if (!ok) {
+            set_error("pcg_plan_create: scene rows must rise from 0 to n");
+            return PPF_ERR_ARG;
+        }
     }
     auto *pl = new (std::nothrow) PcgPlanImpl();
     if (!pl) {
         return PPF_ERR_ARG;
     }
     pl->n = n;
+    pl->nscene = nscene;
+    pl->scene_row.assign(nscene + 1, 0u);
+    if (nscene > 1) {
+        std::copy(scene_row, scene_row + nscene + 1, pl->scene_row.begin());
+    } else {
+        pl->scene_row[1] = n;
+    }
     Args &h = pl->h;
     h.n = n;
-    // One round of resident workgroups per kernel (a second, partly filled
-    // round costs as much as the first): the SpMV's register use allows 5
-    // workgroups per CU, the streaming kernels 8. Row ranges are multiples of
-    // 64 rows: one pass of the SpMV, one wave of the streaming kernels.
-    h.G1 = round8(std::min(resident_blocks(k_spmv_split<true, 4>, 1280u), cdiv(n, 64)));
-    h.R1 = cdiv(cdiv(n, h.G1), 64) * 64;
-    const uint32_t g2 = resident_blocks(k_update<false>, 2048u);
-    h.G2 = std::min(g2, cdiv(n, 64));
-    h.R2 = cdiv(cdiv(n, h.G2), 64) * 64;
-    h.G3 = std::min(g2, cdiv(3 * n, 64));
-    h.R3 = cdiv(cdiv(3 * n, h.G3), 64) * 64;
 
     const size_t n3 = 3 * (size_t)n;
     const size_t scan_words = scan_scratch_words(n + 1);
     size_t bytes = carve_bytes(n + 1, 4) * 2 + carve_bytes(n, 4) +
                    carve_bytes(9 * (size_t)n, 4) + carve_bytes(n3, 4) * 5 +
-                   carve_bytes(2 * (size_t)h.G1 + 2 * (size_t)h.G2, 4) +
-                   carve_bytes(1, sizeof(Scalars)) + carve_bytes(1, sizeof(Args)) +
-                   carve_bytes(scan_words, 4) + carve_bytes(4, 4);
+                   carve_bytes(1, sizeof(Args)) + carve_bytes(scan_words, 4) +
+                   carve_bytes(4, 4) + carve_bytes(nscene + 1, 4);
     hipError_t e = hipMalloc(&pl->mem, bytes);
     if (e != hipSuccess) {
         set_error("pcg_plan_create: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
@@ -1280,12 +1337,11 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     h.p = carve<float>(cur, n3);
     h.Ap = carve<float>(cur, n3);
     h.gy = carve<float>(cur, n3);
-    h.part = carve<float>(cur, 2 * (size_t)h.G1 + 2 * (size_t)h.G2);
-    h.s = carve<Scalars>(cur, 1);
     pl->d_args = carve<Args>(cur, 1);
     pl->scan_tmp = carve<uint32_t>(cur, scan_words);
     pl->flags = carve<uint32_t>(cur, 4);
-    e = hipHostMalloc(reinterpret_cast<void **>(&pl->h_scal), sizeof(Scalars),
+    uint32_t *d_srow = carve<uint32_t>(cur, nscene + 1);
+    e = hipHostMalloc(reinterpret_cast<void **>(&pl->h_scal), nscene * sizeof(Scalars),
                       hipHostMallocDefault);
     if (e != hipSuccess) {
         set_error("pcg_plan_create: hipHostMalloc: %s", hipGetErrorString(e));
@@ -1293,6 +1349,30 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
         return PPF_ERR_HIP;
     }
     int rc = set_pattern(pl, nnz, rp, col, st);
+    if (rc == PPF_OK && nscene > 1) {
+        uint32_t bad = 0;
+        e = hipMemcpyAsync(d_srow, pl->scene_row.data(), (nscene + 1) * 4,
+                           hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemsetAsync(pl->flags + 3, 0, 4, st);
+        if (e == hipSuccess) {
+            k_check_scenes<<<cdiv(n, kNT), kNT, 0, st>>>(n, nscene, d_srow, rp, col,
+                                                         pl->flags + 3);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&bad, pl->flags + 3, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            set_error("pcg_plan_create: %s", hipGetErrorString(e));
+            rc = PPF_ERR_HIP;
+        } else if (bad) {
+            set_error("pcg_plan_create: %u entries couple two scenes", bad);
+            rc = PPF_ERR_ARG;
+        }
+    }
+    if (rc == PPF_OK) {
+        // a batch starts from mid-size tiles; the autotuner refines them
+        rc = nscene > 1 ? set_tiles(pl, 512u, 512u, st) : set_tiles(pl, 0u, 0u, st);
+    }
     if (rc == PPF_OK) {
         rc = capture_graph(pl);
     }
@@ -1303,6 +1383,13 @@ int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
     *out = pl;
     return PPF_OK;
 }
+
+int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
+                    const uint32_t *col, hipStream_t st, PcgPlanImpl **out) {
+    return pcg_plan_create_scenes(n, nnz, rp, col, 1u, nullptr, st, out);
+}
+
+uint32_t pcg_plan_scenes(const PcgPlanImpl *pl) { return pl->nscene; }
 
 PcgPattern pcg_pattern(const PcgPlanImpl *pl) {
     const Args &h = pl->h;
@@ -1344,12 +1431,21 @@ void pcg_plan_destroy(PcgPlanImpl *pl) {
     if (pl->ell) {
         (void)hipFree(pl->ell);
     }
+    if (pl->tmem) {
+        (void)hipFree(pl->tmem);
+    }
     delete pl;
 }
 
 int pcg_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
              const float *x, float *y, hipStream_t st) {
     Args &h = pl->h;
+    if (pl->compacted) { // the last solve ended on a subset of the scenes
+        int rc = set_tiles(pl, pl->tile_r1, pl->tile_r2, st);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+    }
     h.val = val;
     h.cdiag = cdiag;
     h.b = nullptr;
@@ -1357,8 +1453,8 @@ int pcg_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
     h.err_hist = nullptr;
     PPF_HIP(hipMemcpyAsync(pl->d_args, &h, sizeof(Args), hipMemcpyHostToDevice,
                            st));
-    PPF_HIP(hipMemsetAsync(&h.s->bad, 0, sizeof(uint32_t), st));
-    k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
+    PPF_HIP(hipMemsetAsync(h.s, 0, pl->nscene * sizeof(Scalars), st));
+    k_diag_invert<<<h.G2, kNT, 0, st>>>(pl->d_args);
     PPF_HIP(hipGetLastError());
     PPF_HIP(launch_spmv<false>(*pl, x, y, st));
     PPF_HIP(hipStreamSynchronize(st));
@@ -1373,6 +1469,12 @@ int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
         return PPF_ERR_ARG;
     }
     Args &h = pl->h;
+    if (pl->compacted) { // the last solve ended on a subset of the scenes
+        int rc = set_tiles(pl, pl->tile_r1, pl->tile_r2, st);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+    }
     h.val = val;
     h.cdiag = cdiag;
     h.b = nullptr;
@@ -1380,10 +1482,8 @@ int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
     h.err_hist = nullptr;
     PPF_HIP(hipMemcpyAsync(pl->d_args, &h, sizeof(Args), hipMemcpyHostToDevice,
                            st));
-    Scalars init{};
-    PPF_HIP(hipMemcpyAsync(h.s, &init, sizeof(Scalars), hipMemcpyHostToDevice,
-                           st));
-    k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
+    PPF_HIP(hipMemsetAsync(h.s, 0, pl->nscene * sizeof(Scalars), st));
+    k_diag_invert<<<h.G2, kNT, 0, st>>>(pl->d_args);
     PPF_HIP(hipGetLastError());
     hipEvent_t e0, e1;
     PPF_HIP(hipEventCreate(&e0));
@@ -1411,7 +1511,18 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
         set_error("pcg_solve: invalid argument");
         return PPF_ERR_ARG;
     }
+    const uint32_t ns = pl->nscene;
+    if (ns > 1 && (fac || err_hist)) {
+        set_error("pcg_solve: a plan of %u scenes solves with block Jacobi and no history", ns);
+        return PPF_ERR_ARG;
+    }
     Args &h = pl->h;
+    if (pl->compacted) { // the last solve ended on a subset of the scenes
+        int rc = set_tiles(pl, pl->tile_r1, pl->tile_r2, st);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+    }
     h.val = val;
     h.cdiag = cdiag;
     h.b = b;
@@ -1430,13 +1541,17 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
     }
     PPF_HIP(hipMemcpyAsync(pl->d_args, &h, sizeof(Args), hipMemcpyHostToDevice,
                            st));
-    Scalars init{};
-    init.tol = tol;
-    init.max_iter = max_iter;
-    PPF_HIP(hipMemcpyAsync(h.s, &init, sizeof(Scalars), hipMemcpyHostToDevice,
-                           st));
+    // the pinned status buffer stages the initial scalars of every scene
+    PPF_HIP(hipStreamSynchronize(st));
+    for (uint32_t c = 0; c < ns; ++c) {
+        Scalars init{};
+        init.tol = tol;
+        init.max_iter = max_iter;
+        pl->h_scal[c] = init;
+    }
+    PPF_HIP(hipMemcpyAsync(h.s, pl->h_scal, ns * sizeof(Scalars), hipMemcpyHostToDevice, st));
     if (!Gd) {
-        k_diag_invert<<<cdiv(pl->n, kNT), kNT, 0, st>>>(pl->d_args);
+        k_diag_invert<<<h.G2, kNT, 0, st>>>(pl->d_args);
         PPF_HIP(hipGetLastError());
     }
     PPF_HIP(launch_spmv<false>(*pl, x, h.Ap, st));
@@ -1448,32 +1563,135 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
         k_init_residual<false><<<h.G2, kNT, 0, st>>>(pl->d_args);
     }
     PPF_HIP(hipGetLastError());
-    k_init_scalars<<<1, kNT, 0, st>>>(pl->d_args);
+    k_init_scalars<<<ns, kNT, 0, st>>>(pl->d_args);
     PPF_HIP(hipGetLastError());
-    // Replay graphs until the device bookkeeping reports done. The first
-    // status read follows one graph; the iteration cap bounds the loop.
+    // Replay graphs until the device bookkeeping reports every scene done.
+    // The first status read follows one graph; the iteration cap bounds the
+    // loop. With many scenes, once the scenes still iterating hold under 3/4
+    // of the tiled rows, the tiles are cut again over those scenes only (the
+    // partial sums are written and read within one iteration, so the switch
+    // between graphs is safe).
     const uint32_t max_launch = max_iter / kGraphIters + 2;
+    uint32_t tiled_rows = pl->n;
+    std::vector<uint8_t> active(ns, 1u);
     for (uint32_t g = 0;; ++g) {
-        PPF_HIP(hipMemcpyAsync(pl->h_scal, h.s, sizeof(Scalars),
+        PPF_HIP(hipMemcpyAsync(pl->h_scal, h.s, ns * sizeof(Scalars),
                                hipMemcpyDeviceToHost, st));
         PPF_HIP(hipStreamSynchronize(st));
-        if (pl->h_scal->done) {
+        uint32_t live_rows = 0, live = 0;
+        for (uint32_t c = 0; c < ns; ++c) {
+            active[c] = pl->h_scal[c].done == 0;
+            if (active[c]) {
+                ++live;
+                live_rows += pl->scene_row[c + 1] - pl->scene_row[c];
+            }
+        }
+        if (live == 0) {
             break;
         }
         if (g >= max_launch) {
             set_error("pcg_solve: bookkeeping did not terminate");
             return PPF_ERR_SOLVER;
         }
+        if (ns > 1 && 4ull * live_rows < 3ull * tiled_rows) {
+            int rc = set_tiles(pl, pl->tile_r1, pl->tile_r2, st, &active);
+            if (rc == PPF_OK) {
+                PPF_HIP(hipMemcpyAsync(pl->d_args, &h, sizeof(Args), hipMemcpyHostToDevice, st));
+                rc = capture_graph(pl);
+            }
+            if (rc != PPF_OK) {
+                return rc;
+            }
+            tiled_rows = live_rows;
+        }
         PPF_HIP(hipGraphLaunch(pl->exec, st));
     }
-    const Scalars &s = *pl->h_scal;
-    res->iter = s.rep_iter;
-    res->resid = s.rep_resid;
-    res->status = s.status;
-    res->break_cause = s.cause;
-    res->break_iter = s.break_iter;
-    res->err0 = s.err0;
+    for (uint32_t c = 0; c < ns; ++c) {
+        const Scalars &sc = pl->h_scal[c];
+        res[c].iter = sc.rep_iter;
+        res[c].resid = sc.rep_resid;
+        res[c].status = sc.status;
+        res[c].break_cause = sc.cause;
+        res[c].break_iter = sc.break_iter;
+        res[c].err0 = sc.err0;
+    }
     return PPF_OK;
+}
+
+// Tile sizes of a many-scene plan by measurement: each candidate pair (SpMV
+// rows, vector-kernel rows per tile) runs the same solve (capped at 3 graphs
+// of iterations, x in the plan's scratch), the fastest is kept.
+int pcg_plan_autotune(PcgPlanImpl *pl, const float *val, const float *cdiag, const float *b,
+                      float tol, uint32_t *best_r1, uint32_t *best_r2, float *best_ms,
+                      hipStream_t st) {
+    if (!pl || !val || !cdiag || !b) {
+        set_error("pcg_plan_autotune: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    static const uint32_t k1[] = {128u, 256u, 512u, 1024u, 2048u};
+    static const uint32_t k2[] = {256u, 512u, 1024u, 2048u};
+    std::vector<ppf_pcg_result> res(pl->nscene);
+    hipEvent_t e0, e1;
+    PPF_HIP(hipEventCreate(&e0));
+    PPF_HIP(hipEventCreate(&e1));
+    float best = 3.4e38f;
+    uint32_t b1 = 512u, b2 = 512u;
+    int rc = PPF_OK;
+    for (uint32_t r1 : k1) {
+        for (uint32_t r2 : k2) {
+            rc = set_tiles(pl, r1, r2, st);
+            if (rc == PPF_OK) rc = capture_graph(pl);
+            float ms = 0.0f;
+            for (int rep = 0; rc == PPF_OK && rep < 2; ++rep) { // the first run warms up
+                hipError_t e = hipMemsetAsync(pl->h.gy, 0, 12 * (size_t)pl->n, st);
+                if (e == hipSuccess) e = hipEventRecord(e0, st);
+                if (e != hipSuccess) {
+                    set_error("pcg_plan_autotune: %s", hipGetErrorString(e));
+                    rc = PPF_ERR_HIP;
+                    break;
+                }
+                rc = pcg_solve(pl, val, cdiag, b, pl->h.gy, tol, 3 * kGraphIters, nullptr,
+                               res.data(), st, nullptr);
+                if (rc == PPF_OK) {
+                    e = hipEventRecord(e1, st);
+                    if (e == hipSuccess) e = hipEventSynchronize(e1);
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                    if (e != hipSuccess) {
+                        set_error("pcg_plan_autotune: %s", hipGetErrorString(e));
+                        rc = PPF_ERR_HIP;
+                    }
+                }
+            }
+            if (rc != PPF_OK) {
+                break;
+            }
+            if (ms < best) {
+                best = ms, b1 = r1, b2 = r2;
+            }
+        }
+        if (rc != PPF_OK) {
+            break;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == PPF_OK) rc = set_tiles(pl, b1, b2, st);
+    if (rc == PPF_OK) rc = capture_graph(pl);
+    if (rc == PPF_OK) {
+        if (best_r1) *best_r1 = b1;
+        if (best_r2) *best_r2 = b2;
+        if (best_ms) *best_ms = best;
+    }
+    return rc;
+}
+
+int pcg_plan_set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st) {
+    if (!pl || (pl->nscene > 1 && (r1 == 0 || r2 == 0)) || (r1 == 0) != (r2 == 0)) {
+        set_error("pcg_plan_set_tiles: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    int rc = set_tiles(pl, r1, r2, st);
+    return rc == PPF_OK ? capture_graph(pl) : rc;
 }
 
 } // namespace ppf

@@ -9,6 +9,21 @@ struct PcgPlanImpl;
 
 int pcg_plan_create(uint32_t n, uint32_t nnz, const uint32_t *rp,
                     const uint32_t *col, hipStream_t st, PcgPlanImpl **out);
+// A plan over the block-diagonal union of nscene systems: scene c owns rows
+// scene_row[c] .. scene_row[c + 1] (host array, nscene + 1 entries) and its
+// entries couple only those rows. One launch per kernel serves all scenes;
+// each scene keeps its own scalars, stopping rule and result.
+int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                           uint32_t nscene, const uint32_t *scene_row, hipStream_t st,
+                           PcgPlanImpl **out);
+uint32_t pcg_plan_scenes(const PcgPlanImpl *pl);
+// Rows per tile of the SpMV and of the vector kernels (every scene cut into
+// tiles; 0, 0: one resident round over a one-scene plan).
+int pcg_plan_set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st);
+// Measure candidate tile sizes on this system and keep the fastest.
+int pcg_plan_autotune(PcgPlanImpl *pl, const float *val, const float *cdiag, const float *b,
+                      float tol, uint32_t *best_r1, uint32_t *best_r2, float *best_ms,
+                      hipStream_t st);
 void pcg_plan_destroy(PcgPlanImpl *pl);
 // Point the plan at a new pattern of the same n (buffers grow as needed; the
 // iteration graph is re-captured only when the SpMV form changes).
@@ -22,6 +37,7 @@ int pcg_plan_repattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
 struct PcgFactor {
     const uint16_t *Gd, *Gl, *Gr;
 };
+// res: one result per scene. Many-scene plans take no factor and no history.
 int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
               const float *b, float *x, float tol, uint32_t max_iter,
               float *err_hist, ppf_pcg_result *res, hipStream_t st,

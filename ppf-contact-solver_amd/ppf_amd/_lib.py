@@ -28,6 +28,11 @@ _PROTOS = {
     "ppf_device_info": (ctypes.c_int, [ctypes.c_char_p, u32, ctypes.POINTER(u32)]),
     "ppf_pcg_plan_create": (ctypes.c_int, [u32, u32, vp, vp, vp, ctypes.POINTER(vp)]),
     "ppf_pcg_plan_destroy": (None, [vp]),
+    "ppf_pcg_plan_create_batch": (ctypes.c_int, [u32, u32, vp, vp, u32, vp, vp,
+                                                 ctypes.POINTER(vp)]),
+    "ppf_pcg_plan_set_tiles": (ctypes.c_int, [vp, u32, u32, vp]),
+    "ppf_pcg_plan_autotune": (ctypes.c_int, [vp, vp, vp, vp, f32, vp, ctypes.POINTER(f32), vp]),
+    "ppf_pcg_solve_batch": (ctypes.c_int, [vp, vp, vp, vp, vp, f32, u32, vp, vp]),
     "ppf_pcg_plan_repattern": (ctypes.c_int, [vp, u32, vp, vp, vp]),
     "ppf_lpc_weight_count": (u32, []),
     "ppf_lpc_create": (ctypes.c_int, [ctypes.POINTER(vp)]),

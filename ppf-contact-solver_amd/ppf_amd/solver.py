@@ -7,6 +7,7 @@ in libppf_mi355x.so; tensors are torch device buffers.
 """
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -109,6 +110,83 @@ class BlockJacobiPCG:
                                          _ptr(y), reps, ctypes.byref(ms), _stream()),
               "ppf_pcg_time_spmv")
         return ms.value
+
+
+class BatchPCG(BlockJacobiPCG):
+    """One plan over many independent systems stacked block-diagonally:
+    scene c owns rows scene_row[c] .. scene_row[c + 1]. Every kernel launch
+    serves all scenes; each scene has its own scalars and result."""
+
+    def __init__(self, row_offset, col_index, scene_row):
+        self.lib = _lib.load()
+        self.n = row_offset.numel() - 1
+        self.nnz = col_index.numel()
+        _need(row_offset, torch.int32, self.n + 1, "row_offset")
+        _need(col_index, torch.int32, self.nnz, "col_index")
+        rows = np.ascontiguousarray(np.asarray(scene_row, np.uint32))
+        self.nscene = len(rows) - 1
+        self._keep = (row_offset, col_index)
+        h = ctypes.c_void_p()
+        check(self.lib.ppf_pcg_plan_create_batch(self.n, self.nnz, _ptr(row_offset),
+                                                 _ptr(col_index), self.nscene,
+                                                 rows.ctypes.data_as(ctypes.c_void_p),
+                                                 _stream(), ctypes.byref(h)),
+              "ppf_pcg_plan_create_batch")
+        self._h = h
+
+    def set_tiles(self, spmv_rows, vector_rows):
+        check(self.lib.ppf_pcg_plan_set_tiles(self._h, spmv_rows, vector_rows, _stream()),
+              "ppf_pcg_plan_set_tiles")
+
+    def autotune(self, value, cdiag, b, tol=1e-3):
+        """Measure candidate tile sizes on this system, keep the fastest:
+        returns (spmv rows, vector rows, ms of the capped timing solve)."""
+        self._check_operands(value, cdiag)
+        _need(b, torch.float32, 3 * self.n, "b")
+        rows = np.zeros(2, np.uint32)
+        ms = ctypes.c_float()
+        check(self.lib.ppf_pcg_plan_autotune(self._h, _ptr(value), _ptr(cdiag), _ptr(b), tol,
+                                             rows.ctypes.data_as(ctypes.c_void_p),
+                                             ctypes.byref(ms), _stream()),
+              "ppf_pcg_plan_autotune")
+        return int(rows[0]), int(rows[1]), ms.value
+
+    def solve(self, value, cdiag, b, x, tol=1e-3, max_iter=10000):
+        """Solve every scene in place; returns one PcgResult per scene."""
+        self._check_operands(value, cdiag)
+        _need(b, torch.float32, 3 * self.n, "b")
+        _need(x, torch.float32, 3 * self.n, "x")
+        res = (PcgResult * self.nscene)()
+        check(self.lib.ppf_pcg_solve_batch(self._h, _ptr(value), _ptr(cdiag), _ptr(b), _ptr(x),
+                                           tol, max_iter, ctypes.cast(res, ctypes.c_void_p),
+                                           _stream()), "ppf_pcg_solve_batch")
+        return list(res)
+
+
+def stack_systems(systems):
+    """Block-diagonal union of systems (dicts of row_offset, col_index,
+    value, cdiag, b; numpy or torch): the stacked arrays and the first row of
+    every scene."""
+    def host(a):
+        return a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    rp, ci, val, cd, b, rows = [np.zeros(1, np.int64)], [], [], [], [], [0]
+    base = nz = 0
+    for s in systems:
+        r = host(s["row_offset"]).astype(np.int64)
+        n = len(r) - 1
+        rp.append(r[1:] + nz)
+        ci.append(host(s["col_index"]).astype(np.int64) + base)
+        val.append(host(s["value"]).reshape(-1))
+        cd.append(host(s["cdiag"]).reshape(-1))
+        b.append(host(s["b"]).reshape(-1))
+        base += n
+        nz += int(r[-1])
+        rows.append(base)
+    return dict(row_offset=np.concatenate(rp).astype(np.int32),
+                col_index=np.concatenate(ci).astype(np.int32),
+                value=np.concatenate(val).astype(np.float32),
+                cdiag=np.concatenate(cd).astype(np.float32),
+                b=np.concatenate(b).astype(np.float32)), np.array(rows, np.uint32)
 
 
 def bj_invert(blocks):
