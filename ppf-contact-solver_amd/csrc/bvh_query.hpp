@@ -12,7 +12,14 @@
 namespace ppf {
 
 constexpr int kQueryThreads = 128; // threads per query workgroup
-constexpr int kQueryStack = 64;    // stack entries per thread
+// Stack entries per thread. A depth-first walk that tests children before
+// pushing them holds at most depth + 1 entries. A Karras tree over 30-bit
+// Morton codes is at most 30 levels deep over distinct codes, plus log2 of
+// the largest run of equal codes (ties split by index): 48 entries cover runs
+// of 2^17. The C3 trees (1-1.5M leaves) are 28-30 levels deep. A deeper walk
+// raises the overflow flag (a capacity error), it never writes past the stack.
+// 48 rather than 64 entries lets 12 one-wave workgroups share a CU's LDS.
+constexpr int kQueryStack = 48;
 
 struct QBox {
     float mn[3], mx[3];
