@@ -140,6 +140,9 @@ def run_drape(args, dev):
         learned = run_learned(args, dev, sysd)
     n, nnz = sysd["row_offset"].numel() - 1, sysd["col_index"].numel()
     plan = BlockJacobiPCG(sysd["row_offset"], sysd["col_index"])
+    # the row order the stepper solves in (the kernel its iterations run)
+    plan.set_order(gpu.solve_order())
+    plan.repattern(sysd["row_offset"], sysd["col_index"])
     roof = spmv_roofline(plan, sysd["value"], sysd["cdiag"], sysd["b"], n, nnz,
                          "pcg spmv on the last Newton system of the timed steps",
                          traffic_from_profiles("pcg_spmv_drape_stack.md"))

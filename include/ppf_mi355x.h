@@ -140,6 +140,17 @@ int ppf_pcg_solve_hist(ppf_pcg_plan *plan, const float *d_value,
 int ppf_pcg_spmv(ppf_pcg_plan *plan, const float *d_value, const float *d_cdiag,
                  const float *d_x, float *d_y, void *stream);
 
+/* Solve in another row order: new row k is original row d_order[k] (device
+ * array of n, kept by the caller until the next call). The upper pattern is
+ * rebuilt in that order at the next ppf_pcg_plan_repattern (blocks whose rows
+ * swap order move to the other row, transposed); every entry point keeps
+ * taking and returning values, right-hand sides and iterates in the original
+ * order. A locality order (e.g. the vertices' Morton order) keeps the
+ * transposed half's block reads in L2. d_order null: the original order. One-
+ * scene plans only. No reference counterpart (the reference solves in the
+ * mesh's order). */
+int ppf_pcg_plan_set_order(ppf_pcg_plan *plan, const uint32_t *d_order, void *stream);
+
 /* Batched PCG: one plan over many independent systems ("scenes") stacked
  * block-diagonally: scene c owns rows h_scene_row[c] .. h_scene_row[c + 1]
  * (host array of nscene + 1 rising row indices, 0 .. n) and its blocks couple
@@ -538,6 +549,12 @@ int ppf_stepper_update_pins(ppf_stepper *h, const ppf_scene *scene, void *stream
 int ppf_stepper_system(ppf_stepper *h, uint32_t *n, uint32_t *nnz, uint32_t *d_row_offset,
                        uint32_t *d_col_index, float *d_value, float *d_cdiag, float *d_rhs,
                        void *stream);
+
+/* The row order the stepper's solves run in (new row k = vertex
+ * d_order[k]; n entries copied into d_order): the mesh pieces one after the
+ * other, each in the Morton order of its initial positions (see
+ * ppf_pcg_plan_set_order). */
+int ppf_stepper_solve_order(ppf_stepper *h, uint32_t *d_order, void *stream);
 
 /* Records of the stepper's last post-step intersection check (the first
  * min(cap, 256) of them; contact::check_intersection, contact.cu:3056-3175).

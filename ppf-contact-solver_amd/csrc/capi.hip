@@ -289,6 +289,14 @@ int ppf_pcg_plan_create_batch(uint32_t n, uint32_t nnz, const uint32_t *d_row_of
     return PPF_OK;
 }
 
+int ppf_pcg_plan_set_order(ppf_pcg_plan *plan, const uint32_t *d_order, void *stream) {
+    if (!plan) {
+        set_error("ppf_pcg_plan_set_order: plan is null");
+        return PPF_ERR_ARG;
+    }
+    return pcg_plan_set_order(plan->impl, d_order, (hipStream_t)stream);
+}
+
 int ppf_pcg_plan_set_tiles(ppf_pcg_plan *plan, uint32_t spmv_rows, uint32_t vector_rows,
                            void *stream) {
     if (!plan) {
@@ -719,6 +727,16 @@ int ppf_stepper_system(ppf_stepper *h, uint32_t *n, uint32_t *nnz, uint32_t *d_r
         cp(d_rhs, rhs, 3 * nr * 4)) {
         return PPF_ERR_HIP;
     }
+    return PPF_OK;
+}
+
+int ppf_stepper_solve_order(ppf_stepper *h, uint32_t *d_order, void *stream) {
+    if (!h || !d_order || !h->impl.solve_order()) {
+        set_error("ppf_stepper_solve_order: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    PPF_HIP(hipMemcpyAsync(d_order, h->impl.solve_order(), h->impl.nv() * sizeof(uint32_t),
+                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
     return PPF_OK;
 }
 

@@ -250,7 +250,7 @@ void Stepper::release() {
     for (void *p : {(void *)ex_, (void *)target_, (void *)dx_, (void *)force_, (void *)cdiag_,
                     (void *)eval_, (void *)cgrad_, (void *)mask_, (void *)scal_,
                     (void *)pairs_, (void *)mrp_, (void *)mci_, (void *)mcnt_,
-                    (void *)scan_, (void *)mval_}) {
+                    (void *)scan_, (void *)mval_, (void *)order_}) {
         if (p) {
             (void)hipFree(p);
         }
@@ -270,11 +270,119 @@ void Stepper::release() {
     }
     ex_ = target_ = dx_ = force_ = cdiag_ = eval_ = cgrad_ = nullptr;
     mask_ = scal_ = h_scal_ = nullptr;
+    order_ = nullptr;
+    order_changed_ = false;
     pairs_ = nullptr;
     mrp_ = mci_ = mcnt_ = scan_ = nullptr;
     mval_ = nullptr;
     pair_cap_ = mcap_ = 0;
 }
+
+namespace {
+
+uint32_t spread10(uint32_t v) { // 10 bits -> every third bit of 30
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+uint32_t spread16(uint32_t v) { // 16 bits -> every other bit of 32
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// Row order of the solve: mesh pieces (connected by faces) one after the
+// other, each in the Morton order of its vertices' initial positions within
+// the piece's own box, so that a vertex's mesh neighbours are near it in the
+// order (one Morton curve over all vertices would interleave stacked
+// sheets). An axis along which a piece is thinner than 1% of its largest
+// extent is left out (a sheet gets a 2D curve: its noise across the sheet
+// would otherwise scramble the order).
+void piece_morton_order(uint32_t nv, const std::vector<float> &x,
+                        const std::vector<uint32_t> &face,
+                        std::vector<uint32_t> &order) {
+    std::vector<uint32_t> up(nv);
+    for (uint32_t v = 0; v < nv; ++v) {
+        up[v] = v;
+    }
+    auto find = [&](uint32_t v) {
+        while (up[v] != v) {
+            up[v] = up[up[v]];
+            v = up[v];
+        }
+        return v;
+    };
+    for (size_t f = 0; f + 2 < face.size(); f += 3) {
+        if (face[f] >= nv || face[f + 1] >= nv || face[f + 2] >= nv) {
+            continue; // malformed faces are the assembly's to reject
+        }
+        for (int k = 1; k < 3; ++k) {
+            const uint32_t a = find(face[f]), b = find(face[f + k]);
+            if (a != b) {
+                up[std::max(a, b)] = std::min(a, b); // a piece is named by its lowest vertex
+            }
+        }
+    }
+    std::vector<uint32_t> piece(nv);
+    std::vector<float> box(6 * (size_t)nv);
+    for (uint32_t v = 0; v < nv; ++v) {
+        piece[v] = find(v);
+        if (piece[v] == v) {
+            for (int d = 0; d < 3; ++d) {
+                box[6 * (size_t)v + d] = FLT_MAX;
+                box[6 * (size_t)v + 3 + d] = -FLT_MAX;
+            }
+        }
+    }
+    for (uint32_t v = 0; v < nv; ++v) {
+        float *b = &box[6 * (size_t)piece[v]];
+        for (int d = 0; d < 3; ++d) {
+            b[d] = std::min(b[d], x[3 * (size_t)v + d]);
+            b[3 + d] = std::max(b[3 + d], x[3 * (size_t)v + d]);
+        }
+    }
+    std::vector<uint64_t> key(nv);
+    for (uint32_t v = 0; v < nv; ++v) {
+        const float *b = &box[6 * (size_t)piece[v]];
+        float ext[3], big = 0.0f;
+        for (int d = 0; d < 3; ++d) {
+            ext[d] = b[3 + d] - b[d];
+            big = std::max(big, ext[d]);
+        }
+        int axes[3], na = 0;
+        for (int d = 0; d < 3; ++d) {
+            if (ext[d] > 0.01f * big) {
+                axes[na++] = d;
+            }
+        }
+        uint32_t code = 0;
+        const uint32_t cells = na == 3 ? 1024u : 65536u;
+        for (int k = 0; k < na; ++k) {
+            const int d = axes[k];
+            const float t = (x[3 * (size_t)v + d] - b[d]) / ext[d];
+            const uint32_t q = (uint32_t)std::min((float)(cells - 1),
+                                                  std::max(0.0f, t * (float)cells));
+            code |= na == 3 ? spread10(q) << (2 - k)
+                            : (na == 2 ? spread16(q) << (1 - k) : q);
+        }
+        key[v] = ((uint64_t)piece[v] << 32) | code;
+    }
+    order.resize(nv);
+    for (uint32_t v = 0; v < nv; ++v) {
+        order[v] = v;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+}
+
+} // namespace
 
 int Stepper::create(const ppf_scene &sc, hipStream_t s) {
     release();
@@ -296,6 +404,7 @@ int Stepper::create(const ppf_scene &sc, hipStream_t s) {
     PPF_HIP(hipMalloc(&cdiag_, 9 * (size_t)nv_ * sizeof(float)));
     PPF_HIP(hipMalloc(&eval_, 9 * ((size_t)elastic_.nnz() + 1) * sizeof(float)));
     PPF_HIP(hipMalloc(&mask_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&order_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
     PPF_HIP(hipMalloc(&scal_, 4 * sizeof(uint32_t)));
     PPF_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_scal_), 4 * sizeof(uint32_t),
                           hipHostMallocDefault));
@@ -304,6 +413,22 @@ int Stepper::create(const ppf_scene &sc, hipStream_t s) {
     PPF_HIP(hipMalloc(&scan_, scan_scratch_words(nv_ + 1) * sizeof(uint32_t)));
     for (hipEvent_t &e : ev_) {
         PPF_HIP(hipEventCreate(&e));
+    }
+    if (nv_ > 1) {
+        std::vector<float> hx(n3);
+        std::vector<uint32_t> hf(3 * (size_t)nf_), order;
+        PPF_HIP(hipMemcpyAsync(hx.data(), sc.mesh.x, n3 * sizeof(float), hipMemcpyDeviceToHost,
+                               s));
+        if (nf_) {
+            PPF_HIP(hipMemcpyAsync(hf.data(), sc.mesh.face, hf.size() * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, s));
+        }
+        PPF_HIP(hipStreamSynchronize(s));
+        piece_morton_order(nv_, hx, hf, order);
+        PPF_HIP(hipMemcpyAsync(order_, order.data(), nv_ * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, s));
+        PPF_HIP(hipStreamSynchronize(s)); // the host order ends here
+        order_changed_ = true;
     }
     return refresh_pins(sc, s);
 }
@@ -476,9 +601,22 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         k_dirichlet_diag<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, ex_, target_, cdiag_, force_);
         PPF_HIP(hipGetLastError());
     }
-    // One solver plan per scene: later iterations only swap the pattern in.
-    rc = plan_ ? pcg_plan_repattern(plan_, nnz, mrp_, mci_, s)
-               : pcg_plan_create(n, nnz, mrp_, mci_, s, &plan_);
+    // One solver plan per scene: later iterations only swap the pattern in
+    // (in the current row order).
+    if (!plan_) {
+        rc = pcg_plan_create(n, nnz, mrp_, mci_, s, &plan_);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+    }
+    if (order_changed_) {
+        rc = pcg_plan_set_order(plan_, order_, s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        order_changed_ = false;
+    }
+    rc = pcg_plan_repattern(plan_, nnz, mrp_, mci_, s);
     if (rc != PPF_OK) {
         return rc;
     }
@@ -487,7 +625,11 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
     if (lpc_) {
         // learned factor; block Jacobi when it cannot be formed or the solve
         // does not reach the tolerance
-        rc = lpc_->prepare(pcg_pattern(plan_), mval_, cdiag_, force_, s);
+        const float *ov = nullptr, *oc = nullptr, *ob = nullptr;
+        rc = pcg_order_system(plan_, mval_, cdiag_, force_, &ov, &oc, &ob, s);
+        if (rc == PPF_OK) {
+            rc = lpc_->prepare(pcg_pattern(plan_), ov, oc, ob, s);
+        }
         if (rc == PPF_OK) {
             rc = lpc_->forward(s);
         }

@@ -34,6 +34,8 @@ class Stepper {
                 const float **cdiag, const float **rhs) const {
         *nnz = sys_nnz_, *rp = mrp_, *ci = mci_, *val = mval_, *cdiag = cdiag_, *rhs = force_;
     }
+    // Row order of the solve (new row k = vertex order[k]), null before create.
+    const uint32_t *solve_order() const { return order_; }
 
   private:
     int newton_iteration(const ppf_scene &sc, const ppf_step_params &p, float dt,
@@ -51,6 +53,9 @@ class Stepper {
     float *ex_ = nullptr, *target_ = nullptr, *dx_ = nullptr, *force_ = nullptr;
     float *cdiag_ = nullptr, *eval_ = nullptr, *cgrad_ = nullptr;
     uint32_t *mask_ = nullptr, *scal_ = nullptr, *h_scal_ = nullptr;
+    // row order of the solve (mesh pieces, each in Morton order), set once
+    uint32_t *order_ = nullptr;
+    bool order_changed_ = false;
     ppf_pair *pairs_ = nullptr;
     uint64_t pair_cap_ = 0;
     uint32_t *mrp_ = nullptr, *mci_ = nullptr, *mcnt_ = nullptr, *scan_ = nullptr;

@@ -258,6 +258,139 @@ __global__ void k_build_ell(uint32_t n, uint32_t W, const uint32_t *__restrict__
     }
 }
 
+// ---- row order (locality) ----------------------------------------------------
+//
+// A plan may solve in another row order (new row k = original row order[k]),
+// e.g. the Morton order of the vertices, so that the rows a mirrored block
+// comes from were read shortly before and its gather hits L2. The upper
+// pattern is rebuilt in the new order (a block whose rows swap order moves to
+// the other row, transposed), and values, right-hand side and iterate are
+// permuted on the way in and out of each solve.
+
+__global__ void k_inverse_order(uint32_t n, const uint32_t *__restrict__ order,
+                                uint32_t *__restrict__ inv) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        inv[order[k]] = k;
+    }
+}
+
+// Entries per new row; flags entries that are not strictly ascending upper
+// columns < n in the original pattern (they are skipped).
+__global__ void k_order_count(uint32_t n, const uint32_t *__restrict__ rp,
+                              const uint32_t *__restrict__ col,
+                              const uint32_t *__restrict__ inv, uint32_t *__restrict__ cnt,
+                              uint32_t *__restrict__ bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const uint32_t a = inv[i];
+    uint32_t prev = 0;
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        const uint32_t j = col[k];
+        if (j >= n || j < i || (k > rp[i] && j <= prev)) {
+            atomicAdd(bad, 1u);
+            continue;
+        }
+        prev = j;
+        atomicAdd(cnt + min(a, inv[j]), 1u);
+    }
+}
+
+__global__ void k_order_fill(uint32_t n, const uint32_t *__restrict__ rp,
+                             const uint32_t *__restrict__ col,
+                             const uint32_t *__restrict__ inv,
+                             const uint32_t *__restrict__ prp, uint32_t *__restrict__ cursor,
+                             uint32_t *__restrict__ pcol, uint32_t *__restrict__ smap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const uint32_t a = inv[i];
+    uint32_t prev = 0;
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        const uint32_t j = col[k];
+        if (j >= n || j < i || (k > rp[i] && j <= prev)) {
+            continue;
+        }
+        prev = j;
+        const uint32_t b = inv[j];
+        const uint32_t r = min(a, b);
+        const uint32_t e = prp[r] + atomicAdd(cursor + r, 1u);
+        pcol[e] = max(a, b);
+        smap[e] = k | (b < a ? kMirror : 0u);
+    }
+}
+
+// Columns of each new row ascending (the fill order is arbitrary; the sort
+// makes the pattern deterministic). Rows are short: insertion sort.
+__global__ void k_order_sort(uint32_t n, const uint32_t *__restrict__ prp,
+                             uint32_t *__restrict__ pcol, uint32_t *__restrict__ smap) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) {
+        return;
+    }
+    const uint32_t lo = prp[r], hi = prp[r + 1];
+    for (uint32_t a = lo + 1; a < hi; ++a) {
+        const uint32_t c = pcol[a], m = smap[a];
+        uint32_t b = a;
+        while (b > lo && pcol[b - 1] > c) {
+            pcol[b] = pcol[b - 1];
+            smap[b] = smap[b - 1];
+            --b;
+        }
+        pcol[b] = c;
+        smap[b] = m;
+    }
+}
+
+// Blocks of the reordered pattern: entry e takes original slot smap[e],
+// transposed when the slot's rows swapped order.
+__global__ void k_order_values(uint32_t nnz, const uint32_t *__restrict__ smap,
+                               const float *__restrict__ val, float *__restrict__ pval) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) {
+        return;
+    }
+    const uint32_t m = smap[e];
+    const float *a = val + 9 * (size_t)(m & ~kMirror);
+    float b[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        b[c] = a[c];
+    }
+    float *o = pval + 9 * (size_t)e;
+    if (m & kMirror) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            o[c] = b[3 * (c % 3) + c / 3];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            o[c] = b[c];
+        }
+    }
+}
+
+// dst[k] = src[order[k]] (w floats per row), or the inverse (scatter).
+template <bool SCATTER>
+__global__ void k_order_rows(uint32_t n, uint32_t w, const uint32_t *__restrict__ order,
+                             const float *__restrict__ src, float *__restrict__ dst) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * w) {
+        return;
+    }
+    const uint32_t k = t / w, c = t % w;
+    const size_t o = (size_t)order[k] * w + c;
+    if (SCATTER) {
+        dst[o] = src[t];
+    } else {
+        dst[t] = src[o];
+    }
+}
+
 // Split form of k_spmv_ell: the same entry list per row (the row's stored
 // slots, then its mirrored slots) and the same lane assignment (lane q sums
 // entries q, q+4, q+8, ... in that order), so the sums are bit-identical to
@@ -949,6 +1082,17 @@ struct PcgPlanImpl {
     std::vector<uint32_t> scene_row; // [nscene + 1] first row of each scene
     uint32_t tile_r1 = 0, tile_r2 = 0; // rows per tile; 0: one resident round
     bool compacted = false;    // tiles cover only the scenes still iterating
+    // row order: new row k is original row order[k] (caller's device array);
+    // null: the original order
+    const uint32_t *order = nullptr;
+    bool order_built = false;  // the pattern was built in the current order
+    void *omem = nullptr;      // inv | prp | cursor (n-sized) | pcd | pb | px
+    uint32_t *inv = nullptr, *prp = nullptr, *cursor = nullptr;
+    float *pcd = nullptr, *pb = nullptr, *px = nullptr;
+    void *emem = nullptr;      // pcol | smap | pval, grown with nnz
+    size_t ecap = 0;
+    uint32_t *pcol = nullptr, *smap = nullptr;
+    float *pval = nullptr;
     void *tmem = nullptr;      // scalars | scene tile starts | tiles | partials
     size_t tmem_cap = 0;
 };
@@ -1018,8 +1162,8 @@ namespace {
 
 // Transpose table, row-length choice of the SpMV form and the ELL table for
 // a pattern. Buffers sized by nnz (or n * row width) grow when needed.
-int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
-                hipStream_t st) {
+int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                       hipStream_t st) {
     const uint32_t n = pl->n;
     Args &h = pl->h;
     if (nnz >= kMirror) {
@@ -1115,6 +1259,67 @@ int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_
         PPF_HIP(hipGetLastError());
     }
     return PPF_OK;
+}
+
+// The plan's pattern: the caller's, or the caller's rebuilt in the plan's row
+// order (the original entries are checked the same way either way).
+int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                hipStream_t st) {
+    if (!pl->order) {
+        return set_pattern_arrays(pl, nnz, rp, col, st);
+    }
+    const uint32_t n = pl->n;
+    if (nnz >= kMirror) {
+        set_error("pcg: %u stored blocks exceed the entry index range", nnz);
+        return PPF_ERR_ARG;
+    }
+    if ((size_t)nnz + 1 > pl->ecap) {
+        if (pl->emem) {
+            PPF_HIP(hipFree(pl->emem));
+            pl->emem = nullptr;
+            pl->ecap = 0;
+        }
+        const size_t cap = (size_t)nnz + nnz / 4 + 256;
+        PPF_HIP(hipMalloc(&pl->emem, cap * (4 + 4 + 36)));
+        pl->ecap = cap;
+        pl->pcol = static_cast<uint32_t *>(pl->emem);
+        pl->smap = pl->pcol + cap;
+        pl->pval = reinterpret_cast<float *>(pl->smap + cap);
+    }
+    uint32_t *flags = pl->flags;
+    uint32_t bad = 0;
+    PPF_HIP(hipMemsetAsync(flags + 1, 0, sizeof(uint32_t), st));
+    k_check_offsets<<<cdiv(n + 1, kNT), kNT, 0, st>>>(n, nnz, rp, flags + 1);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipMemcpyAsync(&bad, flags + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PPF_HIP(hipStreamSynchronize(st));
+    if (bad) {
+        set_error("pcg: row_offset is not a valid CSR offset array (0 .. nnz, non-decreasing)");
+        return PPF_ERR_ARG;
+    }
+    const unsigned gb = cdiv(n, kNT);
+    PPF_HIP(hipMemsetAsync(pl->cursor, 0, (n + 1) * sizeof(uint32_t), st));
+    PPF_HIP(hipMemsetAsync(flags, 0, sizeof(uint32_t), st));
+    k_order_count<<<gb, kNT, 0, st>>>(n, rp, col, pl->inv, pl->cursor, flags);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipMemcpyAsync(&bad, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PPF_HIP(hipStreamSynchronize(st));
+    if (bad) {
+        set_error("pcg: %u pattern entries are not strictly ascending upper-triangle "
+                  "columns < n",
+                  bad);
+        return PPF_ERR_ARG;
+    }
+    PPF_HIP(exclusive_scan_u32(pl->cursor, pl->prp, n + 1, pl->scan_tmp, st));
+    PPF_HIP(hipMemsetAsync(pl->cursor, 0, (n + 1) * sizeof(uint32_t), st));
+    k_order_fill<<<gb, kNT, 0, st>>>(n, rp, col, pl->inv, pl->prp, pl->cursor, pl->pcol,
+                                     pl->smap);
+    PPF_HIP(hipGetLastError());
+    k_order_sort<<<gb, kNT, 0, st>>>(n, pl->prp, pl->pcol, pl->smap);
+    PPF_HIP(hipGetLastError());
+    const int rc = set_pattern_arrays(pl, nnz, pl->prp, pl->pcol, st);
+    pl->order_built = rc == PPF_OK;
+    return rc;
 }
 
 // Tiles of the SpMV (tile1) and of the vector kernels (tile2). Rows per tile
@@ -1434,6 +1639,12 @@ void pcg_plan_destroy(PcgPlanImpl *pl) {
     if (pl->tmem) {
         (void)hipFree(pl->tmem);
     }
+    if (pl->omem) {
+        (void)hipFree(pl->omem);
+    }
+    if (pl->emem) {
+        (void)hipFree(pl->emem);
+    }
     delete pl;
 }
 
@@ -1446,8 +1657,20 @@ int pcg_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
             return rc;
         }
     }
-    h.val = val;
-    h.cdiag = cdiag;
+    const float *ov, *oc, *ob;
+    int rc = pcg_order_system(pl, val, cdiag, nullptr, &ov, &oc, &ob, st);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    const float *xin = x;
+    float *yout = y;
+    if (pl->order) {
+        k_order_rows<false><<<cdiv(3 * pl->n, kNT), kNT, 0, st>>>(pl->n, 3, pl->order, x, pl->px);
+        PPF_HIP(hipGetLastError());
+        xin = pl->px, yout = pl->pb;
+    }
+    h.val = ov;
+    h.cdiag = oc;
     h.b = nullptr;
     h.x = nullptr;
     h.err_hist = nullptr;
@@ -1456,7 +1679,11 @@ int pcg_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
     PPF_HIP(hipMemsetAsync(h.s, 0, pl->nscene * sizeof(Scalars), st));
     k_diag_invert<<<h.G2, kNT, 0, st>>>(pl->d_args);
     PPF_HIP(hipGetLastError());
-    PPF_HIP(launch_spmv<false>(*pl, x, y, st));
+    PPF_HIP(launch_spmv<false>(*pl, xin, yout, st));
+    if (pl->order) {
+        k_order_rows<true><<<cdiv(3 * pl->n, kNT), kNT, 0, st>>>(pl->n, 3, pl->order, pl->pb, y);
+        PPF_HIP(hipGetLastError());
+    }
     PPF_HIP(hipStreamSynchronize(st));
     return PPF_OK;
 }
@@ -1475,8 +1702,18 @@ int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
             return rc;
         }
     }
-    h.val = val;
-    h.cdiag = cdiag;
+    const float *ov, *oc, *ob;
+    int rc = pcg_order_system(pl, val, cdiag, nullptr, &ov, &oc, &ob, st);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    if (pl->order) { // time the plan's own kernel on the reordered vectors
+        k_order_rows<false><<<cdiv(3 * pl->n, kNT), kNT, 0, st>>>(pl->n, 3, pl->order, x, pl->px);
+        PPF_HIP(hipGetLastError());
+        x = pl->px, y = pl->pb;
+    }
+    h.val = ov;
+    h.cdiag = oc;
     h.b = nullptr;
     h.x = nullptr;
     h.err_hist = nullptr;
@@ -1522,6 +1759,24 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
         if (rc != PPF_OK) {
             return rc;
         }
+    }
+    if (pl->order && ns > 1) {
+        set_error("pcg_solve: a reordered plan holds one scene");
+        return PPF_ERR_ARG;
+    }
+    {
+        const float *ov, *oc, *ob;
+        int rc = pcg_order_system(pl, val, cdiag, b, &ov, &oc, &ob, st);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        val = ov, cdiag = oc, b = ob;
+    }
+    float *const x_caller = x;
+    if (pl->order) {
+        k_order_rows<false><<<cdiv(3 * pl->n, kNT), kNT, 0, st>>>(pl->n, 3, pl->order, x, pl->px);
+        PPF_HIP(hipGetLastError());
+        x = pl->px;
     }
     h.val = val;
     h.cdiag = cdiag;
@@ -1606,6 +1861,12 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
         }
         PPF_HIP(hipGraphLaunch(pl->exec, st));
     }
+    if (pl->order) {
+        k_order_rows<true><<<cdiv(3 * pl->n, kNT), kNT, 0, st>>>(pl->n, 3, pl->order, x,
+                                                                 x_caller);
+        PPF_HIP(hipGetLastError());
+        PPF_HIP(hipStreamSynchronize(st));
+    }
     for (uint32_t c = 0; c < ns; ++c) {
         const Scalars &sc = pl->h_scal[c];
         res[c].iter = sc.rep_iter;
@@ -1683,6 +1944,58 @@ int pcg_plan_autotune(PcgPlanImpl *pl, const float *val, const float *cdiag, con
         if (best_ms) *best_ms = best;
     }
     return rc;
+}
+
+int pcg_plan_set_order(PcgPlanImpl *pl, const uint32_t *order, hipStream_t st) {
+    if (!pl || pl->nscene != 1) {
+        set_error("pcg_plan_set_order: one-scene plans only");
+        return PPF_ERR_ARG;
+    }
+    pl->order = order;
+    pl->order_built = false;
+    if (!order) {
+        return PPF_OK;
+    }
+    const uint32_t n = pl->n;
+    if (!pl->omem) {
+        const size_t bytes = carve_bytes(n, 4) + 2 * carve_bytes(n + 1, 4) +
+                             carve_bytes(9 * (size_t)n, 4) + 2 * carve_bytes(3 * (size_t)n, 4);
+        PPF_HIP(hipMalloc(&pl->omem, bytes));
+        char *cur = static_cast<char *>(pl->omem);
+        pl->inv = carve<uint32_t>(cur, n);
+        pl->prp = carve<uint32_t>(cur, n + 1);
+        pl->cursor = carve<uint32_t>(cur, n + 1);
+        pl->pcd = carve<float>(cur, 9 * (size_t)n);
+        pl->pb = carve<float>(cur, 3 * (size_t)n);
+        pl->px = carve<float>(cur, 3 * (size_t)n);
+    }
+    k_inverse_order<<<cdiv(n, kNT), kNT, 0, st>>>(n, order, pl->inv);
+    PPF_HIP(hipGetLastError());
+    return PPF_OK;
+}
+
+bool pcg_plan_ordered(const PcgPlanImpl *pl) { return pl->order != nullptr; }
+
+int pcg_order_system(PcgPlanImpl *pl, const float *val, const float *cdiag, const float *b,
+                     const float **oval, const float **ocdiag, const float **ob,
+                     hipStream_t st) {
+    if (!pl->order) {
+        *oval = val, *ocdiag = cdiag, *ob = b;
+        return PPF_OK;
+    }
+    if (!pl->order_built) {
+        set_error("pcg: a new row order takes effect at the next pattern (repattern first)");
+        return PPF_ERR_ARG;
+    }
+    const uint32_t n = pl->n;
+    k_order_values<<<cdiv(pl->nnz, kNT), kNT, 0, st>>>(pl->nnz, pl->smap, val, pl->pval);
+    k_order_rows<false><<<cdiv(9 * n, kNT), kNT, 0, st>>>(n, 9, pl->order, cdiag, pl->pcd);
+    if (b) {
+        k_order_rows<false><<<cdiv(3 * n, kNT), kNT, 0, st>>>(n, 3, pl->order, b, pl->pb);
+    }
+    PPF_HIP(hipGetLastError());
+    *oval = pl->pval, *ocdiag = pl->pcd, *ob = b ? pl->pb : nullptr;
+    return PPF_OK;
 }
 
 int pcg_plan_set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st) {

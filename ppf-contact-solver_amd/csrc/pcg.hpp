@@ -17,6 +17,18 @@ int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const u
                            uint32_t nscene, const uint32_t *scene_row, hipStream_t st,
                            PcgPlanImpl **out);
 uint32_t pcg_plan_scenes(const PcgPlanImpl *pl);
+// Solve in another row order: new row k is original row order[k] (device
+// array of n, owned by the caller and left untouched until the order changes).
+// Takes effect at the next pattern (pcg_plan_repattern); values, right-hand
+// side and iterate stay in the original order at every entry point.
+int pcg_plan_set_order(PcgPlanImpl *pl, const uint32_t *order, hipStream_t st);
+bool pcg_plan_ordered(const PcgPlanImpl *pl);
+// The system in the plan's row order (the caller's arrays when the plan keeps
+// the original order): what a factor computed on pcg_pattern() must see.
+// b may be null.
+int pcg_order_system(PcgPlanImpl *pl, const float *val, const float *cdiag, const float *b,
+                     const float **oval, const float **ocdiag, const float **ob,
+                     hipStream_t st);
 // Rows per tile of the SpMV and of the vector kernels (every scene cut into
 // tiles; 0, 0: one resident round over a one-scene plan).
 int pcg_plan_set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st);

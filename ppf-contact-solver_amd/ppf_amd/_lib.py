@@ -31,6 +31,8 @@ _PROTOS = {
     "ppf_pcg_plan_create_batch": (ctypes.c_int, [u32, u32, vp, vp, u32, vp, vp,
                                                  ctypes.POINTER(vp)]),
     "ppf_pcg_plan_set_tiles": (ctypes.c_int, [vp, u32, u32, vp]),
+    "ppf_pcg_plan_set_order": (ctypes.c_int, [vp, vp, vp]),
+    "ppf_stepper_solve_order": (ctypes.c_int, [vp, vp, vp]),
     "ppf_pcg_plan_autotune": (ctypes.c_int, [vp, vp, vp, vp, f32, vp, ctypes.POINTER(f32), vp]),
     "ppf_pcg_solve_batch": (ctypes.c_int, [vp, vp, vp, vp, vp, f32, u32, vp, vp]),
     "ppf_pcg_plan_repattern": (ctypes.c_int, [vp, u32, vp, vp, vp]),

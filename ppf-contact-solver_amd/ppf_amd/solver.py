@@ -58,6 +58,16 @@ class BlockJacobiPCG:
         self.nnz = nnz
         self._keep = (row_offset, col_index)
 
+    def set_order(self, order):
+        """Solve in another row order (new row k = original row order[k], an
+        int32 device tensor kept alive here); takes effect at the next
+        repattern. None: the original order."""
+        if order is not None:
+            _need(order, torch.int32, self.n, "order")
+        self._order = order
+        check(self.lib.ppf_pcg_plan_set_order(self._h, _ptr(order) if order is not None
+                                              else None, _stream()), "ppf_pcg_plan_set_order")
+
     def close(self):
         if getattr(self, "_h", None):
             self.lib.ppf_pcg_plan_destroy(self._h)

@@ -124,6 +124,16 @@ class Stepper:
         check(self.lib.ppf_stepper_set_preconditioner(self._h, lpc._h if lpc else None),
               "ppf_stepper_set_preconditioner")
 
+    def solve_order(self):
+        """Row order of the stepper's solves (int32 device tensor, new row k =
+        vertex order[k])."""
+        dev = self.t["mesh"]["x"].device
+        n = self.t["mesh"]["x"].numel() // 3
+        out = torch.empty(n, dtype=torch.int32, device=dev)
+        check(self.lib.ppf_stepper_solve_order(self._h, _ptr(out), _stream()),
+              "ppf_stepper_solve_order")
+        return out
+
     def system(self):
         """Copy of the last Newton system solved: dict of device tensors
         row_offset (n+1, int32), col_index (nnz, int32), value (9 nnz),

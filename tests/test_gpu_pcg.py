@@ -208,6 +208,82 @@ def test_plan_repattern_matches_fresh_plan(dev):
     plan.close()
 
 
+def permuted_upper(s, order):
+    """P A P^T of an upper-stored system on the host: new row k = old row
+    order[k]; a block whose rows swap order moves to the other row,
+    transposed; columns ascending per row."""
+    rp = np.asarray(s["row_offset"], np.int64)
+    ci = np.asarray(s["col_index"], np.int64)
+    n = len(rp) - 1
+    inv = np.empty(n, np.int64)
+    inv[order] = np.arange(n)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    a, b = inv[rows], inv[ci]
+    r, c = np.minimum(a, b), np.maximum(a, b)
+    blk = np.asarray(s["value"], np.float32).reshape(-1, 3, 3)
+    blk = np.where((b < a)[:, None, None], blk.transpose(0, 2, 1), blk)
+    key = np.lexsort((c, r))
+    prp = np.zeros(n + 1, np.int64)
+    np.add.at(prp, r + 1, 1)
+    prp = np.cumsum(prp)
+    cd = np.asarray(s["cdiag"], np.float32).reshape(-1, 9)[order]
+    bb = np.asarray(s["b"], np.float32).reshape(-1, 3)[order]
+    return dict(row_offset=prp.astype(np.int32), col_index=c[key].astype(np.int32),
+                value=np.ascontiguousarray(blk[key]).reshape(-1), cdiag=cd.reshape(-1),
+                b=bb.reshape(-1))
+
+
+@pytest.mark.parametrize("kind", ["grid", "longrows"])
+def test_plan_row_order(dev, kind):
+    """A plan solving in another row order gives, in the original order,
+    bitwise the solution of the host-permuted system solved by a plain plan;
+    the identity order changes nothing; the order takes effect at the next
+    repattern (solving before it is refused)."""
+    from ppf_amd._lib import PPFError
+    s = (scenes.grid_spd_system(20, seed_matrix=3, seed_rhs=4) if kind == "grid"
+         else scenes.long_row_spd_system(n=300, seed=9))
+    n = len(s["row_offset"]) - 1
+    t = to_dev(s, dev)
+    order = np.random.default_rng(5).permutation(n).astype(np.int32)
+    plan = BlockJacobiPCG(t["row_offset"], t["col_index"])
+    od = torch.from_numpy(order).to(dev)
+    plan.set_order(od)
+    x = torch.zeros_like(t["b"])
+    with pytest.raises(PPFError):
+        plan.solve(t["value"], t["cdiag"], t["b"], x)
+    plan.repattern(t["row_offset"], t["col_index"])
+    r = plan.solve(t["value"], t["cdiag"], t["b"], x)
+    y = plan.spmv(t["value"], t["cdiag"], t["b"])
+    ps = permuted_upper(s, order)
+    pt = to_dev(ps, dev)
+    pplan = BlockJacobiPCG(pt["row_offset"], pt["col_index"])
+    px = torch.zeros_like(pt["b"])
+    pr = pplan.solve(pt["value"], pt["cdiag"], pt["b"], px)
+    pplan.close()
+    print(f"{kind}: ordered {r.iter} it, host-permuted {pr.iter} it")
+    assert r.status == pr.status == 0 and r.iter == pr.iter
+    xo = x.cpu().numpy().reshape(-1, 3)
+    xp = px.cpu().numpy().reshape(-1, 3)
+    assert np.array_equal(xo[order].view(np.uint32), xp.view(np.uint32))
+    # the SpMV in the original order: y(b) = M b, the host-permuted plan's
+    # product of the permuted b, scattered back
+    bp = np.asarray(s["b"], np.float32).reshape(-1, 3)[order].reshape(-1)
+    pplan = BlockJacobiPCG(pt["row_offset"], pt["col_index"])
+    yp = pplan.spmv(pt["value"], pt["cdiag"], torch.from_numpy(bp).to(dev)).cpu().numpy()
+    pplan.close()
+    assert np.array_equal(y.cpu().numpy().reshape(-1, 3)[order].view(np.uint32),
+                          yp.reshape(-1, 3).view(np.uint32))
+    # identity order: bitwise the unordered plan
+    plan.set_order(torch.arange(n, dtype=torch.int32, device=dev))
+    plan.repattern(t["row_offset"], t["col_index"])
+    xi = torch.zeros_like(t["b"])
+    ri = plan.solve(t["value"], t["cdiag"], t["b"], xi)
+    fresh = solve_gpu(s, dev, hist=False)
+    assert ri.iter == fresh["iter"]
+    assert np.array_equal(xi.cpu().numpy().view(np.uint32), fresh["x"].view(np.uint32))
+    plan.close()
+
+
 def test_plan_rejects_malformed_pattern(dev):
     from ppf_amd._lib import PPFError
     ro = torch.tensor([0, 2, 3], dtype=torch.int32, device=dev)
