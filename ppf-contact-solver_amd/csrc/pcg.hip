@@ -12,15 +12,20 @@
 //   * the operator is the reference's split form, an upper-triangle block CSR
 //     applied both ways plus a per-row diagonal. The diagonal blocks are merged
 //     once per solve into D_i = B_ii + C_i, and the transpose of the strict
-//     upper triangle is walked through a per-column (row, slot) table, so every
-//     stored block is read from HBM once per SpMV and re-read from the L2 of
-//     the same XCD (rows are dealt to XCDs in contiguous ranges).
+//     upper triangle is walked through a per-column (row, slot) table. Rows
+//     are dealt to XCDs in contiguous ranges, and a plan may solve in a
+//     locality row order, so that a mirrored block's second read mostly hits
+//     the L2 of the XCD that read it as a stored block.
 //   * one iteration = 3 kernels: SpMV + p.Ap + sum|terms|; the fused x/r/z
-//     update with the r.z and |r|_1 partials; p = z + beta p. Each kernel
-//     writes one partial per workgroup; every workgroup of the next kernel
-//     reduces those partials itself in a fixed order, so the scalars are
-//     bit-identical everywhere with no atomics and no grid barrier, and every
-//     run is deterministic.
+//     update with the r.z and |r|_1 partials; p = z + beta p (5 with the
+//     factor preconditioner: G r and G^T (G r) between the last two). Each
+//     kernel writes one partial per workgroup tile; every workgroup of the
+//     next kernel reduces its scene's partials itself in a fixed order, so
+//     the scalars are bit-identical everywhere with no atomics and no grid
+//     barrier, and every run is deterministic.
+//   * a plan may hold many independent systems (scenes) stacked
+//     block-diagonally: tiles never straddle scenes and each scene keeps its
+//     own scalars and stopping rule.
 //   * iterations are captured 8 at a time into a hipGraph; once the device
 //     bookkeeping marks the solve done, the remaining replays exit at entry.
 #include "common.hpp"
