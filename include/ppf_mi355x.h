@@ -104,7 +104,8 @@ int ppf_lpc_set_weights(ppf_lpc *h, const float *host_weights, uint32_t count, v
 int ppf_lpc_get_weights(ppf_lpc *h, float *host_weights, uint32_t count, void *stream);
 
 /* Graph, features, network and factor for the system (plan's pattern,
- * d_value, d_cdiag, right-hand side d_b). */
+ * d_value, d_cdiag, right-hand side d_b; in the plan's row order when it has
+ * one, ppf_pcg_plan_set_order). */
 int ppf_lpc_factor(ppf_lpc *h, ppf_pcg_plan *plan, const float *d_value,
                    const float *d_cdiag, const float *d_b, void *stream);
 
@@ -115,7 +116,8 @@ int ppf_lpc_outputs(ppf_lpc *h, float *d_t, float *d_u, float *d_gd, float *d_gl
                     void *stream);
 
 /* Loss and weight gradient (host, ppf_lpc_weight_count floats) for the
- * probes d_probes (nprobe x n x 3), after ppf_lpc_factor on the same system. */
+ * probes d_probes (nprobe x n x 3), after ppf_lpc_factor on the same system.
+ * Plans in the original row order only. */
 int ppf_lpc_gradient(ppf_lpc *h, ppf_pcg_plan *plan, const float *d_value,
                      const float *d_cdiag, const float *d_probes, uint32_t nprobe,
                      float *loss, float *host_grad, void *stream);

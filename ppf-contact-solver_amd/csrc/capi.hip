@@ -194,7 +194,12 @@ int ppf_lpc_factor(ppf_lpc *h, ppf_pcg_plan *plan, const float *d_value,
         return PPF_ERR_ARG;
     }
     const hipStream_t s = (hipStream_t)stream;
-    int rc = h->impl.prepare(pcg_pattern(plan->impl), d_value, d_cdiag, d_b, s);
+    // the factor lives in the plan's row order
+    const float *ov = nullptr, *oc = nullptr, *ob = nullptr;
+    int rc = pcg_order_system(plan->impl, d_value, d_cdiag, d_b, &ov, &oc, &ob, s);
+    if (rc == PPF_OK) {
+        rc = h->impl.prepare(pcg_pattern(plan->impl), ov, oc, ob, s);
+    }
     return rc == PPF_OK ? h->impl.forward(s) : rc;
 }
 
@@ -218,6 +223,10 @@ int ppf_lpc_gradient(ppf_lpc *h, ppf_pcg_plan *plan, const float *d_value,
                      float *loss, float *host_grad, void *stream) {
     if (!h || !plan || !d_value || !d_cdiag || !d_probes || !loss || !h->impl.Gd()) {
         set_error("ppf_lpc_gradient: invalid argument (or no factor computed)");
+        return PPF_ERR_ARG;
+    }
+    if (pcg_plan_ordered(plan->impl)) {
+        set_error("ppf_lpc_gradient: training runs on plans in the original row order");
         return PPF_ERR_ARG;
     }
     const hipStream_t s = (hipStream_t)stream;

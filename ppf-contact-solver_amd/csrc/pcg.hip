@@ -826,13 +826,16 @@ __device__ inline void load_g16(gptr<const uint16_t> g, size_t blk, float *m) {
 // The factor passes share the SpMV's shape: P lanes per row, lane q taking
 // entries q, q + P, ... of the row, M of them per pass with every load of the
 // pass issued before any is used (one dependent round trip per pass), the
-// lanes' sums combined by shuffles.
-constexpr uint32_t kFacP = 2, kFacM = 4;
+// lanes' sums combined by shuffles. 4 x 4 measured fastest for both passes on
+// the C3 systems: 188-193 us per factor-PCG iteration, against 211 us at
+// 2 x 4, 211 at 4 x 2, 218 at 8 x 2, 254 at 2 x 2, and 197 us with G^T y
+// alone at 2 x 4.
+constexpr uint32_t kFacGP = 4, kFacGM = 4, kFacTP = 4, kFacTM = 4;
 
 // gy = G r: row j's entries are G_jj, then G_ji for its mirrored entries
 // (slot s = (i, j), i < j), read in order from the row-ordered copy Gr.
+template <uint32_t P, uint32_t M>
 __global__ __launch_bounds__(kNT) void k_fac_g(const Args *__restrict__ A, int init) {
-    constexpr uint32_t P = kFacP, M = kFacM;
     if (!init && as_global(A->s)->done) {
         return;
     }
@@ -889,8 +892,9 @@ __global__ __launch_bounds__(kNT) void k_fac_g(const Args *__restrict__ A, int i
 // z = G^T gy: row i's entries are G_ii (transposed) and, for its stored
 // off-diagonal slots s = (i, j), G_ji^T gy_j (the stored diagonal slot
 // carries no factor block). Then the r.z partials (and p = z when init).
+template <uint32_t P, uint32_t M>
 __global__ __launch_bounds__(kNT) void k_fac_gt(const Args *__restrict__ A, int init) {
-    constexpr uint32_t P = kFacP, M = kFacM, kRows = kNT / kFacP;
+    constexpr uint32_t kRows = kNT / P;
     __shared__ float lds[kNT / kWave];
     const uint32_t L = blockIdx.x;
     const Tile T = load_tile(A->tile2, L);
@@ -1144,6 +1148,12 @@ hipError_t launch_spmv(const PcgPlanImpl &pl, const float *v, float *y,
     return hipGetLastError();
 }
 
+// The two factor passes (G r, then G^T (G r) with the r.z partials).
+void launch_fac(const PcgPlanImpl &pl, int init, hipStream_t st) {
+    k_fac_g<kFacGP, kFacGM><<<cdiv(pl.h.n, kNT / kFacGP), kNT, 0, st>>>(pl.d_args, init);
+    k_fac_gt<kFacTP, kFacTM><<<pl.h.G2, kNT, 0, st>>>(pl.d_args, init);
+}
+
 hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
     const Args &h = pl.h;
     hipError_t e = launch_spmv<true>(pl, h.p, h.Ap, st);
@@ -1152,8 +1162,7 @@ hipError_t launch_iteration(const PcgPlanImpl &pl, uint32_t k, hipStream_t st) {
     }
     if (h.Gd) {
         k_update<true><<<h.G2, kNT, 0, st>>>(pl.d_args, k);
-        k_fac_g<<<cdiv(h.n, kNT / kFacP), kNT, 0, st>>>(pl.d_args, 0);
-        k_fac_gt<<<h.G2, kNT, 0, st>>>(pl.d_args, 0);
+        launch_fac(pl, 0, st);
     } else {
         k_update<false><<<h.G2, kNT, 0, st>>>(pl.d_args, k);
     }
@@ -1817,8 +1826,7 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
     PPF_HIP(launch_spmv<false>(*pl, x, h.Ap, st));
     if (Gd) {
         k_init_residual<true><<<h.G2, kNT, 0, st>>>(pl->d_args);
-        k_fac_g<<<cdiv(pl->n, kNT / kFacP), kNT, 0, st>>>(pl->d_args, 1);
-        k_fac_gt<<<h.G2, kNT, 0, st>>>(pl->d_args, 1);
+        launch_fac(*pl, 1, st);
     } else {
         k_init_residual<false><<<h.G2, kNT, 0, st>>>(pl->d_args);
     }
