@@ -7,7 +7,9 @@
 // edges and faces) and edge-edge (edge < index, no shared vertex). Queries run
 // in BVH leaf (Morton) order, one thread each, with the traversal stack in LDS.
 // Children are tested before they are pushed. Two passes -- count, exclusive
-// scan, fill -- give every query a contiguous, deterministic output range.
+// scan, fill -- give every query a contiguous, deterministic output range. The
+// count pass keeps each query's first pairs; the fill pass copies those and
+// walks the trees again only for queries with more.
 #include "bvh_query.hpp"
 #include "ccd.hpp"
 #include "distance.hpp"
@@ -22,6 +24,10 @@ constexpr int kQT = kQueryThreads;
 constexpr int kStack = kQueryStack;
 using Q = QBox;
 
+// Pairs the count pass keeps per query. The fill pass then copies them
+// instead of walking the tree again; only queries with more pairs walk again.
+constexpr uint32_t kStash = 4;
+
 struct Ctx {
     ppf_contact_mesh m;
     BvhView fb, eb, vb;
@@ -30,6 +36,9 @@ struct Ctx {
     ppf_pair *out;
     uint64_t cap;
     uint32_t *overflow; // [0] stack overflow, [1] output overflow
+    ppf_pair *stash;    // kStash per query, filled by the count pass (may be null)
+    uint32_t *redo;     // fill pass: only these leaf positions (null: all)
+    uint32_t nredo;
 };
 
 __device__ inline f3 X(const ppf_contact_mesh &m, uint32_t v) {
@@ -42,8 +51,11 @@ __device__ inline bool vcol(const ppf_contact_mesh &m, uint32_t v) {
     return (m.vflag[v] & 2u) != 0;
 }
 
+// The count pass (FILL false) counts the query's pairs in `slot` and keeps
+// its first kStash in the stash; the fill pass writes them from the query's
+// offset on.
 template <bool FILL>
-__device__ inline void emit(const Ctx &c, uint64_t &slot, uint32_t kind,
+__device__ inline void emit(const Ctx &c, uint64_t &slot, uint32_t qpos, uint32_t kind,
                             uint32_t a, uint32_t b, uint32_t count) {
     if (FILL) {
         if (slot < c.cap) {
@@ -51,16 +63,34 @@ __device__ inline void emit(const Ctx &c, uint64_t &slot, uint32_t kind,
         } else {
             atomicOr(c.overflow + 1, 1u);
         }
-        ++slot;
+    } else if (c.stash && slot < kStash) {
+        c.stash[(size_t)qpos * kStash + slot] = ppf_pair{kind, a, b, count};
     }
+    ++slot;
+}
+
+// Leaf position this thread queries: one per leaf, or (a fill pass after a
+// stashed count) the listed ones. False: none.
+template <bool FILL>
+__device__ inline bool query_pos(const Ctx &c, uint32_t n, uint32_t &leaf_pos) {
+    const uint32_t t = blockIdx.x * kQT + threadIdx.x;
+    if (!FILL || !c.redo) {
+        leaf_pos = t;
+        return t < n;
+    }
+    if (t >= c.nredo) {
+        return false;
+    }
+    leaf_pos = c.redo[t];
+    return true;
 }
 
 template <bool FILL>
 __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
     __shared__ uint32_t stk_all[kStack * kQT];
-    const uint32_t leaf_pos = blockIdx.x * kQT + threadIdx.x;
+    uint32_t leaf_pos;
     const ppf_contact_mesh &m = c.m;
-    if (leaf_pos >= c.vb.n) {
+    if (!query_pos<FILL>(c, c.vb.n, leaf_pos)) {
         return;
     }
     uint32_t *stk = stk_all + threadIdx.x;
@@ -92,7 +122,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
         const float gh = 0.5f * (m.v_ghat[i] + m.f_ghat[fi]);
         const float r = gh + off;
         if (dot3(e, e) < r * r) {
-            emit<FILL>(c, slot, 0, i, fi, 1);
+            emit<FILL>(c, slot, leaf_pos, 0, i, fi, 1);
             return true;
         }
         return false;
@@ -141,7 +171,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
             count = 1;
         }
         if (count) {
-            emit<FILL>(c, slot, 1u | (nofric << 8), i, ei, count);
+            emit<FILL>(c, slot, leaf_pos, 1u | (nofric << 8), i, ei, count);
             return true;
         }
         return false;
@@ -197,7 +227,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
             count = 1;
         }
         if (count) {
-            emit<FILL>(c, slot, 2u | (nofric << 8), i, j, count);
+            emit<FILL>(c, slot, leaf_pos, 2u | (nofric << 8), i, j, count);
             return true;
         }
         return false;
@@ -211,9 +241,9 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
 template <bool FILL>
 __global__ __launch_bounds__(kQT) void k_edge_queries(Ctx c) {
     __shared__ uint32_t stk_all[kStack * kQT];
-    const uint32_t leaf_pos = blockIdx.x * kQT + threadIdx.x;
+    uint32_t leaf_pos;
     const ppf_contact_mesh &m = c.m;
-    if (leaf_pos >= c.eb.n) {
+    if (!query_pos<FILL>(c, c.eb.n, leaf_pos)) {
         return;
     }
     uint32_t *stk = stk_all + threadIdx.x;
@@ -257,13 +287,58 @@ __global__ __launch_bounds__(kQT) void k_edge_queries(Ctx c) {
         const float gh = 0.5f * (m.e_ghat[ei] + m.e_ghat[ej]);
         const float r = gh + off;
         if (dot3(e, e) < r * r) {
-            emit<FILL>(c, slot, 3, ei, ej, 1);
+            emit<FILL>(c, slot, qpos, 3, ei, ej, 1);
             return true;
         }
         return false;
     });
     if (!FILL) {
         c.cnt[qpos] = total;
+    }
+}
+
+// Copies the stashed pairs of every query with at most kStash pairs to its
+// output range (one thread per stash entry) and lists the other queries,
+// vertex leaves from redo[0] on, edge leaves from redo[nv] on, counted in
+// overflow[2] and overflow[3]. The list order varies from run to run; the
+// output does not (every query writes its own range).
+__global__ __launch_bounds__(256) void k_unstash(Ctx c) {
+    const uint32_t nq = c.vb.n + c.eb.n;
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t q = (uint32_t)(t / kStash), k = (uint32_t)(t % kStash);
+    uint32_t n = 0;
+    if (q < nq) {
+        n = c.cnt[q];
+        if (k < n && n <= kStash) {
+            const uint64_t slot = (uint64_t)c.ofs[q] + k;
+            if (slot < c.cap) {
+                c.out[slot] = c.stash[t];
+            } else {
+                atomicOr(c.overflow + 1, 1u);
+            }
+        }
+    }
+    // one append per wave and kind
+    const bool redo = q < nq && k == 0 && n > kStash;
+    const bool edge = q >= c.vb.n;
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int kind = 0; kind < 2; ++kind) {
+        const bool mine = redo && edge == (kind == 1);
+        const uint64_t m = __ballot(mine);
+        if (m == 0) {
+            continue;
+        }
+        const uint32_t first = (uint32_t)__builtin_ctzll(m);
+        uint32_t at = 0;
+        if (lane == first) {
+            at = atomicAdd(c.overflow + 2 + kind, (uint32_t)__popcll(m));
+        }
+        at = __shfl(at, (int)first);
+        if (mine) {
+            uint32_t *list = kind ? c.redo + c.vb.n : c.redo;
+            list[at + lanes_below(m)] = kind ? q - c.vb.n : q;
+        }
     }
 }
 
@@ -418,6 +493,8 @@ Narrow::~Narrow() {
     if (cnt_) (void)hipFree(cnt_);
     if (ofs_) (void)hipFree(ofs_);
     if (scratch_) (void)hipFree(scratch_);
+    if (stash_) (void)hipFree(stash_);
+    if (redo_) (void)hipFree(redo_);
     if (flags_) (void)hipFree(flags_);
     if (h_flags_) (void)hipHostFree(h_flags_);
 }
@@ -429,14 +506,21 @@ int Narrow::reserve(uint32_t queries) {
     if (cnt_) (void)hipFree(cnt_);
     if (ofs_) (void)hipFree(ofs_);
     if (scratch_) (void)hipFree(scratch_);
-    cnt_ = ofs_ = scratch_ = nullptr;
+    if (stash_) (void)hipFree(stash_);
+    if (redo_) (void)hipFree(redo_);
+    cnt_ = ofs_ = scratch_ = redo_ = nullptr;
+    stash_ = nullptr;
+    qcap_ = 0;
     PPF_HIP(hipMalloc(&cnt_, (queries + 1) * sizeof(uint32_t)));
     PPF_HIP(hipMalloc(&ofs_, (queries + 1) * sizeof(uint32_t)));
     PPF_HIP(hipMalloc(&scratch_, scan_scratch_words(queries + 1) * sizeof(uint32_t)));
+    PPF_HIP(hipMalloc(&stash_, (size_t)queries * kStash * sizeof(ppf_pair)));
+    PPF_HIP(hipMalloc(&redo_, (size_t)queries * sizeof(uint32_t)));
     if (!flags_) {
         PPF_HIP(hipMalloc(&flags_, 4 * sizeof(uint32_t)));
+        // four flag words, then the pair total
         PPF_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_flags_),
-                              4 * sizeof(uint32_t), hipHostMallocDefault));
+                              8 * sizeof(uint32_t), hipHostMallocDefault));
     }
     qcap_ = queries;
     return PPF_OK;
@@ -483,6 +567,7 @@ int Narrow::proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,
         return rc;
     }
     PPF_HIP(hipMemsetAsync(flags_, 0, 4 * sizeof(uint32_t), s));
+    const bool fill = out && cap;
     Ctx c;
     c.m = m;
     c.fb = face_.view();
@@ -493,6 +578,9 @@ int Narrow::proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,
     c.out = out;
     c.cap = cap;
     c.overflow = flags_;
+    c.stash = fill ? stash_ : nullptr;
+    c.redo = nullptr;
+    c.nredo = 0;
     if (m.nv) {
         k_vertex_queries<false><<<cdiv(m.nv, kQT), kQT, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
@@ -503,24 +591,38 @@ int Narrow::proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,
     }
     PPF_HIP(hipMemsetAsync(cnt_ + nq, 0, sizeof(uint32_t), s));
     PPF_HIP(exclusive_scan_u32(cnt_, ofs_, nq + 1, scratch_, s));
-    uint32_t total = 0;
-    PPF_HIP(hipMemcpyAsync(&total, ofs_ + nq, sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, s));
-    PPF_HIP(hipStreamSynchronize(s));
-    *count = total;
-    if (out && cap) {
-        if (m.nv) {
-            k_vertex_queries<true><<<cdiv(m.nv, kQT), kQT, 0, s>>>(c);
-            PPF_HIP(hipGetLastError());
-        }
-        if (m.ne) {
-            k_edge_queries<true><<<cdiv(m.ne, kQT), kQT, 0, s>>>(c);
-            PPF_HIP(hipGetLastError());
-        }
+    if (fill && nq) {
+        c.redo = redo_;
+        const uint64_t entries = (uint64_t)nq * kStash;
+        k_unstash<<<(uint32_t)((entries + 255) / 256), 256, 0, s>>>(c);
+        PPF_HIP(hipGetLastError());
     }
+    // total and the redo counts in one readback
+    PPF_HIP(hipMemcpyAsync(h_flags_ + 4, ofs_ + nq, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s));
     PPF_HIP(hipMemcpyAsync(h_flags_, flags_, 4 * sizeof(uint32_t),
                            hipMemcpyDeviceToHost, s));
     PPF_HIP(hipStreamSynchronize(s));
+    *count = h_flags_[4];
+    if (fill) {
+        const uint32_t nv_redo = h_flags_[2], ne_redo = h_flags_[3];
+        if (nv_redo) {
+            Ctx cv = c;
+            cv.nredo = nv_redo;
+            k_vertex_queries<true><<<cdiv(nv_redo, kQT), kQT, 0, s>>>(cv);
+            PPF_HIP(hipGetLastError());
+        }
+        if (ne_redo) {
+            Ctx ce = c;
+            ce.redo = redo_ + m.nv;
+            ce.nredo = ne_redo;
+            k_edge_queries<true><<<cdiv(ne_redo, kQT), kQT, 0, s>>>(ce);
+            PPF_HIP(hipGetLastError());
+        }
+        PPF_HIP(hipMemcpyAsync(h_flags_, flags_, 4 * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, s));
+        PPF_HIP(hipStreamSynchronize(s));
+    }
     if (h_flags_[0]) {
         set_error("proximity: a BVH traversal exceeded its %d-entry stack", kStack);
         return PPF_ERR_CAPACITY;

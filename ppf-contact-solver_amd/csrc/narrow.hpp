@@ -33,6 +33,8 @@ class Narrow {
     uint32_t static_nv_ = 0;
     uint32_t qcap_ = 0;
     uint32_t *cnt_ = nullptr, *ofs_ = nullptr, *scratch_ = nullptr;
+    ppf_pair *stash_ = nullptr;  // the count pass's first pairs of every query
+    uint32_t *redo_ = nullptr;   // queries with more pairs than the stash holds
     uint32_t *flags_ = nullptr, *h_flags_ = nullptr;
 };
 

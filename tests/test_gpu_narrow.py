@@ -66,6 +66,35 @@ def test_pairs_match_oracle(dev, name):
     nph.close()
 
 
+def query_sizes(p):
+    """Pairs per query: vertex queries emit kinds 0-2, edge queries kind 3."""
+    key = (p["kind"] & 255 == 3).astype(np.int64) << 32 | p["a"].astype(np.int64)
+    return np.unique(key, return_counts=True)[1]
+
+
+def test_short_and_long_queries(dev):
+    """The count pass keeps each query's first 4 pairs and the fill pass copies
+    them; queries with more pairs walk the trees again. Both kinds of query
+    occur here, and a buffer smaller than the pair count receives exactly the
+    first pairs of the full result."""
+    m = SCENES["sheets3"]()
+    sizes = query_sizes(oracle.proximity(m))
+    assert (sizes <= 4).any() and (sizes > 4).any(), np.bincount(sizes)
+    t = mesh_to_device(m, dev)
+    nph = NarrowPhase()
+    nph.build(t)
+    full = pairs_to_numpy(nph.proximity(t))
+    for cap in (1, len(full) // 3, len(full) - 1):
+        out = torch.full((cap, 4), -1, dtype=torch.int32, device=dev)
+        with pytest.raises(Exception, match="too small"):
+            nph.proximity(t, out)
+        assert np.array_equal(pairs_to_numpy(out), full[:cap]), cap
+    # a count-only call between fills leaves the next fill unchanged
+    assert nph.count(t) == len(full)
+    assert np.array_equal(pairs_to_numpy(nph.proximity(t)), full)
+    nph.close()
+
+
 def test_flags_filter_pairs(dev):
     m = SCENES["sheets3"]()
     rng = np.random.default_rng(0)
