@@ -180,8 +180,24 @@ __global__ void k_edge_features(PcgPattern pt, uint32_t E, const uint32_t *eoff,
     }
 }
 
-// sum of |Linv_j b_j|^2 (for the RMS of the normalised right-hand side)
-__global__ void k_bhat_norm(uint32_t n, const float *linv, const float *b, float *acc) {
+// Sums over many workgroups: each workgroup writes its sum to parts[blockIdx.x]
+// and one workgroup adds those in a fixed order, so the total (and what
+// depends on it: features, factor, loss, weights) is the same on every run.
+__global__ void k_sum_parts(const float *parts, uint32_t nb, float *out) {
+    float v = 0.0f;
+    for (uint32_t i = threadIdx.x; i < nb; i += kNT) {
+        v += parts[i];
+    }
+    __shared__ float lds[kNT / kWave];
+    v = block_sum<kNT>(v, lds);
+    if (threadIdx.x == 0) {
+        *out = v;
+    }
+}
+
+// per-workgroup sums of |Linv_j b_j|^2 (for the RMS of the normalised
+// right-hand side)
+__global__ void k_bhat_norm(uint32_t n, const float *linv, const float *b, float *parts) {
     const uint32_t j = blockIdx.x * kNT + threadIdx.x;
     float v = 0.0f;
     if (j < n) {
@@ -191,7 +207,7 @@ __global__ void k_bhat_norm(uint32_t n, const float *linv, const float *b, float
     __shared__ float lds[kNT / kWave];
     v = block_sum<kNT>(v, lds);
     if (threadIdx.x == 0) {
-        atomicAdd(acc, v);
+        parts[blockIdx.x] = v;
     }
 }
 
@@ -509,7 +525,7 @@ __global__ void k_probe_y(PcgPattern pt, uint32_t K, const float *ghd, const flo
 
 // w = 2 Linv (A v) in place over Av, and the partial sums of y . (w / 2).
 __global__ void k_probe_w(uint32_t n, uint32_t K, const float *linv, const float *y, float *av,
-                          float *acc) {
+                          float *parts) {
     const uint32_t t = blockIdx.x * kNT + threadIdx.x;
     float q = 0.0f;
     if (t < n * K) {
@@ -523,14 +539,14 @@ __global__ void k_probe_w(uint32_t n, uint32_t K, const float *linv, const float
     __shared__ float lds[kNT / kWave];
     q = block_sum<kNT>(q, lds);
     if (threadIdx.x == 0) {
-        atomicAdd(acc, q);
+        parts[blockIdx.x] = q;
     }
 }
 
 // dL/dGhat and the log-determinant part of the loss. acc[0] = K q.
 __global__ void k_dghat(PcgPattern pt, uint32_t K, const float *z, const float *w,
                         const float *ghd, const float *acc, const uint32_t *srow, float *dgd,
-                        float *dgl, float *logdet) {
+                        float *dgl, float *parts) {
     const uint32_t k = blockIdx.x * kNT + threadIdx.x;
     const float inv = 1.0f / acc[0]; // 1 / (K q)
     const float dcoef = 2.0f / (3.0f * (float)pt.n);
@@ -570,7 +586,7 @@ __global__ void k_dghat(PcgPattern pt, uint32_t K, const float *z, const float *
     __shared__ float lds[kNT / kWave];
     ld = block_sum<kNT>(ld, lds);
     if (threadIdx.x == 0) {
-        atomicAdd(logdet, ld);
+        parts[blockIdx.x] = ld;
     }
 }
 
@@ -887,6 +903,7 @@ int Lpc::reserve(uint32_t n, uint32_t nnz, uint32_t E) {
     dXn_ = reinterpret_cast<uint16_t *>(carve<float>(cur, 2 * kH * N));
     dXe_ = reinterpret_cast<uint16_t *>(carve<float>(cur, kH * EE));
     part_ = carve<float>(cur, part_words);
+    part_words_ = part_words;
     scan_ = carve<uint32_t>(cur, scan_scratch_words(N + 1));
     if ((size_t)(cur - static_cast<char *>(mem_)) > bytes) {
         set_error("lpc: workspace layout exceeds its allocation");
@@ -933,7 +950,9 @@ int Lpc::prepare(const PcgPattern &pt, const float *val, const float *cdiag, con
     k_edge_features<<<cdiv(pt.n, kNT), kNT, 0, s>>>(pt, E, eoff_, esrc_, eslot_, val, linv_, f_,
                                                      fnorm_);
     PPF_HIP(hipGetLastError());
-    k_bhat_norm<<<cdiv(pt.n, kNT), kNT, 0, s>>>(pt.n, linv_, b, scal_);
+    k_bhat_norm<<<cdiv(pt.n, kNT), kNT, 0, s>>>(pt.n, linv_, b, part_);
+    PPF_HIP(hipGetLastError());
+    k_sum_parts<<<1, kNT, 0, s>>>(part_, cdiv(pt.n, kNT), scal_);
     PPF_HIP(hipGetLastError());
     k_node_features<<<cdiv(pt.n, kNT), kNT, 0, s>>>(pt.n, eoff_, fnorm_, linv_, b, scal_, x_);
     PPF_HIP(hipGetLastError());
@@ -1015,11 +1034,19 @@ int Lpc::gradient(PcgPlanImpl *plan, const float *val, const float *cdiag, const
             return rc;
         }
     }
-    k_probe_w<<<cdiv(K * n, kNT), kNT, 0, s>>>(n, K, linv_, py_, pav_, scal_);
-    PPF_HIP(hipGetLastError());
     const uint32_t m = std::max(n, nnz_);
+    if (cdiv(K * n, kNT) > part_words_ || cdiv(m, kNT) > part_words_) {
+        set_error("lpc: %u probes exceed the partial-sum scratch", K);
+        return PPF_ERR_ARG;
+    }
+    k_probe_w<<<cdiv(K * n, kNT), kNT, 0, s>>>(n, K, linv_, py_, pav_, part_);
+    PPF_HIP(hipGetLastError());
+    k_sum_parts<<<1, kNT, 0, s>>>(part_, cdiv(K * n, kNT), scal_);
+    PPF_HIP(hipGetLastError());
     k_dghat<<<cdiv(m, kNT), kNT, 0, s>>>(pt_, K, probes, pav_, ghd_, scal_, srow_, dgd_, dgl_,
-                                         scal_ + 1);
+                                         part_);
+    PPF_HIP(hipGetLastError());
+    k_sum_parts<<<1, kNT, 0, s>>>(part_, cdiv(m, kNT), scal_ + 1);
     PPF_HIP(hipGetLastError());
     k_dhead_out<<<cdiv(m, kNT), kNT, 0, s>>>(pt_, t_, u_, dgd_, dgl_);
     PPF_HIP(hipGetLastError());

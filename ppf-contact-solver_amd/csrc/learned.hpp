@@ -91,6 +91,7 @@ class Lpc {
     float *dh_ = nullptr, *dgd_ = nullptr, *dgl_ = nullptr, *scal_ = nullptr;
     uint16_t *dZ_ = nullptr, *dY_ = nullptr, *dXn_ = nullptr, *dXe_ = nullptr;
     float *part_ = nullptr;
+    size_t part_words_ = 0;
     float *pv_ = nullptr, *pav_ = nullptr, *py_ = nullptr;
     uint32_t nprobe_cap_ = 0;
 };

@@ -178,6 +178,11 @@ __global__ void k_check_offsets(uint32_t n, uint32_t nnz,
 
 // Order every column's mirror list by source row, as the reference's
 // transpose table (builder.rs:1336-1345), so the walk order is deterministic.
+// Lists of at most kSortRegs entries are sorted in registers by a bitonic
+// network (source rows are distinct within a column, padding sorts last);
+// longer ones by insertion in place.
+constexpr int kSortRegs = 16;
+
 __global__ void k_sort_transpose(uint32_t n, const uint32_t *__restrict__ tp,
                                  uint32_t *__restrict__ tcol,
                                  uint32_t *__restrict__ tslot) {
@@ -186,6 +191,45 @@ __global__ void k_sort_transpose(uint32_t n, const uint32_t *__restrict__ tp,
         return;
     }
     const uint32_t a = tp[j], b = tp[j + 1];
+    if (b - a <= 1) {
+        return;
+    }
+    if (b - a <= (uint32_t)kSortRegs) {
+        const uint32_t len = b - a;
+        uint32_t key[kSortRegs], val[kSortRegs];
+#pragma unroll
+        for (int u = 0; u < kSortRegs; ++u) {
+            const bool in = (uint32_t)u < len;
+            key[u] = in ? tcol[a + u] : 0xffffffffu;
+            val[u] = in ? tslot[a + u] : 0u;
+        }
+#pragma unroll
+        for (int k = 2; k <= kSortRegs; k <<= 1) {
+#pragma unroll
+            for (int h = k >> 1; h > 0; h >>= 1) {
+#pragma unroll
+                for (int u = 0; u < kSortRegs; ++u) {
+                    const int l = u ^ h;
+                    if (l > u) {
+                        const bool up = (u & k) == 0;
+                        if ((key[u] > key[l]) == up) {
+                            const uint32_t tk = key[u], tv = val[u];
+                            key[u] = key[l], val[u] = val[l];
+                            key[l] = tk, val[l] = tv;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSortRegs; ++u) {
+            if ((uint32_t)u < len) {
+                tcol[a + u] = key[u];
+                tslot[a + u] = val[u];
+            }
+        }
+        return;
+    }
     for (uint32_t u = a + 1; u < b; ++u) {
         const uint32_t kc = tcol[u], ks = tslot[u];
         uint32_t v = u;
