@@ -115,6 +115,12 @@ __device__ inline unsigned xcd_logical_block() {
     return (b % kXcd) * (g / kXcd) + b / kXcd;
 }
 
+// Grid of at least `blocks` workgroups that xcd_logical_block can remap (the
+// kernel treats logical ids past its work as empty).
+inline uint32_t xcd_grid(uint32_t blocks) {
+    return (blocks + kXcd - 1) / kXcd * kXcd;
+}
+
 } // namespace ppf
 
 // Error plumbing for the C ABI: every entry point returns 0 on success and a

@@ -66,7 +66,7 @@ __device__ inline void wave_min_out(uint32_t *out, float best, float above) {
 // wave queue.
 __global__ __launch_bounds__(kWQ) void k_ls_vertex(LsCtx c) {
     __shared__ WaveScratch ws;
-    const uint32_t leaf = blockIdx.x * kWQ + threadIdx.x;
+    const uint32_t leaf = xcd_logical_block() * kWQ + threadIdx.x;
     const bool active = leaf < c.vb.n;
     const ppf_contact_mesh &m = c.m;
     const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kWQ) void k_ls_vertex(LsCtx c) {
 // static vertex vs dynamic face (contact.cu:1996-2034)
 __global__ __launch_bounds__(kWQ) void k_ls_static_vertex(LsCtx c) {
     __shared__ WaveScratch ws;
-    const uint32_t i = blockIdx.x * kWQ + threadIdx.x;
+    const uint32_t i = xcd_logical_block() * kWQ + threadIdx.x;
     const bool active = i < c.sm.nv;
     const ppf_contact_mesh &m = c.m;
     const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kWQ) void k_ls_static_vertex(LsCtx c) {
 // edge-edge and edge vs static edge, seeded by the vertex-face minimum.
 __global__ __launch_bounds__(kWQ) void k_ls_edge(LsCtx c) {
     __shared__ WaveScratch ws;
-    const uint32_t leaf = blockIdx.x * kWQ + threadIdx.x;
+    const uint32_t leaf = xcd_logical_block() * kWQ + threadIdx.x;
     const bool active = leaf < c.eb.n;
     const ppf_contact_mesh &m = c.m;
     const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
@@ -386,15 +386,15 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
     c.seb = S.nv ? sedge_.view() : BvhView{};
     c.out = out_;
     if (m.nv) {
-        k_ls_vertex<<<cdiv(m.nv, kWQ), kWQ, 0, s>>>(c);
+        k_ls_vertex<<<xcd_grid(cdiv(m.nv, kWQ)), kWQ, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     if (S.nv && m.nf) {
-        k_ls_static_vertex<<<cdiv(S.nv, kWQ), kWQ, 0, s>>>(c);
+        k_ls_static_vertex<<<xcd_grid(cdiv(S.nv, kWQ)), kWQ, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     if (m.ne) {
-        k_ls_edge<<<cdiv(m.ne, kWQ), kWQ, 0, s>>>(c);
+        k_ls_edge<<<xcd_grid(cdiv(m.ne, kWQ)), kWQ, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     PPF_HIP(hipMemcpyAsync(h_out_, out_, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));

@@ -70,10 +70,11 @@ __device__ inline void emit(const Ctx &c, uint64_t &slot, uint32_t qpos, uint32_
 }
 
 // Leaf position this thread queries: one per leaf, or (a fill pass after a
-// stashed count) the listed ones. False: none.
+// stashed count) the listed ones. False: none. Consecutive leaves share an
+// XCD (and its L2), as do their tree walks.
 template <bool FILL>
 __device__ inline bool query_pos(const Ctx &c, uint32_t n, uint32_t &leaf_pos) {
-    const uint32_t t = blockIdx.x * kQT + threadIdx.x;
+    const uint32_t t = xcd_logical_block() * kQT + threadIdx.x;
     if (!FILL || !c.redo) {
         leaf_pos = t;
         return t < n;
@@ -582,11 +583,11 @@ int Narrow::proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,
     c.redo = nullptr;
     c.nredo = 0;
     if (m.nv) {
-        k_vertex_queries<false><<<cdiv(m.nv, kQT), kQT, 0, s>>>(c);
+        k_vertex_queries<false><<<xcd_grid(cdiv(m.nv, kQT)), kQT, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     if (m.ne) {
-        k_edge_queries<false><<<cdiv(m.ne, kQT), kQT, 0, s>>>(c);
+        k_edge_queries<false><<<xcd_grid(cdiv(m.ne, kQT)), kQT, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     PPF_HIP(hipMemsetAsync(cnt_ + nq, 0, sizeof(uint32_t), s));
@@ -609,14 +610,14 @@ int Narrow::proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,
         if (nv_redo) {
             Ctx cv = c;
             cv.nredo = nv_redo;
-            k_vertex_queries<true><<<cdiv(nv_redo, kQT), kQT, 0, s>>>(cv);
+            k_vertex_queries<true><<<xcd_grid(cdiv(nv_redo, kQT)), kQT, 0, s>>>(cv);
             PPF_HIP(hipGetLastError());
         }
         if (ne_redo) {
             Ctx ce = c;
             ce.redo = redo_ + m.nv;
             ce.nredo = ne_redo;
-            k_edge_queries<true><<<cdiv(ne_redo, kQT), kQT, 0, s>>>(ce);
+            k_edge_queries<true><<<xcd_grid(cdiv(ne_redo, kQT)), kQT, 0, s>>>(ce);
             PPF_HIP(hipGetLastError());
         }
         PPF_HIP(hipMemcpyAsync(h_flags_, flags_, 4 * sizeof(uint32_t),
