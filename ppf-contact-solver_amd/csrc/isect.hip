@@ -98,7 +98,7 @@ __device__ void record(const IsCtx &c, uint32_t type, uint32_t e0, uint32_t e1,
 
 __global__ __launch_bounds__(kQT) void k_isect_edges(IsCtx c) {
     __shared__ uint32_t stk_all[kStack * kQT];
-    const uint32_t leaf = blockIdx.x * kQT + threadIdx.x;
+    const uint32_t leaf = xcd_logical_block() * kQT + threadIdx.x;
     if (leaf >= c.eb.n) {
         return;
     }
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kQT) void k_isect_edges(IsCtx c) {
 // point-point among surface vertices (contact.cu:2982-3054, 3135-3155)
 __global__ __launch_bounds__(kQT) void k_isect_verts(IsCtx c) {
     __shared__ uint32_t stk_all[kStack * kQT];
-    const uint32_t leaf = blockIdx.x * kQT + threadIdx.x;
+    const uint32_t leaf = xcd_logical_block() * kQT + threadIdx.x;
     if (leaf >= c.vb.n) {
         return;
     }
@@ -287,11 +287,11 @@ int IntersectionCheck::run(const BvhView &face, const BvhView &edge, const BvhVi
     c.cap = rec_cap_;
     c.ctr = ctr_;
     if (m.ne) {
-        k_isect_edges<<<cdiv(m.ne, kQT), kQT, 0, s>>>(c);
+        k_isect_edges<<<xcd_grid(cdiv(m.ne, kQT)), kQT, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     if (c.nsurf) {
-        k_isect_verts<<<cdiv(m.nv, kQT), kQT, 0, s>>>(c);
+        k_isect_verts<<<xcd_grid(cdiv(m.nv, kQT)), kQT, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
     PPF_HIP(hipMemcpyAsync(h_ctr_, ctr_, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));

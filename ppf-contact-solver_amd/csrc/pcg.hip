@@ -32,6 +32,7 @@
 #include "eig3.hpp"
 #include "pcg.hpp"
 #include "scan.hpp"
+#include "sort.hpp"
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -133,12 +134,13 @@ __global__ void k_count_transpose(uint32_t n, const uint32_t *__restrict__ rp,
     dslot[i] = ds;
 }
 
-__global__ void k_fill_transpose(uint32_t n, const uint32_t *__restrict__ rp,
+// Radix-sort input for the transpose table: the key of a stored slot is its
+// column when the entry is an accepted off-diagonal one (the acceptance rule
+// of k_count_transpose), n otherwise; the value is the slot and row[] its row.
+__global__ void k_transpose_keys(uint32_t n, const uint32_t *__restrict__ rp,
                                  const uint32_t *__restrict__ col,
-                                 const uint32_t *__restrict__ tp,
-                                 uint32_t *__restrict__ tfill,
-                                 uint32_t *__restrict__ tcol,
-                                 uint32_t *__restrict__ tslot) {
+                                 uint32_t *__restrict__ key, uint32_t *__restrict__ val,
+                                 uint32_t *__restrict__ row) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) {
         return;
@@ -146,18 +148,30 @@ __global__ void k_fill_transpose(uint32_t n, const uint32_t *__restrict__ rp,
     uint32_t prev = 0;
     for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
         const uint32_t j = col[k];
-        // Same acceptance rule as k_count_transpose, so every write stays
-        // inside the column's counted range.
-        if (j >= n || j < i || (k > rp[i] && j <= prev)) {
-            continue;
+        const bool ok = !(j >= n || j < i || (k > rp[i] && j <= prev));
+        if (ok) {
+            prev = j;
         }
-        prev = j;
-        if (j > i) {
-            const uint32_t pos = tp[j] + atomicAdd(tfill + j, 1u);
-            tcol[pos] = i;
-            tslot[pos] = k;
-        }
+        key[k] = ok && j > i ? j : n;
+        val[k] = k;
+        row[k] = i;
     }
+}
+
+// After a stable sort by column of the slots (which are in row order), the
+// first tp[n] entries are every column's mirror list ordered by source row,
+// as the reference's transpose table (builder.rs:1336-1345).
+__global__ void k_transpose_place(uint32_t nnz, const uint32_t *__restrict__ tp_end,
+                                  const uint32_t *__restrict__ val,
+                                  const uint32_t *__restrict__ row,
+                                  uint32_t *__restrict__ tcol, uint32_t *__restrict__ tslot) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nnz || u >= *tp_end) {
+        return;
+    }
+    const uint32_t k = val[u];
+    tcol[u] = row[k];
+    tslot[u] = k;
 }
 
 // row_offset must start at 0, be non-decreasing and end at nnz.
@@ -173,73 +187,6 @@ __global__ void k_check_offsets(uint32_t n, uint32_t nnz,
                     v <= nnz;
     if (!ok) {
         atomicAdd(bad, 1u);
-    }
-}
-
-// Order every column's mirror list by source row, as the reference's
-// transpose table (builder.rs:1336-1345), so the walk order is deterministic.
-// Lists of at most kSortRegs entries are sorted in registers by a bitonic
-// network (source rows are distinct within a column, padding sorts last);
-// longer ones by insertion in place.
-constexpr int kSortRegs = 16;
-
-__global__ void k_sort_transpose(uint32_t n, const uint32_t *__restrict__ tp,
-                                 uint32_t *__restrict__ tcol,
-                                 uint32_t *__restrict__ tslot) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) {
-        return;
-    }
-    const uint32_t a = tp[j], b = tp[j + 1];
-    if (b - a <= 1) {
-        return;
-    }
-    if (b - a <= (uint32_t)kSortRegs) {
-        const uint32_t len = b - a;
-        uint32_t key[kSortRegs], val[kSortRegs];
-#pragma unroll
-        for (int u = 0; u < kSortRegs; ++u) {
-            const bool in = (uint32_t)u < len;
-            key[u] = in ? tcol[a + u] : 0xffffffffu;
-            val[u] = in ? tslot[a + u] : 0u;
-        }
-#pragma unroll
-        for (int k = 2; k <= kSortRegs; k <<= 1) {
-#pragma unroll
-            for (int h = k >> 1; h > 0; h >>= 1) {
-#pragma unroll
-                for (int u = 0; u < kSortRegs; ++u) {
-                    const int l = u ^ h;
-                    if (l > u) {
-                        const bool up = (u & k) == 0;
-                        if ((key[u] > key[l]) == up) {
-                            const uint32_t tk = key[u], tv = val[u];
-                            key[u] = key[l], val[u] = val[l];
-                            key[l] = tk, val[l] = tv;
-                        }
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kSortRegs; ++u) {
-            if ((uint32_t)u < len) {
-                tcol[a + u] = key[u];
-                tslot[a + u] = val[u];
-            }
-        }
-        return;
-    }
-    for (uint32_t u = a + 1; u < b; ++u) {
-        const uint32_t kc = tcol[u], ks = tslot[u];
-        uint32_t v = u;
-        while (v > a && tcol[v - 1] > kc) {
-            tcol[v] = tcol[v - 1];
-            tslot[v] = tslot[v - 1];
-            --v;
-        }
-        tcol[v] = kc;
-        tslot[v] = ks;
     }
 }
 
@@ -1119,10 +1066,12 @@ struct PcgPlanImpl {
     Args *d_args = nullptr;
     Scalars *h_scal = nullptr; // pinned
     void *mem = nullptr;       // one device allocation for the n-sized scratch
-    uint32_t *tp = nullptr, *tfill = nullptr, *dslot = nullptr;
+    uint32_t *tp = nullptr, *dslot = nullptr;
     uint32_t *scan_tmp = nullptr, *flags = nullptr;
     uint32_t *pat = nullptr;   // transpose columns | slots, grown with nnz
     size_t pat_cap = 0;
+    uint32_t *tsort = nullptr; // sort keys | values | rows | radix scratch, grown with nnz
+    size_t tsort_cap = 0;
     void *ell = nullptr;       // ELL entry table (when rows are short)
     size_t ell_cap = 0;        // entries
     uint32_t graph_key = 0xffffffffu; // SpMV form the graph was captured for
@@ -1238,6 +1187,16 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
         pl->pat_cap = cap;
     }
     uint32_t *tcol = pl->pat, *tslot = pl->pat + pl->pat_cap / 2;
+    const size_t sort_words = 3 * (size_t)nnz + radix_sort_scratch_words(nnz);
+    if (sort_words > pl->tsort_cap) {
+        if (pl->tsort) {
+            PPF_HIP(hipFree(pl->tsort));
+            pl->tsort = nullptr;
+        }
+        const size_t cap = sort_words + sort_words / 4 + 256;
+        PPF_HIP(hipMalloc(&pl->tsort, cap * sizeof(uint32_t)));
+        pl->tsort_cap = cap;
+    }
     pl->nnz = nnz;
     h.rp = rp;
     h.col = col;
@@ -1247,7 +1206,6 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
     h.dslot = pl->dslot;
     uint32_t *flags = pl->flags;
     PPF_HIP(hipMemsetAsync(pl->tp, 0, (n + 1) * sizeof(uint32_t), st));
-    PPF_HIP(hipMemsetAsync(pl->tfill, 0, (n + 1) * sizeof(uint32_t), st));
     PPF_HIP(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), st));
     const unsigned gb = cdiv(n, kNT);
     uint32_t bad = 0;
@@ -1262,10 +1220,18 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
     k_count_transpose<<<gb, kNT, 0, st>>>(n, rp, col, pl->tp, pl->dslot, flags);
     PPF_HIP(hipGetLastError());
     PPF_HIP(exclusive_scan_u32(pl->tp, pl->tp, n + 1, pl->scan_tmp, st));
-    k_fill_transpose<<<gb, kNT, 0, st>>>(n, rp, col, pl->tp, pl->tfill, tcol, tslot);
-    PPF_HIP(hipGetLastError());
-    k_sort_transpose<<<gb, kNT, 0, st>>>(n, pl->tp, tcol, tslot);
-    PPF_HIP(hipGetLastError());
+    if (nnz) {
+        uint32_t *key = pl->tsort, *val = key + nnz, *row = val + nnz, *scratch = row + nnz;
+        k_transpose_keys<<<gb, kNT, 0, st>>>(n, rp, col, key, val, row);
+        PPF_HIP(hipGetLastError());
+        uint32_t bits = 1;
+        while (bits < 32 && (n >> bits) != 0) {
+            ++bits;
+        }
+        PPF_HIP(radix_sort_pairs(key, val, nnz, bits, scratch, st));
+        k_transpose_place<<<cdiv(nnz, kNT), kNT, 0, st>>>(nnz, pl->tp + n, val, row, tcol, tslot);
+        PPF_HIP(hipGetLastError());
+    }
     PPF_HIP(hipMemcpyAsync(&bad, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     PPF_HIP(hipStreamSynchronize(st));
     if (bad) {
@@ -1580,7 +1546,7 @@ int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const u
 
     const size_t n3 = 3 * (size_t)n;
     const size_t scan_words = scan_scratch_words(n + 1);
-    size_t bytes = carve_bytes(n + 1, 4) * 2 + carve_bytes(n, 4) +
+    size_t bytes = carve_bytes(n + 1, 4) + carve_bytes(n, 4) +
                    carve_bytes(9 * (size_t)n, 4) + carve_bytes(n3, 4) * 5 +
                    carve_bytes(1, sizeof(Args)) + carve_bytes(scan_words, 4) +
                    carve_bytes(4, 4) + carve_bytes(nscene + 1, 4);
@@ -1592,7 +1558,6 @@ int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const u
     }
     char *cur = static_cast<char *>(pl->mem);
     pl->tp = carve<uint32_t>(cur, n + 1);
-    pl->tfill = carve<uint32_t>(cur, n + 1);
     pl->dslot = carve<uint32_t>(cur, n);
     h.P = carve<float>(cur, 9 * (size_t)n);
     h.r = carve<float>(cur, n3);
@@ -1690,6 +1655,9 @@ void pcg_plan_destroy(PcgPlanImpl *pl) {
     }
     if (pl->pat) {
         (void)hipFree(pl->pat);
+    }
+    if (pl->tsort) {
+        (void)hipFree(pl->tsort);
     }
     if (pl->ell) {
         (void)hipFree(pl->ell);
