@@ -26,7 +26,7 @@ using Q = QBox;
 
 // Pairs the count pass keeps per query. The fill pass then copies them
 // instead of walking the tree again; only queries with more pairs walk again.
-constexpr uint32_t kStash = 4;
+constexpr uint32_t kStash = 8;
 
 struct Ctx {
     ppf_contact_mesh m;

@@ -73,13 +73,13 @@ def query_sizes(p):
 
 
 def test_short_and_long_queries(dev):
-    """The count pass keeps each query's first 4 pairs and the fill pass copies
+    """The count pass keeps each query's first 8 pairs and the fill pass copies
     them; queries with more pairs walk the trees again. Both kinds of query
     occur here, and a buffer smaller than the pair count receives exactly the
     first pairs of the full result."""
     m = SCENES["sheets3"]()
     sizes = query_sizes(oracle.proximity(m))
-    assert (sizes <= 4).any() and (sizes > 4).any(), np.bincount(sizes)
+    assert (sizes <= 8).any() and (sizes > 8).any(), np.bincount(sizes)
     t = mesh_to_device(m, dev)
     nph = NarrowPhase()
     nph.build(t)
