@@ -407,7 +407,8 @@ def run_broad_narrow(args, dev):
     triangles (centroids uniform in a 1 m cube, vertices within 5e-3 m of
     them, seed 2), gap 1e-3. A step is one LBVH build of the face, edge and
     vertex trees plus the point-face / point-edge / point-point / edge-edge
-    proximity enumeration (count pass, scan, fill pass)."""
+    proximity enumeration (count pass keeping each query's first 8 pairs, scan,
+    copy of the kept pairs, fill pass for queries with more)."""
     import numpy as np
     import torch
 
