@@ -120,10 +120,11 @@ __device__ uint32_t traverse(const BvhView &bvh, const QBox &q, uint32_t *stk,
 constexpr int kWaveQueries = 64;        // one wave per workgroup
 constexpr int kWaveQueue = 2 * kWaveQueries;
 
-struct WaveScratch {
-    uint32_t stk[kQueryStack * kWaveQueries];
+template <int STACK> struct WaveScratchT {
+    uint32_t stk[STACK * kWaveQueries];
     uint32_t qid[kWaveQueue], prim[kWaveQueue];
 };
+using WaveScratch = WaveScratchT<kQueryStack>;
 
 __device__ inline uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -132,9 +133,9 @@ __device__ inline uint32_t lanes_below(uint64_t m) {
 
 // `active`: this lane has a query (lanes past the end pass false). All lanes
 // of the workgroup must call it (the rounds are wave uniform).
-template <class A, class W>
+template <int STACK, class A, class W>
 __device__ void traverse_wave(const BvhView &bvh, const QBox &q, bool active, uint32_t qid,
-                              WaveScratch &ws, uint32_t *overflow, A &&accept, W &&work) {
+                              WaveScratchT<STACK> &ws, uint32_t *overflow, A &&accept, W &&work) {
     const uint32_t lane = threadIdx.x;
     uint32_t *stk = ws.stk + lane;
     uint32_t sp = 0;
@@ -157,7 +158,7 @@ __device__ void traverse_wave(const BvhView &bvh, const QBox &q, bool active, ui
                 const uint32_t l = bvh.node[2 * (size_t)v] - 1, r = c1 - 1;
                 const bool ol = overlaps(load_box(bvh.box, l), q);
                 const bool orr = overlaps(load_box(bvh.box, r), q);
-                if (sp + 2 > (uint32_t)kQueryStack) {
+                if (sp + 2 > (uint32_t)STACK) {
                     atomicOr(overflow, 1u);
                     sp = 0;
                 } else {

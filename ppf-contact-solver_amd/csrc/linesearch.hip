@@ -12,12 +12,19 @@
 #include "bvh_query.hpp"
 #include "ccd.hpp"
 #include "linesearch.hpp"
+#include <type_traits>
 
 namespace ppf {
 
 namespace {
 
 constexpr int kStack = kQueryStack;
+// The sweeps first run with shorter stacks: 36 entries (10 KB of LDS per
+// one-wave workgroup) let 16 workgroups share a CU, 4 waves per SIMD, against
+// 12 with 48. A walk deeper than 35 levels flags an overflow, and the pass
+// runs again with 48-entry stacks.
+constexpr int kShortStack = 36;
+constexpr int kLsWaves = 4;
 using Box6 = QBox;
 
 struct LsCtx {
@@ -64,8 +71,9 @@ __device__ inline void wave_min_out(uint32_t *out, float best, float above) {
 // (contact.cu:2266-2316) and vertex vs static face (contact.cu:1956-1994).
 // The filters run on the query's lane while it walks; the sweeps run from the
 // wave queue.
-__global__ __launch_bounds__(kWQ) void k_ls_vertex(LsCtx c) {
-    __shared__ WaveScratch ws;
+template <int STACK>
+__global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_vertex(LsCtx c) {
+    __shared__ WaveScratchT<STACK> ws;
     const uint32_t leaf = xcd_logical_block() * kWQ + threadIdx.x;
     const bool active = leaf < c.vb.n;
     const ppf_contact_mesh &m = c.m;
@@ -149,8 +157,9 @@ __global__ __launch_bounds__(kWQ) void k_ls_vertex(LsCtx c) {
 }
 
 // static vertex vs dynamic face (contact.cu:1996-2034)
-__global__ __launch_bounds__(kWQ) void k_ls_static_vertex(LsCtx c) {
-    __shared__ WaveScratch ws;
+template <int STACK>
+__global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_static_vertex(LsCtx c) {
+    __shared__ WaveScratchT<STACK> ws;
     const uint32_t i = xcd_logical_block() * kWQ + threadIdx.x;
     const bool active = i < c.sm.nv;
     const ppf_contact_mesh &m = c.m;
@@ -179,8 +188,9 @@ __global__ __launch_bounds__(kWQ) void k_ls_static_vertex(LsCtx c) {
 }
 
 // edge-edge and edge vs static edge, seeded by the vertex-face minimum.
-__global__ __launch_bounds__(kWQ) void k_ls_edge(LsCtx c) {
-    __shared__ WaveScratch ws;
+template <int STACK>
+__global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_edge(LsCtx c) {
+    __shared__ WaveScratchT<STACK> ws;
     const uint32_t leaf = xcd_logical_block() * kWQ + threadIdx.x;
     const bool active = leaf < c.eb.n;
     const ppf_contact_mesh &m = c.m;
@@ -385,20 +395,33 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
     c.sfb = S.nv ? sface_.view() : BvhView{};
     c.seb = S.nv ? sedge_.view() : BvhView{};
     c.out = out_;
-    if (m.nv) {
-        k_ls_vertex<<<xcd_grid(cdiv(m.nv, kWQ)), kWQ, 0, s>>>(c);
-        PPF_HIP(hipGetLastError());
+    auto sweeps = [&](auto stack) -> int {
+        constexpr int D = decltype(stack)::value;
+        if (m.nv) {
+            k_ls_vertex<D><<<xcd_grid(cdiv(m.nv, kWQ)), kWQ, 0, s>>>(c);
+            PPF_HIP(hipGetLastError());
+        }
+        if (S.nv && m.nf) {
+            k_ls_static_vertex<D><<<xcd_grid(cdiv(S.nv, kWQ)), kWQ, 0, s>>>(c);
+            PPF_HIP(hipGetLastError());
+        }
+        if (m.ne) {
+            k_ls_edge<D><<<xcd_grid(cdiv(m.ne, kWQ)), kWQ, 0, s>>>(c);
+            PPF_HIP(hipGetLastError());
+        }
+        PPF_HIP(hipMemcpyAsync(h_out_, out_, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        PPF_HIP(hipStreamSynchronize(s));
+        return PPF_OK;
+    };
+    rc = sweeps(std::integral_constant<int, kShortStack>{});
+    if (rc == PPF_OK && h_out_[3]) {
+        // a walk overflowed the short stacks: every sweep again from the start
+        PPF_HIP(hipMemcpyAsync(out_, init, sizeof(init), hipMemcpyHostToDevice, s));
+        rc = sweeps(std::integral_constant<int, kStack>{});
     }
-    if (S.nv && m.nf) {
-        k_ls_static_vertex<<<xcd_grid(cdiv(S.nv, kWQ)), kWQ, 0, s>>>(c);
-        PPF_HIP(hipGetLastError());
+    if (rc != PPF_OK) {
+        return rc;
     }
-    if (m.ne) {
-        k_ls_edge<<<xcd_grid(cdiv(m.ne, kWQ)), kWQ, 0, s>>>(c);
-        PPF_HIP(hipGetLastError());
-    }
-    PPF_HIP(hipMemcpyAsync(h_out_, out_, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    PPF_HIP(hipStreamSynchronize(s));
     if (h_out_[3]) {
         set_error("line search: a BVH traversal exceeded its %d-entry stack", kStack);
         return PPF_ERR_CAPACITY;
