@@ -15,6 +15,7 @@
 #ifndef PPF_MI355X_H
 #define PPF_MI355X_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -32,6 +33,19 @@ enum {
 
 /* Last error message of the calling thread ("" if none). */
 const char *ppf_last_error(void);
+
+/* Device and pinned-host allocations / frees the library has made since it
+ * was loaded (replaces the reference's g_device_alloc_count /
+ * g_device_free_count, main/main.cu:280-286, 1635-1646, which it logs per
+ * step as device-alloc / device-free). Buffers grow to a high-water mark and
+ * are kept, so once warmed up a time step adds nothing to either count. */
+void ppf_alloc_counts(uint64_t *allocs, uint64_t *frees);
+
+/* Device allocation and release through the same counters, for code layered
+ * on this library (the solver-backend boundary; the reference's
+ * mem::malloc_device / mem::free_device under buffer/buffer.hpp). */
+int ppf_device_alloc(void **ptr, size_t bytes);
+void ppf_device_free(void *ptr);
 
 /* Library / device identity: writes the gfx arch name of device 0. */
 int ppf_device_info(char *arch, uint32_t arch_len, uint32_t *cu_count);
@@ -515,6 +529,7 @@ typedef struct ppf_step_result {
     float dt;
     float ms_elastic, ms_contact, ms_solve, ms_line_search, ms_total;
     uint32_t lpc_fallbacks; /* solves the learned factor handed to block Jacobi */
+    uint32_t device_alloc, device_free; /* allocations / frees during the step */
 } ppf_step_result;
 
 typedef struct ppf_stepper ppf_stepper;

@@ -138,6 +138,25 @@ struct oracle_step_result {
     float dt;
 };
 
+// The last Newton system the oracle solved (merged upper blocks with the
+// eliminated ones zeroed, diagonal, right-hand side), for the tests.
+static struct {
+    uint32_t n = 0;
+    std::vector<uint32_t> rp, ci;
+    std::vector<float> val, cdiag, rhs;
+} g_last;
+
+uint32_t oracle_last_system(uint32_t *n, uint32_t *rp, uint32_t *ci, float *val, float *cdiag,
+                            float *rhs) {
+    if (n) *n = g_last.n;
+    if (rp) std::copy(g_last.rp.begin(), g_last.rp.end(), rp);
+    if (ci) std::copy(g_last.ci.begin(), g_last.ci.end(), ci);
+    if (val) std::copy(g_last.val.begin(), g_last.val.end(), val);
+    if (cdiag) std::copy(g_last.cdiag.begin(), g_last.cdiag.end(), cdiag);
+    if (rhs) std::copy(g_last.rhs.begin(), g_last.rhs.end(), rhs);
+    return (uint32_t)g_last.ci.size();
+}
+
 // One step; mesh.x (curr) and prev are updated on success. Returns 0.
 int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_result *out) {
     *out = oracle_step_result{};
@@ -282,32 +301,47 @@ int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_resul
             }
         }
         if (npinned) {
-            for (uint32_t i = 0; i < n; ++i) {
-                const bool mi = mask[i] != 0;
-                for (uint32_t k = mrp[i]; k < mrp[i + 1]; ++k) {
-                    const uint32_t j = mci[k];
-                    const bool mj = mask[j] != 0;
-                    float *b = &mval[9 * (size_t)k];
-                    if (mi != mj) {
-                        const uint32_t pv = mj ? j : i, row = mj ? i : j;
-                        float pr[3];
-                        for (int d = 0; d < 3; ++d) pr[d] = ex[3 * pv + d] - target[3 * pv + d];
-                        for (int r = 0; r < 3; ++r) {
-                            float c = mj ? (b[r] * pr[0] + b[r + 3] * pr[1]) + b[r + 6] * pr[2]
-                                         : (b[3 * r] * pr[0] + b[3 * r + 1] * pr[1]) +
-                                               b[3 * r + 2] * pr[2];
-                            force[3 * row + r] += -c;
-                        }
-                    }
-                    if (mi || mj) std::fill(b, b + 9, 0.0f);
+            // Lifts of the couplings to pinned vertices into the free rows
+            // (main.cu:954-1049; the reference scatters them atomically, in no
+            // fixed order). Here each free row takes its own lifts (row i
+            // free, column j pinned: f_i -= B_ij p_j) in entry order, then the
+            // mirror lifts (row j pinned, column i free: f_i -= B_ji^T p_j) in
+            // (source row, entry) order. Blocks touching a pinned vertex are
+            // zeroed.
+            auto lift = [&](uint32_t row, const float *b, uint32_t pv, bool tr) {
+                float pr[3];
+                for (int d = 0; d < 3; ++d) pr[d] = ex[3 * pv + d] - target[3 * pv + d];
+                for (int r = 0; r < 3; ++r) {
+                    float c = !tr ? (b[r] * pr[0] + b[r + 3] * pr[1]) + b[r + 6] * pr[2]
+                                  : (b[3 * r] * pr[0] + b[3 * r + 1] * pr[1]) + b[3 * r + 2] * pr[2];
+                    force[3 * row + r] += -c;
                 }
+            };
+            for (uint32_t i = 0; i < n; ++i) {
+                if (mask[i]) continue;
+                for (uint32_t k = mrp[i]; k < mrp[i + 1]; ++k)
+                    if (mask[mci[k]]) lift(i, &mval[9 * (size_t)k], mci[k], false);
             }
+            for (uint32_t i = 0; i < n; ++i) {
+                if (!mask[i]) continue;
+                for (uint32_t k = mrp[i]; k < mrp[i + 1]; ++k)
+                    if (!mask[mci[k]]) lift(mci[k], &mval[9 * (size_t)k], i, true);
+            }
+            for (uint32_t i = 0; i < n; ++i)
+                for (uint32_t k = mrp[i]; k < mrp[i + 1]; ++k)
+                    if (mask[i] || mask[mci[k]]) std::fill(&mval[9 * (size_t)k], &mval[9 * (size_t)k] + 9, 0.0f);
             for (uint32_t i = 0; i < n; ++i)
                 if (mask[i]) {
                     for (int e = 0; e < 9; ++e) cdiag[9 * (size_t)i + e] = (e % 4 == 0) ? 1.0f : 0.0f;
                     for (int d = 0; d < 3; ++d) force[3 * i + d] = ex[3 * i + d] - target[3 * i + d];
                 }
         }
+        g_last.n = n;
+        g_last.rp = mrp;
+        g_last.ci = mci;
+        g_last.val = mval;
+        g_last.cdiag = cdiag;
+        g_last.rhs = force;
         PcgResult pr{};
         oracle_pcg(n, mrp.data(), mci.data(), mval.data(), cdiag.data(), force.data(), dx.data(),
                    p->cg_tol, p->cg_max_iter, nullptr, &pr);

@@ -562,6 +562,22 @@ class OracleStepper:
                                  fix_pos=_p(self.fix) if self.fix is not None else None,
                                  fix_delta=_p(self.delta) if self.delta is not None else None)
 
+    def last_system(self):
+        """The last Newton system the oracle solved: row_offset, col_index,
+        value (nnz, 9), cdiag (n, 9), b (3n); eliminated blocks are zero."""
+        lib = self.lib
+        lib.oracle_last_system.argtypes = [ctypes.POINTER(u32), vp, vp, vp, vp, vp]
+        lib.oracle_last_system.restype = u32
+        n = u32()
+        nnz = lib.oracle_last_system(ctypes.byref(n), None, None, None, None, None)
+        n = n.value
+        out = {"row_offset": np.zeros(n + 1, np.uint32), "col_index": np.zeros(nnz, np.uint32),
+               "value": np.zeros((nnz, 9), np.float32), "cdiag": np.zeros((n, 9), np.float32),
+               "b": np.zeros(3 * n, np.float32)}
+        lib.oracle_last_system(None, *(_p(out[k]) for k in ("row_offset", "col_index", "value",
+                                                             "cdiag", "b")))
+        return out
+
     def advance(self, params):
         kw = dict(params)
         kw["gravity"] = (f32 * 3)(*params["gravity"])

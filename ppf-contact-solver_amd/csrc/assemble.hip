@@ -607,7 +607,7 @@ void ContactAssembler::release() {
     for (void *p : {(void *)pe_, (void *)buf_, (void *)out_row_, (void *)out_col_,
                     (void *)out_val_, (void *)rp_}) {
         if (p) {
-            (void)hipFree(p);
+            (void)dev_free(p);
         }
     }
     pe_ = nullptr;
@@ -629,12 +629,12 @@ int ContactAssembler::reserve(uint64_t npairs, uint32_t nv) {
     // words: fkey fval (nf each), hrow hcol hval head pos (nh each),
     // vstart (nv + 1), sort scratch, scan scratch.
     const size_t words = 2 * nf + 5 * nh + (nv + 1) + sortw + scanw;
-    PPF_HIP(hipMalloc(&pe_, npairs * sizeof(PairEval)));
-    PPF_HIP(hipMalloc(&buf_, words * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&out_row_, nh * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&out_col_, nh * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&out_val_, 9 * nh * sizeof(float)));
-    PPF_HIP(hipMalloc(&rp_, (nv + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&pe_, npairs * sizeof(PairEval)));
+    PPF_HIP(dev_alloc(&buf_, words * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&out_row_, nh * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&out_col_, nh * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&out_val_, 9 * nh * sizeof(float)));
+    PPF_HIP(dev_alloc(&rp_, (nv + 1) * sizeof(uint32_t)));
     pcap_ = npairs;
     vcap_ = nv;
     return PPF_OK;

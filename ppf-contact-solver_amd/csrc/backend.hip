@@ -130,11 +130,34 @@ struct Backend {
     std::vector<cts_IntersectionRecord> records;
     uint32_t record_count = 0;
     std::string log_dir;
+    // Scratch of the host calls between steps (index lists, gathered
+    // positions, the contact pattern): grown to a high-water mark and kept,
+    // so a host that calls them every step allocates nothing once warmed up
+    // (the reference's pooled buffers, buffer/buffer.hpp).
+    struct Scratch {
+        void *p = nullptr;
+        size_t bytes = 0;
+    };
+    Scratch scr_idx, scr_out, scr_dyn;
+    void *grow(Scratch &sc, size_t bytes) {
+        if (bytes > sc.bytes) {
+            ppf_device_free(sc.p);
+            sc = Scratch{};
+            const size_t cap = bytes + bytes / 4 + 256;
+            ppf_or_die(ppf_device_alloc(&sc.p, cap), "ppf_device_alloc");
+            sc.bytes = cap;
+        }
+        return sc.p;
+    }
+    // Contact pattern carried in by update_dyn (a resumed run), reported by
+    // fetch_dyn until the next step replaces it (csrmat.cu:136-153).
+    bool carried = false;
+    std::vector<uint32_t> carried_rp, carried_ci;
 
     template <class T> T *put(const T *h, size_t n) {
         void *p = nullptr;
         const size_t bytes = (n ? n : 1) * sizeof(T);
-        hip_or_die(hipMalloc(&p, bytes), "hipMalloc");
+        ppf_or_die(ppf_device_alloc(&p, bytes), "ppf_device_alloc");
         allocs.push_back(p);
         if (n) {
             hip_or_die(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
@@ -149,9 +172,16 @@ struct Backend {
             stepper = nullptr;
         }
         for (void *p : allocs) {
-            (void)hipFree(p);
+            ppf_device_free(p);
         }
         allocs.clear();
+        for (Scratch *sc : {&scr_idx, &scr_out, &scr_dyn}) {
+            ppf_device_free(sc->p);
+            *sc = Scratch{};
+        }
+        carried = false;
+        carried_rp.clear();
+        carried_ci.clear();
         if (s) {
             (void)hipStreamDestroy(s);
             s = nullptr;
@@ -299,6 +329,8 @@ struct Backend {
         put_line("matrix_assembly", r.ms_elastic + r.ms_contact, true);
         put_line("linsolve", r.ms_solve, true);
         put_line("line_search", r.ms_line_search, true);
+        put_line("device-alloc", r.device_alloc, true);
+        put_line("device-free", r.device_free, true);
     }
 };
 
@@ -540,15 +572,16 @@ uint32_t check_state(Backend &b, const float *d_x) {
     return count;
 }
 
-uint32_t *device_indices(const uint32_t *idx, uint32_t count, uint32_t nv) {
+// The host's vertex indices, checked, in the backend's index scratch.
+uint32_t *device_indices(Backend &b, const uint32_t *idx, uint32_t count) {
     for (uint32_t k = 0; k < count; ++k) {
-        if (idx[k] >= nv) {
-            fatal(CTS_FATAL_SOLVER, "vertex index %u out of range (%u vertices)", idx[k], nv);
+        if (idx[k] >= b.nv) {
+            fatal(CTS_FATAL_SOLVER, "vertex index %u out of range (%u vertices)", idx[k], b.nv);
         }
     }
-    uint32_t *d = nullptr;
-    hip_or_die(hipMalloc(&d, count * sizeof(uint32_t)), "hipMalloc");
-    hip_or_die(hipMemcpy(d, idx, count * sizeof(uint32_t), hipMemcpyHostToDevice), "hipMemcpy");
+    auto *d = static_cast<uint32_t *>(b.grow(b.scr_idx, count * sizeof(uint32_t)));
+    hip_or_die(hipMemcpyAsync(d, idx, count * sizeof(uint32_t), hipMemcpyHostToDevice, b.s),
+               "hipMemcpyAsync");
     return d;
 }
 
@@ -642,6 +675,7 @@ void advance(cts_StepResult *result) {
     sp.check_intersection = p.disable_contact ? 0 : 1;
     ppf_step_result r{};
     ppf_or_die(ppf_stepper_advance(g.stepper, &g.sc, &sp, &r, g.s), "advance");
+    g.carried = false; // the step's own contact pattern replaces a carried one
     cts_StepResult out{};
     out.ccd_success = r.status != 2;
     out.pcg_success = r.status != 1;
@@ -702,48 +736,57 @@ void fetch_rest_angles(void) {
 void fetch_dyn_counts(uint32_t *n_value, uint32_t *n_offset) {
     require_ready("fetch_dyn_counts");
     uint32_t nnz = 0;
-    ppf_or_die(ppf_stepper_contact_pattern(g.stepper, &nnz, nullptr, nullptr, nullptr, g.s),
-               "fetch_dyn_counts");
+    if (g.carried) {
+        nnz = g.carried_rp[g.nv];
+    } else {
+        ppf_or_die(ppf_stepper_contact_pattern(g.stepper, &nnz, nullptr, nullptr, nullptr, g.s),
+                   "fetch_dyn_counts");
+    }
     if (n_value) *n_value = nnz;
     if (n_offset) *n_offset = g.nv + 1;
 }
 
+// The contact pattern the next step starts from: the last step's contact
+// blocks (upper triangle, rows = vertices), or the pattern update_dyn carried
+// in if no step has run since (main.cu:1970-1986, csrmat.cu:136-153). Values
+// are written when `value` is non-null (the carried pattern has none: zeros).
 void fetch_dyn(uint32_t *index, float *value, uint32_t *offset) {
     require_ready("fetch_dyn");
+    const size_t rows = (size_t)g.nv + 1;
+    if (g.carried) {
+        const uint32_t nnz = g.carried_rp[g.nv];
+        if (offset) std::memcpy(offset, g.carried_rp.data(), rows * sizeof(uint32_t));
+        if (index && nnz) std::memcpy(index, g.carried_ci.data(), nnz * sizeof(uint32_t));
+        if (value && nnz) std::memset(value, 0, 9 * (size_t)nnz * sizeof(float));
+        return;
+    }
     uint32_t nnz = 0;
     ppf_or_die(ppf_stepper_contact_pattern(g.stepper, &nnz, nullptr, nullptr, nullptr, g.s),
                "fetch_dyn");
-    uint32_t *d_off = nullptr, *d_idx = nullptr;
-    float *d_val = nullptr;
-    hip_or_die(hipMalloc(&d_off, ((size_t)g.nv + 1) * sizeof(uint32_t)), "hipMalloc");
-    hip_or_die(hipMalloc(&d_idx, ((size_t)nnz + 1) * sizeof(uint32_t)), "hipMalloc");
-    hip_or_die(hipMalloc(&d_val, (9 * (size_t)nnz + 1) * sizeof(float)), "hipMalloc");
-    ppf_or_die(ppf_stepper_contact_pattern(g.stepper, &nnz, d_off, d_idx, value ? d_val : nullptr,
-                                           g.s),
+    const size_t off_b = rows * 4, idx_b = (size_t)nnz * 4, val_b = value ? 36 * (size_t)nnz : 0;
+    auto *base = static_cast<char *>(g.grow(g.scr_dyn, off_b + idx_b + val_b + 16));
+    auto *d_off = reinterpret_cast<uint32_t *>(base);
+    auto *d_idx = reinterpret_cast<uint32_t *>(base + off_b);
+    auto *d_val = value ? reinterpret_cast<float *>(base + off_b + idx_b) : nullptr;
+    ppf_or_die(ppf_stepper_contact_pattern(g.stepper, &nnz, d_off, d_idx, d_val, g.s),
                "fetch_dyn");
-    hip_or_die(hipStreamSynchronize(g.s), "hipStreamSynchronize");
     if (offset) {
-        hip_or_die(hipMemcpy(offset, d_off, ((size_t)g.nv + 1) * sizeof(uint32_t),
-                             hipMemcpyDeviceToHost),
-                   "fetch_dyn");
+        hip_or_die(hipMemcpyAsync(offset, d_off, off_b, hipMemcpyDeviceToHost, g.s), "fetch_dyn");
     }
     if (index && nnz) {
-        hip_or_die(hipMemcpy(index, d_idx, (size_t)nnz * sizeof(uint32_t), hipMemcpyDeviceToHost),
-                   "fetch_dyn");
+        hip_or_die(hipMemcpyAsync(index, d_idx, idx_b, hipMemcpyDeviceToHost, g.s), "fetch_dyn");
     }
     if (value && nnz) {
-        hip_or_die(hipMemcpy(value, d_val, 9 * (size_t)nnz * sizeof(float),
-                             hipMemcpyDeviceToHost),
-                   "fetch_dyn");
+        hip_or_die(hipMemcpyAsync(value, d_val, val_b, hipMemcpyDeviceToHost, g.s), "fetch_dyn");
     }
-    (void)hipFree(d_off);
-    (void)hipFree(d_idx);
-    (void)hipFree(d_val);
+    hip_or_die(hipStreamSynchronize(g.s), "fetch_dyn");
 }
 
+// A carried-over contact pattern (resume, csrmat.cu:136-153): kept as the
+// pattern fetch_dyn reports until the next step. The stepper rebuilds its
+// contact blocks from the step's own pairs every Newton iteration, so the
+// pattern does not change what a step computes. Checked for well-formedness.
 void update_dyn(const uint32_t *index, const uint32_t *offset) {
-    // A carried-over contact pattern (resume): the pattern is rebuilt every
-    // Newton iteration here, so it is only checked for well-formedness.
     require_ready("update_dyn");
     if (!index || !offset) {
         return;
@@ -761,6 +804,9 @@ void update_dyn(const uint32_t *index, const uint32_t *offset) {
             }
         }
     }
+    g.carried_rp.assign(offset, offset + g.nv + 1);
+    g.carried_ci.assign(index, index + offset[g.nv]);
+    g.carried = true;
 }
 
 void update_constraint(const cts_Constraint *constraint) {
@@ -815,12 +861,11 @@ void override_velocity(const uint32_t *indices, uint32_t count, float vx, float 
     if (count == 0 || dt <= 0.0f) {
         return;
     }
-    uint32_t *d = device_indices(indices, count, g.nv);
+    uint32_t *d = device_indices(g, indices, count);
     k_override_velocity<<<(count + kThreads - 1) / kThreads, kThreads, 0, g.s>>>(
         d, count, g.curr, g.prev, vx * dt, vy * dt, vz * dt);
     hip_or_die(hipGetLastError(), "override_velocity");
     hip_or_die(hipStreamSynchronize(g.s), "override_velocity");
-    (void)hipFree(d);
 }
 
 void gather_current_positions(const uint32_t *indices, uint32_t count, float *out) {
@@ -828,16 +873,14 @@ void gather_current_positions(const uint32_t *indices, uint32_t count, float *ou
     if (count == 0) {
         return;
     }
-    uint32_t *d = device_indices(indices, count, g.nv);
-    float *d_out = nullptr;
-    hip_or_die(hipMalloc(&d_out, 3 * (size_t)count * sizeof(float)), "hipMalloc");
+    uint32_t *d = device_indices(g, indices, count);
+    auto *d_out = static_cast<float *>(g.grow(g.scr_out, 3 * (size_t)count * sizeof(float)));
     k_gather<<<(count + kThreads - 1) / kThreads, kThreads, 0, g.s>>>(d, count, g.curr, d_out);
     hip_or_die(hipGetLastError(), "gather_current_positions");
-    hip_or_die(hipStreamSynchronize(g.s), "gather_current_positions");
-    hip_or_die(hipMemcpy(out, d_out, 3 * (size_t)count * sizeof(float), hipMemcpyDeviceToHost),
+    hip_or_die(hipMemcpyAsync(out, d_out, 3 * (size_t)count * sizeof(float),
+                              hipMemcpyDeviceToHost, g.s),
                "gather_current_positions");
-    (void)hipFree(d);
-    (void)hipFree(d_out);
+    hip_or_die(hipStreamSynchronize(g.s), "gather_current_positions");
 }
 
 void override_angular_velocity(const uint32_t *indices, uint32_t count, float wx, float wy,
@@ -846,12 +889,11 @@ void override_angular_velocity(const uint32_t *indices, uint32_t count, float wx
     if (count == 0 || dt <= 0.0f) {
         return;
     }
-    uint32_t *d = device_indices(indices, count, g.nv);
+    uint32_t *d = device_indices(g, indices, count);
     k_override_angular<<<(count + kThreads - 1) / kThreads, kThreads, 0, g.s>>>(
         d, count, g.curr, g.prev, wx, wy, wz, cx, cy, cz, dt);
     hip_or_die(hipGetLastError(), "override_angular_velocity");
     hip_or_die(hipStreamSynchronize(g.s), "override_angular_velocity");
-    (void)hipFree(d);
 }
 
 void init_collision_windows(const uint32_t *vert_dmap, uint32_t vert_count,

@@ -11,12 +11,21 @@
 #include "narrow.hpp"
 #include "newton.hpp"
 #include "pcg.hpp"
+#include <atomic>
 #include <cstdarg>
 #include <cstring>
 
 namespace ppf {
 
 static thread_local char g_err[1024];
+static std::atomic<uint64_t> g_allocs{0}, g_frees{0};
+
+void count_alloc() { g_allocs.fetch_add(1, std::memory_order_relaxed); }
+void count_free() { g_frees.fetch_add(1, std::memory_order_relaxed); }
+void alloc_totals(uint64_t *allocs, uint64_t *frees) {
+    *allocs = g_allocs.load(std::memory_order_relaxed);
+    *frees = g_frees.load(std::memory_order_relaxed);
+}
 
 void set_error(const char *fmt, ...) {
     va_list ap;
@@ -106,6 +115,24 @@ struct ppf_lpc {
 extern "C" {
 
 const char *ppf_last_error(void) { return g_err; }
+
+int ppf_device_alloc(void **ptr, size_t bytes) {
+    if (!ptr) {
+        set_error("ppf_device_alloc: null pointer");
+        return PPF_ERR_ARG;
+    }
+    PPF_HIP(ppf::dev_alloc(ptr, bytes ? bytes : 1));
+    return PPF_OK;
+}
+
+void ppf_device_free(void *ptr) { (void)ppf::dev_free(ptr); }
+
+void ppf_alloc_counts(uint64_t *allocs, uint64_t *frees) {
+    uint64_t a = 0, f = 0;
+    ppf::alloc_totals(&a, &f);
+    if (allocs) *allocs = a;
+    if (frees) *frees = f;
+}
 
 int ppf_device_info(char *arch, uint32_t arch_len, uint32_t *cu_count) {
     int ndev = 0;

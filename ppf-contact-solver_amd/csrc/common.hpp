@@ -127,7 +127,42 @@ inline uint32_t xcd_grid(uint32_t blocks) {
 // nonzero code otherwise; ppf_last_error() holds the message.
 namespace ppf {
 void set_error(const char *fmt, ...);
+
+// Every device and pinned-host allocation of the library goes through these,
+// which count them (the reference's g_device_alloc_count /
+// g_device_free_count, main/main.cu:280-286, 1635-1646): after warm-up a
+// time step allocates and frees nothing, buffers grow to a high-water mark.
+void count_alloc();
+void count_free();
+void alloc_totals(uint64_t *allocs, uint64_t *frees);
+template <class T> hipError_t dev_alloc(T **p, size_t bytes) {
+    const hipError_t e = hipMalloc(reinterpret_cast<void **>(p), bytes);
+    if (e == hipSuccess) {
+        count_alloc();
+    }
+    return e;
 }
+inline hipError_t dev_free(void *p) {
+    if (p) {
+        count_free();
+    }
+    return hipFree(p);
+}
+template <class T> hipError_t host_alloc(T **p, size_t bytes) {
+    const hipError_t e =
+        hipHostMalloc(reinterpret_cast<void **>(p), bytes, hipHostMallocDefault);
+    if (e == hipSuccess) {
+        count_alloc();
+    }
+    return e;
+}
+inline hipError_t host_free(void *p) {
+    if (p) {
+        count_free();
+    }
+    return hipHostFree(p);
+}
+} // namespace ppf
 
 #define PPF_HIP(call)                                                          \
     do {                                                                       \

@@ -971,7 +971,7 @@ void ElasticAssembler::release() {
                     (void *)vtx_off_, (void *)vtx_src_, (void *)fblk_, (void *)hblk_,
                     (void *)ff_, (void *)hf_, (void *)vf_}) {
         if (p) {
-            (void)hipFree(p);
+            (void)dev_free(p);
         }
     }
     rp_ = ci_ = slot_off_ = slot_src_ = vtx_off_ = vtx_src_ = nullptr;
@@ -999,14 +999,14 @@ int ElasticAssembler::create(const ppf_shell &topo, hipStream_t s) {
     const size_t sw = radix_sort_scratch_words(nmax) > scan_scratch_words(nmax + 1)
                           ? radix_sort_scratch_words(nmax)
                           : scan_scratch_words(nmax + 1);
-    PPF_HIP(hipMalloc(&key, nmax * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&head, (nmax + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&sid, (nmax + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&scratch, sw * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&slot_src_, (size_t)np * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&vtx_src_, (size_t)(nc ? nc : 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&rp_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&vtx_off_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&key, nmax * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&head, (nmax + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&sid, (nmax + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&scratch, sw * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&slot_src_, (size_t)np * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&vtx_src_, (size_t)(nc ? nc : 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&rp_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&vtx_off_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
     // Blocks: stable sort by column, then stable sort by row.
     k_pair_init<<<cdiv(np, kNT), kNT, 0, s>>>(t, np, key, slot_src_);
     PPF_HIP(hipGetLastError());
@@ -1022,8 +1022,8 @@ int ElasticAssembler::create(const ppf_shell &topo, hipStream_t s) {
     PPF_HIP(hipMemcpyAsync(&last_head, head + np - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     PPF_HIP(hipStreamSynchronize(s));
     nnz_ = last_sid + last_head;
-    PPF_HIP(hipMalloc(&ci_, (size_t)nnz_ * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&slot_off_, ((size_t)nnz_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&ci_, (size_t)nnz_ * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&slot_off_, ((size_t)nnz_ + 1) * sizeof(uint32_t)));
     PPF_HIP(hipMemsetAsync(head + np, 0, sizeof(uint32_t), s));
     uint32_t *rcount = key; // reuse: per-row slot counts
     PPF_HIP(hipMemsetAsync(rcount, 0, (size_t)nv_ * sizeof(uint32_t), s));
@@ -1049,16 +1049,16 @@ int ElasticAssembler::create(const ppf_shell &topo, hipStream_t s) {
     k_row_offsets<<<1, 1, 0, s>>>(nv_, nullptr, vtx_off_, nc);
     PPF_HIP(hipGetLastError());
     // Staging: blocks, element forces, activity bytes (after the forces).
-    PPF_HIP(hipMalloc(&fblk_, (9ull * kFacePairs * nf_ + 1) * sizeof(float)));
-    PPF_HIP(hipMalloc(&hblk_, (9ull * kHingePairs * nh_ + 1) * sizeof(float)));
-    PPF_HIP(hipMalloc(&ff_, (9ull * nf_ + 1) * sizeof(float) + nf_ + nh_ + 8));
-    PPF_HIP(hipMalloc(&hf_, (12ull * nh_ + 1) * sizeof(float)));
-    PPF_HIP(hipMalloc(&vf_, (3ull * nv_ + 1) * sizeof(float)));
+    PPF_HIP(dev_alloc(&fblk_, (9ull * kFacePairs * nf_ + 1) * sizeof(float)));
+    PPF_HIP(dev_alloc(&hblk_, (9ull * kHingePairs * nh_ + 1) * sizeof(float)));
+    PPF_HIP(dev_alloc(&ff_, (9ull * nf_ + 1) * sizeof(float) + nf_ + nh_ + 8));
+    PPF_HIP(dev_alloc(&hf_, (12ull * nh_ + 1) * sizeof(float)));
+    PPF_HIP(dev_alloc(&vf_, (3ull * nv_ + 1) * sizeof(float)));
     PPF_HIP(hipStreamSynchronize(s));
-    (void)hipFree(key);
-    (void)hipFree(head);
-    (void)hipFree(sid);
-    (void)hipFree(scratch);
+    (void)dev_free(key);
+    (void)dev_free(head);
+    (void)dev_free(sid);
+    (void)dev_free(scratch);
     return PPF_OK;
 }
 

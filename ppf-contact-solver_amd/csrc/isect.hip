@@ -235,9 +235,9 @@ uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 } // namespace
 
 IntersectionCheck::~IntersectionCheck() {
-    if (rec_) (void)hipFree(rec_);
-    if (ctr_) (void)hipFree(ctr_);
-    if (h_ctr_) (void)hipHostFree(h_ctr_);
+    if (rec_) (void)dev_free(rec_);
+    if (ctr_) (void)dev_free(ctr_);
+    if (h_ctr_) (void)host_free(h_ctr_);
 }
 
 int IntersectionCheck::run(const BvhView &face, const BvhView &edge, const BvhView &vert,
@@ -246,17 +246,16 @@ int IntersectionCheck::run(const BvhView &face, const BvhView &edge, const BvhVi
                            uint32_t surface_vert_count, ppf_isect_record *records,
                            uint32_t cap, uint32_t *count, hipStream_t s) {
     if (!ctr_) {
-        PPF_HIP(hipMalloc(&ctr_, 2 * sizeof(uint32_t)));
-        PPF_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_ctr_), 2 * sizeof(uint32_t),
-                              hipHostMallocDefault));
+        PPF_HIP(dev_alloc(&ctr_, 2 * sizeof(uint32_t)));
+        PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_ctr_), 2 * sizeof(uint32_t)));
     }
     const uint32_t need = cap < 256u ? 256u : cap;
     if (need > rec_cap_) {
         if (rec_) {
-            PPF_HIP(hipFree(rec_));
+            PPF_HIP(dev_free(rec_));
             rec_ = nullptr;
         }
-        PPF_HIP(hipMalloc(&rec_, (size_t)need * sizeof(ppf_isect_record)));
+        PPF_HIP(dev_alloc(&rec_, (size_t)need * sizeof(ppf_isect_record)));
         rec_cap_ = need;
     }
     ppf_static_mesh none{};

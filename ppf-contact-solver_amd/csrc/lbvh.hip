@@ -346,7 +346,7 @@ void Lbvh::release() {
                     (void *)idx_, (void *)scratch_, (void *)cent_, (void *)part_,
                     (void *)box_}) {
         if (p) {
-            (void)hipFree(p);
+            (void)dev_free(p);
         }
     }
     node_ = parent_ = depth_ = level_ = code_ = idx_ = scratch_ = nullptr;
@@ -362,16 +362,16 @@ int Lbvh::reserve(uint32_t n) {
     release();
     const size_t nodes = 2 * (size_t)n;
     const size_t nb = cdiv(n, kNT);
-    PPF_HIP(hipMalloc(&node_, 2 * nodes * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&parent_, nodes * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&depth_, n * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&level_, n * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&code_, n * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&idx_, n * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&scratch_, radix_sort_scratch_words(n) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&cent_, 3 * (size_t)n * sizeof(float)));
-    PPF_HIP(hipMalloc(&part_, (6 * nb + 6) * sizeof(float)));
-    PPF_HIP(hipMalloc(&box_, nodes * sizeof(Aabb)));
+    PPF_HIP(dev_alloc(&node_, 2 * nodes * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&parent_, nodes * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&depth_, n * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&level_, n * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&code_, n * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&idx_, n * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&scratch_, radix_sort_scratch_words(n) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&cent_, 3 * (size_t)n * sizeof(float)));
+    PPF_HIP(dev_alloc(&part_, (6 * nb + 6) * sizeof(float)));
+    PPF_HIP(dev_alloc(&box_, nodes * sizeof(Aabb)));
     cap_ = n;
     return PPF_OK;
 }

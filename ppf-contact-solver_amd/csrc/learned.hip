@@ -797,10 +797,10 @@ template <class T> T *carve(char *&cur, size_t count) {
 Lpc::~Lpc() { release(); }
 
 void Lpc::release() {
-    if (mem_) (void)hipFree(mem_);
+    if (mem_) (void)dev_free(mem_);
     for (void *p : {(void *)w_, (void *)grad_, (void *)am_, (void *)av_, (void *)wt_, (void *)wr_,
                     (void *)pv_, (void *)pav_, (void *)py_}) {
-        if (p) (void)hipFree(p);
+        if (p) (void)dev_free(p);
     }
     mem_ = nullptr;
     w_ = grad_ = am_ = av_ = pv_ = pav_ = py_ = nullptr;
@@ -820,12 +820,12 @@ int Lpc::set_weights(const float *host, uint32_t count, hipStream_t s) {
     }
     if (!w_) {
         const size_t b = LpcLayout::count * sizeof(float);
-        PPF_HIP(hipMalloc(&w_, b));
-        PPF_HIP(hipMalloc(&grad_, b));
-        PPF_HIP(hipMalloc(&am_, b));
-        PPF_HIP(hipMalloc(&av_, b));
-        PPF_HIP(hipMalloc(&wt_, kDenseTotal * sizeof(uint16_t)));
-        PPF_HIP(hipMalloc(&wr_, kDenseTotal * sizeof(uint16_t)));
+        PPF_HIP(dev_alloc(&w_, b));
+        PPF_HIP(dev_alloc(&grad_, b));
+        PPF_HIP(dev_alloc(&am_, b));
+        PPF_HIP(dev_alloc(&av_, b));
+        PPF_HIP(dev_alloc(&wt_, kDenseTotal * sizeof(uint16_t)));
+        PPF_HIP(dev_alloc(&wr_, kDenseTotal * sizeof(uint16_t)));
     }
     PPF_HIP(hipMemcpyAsync(w_, host, LpcLayout::count * sizeof(float), hipMemcpyHostToDevice, s));
     PPF_HIP(hipMemsetAsync(am_, 0, LpcLayout::count * sizeof(float), s));
@@ -849,7 +849,7 @@ int Lpc::reserve(uint32_t n, uint32_t nnz, uint32_t E) {
         return PPF_OK;
     }
     if (mem_) {
-        (void)hipFree(mem_);
+        (void)dev_free(mem_);
         mem_ = nullptr;
     }
     ncap_ = std::max(n, ncap_), nzcap_ = std::max(nnz, nzcap_), ecap_ = std::max(E, ecap_);
@@ -867,7 +867,7 @@ int Lpc::reserve(uint32_t n, uint32_t nnz, uint32_t E) {
                    b(kH * rows, 2) * 2 + b(2 * kH * N, 4) + b(kH * EE, 4) + b(part_words, 4) +
                    b(scan_scratch_words(N + 1), 4) + b(10 * N, 2) + b(10 * Z, 2) +
                    b(10 * ((size_t)N + Z), 2);
-    PPF_HIP(hipMalloc(&mem_, bytes));
+    PPF_HIP(dev_alloc(&mem_, bytes));
     mem_bytes_ = bytes;
     char *cur = static_cast<char *>(mem_);
     eoff_ = carve<uint32_t>(cur, N + 1);
@@ -1017,11 +1017,11 @@ int Lpc::gradient(PcgPlanImpl *plan, const float *val, const float *cdiag, const
     }
     if (K > nprobe_cap_ || !pv_) {
         for (float *p : {pv_, pav_, py_}) {
-            if (p) (void)hipFree(p);
+            if (p) (void)dev_free(p);
         }
-        PPF_HIP(hipMalloc(&pv_, (size_t)K * 3 * n * sizeof(float)));
-        PPF_HIP(hipMalloc(&pav_, (size_t)K * 3 * n * sizeof(float)));
-        PPF_HIP(hipMalloc(&py_, (size_t)K * 3 * n * sizeof(float)));
+        PPF_HIP(dev_alloc(&pv_, (size_t)K * 3 * n * sizeof(float)));
+        PPF_HIP(dev_alloc(&pav_, (size_t)K * 3 * n * sizeof(float)));
+        PPF_HIP(dev_alloc(&py_, (size_t)K * 3 * n * sizeof(float)));
         nprobe_cap_ = K;
     }
     PPF_HIP(hipMemsetAsync(grad_, 0, LpcLayout::count * sizeof(float), s));

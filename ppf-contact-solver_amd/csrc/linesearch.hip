@@ -337,17 +337,16 @@ int ccd_batch(int kind, uint32_t n, const float *x0, const float *x1, const floa
 }
 
 LineSearch::~LineSearch() {
-    if (out_) (void)hipFree(out_);
-    if (h_out_) (void)hipHostFree(h_out_);
+    if (out_) (void)dev_free(out_);
+    if (h_out_) (void)host_free(h_out_);
 }
 
 int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
                     const ppf_ls_params &lp, bool rebuild, float *toi, int *overlap,
                     hipStream_t s) {
     if (!out_) {
-        PPF_HIP(hipMalloc(&out_, 4 * sizeof(uint32_t)));
-        PPF_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_out_), 4 * sizeof(uint32_t),
-                              hipHostMallocDefault));
+        PPF_HIP(dev_alloc(&out_, 4 * sizeof(uint32_t)));
+        PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_out_), 4 * sizeof(uint32_t)));
     }
     ppf_static_mesh none{};
     const ppf_static_mesh &S = sm ? *sm : none;

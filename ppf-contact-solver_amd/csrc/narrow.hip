@@ -491,37 +491,37 @@ uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 } // namespace
 
 Narrow::~Narrow() {
-    if (cnt_) (void)hipFree(cnt_);
-    if (ofs_) (void)hipFree(ofs_);
-    if (scratch_) (void)hipFree(scratch_);
-    if (stash_) (void)hipFree(stash_);
-    if (redo_) (void)hipFree(redo_);
-    if (flags_) (void)hipFree(flags_);
-    if (h_flags_) (void)hipHostFree(h_flags_);
+    if (cnt_) (void)dev_free(cnt_);
+    if (ofs_) (void)dev_free(ofs_);
+    if (scratch_) (void)dev_free(scratch_);
+    if (stash_) (void)dev_free(stash_);
+    if (redo_) (void)dev_free(redo_);
+    if (flags_) (void)dev_free(flags_);
+    if (h_flags_) (void)host_free(h_flags_);
 }
 
 int Narrow::reserve(uint32_t queries) {
     if (queries <= qcap_) {
         return PPF_OK;
     }
-    if (cnt_) (void)hipFree(cnt_);
-    if (ofs_) (void)hipFree(ofs_);
-    if (scratch_) (void)hipFree(scratch_);
-    if (stash_) (void)hipFree(stash_);
-    if (redo_) (void)hipFree(redo_);
+    if (cnt_) (void)dev_free(cnt_);
+    if (ofs_) (void)dev_free(ofs_);
+    if (scratch_) (void)dev_free(scratch_);
+    if (stash_) (void)dev_free(stash_);
+    if (redo_) (void)dev_free(redo_);
     cnt_ = ofs_ = scratch_ = redo_ = nullptr;
     stash_ = nullptr;
     qcap_ = 0;
-    PPF_HIP(hipMalloc(&cnt_, (queries + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&ofs_, (queries + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&scratch_, scan_scratch_words(queries + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&stash_, (size_t)queries * kStash * sizeof(ppf_pair)));
-    PPF_HIP(hipMalloc(&redo_, (size_t)queries * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&cnt_, (queries + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&ofs_, (queries + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&scratch_, scan_scratch_words(queries + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&stash_, (size_t)queries * kStash * sizeof(ppf_pair)));
+    PPF_HIP(dev_alloc(&redo_, (size_t)queries * sizeof(uint32_t)));
     if (!flags_) {
-        PPF_HIP(hipMalloc(&flags_, 4 * sizeof(uint32_t)));
+        PPF_HIP(dev_alloc(&flags_, 4 * sizeof(uint32_t)));
         // four flag words, then the pair total
-        PPF_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_flags_),
-                              8 * sizeof(uint32_t), hipHostMallocDefault));
+        PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_flags_),
+                              8 * sizeof(uint32_t)));
     }
     qcap_ = queries;
     return PPF_OK;

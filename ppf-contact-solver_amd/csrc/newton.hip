@@ -18,6 +18,7 @@
 // scalars that steer the loop (pair counts, toi, PCG status).
 #include "newton.hpp"
 #include "scan.hpp"
+#include "sort.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -94,101 +95,147 @@ __global__ void k_add(uint64_t n, const float *__restrict__ a, const float *__re
     }
 }
 
-// Union of two upper block CSRs (A = elastic pattern, B = contact blocks):
-// per-row counts, then the merged columns and values (A + B where both).
+// Merged upper block CSR of the Newton system, A = elastic pattern, B =
+// contact blocks (A + B where both hold a column), with the Dirichlet
+// elimination of pinned vertices (main.cu:954-1049) folded in: a block that
+// touches a pinned vertex is not kept (the reference zeroes it in place; the
+// solve then reads no all-zero blocks and a pinned row is its diagonal only),
+// and its coupling is lifted into the free row's right-hand side:
+//   row i free, column j pinned:  f_i -= B_ij p_j       (own lift)
+//   row i pinned, column j free:  f_j -= B_ij^T p_i     (mirror lift)
+// with p = eval_x - target on the pinned vertex. The reference scatters the
+// lifts with float atomics; here a free row takes its own lifts in entry
+// order, then its mirror lifts in (source row, entry) order, so every run
+// sums in the same order. mask == null: no pins, nothing is eliminated.
+__device__ inline bool pinned(const uint32_t *mask, uint32_t v) {
+    return mask && mask[v] != 0u;
+}
+
 __global__ void k_merge_count(uint32_t n, const uint32_t *__restrict__ rpa,
                               const uint32_t *__restrict__ cia,
                               const uint32_t *__restrict__ rpb,
-                              const uint32_t *__restrict__ cib, uint32_t *__restrict__ cnt) {
+                              const uint32_t *__restrict__ cib,
+                              const uint32_t *__restrict__ mask, uint32_t *__restrict__ cnt,
+                              uint32_t *__restrict__ mcnt) {
     const uint32_t i = blockIdx.x * kNT + threadIdx.x;
     if (i >= n) {
         return;
     }
-    uint32_t a = rpa[i], ae = rpa[i + 1], b = rpb[i], be = rpb[i + 1], c = 0;
+    const bool mi = pinned(mask, i);
+    uint32_t a = rpa[i], ae = rpa[i + 1], b = rpb[i], be = rpb[i + 1], c = 0, m = 0;
     while (a < ae || b < be) {
+        uint32_t j;
         if (b >= be || (a < ae && cia[a] < cib[b])) {
-            ++a;
+            j = cia[a++];
         } else if (a >= ae || cib[b] < cia[a]) {
-            ++b;
+            j = cib[b++];
         } else {
+            j = cia[a];
             ++a;
             ++b;
         }
-        ++c;
+        const bool mj = pinned(mask, j);
+        c += (!mi && !mj) ? 1u : 0u;
+        m += (mi && !mj) ? 1u : 0u;
     }
     cnt[i] = c;
+    if (mcnt) {
+        mcnt[i] = m;
+    }
 }
 
+// Kept blocks, own lifts, and one record per mirror lift: the free row it
+// goes to (key), and the lifted vector B_ij^T p_i (3 floats at rec[3 l]).
 __global__ void k_merge_fill(uint32_t n, const uint32_t *__restrict__ rpa,
                              const uint32_t *__restrict__ cia, const float *__restrict__ va,
                              const uint32_t *__restrict__ rpb, const uint32_t *__restrict__ cib,
-                             const float *__restrict__ vb, const uint32_t *__restrict__ rpm,
-                             uint32_t *__restrict__ cim, float *__restrict__ vm) {
+                             const float *__restrict__ vb, const uint32_t *__restrict__ mask,
+                             const float *__restrict__ ex, const float *__restrict__ target,
+                             const uint32_t *__restrict__ rpm, uint32_t *__restrict__ cim,
+                             float *__restrict__ vm, const uint32_t *__restrict__ lrp,
+                             uint32_t *__restrict__ lkey, uint32_t *__restrict__ lidx,
+                             float *__restrict__ lrec, float *__restrict__ force) {
     const uint32_t i = blockIdx.x * kNT + threadIdx.x;
     if (i >= n) {
         return;
     }
+    const bool mi = pinned(mask, i);
     uint32_t a = rpa[i], ae = rpa[i + 1], b = rpb[i], be = rpb[i + 1], o = rpm[i];
+    uint32_t l = mi ? lrp[i] : 0u;
+    f3 pi = mk3(0.0f, 0.0f, 0.0f);
+    if (mi) {
+        pi = X(ex, i) - X(target, i);
+    }
+    float f[3] = {0.0f, 0.0f, 0.0f};
+    bool lifted = false;
+    if (mask && !mi) {
+        const f3 fi = X(force, i);
+        f[0] = fi.x, f[1] = fi.y, f[2] = fi.z;
+    }
     while (a < ae || b < be) {
-        float *dst = vm + 9 * (size_t)o;
+        float blk[9];
+        uint32_t j;
         if (b >= be || (a < ae && cia[a] < cib[b])) {
-            cim[o] = cia[a];
-            for (int e = 0; e < 9; ++e) dst[e] = va[9 * (size_t)a + e];
+            j = cia[a];
+            for (int e = 0; e < 9; ++e) blk[e] = va[9 * (size_t)a + e];
             ++a;
         } else if (a >= ae || cib[b] < cia[a]) {
-            cim[o] = cib[b];
-            for (int e = 0; e < 9; ++e) dst[e] = vb[9 * (size_t)b + e];
+            j = cib[b];
+            for (int e = 0; e < 9; ++e) blk[e] = vb[9 * (size_t)b + e];
             ++b;
         } else {
-            cim[o] = cia[a];
-            for (int e = 0; e < 9; ++e) dst[e] = va[9 * (size_t)a + e] + vb[9 * (size_t)b + e];
+            j = cia[a];
+            for (int e = 0; e < 9; ++e) blk[e] = va[9 * (size_t)a + e] + vb[9 * (size_t)b + e];
             ++a;
             ++b;
         }
-        ++o;
+        const bool mj = pinned(mask, j);
+        if (!mi && !mj) {
+            cim[o] = j;
+            float *dst = vm + 9 * (size_t)o;
+            for (int e = 0; e < 9; ++e) dst[e] = blk[e];
+            ++o;
+        } else if (!mi) { // free row, pinned column: f_i -= B_ij p_j
+            const f3 p = X(ex, j) - X(target, j);
+            for (int r = 0; r < 3; ++r) {
+                const float c = (blk[r] * p.x + blk[r + 3] * p.y) + blk[r + 6] * p.z;
+                f[r] += -c;
+            }
+            lifted = true;
+        } else if (!mj) { // pinned row, free column: f_j -= B_ij^T p_i, later
+            float *rec = lrec + 3 * (size_t)l;
+            for (int r = 0; r < 3; ++r) {
+                rec[r] = (blk[3 * r] * pi.x + blk[3 * r + 1] * pi.y) + blk[3 * r + 2] * pi.z;
+            }
+            lkey[l] = j;
+            lidx[l] = l;
+            ++l;
+        }
+    }
+    if (lifted) {
+        store3(force + 3 * (size_t)i, mk3(f[0], f[1], f[2]));
     }
 }
 
-// Dirichlet elimination (main.cu:954-1049): move the coupling of free rows to
-// pinned columns into the right-hand side, zero every block touching a pinned
-// vertex. The lifts into other rows use float atomics, as the reference.
-__global__ void k_dirichlet_lift(uint32_t n, const uint32_t *__restrict__ mask,
-                                 const uint32_t *__restrict__ rp,
-                                 const uint32_t *__restrict__ ci, float *__restrict__ val,
-                                 const float *__restrict__ ex, const float *__restrict__ target,
-                                 float *__restrict__ force) {
-    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
-    if (i >= n) {
+// Mirror lifts, records sorted (stably) by the free row they go to: the
+// first record of each row's run subtracts the run in order.
+__global__ void k_mirror_lift(uint32_t nl, const uint32_t *__restrict__ lkey,
+                              const uint32_t *__restrict__ lidx,
+                              const float *__restrict__ lrec, float *__restrict__ force) {
+    const uint32_t t = blockIdx.x * kNT + threadIdx.x;
+    if (t >= nl || (t > 0 && lkey[t - 1] == lkey[t])) {
         return;
     }
-    const bool mi = mask[i] != 0u;
-    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
-        const uint32_t j = ci[k];
-        const bool mj = mask[j] != 0u;
-        float *b = val + 9 * (size_t)k;
-        if (mi != mj) {
-            if (mj) {
-                const f3 p = X(ex, j) - X(target, j);
-                const float c0 = b[0] * p.x + b[3] * p.y + b[6] * p.z;
-                const float c1 = b[1] * p.x + b[4] * p.y + b[7] * p.z;
-                const float c2 = b[2] * p.x + b[5] * p.y + b[8] * p.z;
-                atomicAdd(force + 3 * (size_t)i, -c0);
-                atomicAdd(force + 3 * (size_t)i + 1, -c1);
-                atomicAdd(force + 3 * (size_t)i + 2, -c2);
-            } else {
-                const f3 p = X(ex, i) - X(target, i);
-                const float c0 = b[0] * p.x + b[1] * p.y + b[2] * p.z;
-                const float c1 = b[3] * p.x + b[4] * p.y + b[5] * p.z;
-                const float c2 = b[6] * p.x + b[7] * p.y + b[8] * p.z;
-                atomicAdd(force + 3 * (size_t)j, -c0);
-                atomicAdd(force + 3 * (size_t)j + 1, -c1);
-                atomicAdd(force + 3 * (size_t)j + 2, -c2);
-            }
-        }
-        if (mi || mj) {
-            for (int e = 0; e < 9; ++e) b[e] = 0.0f;
+    const uint32_t row = lkey[t];
+    const f3 fi = X(force, row);
+    float f[3] = {fi.x, fi.y, fi.z};
+    for (uint32_t u = t; u < nl && lkey[u] == row; ++u) {
+        const float *rec = lrec + 3 * (size_t)lidx[u];
+        for (int r = 0; r < 3; ++r) {
+            f[r] += -rec[r];
         }
     }
+    store3(force + 3 * (size_t)row, mk3(f[0], f[1], f[2]));
 }
 
 __global__ void k_dirichlet_diag(uint32_t n, const uint32_t *__restrict__ mask,
@@ -250,13 +297,14 @@ void Stepper::release() {
     for (void *p : {(void *)ex_, (void *)target_, (void *)dx_, (void *)force_, (void *)cdiag_,
                     (void *)eval_, (void *)cgrad_, (void *)mask_, (void *)scal_,
                     (void *)pairs_, (void *)mrp_, (void *)mci_, (void *)mcnt_,
-                    (void *)scan_, (void *)mval_, (void *)order_}) {
+                    (void *)scan_, (void *)mval_, (void *)order_, (void *)lcnt_,
+                    (void *)lrp_, (void *)lbuf_}) {
         if (p) {
-            (void)hipFree(p);
+            (void)dev_free(p);
         }
     }
     if (h_scal_) {
-        (void)hipHostFree(h_scal_);
+        (void)host_free(h_scal_);
     }
     if (plan_) {
         pcg_plan_destroy(plan_);
@@ -274,8 +322,9 @@ void Stepper::release() {
     order_changed_ = false;
     pairs_ = nullptr;
     mrp_ = mci_ = mcnt_ = scan_ = nullptr;
+    lcnt_ = lrp_ = lbuf_ = nullptr;
     mval_ = nullptr;
-    pair_cap_ = mcap_ = 0;
+    pair_cap_ = mcap_ = lcap_ = 0;
 }
 
 namespace {
@@ -396,21 +445,22 @@ int Stepper::create(const ppf_scene &sc, hipStream_t s) {
         return rc;
     }
     const size_t n3 = 3 * (size_t)nv_;
-    PPF_HIP(hipMalloc(&ex_, n3 * sizeof(float)));
-    PPF_HIP(hipMalloc(&target_, n3 * sizeof(float)));
-    PPF_HIP(hipMalloc(&dx_, n3 * sizeof(float)));
-    PPF_HIP(hipMalloc(&force_, n3 * sizeof(float)));
-    PPF_HIP(hipMalloc(&cgrad_, n3 * sizeof(float)));
-    PPF_HIP(hipMalloc(&cdiag_, 9 * (size_t)nv_ * sizeof(float)));
-    PPF_HIP(hipMalloc(&eval_, 9 * ((size_t)elastic_.nnz() + 1) * sizeof(float)));
-    PPF_HIP(hipMalloc(&mask_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&order_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&scal_, 4 * sizeof(uint32_t)));
-    PPF_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_scal_), 4 * sizeof(uint32_t),
-                          hipHostMallocDefault));
-    PPF_HIP(hipMalloc(&mrp_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&mcnt_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&scan_, scan_scratch_words(nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&ex_, n3 * sizeof(float)));
+    PPF_HIP(dev_alloc(&target_, n3 * sizeof(float)));
+    PPF_HIP(dev_alloc(&dx_, n3 * sizeof(float)));
+    PPF_HIP(dev_alloc(&force_, n3 * sizeof(float)));
+    PPF_HIP(dev_alloc(&cgrad_, n3 * sizeof(float)));
+    PPF_HIP(dev_alloc(&cdiag_, 9 * (size_t)nv_ * sizeof(float)));
+    PPF_HIP(dev_alloc(&eval_, 9 * ((size_t)elastic_.nnz() + 1) * sizeof(float)));
+    PPF_HIP(dev_alloc(&mask_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&order_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&scal_, 4 * sizeof(uint32_t)));
+    PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_scal_), 4 * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&mrp_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&mcnt_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&lcnt_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&lrp_, ((size_t)nv_ + 1) * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&scan_, scan_scratch_words(nv_ + 1) * sizeof(uint32_t)));
     for (hipEvent_t &e : ev_) {
         PPF_HIP(hipEventCreate(&e));
     }
@@ -456,11 +506,11 @@ int Stepper::reserve_pairs(uint64_t n) {
         return PPF_OK;
     }
     if (pairs_) {
-        PPF_HIP(hipFree(pairs_));
+        PPF_HIP(dev_free(pairs_));
         pairs_ = nullptr;
     }
     const uint64_t cap = n + n / 2 + 1024;
-    PPF_HIP(hipMalloc(&pairs_, cap * sizeof(ppf_pair)));
+    PPF_HIP(dev_alloc(&pairs_, cap * sizeof(ppf_pair)));
     pair_cap_ = cap;
     return PPF_OK;
 }
@@ -469,14 +519,29 @@ int Stepper::reserve_merged(uint64_t nnz) {
     if (nnz <= mcap_) {
         return PPF_OK;
     }
-    if (mci_) (void)hipFree(mci_);
-    if (mval_) (void)hipFree(mval_);
+    if (mci_) (void)dev_free(mci_);
+    if (mval_) (void)dev_free(mval_);
     mci_ = nullptr;
     mval_ = nullptr;
     const uint64_t cap = nnz + nnz / 4 + 1024;
-    PPF_HIP(hipMalloc(&mci_, cap * sizeof(uint32_t)));
-    PPF_HIP(hipMalloc(&mval_, 9 * cap * sizeof(float)));
+    PPF_HIP(dev_alloc(&mci_, cap * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&mval_, 9 * cap * sizeof(float)));
     mcap_ = cap;
+    return PPF_OK;
+}
+
+// Mirror-lift records: keys, indices, 3 floats each, and the sort's scratch.
+int Stepper::reserve_lifts(uint64_t nl) {
+    if (nl <= lcap_) {
+        return PPF_OK;
+    }
+    if (lbuf_) {
+        (void)dev_free(lbuf_);
+        lbuf_ = nullptr;
+    }
+    const uint64_t cap = nl + nl / 2 + 256;
+    PPF_HIP(dev_alloc(&lbuf_, (5 * cap + radix_sort_scratch_words(cap)) * sizeof(uint32_t)));
+    lcap_ = cap;
     return PPF_OK;
 }
 
@@ -576,28 +641,50 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
     k_add<<<cdiv(n3, kNT), kNT, 0, s>>>(n3, force_, cgrad_, force_);
     PPF_HIP(hipGetLastError());
     PPF_HIP(hipEventRecord(ev_[2], s));
-    // Merged matrix, Dirichlet elimination, PCG.
+    // Merged matrix with the Dirichlet elimination folded in, then PCG.
+    const uint32_t *pm = npinned_ ? mask_ : nullptr;
     k_merge_count<<<cdiv(n, kNT), kNT, 0, s>>>(n, elastic_.row_offset(), elastic_.col_index(),
-                                                contact_.row_offset(), contact_.cols(), mcnt_);
+                                                contact_.row_offset(), contact_.cols(), pm,
+                                                mcnt_, pm ? lcnt_ : nullptr);
     PPF_HIP(hipGetLastError());
     PPF_HIP(hipMemsetAsync(mcnt_ + n, 0, sizeof(uint32_t), s));
     PPF_HIP(exclusive_scan_u32(mcnt_, mrp_, n + 1, scan_, s));
-    uint32_t nnz = 0;
-    PPF_HIP(hipMemcpyAsync(&nnz, mrp_ + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipMemcpyAsync(h_scal_, mrp_ + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    h_scal_[1] = 0;
+    if (pm) {
+        PPF_HIP(hipMemsetAsync(lcnt_ + n, 0, sizeof(uint32_t), s));
+        PPF_HIP(exclusive_scan_u32(lcnt_, lrp_, n + 1, scan_, s));
+        PPF_HIP(hipMemcpyAsync(h_scal_ + 1, lrp_ + n, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               s));
+    }
     PPF_HIP(hipStreamSynchronize(s));
+    const uint32_t nnz = h_scal_[0], nlift = h_scal_[1];
     rc = reserve_merged(nnz);
+    if (rc == PPF_OK) {
+        rc = reserve_lifts(nlift);
+    }
     if (rc != PPF_OK) {
         return rc;
     }
     sys_nnz_ = nnz;
-    k_merge_fill<<<cdiv(n, kNT), kNT, 0, s>>>(n, elastic_.row_offset(), elastic_.col_index(),
-                                               eval_, contact_.row_offset(), contact_.cols(),
-                                               contact_.values(), mrp_, mci_, mval_);
+    uint32_t *lkey = lbuf_, *lidx = lbuf_ + lcap_;
+    float *lrec = reinterpret_cast<float *>(lbuf_ + 2 * lcap_);
+    uint32_t *lsort = lbuf_ + 5 * lcap_;
+    k_merge_fill<<<cdiv(n, kNT), kNT, 0, s>>>(
+        n, elastic_.row_offset(), elastic_.col_index(), eval_, contact_.row_offset(),
+        contact_.cols(), contact_.values(), pm, ex_, target_, mrp_, mci_, mval_, lrp_, lkey,
+        lidx, lrec, force_);
     PPF_HIP(hipGetLastError());
-    if (npinned_) {
-        k_dirichlet_lift<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, mrp_, mci_, mval_, ex_,
-                                                      target_, force_);
+    if (nlift) {
+        uint32_t bits = 1;
+        while (bits < 32 && (n >> bits) != 0) {
+            ++bits;
+        }
+        PPF_HIP(radix_sort_pairs(lkey, lidx, nlift, bits, lsort, s));
+        k_mirror_lift<<<cdiv(nlift, kNT), kNT, 0, s>>>(nlift, lkey, lidx, lrec, force_);
         PPF_HIP(hipGetLastError());
+    }
+    if (pm) {
         k_dirichlet_diag<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, ex_, target_, cdiag_, force_);
         PPF_HIP(hipGetLastError());
     }
@@ -729,6 +816,8 @@ int Stepper::advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_res
                      hipStream_t s) {
     *out = ppf_step_result{};
     records_count_ = 0;
+    uint64_t alloc0 = 0, free0 = 0;
+    alloc_totals(&alloc0, &free0);
     const uint32_t n = nv_;
     const size_t n3 = 3 * (size_t)n;
     PPF_HIP(hipEventRecord(ev_[5], s));
@@ -813,6 +902,10 @@ int Stepper::advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_res
     PPF_HIP(hipEventRecord(ev_[6], s));
     PPF_HIP(hipEventSynchronize(ev_[6]));
     PPF_HIP(hipEventElapsedTime(&out->ms_total, ev_[5], ev_[6]));
+    uint64_t alloc1 = 0, free1 = 0;
+    alloc_totals(&alloc1, &free1);
+    out->device_alloc = (uint32_t)(alloc1 - alloc0);
+    out->device_free = (uint32_t)(free1 - free0);
     return PPF_OK;
 }
 

@@ -29,7 +29,8 @@ class Stepper {
     uint32_t records(ppf_isect_record *out, uint32_t cap) const;
     const ContactAssembler &contact() const { return contact_; }
     // The last Newton system solved (merged upper blocks after Dirichlet
-    // elimination, per-row diagonal, right-hand side).
+    // elimination -- no block touches a pinned vertex --, per-row diagonal,
+    // right-hand side).
     void system(uint32_t *nnz, const uint32_t **rp, const uint32_t **ci, const float **val,
                 const float **cdiag, const float **rhs) const {
         *nnz = sys_nnz_, *rp = mrp_, *ci = mci_, *val = mval_, *cdiag = cdiag_, *rhs = force_;
@@ -43,6 +44,7 @@ class Stepper {
                          float *recale_out, hipStream_t s);
     int reserve_pairs(uint64_t n);
     int reserve_merged(uint64_t nnz);
+    int reserve_lifts(uint64_t nl);
     void release();
     uint32_t nv_ = 0, ne_ = 0, nf_ = 0, nh_ = 0;
     ElasticAssembler elastic_;
@@ -61,6 +63,9 @@ class Stepper {
     uint32_t *mrp_ = nullptr, *mci_ = nullptr, *mcnt_ = nullptr, *scan_ = nullptr;
     float *mval_ = nullptr;
     uint64_t mcap_ = 0;
+    // Dirichlet mirror lifts: per-row counts / offsets, records (grown)
+    uint32_t *lcnt_ = nullptr, *lrp_ = nullptr, *lbuf_ = nullptr;
+    uint64_t lcap_ = 0;
     uint32_t npinned_ = 0;
     PcgPlanImpl *plan_ = nullptr;
     Lpc *lpc_ = nullptr;

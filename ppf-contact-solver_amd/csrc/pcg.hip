@@ -263,11 +263,18 @@ __global__ void k_build_ell(uint32_t n, uint32_t W, const uint32_t *__restrict__
 // the other row, transposed), and values, right-hand side and iterate are
 // permuted on the way in and out of each solve.
 
+// inv[order[k]] = k, with inv preset to kNone: an entry >= n is flagged
+// before any write, and a slot claimed twice (a duplicate, which leaves
+// another slot unset) is flagged by the exchange.
 __global__ void k_inverse_order(uint32_t n, const uint32_t *__restrict__ order,
-                                uint32_t *__restrict__ inv) {
+                                uint32_t *__restrict__ inv, uint32_t *__restrict__ bad) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) {
-        inv[order[k]] = k;
+    if (k >= n) {
+        return;
+    }
+    const uint32_t v = order[k];
+    if (v >= n || atomicExch(inv + v, k) != kNone) {
+        atomicAdd(bad, 1u);
     }
 }
 
@@ -1179,22 +1186,22 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
     }
     if (2 * (size_t)nnz > pl->pat_cap) {
         if (pl->pat) {
-            PPF_HIP(hipFree(pl->pat));
+            PPF_HIP(dev_free(pl->pat));
             pl->pat = nullptr;
         }
         const size_t cap = 2 * ((size_t)nnz + nnz / 4 + 256);
-        PPF_HIP(hipMalloc(&pl->pat, cap * sizeof(uint32_t)));
+        PPF_HIP(dev_alloc(&pl->pat, cap * sizeof(uint32_t)));
         pl->pat_cap = cap;
     }
     uint32_t *tcol = pl->pat, *tslot = pl->pat + pl->pat_cap / 2;
     const size_t sort_words = 3 * (size_t)nnz + radix_sort_scratch_words(nnz);
     if (sort_words > pl->tsort_cap) {
         if (pl->tsort) {
-            PPF_HIP(hipFree(pl->tsort));
+            PPF_HIP(dev_free(pl->tsort));
             pl->tsort = nullptr;
         }
         const size_t cap = sort_words + sort_words / 4 + 256;
-        PPF_HIP(hipMalloc(&pl->tsort, cap * sizeof(uint32_t)));
+        PPF_HIP(dev_alloc(&pl->tsort, cap * sizeof(uint32_t)));
         pl->tsort_cap = cap;
     }
     pl->nnz = nnz;
@@ -1271,10 +1278,10 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
         const uint32_t W = 4 * h.ell_m;
         if ((size_t)n * W > pl->ell_cap) {
             if (pl->ell) {
-                PPF_HIP(hipFree(pl->ell));
+                PPF_HIP(dev_free(pl->ell));
                 pl->ell = nullptr;
             }
-            PPF_HIP(hipMalloc(&pl->ell, (size_t)n * W * sizeof(uint2)));
+            PPF_HIP(dev_alloc(&pl->ell, (size_t)n * W * sizeof(uint2)));
             pl->ell_cap = (size_t)n * W;
         }
         h.ent = static_cast<const uint2 *>(pl->ell);
@@ -1299,12 +1306,12 @@ int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_
     }
     if ((size_t)nnz + 1 > pl->ecap) {
         if (pl->emem) {
-            PPF_HIP(hipFree(pl->emem));
+            PPF_HIP(dev_free(pl->emem));
             pl->emem = nullptr;
             pl->ecap = 0;
         }
         const size_t cap = (size_t)nnz + nnz / 4 + 256;
-        PPF_HIP(hipMalloc(&pl->emem, cap * (4 + 4 + 36)));
+        PPF_HIP(dev_alloc(&pl->emem, cap * (4 + 4 + 36)));
         pl->ecap = cap;
         pl->pcol = static_cast<uint32_t *>(pl->emem);
         pl->smap = pl->pcol + cap;
@@ -1418,11 +1425,11 @@ int set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st,
             return PPF_ERR_ARG;
         }
         if (pl->tmem) {
-            PPF_HIP(hipFree(pl->tmem));
+            PPF_HIP(dev_free(pl->tmem));
             pl->tmem = nullptr;
             pl->tmem_cap = 0;
         }
-        PPF_HIP(hipMalloc(&pl->tmem, bytes));
+        PPF_HIP(dev_alloc(&pl->tmem, bytes));
         pl->tmem_cap = bytes;
     }
     char *cur = static_cast<char *>(pl->tmem);
@@ -1550,9 +1557,9 @@ int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const u
                    carve_bytes(9 * (size_t)n, 4) + carve_bytes(n3, 4) * 5 +
                    carve_bytes(1, sizeof(Args)) + carve_bytes(scan_words, 4) +
                    carve_bytes(4, 4) + carve_bytes(nscene + 1, 4);
-    hipError_t e = hipMalloc(&pl->mem, bytes);
+    hipError_t e = dev_alloc(&pl->mem, bytes);
     if (e != hipSuccess) {
-        set_error("pcg_plan_create: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        set_error("pcg_plan_create: dev_alloc(%zu): %s", bytes, hipGetErrorString(e));
         delete pl;
         return PPF_ERR_HIP;
     }
@@ -1569,8 +1576,7 @@ int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const u
     pl->scan_tmp = carve<uint32_t>(cur, scan_words);
     pl->flags = carve<uint32_t>(cur, 4);
     uint32_t *d_srow = carve<uint32_t>(cur, nscene + 1);
-    e = hipHostMalloc(reinterpret_cast<void **>(&pl->h_scal), nscene * sizeof(Scalars),
-                      hipHostMallocDefault);
+    e = host_alloc(reinterpret_cast<void **>(&pl->h_scal), nscene * sizeof(Scalars));
     if (e != hipSuccess) {
         set_error("pcg_plan_create: hipHostMalloc: %s", hipGetErrorString(e));
         ppf::pcg_plan_destroy(pl);
@@ -1648,28 +1654,28 @@ void pcg_plan_destroy(PcgPlanImpl *pl) {
         (void)hipStreamDestroy(pl->cap);
     }
     if (pl->h_scal) {
-        (void)hipHostFree(pl->h_scal);
+        (void)host_free(pl->h_scal);
     }
     if (pl->mem) {
-        (void)hipFree(pl->mem);
+        (void)dev_free(pl->mem);
     }
     if (pl->pat) {
-        (void)hipFree(pl->pat);
+        (void)dev_free(pl->pat);
     }
     if (pl->tsort) {
-        (void)hipFree(pl->tsort);
+        (void)dev_free(pl->tsort);
     }
     if (pl->ell) {
-        (void)hipFree(pl->ell);
+        (void)dev_free(pl->ell);
     }
     if (pl->tmem) {
-        (void)hipFree(pl->tmem);
+        (void)dev_free(pl->tmem);
     }
     if (pl->omem) {
-        (void)hipFree(pl->omem);
+        (void)dev_free(pl->omem);
     }
     if (pl->emem) {
-        (void)hipFree(pl->emem);
+        (void)dev_free(pl->emem);
     }
     delete pl;
 }
@@ -1985,7 +1991,7 @@ int pcg_plan_set_order(PcgPlanImpl *pl, const uint32_t *order, hipStream_t st) {
     if (!pl->omem) {
         const size_t bytes = carve_bytes(n, 4) + 2 * carve_bytes(n + 1, 4) +
                              carve_bytes(9 * (size_t)n, 4) + 2 * carve_bytes(3 * (size_t)n, 4);
-        PPF_HIP(hipMalloc(&pl->omem, bytes));
+        PPF_HIP(dev_alloc(&pl->omem, bytes));
         char *cur = static_cast<char *>(pl->omem);
         pl->inv = carve<uint32_t>(cur, n);
         pl->prp = carve<uint32_t>(cur, n + 1);
@@ -1994,8 +2000,20 @@ int pcg_plan_set_order(PcgPlanImpl *pl, const uint32_t *order, hipStream_t st) {
         pl->pb = carve<float>(cur, 3 * (size_t)n);
         pl->px = carve<float>(cur, 3 * (size_t)n);
     }
-    k_inverse_order<<<cdiv(n, kNT), kNT, 0, st>>>(n, order, pl->inv);
+    PPF_HIP(hipMemsetAsync(pl->inv, 0xff, (size_t)n * sizeof(uint32_t), st));
+    PPF_HIP(hipMemsetAsync(pl->flags, 0, sizeof(uint32_t), st));
+    k_inverse_order<<<cdiv(n, kNT), kNT, 0, st>>>(n, order, pl->inv, pl->flags);
     PPF_HIP(hipGetLastError());
+    uint32_t bad = 0;
+    PPF_HIP(hipMemcpyAsync(&bad, pl->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PPF_HIP(hipStreamSynchronize(st));
+    if (bad) {
+        pl->order = nullptr; // the plan keeps solving in the original order
+        set_error("pcg_plan_set_order: order is not a permutation of 0..n-1 (%u entries out "
+                  "of range or repeated)",
+                  bad);
+        return PPF_ERR_ARG;
+    }
     return PPF_OK;
 }
 

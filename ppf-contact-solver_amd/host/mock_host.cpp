@@ -5,10 +5,11 @@
 // crates/ppf-cts-solver/src/builder.rs lays them out, then drives the solver
 // backend in the order Backend::run uses (backend.rs:626-1025):
 //   set_log_path, initialize, then per step update_constraint (moving pins),
-//   refresh_collision_active, advance, fetch; on a failed step
-//   fetch_intersection_records.
-// Output pack: curr / prev after the last step, and per step time, prev_dt
-// and the six StepResult flags.
+//   refresh_collision_active, advance, fetch, and -- as a checkpointing host
+//   does -- fetch_dyn_counts / fetch_dyn and gather_current_positions; on a
+//   failed step fetch_intersection_records.
+// Output pack: curr / prev after the last step, and per step time, prev_dt,
+// the six StepResult flags and the library's cumulative device allocations.
 //
 //   mock_host <scene.pack> <out.pack> [log_dir]
 #include <cstdint>
@@ -20,6 +21,7 @@
 #include <vector>
 
 #include "ppf_backend.h"
+#include "ppf_mi355x.h" // ppf_alloc_counts: the library's allocation counters
 
 namespace {
 
@@ -328,6 +330,18 @@ int main(int argc, char **argv) {
         init_collision_windows(dmap.data(), (uint32_t)dmap.size(), win.data(), counts.data(),
                                (uint32_t)counts.size());
     }
+    // A resumed run hands the checkpoint's contact pattern back (update_dyn);
+    // what fetch_dyn reports before the first step is written out.
+    std::vector<uint32_t> dyn0_off, dyn0_idx;
+    if (pk.has("carry_dyn_offset")) {
+        const std::vector<uint32_t> off = pk.u("carry_dyn_offset"), idx = pk.u("carry_dyn_index");
+        update_dyn(idx.data(), off.data());
+        uint32_t nval = 0, noff = 0;
+        fetch_dyn_counts(&nval, &noff);
+        dyn0_off.resize(noff);
+        dyn0_idx.resize(nval);
+        fetch_dyn(dyn0_idx.data(), nullptr, dyn0_off.data());
+    }
     const uint32_t steps = pk.u("steps")[0];
     std::vector<float> sched, dsched;
     if (pk.has("pin_schedule")) {
@@ -336,7 +350,12 @@ int main(int argc, char **argv) {
     }
     std::vector<double> times;
     std::vector<float> dts;
-    std::vector<uint32_t> flags;
+    std::vector<uint32_t> flags, allocs, frees, dyn_off, dyn_idx;
+    std::vector<uint32_t> tracked;
+    for (uint32_t v = 0; v < nv && tracked.size() < 16; v += 1 + nv / 16) {
+        tracked.push_back(v);
+    }
+    std::vector<float> gathered(3 * tracked.size());
     int rc = 0;
     for (uint32_t k = 0; k < steps; ++k) {
         if (!sched.empty()) {
@@ -360,6 +379,18 @@ int main(int argc, char **argv) {
             break;
         }
         fetch();
+        // a checkpointing host: the contact pattern (state_<N>.bin.gz,
+        // backend.rs:520-575) and a few tracked vertices every step
+        uint32_t nval = 0, noff = 0;
+        fetch_dyn_counts(&nval, &noff);
+        dyn_off.resize(noff);
+        dyn_idx.resize(nval);
+        fetch_dyn(dyn_idx.data(), nullptr, dyn_off.data());
+        gather_current_positions(tracked.data(), (uint32_t)tracked.size(), gathered.data());
+        uint64_t na = 0, nf = 0;
+        ppf_alloc_counts(&na, &nf);
+        allocs.push_back((uint32_t)na);
+        frees.push_back((uint32_t)nf);
         printf("step %u: time %.6f dt %.6g\n", k, param.time, param.prev_dt);
         fflush(stdout);
     }
@@ -369,7 +400,7 @@ int main(int argc, char **argv) {
     if (!fp) {
         return 2;
     }
-    const uint32_t count = 6;
+    const uint32_t count = 10;
     fwrite("PPFPACK1", 1, 8, fp);
     fwrite(&count, 4, 1, fp);
     write_array(fp, "curr", 'f', curr.data(), curr.size());
@@ -379,6 +410,11 @@ int main(int argc, char **argv) {
     write_array(fp, "flags", 'u', flags.data(), flags.size());
     const uint32_t dyn[2] = {n_value, n_offset};
     write_array(fp, "dyn_counts", 'u', dyn, 2);
+    // device allocations / frees of the library after each step (cumulative)
+    write_array(fp, "allocs", 'u', allocs.data(), allocs.size());
+    write_array(fp, "frees", 'u', frees.data(), frees.size());
+    write_array(fp, "dyn0_offset", 'u', dyn0_off.data(), dyn0_off.size());
+    write_array(fp, "dyn0_index", 'u', dyn0_idx.data(), dyn0_idx.size());
     fclose(fp);
     return rc;
 }

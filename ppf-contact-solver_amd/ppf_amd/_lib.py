@@ -25,6 +25,9 @@ class PcgResult(ctypes.Structure):
 
 _PROTOS = {
     "ppf_last_error": (ctypes.c_char_p, []),
+    "ppf_alloc_counts": (None, [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "ppf_device_alloc": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),
+    "ppf_device_free": (None, [vp]),
     "ppf_device_info": (ctypes.c_int, [ctypes.c_char_p, u32, ctypes.POINTER(u32)]),
     "ppf_pcg_plan_create": (ctypes.c_int, [u32, u32, vp, vp, vp, ctypes.POINTER(vp)]),
     "ppf_pcg_plan_destroy": (None, [vp]),
@@ -124,3 +127,11 @@ def check(rc, what):
     if rc != 0:
         msg = load().ppf_last_error().decode(errors="replace")
         raise PPFError(f"{what} failed (code {rc}): {msg}")
+
+
+def alloc_counts():
+    """(allocations, frees) of device and pinned-host memory the library has
+    made since it was loaded."""
+    a, f = ctypes.c_uint64(), ctypes.c_uint64()
+    load().ppf_alloc_counts(ctypes.byref(a), ctypes.byref(f))
+    return a.value, f.value

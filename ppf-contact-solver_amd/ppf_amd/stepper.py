@@ -44,7 +44,7 @@ class StepResultC(ctypes.Structure):
                 ("pcg_iter_max", u32), ("pairs_max", u32), ("intersections", u32),
                 ("toi_advanced", f32), ("dt", f32), ("ms_elastic", f32), ("ms_contact", f32),
                 ("ms_solve", f32), ("ms_line_search", f32), ("ms_total", f32),
-                ("lpc_fallbacks", u32)]
+                ("lpc_fallbacks", u32), ("device_alloc", u32), ("device_free", u32)]
 
 
 def params_struct(p):

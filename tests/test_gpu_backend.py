@@ -86,7 +86,12 @@ def test_mock_host_drape_matches_stepper_and_oracle(dev, tmp_path):
         assert r["status"] == 0
         p["prev_dt"] = r["dt"]
     xo = orc.sc["mesh"]["x"]
-    print("host vs oracle after", steps, "steps:", np.abs(xh - xo).max() / np.abs(xo).max())
+    err = np.abs(xh - xo).max() / np.abs(xo).max()
+    print("host vs oracle after", steps, "free-running steps:", err)
+    # Free-running (the oracle is not reset to the host's state each step),
+    # so the per-step solve differences (<= 1e-4 of the coordinate scale at
+    # cg_tol 1e-3, test_gpu_step.py) accumulate over the 12 steps.
+    assert err < 1e-3
     assert os.path.exists(tmp_path / "advance.newton_steps.out")
     assert out["dyn_counts"][1] == sc["mesh"]["nv"] + 1
 

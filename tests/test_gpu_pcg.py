@@ -284,6 +284,32 @@ def test_plan_row_order(dev, kind):
     plan.close()
 
 
+def test_plan_rejects_order_that_is_not_a_permutation(dev):
+    """An order with an entry >= n or a repeated entry is refused
+    (PPF_ERR_ARG) before the plan uses it; the plan keeps its original order
+    and its solve is bitwise the fresh plan's."""
+    from ppf_amd._lib import PPFError
+    s = scenes.grid_spd_system(12, seed_matrix=3, seed_rhs=4)
+    n = len(s["row_offset"]) - 1
+    t = to_dev(s, dev)
+    plan = BlockJacobiPCG(t["row_offset"], t["col_index"])
+    good = np.random.default_rng(2).permutation(n).astype(np.int32)
+    dup = good.copy()
+    dup[7] = dup[3]
+    big = good.copy()
+    big[5] = n + 11
+    for bad in (dup, big):
+        with pytest.raises(PPFError):
+            plan.set_order(torch.from_numpy(bad).to(dev))
+    plan.repattern(t["row_offset"], t["col_index"])
+    x = torch.zeros_like(t["b"])
+    r = plan.solve(t["value"], t["cdiag"], t["b"], x)
+    fresh = solve_gpu(s, dev, hist=False)
+    assert r.status == 0 and r.iter == fresh["iter"]
+    assert np.array_equal(x.cpu().numpy().view(np.uint32), fresh["x"].view(np.uint32))
+    plan.close()
+
+
 def test_plan_rejects_malformed_pattern(dev):
     from ppf_amd._lib import PPFError
     ro = torch.tensor([0, 2, 3], dtype=torch.int32, device=dev)

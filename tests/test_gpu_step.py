@@ -173,12 +173,12 @@ def test_last_newton_system_is_a_solvable_dirichlet_system(dev):
     assert np.all(ci >= rows)
     val = s["value"].cpu().numpy().reshape(-1, 9)
     cd = s["cdiag"].cpu().numpy().reshape(-1, 9)
-    # pinned rows and columns are eliminated: identity diagonal, no coupling
+    # pinned rows and columns are eliminated: no stored block touches a pinned
+    # vertex (the solve reads none), and a pinned row's diagonal is the identity
     for v in pins:
         touch = (rows == v) | (ci == v)
-        assert np.all(val[touch & (ci != rows)] == 0.0)
-        diag = val[(rows == v) & (ci == v)].sum(0) + cd[v]
-        assert np.allclose(diag, np.eye(3).reshape(9))
+        assert not np.any(touch)
+        assert np.array_equal(cd[v], np.eye(3, dtype=np.float32).reshape(9))
     plan = BlockJacobiPCG(s["row_offset"], s["col_index"])
     x = torch.zeros_like(s["b"])
     res_ = plan.solve(s["value"], s["cdiag"], s["b"], x)
@@ -247,3 +247,65 @@ def test_stacked_sheets_on_spinning_cylinder(dev):
         p["prev_dt"] = rg["dt"]
         clock += rg["dt"]
     gpu.close()
+
+
+def test_dirichlet_elimination_bitwise_vs_oracle(dev):
+    """One Newton iteration of a drape with moving pins (pinned vertices 0 and
+    70 have free neighbours with higher indices -- their couplings are lifted
+    by transpose --, 11 and 143 free neighbours with lower ones). The right-hand
+    side after the lifts (main.cu:954-1049) equals the oracle's bit for bit:
+    both sum each free row's own lifts in entry order, then its mirror lifts
+    in source-row order. The kept blocks equal the oracle's non-eliminated
+    blocks exactly, and a second run gives the same bits."""
+    res = 12
+    pins = [0, res - 1, 70, res * res - 1]
+    sc = scenes.drape_scene(res=res, height=0.004, pins=pins)
+    sc["fix_pos"][pins] += np.array([1e-3, -2e-3, 1.5e-3], np.float32)
+    p = scenes.step_params(max_newton_steps=2, min_newton_steps=10)
+    sco = copy.deepcopy(sc)
+    orc = oracle.OracleStepper(sco)
+    ro = orc.advance(p)
+    o = orc.last_system()
+    runs = []
+    for _ in range(2):
+        t = scene_to_device(copy.deepcopy(sc), dev)
+        gpu = Stepper(t)
+        rg = gpu.advance(p)
+        assert rg["status"] == ro["status"] == 5 and rg["newton_steps"] == 1
+        runs.append({k: v.cpu().numpy() for k, v in gpu.system().items()})
+        gpu.close()
+    g = runs[0]
+    assert np.array_equal(g["b"].view(np.uint32), runs[1]["b"].view(np.uint32))
+    assert np.array_equal(g["b"].view(np.uint32), o["b"].view(np.uint32))
+    n = res * res
+    assert np.array_equal(g["cdiag"].reshape(-1, 9).view(np.uint32), o["cdiag"].view(np.uint32))
+    grow = np.repeat(np.arange(n), np.diff(g["row_offset"]))
+    gb = {(int(r), int(c)): v for r, c, v in zip(grow, g["col_index"],
+                                                  g["value"].reshape(-1, 9))}
+    orow = np.repeat(np.arange(n), np.diff(o["row_offset"].astype(np.int64)))
+    pinned = np.zeros(n, bool)
+    pinned[pins] = True
+    kept = 0
+    for r, c, v in zip(orow, o["col_index"], o["value"]):
+        if pinned[r] or pinned[c]:
+            assert not np.any(v) and (int(r), int(c)) not in gb
+        else:
+            assert np.array_equal(gb[(int(r), int(c))].view(np.uint32), v.view(np.uint32))
+            kept += 1
+    assert kept == len(gb)
+    # With the pins held in place the lifts vanish (p = 0), and nothing else
+    # in a free row changes: the difference is the lifts alone, non-zero on
+    # the pins' neighbours on both sides of them in index order, zero elsewhere.
+    sc0 = scenes.drape_scene(res=res, height=0.004, pins=pins)
+    o0 = oracle.OracleStepper(sc0)
+    o0.advance(p)
+    b0 = o0.last_system()["b"].reshape(-1, 3)
+    d = np.abs(g["b"].reshape(-1, 3) - b0).max(1)
+    d[pins] = 0
+    nbr = np.zeros(n, bool)
+    for r, c in zip(orow, o["col_index"]):
+        if pinned[r] != pinned[c]:
+            nbr[c if pinned[r] else r] = True
+    assert np.all(d[~nbr] == 0)
+    assert np.all(d[[1, 12, 71, 82]] > 0.1)  # mirror lifts (pins 0 and 70)
+    assert np.all(d[[10, 69, 130, 142]] > 0.1)  # own lifts (pins 11, 70, 143)
