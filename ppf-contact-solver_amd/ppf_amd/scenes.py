@@ -487,6 +487,14 @@ def _dihedral64(p0, p1, p2, p3):
     return np.arctan2(s, c)
 
 
+def convert_prop(young, poisson):
+    """Lame parameters of a Young's modulus and Poisson ratio, in f32
+    (builder.rs:1739-1743)."""
+    e, nu = np.float32(young), np.float32(poisson)
+    one, two = np.float32(1.0), np.float32(2.0)
+    return float(e / (two * (one + nu))), float(e * nu / ((one + nu) * (one - two * nu)))
+
+
 def shell_rest(rest, faces, density=0.2, young=1e4, poisson=0.25, bend=10.0):
     """Rest-state shell data the elastic assembly reads (builder.rs:157-272,
     339-644, mesh.rs hinges): lumped vertex mass / area, face mass,
@@ -508,16 +516,20 @@ def shell_rest(rest, faces, density=0.2, young=1e4, poisson=0.25, bend=10.0):
                   np.stack([np.zeros(nf), np.einsum("ij,ij->i", e2, w)], 1)], 1)
     inv = np.linalg.inv(D)                              # (nf, 2, 2) row-major
     inv_rest = inv.transpose(0, 2, 1).reshape(nf, 4)    # column-major
-    fmass = density * area
-    varea = np.zeros(nv)
-    np.add.at(varea, faces.reshape(-1).astype(np.int64), np.repeat(area / 3.0, 3))
-    vmass = density * varea
+    # FaceProp area and mass = density * area (scene.rs:1791-1799); a vertex
+    # gets a third of each incident face's area and mass (builder.rs:571-605)
+    area = area.astype(np.float32)
+    fmass = np.float32(density) * area
+    fi = faces.reshape(-1).astype(np.int64)
+    varea = np.zeros(nv, np.float32)
+    vmass = np.zeros(nv, np.float32)
+    np.add.at(varea, fi, np.repeat(area / np.float32(3.0), 3))
+    np.add.at(vmass, fi, np.repeat(fmass / np.float32(3.0), 3))
     hinge, hf = shell_hinges(faces, with_faces=True)
     H = hinge.astype(np.int64)
     hl = np.linalg.norm(rest[H[:, 1]] - rest[H[:, 0]], axis=1)
     harea = area[hf[:, 0]] + area[hf[:, 1]] if len(H) else np.zeros(0)
-    mu = young / (2.0 * (1.0 + poisson))
-    lam = young * poisson / ((1.0 + poisson) * (1.0 - 2.0 * poisson))
+    mu, lam = convert_prop(young, poisson)
     hrest = (_dihedral64(rest[H[:, 2]], rest[H[:, 1]], rest[H[:, 0]], rest[H[:, 3]])
              if len(H) else np.zeros(0))
     v2f_off, v2f = vertex_face_csr(nv, faces)
@@ -638,9 +650,21 @@ def drape_scene(res=64, size=1.0, height=0.05, radius=0.3, subdiv=3, ghat=1e-3, 
                 fric=fric, fix_pos=fix_pos)
 
 
-def step_params(dt=1.0 / 60.0, **kw):
+MAX_SUBSTEP_FRAME_FRACTION = 0.9999  # builder.rs:268
+
+
+def clamp_substep_dt(dt, fps):
+    """A requested substep dt kept strictly under one frame, in f32
+    (builder.rs:270-275): min(dt, 0.9999 / fps)."""
+    return float(min(np.float32(dt), np.float32(MAX_SUBSTEP_FRAME_FRACTION) / np.float32(fps)))
+
+
+def step_params(dt=1.0 / 60.0, fps=60.0, **kw):
     """ParamSet defaults the step reads (crates/ppf-cts-core/src/datamodel/
-    params.rs:215-280), with gravity (0, -9.8, 0)."""
+    params.rs:215-280), with gravity (0, -9.8, 0). dt is clamped under one
+    frame of `fps` as the scene builder does (builder.rs:348): the configs'
+    dt = 1/60 at 60 fps steps 0.016665 s."""
+    dt = clamp_substep_dt(dt, fps)
     p = dict(dt=dt, prev_dt=dt, gravity=(0.0, -9.8, 0.0), cg_tol=1e-3, cg_max_iter=10000,
              target_toi=0.25, line_search_max_t=1.25, ccd_eps=1e-7, max_dx=1.0,
              friction_eps=1e-5, friction_mode=0, min_newton_steps=0, max_newton_steps=2048,
