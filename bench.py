@@ -39,13 +39,44 @@ METRIC = ("ms/Newton step + PCG iters to 1e-3 rel residual, learned precond "
 HBM_PEAK_GBS = 8000.0
 
 
-def traffic_from_profiles(name):
-    """HBM bytes per SpMV launch from a committed rocprofv3 PMC summary."""
+def traffic_from_profiles(name, rows, stored_blocks):
+    """HBM bytes per SpMV launch from a committed rocprofv3 PMC summary of the
+    same system: the summary's `traffic:` line names the rows and stored
+    blocks of the system it measured; any other system gets None."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
-    return float(m.group(1)) if m else None
+    for m in re.finditer(r"traffic: rows=(\d+) stored_blocks=(\d+) bytes_per_launch=([0-9.eE+]+)",
+                         open(path).read()):
+        if int(m.group(1)) == rows and int(m.group(2)) == stored_blocks:
+            return float(m.group(3))
+    return None
+
+
+def host_cpu():
+    """nproc, CPU model, sockets, cores per socket, threads per core of this host."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        text = open("/proc/cpuinfo").read()
+        model = re.search(r"model name\s*:\s*(.*)", text)
+        info["model"] = model.group(1).strip() if model else None
+        sockets = set(re.findall(r"physical id\s*:\s*(\d+)", text))
+        cores = re.search(r"cpu cores\s*:\s*(\d+)", text)
+        sib = re.search(r"siblings\s*:\s*(\d+)", text)
+        info["sockets"] = len(sockets) or None
+        if cores:
+            info["cores_per_socket"] = int(cores.group(1))
+            if sib:
+                info["threads_per_core"] = int(sib.group(1)) // int(cores.group(1))
+    except OSError:
+        pass
+    return info
+
+
+def progress(msg):
+    """One line per unit of work on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def oracle_module():
@@ -116,11 +147,17 @@ def run_drape(args, dev):
             raise RuntimeError(f"step failed: {r}")
         p["prev_dt"] = r["dt"]
         clock[0] += r["dt"]
+        progress(f"time step: newton {r['newton_steps']} pcg {r['pcg_iter_total']} pairs "
+                 f"{r['pairs_max']} ms {r['ms_total']:.1f}")
         return r
 
     for _ in range(args.warmup):
         one_step()
     torch.cuda.synchronize()
+    # the state the timed window starts from (the CPU baseline steps from it)
+    start = {"x": t["mesh"]["x"].cpu().numpy().reshape(-1, 3).copy(),
+             "prev": t["prev"].cpu().numpy().reshape(-1, 3).copy(),
+             "clock": clock[0], "prev_dt": p["prev_dt"]}
     newton = pcg = pairs = 0
     phase = {"elastic": 0.0, "contact": 0.0, "solve": 0.0, "line_search": 0.0}
     t0 = time.perf_counter()
@@ -145,7 +182,7 @@ def run_drape(args, dev):
     plan.repattern(sysd["row_offset"], sysd["col_index"])
     roof = spmv_roofline(plan, sysd["value"], sysd["cdiag"], sysd["b"], n, nnz,
                          "pcg spmv on the last Newton system of the timed steps",
-                         traffic_from_profiles("pcg_spmv_drape_stack.md"))
+                         traffic_from_profiles("pcg_spmv_drape_stack.md", n, nnz))
     plan.close()
     gpu.close()
     workload = {"workload": f"stacked-layer drape: {args.layers} sheets of {args.res}x{args.res} "
@@ -162,7 +199,7 @@ def run_drape(args, dev):
              "roofline": roof, "cpu_baseline": None}
     out = head(args, per_newton, per_newton, workload, extra)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_drape(args)
+        out["cpu_baseline"] = cpu_drape(args, start)
     return out
 
 
@@ -183,7 +220,7 @@ def run_learned(args, dev, last_system):
                                               steps=12)
     t_collect = time.perf_counter() - t0
     t0 = time.perf_counter()
-    lpc, hist = lpc_train.train(systems, steps=args.lpc_steps, lr=2e-3)
+    lpc, hist = lpc_train.train(systems, steps=args.lpc_steps, lr=2e-3, log=progress)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     same = lpc_train.compare(lpc, [last_system])[0]
@@ -208,6 +245,8 @@ def run_learned(args, dev, last_system):
             raise RuntimeError(f"learned step failed: {r}")
         p["prev_dt"] = r["dt"]
         clock[0] += r["dt"]
+        progress(f"learned time step: newton {r['newton_steps']} pcg {r['pcg_iter_total']} "
+                 f"ms {r['ms_total']:.1f}")
         return r
 
     for _ in range(args.warmup):
@@ -242,22 +281,36 @@ def run_learned(args, dev, last_system):
                          "collect_s": round(t_collect, 2), "train_s": round(t_train, 2)}}
 
 
-def cpu_drape(args):
-    """The oracle's first time step of the same scene (a bounded sample)."""
+def cpu_drape(args, start):
+    """The oracle's first time step of the timed window (time step W+1), from
+    the GPU's state at the start of the window: a bounded sample of the same
+    workload, reported per Newton step."""
+    import numpy as np
+
     from ppf_amd import scenes
     oracle, threads = oracle_module()
     sc = scenes.stacked_drape_scene(res=args.res, layers=args.layers)
+    ns = args.layers * args.res * args.res
+    sc["mesh"]["x"][:] = start["x"]
+    sc["prev"][:] = start["prev"]
+    sc["fix_delta"] = np.zeros_like(sc["fix_pos"])
+    now = scenes.spin_fix_positions(sc, start["clock"])[ns:].copy()
+    aim = scenes.spin_fix_positions(sc, start["clock"] + scenes.step_params()["dt"])[ns:].copy()
+    sc["fix_pos"][ns:] = aim
+    sc["fix_delta"][ns:] = aim - now
     st = oracle.OracleStepper(sc)
     p = scenes.step_params()
-    scenes.spin_fix_positions(sc, p["dt"])
+    p["prev_dt"] = start["prev_dt"]
     t0 = time.perf_counter()
     r = st.advance(p)
     dt = time.perf_counter() - t0
     return {"value": round(dt * 1e3 / max(1, r["newton_steps"]), 1), "unit": "ms/Newton step",
-            "cores": threads, "kind": "port",
-            "sample": f"the first time step of the same scene ({r['newton_steps']} Newton "
-                      f"steps, {r['pcg_iter_total']} PCG iterations, {dt:.1f} s); "
-                      "reference-algorithm CPU restatement, not the reference binary"}
+            "cores": threads, "kind": "port", "host": host_cpu(),
+            "sample": f"time step {args.warmup + 1} of the same scene (the first of the timed "
+                      f"window, from the GPU's state): {r['newton_steps']} Newton steps, "
+                      f"{r['pcg_iter_total']} PCG iterations, {r['pairs_max']} contact pairs, "
+                      f"{dt:.1f} s; reference-algorithm CPU restatement (OpenMP, {threads} "
+                      "threads), not the reference binary"}
 
 
 def run_grid(args, dev):
@@ -293,7 +346,7 @@ def run_grid(args, dev):
         raise RuntimeError(f"PCG did not converge: status {r.status}")
     roof = spmv_roofline(plan, t["value"], t["cdiag"], t["b"], n, nnz,
                          "pcg spmv (k_spmv_ell<true,3> on this grid)",
-                         traffic_from_profiles("pcg_spmv_grid1024.md"))
+                         traffic_from_profiles("pcg_spmv_grid1024.md", n, nnz))
     plan.close()
     workload = {"workload": f"kernel-isolation PCG: {res}x{res} triangulated grid, {n} rows "
                             f"of 3x3 fp32 blocks ({nnz} stored upper blocks), block-Jacobi "
@@ -307,7 +360,7 @@ def run_grid(args, dev):
         ro = oracle.pcg(s, tol=1e-3)
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(dt * 1e3, 1), "unit": "ms/solve",
-                               "cores": threads, "kind": "port",
+                               "cores": threads, "kind": "port", "host": host_cpu(),
                                "sample": f"one full solve of the same system ({ro['iter']} "
                                          "iterations); reference-algorithm CPU restatement, "
                                          "not the reference binary"}
@@ -315,91 +368,100 @@ def run_grid(args, dev):
 
 
 def run_batch(args, dev):
-    """Many uneven scenes in one solve: 64 draped sheets (side log-uniform in
-    32..384 vertices, sphere radius uniform in 0.1..0.4 m, seed 5), each
-    stepped 3 time steps by the stepper (contact starts on step 2), their last
-    Newton systems stacked block-diagonally. One batched plan (tile sizes from
-    the autotuner) against the same 64 systems solved one plan at a time."""
+    """Many uneven scenes stepped together: 64 draped sheets (side
+    log-uniform in 32..384 vertices, sphere radius uniform in 0.1..0.4 m,
+    seed 5), all packed into one mesh with per-vertex scene ids and advanced
+    by one many-scene stepper (every stage one launch over all scenes, each
+    scene with its own Newton loop, solver scalars and line-search minimum),
+    against the same 64 scenes advanced one stepper at a time. W untimed time
+    steps, then K timed ones; a step is one time step of all 64 scenes."""
     import numpy as np
     import torch
 
     from ppf_amd import scenes
-    from ppf_amd.solver import BatchPCG, BlockJacobiPCG, stack_systems
     from ppf_amd.stepper import Stepper, scene_to_device
 
     rng = np.random.default_rng(5)
     count = 64
     sides = np.exp(rng.uniform(np.log(32), np.log(384), count)).astype(int)
     radii = rng.uniform(0.1, 0.4, count)
-    systems = []
-    for side, rad in zip(sides, radii):
-        sc = scenes.drape_scene(res=int(side), radius=float(rad), height=0.002)
-        st = Stepper(scene_to_device(sc, dev))
-        p = scenes.step_params()
-        for _ in range(3):
-            r = st.advance(p)
+    scs = [scenes.drape_scene(res=int(side), radius=float(rad), height=0.002)
+           for side, rad in zip(sides, radii)]
+    p = scenes.step_params()
+
+    # one stepper per scene
+    singles = []
+    for sc in scs:
+        t = scene_to_device(sc, dev)
+        singles.append((t, Stepper(t), [p["dt"]]))
+
+    def step_singles():
+        out = []
+        for t, st, pd in singles:
+            q = dict(p)
+            q["prev_dt"] = pd[0]
+            r = st.advance(q)
             if r["status"] != 0:
-                raise RuntimeError(f"batch scene {side}: step failed: {r}")
-            p["prev_dt"] = r["dt"]
-        systems.append(st.system())
-        st.close()
+                raise RuntimeError(f"single scene step failed: {r}")
+            pd[0] = r["dt"]
+            out.append(r)
+        return out
 
-    def timed(fn):
-        fn()  # warm (graph capture, first touch)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            out = fn()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / args.steps, out
+    packed, rows = scenes.pack_scenes(scs)
+    tb = scene_to_device(packed, dev)
+    batch = Stepper(tb)
+    prev_dt = [p["dt"]] * count
 
-    plans = [BlockJacobiPCG(s["row_offset"], s["col_index"]) for s in systems]
-    xs = [torch.zeros_like(s["b"]) for s in systems]
-
-    def one_by_one():
-        res = []
-        for s, pl, x in zip(systems, plans, xs):
-            x.zero_()
-            res.append(pl.solve(s["value"], s["cdiag"], s["b"], x))
+    def step_batch():
+        res = batch.advance_scenes(p, prev_dt)
+        for c, r in enumerate(res):
+            if r["status"] != 0:
+                raise RuntimeError(f"batched scene {c} step failed: {r}")
+            prev_dt[c] = r["dt"]
         return res
 
-    seq_ms, seq = timed(one_by_one)
-    for pl in plans:
-        pl.close()
-    stacked, rows = stack_systems(systems)
-    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in stacked.items()}
-    plan = BatchPCG(t["row_offset"], t["col_index"], rows)
-    r1, r2, _ = plan.autotune(t["value"], t["cdiag"], t["b"])
-    x = torch.zeros_like(t["b"])
+    def timed(fn, label):
+        for k in range(args.warmup):
+            fn()
+            progress(f"{label} warm-up step {k + 1}")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        newton = pcg = 0
+        for k in range(args.steps):
+            res = fn()
+            newton += sum(r["newton_steps"] for r in res)
+            pcg += sum(r["pcg_iter_total"] for r in res)
+            progress(f"{label} step {k + 1}: newton {newton} pcg {pcg}")
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, newton, pcg
 
-    def batched():
-        x.zero_()
-        return plan.solve(t["value"], t["cdiag"], t["b"], x)
-
-    bat_ms, bat = timed(batched)
-    plan.close()
-    if any(r.status != 0 for r in bat) or any(r.status != 0 for r in seq):
-        raise RuntimeError("a batched or single solve did not converge")
+    seq_ms, seq_newton, seq_pcg = timed(step_singles, "one stepper per scene")
+    bat_ms, bat_newton, bat_pcg = timed(step_batch, "batched")
+    xb = tb["mesh"]["x"].cpu().numpy().reshape(-1, 3)
+    gap = 0.0
+    for c, (t, st, _) in enumerate(singles):
+        xs = t["mesh"]["x"].cpu().numpy().reshape(-1, 3)
+        xc = xb[rows[c]:rows[c + 1]]
+        gap = max(gap, float(np.abs(xs - xc).max() / np.abs(xs).max()))
+        st.close()
+    batch.close()
     n = int(rows[-1])
     workload = {"workload": f"{count} draped sheets, side log-uniform 32..384 vertices "
-                            "(seed 5), sphere radius 0.1..0.4 m, gap 1e-3, friction 0.5, "
-                            "dt 1/60; the last Newton system of time step 3 of each, "
-                            "block-Jacobi PCG to 1e-3",
-                "scenes": count, "rows": n, "stored_blocks": int(len(stacked["col_index"])),
-                "rows_min_max": [int(np.diff(rows).min()), int(np.diff(rows).max())]}
-    out = {"metric": "ms per PCG solve of 64 uneven scenes, one batched plan",
-           "value": round(bat_ms, 4), "unit": "ms/solve", "n_gpus": 1, "steps": args.steps,
-           "warmup": 1, "ms_per_step": round(bat_ms, 4), "higher_is_better": False,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": workload,
-           "one_plan_per_scene_ms": round(seq_ms, 4),
-           "speedup": round(seq_ms / bat_ms, 3),
-           "tiles": {"spmv_rows": r1, "vector_rows": r2},
-           "iters": {"batched_max": max(r.iter for r in bat),
-                     "batched_sum": sum(r.iter for r in bat),
-                     "single_sum": sum(r.iter for r in seq)},
-           "roofline": None, "cpu_baseline": None}
-    return out
+                            "(seed 5), sphere radius 0.1..0.4 m, 2 mm above it, gap 1e-3, "
+                            "friction 0.5, dt 1/60; time steps "
+                            f"{args.warmup + 1}..{args.warmup + args.steps} of all scenes",
+                "scenes": count, "vertices": n,
+                "vertices_min_max": [int(np.diff(rows).min()), int(np.diff(rows).max())]}
+    return {"metric": "ms per time step of 64 uneven scenes, one many-scene stepper",
+            "value": round(bat_ms, 4), "unit": "ms/time step", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(bat_ms, 4),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic", "config": workload,
+            "one_stepper_per_scene_ms": round(seq_ms, 4),
+            "speedup": round(seq_ms / bat_ms, 3),
+            "newton_steps": {"batched": bat_newton, "one_per_scene": seq_newton},
+            "pcg_iters": {"batched": bat_pcg, "one_per_scene": seq_pcg},
+            "max_position_gap": gap, "roofline": None, "cpu_baseline": None}
 
 
 def run_broad_narrow(args, dev):
@@ -459,7 +521,7 @@ def run_broad_narrow(args, dev):
         o = oracle.proximity(m)
         dt = time.perf_counter() - t0
         out_line["cpu_baseline"] = {"value": round(dt * 1e3, 1), "unit": "ms/step",
-                                    "cores": threads, "kind": "port",
+                                    "cores": threads, "kind": "port", "host": host_cpu(),
                                     "sample": f"the same 2M triangles, trees and enumeration "
                                               f"({len(o)} pairs); reference-algorithm CPU "
                                               "restatement, not the reference binary"}

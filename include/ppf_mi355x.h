@@ -15,6 +15,7 @@
 #ifndef PPF_MI355X_H
 #define PPF_MI355X_H
 
+#include <stdbool.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -239,6 +240,13 @@ typedef struct ppf_contact_mesh {
     const uint32_t *v2e_off, *v2e, *v2f_off, *v2f, *e2f_off, *e2f;
     const float *v_ghat, *v_off, *e_ghat, *e_off, *f_ghat, *f_off;
     const uint32_t *vflag, *vbody, *eflag, *fflag;
+    /* Many independent scenes in one mesh (no counterpart in the reference,
+     * which steps one scene per process): the scene id of every vertex, the
+     * vertices, edges and faces of a scene contiguous and scene ids rising
+     * with them; elements of different scenes never meet. nscene scenes;
+     * vscene null (or nscene <= 1) = one scene. */
+    const uint32_t *vscene;
+    uint32_t nscene;
 } ppf_contact_mesh;
 
 /* One accepted stage-0 pair: kind (low byte) 0 point-face, 1 point-edge,
@@ -376,7 +384,29 @@ typedef struct ppf_static_mesh {
     /* per-element friction (the collision mesh's Vertex/Edge/FaceParam
      * friction); read by the contact assembly only */
     const float *v_fric, *e_fric, *f_fric;
+    /* scene id of every collision-mesh vertex (as ppf_contact_mesh.vscene;
+     * null = one scene) */
+    const uint32_t *vscene;
 } ppf_static_mesh;
+
+/* Analytic colliders (Sphere and Floor of data.constraint, data.hpp:449-467;
+ * the same layout as data.rs:689-709). A sphere pushes vertices out of it
+ * (reverse: keeps them inside; bowl: a sphere whose upper half is a
+ * cylinder), a floor keeps them above the plane through `ground` normal to
+ * `up`; thickness > 0 lets vertices deeper than it pass through; kinematic
+ * colliders move and are held by the barrier alone (no line-search clamp). */
+typedef struct ppf_sphere {
+    float center[3];
+    float ghat, friction, radius, thickness;
+    bool bowl, reverse, kinematic;
+} ppf_sphere;
+
+typedef struct ppf_floor {
+    float ground[3];
+    float ghat, friction, thickness;
+    float up[3];
+    bool kinematic;
+} ppf_floor;
 
 /* Line-search inputs besides the mesh (whose x is the step's start x0): the
  * target positions x1 (3 nv), VertexProp / EdgeProp / FaceProp mass (the
@@ -387,6 +417,12 @@ typedef struct ppf_ls_params {
     const float *vmass, *emass, *fmass;
     float max_t;
     float ccd_eps;
+    /* analytic colliders (device arrays; vertex_constraint_line_search,
+     * contact.cu:2318-2470); the mesh's vflag bit0 marks pinned vertices */
+    const ppf_sphere *spheres;
+    uint32_t nsphere;
+    const ppf_floor *floors;
+    uint32_t nfloor;
 } ppf_ls_params;
 
 typedef struct ppf_ccd ppf_ccd;
@@ -445,6 +481,10 @@ typedef struct ppf_shell {
 typedef struct ppf_elastic_params {
     float dt, air_density, air_friction;
     float wind[3];
+    /* many scenes in one shell: the time step of vertex i is
+     * scene_dt[vscene[i]] (both device arrays; null: dt for every vertex) */
+    const float *scene_dt;
+    const uint32_t *vscene;
 } ppf_elastic_params;
 
 typedef struct ppf_elastic ppf_elastic;
@@ -495,6 +535,11 @@ typedef struct ppf_scene {
     const float *v_fric, *e_fric, *f_fric;
     float *fix_pos;
     const float *fix_delta;
+    /* analytic colliders (host arrays, read at every advance) */
+    const ppf_sphere *spheres;
+    uint32_t nsphere;
+    const ppf_floor *floors;
+    uint32_t nfloor;
 } ppf_scene;
 
 /* The ParamSet fields the step reads (data.hpp:515-568). */
@@ -511,14 +556,19 @@ typedef struct ppf_step_params {
     float wind[3];
     int32_t disable_contact;
     int32_t check_intersection;
+    /* ParamSet.constraint_tol: the smallest gap a kinematic floor's barrier
+     * sees, as a fraction of its ghat (default 0.01) */
+    float constraint_tol;
 } ppf_step_params;
 
 /* Outcome of one step (StepResult, data.hpp:570-595). status: 0 success,
  * 1 PCG failed, 2 CCD failed (toi <= FLT_EPSILON), 3 a contact pair starts
  * inside its offset, 4 intersection after the step, 5 Newton stalled
- * (max_newton_steps). The step is committed (prev <- curr, curr <- the new
- * positions) on 0 and on 4, as main.cu:1515-1629 commits before the host sees
- * the intersection flag. Times are HIP-event milliseconds on the stream. */
+ * (max_newton_steps), 6 a pinned vertex is driven through a static analytic
+ * collider (StepResult.pin_feasible = false, main.cu:1240-1271). The step is
+ * committed (prev <- curr, curr <- the new positions) on 0 and on 4, as
+ * main.cu:1515-1629 commits before the host sees the intersection flag. Times
+ * are HIP-event milliseconds on the stream. */
 typedef struct ppf_step_result {
     int32_t status;
     uint32_t newton_steps;
@@ -546,6 +596,19 @@ void ppf_stepper_destroy(ppf_stepper *h);
  * prev / curr. */
 int ppf_stepper_advance(ppf_stepper *h, const ppf_scene *scene, const ppf_step_params *p,
                         ppf_step_result *out, void *stream);
+
+/* One time step of every scene of a stepper created over many scenes
+ * (scene->mesh.vscene / nscene, scene->collision.vscene). No counterpart in
+ * the reference, which steps one scene per process (main_helper::advance per
+ * scene): each scene runs its own Newton loop -- dt, toi bookkeeping, Newton
+ * count, status, commit --, and every stage (trees, proximity, assembly,
+ * block-Jacobi PCG with per-scene scalars, line search with per-scene minima,
+ * intersection check) is one launch over all scenes; elements of different
+ * scenes never meet. prev_dt: host array, one entry per scene (the step
+ * before's dt; p->prev_dt is not read); out: one result per scene. */
+int ppf_stepper_advance_scenes(ppf_stepper *h, const ppf_scene *scene,
+                               const ppf_step_params *p, const float *prev_dt,
+                               ppf_step_result *out, void *stream);
 
 /* Precondition the stepper's Newton solves with the learned factor of h
  * (recomputed for every Newton system; ms_solve includes it), falling back

@@ -484,3 +484,259 @@ uint64_t oracle_contact_assemble(const oracle_mesh_view *m,
     return nb;
 }
 }
+
+// ---- analytic colliders ----------------------------------------------------
+//
+// Per vertex against analytic spheres and floors (data.constraint.sphere /
+// floor; contact/contact.cu:1154-1382, embed_vertex_constraint_force_hessian):
+// push barrier with stiffness n.(K_ii n) + m / gap^2 (K_ii the vertex's
+// elastic diagonal block) plus friction for static colliders, the mass-only
+// stiffness m / ghat^2 and no friction for kinematic spheres. Pinned and
+// massless vertices are skipped (Dirichlet rows; static solids). The terms of
+// each collider are added to the vertex's force row and diagonal block in
+// collider order (push, then friction), onto what is already there.
+
+namespace {
+
+// The friction terms alone (energy/model/friction.hpp:19-81) for the push
+// force fp: gradient g, Hessian Hf.
+void friction_terms(const V3 &nrm, const V3 &dx, float mu, float eps, const V3 &fp, V3 &g,
+                    float Hf[9]) {
+    const float contact = -orc::dot(nrm, fp);
+    const float ndx = orc::dot(nrm, dx);
+    const V3 u = orc::sub(dx, v3(nrm[0] * ndx, nrm[1] * ndx, nrm[2] * ndx));
+    const float un = std::sqrt(orc::sqnorm(u));
+    const bool kinetic = un > eps;
+    const V3 sd = kinetic ? v3(u[0] / un, u[1] / un, u[2] / un) : v3(0, 0, 0);
+    const float lam = mu > 0.0f ? mu * contact / std::fmax(eps, un) : 0.0f;
+    g = orc::scale(lam, u);
+    if (kinetic) {
+        const V3 wd = orc::cross(nrm, sd);
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) Hf[r + 3 * c] = lam * (wd[r] * wd[c]);
+    } else {
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) {
+                const float id = r == c ? 1.0f : 0.0f;
+                Hf[r + 3 * c] = lam * (id - nrm[r] * nrm[c]);
+            }
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+struct oracle_sphere {
+    float center[3];
+    float ghat, friction, radius, thickness;
+    bool bowl, reverse, kinematic;
+};
+struct oracle_floor {
+    float ground[3];
+    float ghat, friction, thickness;
+    float up[3];
+    bool kinematic;
+};
+
+// Adds every vertex's analytic terms to grad (3 nv) and cdiag (9 nv). Returns
+// the number of (vertex, collider) contacts.
+uint32_t oracle_analytic_assemble(const oracle_mesh_view *m, const oracle_contact_params *cp,
+                                  const oracle_sphere *sp, uint32_t nsp,
+                                  const oracle_floor *fl, uint32_t nfl,
+                                  float constraint_tol, float *grad, float *cdiag) {
+    uint32_t contacts = 0;
+    for (uint32_t i = 0; i < m->nv; ++i) {
+        const float mass = cp->mass[i];
+        if ((m->vflag[i] & 1u) || !(mass > 0.0f)) {
+            continue;
+        }
+        const V3 x = v3(m->x[3 * i], m->x[3 * i + 1], m->x[3 * i + 2]);
+        const V3 x0 = v3(cp->x0[3 * i], cp->x0[3 * i + 1], cp->x0[3 * i + 2]);
+        const V3 dx = orc::sub(x, x0);
+        float lh[9];
+        lookup(i, i, cp->el_rp, cp->el_col, cp->el_val, lh);
+        V3 F = v3(grad[3 * i], grad[3 * i + 1], grad[3 * i + 2]);
+        float *H = cdiag + 9 * (size_t)i;
+        for (uint32_t j = 0; j < nsp; ++j) {
+            const oracle_sphere &s = sp[j];
+            const float ghat = s.ghat;
+            const float mu = combine(s.friction, cp->v_fric[i], cp->friction_mode);
+            bool reverse = s.reverse;
+            float radius = s.radius;
+            V3 center = v3(s.center[0], s.center[1], s.center[2]);
+            if (s.bowl && x[1] > center[1]) {
+                center = v3(center[0], x[1], center[2]);
+            }
+            const float d2 = orc::sqnorm(orc::sub(x, center));
+            if (s.kinematic) {
+                if (d2 == 0.0f) {
+                    continue;
+                }
+                const V3 normal = orc::normalized(orc::sub(x, center));
+                const float eff = reverse ? (radius - ghat) : (radius + ghat);
+                const V3 target = orc::scale(eff, normal);
+                const V3 o = orc::sub(x, center);
+                if (s.bowl) {
+                    reverse = true;
+                }
+                const float k = mass / (ghat * ghat);
+                const float r2 = eff * eff;
+                const float dist = std::sqrt(d2);
+                const float depth = reverse ? (dist - s.radius) : (s.radius - dist);
+                if (s.thickness > 0.0f && depth > s.thickness) {
+                    continue;
+                }
+                const bool hit = reverse ? d2 > r2 : d2 < r2;
+                if (hit) {
+                    ++contacts;
+                    const V3 nn = reverse ? v3(-normal[0], -normal[1], -normal[2]) : normal;
+                    V3 fp;
+                    float Hp[9];
+                    push_terms(orc::sub(o, target), nn, ghat, k, fp, Hp);
+                    F = orc::add(F, fp);
+                    for (int e = 0; e < 9; ++e) H[e] = H[e] + Hp[e];
+                }
+                continue;
+            }
+            radius = reverse ? radius - ghat : radius + ghat;
+            const float r2 = radius * radius;
+            const bool hit = reverse ? d2 > r2 : d2 < r2;
+            if (!hit) {
+                continue;
+            }
+            const float dist = std::sqrt(d2);
+            const float depth = reverse ? (dist - s.radius) : (s.radius - dist);
+            if (s.thickness > 0.0f && depth > s.thickness) {
+                continue;
+            }
+            ++contacts;
+            V3 normal = orc::normalized(orc::sub(x, center));
+            const V3 proj = orc::scale(radius, normal);
+            const V3 o = orc::sub(x, center);
+            if (reverse) {
+                normal = v3(-normal[0], -normal[1], -normal[2]);
+            }
+            const float gap = reverse ? s.radius - std::sqrt(d2) : std::sqrt(d2) - s.radius;
+            const float k = orc::dot(normal, orc::matvec(lh, normal.x)) + mass / (gap * gap);
+            V3 fp, fg;
+            float Hp[9], Hf[9];
+            push_terms(orc::sub(o, proj), normal, ghat, k, fp, Hp);
+            F = orc::add(F, fp);
+            for (int e = 0; e < 9; ++e) H[e] = H[e] + Hp[e];
+            friction_terms(normal, dx, mu, cp->friction_eps, fp, fg, Hf);
+            F = orc::add(F, fg);
+            for (int e = 0; e < 9; ++e) H[e] = H[e] + Hf[e];
+        }
+        for (uint32_t j = 0; j < nfl; ++j) {
+            const oracle_floor &f = fl[j];
+            const float ghat = f.ghat;
+            const float mu = combine(f.friction, cp->v_fric[i], cp->friction_mode);
+            const V3 up = v3(f.up[0], f.up[1], f.up[2]);
+            const V3 ground = orc::add(v3(f.ground[0], f.ground[1], f.ground[2]),
+                                       orc::scale(ghat, up));
+            const V3 e = orc::sub(x, ground);
+            const float eu = orc::dot(e, up);
+            if (!(eu < 0.0f)) {
+                continue;
+            }
+            if (f.thickness > 0.0f && -eu > f.thickness) {
+                continue;
+            }
+            ++contacts;
+            const V3 proj = orc::scale(-eu, up);
+            float gap = orc::dot(orc::sub(x, v3(f.ground[0], f.ground[1], f.ground[2])), up);
+            if (f.kinematic) {
+                gap = std::fmax(gap, constraint_tol * ghat);
+            }
+            const float k = orc::dot(up, orc::matvec(lh, up.x)) + mass / (gap * gap);
+            V3 fp, fg;
+            float Hp[9], Hf[9];
+            push_terms(v3(-proj[0], -proj[1], -proj[2]), up, ghat, k, fp, Hp);
+            F = orc::add(F, fp);
+            for (int q = 0; q < 9; ++q) H[q] = H[q] + Hp[q];
+            friction_terms(up, dx, mu, cp->friction_eps, fp, fg, Hf);
+            F = orc::add(F, fg);
+            for (int q = 0; q < 9; ++q) H[q] = H[q] + Hf[q];
+        }
+        grad[3 * i] = F[0], grad[3 * i + 1] = F[1], grad[3 * i + 2] = F[2];
+    }
+    return contacts;
+}
+
+// The analytic part of the line search (contact.cu:2318-2470): per vertex
+// the fraction of the swept step x0 -> x0 + max_t (x1 - x0) at which it
+// reaches a static sphere or floor, times max_t, folded into *toi (a minimum
+// in [0, max_t]); a pinned vertex driven from outside to inside a static
+// collider sets *pin_bad to the smallest such vertex index. Kinematic
+// colliders are not swept.
+void oracle_analytic_toi(uint32_t nv, const float *x0a, const float *x1a, const uint32_t *vflag,
+                         const float *vmass, const oracle_sphere *sp, uint32_t nsp,
+                         const oracle_floor *fl, uint32_t nfl, float max_t, float *toi,
+                         uint32_t *pin_bad) {
+    for (uint32_t i = 0; i < nv; ++i) {
+        const V3 y0 = v3(x0a[3 * i], x0a[3 * i + 1], x0a[3 * i + 2]);
+        const V3 y1 = v3(x1a[3 * i], x1a[3 * i + 1], x1a[3 * i + 2]);
+        const V3 x1 = orc::add(orc::scale(max_t, orc::sub(y1, y0)), y0);
+        const V3 &x0 = y0;
+        auto centers = [&](const oracle_sphere &s, V3 &c0, V3 &c1) {
+            const V3 c = v3(s.center[0], s.center[1], s.center[2]);
+            c0 = (s.bowl && x0[1] > c[1]) ? v3(c[0], x0[1], c[2]) : c;
+            c1 = (s.bowl && x1[1] > c[1]) ? v3(c[0], x1[1], c[2]) : c;
+        };
+        if (vflag[i] & 1u) {
+            for (uint32_t j = 0; j < nsp; ++j) {
+                const oracle_sphere &s = sp[j];
+                if (s.kinematic) continue;
+                V3 c0, c1;
+                centers(s, c0, c1);
+                const float r = s.radius;
+                const float r0 = std::sqrt(orc::sqnorm(orc::sub(x0, c0)));
+                const float r1 = std::sqrt(orc::sqnorm(orc::sub(x1, c1)));
+                const float d0 = s.reverse ? (r - r0) : (r0 - r);
+                const float d1 = s.reverse ? (r - r1) : (r1 - r);
+                if (d0 >= 0.0f && d1 < 0.0f) *pin_bad = std::min(*pin_bad, i);
+            }
+            for (uint32_t j = 0; j < nfl; ++j) {
+                const oracle_floor &f = fl[j];
+                if (f.kinematic) continue;
+                const V3 up = v3(f.up[0], f.up[1], f.up[2]);
+                const V3 g = v3(f.ground[0], f.ground[1], f.ground[2]);
+                const float h0 = orc::dot(up, orc::sub(x0, g));
+                const float h1 = orc::dot(up, orc::sub(x1, g));
+                if (h0 >= 0.0f && h1 < 0.0f) *pin_bad = std::min(*pin_bad, i);
+            }
+        } else if (vmass[i] > 0.0f) {
+            for (uint32_t j = 0; j < nsp; ++j) {
+                const oracle_sphere &s = sp[j];
+                if (s.kinematic) continue;
+                V3 c0, c1;
+                centers(s, c0, c1);
+                const float r = s.radius;
+                const float r0 = std::sqrt(orc::sqnorm(orc::sub(x0, c0)));
+                const float r1 = std::sqrt(orc::sqnorm(orc::sub(x1, c1)));
+                const float depth0 = s.reverse ? (r0 - r) : (r - r0);
+                if (s.thickness > 0.0f && depth0 > s.thickness) continue;
+                if ((r0 - r) * (r1 - r) <= 0.0f) {
+                    const float t = (r - r0) / (r1 - r0);
+                    *toi = std::fmin(*toi, max_t * t);
+                }
+            }
+            for (uint32_t j = 0; j < nfl; ++j) {
+                const oracle_floor &f = fl[j];
+                if (f.kinematic) continue;
+                const V3 up = v3(f.up[0], f.up[1], f.up[2]);
+                const V3 g = v3(f.ground[0], f.ground[1], f.ground[2]);
+                const float h0 = orc::dot(up, orc::sub(x0, g));
+                const float h1 = orc::dot(up, orc::sub(x1, g));
+                if (f.thickness > 0.0f && -h0 > f.thickness) continue;
+                if (h1 < 0.0f) {
+                    const float t = -h0 / (h1 - h0);
+                    *toi = std::fmin(*toi, max_t * t);
+                }
+            }
+        }
+    }
+}
+
+} // extern "C"

@@ -101,6 +101,25 @@ float oracle_line_search(const MeshView *m, const StaticView *sm, const LsParams
 uint32_t oracle_intersection_check(const MeshView *m, const StaticView *sm, const float *vmass,
                                    const float *emass, const float *fmass,
                                    uint32_t surface_vert_count, IsectRecord *rec, uint32_t cap);
+struct oracle_sphere {
+    float center[3];
+    float ghat, friction, radius, thickness;
+    bool bowl, reverse, kinematic;
+};
+struct oracle_floor {
+    float ground[3];
+    float ghat, friction, thickness;
+    float up[3];
+    bool kinematic;
+};
+uint32_t oracle_analytic_assemble(const MeshView *m, const ContactParams *cp,
+                                  const oracle_sphere *sp, uint32_t nsp,
+                                  const oracle_floor *fl, uint32_t nfl,
+                                  float constraint_tol, float *grad, float *cdiag);
+void oracle_analytic_toi(uint32_t nv, const float *x0a, const float *x1a, const uint32_t *vflag,
+                         const float *vmass, const oracle_sphere *sp, uint32_t nsp,
+                         const oracle_floor *fl, uint32_t nfl, float max_t, float *toi,
+                         uint32_t *pin_bad);
 
 struct oracle_scene {
     MeshView mesh;
@@ -111,6 +130,10 @@ struct oracle_scene {
     const float *v_fric, *e_fric, *f_fric;
     float *fix_pos;
     const float *fix_delta;
+    const oracle_sphere *spheres;
+    uint32_t nsphere;
+    const oracle_floor *floors;
+    uint32_t nfloor;
 };
 
 struct oracle_step_params {
@@ -126,6 +149,7 @@ struct oracle_step_params {
     float wind[3];
     int32_t disable_contact;
     int32_t check_intersection;
+    float constraint_tol;
 };
 
 struct oracle_step_result {
@@ -270,6 +294,10 @@ int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_resul
         const uint64_t nb = oracle_contact_assemble(&m, &cp, pairs.data(), pairs.size(),
                                                     cgrad.data(), crow.data(), ccol.data(),
                                                     cval.data(), cap);
+        // analytic spheres and floors (contact.cu:1154-1382) into the contact
+        // gradient and the per-vertex diagonal
+        oracle_analytic_assemble(&m, &cp, sc->spheres, sc->nsphere, sc->floors, sc->nfloor,
+                                 p->constraint_tol, cgrad.data(), cdiag.data());
         for (size_t k = 0; k < n3; ++k) force[k] = force[k] + cgrad[k];
         // merged upper matrix: elastic pattern + contact blocks (A + B)
         std::vector<uint32_t> mrp(n + 1, 0), mci;
@@ -369,8 +397,23 @@ int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_resul
             LsParams lp{ex.data(), sc->shell.vmass, sc->emass, sc->shell.fmass,
                         p->line_search_max_t, p->ccd_eps};
             int overlap = 0;
-            const float t = oracle_line_search(&lm, sc->collision.nv ? &sc->collision : nullptr,
-                                               &lp, &overlap);
+            float t = oracle_line_search(&lm, sc->collision.nv ? &sc->collision : nullptr,
+                                         &lp, &overlap);
+            // the analytic colliders' entry times join the minimum (in max_t
+            // units); pins driven into a static collider fail the step
+            // (main.cu:1240-1271)
+            if (sc->nsphere || sc->nfloor) {
+                float ta = p->line_search_max_t;
+                uint32_t pin_bad = 0xffffffffu;
+                oracle_analytic_toi(n, target.data(), ex.data(), sc->mesh.vflag,
+                                    sc->shell.vmass, sc->spheres, sc->nsphere, sc->floors,
+                                    sc->nfloor, p->line_search_max_t, &ta, &pin_bad);
+                if (pin_bad != 0xffffffffu) {
+                    out->status = 6;
+                    break;
+                }
+                t = std::fmin(t, ta / p->line_search_max_t);
+            }
             toi = std::fmin(toi, t);
             if (overlap) {
                 out->status = 3;

@@ -259,6 +259,63 @@ def contact_assemble(m, x0, mass, fric, pairs, friction_mode=0, friction_eps=1e-
     return grad, rows[:n], cols[:n], vals[:n]
 
 
+def analytic_assemble(m, x0, mass, v_fric, spheres=(), floors=(), friction_mode=0,
+                      friction_eps=1e-5, constraint_tol=0.01, elastic=None, grad=None,
+                      cdiag=None):
+    """Analytic sphere / floor terms of every vertex (contact.cu:1154-1382)
+    added to grad (3 nv) and cdiag (nv, 9), both zero when not given.
+    Returns (grad, cdiag, contacts)."""
+    lib = load()
+    if not hasattr(lib, "_ana_ready"):
+        lib.oracle_analytic_assemble.argtypes = [ctypes.POINTER(ContactMesh),
+                                                 ctypes.POINTER(ContactParams), vp, u32, vp,
+                                                 u32, f32, vp, vp]
+        lib.oracle_analytic_assemble.restype = u32
+        lib._ana_ready = True
+    s, keep = _mesh_struct(m)
+    x0 = _f32(x0).reshape(-1)
+    mass = _f32(mass)
+    fv = _f32(v_fric)
+    el = elastic or (None, None, None)
+    el_rp = _u32(el[0]) if el[0] is not None else None
+    el_col = _u32(el[1]) if el[1] is not None else None
+    el_val = _f32(el[2]).reshape(-1) if el[2] is not None else None
+    p = ContactParams(x0=_p(x0), mass=_p(mass), v_fric=_p(fv), e_fric=None, f_fric=None,
+                      friction_mode=friction_mode, friction_eps=friction_eps,
+                      el_rp=_p(el_rp) if el_rp is not None else None,
+                      el_col=_p(el_col) if el_col is not None else None,
+                      el_val=_p(el_val) if el_val is not None else None, sm=None)
+    sp, nsp, fl, nfl = collider_arrays({"spheres": list(spheres), "floors": list(floors)})
+    nv = int(m["nv"])
+    grad = np.zeros(3 * nv, np.float32) if grad is None else _f32(grad).reshape(-1).copy()
+    cdiag = np.zeros((nv, 9), np.float32) if cdiag is None else _f32(cdiag).reshape(nv, 9).copy()
+    n = lib.oracle_analytic_assemble(ctypes.byref(s), ctypes.byref(p), ctypes.cast(sp, vp), nsp,
+                                     ctypes.cast(fl, vp), nfl, constraint_tol, _p(grad),
+                                     _p(cdiag))
+    return grad, cdiag, int(n)
+
+
+def analytic_toi(x0, x1, vflag, vmass, spheres=(), floors=(), max_t=1.25):
+    """The analytic part of the line search (contact.cu:2318-2470): (toi in
+    [0, max_t] over the swept step x0 -> x0 + max_t (x1 - x0), the smallest
+    pinned vertex driven into a static collider or None)."""
+    lib = load()
+    if not hasattr(lib, "_anatoi_ready"):
+        lib.oracle_analytic_toi.argtypes = [u32, vp, vp, vp, vp, vp, u32, vp, u32, f32, vp, vp]
+        lib.oracle_analytic_toi.restype = None
+        lib._anatoi_ready = True
+    x0 = _f32(x0).reshape(-1)
+    x1 = _f32(x1).reshape(-1)
+    vf = _u32(vflag)
+    vm = _f32(vmass)
+    sp, nsp, fl, nfl = collider_arrays({"spheres": list(spheres), "floors": list(floors)})
+    toi = np.array([max_t], np.float32)
+    bad = np.array([0xFFFFFFFF], np.uint32)
+    lib.oracle_analytic_toi(len(vm), _p(x0), _p(x1), _p(vf), _p(vm), ctypes.cast(sp, vp), nsp,
+                            ctypes.cast(fl, vp), nfl, max_t, _p(toi), _p(bad))
+    return float(toi[0]), (None if bad[0] == 0xFFFFFFFF else int(bad[0]))
+
+
 # ---- CCD / line search ------------------------------------------------------
 
 class StaticMesh(ctypes.Structure):
@@ -501,10 +558,43 @@ def static_proximity(m, sm, vmass, emass, fmass):
 
 # ---- one time step ------------------------------------------------------------
 
+class OracleSphere(ctypes.Structure):
+    """Analytic sphere (data.hpp:449-458)."""
+    _fields_ = [("center", f32 * 3), ("ghat", f32), ("friction", f32), ("radius", f32),
+                ("thickness", f32), ("bowl", ctypes.c_bool), ("reverse", ctypes.c_bool),
+                ("kinematic", ctypes.c_bool)]
+
+
+class OracleFloor(ctypes.Structure):
+    """Analytic floor (data.hpp:460-467)."""
+    _fields_ = [("ground", f32 * 3), ("ghat", f32), ("friction", f32), ("thickness", f32),
+                ("up", f32 * 3), ("kinematic", ctypes.c_bool)]
+
+
+def collider_arrays(sc, sphere_t=OracleSphere, floor_t=OracleFloor):
+    """ctypes arrays of a scene's analytic colliders (sc['spheres'] /
+    sc['floors']: lists of dicts with the Sphere / Floor fields)."""
+    sp = sc.get("spheres") or []
+    fl = sc.get("floors") or []
+    a = (sphere_t * max(1, len(sp)))()
+    for k, d in enumerate(sp):
+        a[k] = sphere_t(center=(f32 * 3)(*d["center"]), ghat=d["ghat"], friction=d["friction"],
+                        radius=d["radius"], thickness=d.get("thickness", 0.0),
+                        bowl=bool(d.get("bowl", False)), reverse=bool(d.get("reverse", False)),
+                        kinematic=bool(d.get("kinematic", False)))
+    b = (floor_t * max(1, len(fl)))()
+    for k, d in enumerate(fl):
+        b[k] = floor_t(ground=(f32 * 3)(*d["ground"]), ghat=d["ghat"], friction=d["friction"],
+                       thickness=d.get("thickness", 0.0), up=(f32 * 3)(*d.get("up", (0, 1, 0))),
+                       kinematic=bool(d.get("kinematic", False)))
+    return a, len(sp), b, len(fl)
+
+
 class OracleScene(ctypes.Structure):
     _fields_ = [("mesh", ContactMesh), ("shell", Shell), ("collision", StaticMesh),
                 ("prev", vp), ("emass", vp), ("v_fric", vp), ("e_fric", vp), ("f_fric", vp),
-                ("fix_pos", vp), ("fix_delta", vp)]
+                ("fix_pos", vp), ("fix_delta", vp), ("spheres", vp), ("nsphere", u32),
+                ("floors", vp), ("nfloor", u32)]
 
 
 class OracleStepParams(ctypes.Structure):
@@ -514,7 +604,7 @@ class OracleStepParams(ctypes.Structure):
                 ("friction_mode", ctypes.c_int32), ("min_newton_steps", u32),
                 ("max_newton_steps", u32), ("air_density", f32), ("air_friction", f32),
                 ("wind", f32 * 3), ("disable_contact", ctypes.c_int32),
-                ("check_intersection", ctypes.c_int32)]
+                ("check_intersection", ctypes.c_int32), ("constraint_tol", f32)]
 
 
 class OracleStepResult(ctypes.Structure):
@@ -579,10 +669,16 @@ class OracleStepper:
         return out
 
     def advance(self, params):
-        kw = dict(params)
+        kw = {k: v for k, v in params.items() if k in dict(OracleStepParams._fields_)}
         kw["gravity"] = (f32 * 3)(*params["gravity"])
         kw["wind"] = (f32 * 3)(*params["wind"])
         p = OracleStepParams(**kw)
         r = OracleStepResult()
+        # the analytic colliders as they are now (kinematic ones move)
+        sp, nsp, fl, nfl = collider_arrays(self.sc)
+        self.scene.spheres = ctypes.cast(sp, vp)
+        self.scene.nsphere = nsp
+        self.scene.floors = ctypes.cast(fl, vp)
+        self.scene.nfloor = nfl
         self.lib.oracle_step(ctypes.byref(self.scene), ctypes.byref(p), ctypes.byref(r))
         return {k: getattr(r, k) for k, _ in OracleStepResult._fields_}

@@ -88,6 +88,42 @@ __device__ inline void lookup(uint32_t i, uint32_t j, const uint32_t *rp,
 }
 
 
+// Friction (energy/model/friction.hpp:19-81) of the push force fp: gradient
+// g and Hessian Hf alone (normal force -n.fp, tangential displacement of dx,
+// kinetic above eps).
+__device__ inline void friction_terms(f3 nrm, f3 dx, float mu, float eps, f3 fp, f3 &g,
+                                      float Hf[9]) {
+    const float contact = -dot3(nrm, fp);
+    const float ndx = dot3(nrm, dx);
+    const f3 u = dx - mk3(nrm.x * ndx, nrm.y * ndx, nrm.z * ndx);
+    const float un = sqrtf(dot3(u, u));
+    const bool kinetic = un > eps;
+    const f3 sd = kinetic ? mk3(u.x / un, u.y / un, u.z / un) : mk3(0, 0, 0);
+    const float lam = mu > 0.0f ? mu * contact / fmaxf(eps, un) : 0.0f;
+    g = lam * u;
+    const float na[3] = {nrm.x, nrm.y, nrm.z};
+    if (kinetic) {
+        const f3 wd = cross3(nrm, sd);
+        const float wa[3] = {wd.x, wd.y, wd.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                Hf[r + 3 * c] = lam * (wa[r] * wa[c]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const float id = r == c ? 1.0f : 0.0f;
+                Hf[r + 3 * c] = lam * (id - na[r] * na[c]);
+            }
+        }
+    }
+}
+
 // Friction (energy/model/friction.hpp:19-81) on the pair force f and Hessian
 // H: normal force -n.f, tangential displacement of dx, kinetic above eps.
 __device__ inline void add_friction(f3 nrm, f3 dx, float mu, float eps, f3 &f, float H[9]) {
@@ -599,6 +635,151 @@ __global__ __launch_bounds__(kNT) void k_block_reduce(
 
 uint32_t cdiv64(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Analytic spheres and floors per vertex (data.constraint.sphere / floor;
+// embed_vertex_constraint_force_hessian, contact/contact.cu:1154-1382): the
+// push barrier with stiffness n.(K_ii n) + m / gap^2 (K_ii the vertex's
+// elastic diagonal block) and friction for static colliders, stiffness
+// m / ghat^2 and no friction for kinematic spheres. Pinned vertices (their
+// rows are eliminated) and massless ones (static solids) are skipped. Each
+// collider's push, then its friction, is added to the vertex's gradient row
+// and diagonal block, in collider order (one thread per vertex: no atomics).
+__global__ __launch_bounds__(kNT) void k_analytic(ppf_contact_mesh m, ppf_contact_params cp,
+                                                  const ppf_sphere *__restrict__ sp,
+                                                  uint32_t nsp, const ppf_floor *__restrict__ fl,
+                                                  uint32_t nfl, float constraint_tol,
+                                                  float *__restrict__ grad,
+                                                  float *__restrict__ cdiag) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= m.nv) {
+        return;
+    }
+    const float mass = cp.mass[i];
+    if ((m.vflag[i] & 1u) || !(mass > 0.0f)) {
+        return;
+    }
+    const f3 x = X(m.x, i);
+    const f3 dx = x - X(cp.x0, i);
+    float lh[9];
+    lookup(i, i, cp.el_rp, cp.el_col, cp.el_val, lh);
+    f3 F = X(grad, i);
+    float *Hg = cdiag + 9 * (size_t)i;
+    float H[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        H[e] = Hg[e];
+    }
+    f3 fp, fg;
+    float Hp[9], Hf[9];
+    for (uint32_t j = 0; j < nsp; ++j) {
+        const ppf_sphere s = sp[j];
+        const float ghat = s.ghat;
+        const float mu = combine(s.friction, cp.v_fric[i], cp.friction_mode);
+        bool reverse = s.reverse;
+        float radius = s.radius;
+        f3 center = mk3(s.center[0], s.center[1], s.center[2]);
+        if (s.bowl && x.y > center.y) {
+            center = mk3(center.x, x.y, center.z);
+        }
+        const f3 o = x - center;
+        const float d2 = dot3(o, o);
+        if (s.kinematic) {
+            if (d2 == 0.0f) {
+                continue;
+            }
+            const f3 normal = normalized(o);
+            const float eff = reverse ? (radius - ghat) : (radius + ghat);
+            const f3 target = eff * normal;
+            if (s.bowl) {
+                reverse = true;
+            }
+            const float k = mass / (ghat * ghat);
+            const float r2 = eff * eff;
+            const float dist = sqrtf(d2);
+            const float depth = reverse ? (dist - s.radius) : (s.radius - dist);
+            if (s.thickness > 0.0f && depth > s.thickness) {
+                continue;
+            }
+            if (reverse ? d2 > r2 : d2 < r2) {
+                const f3 nn = reverse ? mk3(-normal.x, -normal.y, -normal.z) : normal;
+                push_terms(o - target, nn, ghat, k, fp, Hp);
+                F = F + fp;
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    H[e] = H[e] + Hp[e];
+                }
+            }
+            continue;
+        }
+        radius = reverse ? radius - ghat : radius + ghat;
+        const float r2 = radius * radius;
+        if (!(reverse ? d2 > r2 : d2 < r2)) {
+            continue;
+        }
+        const float dist = sqrtf(d2);
+        const float depth = reverse ? (dist - s.radius) : (s.radius - dist);
+        if (s.thickness > 0.0f && depth > s.thickness) {
+            continue;
+        }
+        f3 normal = normalized(o);
+        const f3 proj = radius * normal;
+        if (reverse) {
+            normal = mk3(-normal.x, -normal.y, -normal.z);
+        }
+        const float gap = reverse ? s.radius - sqrtf(d2) : sqrtf(d2) - s.radius;
+        const float k = dot3(normal, mv(lh, normal)) + mass / (gap * gap);
+        push_terms(o - proj, normal, ghat, k, fp, Hp);
+        F = F + fp;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            H[e] = H[e] + Hp[e];
+        }
+        friction_terms(normal, dx, mu, cp.friction_eps, fp, fg, Hf);
+        F = F + fg;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            H[e] = H[e] + Hf[e];
+        }
+    }
+    for (uint32_t j = 0; j < nfl; ++j) {
+        const ppf_floor f = fl[j];
+        const float ghat = f.ghat;
+        const float mu = combine(f.friction, cp.v_fric[i], cp.friction_mode);
+        const f3 up = mk3(f.up[0], f.up[1], f.up[2]);
+        const f3 g0 = mk3(f.ground[0], f.ground[1], f.ground[2]);
+        const f3 e = x - (g0 + ghat * up);
+        const float eu = dot3(e, up);
+        if (!(eu < 0.0f)) {
+            continue;
+        }
+        if (f.thickness > 0.0f && -eu > f.thickness) {
+            continue;
+        }
+        const f3 proj = (-eu) * up;
+        float gap = dot3(x - g0, up);
+        if (f.kinematic) {
+            gap = fmaxf(gap, constraint_tol * ghat);
+        }
+        const float k = dot3(up, mv(lh, up)) + mass / (gap * gap);
+        push_terms(mk3(-proj.x, -proj.y, -proj.z), up, ghat, k, fp, Hp);
+        F = F + fp;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            H[q] = H[q] + Hp[q];
+        }
+        friction_terms(up, dx, mu, cp.friction_eps, fp, fg, Hf);
+        F = F + fg;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            H[q] = H[q] + Hf[q];
+        }
+    }
+    store3(grad + 3 * (size_t)i, F);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        Hg[e] = H[e];
+    }
+}
+
 } // namespace
 
 ContactAssembler::~ContactAssembler() { release(); }
@@ -622,7 +803,7 @@ int ContactAssembler::reserve(uint64_t npairs, uint32_t nv) {
         return PPF_OK;
     }
     release();
-    npairs = npairs < 1 ? 1 : npairs;
+    npairs = grow_cap(npairs, 256);
     const uint64_t nh = 16 * npairs, nf = 4 * npairs;
     const size_t sortw = radix_sort_scratch_words(nh);
     const size_t scanw = scan_scratch_words(nh + 1);
@@ -706,6 +887,19 @@ int ContactAssembler::assemble(const ppf_contact_mesh &m, const ppf_contact_para
                                                    out_val_);
     PPF_HIP(hipGetLastError());
     k_lower_bound<<<cdiv64(nv + 1, kNT), kNT, 0, s>>>(out_row_, nnz_, nv, rp_);
+    PPF_HIP(hipGetLastError());
+    return PPF_OK;
+}
+
+int analytic_assemble(const ppf_contact_mesh &m, const ppf_contact_params &cp,
+                      const ppf_sphere *spheres, uint32_t nsphere, const ppf_floor *floors,
+                      uint32_t nfloor, float constraint_tol, float *grad, float *cdiag,
+                      hipStream_t s) {
+    if (m.nv == 0 || (nsphere == 0 && nfloor == 0)) {
+        return PPF_OK;
+    }
+    k_analytic<<<cdiv64(m.nv, kNT), kNT, 0, s>>>(m, cp, spheres, nsphere, floors, nfloor,
+                                                  constraint_tol, grad, cdiag);
     PPF_HIP(hipGetLastError());
     return PPF_OK;
 }

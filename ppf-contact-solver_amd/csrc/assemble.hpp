@@ -30,4 +30,11 @@ class ContactAssembler {
     uint32_t vcap_ = 0;
 };
 
+// Analytic sphere / floor terms of every vertex added to grad (3 nv) and the
+// per-vertex diagonal cdiag (9 nv); spheres and floors are device arrays.
+int analytic_assemble(const ppf_contact_mesh &m, const ppf_contact_params &cp,
+                      const ppf_sphere *spheres, uint32_t nsphere, const ppf_floor *floors,
+                      uint32_t nfloor, float constraint_tol, float *grad, float *cdiag,
+                      hipStream_t s);
+
 } // namespace ppf

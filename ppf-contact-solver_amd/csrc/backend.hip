@@ -121,6 +121,10 @@ struct Backend {
     uint32_t *sh_vflag = nullptr, *sh_fflag = nullptr, *hflag = nullptr;
     ppf_scene sc{};
     ppf_stepper *stepper = nullptr;
+    // analytic colliders of the current constraint (copied: the host's
+    // arrays need not outlive the call that handed them over)
+    std::vector<ppf_sphere> spheres;
+    std::vector<ppf_floor> floors;
     // collision windows
     bool cw = false;
     std::vector<uint32_t> cw_dmap, cw_counts;
@@ -251,6 +255,24 @@ struct Backend {
         hip_or_die(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
 
+    // The constraint's analytic colliders (cts_Sphere / cts_Floor have the
+    // layout of ppf_sphere / ppf_floor: data.rs:689-709).
+    void copy_colliders() {
+        const cts_Constraint &c = host.constraint;
+        spheres.resize(c.sphere.size);
+        floors.resize(c.floor.size);
+        if (c.sphere.size) {
+            std::memcpy(spheres.data(), c.sphere.data, c.sphere.size * sizeof(ppf_sphere));
+        }
+        if (c.floor.size) {
+            std::memcpy(floors.data(), c.floor.data, c.floor.size * sizeof(ppf_floor));
+        }
+        sc.spheres = spheres.empty() ? nullptr : spheres.data();
+        sc.nsphere = (uint32_t)spheres.size();
+        sc.floors = floors.empty() ? nullptr : floors.data();
+        sc.nfloor = (uint32_t)floors.size();
+    }
+
     // fix_index / pull_index and the all-pinned element flags from the
     // constraint, written into the host's prop arrays (main.cu:2018-2076).
     void derive_fixed() {
@@ -336,6 +358,9 @@ struct Backend {
 
 Backend g;
 
+static_assert(sizeof(cts_Sphere) == sizeof(ppf_sphere) && sizeof(cts_Floor) == sizeof(ppf_floor),
+              "analytic collider layouts of the two C ABIs differ");
+
 // Content this backend does not integrate: initialize() refuses it by name.
 const char *unsupported(const cts_DataSet &d, const cts_ParamSet &p) {
     if (d.mesh.mesh.tet.size) return "tetrahedral solids";
@@ -349,8 +374,6 @@ const char *unsupported(const cts_DataSet &d, const cts_ParamSet &p) {
     const cts_Constraint &c = d.constraint;
     if (c.pull.size) return "pull constraints";
     if (c.torque_groups.size || c.torque_vertices.size) return "torques";
-    if (c.sphere.size) return "analytic sphere colliders";
-    if (c.floor.size) return "analytic floor colliders";
     if (c.stitch.size) return "stitches";
     if (p.barrier != CTS_BARRIER_CUBIC) return "non-cubic barriers";
     if (p.precond != CTS_PRECOND_BLOCK_JACOBI) return "the Schwarz preconditioner";
@@ -632,6 +655,7 @@ bool initialize(const cts_DataSet *data, cts_ParamSet *param) {
     hip_or_die(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking), "hipStreamCreate");
     g.derive_fixed();
     upload_scene(g);
+    g.copy_colliders();
     g.upload_flags();
     g.upload_pins();
     ppf_or_die(ppf_stepper_create(&g.sc, g.s, &g.stepper), "ppf_stepper_create");
@@ -673,6 +697,7 @@ void advance(cts_StepResult *result) {
     sp.air_friction = p.air_friction;
     sp.disable_contact = p.disable_contact ? 1 : 0;
     sp.check_intersection = p.disable_contact ? 0 : 1;
+    sp.constraint_tol = p.constraint_tol;
     ppf_step_result r{};
     ppf_or_die(ppf_stepper_advance(g.stepper, &g.sc, &sp, &r, g.s), "advance");
     g.carried = false; // the step's own contact pattern replaces a carried one
@@ -681,7 +706,7 @@ void advance(cts_StepResult *result) {
     out.pcg_success = r.status != 1;
     out.intersection_free = r.status != 4;
     out.newton_progress = r.status != 5;
-    out.pin_feasible = true;
+    out.pin_feasible = r.status != 6;
     out.contact_separated = r.status != 3;
     if (r.status == 0 || r.status == 4) {
         // the step is committed (main.cu:1572-1629)
@@ -815,10 +840,9 @@ void update_constraint(const cts_Constraint *constraint) {
         return;
     }
     const cts_Constraint &c = *constraint;
-    if (c.pull.size || c.torque_groups.size || c.torque_vertices.size || c.sphere.size ||
-        c.floor.size || c.stitch.size) {
-        fatal(CTS_FATAL_SOLVER, "update_constraint: pulls, torques, analytic colliders and "
-                                "stitches are not supported by this backend");
+    if (c.pull.size || c.torque_groups.size || c.torque_vertices.size || c.stitch.size) {
+        fatal(CTS_FATAL_SOLVER, "update_constraint: pulls, torques and stitches are not "
+                                "supported by this backend");
     }
     for (uint32_t k = 0; k < c.fix.size; ++k) {
         if (c.fix.data[k].index >= g.nv) {
@@ -828,6 +852,7 @@ void update_constraint(const cts_Constraint *constraint) {
     }
     g.host.constraint = c;
     g.derive_fixed();
+    g.copy_colliders();
     g.upload_flags();
     g.upload_pins();
     ppf_or_die(ppf_stepper_update_pins(g.stepper, &g.sc, g.s), "update_constraint");

@@ -21,14 +21,25 @@ constexpr int kQueryThreads = 128; // threads per query workgroup
 // 48 rather than 64 entries lets 12 one-wave workgroups share a CU's LDS.
 constexpr int kQueryStack = 48;
 
+// A query box and the scene of its query element (0 in a one-scene mesh).
 struct QBox {
     float mn[3], mx[3];
+    uint32_t scene = 0;
 };
 
+// Box overlap (aabb::overlap), restricted to nodes holding the query's scene:
+// a node whose leaves all belong to another scene is never entered.
 __device__ inline bool overlaps(const Aabb &a, const QBox &q) {
-    return a.active != 0.0f && (a.mn[0] <= q.mx[0] && a.mx[0] >= q.mn[0]) &&
+    const uint32_t tag = __float_as_uint(a.pad);
+    return a.active != 0.0f && (tag == q.scene || tag == kMixedScene) &&
+           (a.mn[0] <= q.mx[0] && a.mx[0] >= q.mn[0]) &&
            (a.mn[1] <= q.mx[1] && a.mx[1] >= q.mn[1]) &&
            (a.mn[2] <= q.mx[2] && a.mx[2] >= q.mn[2]);
+}
+
+// Scene id of a vertex (0 when the mesh holds one scene).
+__device__ inline uint32_t vertex_scene(const uint32_t *vscene, uint32_t nscene, uint32_t v) {
+    return (vscene && nscene > 1) ? vscene[v] : 0u;
 }
 
 __device__ inline Aabb load_box(const Aabb *b, uint32_t i) {
@@ -45,7 +56,8 @@ __device__ inline Aabb load_box(const Aabb *b, uint32_t i) {
 // (aabb::make, contact/aabb.hpp:45-60).
 __device__ inline QBox seg_box(f3 a, f3 b, float margin) {
     QBox q{{fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)},
-           {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}};
+           {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)},
+           0u};
     if (margin != 0.0f) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -60,13 +72,14 @@ __device__ inline QBox seg_box(f3 a, f3 b, float margin) {
 // collision window (an inactive query box, contact.cu:1637, 1728, 2563).
 __device__ inline QBox empty_box() {
     const float inf = __builtin_inff();
-    return QBox{{inf, inf, inf}, {-inf, -inf, -inf}};
+    return QBox{{inf, inf, inf}, {-inf, -inf, -inf}, 0u};
 }
 
 // Point +/- margin (aabb::make, contact/aabb.hpp:62-69).
 __device__ inline QBox point_box(f3 p, float margin) {
     return QBox{{p.x - margin, p.y - margin, p.z - margin},
-                {p.x + margin, p.y + margin, p.z + margin}};
+                {p.x + margin, p.y + margin, p.z + margin},
+                0u};
 }
 
 // Visits every leaf whose box overlaps q; `leaf(prim)` returns whether the

@@ -718,6 +718,17 @@ int ppf_stepper_advance(ppf_stepper *h, const ppf_scene *scene, const ppf_step_p
     return h->impl.advance(*scene, *p, out, (hipStream_t)stream);
 }
 
+int ppf_stepper_advance_scenes(ppf_stepper *h, const ppf_scene *scene,
+                               const ppf_step_params *p, const float *prev_dt,
+                               ppf_step_result *out, void *stream) {
+    if (!h || !scene || !p || !prev_dt || !out || !(p->dt > 0.0f) ||
+        !(p->line_search_max_t > 0.0f) || p->cg_max_iter == 0) {
+        set_error("ppf_stepper_advance_scenes: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    return h->impl.advance_scenes(*scene, *p, prev_dt, out, (hipStream_t)stream);
+}
+
 int ppf_stepper_set_preconditioner(ppf_stepper *s, ppf_lpc *h) {
     if (!s) {
         set_error("ppf_stepper_set_preconditioner: stepper is null");

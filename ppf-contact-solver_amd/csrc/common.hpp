@@ -135,6 +135,18 @@ void set_error(const char *fmt, ...);
 void count_alloc();
 void count_free();
 void alloc_totals(uint64_t *allocs, uint64_t *frees);
+// Capacity of a buffer that grows with the contact set: the next power of
+// two at or above twice the need (at least `floor`), as the reference sizes
+// its pair caches (contact/contact.cu:1389-1396). Growth events are then
+// rare, and a step whose sizes stay below the high-water mark allocates
+// nothing.
+inline uint64_t grow_cap(uint64_t need, uint64_t floor = 1024) {
+    uint64_t c = floor;
+    while (c < 2 * need) {
+        c *= 2;
+    }
+    return c;
+}
 template <class T> hipError_t dev_alloc(T **p, size_t bytes) {
     const hipError_t e = hipMalloc(reinterpret_cast<void **>(p), bytes);
     if (e == hipSuccess) {

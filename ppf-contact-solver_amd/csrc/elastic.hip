@@ -144,13 +144,14 @@ __global__ __launch_bounds__(kNT) void k_vertex(ppf_shell sh, ppf_elastic_params
         v_normalize(n);
     }
     const f3 y = X(sh.x, i), x = X(sh.x0, i), tg = X(sh.target, i);
-    const float dt2 = p.dt * p.dt;
+    const float dt = (p.scene_dt && p.vscene) ? p.scene_dt[p.vscene[i]] : p.dt;
+    const float dt2 = dt * dt;
     f3 f = mk3(0.0f, 0.0f, 0.0f);
     M3 H = m_zero();
     const bool nz = !(fabsf(n.x) <= 1e-5f) || !(fabsf(n.y) <= 1e-5f) || !(fabsf(n.z) <= 1e-5f);
     if (nz && p.air_density != 0.0f) {
         const f3 wind = mk3(p.wind[0], p.wind[1], p.wind[2]);
-        const f3 z = (y - x) - p.dt * wind;
+        const f3 z = (y - x) - dt * wind;
         const M3 P = m_sub(m_eye(), m_outer(n, n));
         const f3 g = v_div(v_mul(n, dot3(n, z)), dt2) + v_div(m_mul(m_scale(p.air_friction, P), z), dt2);
         const M3 h = m_add(m_div(m_outer(n, n), dt2), m_div(m_scale(p.air_friction, P), dt2));

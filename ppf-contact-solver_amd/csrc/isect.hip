@@ -28,7 +28,8 @@ struct IsCtx {
     uint32_t nsurf;
     ppf_isect_record *rec;
     uint32_t cap;
-    uint32_t *ctr; // [0] pairs found, [1] stack overflow
+    uint32_t *ctr; // [0] pairs found, [1] stack overflow, [2 + c] found in scene c
+    uint32_t ns;   // scenes counted separately (0: one scene)
 };
 
 __device__ inline f3 X(const float *x, uint32_t v) { return load3(x + 3 * (size_t)v); }
@@ -77,8 +78,11 @@ __device__ bool edge_triangle(f3 a0, f3 a1, f3 b0, f3 b1, f3 b2) {
 }
 
 __device__ void record(const IsCtx &c, uint32_t type, uint32_t e0, uint32_t e1,
-                       const f3 *a, uint32_t na, const f3 *b, uint32_t nb) {
+                       const f3 *a, uint32_t na, const f3 *b, uint32_t nb, uint32_t scene) {
     const uint32_t slot = atomicAdd(c.ctr, 1u);
+    if (c.ns) {
+        atomicAdd(c.ctr + 2 + scene, 1u);
+    }
     if (slot < c.cap) {
         ppf_isect_record r;
         r.type = type, r.elem0 = e0, r.elem1 = e1, r.num_verts0 = na, r.num_verts1 = nb;
@@ -107,7 +111,8 @@ __global__ __launch_bounds__(kQT) void k_isect_edges(IsCtx c) {
     const uint32_t ei = c.eb.node[2 * (size_t)leaf] - 1;
     const uint32_t a0 = m.edge[2 * (size_t)ei], a1 = m.edge[2 * (size_t)ei + 1];
     const f3 y[2] = {X(m.x, a0), X(m.x, a1)};
-    const QBox q = seg_box(y[0], y[1], 0.0f);
+    QBox q = seg_box(y[0], y[1], 0.0f);
+    q.scene = vertex_scene(m.vscene, m.nscene, a0);
     const bool efix = (m.eflag[ei] & 1u) != 0;
     const bool emass_pos = c.emass[ei] > 0.0f;
     // face-edge (contact.cu:2879-2946)
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(kQT) void k_isect_edges(IsCtx c) {
         }
         const f3 fv[3] = {X(m.x, f0), X(m.x, f1), X(m.x, f2)};
         if (edge_triangle(y[0], y[1], fv[0], fv[1], fv[2])) {
-            record(c, 0, fi, ei, fv, 3, y, 2);
+            record(c, 0, fi, ei, fv, 3, y, 2, q.scene);
             return true;
         }
         return false;
@@ -154,7 +159,7 @@ __global__ __launch_bounds__(kQT) void k_isect_edges(IsCtx c) {
             const f3 r = w[2] * ev[0] + w[3] * ev[1];
             const f3 e = p - r;
             if (dot3(e, e) < off * off) {
-                record(c, 1, ei, ej, y, 2, ev, 2);
+                record(c, 1, ei, ej, y, 2, ev, 2, q.scene);
                 return true;
             }
             return false;
@@ -171,7 +176,7 @@ __global__ __launch_bounds__(kQT) void k_isect_edges(IsCtx c) {
                 const f3 e0 = (c0[0] * q0 + c0[1] * q2) - q1;
                 const f3 e1 = (c1[0] * q0 + c1[1] * q1) - q2;
                 if (fminf(dot3(e0, e0), dot3(e1, e1)) < off * off) {
-                    record(c, 1, ei, ej, y, 2, ev, 2);
+                    record(c, 1, ei, ej, y, 2, ev, 2, q.scene);
                     return true;
                 }
             }
@@ -185,7 +190,7 @@ __global__ __launch_bounds__(kQT) void k_isect_edges(IsCtx c) {
                            f2 = c.sm.face[3 * (size_t)fi + 2];
             const f3 fv[3] = {X(c.sm.x, f0), X(c.sm.x, f1), X(c.sm.x, f2)};
             if (edge_triangle(y[0], y[1], fv[0], fv[1], fv[2])) {
-                record(c, 2, fi, ei, fv, 3, y, 2);
+                record(c, 2, fi, ei, fv, 3, y, 2, q.scene);
                 return true;
             }
             return false;
@@ -207,7 +212,8 @@ __global__ __launch_bounds__(kQT) void k_isect_verts(IsCtx c) {
     }
     uint32_t *stk = stk_all + threadIdx.x;
     const f3 p = X(m.x, i);
-    const QBox q = point_box(p, m.v_off[i]);
+    QBox q = point_box(p, m.v_off[i]);
+    q.scene = vertex_scene(m.vscene, m.nscene, i);
     traverse(c.vb, q, stk, c.ctr + 1, [&](uint32_t j) {
         if (!(j < i)) {
             return false;
@@ -223,7 +229,7 @@ __global__ __launch_bounds__(kQT) void k_isect_verts(IsCtx c) {
         const f3 qj = X(m.x, j);
         const f3 e = p - qj;
         if (dot3(e, e) < off * off) {
-            record(c, 3, i, j, &p, 1, &qj, 1);
+            record(c, 3, i, j, &p, 1, &qj, 1, q.scene);
             return true;
         }
         return false;
@@ -244,10 +250,21 @@ int IntersectionCheck::run(const BvhView &face, const BvhView &edge, const BvhVi
                            const ppf_contact_mesh &m, const ppf_static_mesh *sm,
                            const float *vmass, const float *emass, const float *fmass,
                            uint32_t surface_vert_count, ppf_isect_record *records,
-                           uint32_t cap, uint32_t *count, hipStream_t s) {
-    if (!ctr_) {
-        PPF_HIP(dev_alloc(&ctr_, 2 * sizeof(uint32_t)));
-        PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_ctr_), 2 * sizeof(uint32_t)));
+                           uint32_t cap, uint32_t *count, hipStream_t s,
+                           uint32_t *scene_count) {
+    const uint32_t ns = (m.vscene && m.nscene > 1) ? m.nscene : 0u;
+    if (2 + ns > ctr_cap_) {
+        if (ctr_) {
+            PPF_HIP(dev_free(ctr_));
+            ctr_ = nullptr;
+        }
+        if (h_ctr_) {
+            PPF_HIP(host_free(h_ctr_));
+            h_ctr_ = nullptr;
+        }
+        PPF_HIP(dev_alloc(&ctr_, (2 + ns) * sizeof(uint32_t)));
+        PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_ctr_), (2 + ns) * sizeof(uint32_t)));
+        ctr_cap_ = 2 + ns;
     }
     const uint32_t need = cap < 256u ? 256u : cap;
     if (need > rec_cap_) {
@@ -263,6 +280,8 @@ int IntersectionCheck::run(const BvhView &face, const BvhView &edge, const BvhVi
     if (S.nv && (cm_x_ != S.x || cm_nf_ != S.nf || cm_face_.view().n != S.nf)) {
         PrimSet sf;
         sf.n = S.nf, sf.arity = 3, sf.elem = S.face, sf.ghat = S.f_ghat, sf.offset = S.f_off;
+        sf.vscene = S.vscene;
+        sf.nscene = m.nscene;
         int rc = cm_face_.build(sf, S.x, nullptr, 0.0f, s);
         if (rc != PPF_OK) {
             return rc;
@@ -270,7 +289,7 @@ int IntersectionCheck::run(const BvhView &face, const BvhView &edge, const BvhVi
         cm_x_ = S.x;
         cm_nf_ = S.nf;
     }
-    PPF_HIP(hipMemsetAsync(ctr_, 0, 2 * sizeof(uint32_t), s));
+    PPF_HIP(hipMemsetAsync(ctr_, 0, (2 + ns) * sizeof(uint32_t), s));
     IsCtx c;
     c.m = m;
     c.sm = S;
@@ -285,6 +304,7 @@ int IntersectionCheck::run(const BvhView &face, const BvhView &edge, const BvhVi
     c.rec = rec_;
     c.cap = rec_cap_;
     c.ctr = ctr_;
+    c.ns = ns;
     if (m.ne) {
         k_isect_edges<<<xcd_grid(cdiv(m.ne, kQT)), kQT, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
@@ -293,13 +313,17 @@ int IntersectionCheck::run(const BvhView &face, const BvhView &edge, const BvhVi
         k_isect_verts<<<xcd_grid(cdiv(m.nv, kQT)), kQT, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
     }
-    PPF_HIP(hipMemcpyAsync(h_ctr_, ctr_, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipMemcpyAsync(h_ctr_, ctr_, (2 + ns) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           s));
     PPF_HIP(hipStreamSynchronize(s));
     if (h_ctr_[1]) {
         set_error("intersection check: a BVH traversal exceeded its %d-entry stack", kStack);
         return PPF_ERR_CAPACITY;
     }
     *count = h_ctr_[0];
+    for (uint32_t k = 0; scene_count && k < ns; ++k) {
+        scene_count[k] = h_ctr_[2 + k];
+    }
     const uint32_t keep = h_ctr_[0] < cap ? h_ctr_[0] : cap;
     if (records && keep) {
         PPF_HIP(hipMemcpy(records, rec_, (size_t)keep * sizeof(ppf_isect_record),

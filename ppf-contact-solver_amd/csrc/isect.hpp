@@ -13,7 +13,8 @@ class IntersectionCheck {
     int run(const BvhView &face, const BvhView &edge, const BvhView &vert,
             const ppf_contact_mesh &m, const ppf_static_mesh *sm, const float *vmass,
             const float *emass, const float *fmass, uint32_t surface_vert_count,
-            ppf_isect_record *records, uint32_t cap, uint32_t *count, hipStream_t s);
+            ppf_isect_record *records, uint32_t cap, uint32_t *count, hipStream_t s,
+            uint32_t *scene_count = nullptr);
 
   private:
     Lbvh cm_face_;
@@ -22,6 +23,7 @@ class IntersectionCheck {
     ppf_isect_record *rec_ = nullptr;
     uint32_t rec_cap_ = 0;
     uint32_t *ctr_ = nullptr, *h_ctr_ = nullptr;
+    uint32_t ctr_cap_ = 0;
 };
 
 } // namespace ppf

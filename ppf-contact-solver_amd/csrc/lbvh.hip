@@ -129,19 +129,133 @@ __global__ __launch_bounds__(kNT) void k_morton(uint32_t n, const float *__restr
     idx[i] = i;
 }
 
-__device__ inline int lcp(const uint32_t *__restrict__ k, int n, int i, int j) {
+__device__ inline uint32_t prim_scene(const PrimSet &p, uint32_t prim) {
+    return (p.vscene && p.nscene > 1) ? p.vscene[prim_vertex(p, prim, 0)] : 0u;
+}
+
+// Float <-> unsigned with the same order (negative floats reversed), so that
+// integer atomics take exact float minima and maxima.
+__device__ inline uint32_t ord_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float ord_float(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// Per-scene centroid bounds (6 ordered words per scene: min xyz, max xyz),
+// one atomic per wave and component where a wave's primitives share a scene.
+__global__ __launch_bounds__(kNT) void k_scene_bounds(PrimSet p, const float *__restrict__ cent,
+                                                      uint32_t *__restrict__ sb) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    const bool ok = i < p.n;
+    const uint32_t c = ok ? prim_scene(p, i) : 0u;
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    if (ok) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = hi[d] = cent[3 * (size_t)i + d];
+        }
+    }
+    const uint32_t c0 = __shfl(c, 0);
+    if (__ballot(ok && c != c0) == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            for (int o = 32; o > 0; o >>= 1) {
+                lo[d] = fminf(lo[d], __shfl_xor(lo[d], o));
+                hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], o));
+            }
+        }
+        if ((threadIdx.x & 63) == 0 && __ballot(ok) != 0) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(sb + 6 * c0 + d, ord_bits(lo[d]));
+                atomicMax(sb + 6 * c0 + 3 + d, ord_bits(hi[d]));
+            }
+        }
+    } else if (ok) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(sb + 6 * c + d, ord_bits(lo[d]));
+            atomicMax(sb + 6 * c + 3 + d, ord_bits(hi[d]));
+        }
+    }
+}
+
+// Morton codes quantized in each primitive's own scene box.
+__global__ __launch_bounds__(kNT) void k_morton_scenes(PrimSet p, const float *__restrict__ cent,
+                                                       const uint32_t *__restrict__ sb,
+                                                       uint32_t *__restrict__ code,
+                                                       uint32_t *__restrict__ idx) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= p.n) {
+        return;
+    }
+    const uint32_t c = prim_scene(p, i);
+    uint32_t q[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float lo = ord_float(sb[6 * c + d]), hi = ord_float(sb[6 * c + 3 + d]);
+        const float sc = (hi - lo) > 1e-10f ? 1023.0f / (hi - lo) : 0.0f;
+        q[d] = (uint32_t)fminf(1023.0f, fmaxf(0.0f, (cent[3 * (size_t)i + d] - lo) * sc));
+    }
+    code[i] = expand_bits(q[0]) | (expand_bits(q[1]) << 1) | (expand_bits(q[2]) << 2);
+    idx[i] = i;
+}
+
+// Empty scene boxes (min at the top of the order, max at the bottom).
+__global__ void k_scene_bounds_init(uint32_t ns, uint32_t *__restrict__ sb) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < 6 * ns) {
+        sb[k] = (k % 6) < 3 ? 0xffffffffu : 0u;
+    }
+}
+
+// Scene of each sorted leaf (the key of the second, stable sort pass).
+__global__ __launch_bounds__(kNT) void k_scene_keys(PrimSet p, const uint32_t *__restrict__ idx,
+                                                    uint32_t *__restrict__ key) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i < p.n) {
+        key[i] = prim_scene(p, idx[i]);
+    }
+}
+
+// Morton codes in the final leaf order (from the per-primitive copy).
+__global__ __launch_bounds__(kNT) void k_regather(uint32_t n, const uint32_t *__restrict__ idx,
+                                                  const uint32_t *__restrict__ byprim,
+                                                  uint32_t *__restrict__ code) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i < n) {
+        code[i] = byprim[idx[i]];
+    }
+}
+
+// Common-prefix length of the keys of sorted leaves i and j: the Morton code
+// with the index as tie-break (lbvh.cu:289-303), preceded by the scene id
+// when the tree holds many scenes (sk: scene per sorted leaf, or null).
+__device__ inline int lcp(const uint32_t *__restrict__ k, const uint32_t *__restrict__ sk, int n,
+                          int i, int j) {
     if (j < 0 || j >= n) {
         return -1;
     }
+    int base = 0;
+    if (sk) {
+        const uint32_t sa = sk[i], sb = sk[j];
+        if (sa != sb) {
+            return __clz(sa ^ sb);
+        }
+        base = 32;
+    }
     const uint32_t a = k[i], b = k[j];
     if (a == b) {
-        return 32 + __clz((uint32_t)(i ^ j));
+        return base + 32 + __clz((uint32_t)(i ^ j));
     }
-    return __clz(a ^ b);
+    return base + __clz(a ^ b);
 }
 
 // Karras radix tree over the sorted codes (lbvh.cu:305-385), parents inline.
 __global__ __launch_bounds__(kNT) void k_karras(uint32_t n, const uint32_t *__restrict__ k,
+                                                const uint32_t *__restrict__ sk,
                                                 const uint32_t *__restrict__ idx,
                                                 uint32_t *__restrict__ node,
                                                 uint32_t *__restrict__ parent) {
@@ -157,25 +271,25 @@ __global__ __launch_bounds__(kNT) void k_karras(uint32_t n, const uint32_t *__re
     if (i >= nn - 1) {
         return;
     }
-    const int d = lcp(k, nn, i, i + 1) > lcp(k, nn, i, i - 1) ? 1 : -1;
-    const int lmin = lcp(k, nn, i, i - d);
+    const int d = lcp(k, sk, nn, i, i + 1) > lcp(k, sk, nn, i, i - 1) ? 1 : -1;
+    const int lmin = lcp(k, sk, nn, i, i - d);
     int lmax = 2;
-    while (lcp(k, nn, i, i + lmax * d) > lmin) {
+    while (lcp(k, sk, nn, i, i + lmax * d) > lmin) {
         lmax *= 2;
     }
     int len = 0;
     for (int t = lmax / 2; t >= 1; t /= 2) {
-        if (lcp(k, nn, i, i + (len + t) * d) > lmin) {
+        if (lcp(k, sk, nn, i, i + (len + t) * d) > lmin) {
             len += t;
         }
     }
     const int j = i + len * d;
     const int l = min(i, j), r = max(i, j);
-    const int nodelcp = lcp(k, nn, l, r);
+    const int nodelcp = lcp(k, sk, nn, l, r);
     int s = 0, t = r - l;
     do {
         t = (t + 1) / 2;
-        if (lcp(k, nn, l, l + s + t) > nodelcp) {
+        if (lcp(k, sk, nn, l, l + s + t) > nodelcp) {
             s += t;
         }
     } while (t > 1);
@@ -225,7 +339,7 @@ __global__ __launch_bounds__(kNT) void k_leaf_boxes(PrimSet p, const float *__re
         bx.mx[d] = fmaxf(hi0[d], hi1[d]) + margin;
     }
     bx.active = 1.0f;
-    bx.pad = 0.0f;
+    bx.pad = __uint_as_float(prim_scene(p, prim));
     box[i] = bx;
 }
 
@@ -330,7 +444,8 @@ __global__ __launch_bounds__(kNT) void k_refit_level(const uint32_t *__restrict_
         u.mx[d] = fmaxf(a.mx[d], b.mx[d]);
     }
     u.active = (a.active != 0.0f && b.active != 0.0f) ? 1.0f : 0.0f;
-    u.pad = 0.0f;
+    const uint32_t ta = __float_as_uint(a.pad), tb = __float_as_uint(b.pad);
+    u.pad = __uint_as_float(ta == tb ? ta : kMixedScene);
     box[par] = u;
 }
 
@@ -344,7 +459,7 @@ void Lbvh::release() {
     for (void *p : {(void *)node_, (void *)parent_, (void *)depth_, (void *)level_,
                     (void *)code_,
                     (void *)idx_, (void *)scratch_, (void *)cent_, (void *)part_,
-                    (void *)box_}) {
+                    (void *)box_, (void *)sb_}) {
         if (p) {
             (void)dev_free(p);
         }
@@ -352,6 +467,8 @@ void Lbvh::release() {
     node_ = parent_ = depth_ = level_ = code_ = idx_ = scratch_ = nullptr;
     cent_ = part_ = nullptr;
     box_ = nullptr;
+    sb_ = nullptr;
+    sb_cap_ = 0;
     cap_ = 0;
 }
 
@@ -393,10 +510,45 @@ int Lbvh::build(const PrimSet &p, const float *x0, const float *x1, float t,
     PPF_HIP(hipGetLastError());
     k_bounds<<<1, kNT, 0, s>>>(nb, part_, bounds);
     PPF_HIP(hipGetLastError());
-    k_morton<<<nb, kNT, 0, s>>>(n_, cent_, bounds, code_, idx_);
+    const bool scenes = p.vscene && p.nscene > 1;
+    if (scenes) {
+        // each scene quantized in its own box: a scene's tree does not depend
+        // on the other scenes of the mesh
+        if (6 * (size_t)p.nscene > sb_cap_) {
+            if (sb_) {
+                PPF_HIP(dev_free(sb_));
+                sb_ = nullptr;
+            }
+            PPF_HIP(dev_alloc(&sb_, 6 * (size_t)p.nscene * sizeof(uint32_t)));
+            sb_cap_ = 6 * (size_t)p.nscene;
+        }
+        k_scene_bounds_init<<<cdiv(6 * p.nscene, kNT), kNT, 0, s>>>(p.nscene, sb_);
+        k_scene_bounds<<<nb, kNT, 0, s>>>(p, cent_, sb_);
+        k_morton_scenes<<<nb, kNT, 0, s>>>(p, cent_, sb_, code_, idx_);
+    } else {
+        k_morton<<<nb, kNT, 0, s>>>(n_, cent_, bounds, code_, idx_);
+    }
     PPF_HIP(hipGetLastError());
+    if (scenes) { // per-primitive codes, kept for the regather after the scene pass
+        PPF_HIP(hipMemcpyAsync(level_, code_, n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                               s));
+    }
     PPF_HIP(radix_sort_pairs(code_, idx_, n_, 30, scratch_, s));
-    k_karras<<<nb, kNT, 0, s>>>(n_, code_, idx_, node_, parent_);
+    const uint32_t *sk = nullptr;
+    if (scenes) {
+        // stable second pass by scene: leaves ordered by (scene, code, index)
+        uint32_t bits = 1;
+        while (bits < 32 && ((p.nscene - 1) >> bits) != 0) {
+            ++bits;
+        }
+        k_scene_keys<<<nb, kNT, 0, s>>>(p, idx_, depth_);
+        PPF_HIP(hipGetLastError());
+        PPF_HIP(radix_sort_pairs(depth_, idx_, n_, bits, scratch_, s));
+        k_regather<<<nb, kNT, 0, s>>>(n_, idx_, level_, code_);
+        PPF_HIP(hipGetLastError());
+        sk = depth_;
+    }
+    k_karras<<<nb, kNT, 0, s>>>(n_, code_, sk, idx_, node_, parent_);
     PPF_HIP(hipGetLastError());
     rc = levels(s);
     if (rc != PPF_OK) {

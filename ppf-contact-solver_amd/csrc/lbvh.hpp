@@ -13,13 +13,17 @@
 
 namespace ppf {
 
-// Box of one node: min xyz, active flag, max xyz, pad (32 B, two 16 B loads).
+// Box of one node: min xyz, active flag, max xyz, scene tag (32 B, two 16 B
+// loads). The tag holds the bits of the scene id every leaf below the node
+// belongs to, or kMixedScene when they belong to several (0 throughout for a
+// one-scene tree).
 struct alignas(32) Aabb {
     float mn[3];
     float active;
     float mx[3];
     float pad;
 };
+constexpr uint32_t kMixedScene = 0xffffffffu;
 
 struct BvhView {
     uint32_t n = 0;            // leaves
@@ -37,6 +41,12 @@ struct PrimSet {
     const uint32_t *elem = nullptr;  // n * arity vertex indices (null: identity)
     const float *ghat = nullptr;     // per primitive
     const float *offset = nullptr;   // per primitive
+    // Many scenes: scene id per vertex (a primitive's scene is its first
+    // vertex's) and the scene count. The tree then orders its leaves by
+    // (scene, Morton code), so each scene is one subtree, and tags every node
+    // with its scene. Null / nscene <= 1: one scene.
+    const uint32_t *vscene = nullptr;
+    uint32_t nscene = 0;
 };
 
 class Lbvh {
@@ -61,6 +71,8 @@ class Lbvh {
     uint32_t *code_ = nullptr, *idx_ = nullptr, *scratch_ = nullptr;
     float *cent_ = nullptr, *part_ = nullptr;
     Aabb *box_ = nullptr;
+    uint32_t *sb_ = nullptr; // per-scene centroid bounds (many scenes)
+    size_t sb_cap_ = 0;
     void release();
     int levels(hipStream_t s);
 };

@@ -13,6 +13,7 @@
 #include "ccd.hpp"
 #include "linesearch.hpp"
 #include <type_traits>
+#include <vector>
 
 namespace ppf {
 
@@ -32,7 +33,12 @@ struct LsCtx {
     ppf_static_mesh sm;
     ppf_ls_params lp;
     BvhView fb, eb, vb, sfb, seb;
-    uint32_t *out; // [0] t_vf bits, [1] t_ee bits, [2] overlap, [3] stack overflow
+    uint32_t *out; // [0] t_vf bits, [1] t_ee bits, [2] overlap, [3] stack overflow,
+                   // [4] smallest pinned vertex driven into an analytic collider
+    // many scenes: per scene t_vf bits [0, ns), t_ee bits [ns, 2 ns), overlap
+    // [2 ns, 3 ns); null for one scene
+    uint32_t *sout;
+    uint32_t ns;
 };
 
 __device__ inline f3 X(const float *a, uint32_t v) { return load3(a + 3 * (size_t)v); }
@@ -43,12 +49,21 @@ __device__ inline f3 XS(const LsCtx &c, uint32_t v) {
     return mk3(t * (b.x - a.x) + a.x, t * (b.y - a.y) + a.y, t * (b.z - a.z) + a.z);
 }
 
-__device__ inline void note(const LsCtx &c, bool ovl, float r, float &best) {
+// One sweep's result; `scene` is the query's scene and `ee` selects the
+// edge-edge minimum (many scenes keep their minima per scene).
+__device__ inline void note(const LsCtx &c, bool ovl, float r, float &best, uint32_t scene = 0,
+                            bool ee = false) {
     if (ovl) {
         atomicOr(c.out + 2, 1u);
+        if (c.sout) {
+            atomicOr(c.sout + 2 * c.ns + scene, 1u);
+        }
     }
     if (r < c.lp.max_t) {
         best = fminf(best, r);
+        if (c.sout) {
+            atomicMin(c.sout + (ee ? c.ns : 0u) + scene, __float_as_uint(r));
+        }
     }
 }
 
@@ -80,8 +95,9 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_vertex(LsCtx c) {
     const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
     const uint32_t i = active ? c.vb.node[2 * (size_t)leaf] - 1 : 0;
     const uint32_t vf = active ? m.vflag[i] : 4u;
-    const Box6 q = (vf & 4u) ? empty_box()
-                             : seg_box(X(m.x, i), XS(c, i), 0.5f * m.v_ghat[i] + m.v_off[i]);
+    Box6 q = (vf & 4u) ? empty_box()
+                       : seg_box(X(m.x, i), XS(c, i), 0.5f * m.v_ghat[i] + m.v_off[i]);
+    q.scene = vertex_scene(m.vscene, m.nscene, i);
     const bool idyn = (vf & 1u) == 0, icol = (vf & 2u) != 0;
     const uint32_t ibody = active ? m.vbody[i] : 0u;
     float best = max_t;
@@ -118,7 +134,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_vertex(LsCtx c) {
                 const f3 b[3] = {X(c.lp.x1, p), X(c.lp.x1, j), X(c.lp.x1, k)};
                 r = ccd<3, 1>(a, b, 2.0f * m.v_off[v], m.v_ghat[v], max_t, eps, ovl);
             }
-            note(c, ovl, r, best);
+            note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, v));
         });
     traverse_wave(
         c.vb, q, active, i, ws, c.out + 3,
@@ -134,7 +150,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_vertex(LsCtx c) {
             bool ovl = false;
             const float r = ccd<2, 1>(a, b, m.v_off[v] + m.v_off[j],
                                       0.5f * (m.v_ghat[v] + m.v_ghat[j]), max_t, eps, ovl);
-            note(c, ovl, r, best);
+            note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, v));
         });
     if (c.sm.nf) {
         const bool go = active && idyn && c.lp.vmass[i] != 0.0f;
@@ -150,7 +166,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_vertex(LsCtx c) {
                 const float r = ccd<4, 1>(a, b, m.v_off[v] + c.sm.f_off[fi],
                                           0.5f * (m.v_ghat[v] + c.sm.f_ghat[fi]), max_t, eps,
                                           ovl);
-                note(c, ovl, r, best);
+                note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, v));
             });
     }
     wave_min_out(c.out + 0, best, max_t);
@@ -167,7 +183,8 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_static_vertex(LsCtx c) {
     const uint32_t ii = active ? i : 0;
     const f3 p = X(c.sm.x, ii);
     const float qm = 0.5f * c.sm.v_ghat[ii] + c.sm.v_off[ii];
-    const Box6 q{{p.x - qm, p.y - qm, p.z - qm}, {p.x + qm, p.y + qm, p.z + qm}};
+    const Box6 q{{p.x - qm, p.y - qm, p.z - qm}, {p.x + qm, p.y + qm, p.z + qm},
+                 vertex_scene(c.sm.vscene, m.nscene, ii)};
     float best = max_t;
     traverse_wave(
         c.fb, q, active, ii, ws, c.out + 3,
@@ -182,9 +199,87 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_static_vertex(LsCtx c) {
             const float r = ccd<4, 1>(a, b, m.f_off[fi] + c.sm.v_off[v],
                                       0.5f * (m.f_ghat[fi] + c.sm.v_ghat[v]), max_t, eps,
                                       ovl);
-            note(c, ovl, r, best);
+            note(c, ovl, r, best, vertex_scene(c.sm.vscene, m.nscene, v));
         });
     wave_min_out(c.out + 0, best, max_t);
+}
+
+// Analytic spheres and floors per vertex (vertex_constraint_line_search,
+// contact.cu:2318-2470): the swept step x0 -> x0 + max_t (x1 - x0) against
+// each static collider; the entry time (times max_t) joins the vertex-face
+// minimum. A pinned vertex driven from outside into a static collider cannot
+// yield: the smallest such vertex index is flagged. Kinematic colliders are
+// held by the barrier alone and not swept.
+__global__ __launch_bounds__(256) void k_ls_analytic(LsCtx c) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const ppf_contact_mesh &m = c.m;
+    const float max_t = c.lp.max_t;
+    float toi = max_t;
+    if (i < m.nv) {
+        const f3 y0 = X(m.x, i), y1 = X(c.lp.x1, i);
+        const f3 x0 = y0;
+        const f3 x1 = max_t * (y1 - y0) + y0;
+        const bool pinned = (m.vflag[i] & 1u) != 0;
+        if (pinned || c.lp.vmass[i] > 0.0f) {
+            for (uint32_t j = 0; j < c.lp.nsphere; ++j) {
+                const ppf_sphere sp = c.lp.spheres[j];
+                if (sp.kinematic) {
+                    continue;
+                }
+                const f3 cc = mk3(sp.center[0], sp.center[1], sp.center[2]);
+                const f3 c0 = (sp.bowl && x0.y > cc.y) ? mk3(cc.x, x0.y, cc.z) : cc;
+                const f3 c1 = (sp.bowl && x1.y > cc.y) ? mk3(cc.x, x1.y, cc.z) : cc;
+                const float r = sp.radius;
+                const f3 a0 = x0 - c0, a1 = x1 - c1;
+                const float r0 = sqrtf(dot3(a0, a0)), r1 = sqrtf(dot3(a1, a1));
+                if (pinned) {
+                    const float d0 = sp.reverse ? (r - r0) : (r0 - r);
+                    const float d1 = sp.reverse ? (r - r1) : (r1 - r);
+                    if (d0 >= 0.0f && d1 < 0.0f) {
+                        atomicMin(c.out + 4, i);
+                    }
+                    continue;
+                }
+                const float depth0 = sp.reverse ? (r0 - r) : (r - r0);
+                if (sp.thickness > 0.0f && depth0 > sp.thickness) {
+                    continue;
+                }
+                if ((r0 - r) * (r1 - r) <= 0.0f) {
+                    const float t = (r - r0) / (r1 - r0);
+                    toi = fminf(toi, max_t * t);
+                }
+            }
+            for (uint32_t j = 0; j < c.lp.nfloor; ++j) {
+                const ppf_floor fl = c.lp.floors[j];
+                if (fl.kinematic) {
+                    continue;
+                }
+                const f3 up = mk3(fl.up[0], fl.up[1], fl.up[2]);
+                const f3 g = mk3(fl.ground[0], fl.ground[1], fl.ground[2]);
+                const float h0 = dot3(up, x0 - g), h1 = dot3(up, x1 - g);
+                if (pinned) {
+                    if (h0 >= 0.0f && h1 < 0.0f) {
+                        atomicMin(c.out + 4, i);
+                    }
+                    continue;
+                }
+                if (fl.thickness > 0.0f && -h0 > fl.thickness) {
+                    continue;
+                }
+                if (h1 < 0.0f) {
+                    const float t = -h0 / (h1 - h0);
+                    toi = fminf(toi, max_t * t);
+                }
+            }
+        }
+    }
+    float v = toi;
+    for (int o = 32; o > 0; o >>= 1) {
+        v = fminf(v, __shfl_xor(v, o));
+    }
+    if ((threadIdx.x & 63) == 0 && v < max_t) {
+        atomicMin(c.out + 0, __float_as_uint(v));
+    }
 }
 
 // edge-edge and edge vs static edge, seeded by the vertex-face minimum.
@@ -195,9 +290,11 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_edge(LsCtx c) {
     const bool active = leaf < c.eb.n;
     const ppf_contact_mesh &m = c.m;
     const float max_t = c.lp.max_t, eps = c.lp.ccd_eps;
-    const float seed = __uint_as_float(c.out[0]);
     const uint32_t ei = active ? c.eb.node[2 * (size_t)leaf] - 1 : 0;
     const uint32_t a0 = m.edge[2 * (size_t)ei], a1 = m.edge[2 * (size_t)ei + 1];
+    // the vertex-face minimum of the edge's own scene bounds its sweep box
+    const float seed = __uint_as_float(
+        c.sout ? c.sout[vertex_scene(m.vscene, m.nscene, a0)] : c.out[0]);
     const float ext = 0.5f * m.e_ghat[ei] + m.e_off[ei];
     const f3 p0 = X(m.x, a0), p1 = X(m.x, a1), q0 = X(c.lp.x1, a0), q1 = X(c.lp.x1, a1);
     const f3 s0 = mk3(seed * (q0.x - p0.x) + p0.x, seed * (q0.y - p0.y) + p0.y,
@@ -215,6 +312,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_edge(LsCtx c) {
     if (ef & 4u) {
         q = empty_box();
     }
+    q.scene = vertex_scene(m.vscene, m.nscene, a0);
     const bool edyn = (ef & 1u) == 0;
     const bool acol = (m.vflag[a0] & 2u) != 0;
     const uint32_t abody = m.vbody[a0];
@@ -259,7 +357,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_edge(LsCtx c) {
                     }
                 }
             }
-            note(c, ovl, r, best);
+            note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, e0), true);
         });
     if (c.sm.ne) {
         const bool go = active && edyn && c.lp.emass[ei] > 0.0f;
@@ -276,7 +374,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_edge(LsCtx c) {
                 const float r = ccd<4, 2>(a, b, m.e_off[e] + c.sm.e_off[sj],
                                           0.5f * (m.e_ghat[e] + c.sm.e_ghat[sj]), max_t,
                                           eps, ovl);
-                note(c, ovl, r, best);
+                note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, e0), true);
             });
     }
     wave_min_out(c.out + 1, best, seed);
@@ -343,10 +441,21 @@ LineSearch::~LineSearch() {
 
 int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
                     const ppf_ls_params &lp, bool rebuild, float *toi, int *overlap,
-                    hipStream_t s) {
-    if (!out_) {
-        PPF_HIP(dev_alloc(&out_, 4 * sizeof(uint32_t)));
-        PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_out_), 4 * sizeof(uint32_t)));
+                    hipStream_t s, float *scene_toi, int *scene_overlap, uint32_t *pin_bad) {
+    const uint32_t ns = (m.vscene && m.nscene > 1) ? m.nscene : 0u;
+    const uint32_t words = 5 + 3 * ns;
+    if (words > out_cap_) {
+        if (out_) {
+            PPF_HIP(dev_free(out_));
+            out_ = nullptr;
+        }
+        if (h_out_) {
+            PPF_HIP(host_free(h_out_));
+            h_out_ = nullptr;
+        }
+        PPF_HIP(dev_alloc(&out_, words * sizeof(uint32_t)));
+        PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_out_), words * sizeof(uint32_t)));
+        out_cap_ = words;
     }
     ppf_static_mesh none{};
     const ppf_static_mesh &S = sm ? *sm : none;
@@ -354,6 +463,8 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
     f.n = m.nf, f.arity = 3, f.elem = m.face, f.ghat = m.f_ghat, f.offset = m.f_off;
     e.n = m.ne, e.arity = 2, e.elem = m.edge, e.ghat = m.e_ghat, e.offset = m.e_off;
     v.n = m.nv, v.arity = 1, v.elem = nullptr, v.ghat = m.v_ghat, v.offset = m.v_off;
+    f.vscene = e.vscene = v.vscene = m.vscene;
+    f.nscene = e.nscene = v.nscene = m.nscene;
     const bool fresh = rebuild || face_.view().n != m.nf || edge_.view().n != m.ne ||
                        vert_.view().n != m.nv;
     int rc;
@@ -374,6 +485,8 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
         PrimSet sf, se;
         sf.n = S.nf, sf.arity = 3, sf.elem = S.face, sf.ghat = S.f_ghat, sf.offset = S.f_off;
         se.n = S.ne, se.arity = 2, se.elem = S.edge, se.ghat = S.e_ghat, se.offset = S.e_off;
+        sf.vscene = se.vscene = S.vscene;
+        sf.nscene = se.nscene = m.nscene;
         rc = sface_.build(sf, S.x, nullptr, 0.0f, s);
         if (rc == PPF_OK) rc = sedge_.build(se, S.x, nullptr, 0.0f, s);
         if (rc != PPF_OK) {
@@ -381,9 +494,14 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
         }
         static_x_ = S.x;
     }
-    const uint32_t init[4] = {__builtin_bit_cast(uint32_t, lp.max_t),
-                              __builtin_bit_cast(uint32_t, lp.max_t), 0u, 0u};
-    PPF_HIP(hipMemcpyAsync(out_, init, sizeof(init), hipMemcpyHostToDevice, s));
+    // minima start at max_t (per scene too), flags at zero
+    const uint32_t tmax = __builtin_bit_cast(uint32_t, lp.max_t);
+    std::vector<uint32_t> init(words, 0u);
+    for (uint32_t k = 0; k < words; ++k) {
+        init[k] = (k < 2 || (k >= 5 && k < 5 + 2 * ns)) ? tmax : 0u;
+    }
+    init[4] = 0xffffffffu;
+    PPF_HIP(hipMemcpyAsync(out_, init.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     LsCtx c;
     c.m = m;
     c.sm = S;
@@ -394,6 +512,8 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
     c.sfb = S.nv ? sface_.view() : BvhView{};
     c.seb = S.nv ? sedge_.view() : BvhView{};
     c.out = out_;
+    c.sout = ns ? out_ + 5 : nullptr;
+    c.ns = ns;
     auto sweeps = [&](auto stack) -> int {
         constexpr int D = decltype(stack)::value;
         if (m.nv) {
@@ -404,18 +524,24 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
             k_ls_static_vertex<D><<<xcd_grid(cdiv(S.nv, kWQ)), kWQ, 0, s>>>(c);
             PPF_HIP(hipGetLastError());
         }
+        if (m.nv && (lp.nsphere || lp.nfloor)) { // before the edges: it seeds them
+            k_ls_analytic<<<cdiv(m.nv, 256), 256, 0, s>>>(c);
+            PPF_HIP(hipGetLastError());
+        }
         if (m.ne) {
             k_ls_edge<D><<<xcd_grid(cdiv(m.ne, kWQ)), kWQ, 0, s>>>(c);
             PPF_HIP(hipGetLastError());
         }
-        PPF_HIP(hipMemcpyAsync(h_out_, out_, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        PPF_HIP(hipMemcpyAsync(h_out_, out_, words * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               s));
         PPF_HIP(hipStreamSynchronize(s));
         return PPF_OK;
     };
     rc = sweeps(std::integral_constant<int, kShortStack>{});
     if (rc == PPF_OK && h_out_[3]) {
         // a walk overflowed the short stacks: every sweep again from the start
-        PPF_HIP(hipMemcpyAsync(out_, init, sizeof(init), hipMemcpyHostToDevice, s));
+        PPF_HIP(hipMemcpyAsync(out_, init.data(), words * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, s));
         rc = sweeps(std::integral_constant<int, kStack>{});
     }
     if (rc != PPF_OK) {
@@ -429,6 +555,15 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
     const float tee = __builtin_bit_cast(float, h_out_[1]);
     *toi = fminf(tvf, tee) / lp.max_t;
     *overlap = h_out_[2] ? 1 : 0;
+    for (uint32_t c = 0; c < ns; ++c) {
+        const float svf = __builtin_bit_cast(float, h_out_[5 + c]);
+        const float see = __builtin_bit_cast(float, h_out_[5 + ns + c]);
+        if (scene_toi) scene_toi[c] = fminf(svf, see) / lp.max_t;
+        if (scene_overlap) scene_overlap[c] = h_out_[5 + 2 * ns + c] ? 1 : 0;
+    }
+    if (pin_bad) {
+        *pin_bad = h_out_[4];
+    }
     return PPF_OK;
 }
 

@@ -10,13 +10,17 @@ namespace ppf {
 class LineSearch {
   public:
     ~LineSearch();
+    // A mesh of many scenes also returns every scene's toi and overlap flag
+    // (host arrays of m.nscene; the pair sweeps never cross scenes).
     int run(const ppf_contact_mesh &m, const ppf_static_mesh *sm, const ppf_ls_params &p,
-            bool rebuild, float *toi, int *overlap, hipStream_t s);
+            bool rebuild, float *toi, int *overlap, hipStream_t s, float *scene_toi = nullptr,
+            int *scene_overlap = nullptr, uint32_t *pin_bad = nullptr);
 
   private:
     Lbvh face_, edge_, vert_, sface_, sedge_;
     const float *static_x_ = nullptr;
     uint32_t *out_ = nullptr, *h_out_ = nullptr;
+    uint32_t out_cap_ = 0;
 };
 
 // Batched single-pair ACCD (kind 0 point-triangle, 1 point-edge,

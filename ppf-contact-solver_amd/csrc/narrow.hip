@@ -98,8 +98,9 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
     const uint32_t i = c.vb.node[2 * (size_t)leaf_pos] - 1;
     const f3 p = X(m, i);
     const float qm = 0.5f * m.v_ghat[i] + m.v_off[i];
-    const Q q = (m.vflag[i] & 4u) ? empty_box()
-                                  : Q{{p.x - qm, p.y - qm, p.z - qm}, {p.x + qm, p.y + qm, p.z + qm}};
+    Q q = (m.vflag[i] & 4u) ? empty_box()
+                            : Q{{p.x - qm, p.y - qm, p.z - qm}, {p.x + qm, p.y + qm, p.z + qm}};
+    q.scene = vertex_scene(m.vscene, m.nscene, i);
     uint64_t slot = FILL ? (uint64_t)c.ofs[leaf_pos] : 0;
     uint32_t total = 0;
     // point-face (contact.cu:717-785)
@@ -264,6 +265,7 @@ __global__ __launch_bounds__(kQT) void k_edge_queries(Ctx c) {
     if (m.eflag[ei] & 4u) {
         q = empty_box();
     }
+    q.scene = vertex_scene(m.vscene, m.nscene, a0);
     const uint32_t qpos = c.vb.n + leaf_pos;
     uint64_t slot = FILL ? (uint64_t)c.ofs[qpos] : 0;
     const uint32_t total = traverse(c.eb, q, stk, c.overflow, [&](uint32_t ej) {
@@ -388,7 +390,8 @@ template <bool FILL> __global__ __launch_bounds__(kQT) void k_m2c(SCtx c) {
     uint32_t total = 0;
     if ((m.vflag[i] & 5u) == 0 && c.vmass[i] > 0.0f) {
         const f3 p = X(m, i);
-        const QBox q = point_box(p, 0.5f * m.v_ghat[i] + m.v_off[i]);
+        QBox q = point_box(p, 0.5f * m.v_ghat[i] + m.v_off[i]);
+        q.scene = vertex_scene(m.vscene, m.nscene, i);
         total = traverse(c.sfb, q, stk, c.overflow, [&](uint32_t fi) {
             const uint32_t f0 = c.sm.face[3 * (size_t)fi], f1 = c.sm.face[3 * (size_t)fi + 1],
                            f2 = c.sm.face[3 * (size_t)fi + 2];
@@ -423,7 +426,8 @@ template <bool FILL> __global__ __launch_bounds__(kQT) void k_c2m(SCtx c) {
     const uint32_t qpos = c.vb.n + sv;
     uint64_t slot = FILL ? (uint64_t)c.ofs[qpos] : 0;
     const f3 y = SX(c.sm, sv);
-    const QBox q = point_box(y, 0.5f * c.sm.v_ghat[sv] + c.sm.v_off[sv]);
+    QBox q = point_box(y, 0.5f * c.sm.v_ghat[sv] + c.sm.v_off[sv]);
+    q.scene = vertex_scene(c.sm.vscene, m.nscene, sv);
     const uint32_t total = traverse(c.fb, q, stk, c.overflow, [&](uint32_t fi) {
         if ((m.fflag[fi] & 1u) != 0 || !(c.fmass[fi] > 0.0f)) {
             return false;
@@ -464,7 +468,8 @@ template <bool FILL> __global__ __launch_bounds__(kQT) void k_cm_ee(SCtx c) {
     if ((m.eflag[ei] & 5u) == 0 && c.emass[ei] > 0.0f) {
         const uint32_t a0 = m.edge[2 * (size_t)ei], a1 = m.edge[2 * (size_t)ei + 1];
         const f3 p0 = X(m, a0), p1 = X(m, a1);
-        const QBox q = seg_box(p0, p1, 0.5f * m.e_ghat[ei] + m.e_off[ei]);
+        QBox q = seg_box(p0, p1, 0.5f * m.e_ghat[ei] + m.e_off[ei]);
+        q.scene = vertex_scene(m.vscene, m.nscene, a0);
         total = traverse(c.seb, q, stk, c.overflow, [&](uint32_t sj) {
             const uint32_t b0 = c.sm.edge[2 * (size_t)sj], b1 = c.sm.edge[2 * (size_t)sj + 1];
             const f3 q0 = SX(c.sm, b0), q1 = SX(c.sm, b1);
@@ -532,6 +537,8 @@ int Narrow::build_trees(const ppf_contact_mesh &m, hipStream_t s) {
     f.n = m.nf, f.arity = 3, f.elem = m.face, f.ghat = m.f_ghat, f.offset = m.f_off;
     e.n = m.ne, e.arity = 2, e.elem = m.edge, e.ghat = m.e_ghat, e.offset = m.e_off;
     v.n = m.nv, v.arity = 1, v.elem = nullptr, v.ghat = m.v_ghat, v.offset = m.v_off;
+    f.vscene = e.vscene = v.vscene = m.vscene;
+    f.nscene = e.nscene = v.nscene = m.nscene;
     int rc = face_.build(f, m.x, nullptr, 0.0f, s);
     if (rc == PPF_OK) {
         rc = edge_.build(e, m.x, nullptr, 0.0f, s);
@@ -550,6 +557,8 @@ int Narrow::refit_trees(const ppf_contact_mesh &m, hipStream_t s) {
     f.n = m.nf, f.arity = 3, f.elem = m.face, f.ghat = m.f_ghat, f.offset = m.f_off;
     e.n = m.ne, e.arity = 2, e.elem = m.edge, e.ghat = m.e_ghat, e.offset = m.e_off;
     v.n = m.nv, v.arity = 1, v.elem = nullptr, v.ghat = m.v_ghat, v.offset = m.v_off;
+    f.vscene = e.vscene = v.vscene = m.vscene;
+    f.nscene = e.nscene = v.nscene = m.nscene;
     int rc = face_.refit(f, m.x, nullptr, 0.0f, s);
     if (rc == PPF_OK) {
         rc = edge_.refit(e, m.x, nullptr, 0.0f, s);
@@ -631,7 +640,7 @@ int Narrow::proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,
     return PPF_OK;
 }
 
-int Narrow::build_static(const ppf_static_mesh &sm, hipStream_t s) {
+int Narrow::build_static(const ppf_static_mesh &sm, uint32_t nscene, hipStream_t s) {
     if (sm.nv == 0) {
         return PPF_OK;
     }
@@ -642,6 +651,8 @@ int Narrow::build_static(const ppf_static_mesh &sm, hipStream_t s) {
     PrimSet f, e;
     f.n = sm.nf, f.arity = 3, f.elem = sm.face, f.ghat = sm.f_ghat, f.offset = sm.f_off;
     e.n = sm.ne, e.arity = 2, e.elem = sm.edge, e.ghat = sm.e_ghat, e.offset = sm.e_off;
+    f.vscene = e.vscene = sm.vscene;
+    f.nscene = e.nscene = nscene;
     int rc = sface_.build(f, sm.x, nullptr, 0.0f, s);
     if (rc == PPF_OK) {
         rc = sedge_.build(e, sm.x, nullptr, 0.0f, s);
@@ -660,7 +671,7 @@ int Narrow::static_proximity(const ppf_contact_mesh &m, const ppf_static_mesh &s
     if (sm.nv == 0) {
         return PPF_OK;
     }
-    int rc = build_static(sm, s);
+    int rc = build_static(sm, m.nscene, s);
     if (rc != PPF_OK) {
         return rc;
     }

@@ -27,7 +27,8 @@ class Narrow {
 
   private:
     int reserve(uint32_t queries);
-    int build_static(const ppf_static_mesh &sm, hipStream_t s);
+    // Trees of the collision mesh (nscene: the dynamic mesh's scene count).
+    int build_static(const ppf_static_mesh &sm, uint32_t nscene, hipStream_t s);
     Lbvh face_, edge_, vert_, sface_, sedge_;
     const float *static_x_ = nullptr;
     uint32_t static_nv_ = 0;

@@ -35,10 +35,21 @@ uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 __device__ inline f3 X(const float *p, uint32_t i) { return load3(p + 3 * (size_t)i); }
 
-// Inertial target (main.cu:491-511).
+// Scene of vertex i (0 for a one-scene mesh).
+__device__ inline uint32_t vsc(const uint32_t *vs, uint32_t i) { return vs ? vs[i] : 0u; }
+
+// Per-scene step scalars on the device, ns of each: dt, prev_dt, recale (the
+// max_dx clamp), toi (of the line search) and back (the pin rewind), as
+// consecutive float arrays; max |dx| bits, active flags and pair counts as
+// consecutive word arrays. A one-scene step uses the same arrays with ns = 1.
+enum { kDt = 0, kPrevDt = 1, kRecale = 2, kToi = 3, kBack = 4, kSceneFloats = 5 };
+enum { kMaxNorm = 0, kActive = 1, kPairs = 2, kSceneWords = 3 };
+
+// Inertial target (main.cu:491-511), with each scene's dt and prev_dt.
 __global__ void k_target(uint32_t n, const float *__restrict__ curr,
                          const float *__restrict__ prev, const uint32_t *__restrict__ vflag,
-                         const float *__restrict__ fix, float dt, float prev_dt, f3 g,
+                         const float *__restrict__ fix, const uint32_t *__restrict__ vs,
+                         const float *__restrict__ sf, uint32_t ns, f3 g,
                          float *__restrict__ target) {
     const uint32_t i = blockIdx.x * kNT + threadIdx.x;
     if (i >= n) {
@@ -48,6 +59,8 @@ __global__ void k_target(uint32_t n, const float *__restrict__ curr,
         store3(target + 3 * (size_t)i, X(fix, i));
         return;
     }
+    const uint32_t c = vsc(vs, i);
+    const float dt = sf[kDt * ns + c], prev_dt = sf[kPrevDt * ns + c];
     const f3 x1 = X(curr, i), x0 = X(prev, i);
     const float tr = dt / prev_dt, h2 = dt * dt;
     const f3 d = x1 - x0;
@@ -56,11 +69,17 @@ __global__ void k_target(uint32_t n, const float *__restrict__ curr,
 }
 
 // Walk pins back to the fraction of their step the loop integrated
-// (main.cu:531-545): fix -= back * delta (a static pin's delta is zero).
+// (main.cu:531-545): fix -= back * delta (a static pin's delta is zero), for
+// the scenes whose back is non-zero.
 __global__ void k_rewind(uint32_t n, const uint32_t *__restrict__ vflag, float *__restrict__ fix,
-                         const float *__restrict__ delta, float back) {
+                         const float *__restrict__ delta, const uint32_t *__restrict__ vs,
+                         const float *__restrict__ sf, uint32_t ns) {
     const uint32_t i = blockIdx.x * kNT + threadIdx.x;
     if (i >= n || !(vflag[i] & 1u)) {
+        return;
+    }
+    const float back = sf[kBack * ns + vsc(vs, i)];
+    if (back == 0.0f) {
         return;
     }
     const f3 d = X(delta, i);
@@ -251,41 +270,116 @@ __global__ void k_dirichlet_diag(uint32_t n, const uint32_t *__restrict__ mask,
     store3(force + 3 * (size_t)i, X(ex, i) - X(target, i));
 }
 
-// max_i |dx_i| as float bits (non-negative, so the integer order is the float
-// order).
-__global__ void k_max_norm(uint32_t n, const float *__restrict__ dx, uint32_t *__restrict__ out) {
+// max_i |dx_i| per scene as float bits (non-negative, so the integer order is
+// the float order). A wave whose vertices share one scene adds one atomic.
+__global__ void k_max_norm(uint32_t n, const float *__restrict__ dx,
+                           const uint32_t *__restrict__ vs, uint32_t *__restrict__ out) {
     const uint32_t i = blockIdx.x * kNT + threadIdx.x;
     float v = 0.0f;
+    uint32_t c = 0;
     if (i < n) {
         const f3 d = X(dx, i);
         v = sqrtf(dot3(d, d));
+        c = vsc(vs, i);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        v = fmaxf(v, __shfl_xor(v, o));
-    }
-    if ((threadIdx.x & 63) == 0 && v > 0.0f) {
-        atomicMax(out, __float_as_uint(v));
+    const uint32_t c0 = __shfl(c, 0);
+    if (__ballot(c != c0 && i < n) == 0) {
+        for (int o = 32; o > 0; o >>= 1) {
+            v = fmaxf(v, __shfl_xor(v, o));
+        }
+        if ((threadIdx.x & 63) == 0 && v > 0.0f) {
+            atomicMax(out + c0, __float_as_uint(v));
+        }
+    } else if (i < n && v > 0.0f) {
+        atomicMax(out + c, __float_as_uint(v));
     }
 }
 
-// target <- eval_x; eval_x <- eval_x - recale * dx (main.cu:1177-1187).
-__global__ void k_step(uint64_t n3, float recale, const float *__restrict__ dx,
-                       float *__restrict__ ex, float *__restrict__ target) {
+// recale = min(1, max_dx / max |dx|) per scene (main.cu:1177-1180).
+__global__ void k_recale(uint32_t ns, float max_dx, const uint32_t *__restrict__ su,
+                         float *__restrict__ sf) {
+    const uint32_t c = blockIdx.x * kNT + threadIdx.x;
+    if (c < ns) {
+        sf[kRecale * ns + c] = fminf(1.0f, max_dx / __uint_as_float(su[kMaxNorm * ns + c]));
+    }
+}
+
+// target <- eval_x; eval_x <- eval_x - recale * dx (main.cu:1177-1187), for
+// the scenes still iterating; a finished scene keeps eval_x and gets
+// target = eval_x (its sweep is then empty).
+__global__ void k_step(uint64_t n3, const uint32_t *__restrict__ vs,
+                       const float *__restrict__ sf, const uint32_t *__restrict__ su,
+                       uint32_t ns, const float *__restrict__ dx, float *__restrict__ ex,
+                       float *__restrict__ target) {
     const uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x;
     if (i < n3) {
+        const uint32_t c = vsc(vs, (uint32_t)(i / 3));
         const float x = ex[i];
         target[i] = x;
-        ex[i] = x - recale * dx[i];
+        if (su[kActive * ns + c]) {
+            ex[i] = x - sf[kRecale * ns + c] * dx[i];
+        }
     }
 }
 
-// eval_x <- target + toi (eval_x - target) (main.cu:1322-1329).
-__global__ void k_shorten(uint64_t n3, float toi, const float *__restrict__ target,
+// eval_x <- target + toi (eval_x - target) (main.cu:1322-1329), per scene
+// still iterating.
+__global__ void k_shorten(uint64_t n3, const uint32_t *__restrict__ vs,
+                          const float *__restrict__ sf, const uint32_t *__restrict__ su,
+                          uint32_t ns, const float *__restrict__ target,
                           float *__restrict__ ex) {
     const uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x;
     if (i < n3) {
-        const float t = target[i];
-        ex[i] = t + toi * (ex[i] - t);
+        const uint32_t c = vsc(vs, (uint32_t)(i / 3));
+        if (su[kActive * ns + c]) {
+            const float t = target[i];
+            ex[i] = t + sf[kToi * ns + c] * (ex[i] - t);
+        }
+    }
+}
+
+// A finished scene takes no Newton step: zero right-hand side and start, so
+// its solve ends at once.
+__global__ void k_freeze(uint32_t n, const uint32_t *__restrict__ vs,
+                         const uint32_t *__restrict__ su, uint32_t ns,
+                         float *__restrict__ force, float *__restrict__ dx) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i < n && !su[kActive * ns + vsc(vs, i)]) {
+        store3(force + 3 * (size_t)i, mk3(0.0f, 0.0f, 0.0f));
+        store3(dx + 3 * (size_t)i, mk3(0.0f, 0.0f, 0.0f));
+    }
+}
+
+// Contact pairs per scene (the scene of the pair's query element).
+__global__ void k_scene_pairs(uint64_t np, const ppf_pair *__restrict__ pairs,
+                              const uint32_t *__restrict__ edge,
+                              const uint32_t *__restrict__ vs,
+                              const uint32_t *__restrict__ svs, uint32_t *__restrict__ cnt) {
+    const uint64_t t = (uint64_t)blockIdx.x * kNT + threadIdx.x;
+    if (t >= np) {
+        return;
+    }
+    const ppf_pair q = pairs[t];
+    const uint32_t kind = q.kind & 255u;
+    uint32_t c;
+    if (kind == 3 || kind == 6) {
+        c = vs[edge[2 * (size_t)q.a]];
+    } else if (kind == 5) {
+        c = svs ? svs[q.a] : 0u;
+    } else {
+        c = vs[q.a];
+    }
+    atomicAdd(cnt + c, 1u);
+}
+
+// prev <- curr, curr <- eval_x for the scenes that commit their step.
+__global__ void k_commit(uint32_t n, const uint32_t *__restrict__ vs,
+                         const uint32_t *__restrict__ ok, float *__restrict__ prev,
+                         float *__restrict__ curr, const float *__restrict__ ex) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i < n && ok[vs[i]]) {
+        store3(prev + 3 * (size_t)i, X(curr, i));
+        store3(curr + 3 * (size_t)i, X(ex, i));
     }
 }
 
@@ -298,7 +392,8 @@ void Stepper::release() {
                     (void *)eval_, (void *)cgrad_, (void *)mask_, (void *)scal_,
                     (void *)pairs_, (void *)mrp_, (void *)mci_, (void *)mcnt_,
                     (void *)scan_, (void *)mval_, (void *)order_, (void *)lcnt_,
-                    (void *)lrp_, (void *)lbuf_}) {
+                    (void *)lrp_, (void *)lbuf_, (void *)sf_, (void *)su_, (void *)sph_,
+                    (void *)flr_}) {
         if (p) {
             (void)dev_free(p);
         }
@@ -306,6 +401,20 @@ void Stepper::release() {
     if (h_scal_) {
         (void)host_free(h_scal_);
     }
+    if (h_sf_) {
+        (void)host_free(h_sf_);
+    }
+    if (h_su_) {
+        (void)host_free(h_su_);
+    }
+    sf_ = h_sf_ = nullptr;
+    su_ = h_su_ = nullptr;
+    sph_ = nullptr;
+    flr_ = nullptr;
+    sph_cap_ = flr_cap_ = nsph_ = nflr_ = 0;
+    ns_ = 1;
+    vscene_ = nullptr;
+    scene_row_.clear();
     if (plan_) {
         pcg_plan_destroy(plan_);
         plan_ = nullptr;
@@ -464,7 +573,35 @@ int Stepper::create(const ppf_scene &sc, hipStream_t s) {
     for (hipEvent_t &e : ev_) {
         PPF_HIP(hipEventCreate(&e));
     }
-    if (nv_ > 1) {
+    if (sc.mesh.vscene && sc.mesh.nscene > 1) {
+        // Many scenes: each scene's vertices contiguous, ids rising from 0.
+        ns_ = sc.mesh.nscene;
+        vscene_ = sc.mesh.vscene;
+        std::vector<uint32_t> hv(nv_);
+        PPF_HIP(hipMemcpyAsync(hv.data(), vscene_, (size_t)nv_ * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, s));
+        PPF_HIP(hipStreamSynchronize(s));
+        for (uint32_t v = 0; v < nv_; ++v) {
+            if (hv[v] >= ns_ || (v > 0 && hv[v] < hv[v - 1])) {
+                set_error("stepper: vertex scene ids must rise from 0 to %u (vertex %u has %u)",
+                          ns_ - 1, v, hv[v]);
+                return PPF_ERR_ARG;
+            }
+        }
+        // first vertex of each scene (an empty scene starts where the next does)
+        scene_row_.assign(ns_ + 1, nv_);
+        for (uint32_t v = nv_; v-- > 0;) {
+            scene_row_[hv[v]] = v;
+        }
+        for (uint32_t c = ns_; c-- > 0;) {
+            scene_row_[c] = std::min(scene_row_[c], scene_row_[c + 1]);
+        }
+    }
+    PPF_HIP(dev_alloc(&sf_, 5 * (size_t)ns_ * sizeof(float)));
+    PPF_HIP(dev_alloc(&su_, 3 * (size_t)ns_ * sizeof(uint32_t)));
+    PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_sf_), 5 * (size_t)ns_ * sizeof(float)));
+    PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_su_), 3 * (size_t)ns_ * sizeof(uint32_t)));
+    if (nv_ > 1 && ns_ == 1) {
         std::vector<float> hx(n3);
         std::vector<uint32_t> hf(3 * (size_t)nf_), order;
         PPF_HIP(hipMemcpyAsync(hx.data(), sc.mesh.x, n3 * sizeof(float), hipMemcpyDeviceToHost,
@@ -509,7 +646,7 @@ int Stepper::reserve_pairs(uint64_t n) {
         PPF_HIP(dev_free(pairs_));
         pairs_ = nullptr;
     }
-    const uint64_t cap = n + n / 2 + 1024;
+    const uint64_t cap = grow_cap(n);
     PPF_HIP(dev_alloc(&pairs_, cap * sizeof(ppf_pair)));
     pair_cap_ = cap;
     return PPF_OK;
@@ -523,7 +660,7 @@ int Stepper::reserve_merged(uint64_t nnz) {
     if (mval_) (void)dev_free(mval_);
     mci_ = nullptr;
     mval_ = nullptr;
-    const uint64_t cap = nnz + nnz / 4 + 1024;
+    const uint64_t cap = grow_cap(nnz);
     PPF_HIP(dev_alloc(&mci_, cap * sizeof(uint32_t)));
     PPF_HIP(dev_alloc(&mval_, 9 * cap * sizeof(float)));
     mcap_ = cap;
@@ -539,7 +676,7 @@ int Stepper::reserve_lifts(uint64_t nl) {
         (void)dev_free(lbuf_);
         lbuf_ = nullptr;
     }
-    const uint64_t cap = nl + nl / 2 + 256;
+    const uint64_t cap = grow_cap(nl, 256);
     PPF_HIP(dev_alloc(&lbuf_, (5 * cap + radix_sort_scratch_words(cap)) * sizeof(uint32_t)));
     lcap_ = cap;
     return PPF_OK;
@@ -548,8 +685,9 @@ int Stepper::reserve_lifts(uint64_t nl) {
 int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, float dt,
                               bool first_iteration, ppf_step_result *out, float *toi_out,
                               float *recale_out, hipStream_t s) {
-    const uint32_t n = nv_;
+    const uint32_t n = nv_, ns = ns_;
     const size_t n3 = 3 * (size_t)n;
+    const uint32_t *vs = ns > 1 ? vscene_ : nullptr;
     // Inertia + elastic at the iterate.
     PPF_HIP(hipEventRecord(ev_[0], s));
     ppf_shell sh = sc.shell;
@@ -561,6 +699,8 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
     ep.air_density = p.air_density;
     ep.air_friction = p.air_friction;
     ep.wind[0] = p.wind[0], ep.wind[1] = p.wind[1], ep.wind[2] = p.wind[2];
+    ep.scene_dt = vs ? sf_ + kDt * ns : nullptr;
+    ep.vscene = vs;
     k_dx_init<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, ex_, target_, dx_);
     PPF_HIP(hipGetLastError());
     int rc = elastic_.assemble(sh, ep, force_, cdiag_, eval_, s);
@@ -619,7 +759,14 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         }
         npairs = nself + nstat;
     }
-    if (npairs > out->pairs_max) {
+    if (vs) { // pairs per scene, read with the line search's results
+        PPF_HIP(hipMemsetAsync(su_ + kPairs * ns, 0, ns * sizeof(uint32_t), s));
+        if (npairs) {
+            k_scene_pairs<<<cdiv(npairs, kNT), kNT, 0, s>>>(
+                npairs, pairs_, sc.mesh.edge, vs, sc.collision.vscene, su_ + kPairs * ns);
+            PPF_HIP(hipGetLastError());
+        }
+    } else if (npairs > out->pairs_max) {
         out->pairs_max = (uint32_t)npairs;
     }
     ppf_contact_params cp;
@@ -635,6 +782,10 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
     cp.el_val = eval_;
     cp.sm = sc.collision.nv ? &sc.collision : nullptr;
     rc = contact_.assemble(m, cp, pairs_, npairs, cgrad_, s);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    rc = analytic_assemble(m, cp, sph_, nsph_, flr_, nflr_, p.constraint_tol, cgrad_, cdiag_, s);
     if (rc != PPF_OK) {
         return rc;
     }
@@ -688,10 +839,16 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         k_dirichlet_diag<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, ex_, target_, cdiag_, force_);
         PPF_HIP(hipGetLastError());
     }
-    // One solver plan per scene: later iterations only swap the pattern in
-    // (in the current row order).
+    if (vs) {
+        k_freeze<<<cdiv(n, kNT), kNT, 0, s>>>(n, vs, su_, ns, force_, dx_);
+        PPF_HIP(hipGetLastError());
+    }
+    // One solver plan: later iterations only swap the pattern in (in the
+    // current row order). Many scenes: one plan over their block-diagonal
+    // union, each scene with its own scalars and stopping rule.
     if (!plan_) {
-        rc = pcg_plan_create(n, nnz, mrp_, mci_, s, &plan_);
+        rc = ns > 1 ? pcg_plan_create_scenes(n, nnz, mrp_, mci_, ns, scene_row_.data(), s, &plan_)
+                    : pcg_plan_create(n, nnz, mrp_, mci_, s, &plan_);
         if (rc != PPF_OK) {
             return rc;
         }
@@ -707,9 +864,9 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
     if (rc != PPF_OK) {
         return rc;
     }
-    ppf_pcg_result pr{};
+    std::vector<ppf_pcg_result> pr(ns);
     bool solved = false;
-    if (lpc_) {
+    if (lpc_ && ns == 1) {
         // learned factor; block Jacobi when it cannot be formed or the solve
         // does not reach the tolerance
         const float *ov = nullptr, *oc = nullptr, *ob = nullptr;
@@ -723,9 +880,9 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         if (rc == PPF_OK) {
             const PcgFactor fac = lpc_->factor();
             rc = pcg_solve(plan_, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr,
-                           &pr, s, &fac);
+                           pr.data(), s, &fac);
         }
-        solved = rc == PPF_OK && (pr.status == 0 || pr.status == 1);
+        solved = rc == PPF_OK && (pr[0].status == 0 || pr[0].status == 1);
         if (!solved) {
             ++out->lpc_fallbacks;
             k_dx_init<<<cdiv(n, kNT), kNT, 0, s>>>(n, mask_, ex_, target_, dx_);
@@ -733,32 +890,58 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         }
     }
     if (!solved) {
-        rc = pcg_solve(plan_, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr, &pr,
-                       s);
+        rc = pcg_solve(plan_, mval_, cdiag_, force_, dx_, p.cg_tol, p.cg_max_iter, nullptr,
+                       pr.data(), s);
         if (rc != PPF_OK) {
             return rc;
         }
     }
-    out->pcg_iter_total += pr.iter;
-    if (pr.iter > out->pcg_iter_max) {
-        out->pcg_iter_max = pr.iter;
+    bool any_failed = false;
+    for (uint32_t c = 0; c < ns; ++c) {
+        if (!active_[c]) {
+            continue;
+        }
+        out[c].pcg_iter_total += pr[c].iter;
+        if (pr[c].iter > out[c].pcg_iter_max) {
+            out[c].pcg_iter_max = pr[c].iter;
+        }
+        if (pr[c].status != 0 && pr[c].status != 1) {
+            out[c].status = 1;
+            any_failed = true;
+        }
     }
-    if (pr.status != 0 && pr.status != 1) {
-        out->status = 1;
+    if (ns == 1 && any_failed) {
         return PPF_OK;
     }
+    if (any_failed) { // the failed scenes take no step
+        for (uint32_t c = 0; c < ns; ++c) {
+            h_su_[kActive * ns + c] = (active_[c] && out[c].status == 0) ? 1u : 0u;
+        }
+        PPF_HIP(hipMemcpyAsync(su_ + kActive * ns, h_su_ + kActive * ns, ns * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, s));
+    }
     PPF_HIP(hipEventRecord(ev_[3], s));
-    // Clamp the step (max_dx), move, CCD line search, shorten.
-    PPF_HIP(hipMemsetAsync(scal_, 0, sizeof(uint32_t), s));
-    k_max_norm<<<cdiv(n, kNT), kNT, 0, s>>>(n, dx_, scal_);
+    // Clamp the step (max_dx, per scene on the device), move, CCD line
+    // search, shorten.
+    PPF_HIP(hipMemsetAsync(su_ + kMaxNorm * ns, 0, ns * sizeof(uint32_t), s));
+    k_max_norm<<<cdiv(n, kNT), kNT, 0, s>>>(n, dx_, vs, su_ + kMaxNorm * ns);
     PPF_HIP(hipGetLastError());
-    PPF_HIP(hipMemcpyAsync(h_scal_, scal_, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    PPF_HIP(hipStreamSynchronize(s));
-    const float max_dx = __builtin_bit_cast(float, h_scal_[0]);
-    const float recale = fminf(1.0f, p.max_dx / max_dx);
-    k_step<<<cdiv(n3, kNT), kNT, 0, s>>>(n3, recale, dx_, ex_, target_);
+    k_recale<<<cdiv(ns, kNT), kNT, 0, s>>>(ns, p.max_dx, su_, sf_);
     PPF_HIP(hipGetLastError());
-    float toi = 1.0f;
+    k_step<<<cdiv(n3, kNT), kNT, 0, s>>>(n3, vs, sf_, su_, ns, dx_, ex_, target_);
+    PPF_HIP(hipGetLastError());
+    // the recale factors (and pair counts) come back with the line search's
+    // synchronization
+    PPF_HIP(hipMemcpyAsync(h_sf_ + kRecale * ns, sf_ + kRecale * ns, ns * sizeof(float),
+                           hipMemcpyDeviceToHost, s));
+    if (vs) {
+        PPF_HIP(hipMemcpyAsync(h_su_ + kPairs * ns, su_ + kPairs * ns, ns * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, s));
+    }
+    for (uint32_t c = 0; c < ns; ++c) {
+        toi_out[c] = 1.0f;
+    }
+    bool overlap_any = false;
     if (!p.disable_contact) {
         ppf_contact_mesh lm = sc.mesh;
         lm.x = target_;
@@ -769,38 +952,84 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         lp.fmass = sc.shell.fmass;
         lp.max_t = p.line_search_max_t;
         lp.ccd_eps = p.ccd_eps;
+        lp.spheres = sph_;
+        lp.nsphere = nsph_;
+        lp.floors = flr_;
+        lp.nfloor = nflr_;
         int overlap = 0;
         float t = 1.0f;
+        uint32_t pin_bad = 0xffffffffu;
+        std::vector<int> sovl(ns, 0);
         rc = ls_.run(lm, sc.collision.nv ? &sc.collision : nullptr, lp, first_iteration, &t,
-                     &overlap, s);
+                     &overlap, s, vs ? toi_out : nullptr, vs ? sovl.data() : nullptr, &pin_bad);
         if (rc != PPF_OK) {
             return rc;
         }
-        toi = fminf(toi, t);
-        if (overlap) {
-            out->status = 3;
+        if (pin_bad != 0xffffffffu) {
+            // a pinned vertex is driven into a static analytic collider it
+            // cannot yield to: the step fails (main.cu:1240-1271); analytic
+            // colliders come with one scene only
+            out[0].status = 6;
             return PPF_OK;
+        }
+        if (!vs) {
+            toi_out[0] = fminf(1.0f, t);
+            sovl[0] = overlap;
+        }
+        for (uint32_t c = 0; c < ns; ++c) {
+            if (active_[c] && out[c].status == 0 && sovl[c]) {
+                out[c].status = 3;
+                overlap_any = true;
+            }
         }
     }
     PPF_HIP(hipEventRecord(ev_[4], s));
     PPF_HIP(hipEventSynchronize(ev_[4]));
     float ms;
     PPF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-    out->ms_elastic += ms;
+    const float ms_el = ms;
     PPF_HIP(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
-    out->ms_contact += ms;
+    const float ms_co = ms;
     PPF_HIP(hipEventElapsedTime(&ms, ev_[2], ev_[3]));
-    out->ms_solve += ms;
+    const float ms_so = ms;
     PPF_HIP(hipEventElapsedTime(&ms, ev_[3], ev_[4]));
-    out->ms_line_search += ms;
-    if (toi <= FLT_EPSILON) {
-        out->status = 2;
+    const float ms_ls = ms;
+    bool any_short = false;
+    for (uint32_t c = 0; c < ns; ++c) {
+        if (!active_[c]) {
+            continue;
+        }
+        out[c].ms_elastic += ms_el;
+        out[c].ms_contact += ms_co;
+        out[c].ms_solve += ms_so;
+        out[c].ms_line_search += ms_ls;
+        if (vs && h_su_[kPairs * ns + c] > out[c].pairs_max) {
+            out[c].pairs_max = h_su_[kPairs * ns + c];
+        }
+        recale_out[c] = h_sf_[kRecale * ns + c];
+        if (out[c].status == 0 && toi_out[c] <= FLT_EPSILON) {
+            out[c].status = 2;
+        }
+        if (out[c].status == 0) {
+            h_sf_[kToi * ns + c] = toi_out[c];
+            any_short = true;
+        }
+    }
+    if (ns == 1 && (overlap_any || out->status != 0)) {
         return PPF_OK;
     }
-    k_shorten<<<cdiv(n3, kNT), kNT, 0, s>>>(n3, toi, target_, ex_);
-    PPF_HIP(hipGetLastError());
-    *toi_out = toi;
-    *recale_out = recale;
+    if (any_short) {
+        // the scenes that failed in this iteration take no shortened step
+        for (uint32_t c = 0; c < ns; ++c) {
+            h_su_[kActive * ns + c] = (active_[c] && out[c].status == 0) ? 1u : 0u;
+        }
+        PPF_HIP(hipMemcpyAsync(sf_ + kToi * ns, h_sf_ + kToi * ns, ns * sizeof(float),
+                               hipMemcpyHostToDevice, s));
+        PPF_HIP(hipMemcpyAsync(su_ + kActive * ns, h_su_ + kActive * ns, ns * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, s));
+        k_shorten<<<cdiv(n3, kNT), kNT, 0, s>>>(n3, vs, sf_, su_, ns, target_, ex_);
+        PPF_HIP(hipGetLastError());
+    }
     return PPF_OK;
 }
 
@@ -814,65 +1043,172 @@ uint32_t Stepper::records(ppf_isect_record *out, uint32_t cap) const {
 
 int Stepper::advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_result *out,
                      hipStream_t s) {
-    *out = ppf_step_result{};
+    if (ns_ != 1) {
+        set_error("stepper: a stepper of %u scenes advances them together", ns_);
+        return PPF_ERR_ARG;
+    }
+    return advance_scenes(sc, p, &p.prev_dt, out, s);
+}
+
+// One time step of every scene (main.cu:267-1649 per scene). The scenes run
+// their Newton loops side by side: each keeps its own dt, toi bookkeeping,
+// Newton count and status; every stage is one launch over all of them, and a
+// scene that has finished its loop takes no further steps while the others
+// iterate.
+int Stepper::advance_scenes(const ppf_scene &sc, const ppf_step_params &p, const float *prev_dt,
+                            ppf_step_result *out, hipStream_t s) {
+    const uint32_t ns = ns_;
+    for (uint32_t c = 0; c < ns; ++c) {
+        out[c] = ppf_step_result{};
+    }
+    if ((sc.nsphere && !sc.spheres) || (sc.nfloor && !sc.floors)) {
+        set_error("stepper: analytic collider counts without arrays");
+        return PPF_ERR_ARG;
+    }
+    if (ns > 1 && (sc.nsphere || sc.nfloor)) {
+        set_error("stepper: analytic colliders apply to one scene (a stepper of %u scenes has "
+                  "collision meshes per scene instead)",
+                  ns);
+        return PPF_ERR_ARG;
+    }
+    // the analytic colliders of this step (kinematic ones move between steps)
+    if (sc.nsphere > sph_cap_) {
+        if (sph_) {
+            PPF_HIP(dev_free(sph_));
+            sph_ = nullptr;
+        }
+        PPF_HIP(dev_alloc(&sph_, sc.nsphere * sizeof(ppf_sphere)));
+        sph_cap_ = sc.nsphere;
+    }
+    if (sc.nfloor > flr_cap_) {
+        if (flr_) {
+            PPF_HIP(dev_free(flr_));
+            flr_ = nullptr;
+        }
+        PPF_HIP(dev_alloc(&flr_, sc.nfloor * sizeof(ppf_floor)));
+        flr_cap_ = sc.nfloor;
+    }
+    nsph_ = sc.nsphere;
+    nflr_ = sc.nfloor;
+    if (nsph_) {
+        PPF_HIP(hipMemcpyAsync(sph_, sc.spheres, nsph_ * sizeof(ppf_sphere),
+                               hipMemcpyHostToDevice, s));
+    }
+    if (nflr_) {
+        PPF_HIP(hipMemcpyAsync(flr_, sc.floors, nflr_ * sizeof(ppf_floor), hipMemcpyHostToDevice,
+                               s));
+    }
     records_count_ = 0;
     uint64_t alloc0 = 0, free0 = 0;
     alloc_totals(&alloc0, &free0);
     const uint32_t n = nv_;
     const size_t n3 = 3 * (size_t)n;
+    const uint32_t *vs = ns > 1 ? vscene_ : nullptr;
     PPF_HIP(hipEventRecord(ev_[5], s));
     const f3 g = mk3(p.gravity[0], p.gravity[1], p.gravity[2]);
-    float dt = p.dt;
-    const float prev_dt = p.prev_dt > 0.0f ? p.prev_dt : p.dt;
-    k_target<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.x, sc.prev, sc.mesh.vflag, sc.fix_pos, dt,
-                                           prev_dt, g, target_);
+    std::vector<float> dt(ns, p.dt);
+    std::vector<double> toi_advanced(ns, 0.0);
+    std::vector<uint32_t> step(ns, 1u);
+    std::vector<uint8_t> final_step(ns, 0u);
+    std::vector<float> toi(ns), recale(ns);
+    active_.assign(ns, 1u);
+    for (uint32_t c = 0; c < ns; ++c) {
+        h_sf_[kDt * ns + c] = dt[c];
+        h_sf_[kPrevDt * ns + c] = prev_dt[c] > 0.0f ? prev_dt[c] : p.dt;
+        h_sf_[kBack * ns + c] = 0.0f;
+        h_su_[kActive * ns + c] = 1u;
+    }
+    PPF_HIP(hipMemcpyAsync(sf_, h_sf_, kSceneFloats * ns * sizeof(float), hipMemcpyHostToDevice,
+                           s));
+    PPF_HIP(hipMemcpyAsync(su_ + kActive * ns, h_su_ + kActive * ns, ns * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, s));
+    k_target<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.x, sc.prev, sc.mesh.vflag, sc.fix_pos, vs,
+                                           sf_, ns, g, target_);
     PPF_HIP(hipGetLastError());
     PPF_HIP(hipMemcpyAsync(ex_, sc.mesh.x, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
-    double toi_advanced = 0.0;
-    uint32_t step = 1;
-    bool final_step = false;
-    while (true) {
-        if (!final_step && p.max_newton_steps > 0 && step >= p.max_newton_steps) {
-            out->status = 5;
+    for (uint32_t iter = 0;; ++iter) {
+        uint32_t live = 0;
+        for (uint32_t c = 0; c < ns; ++c) {
+            if (active_[c] && !final_step[c] && p.max_newton_steps > 0 &&
+                step[c] >= p.max_newton_steps) {
+                out[c].status = 5;
+                active_[c] = 0u;
+            }
+            live += active_[c];
+        }
+        if (live == 0) {
             break;
         }
-        if (final_step) {
-            if (sc.fix_pos && sc.fix_delta && toi_advanced < 1.0) {
-                k_rewind<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.vflag, sc.fix_pos, sc.fix_delta,
-                                                      1.0f - (float)toi_advanced);
-                PPF_HIP(hipGetLastError());
+        if (live < ns) { // the scenes that have left the loop take no step
+            for (uint32_t c = 0; c < ns; ++c) {
+                h_su_[kActive * ns + c] = active_[c];
             }
-            dt *= (float)toi_advanced;
-            k_target<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.x, sc.prev, sc.mesh.vflag,
-                                                   sc.fix_pos, dt, prev_dt, g, target_);
-            PPF_HIP(hipGetLastError());
+            PPF_HIP(hipMemcpyAsync(su_ + kActive * ns, h_su_ + kActive * ns,
+                                   ns * sizeof(uint32_t), hipMemcpyHostToDevice, s));
         }
-        float toi = 1.0f, recale = 1.0f;
-        int rc = newton_iteration(sc, p, dt, out->newton_steps == 0, out, &toi, &recale, s);
+        int rc = newton_iteration(sc, p, dt[0], iter == 0, out, toi.data(), recale.data(), s);
         if (rc != PPF_OK) {
             return rc;
         }
-        out->newton_steps += 1;
-        if (out->status != 0) {
+        bool moved = false;
+        for (uint32_t c = 0; c < ns; ++c) {
+            h_sf_[kBack * ns + c] = 0.0f;
+            if (!active_[c]) {
+                continue;
+            }
+            out[c].newton_steps += 1;
+            if (out[c].status != 0) {
+                active_[c] = 0u;
+                continue;
+            }
+            if (final_step[c]) {
+                active_[c] = 0u;
+                continue;
+            }
+            toi_advanced[c] +=
+                std::max(0.0, 1.0 - toi_advanced[c]) * (double)(recale[c] * toi[c]);
+            if (toi_advanced[c] >= p.target_toi && step[c] >= p.min_newton_steps) {
+                // the error-reduction iteration at dt * toi_advanced, with the
+                // pins walked back to that fraction of their step
+                final_step[c] = 1u;
+                if (sc.fix_pos && sc.fix_delta && toi_advanced[c] < 1.0) {
+                    h_sf_[kBack * ns + c] = 1.0f - (float)toi_advanced[c];
+                    moved = true;
+                }
+                dt[c] *= (float)toi_advanced[c];
+            } else {
+                ++step[c];
+            }
+            h_sf_[kDt * ns + c] = dt[c];
+        }
+        bool any = false;
+        for (uint32_t c = 0; c < ns; ++c) {
+            any = any || active_[c];
+        }
+        if (!any) {
             break;
         }
-        if (!final_step) {
-            toi_advanced += std::max(0.0, 1.0 - toi_advanced) * (double)(recale * toi);
-        }
-        if (final_step) {
-            break;
-        } else if (toi_advanced >= p.target_toi && step >= p.min_newton_steps) {
-            final_step = true;
-        } else {
-            ++step;
-            k_target<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.x, sc.prev, sc.mesh.vflag,
-                                                   sc.fix_pos, dt, prev_dt, g, target_);
+        PPF_HIP(hipMemcpyAsync(sf_, h_sf_, kSceneFloats * ns * sizeof(float),
+                               hipMemcpyHostToDevice, s));
+        if (moved) {
+            k_rewind<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.vflag, sc.fix_pos, sc.fix_delta, vs,
+                                                  sf_, ns);
             PPF_HIP(hipGetLastError());
         }
+        k_target<<<cdiv(n, kNT), kNT, 0, s>>>(n, sc.mesh.x, sc.prev, sc.mesh.vflag, sc.fix_pos,
+                                               vs, sf_, ns, g, target_);
+        PPF_HIP(hipGetLastError());
+        // the host's copy of the scalars is staged again after the next
+        // synchronization
+        PPF_HIP(hipStreamSynchronize(s));
     }
-    out->toi_advanced = (float)toi_advanced;
-    out->dt = dt;
-    if (out->status == 0 && p.check_intersection) {
+    bool check = false;
+    for (uint32_t c = 0; c < ns; ++c) {
+        out[c].toi_advanced = (float)toi_advanced[c];
+        out[c].dt = dt[c];
+        check = check || out[c].status == 0;
+    }
+    if (check && p.check_intersection) {
         ppf_contact_mesh m = sc.mesh;
         m.x = ex_;
         int rc = narrow_.build_trees(m, s);
@@ -880,32 +1216,56 @@ int Stepper::advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_res
             return rc;
         }
         uint32_t count = 0;
+        std::vector<uint32_t> per(ns, 0u);
         rc = isect_.run(narrow_.tree(0).view(), narrow_.tree(1).view(), narrow_.tree(2).view(),
                         m, sc.collision.nv ? &sc.collision : nullptr, sc.shell.vmass,
-                        sc.emass, sc.shell.fmass, n, records_, kMaxRecords, &count, s);
+                        sc.emass, sc.shell.fmass, n, records_, kMaxRecords, &count, s,
+                        vs ? per.data() : nullptr);
         if (rc != PPF_OK) {
             return rc;
         }
         records_count_ = count;
-        out->intersections = count;
-        if (count) {
-            out->status = 4;
+        if (!vs) {
+            per[0] = count;
+        }
+        for (uint32_t c = 0; c < ns; ++c) {
+            if (out[c].status == 0) {
+                out[c].intersections = per[c];
+                if (per[c]) {
+                    out[c].status = 4;
+                }
+            }
         }
     }
-    if (out->status == 0 || out->status == 4) {
-        // prev <- curr, curr <- iterate
-        PPF_HIP(hipMemcpyAsync(sc.prev, sc.mesh.x, n3 * sizeof(float), hipMemcpyDeviceToDevice,
-                               s));
-        PPF_HIP(hipMemcpyAsync(const_cast<float *>(sc.mesh.x), ex_, n3 * sizeof(float),
-                               hipMemcpyDeviceToDevice, s));
+    // prev <- curr, curr <- iterate for the scenes that commit
+    if (!vs) {
+        if (out[0].status == 0 || out[0].status == 4) {
+            PPF_HIP(hipMemcpyAsync(sc.prev, sc.mesh.x, n3 * sizeof(float),
+                                   hipMemcpyDeviceToDevice, s));
+            PPF_HIP(hipMemcpyAsync(const_cast<float *>(sc.mesh.x), ex_, n3 * sizeof(float),
+                                   hipMemcpyDeviceToDevice, s));
+        }
+    } else {
+        for (uint32_t c = 0; c < ns; ++c) {
+            h_su_[kActive * ns + c] = (out[c].status == 0 || out[c].status == 4) ? 1u : 0u;
+        }
+        PPF_HIP(hipMemcpyAsync(su_ + kActive * ns, h_su_ + kActive * ns, ns * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, s));
+        k_commit<<<cdiv(n, kNT), kNT, 0, s>>>(n, vs, su_ + kActive * ns, sc.prev,
+                                               const_cast<float *>(sc.mesh.x), ex_);
+        PPF_HIP(hipGetLastError());
     }
     PPF_HIP(hipEventRecord(ev_[6], s));
     PPF_HIP(hipEventSynchronize(ev_[6]));
-    PPF_HIP(hipEventElapsedTime(&out->ms_total, ev_[5], ev_[6]));
+    float ms_total = 0.0f;
+    PPF_HIP(hipEventElapsedTime(&ms_total, ev_[5], ev_[6]));
     uint64_t alloc1 = 0, free1 = 0;
     alloc_totals(&alloc1, &free1);
-    out->device_alloc = (uint32_t)(alloc1 - alloc0);
-    out->device_free = (uint32_t)(free1 - free0);
+    for (uint32_t c = 0; c < ns; ++c) {
+        out[c].ms_total = ms_total;
+        out[c].device_alloc = (uint32_t)(alloc1 - alloc0);
+        out[c].device_free = (uint32_t)(free1 - free0);
+    }
     return PPF_OK;
 }
 

@@ -1,5 +1,7 @@
 // newton.hpp -- one time step of the contact solver (internal interface).
 #pragma once
+#include <vector>
+
 #include "assemble.hpp"
 #include "elastic.hpp"
 #include "isect.hpp"
@@ -10,8 +12,9 @@
 
 namespace ppf {
 
-// Owns every per-step buffer and stage of the Newton loop of one scene; the
-// scene's arrays stay with the caller.
+// Owns every per-step buffer and stage of the Newton loop of one scene, or of
+// many independent scenes stepped together (one launch per stage over all of
+// them); the scene's arrays stay with the caller.
 class Stepper {
   public:
     ~Stepper();
@@ -20,6 +23,11 @@ class Stepper {
     int refresh_pins(const ppf_scene &sc, hipStream_t s);
     int advance(const ppf_scene &sc, const ppf_step_params &p, ppf_step_result *out,
                 hipStream_t s);
+    // One time step of every scene of a many-scene mesh (sc.mesh.vscene):
+    // prev_dt and out hold one entry per scene (p.prev_dt is not read).
+    int advance_scenes(const ppf_scene &sc, const ppf_step_params &p, const float *prev_dt,
+                       ppf_step_result *out, hipStream_t s);
+    uint32_t scenes() const { return ns_; }
     const float *eval_x() const { return ex_; }
     uint32_t nv() const { return nv_; }
     // Learned preconditioner for the Newton solves (null: block Jacobi).
@@ -47,6 +55,19 @@ class Stepper {
     int reserve_lifts(uint64_t nl);
     void release();
     uint32_t nv_ = 0, ne_ = 0, nf_ = 0, nh_ = 0;
+    // scenes: count, the caller's per-vertex scene ids, the first vertex of
+    // each scene (host), per-scene step scalars (device and pinned host: 5
+    // floats and 3 words per scene) and the scenes still in their Newton loop
+    uint32_t ns_ = 1;
+    const uint32_t *vscene_ = nullptr;
+    std::vector<uint32_t> scene_row_;
+    float *sf_ = nullptr, *h_sf_ = nullptr;
+    uint32_t *su_ = nullptr, *h_su_ = nullptr;
+    std::vector<uint8_t> active_;
+    // analytic colliders of the step (device copies of the scene's arrays)
+    ppf_sphere *sph_ = nullptr;
+    ppf_floor *flr_ = nullptr;
+    uint32_t sph_cap_ = 0, flr_cap_ = 0, nsph_ = 0, nflr_ = 0;
     ElasticAssembler elastic_;
     Narrow narrow_;
     ContactAssembler contact_;

@@ -1071,7 +1071,8 @@ struct PcgPlanImpl {
     uint32_t n = 0, nnz = 0;
     Args h{};             // host copy of the kernel arguments
     Args *d_args = nullptr;
-    Scalars *h_scal = nullptr; // pinned
+    Scalars *h_scal = nullptr; // pinned; a one-scene plan holds two (polled in turn)
+    hipEvent_t poll_ev[2] = {nullptr, nullptr}; // status copies of a one-scene plan
     void *mem = nullptr;       // one device allocation for the n-sized scratch
     uint32_t *tp = nullptr, *dslot = nullptr;
     uint32_t *scan_tmp = nullptr, *flags = nullptr;
@@ -1104,6 +1105,11 @@ struct PcgPlanImpl {
     float *pval = nullptr;
     void *tmem = nullptr;      // scalars | scene tile starts | tiles | partials
     size_t tmem_cap = 0;
+    // A plan on a fixed device region (the allocation-free solve): every
+    // buffer is carved from it, nothing is freed, no graph is captured and
+    // the status words are read into host memory the caller provides.
+    char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 };
 
 namespace {
@@ -1117,6 +1123,24 @@ template <class T> T *carve(char *&cur, size_t count) {
 
 size_t carve_bytes(size_t count, size_t elem) {
     return (count * elem + 255) & ~size_t(255);
+}
+
+// Device buffers of a plan: from the plan's fixed region when it has one,
+// else from the device allocator.
+template <class T> hipError_t plan_alloc(PcgPlanImpl *pl, T **p, size_t bytes) {
+    if (!pl->arena) {
+        return dev_alloc(p, bytes);
+    }
+    const size_t b = carve_bytes(bytes, 1);
+    if (b > pl->arena_cap - pl->arena_used) {
+        return hipErrorOutOfMemory;
+    }
+    *p = reinterpret_cast<T *>(pl->arena + pl->arena_used);
+    pl->arena_used += b;
+    return hipSuccess;
+}
+hipError_t plan_free(PcgPlanImpl *pl, void *p) {
+    return pl->arena ? hipSuccess : dev_free(p);
 }
 
 // The SpMV kernel a plan launches (its graph is captured for one).
@@ -1186,22 +1210,23 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
     }
     if (2 * (size_t)nnz > pl->pat_cap) {
         if (pl->pat) {
-            PPF_HIP(dev_free(pl->pat));
+            PPF_HIP(plan_free(pl, pl->pat));
             pl->pat = nullptr;
         }
-        const size_t cap = 2 * ((size_t)nnz + nnz / 4 + 256);
-        PPF_HIP(dev_alloc(&pl->pat, cap * sizeof(uint32_t)));
+        const size_t cap = pl->arena ? 2 * ((size_t)nnz + nnz / 4 + 256) : 2 * grow_cap(nnz);
+        PPF_HIP(plan_alloc(pl, &pl->pat, cap * sizeof(uint32_t)));
         pl->pat_cap = cap;
     }
     uint32_t *tcol = pl->pat, *tslot = pl->pat + pl->pat_cap / 2;
     const size_t sort_words = 3 * (size_t)nnz + radix_sort_scratch_words(nnz);
     if (sort_words > pl->tsort_cap) {
         if (pl->tsort) {
-            PPF_HIP(dev_free(pl->tsort));
+            PPF_HIP(plan_free(pl, pl->tsort));
             pl->tsort = nullptr;
         }
-        const size_t cap = sort_words + sort_words / 4 + 256;
-        PPF_HIP(dev_alloc(&pl->tsort, cap * sizeof(uint32_t)));
+        const size_t cap = pl->arena ? sort_words + sort_words / 4 + 256
+                                     : 3 * grow_cap(nnz) + radix_sort_scratch_words(grow_cap(nnz));
+        PPF_HIP(plan_alloc(pl, &pl->tsort, cap * sizeof(uint32_t)));
         pl->tsort_cap = cap;
     }
     pl->nnz = nnz;
@@ -1278,10 +1303,10 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
         const uint32_t W = 4 * h.ell_m;
         if ((size_t)n * W > pl->ell_cap) {
             if (pl->ell) {
-                PPF_HIP(dev_free(pl->ell));
+                PPF_HIP(plan_free(pl, pl->ell));
                 pl->ell = nullptr;
             }
-            PPF_HIP(dev_alloc(&pl->ell, (size_t)n * W * sizeof(uint2)));
+            PPF_HIP(plan_alloc(pl, &pl->ell, (size_t)n * W * sizeof(uint2)));
             pl->ell_cap = (size_t)n * W;
         }
         h.ent = static_cast<const uint2 *>(pl->ell);
@@ -1310,7 +1335,7 @@ int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_
             pl->emem = nullptr;
             pl->ecap = 0;
         }
-        const size_t cap = (size_t)nnz + nnz / 4 + 256;
+        const size_t cap = grow_cap(nnz + 1);
         PPF_HIP(dev_alloc(&pl->emem, cap * (4 + 4 + 36)));
         pl->ecap = cap;
         pl->pcol = static_cast<uint32_t *>(pl->emem);
@@ -1425,11 +1450,11 @@ int set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st,
             return PPF_ERR_ARG;
         }
         if (pl->tmem) {
-            PPF_HIP(dev_free(pl->tmem));
+            PPF_HIP(plan_free(pl, pl->tmem));
             pl->tmem = nullptr;
             pl->tmem_cap = 0;
         }
-        PPF_HIP(dev_alloc(&pl->tmem, bytes));
+        PPF_HIP(plan_alloc(pl, &pl->tmem, bytes));
         pl->tmem_cap = bytes;
     }
     char *cur = static_cast<char *>(pl->tmem);
@@ -1518,28 +1543,14 @@ __global__ void k_check_scenes(uint32_t n, uint32_t nscene, const uint32_t *__re
 
 } // namespace
 
-int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
-                           uint32_t nscene, const uint32_t *scene_row, hipStream_t st,
-                           PcgPlanImpl **out) {
-    if (!out || n == 0 || !rp || !col || nnz >= kMirror || nscene == 0 ||
-        (nscene > 1 && !scene_row)) {
-        set_error("pcg_plan_create: invalid argument");
-        return PPF_ERR_ARG;
-    }
-    if (nscene > 1) {
-        bool ok = scene_row[0] == 0 && scene_row[nscene] == n;
-        for (uint32_t c = 0; ok && c < nscene; ++c) {
-            ok = scene_row[c] <= scene_row[c + 1];
-        }
-        if (!ok) {
-            set_error("pcg_plan_create: scene rows must rise from 0 to n");
-            return PPF_ERR_ARG;
-        }
-    }
-    auto *pl = new (std::nothrow) PcgPlanImpl();
-    if (!pl) {
-        return PPF_ERR_ARG;
-    }
+namespace {
+
+// Everything a plan holds for its pattern and scenes, on a fresh PcgPlanImpl
+// (n-sized scratch, status words, pattern arrays, tiles, the iteration graph).
+// A plan with a fixed region takes its buffers from it and captures no graph;
+// its status words live in the host memory already in pl->h_scal.
+int plan_init(PcgPlanImpl *pl, uint32_t n, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+              uint32_t nscene, const uint32_t *scene_row, hipStream_t st) {
     pl->n = n;
     pl->nscene = nscene;
     pl->scene_row.assign(nscene + 1, 0u);
@@ -1557,10 +1568,9 @@ int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const u
                    carve_bytes(9 * (size_t)n, 4) + carve_bytes(n3, 4) * 5 +
                    carve_bytes(1, sizeof(Args)) + carve_bytes(scan_words, 4) +
                    carve_bytes(4, 4) + carve_bytes(nscene + 1, 4);
-    hipError_t e = dev_alloc(&pl->mem, bytes);
+    hipError_t e = plan_alloc(pl, &pl->mem, bytes);
     if (e != hipSuccess) {
-        set_error("pcg_plan_create: dev_alloc(%zu): %s", bytes, hipGetErrorString(e));
-        delete pl;
+        set_error("pcg_plan_create: %zu bytes of scratch: %s", bytes, hipGetErrorString(e));
         return PPF_ERR_HIP;
     }
     char *cur = static_cast<char *>(pl->mem);
@@ -1576,11 +1586,22 @@ int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const u
     pl->scan_tmp = carve<uint32_t>(cur, scan_words);
     pl->flags = carve<uint32_t>(cur, 4);
     uint32_t *d_srow = carve<uint32_t>(cur, nscene + 1);
-    e = host_alloc(reinterpret_cast<void **>(&pl->h_scal), nscene * sizeof(Scalars));
-    if (e != hipSuccess) {
-        set_error("pcg_plan_create: hipHostMalloc: %s", hipGetErrorString(e));
-        ppf::pcg_plan_destroy(pl);
-        return PPF_ERR_HIP;
+    if (!pl->arena) {
+        const uint32_t nbuf = nscene == 1 ? 2u : nscene;
+        e = host_alloc(reinterpret_cast<void **>(&pl->h_scal), nbuf * sizeof(Scalars));
+        if (e != hipSuccess) {
+            set_error("pcg_plan_create: hipHostMalloc: %s", hipGetErrorString(e));
+            return PPF_ERR_HIP;
+        }
+        if (nscene == 1) {
+            for (hipEvent_t &ev : pl->poll_ev) {
+                e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+                if (e != hipSuccess) {
+                    set_error("pcg_plan_create: hipEventCreate: %s", hipGetErrorString(e));
+                    return PPF_ERR_HIP;
+                }
+            }
+        }
     }
     int rc = set_pattern(pl, nnz, rp, col, st);
     if (rc == PPF_OK && nscene > 1) {
@@ -1607,9 +1628,37 @@ int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const u
         // a batch starts from mid-size tiles; the autotuner refines them
         rc = nscene > 1 ? set_tiles(pl, 512u, 512u, st) : set_tiles(pl, 0u, 0u, st);
     }
-    if (rc == PPF_OK) {
+    if (rc == PPF_OK && !pl->arena) {
         rc = capture_graph(pl);
     }
+    return rc;
+}
+
+} // namespace
+
+int pcg_plan_create_scenes(uint32_t n, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                           uint32_t nscene, const uint32_t *scene_row, hipStream_t st,
+                           PcgPlanImpl **out) {
+    if (!out || n == 0 || !rp || !col || nnz >= kMirror || nscene == 0 ||
+        (nscene > 1 && !scene_row)) {
+        set_error("pcg_plan_create: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    if (nscene > 1) {
+        bool ok = scene_row[0] == 0 && scene_row[nscene] == n;
+        for (uint32_t c = 0; ok && c < nscene; ++c) {
+            ok = scene_row[c] <= scene_row[c + 1];
+        }
+        if (!ok) {
+            set_error("pcg_plan_create: scene rows must rise from 0 to n");
+            return PPF_ERR_ARG;
+        }
+    }
+    auto *pl = new (std::nothrow) PcgPlanImpl();
+    if (!pl) {
+        return PPF_ERR_ARG;
+    }
+    const int rc = plan_init(pl, n, nnz, rp, col, nscene, scene_row, st);
     if (rc != PPF_OK) {
         ppf::pcg_plan_destroy(pl);
         return rc;
@@ -1652,6 +1701,11 @@ void pcg_plan_destroy(PcgPlanImpl *pl) {
     }
     if (pl->cap) {
         (void)hipStreamDestroy(pl->cap);
+    }
+    for (hipEvent_t ev : pl->poll_ev) {
+        if (ev) {
+            (void)hipEventDestroy(ev);
+        }
     }
     if (pl->h_scal) {
         (void)host_free(pl->h_scal);
@@ -1819,7 +1873,7 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
     h.Gl = fac ? fac->Gl : nullptr;
     h.Gr = fac ? fac->Gr : nullptr;
     const bool Gd = fac != nullptr;
-    if (spmv_key(*pl) != pl->graph_key) {
+    if (!pl->arena && spmv_key(*pl) != pl->graph_key) {
         // the iteration graph holds the preconditioner's kernels
         int rc = capture_graph(pl);
         if (rc != PPF_OK) {
@@ -1851,13 +1905,54 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
     PPF_HIP(hipGetLastError());
     k_init_scalars<<<ns, kNT, 0, st>>>(pl->d_args);
     PPF_HIP(hipGetLastError());
+    const uint32_t max_launch = max_iter / kGraphIters + 2;
+    if (pl->exec && ns == 1) {
+        // One scene: keep one graph of iterations in flight while the host
+        // reads the status copied after the graph before it (two pinned
+        // status words, an event each), so the device never idles on the
+        // host's poll. Graphs launched after the solve is done exit at entry.
+        Scalars *buf[2] = {pl->h_scal, pl->h_scal + 1};
+        for (uint32_t k = 0; k < 2; ++k) {
+            PPF_HIP(hipGraphLaunch(pl->exec, st));
+            PPF_HIP(hipMemcpyAsync(buf[k], h.s, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+            PPF_HIP(hipEventRecord(pl->poll_ev[k], st));
+        }
+        uint32_t k = 0;
+        for (;; ++k) {
+            PPF_HIP(hipEventSynchronize(pl->poll_ev[k & 1]));
+            if (buf[k & 1]->done != 0) {
+                break;
+            }
+            if (k + 2 >= max_launch) {
+                (void)hipStreamSynchronize(st);
+                set_error("pcg_solve: bookkeeping did not terminate");
+                return PPF_ERR_SOLVER;
+            }
+            PPF_HIP(hipGraphLaunch(pl->exec, st));
+            PPF_HIP(hipMemcpyAsync(buf[k & 1], h.s, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+            PPF_HIP(hipEventRecord(pl->poll_ev[k & 1], st));
+        }
+        const Scalars done = *buf[k & 1];
+        if (pl->order) {
+            k_order_rows<true><<<cdiv(3 * pl->n, kNT), kNT, 0, st>>>(pl->n, 3, pl->order, x,
+                                                                     x_caller);
+            PPF_HIP(hipGetLastError());
+        }
+        PPF_HIP(hipStreamSynchronize(st)); // the graph still in flight ends here
+        res[0].iter = done.rep_iter;
+        res[0].resid = done.rep_resid;
+        res[0].status = done.status;
+        res[0].break_cause = done.cause;
+        res[0].break_iter = done.break_iter;
+        res[0].err0 = done.err0;
+        return PPF_OK;
+    }
     // Replay graphs until the device bookkeeping reports every scene done.
     // The first status read follows one graph; the iteration cap bounds the
     // loop. With many scenes, once the scenes still iterating hold under 3/4
     // of the tiled rows, the tiles are cut again over those scenes only (the
     // partial sums are written and read within one iteration, so the switch
     // between graphs is safe).
-    const uint32_t max_launch = max_iter / kGraphIters + 2;
     uint32_t tiled_rows = pl->n;
     std::vector<uint8_t> active(ns, 1u);
     for (uint32_t g = 0;; ++g) {
@@ -1890,7 +1985,13 @@ int pcg_solve(PcgPlanImpl *pl, const float *val, const float *cdiag,
             }
             tiled_rows = live_rows;
         }
-        PPF_HIP(hipGraphLaunch(pl->exec, st));
+        if (pl->exec) {
+            PPF_HIP(hipGraphLaunch(pl->exec, st));
+        } else { // a plan on a fixed region: the graph's iterations, launched one by one
+            for (uint32_t k = 0; k < (uint32_t)kGraphIters; ++k) {
+                PPF_HIP(launch_iteration(*pl, k, st));
+            }
+        }
     }
     if (pl->order) {
         k_order_rows<true><<<cdiv(3 * pl->n, kNT), kNT, 0, st>>>(pl->n, 3, pl->order, x,

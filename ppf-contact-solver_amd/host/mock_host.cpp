@@ -281,14 +281,41 @@ int main(int argc, char **argv) {
         set_vec(cm.param_arrays.face, s_fparam.data(), snf);
     }
 
-    // An analytic sphere collider, which this backend refuses (initialize
-    // must fail with fatal code 4 and name it).
-    cts_Sphere sphere{};
-    if (pk.has("sphere")) {
-        const std::vector<float> sp = pk.f("sphere");
-        for (int d = 0; d < 3; ++d) sphere.center.v[d] = sp[d];
-        sphere.radius = sp[3];
-        set_vec(ds.constraint.sphere, &sphere, 1);
+    // Analytic colliders: rows of 10 floats -- sphere: center xyz, ghat,
+    // friction, radius, thickness, bowl, reverse, kinematic; floor: ground
+    // xyz, ghat, friction, thickness, up xyz, kinematic.
+    std::vector<cts_Sphere> spheres;
+    std::vector<cts_Floor> floors;
+    if (pk.has("spheres")) {
+        const std::vector<float> a = pk.f("spheres");
+        for (size_t k = 0; k + 10 <= a.size(); k += 10) {
+            cts_Sphere sp{};
+            for (int d = 0; d < 3; ++d) sp.center.v[d] = a[k + d];
+            sp.ghat = a[k + 3], sp.friction = a[k + 4], sp.radius = a[k + 5];
+            sp.thickness = a[k + 6];
+            sp.bowl = a[k + 7] != 0.0f, sp.reverse = a[k + 8] != 0.0f;
+            sp.kinematic = a[k + 9] != 0.0f;
+            spheres.push_back(sp);
+        }
+        set_vec(ds.constraint.sphere, spheres.data(), (uint32_t)spheres.size());
+    }
+    if (pk.has("floors")) {
+        const std::vector<float> a = pk.f("floors");
+        for (size_t k = 0; k + 10 <= a.size(); k += 10) {
+            cts_Floor fl{};
+            for (int d = 0; d < 3; ++d) fl.ground.v[d] = a[k + d];
+            fl.ghat = a[k + 3], fl.friction = a[k + 4], fl.thickness = a[k + 5];
+            for (int d = 0; d < 3; ++d) fl.up.v[d] = a[k + 6 + d];
+            fl.kinematic = a[k + 9] != 0.0f;
+            floors.push_back(fl);
+        }
+        set_vec(ds.constraint.floor, floors.data(), (uint32_t)floors.size());
+    }
+    // A stitch, which this backend refuses (initialize must fail with fatal
+    // code 4 and name it).
+    cts_Stitch stitch{};
+    if (pk.has("stitch")) {
+        set_vec(ds.constraint.stitch, &stitch, 1);
     }
 
     // ---- ParamSet -------------------------------------------------------------
@@ -298,6 +325,7 @@ int main(int argc, char **argv) {
     param.prev_dt = (float)pk.d("prev_dt");
     param.playback = 1.0f;
     param.cg_tol = (float)pk.d("cg_tol");
+    param.constraint_tol = pk.has("constraint_tol") ? (float)pk.d("constraint_tol") : 0.01f;
     param.cg_max_iter = (uint32_t)pk.d("cg_max_iter");
     param.target_toi = (float)pk.d("target_toi");
     param.line_search_max_t = (float)pk.d("line_search_max_t");

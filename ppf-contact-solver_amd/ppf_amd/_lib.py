@@ -85,6 +85,7 @@ _PROTOS = {
     "ppf_stepper_create": (ctypes.c_int, [vp, vp, ctypes.POINTER(vp)]),
     "ppf_stepper_destroy": (None, [vp]),
     "ppf_stepper_advance": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "ppf_stepper_advance_scenes": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
     "ppf_stepper_update_pins": (ctypes.c_int, [vp, vp, vp]),
     "ppf_stepper_set_preconditioner": (ctypes.c_int, [vp, vp]),
     "ppf_stepper_system": (ctypes.c_int, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), vp,

@@ -27,13 +27,14 @@ class StaticMeshC(ctypes.Structure):
     _fields_ = [("nv", u32), ("ne", u32), ("nf", u32), ("x", vp), ("edge", vp),
                 ("face", vp), ("v_ghat", vp), ("v_off", vp), ("e_ghat", vp),
                 ("e_off", vp), ("f_ghat", vp), ("f_off", vp), ("v_fric", vp),
-                ("e_fric", vp), ("f_fric", vp)]
+                ("e_fric", vp), ("f_fric", vp), ("vscene", vp)]
 
 
 class LsParamsC(ctypes.Structure):
     """ppf_ls_params (include/ppf_mi355x.h)."""
     _fields_ = [("x1", vp), ("vmass", vp), ("emass", vp), ("fmass", vp),
-                ("max_t", f32), ("ccd_eps", f32)]
+                ("max_t", f32), ("ccd_eps", f32), ("spheres", vp), ("nsphere", u32),
+                ("floors", vp), ("nfloor", u32)]
 
 
 def static_to_device(sm, dev):
@@ -48,6 +49,9 @@ def static_to_device(sm, dev):
     for k in STATIC_INDEX:
         a = np.ascontiguousarray(sm[k], np.uint32).view(np.int32)
         out[k] = torch.from_numpy(a).reshape(-1).to(dev)
+    if sm.get("vscene") is not None:
+        a = np.ascontiguousarray(sm["vscene"], np.uint32).view(np.int32)
+        out["vscene"] = torch.from_numpy(a).reshape(-1).to(dev)
     return out
 
 
@@ -62,6 +66,8 @@ def static_struct(t):
         if not ten.is_cuda or not ten.is_contiguous():
             raise ValueError(f"{k}: expected a contiguous cuda tensor")
         fields[k] = ten.data_ptr() if ten.numel() else None
+    if t.get("vscene") is not None:
+        fields["vscene"] = t["vscene"].data_ptr()
     return StaticMeshC(**fields)
 
 

@@ -53,7 +53,8 @@ def read_pack(path):
 
 PARAM_KEYS = ("dt", "prev_dt", "cg_tol", "cg_max_iter", "target_toi", "line_search_max_t",
               "ccd_eps", "max_dx", "friction_eps", "friction_mode", "min_newton_steps",
-              "max_newton_steps", "air_density", "air_friction", "disable_contact")
+              "max_newton_steps", "air_density", "air_friction", "disable_contact",
+              "constraint_tol")
 
 
 def scene_arrays(sc, params, steps, fix_schedule=None, delta_schedule=None, windows=None):
@@ -93,6 +94,15 @@ def scene_arrays(sc, params, steps, fix_schedule=None, delta_schedule=None, wind
         a["pin_schedule"] = f32(fix_schedule)
         a["pin_delta_schedule"] = f32(delta_schedule if delta_schedule is not None
                                       else np.zeros_like(fix_schedule))
+    if sc.get("spheres"):
+        a["spheres"] = f32([[*d["center"], d["ghat"], d["friction"], d["radius"],
+                             d.get("thickness", 0.0), float(d.get("bowl", False)),
+                             float(d.get("reverse", False)), float(d.get("kinematic", False))]
+                            for d in sc["spheres"]])
+    if sc.get("floors"):
+        a["floors"] = f32([[*d["ground"], d["ghat"], d["friction"], d.get("thickness", 0.0),
+                            *d.get("up", (0.0, 1.0, 0.0)), float(d.get("kinematic", False))]
+                           for d in sc["floors"]])
     st = sc.get("static")
     if st:
         a["s_x"], a["s_edge"], a["s_face"] = f32(st["x"]), u32(st["edge"]), u32(st["face"])

@@ -35,7 +35,7 @@ class ShellC(ctypes.Structure):
 class ElasticParamsC(ctypes.Structure):
     """ppf_elastic_params (include/ppf_mi355x.h)."""
     _fields_ = [("dt", f32), ("air_density", f32), ("air_friction", f32),
-                ("wind", f32 * 3)]
+                ("wind", f32 * 3), ("scene_dt", ctypes.c_void_p), ("vscene", ctypes.c_void_p)]
 
 
 def shell_to_device(sh, dev):

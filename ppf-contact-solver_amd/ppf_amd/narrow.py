@@ -34,7 +34,8 @@ class ContactMeshC(ctypes.Structure):
                 ("e2f_off", vp), ("e2f", vp),
                 ("v_ghat", vp), ("v_off", vp), ("e_ghat", vp), ("e_off", vp),
                 ("f_ghat", vp), ("f_off", vp),
-                ("vflag", vp), ("vbody", vp), ("eflag", vp), ("fflag", vp)]
+                ("vflag", vp), ("vbody", vp), ("eflag", vp), ("fflag", vp),
+                ("vscene", vp), ("nscene", u32)]
 
 
 def mesh_to_device(m, dev):
@@ -45,6 +46,10 @@ def mesh_to_device(m, dev):
     for k in INDEX_FIELDS:
         a = np.ascontiguousarray(m[k], np.uint32).view(np.int32)
         out[k] = torch.from_numpy(a).reshape(-1).to(dev)
+    if m.get("vscene") is not None:  # many scenes in one mesh
+        a = np.ascontiguousarray(m["vscene"], np.uint32).view(np.int32)
+        out["vscene"] = torch.from_numpy(a).reshape(-1).to(dev)
+        out["nscene"] = int(m["nscene"])
     return out
 
 
@@ -55,6 +60,9 @@ def mesh_struct(t):
         if not ten.is_cuda or not ten.is_contiguous():
             raise ValueError(f"{k}: expected a contiguous cuda tensor")
         fields[k] = ten.data_ptr() if ten.numel() else None
+    if t.get("vscene") is not None:
+        fields["vscene"] = t["vscene"].data_ptr()
+        fields["nscene"] = t["nscene"]
     return ContactMeshC(**fields)
 
 

@@ -669,7 +669,7 @@ def step_params(dt=1.0 / 60.0, fps=60.0, **kw):
              target_toi=0.25, line_search_max_t=1.25, ccd_eps=1e-7, max_dx=1.0,
              friction_eps=1e-5, friction_mode=0, min_newton_steps=0, max_newton_steps=2048,
              air_density=1e-3, air_friction=0.2, wind=(0.0, 0.0, 0.0), disable_contact=0,
-             check_intersection=1)
+             check_intersection=1, constraint_tol=0.01)
     p.update(kw)
     return p
 
@@ -750,3 +750,94 @@ def spin_fix_positions(sc, t, omega=2.0):
     fix[sc["cyl"], 1] = s * r[:, 0] + c * r[:, 1]
     fix[sc["cyl"], 2] = r[:, 2]
     return fix
+
+
+def pack_scenes(scs):
+    """Many independent scenes (drape_scene layout, each with a static mesh
+    or none) as one scene dict for a many-scene stepper: vertices, edges,
+    faces and hinges of scene c follow those of scene c - 1, every index is
+    shifted accordingly, and mesh['vscene'] / static['vscene'] give each
+    vertex's scene. Scene c's vertices are rows[c] .. rows[c + 1] of the
+    result (returned beside it)."""
+    ns = len(scs)
+    nvs = [int(sc["mesh"]["nv"]) for sc in scs]
+    nes = [int(sc["mesh"]["ne"]) for sc in scs]
+    nfs = [int(sc["mesh"]["nf"]) for sc in scs]
+    rows = np.concatenate([[0], np.cumsum(nvs)]).astype(np.int64)
+    eoff = np.concatenate([[0], np.cumsum(nes)]).astype(np.int64)
+    foff = np.concatenate([[0], np.cumsum(nfs)]).astype(np.int64)
+    cat = lambda parts, dt: np.ascontiguousarray(np.concatenate(parts), dt)
+
+    def csr(key_off, vals, row_base, val_base):
+        offs, cols = [], []
+        base = 0
+        for c, sc in enumerate(scs):
+            o = sc["mesh"][key_off].astype(np.int64)
+            offs.append(o[:-1] + base if c < ns - 1 else o + base)
+            cols.append(sc["mesh"][vals].astype(np.int64) + val_base[c])
+            base += int(o[-1])
+        return cat(offs, np.uint32), cat(cols, np.uint32)
+
+    m = dict(nv=int(rows[-1]), ne=int(eoff[-1]), nf=int(foff[-1]), nscene=ns)
+    m["x"] = cat([sc["mesh"]["x"].reshape(-1, 3) for sc in scs], np.float32)
+    m["edge"] = cat([sc["mesh"]["edge"].astype(np.int64) + rows[c] for c, sc in enumerate(scs)],
+                    np.uint32)
+    m["face"] = cat([sc["mesh"]["face"].astype(np.int64) + rows[c] for c, sc in enumerate(scs)],
+                    np.uint32)
+    m["v2e_off"], m["v2e"] = csr("v2e_off", "v2e", rows, eoff)
+    m["v2f_off"], m["v2f"] = csr("v2f_off", "v2f", rows, foff)
+    m["e2f_off"], m["e2f"] = csr("e2f_off", "e2f", eoff, foff)
+    for k in ("v_ghat", "v_off", "e_ghat", "e_off", "f_ghat", "f_off"):
+        m[k] = cat([sc["mesh"][k] for sc in scs], np.float32)
+    for k in ("vflag", "vbody", "eflag", "fflag"):
+        m[k] = cat([sc["mesh"][k] for sc in scs], np.uint32)
+    m["vscene"] = np.repeat(np.arange(ns), nvs).astype(np.uint32)
+
+    sh = dict(nv=m["nv"], nf=m["nf"])
+    nhs = [int(sc["shell"]["nh"]) for sc in scs]
+    sh["nh"] = int(sum(nhs))
+    sh["face"] = m["face"]
+    sh["hinge"] = cat([sc["shell"]["hinge"].astype(np.int64).reshape(-1, 4) + rows[c]
+                       for c, sc in enumerate(scs)], np.uint32)
+    offs, cols, base = [], [], 0
+    for c, sc in enumerate(scs):
+        o = sc["shell"]["v2f_off"].astype(np.int64)
+        offs.append(o[:-1] + base if c < ns - 1 else o + base)
+        cols.append(sc["shell"]["v2f"].astype(np.int64) + foff[c])
+        base += int(o[-1])
+    sh["v2f_off"], sh["v2f"] = cat(offs, np.uint32), cat(cols, np.uint32)
+    for k in ("vmass", "varea", "fmass", "inv_rest", "f_mu", "f_lambda", "h_rest", "h_length",
+              "h_area", "h_bend"):
+        sh[k] = cat([np.asarray(sc["shell"][k], np.float32).reshape(
+            (-1, 4) if k == "inv_rest" else (-1,)) for sc in scs], np.float32)
+    for k in ("vflag", "fflag", "hflag"):
+        sh[k] = cat([sc["shell"][k] for sc in scs], np.uint32)
+
+    statics = [sc.get("static") for sc in scs]
+    static = None
+    if any(s is not None for s in statics):
+        snv = [int(s["nv"]) if s is not None else 0 for s in statics]
+        srow = np.concatenate([[0], np.cumsum(snv)]).astype(np.int64)
+        parts = [s for s in statics if s is not None]
+        static = dict(nv=int(srow[-1]), ne=int(sum(int(s["ne"]) for s in parts)),
+                      nf=int(sum(int(s["nf"]) for s in parts)))
+        static["x"] = cat([s["x"].reshape(-1, 3) for s in parts], np.float32)
+        static["edge"] = cat([s["edge"].astype(np.int64) + srow[c]
+                              for c, s in enumerate(statics) if s is not None], np.uint32)
+        static["face"] = cat([s["face"].astype(np.int64) + srow[c]
+                              for c, s in enumerate(statics) if s is not None], np.uint32)
+        for k in ("v_ghat", "v_off", "e_ghat", "e_off", "f_ghat", "f_off", "v_fric", "e_fric",
+                  "f_fric"):
+            if all(k in s for s in parts):
+                static[k] = cat([s[k] for s in parts], np.float32)
+        static["vscene"] = np.repeat(np.arange(ns), snv).astype(np.uint32)
+
+    fric = {k: cat([sc["fric"][k] for sc in scs], np.float32) for k in ("v", "e", "f")}
+    fix = None
+    if any(sc.get("fix_pos") is not None for sc in scs):
+        fix = cat([sc["fix_pos"] if sc.get("fix_pos") is not None else sc["mesh"]["x"]
+                   for sc in scs], np.float32)
+    packed = dict(mesh=m, shell=sh, static=static,
+                  prev=cat([sc["prev"].reshape(-1, 3) for sc in scs], np.float32),
+                  emass=cat([sc["emass"] for sc in scs], np.float32), fric=fric, fix_pos=fix)
+    return packed, rows

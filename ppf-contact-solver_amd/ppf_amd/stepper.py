@@ -21,11 +21,44 @@ f32 = ctypes.c_float
 i32 = ctypes.c_int32
 
 
+class SphereC(ctypes.Structure):
+    """ppf_sphere (include/ppf_mi355x.h; data.rs:689-698)."""
+    _fields_ = [("center", f32 * 3), ("ghat", f32), ("friction", f32), ("radius", f32),
+                ("thickness", f32), ("bowl", ctypes.c_bool), ("reverse", ctypes.c_bool),
+                ("kinematic", ctypes.c_bool)]
+
+
+class FloorC(ctypes.Structure):
+    """ppf_floor (include/ppf_mi355x.h; data.rs:702-709)."""
+    _fields_ = [("ground", f32 * 3), ("ghat", f32), ("friction", f32), ("thickness", f32),
+                ("up", f32 * 3), ("kinematic", ctypes.c_bool)]
+
+
+def collider_arrays(t):
+    """ctypes arrays of the analytic colliders in t['spheres'] / t['floors']
+    (lists of dicts with the Sphere / Floor fields)."""
+    sp = t.get("spheres") or []
+    fl = t.get("floors") or []
+    a = (SphereC * max(1, len(sp)))()
+    for k, d in enumerate(sp):
+        a[k] = SphereC(center=(f32 * 3)(*d["center"]), ghat=d["ghat"], friction=d["friction"],
+                       radius=d["radius"], thickness=d.get("thickness", 0.0),
+                       bowl=bool(d.get("bowl", False)), reverse=bool(d.get("reverse", False)),
+                       kinematic=bool(d.get("kinematic", False)))
+    b = (FloorC * max(1, len(fl)))()
+    for k, d in enumerate(fl):
+        b[k] = FloorC(ground=(f32 * 3)(*d["ground"]), ghat=d["ghat"], friction=d["friction"],
+                      thickness=d.get("thickness", 0.0), up=(f32 * 3)(*d.get("up", (0, 1, 0))),
+                      kinematic=bool(d.get("kinematic", False)))
+    return a, len(sp), b, len(fl)
+
+
 class SceneC(ctypes.Structure):
     """ppf_scene (include/ppf_mi355x.h)."""
     _fields_ = [("mesh", ContactMeshC), ("shell", ShellC), ("collision", StaticMeshC),
                 ("prev", vp), ("emass", vp), ("v_fric", vp), ("e_fric", vp), ("f_fric", vp),
-                ("fix_pos", vp), ("fix_delta", vp)]
+                ("fix_pos", vp), ("fix_delta", vp), ("spheres", vp), ("nsphere", u32),
+                ("floors", vp), ("nfloor", u32)]
 
 
 class StepParamsC(ctypes.Structure):
@@ -35,7 +68,8 @@ class StepParamsC(ctypes.Structure):
                 ("ccd_eps", f32), ("max_dx", f32), ("friction_eps", f32),
                 ("friction_mode", i32), ("min_newton_steps", u32), ("max_newton_steps", u32),
                 ("air_density", f32), ("air_friction", f32), ("wind", f32 * 3),
-                ("disable_contact", i32), ("check_intersection", i32)]
+                ("disable_contact", i32), ("check_intersection", i32),
+                ("constraint_tol", f32)]
 
 
 class StepResultC(ctypes.Structure):
@@ -48,7 +82,7 @@ class StepResultC(ctypes.Structure):
 
 
 def params_struct(p):
-    kw = dict(p)
+    kw = {k: v for k, v in p.items() if k in dict(StepParamsC._fields_)}
     kw["gravity"] = (f32 * 3)(*p["gravity"])
     kw["wind"] = (f32 * 3)(*p["wind"])
     return StepParamsC(**kw)
@@ -77,6 +111,8 @@ def scene_to_device(sc, dev):
     t["fric"] = {k: f(v) for k, v in sc["fric"].items()}
     t["fix_pos"] = f(sc["fix_pos"]) if sc.get("fix_pos") is not None else None
     t["fix_delta"] = f(sc["fix_delta"]) if sc.get("fix_delta") is not None else None
+    t["spheres"] = list(sc.get("spheres") or [])
+    t["floors"] = list(sc.get("floors") or [])
     return t
 
 
@@ -155,8 +191,29 @@ class Stepper:
                                           _stream()), "ppf_stepper_system")
         return out
 
+    def advance_scenes(self, params, prev_dt):
+        """One time step of every scene of a many-scene stepper; prev_dt is
+        a sequence of one value per scene. Returns one result dict per scene."""
+        ns = int(self.t["mesh"]["nscene"])
+        p = params_struct(params)
+        pd = (f32 * ns)(*[float(v) for v in prev_dt])
+        r = (StepResultC * ns)()
+        check(self.lib.ppf_stepper_advance_scenes(self._h, ctypes.byref(self._sc), ctypes.byref(p),
+                                                  pd, r, _stream()), "ppf_stepper_advance_scenes")
+        return [result_dict(x) for x in r]
+
+    def _colliders(self):
+        # the analytic colliders as they are now (kinematic ones move)
+        self._coll = collider_arrays(self.t)
+        sp, nsp, fl, nfl = self._coll
+        self._sc.spheres = ctypes.cast(sp, vp)
+        self._sc.nsphere = nsp
+        self._sc.floors = ctypes.cast(fl, vp)
+        self._sc.nfloor = nfl
+
     def advance(self, params):
         p = params_struct(params)
+        self._colliders()
         r = StepResultC()
         check(self.lib.ppf_stepper_advance(self._h, ctypes.byref(self._sc), ctypes.byref(p),
                                            ctypes.byref(r), _stream()), "ppf_stepper_advance")

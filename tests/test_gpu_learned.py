@@ -194,3 +194,105 @@ def test_training_beats_block_jacobi_on_unseen_systems(dev):
     lpc.close()
     for p in plans:
         p.close()
+
+
+def contact_system(dev, res=12, layers=2, steps=4, seed=3):
+    """The last Newton system of a small stacked drape in contact (sheets on
+    the spinning cylinder, Dirichlet-eliminated pins): contact blocks beside
+    the elastic ones, as on the stacked-layer scene."""
+    from ppf_amd.stepper import Stepper, scene_to_device
+    sc = scenes.stacked_drape_scene(res=res, layers=layers, seed=seed, n_around=24, n_along=8,
+                                    clearance=1.2e-3)
+    ns = layers * res * res
+    t = scene_to_device(sc, dev)
+    t["fix_delta"] = torch.zeros_like(t["mesh"]["x"])
+    st = Stepper(t)
+    p = scenes.step_params()
+    clock, pairs = 0.0, 0
+    for _ in range(steps):
+        now = scenes.spin_fix_positions(sc, clock)[ns:].copy()
+        aim = scenes.spin_fix_positions(sc, clock + p["dt"])[ns:].copy()
+        t["fix_pos"].view(-1, 3)[ns:] = torch.from_numpy(aim).to(dev)
+        t["fix_delta"].view(-1, 3)[ns:] = torch.from_numpy(aim - now).to(dev)
+        r = st.advance(p)
+        assert r["status"] == 0, r
+        pairs = r["pairs_max"]
+        p["prev_dt"] = r["dt"]
+        clock += r["dt"]
+    s = st.system()
+    st.close()
+    h = {k: v.cpu().numpy() for k, v in s.items()}
+    h["n"] = len(h["row_offset"]) - 1
+    h["cdiag"] = h["cdiag"].reshape(-1, 9)
+    return h, pairs
+
+
+def test_forward_and_gradient_on_contact_system(dev):
+    """Forward outputs, factor and loss gradient on a stacked-drape contact
+    system (contact and Dirichlet rows beside the elastic ones) against the
+    fp64 oracle, to the same bars as on the cloth systems."""
+    s, pairs = contact_system(dev)
+    assert pairs > 0
+    s = {k: (np.asarray(v, np.int32) if k in ("row_offset", "col_index") else v)
+         for k, v in s.items()}
+    t = to_dev(s, dev)
+    w = oracle_weights(9)
+    plan = BlockJacobiPCG(t["row_offset"], t["col_index"])
+    lpc = LearnedPreconditioner(flatten(w))
+    lpc.factor(plan, t["value"], t["cdiag"], t["b"])
+    n, nnz = s["n"], len(s["col_index"])
+    td, ud, gd, gl = (a.cpu().numpy() for a in lpc.outputs(n, nnz, dev))
+    g = OL.system_graph(s)
+    x, f = OL.features(g, s["b"])
+    to, uo, _ = OL.forward(w, g, x, f)
+    m = slot_to_lower(s, g)
+    off = m >= 0
+    ud = ud.reshape(-1, 9)
+    terr = np.abs(td.reshape(-1, 6) - to).max() / np.abs(to).max()
+    uerr = np.abs(ud[off] - uo[m[off]]).max() / np.abs(uo).max()
+    Gd, Ge = OL.factor(to, uo)
+    Gdiag, Glow = OL.preconditioner(g, Gd, Ge)
+    gd = gd.reshape(-1, 3, 3).transpose(0, 2, 1)
+    gl = gl.reshape(-1, 3, 3).transpose(0, 2, 1)
+    derr = np.abs(gd - Gdiag).max() / np.abs(Gdiag).max()
+    lerr = np.abs(gl[off] - Glow[m[off]]).max() / np.abs(Gdiag).max()
+    z = OL.probe_vectors(n, 4, seed=12)
+    zt = torch.from_numpy(z.astype(np.float32)).to(dev).contiguous()
+    loss, grad = lpc.gradient(plan, t["value"], t["cdiag"], zt)
+    lo, go = OL.backward(w, g, x, f, z)
+    go = flatten(go).astype(np.float64)
+    cos = grad @ go / (np.linalg.norm(grad) * np.linalg.norm(go))
+    ratio = np.linalg.norm(grad) / np.linalg.norm(go)
+    print(f"contact system: n {n}, {nnz} blocks, {pairs} pairs; t {terr:.2e} u {uerr:.2e} "
+          f"Gd {derr:.2e} Gl {lerr:.2e}; loss {loss:.5f}/{lo:.5f} grad cos {cos:.5f} "
+          f"norm ratio {ratio:.4f}")
+    assert terr < 0.04 and uerr < 0.04
+    assert derr < 0.02 and lerr < 0.02
+    assert abs(loss - lo) < 0.02 * abs(lo) + 1e-3
+    assert cos > 0.98 and abs(ratio - 1.0) < 0.1
+    lpc.close()
+    plan.close()
+
+
+@pytest.mark.timeout(600)
+def test_training_on_stacked_contact_systems_unseen_seeds(dev):
+    """Trained on the Newton systems the block-Jacobi solver produces on the
+    stacked drape (8 sheets of 64 x 64 on the spinning cylinder, seeds 0-2),
+    the learned factor needs fewer PCG iterations than block Jacobi on the
+    contact systems of unseen seeds (4 and 5)."""
+    from ppf_amd import lpc_train
+    train = lpc_train.collect_drape_systems(dev, res=64, layers=8, seeds=(0, 1, 2), steps=12)
+    lpc, hist = lpc_train.train(train, steps=800, lr=2e-3)
+    del train
+    print(f"training loss {np.mean(hist[:20]):.4f} -> {np.mean(hist[-20:]):.4f}")
+    assert np.mean(hist[-20:]) < np.mean(hist[:20])
+    test = lpc_train.collect_drape_systems(dev, res=64, layers=8, seeds=(4, 5), steps=12,
+                                           every=3)
+    rows = lpc_train.compare(lpc, test)
+    for r in rows:
+        print(f"unseen: block Jacobi {r['bj_iter']} iterations, learned {r['lpc_iter']}")
+        assert r["bj_status"] == 0 and r["lpc_status"] == 0
+    ratio = sum(r["lpc_iter"] for r in rows) / sum(r["bj_iter"] for r in rows)
+    print(f"learned / block-Jacobi iterations on unseen seeds: {ratio:.3f}")
+    assert ratio < 0.8
+    lpc.close()
