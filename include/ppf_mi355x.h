@@ -208,6 +208,23 @@ int ppf_pcg_time_spmv(ppf_pcg_plan *plan, const float *d_value,
                       const float *d_cdiag, const float *d_x, float *d_y,
                       uint32_t reps, float *avg_ms, void *stream);
 
+/* ppf_pcg_plan_create + ppf_pcg_solve in one call that allocates nothing
+ * (no device, pinned or graph allocation; the caller passes inputs and
+ * outputs). No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture of their own, tight serving loops,
+ * harnesses that forbid hipMalloc). The plan's scratch lives in a fixed
+ * 32 MiB device region of the library, one per GPU, held until the process
+ * ends: it fits about 104 n + 256 n (the widest ELL table) + 24 nnz + 0.5 MiB
+ * bytes, e.g. 32,768 rows with 262,144 stored blocks; larger systems return
+ * PPF_ERR_ARG. Same kernels, tiles and iterates as a plan's solve (results
+ * equal bit for bit). Every call on a GPU shares that GPU's region: calls
+ * that overlap on one GPU (several host threads or streams) race. Blocks
+ * until the solve ends. */
+int ppf_pcg_solve_noalloc(uint32_t n, uint32_t nnz, const uint32_t *d_row_offset,
+                          const uint32_t *d_col_index, const float *d_value,
+                          const float *d_cdiag, const float *d_b, float *d_x, float tol,
+                          uint32_t max_iter, ppf_pcg_result *out, void *stream);
+
 /* Block-Jacobi inverse of n symmetric 3x3 blocks (solver::invert,
  * solver.cu:357-400). *d_bad (device uint32) counts blocks the reference
  * would trap on. */

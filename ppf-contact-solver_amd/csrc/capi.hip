@@ -405,6 +405,14 @@ int ppf_pcg_time_spmv(ppf_pcg_plan *plan, const float *d_value,
                          (hipStream_t)stream);
 }
 
+int ppf_pcg_solve_noalloc(uint32_t n, uint32_t nnz, const uint32_t *d_row_offset,
+                          const uint32_t *d_col_index, const float *d_value,
+                          const float *d_cdiag, const float *d_b, float *d_x, float tol,
+                          uint32_t max_iter, ppf_pcg_result *out, void *stream) {
+    return pcg_solve_noalloc(n, nnz, d_row_offset, d_col_index, d_value, d_cdiag, d_b, d_x,
+                             tol, max_iter, out, (hipStream_t)stream);
+}
+
 int ppf_bj_invert(uint32_t n, const float *d_in, float *d_out, uint32_t *d_bad,
                   void *stream) {
     if (n == 0) {

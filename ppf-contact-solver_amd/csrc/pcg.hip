@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -2149,6 +2150,109 @@ int pcg_plan_set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st
     }
     int rc = set_tiles(pl, r1, r2, st);
     return rc == PPF_OK ? capture_graph(pl) : rc;
+}
+
+} // namespace ppf
+
+// ---- the allocation-free solve ---------------------------------------------
+
+// Fixed device region of ppf_pcg_solve_noalloc: each GPU holds its own copy
+// from module load until the process ends. Host code never passes this name
+// to a kernel or a copy (on the host it is only a stand-in); it uses the
+// device address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[ppf::kNoAllocScratchBytes];
+
+namespace ppf {
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current.
+int scratch_region(void **out) {
+    int dev = 0;
+    PPF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("pcg_solve_noalloc: device %d beyond the %d the library tracks", dev,
+                  kMaxDevices);
+        return PPF_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error("pcg_solve_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return PPF_ERR_HIP;
+        }
+        g_scratch_dev[dev] = ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return PPF_OK;
+}
+
+// Upper bound of what plan_init carves for one scene of n rows and nnz
+// stored blocks: n-sized scratch, transpose table, sort keys, the widest
+// ELL table (32 entries a row) and tiles for the largest possible grid.
+size_t noalloc_bytes(uint32_t n, uint32_t nnz) {
+    constexpr size_t kMaxTiles = 8192; // 256 CUs x 32 workgroups
+    const size_t n3 = 3 * (size_t)n;
+    const size_t mem = carve_bytes(n + 1, 4) + carve_bytes(n, 4) +
+                       carve_bytes(9 * (size_t)n, 4) + carve_bytes(n3, 4) * 5 +
+                       carve_bytes(1, sizeof(Args)) + carve_bytes(scan_scratch_words(n + 1), 4) +
+                       carve_bytes(4, 4) + carve_bytes(2, 4);
+    const size_t pat = carve_bytes(2 * ((size_t)nnz + nnz / 4 + 256), 4);
+    const size_t sort_words = 3 * (size_t)nnz + radix_sort_scratch_words(nnz);
+    const size_t tsort = carve_bytes(sort_words + sort_words / 4 + 256, 4);
+    const size_t ell = carve_bytes((size_t)n * 32, sizeof(uint2));
+    const size_t tiles = carve_bytes(1, sizeof(Scalars)) + 2 * carve_bytes(2, 4) +
+                         2 * carve_bytes(kMaxTiles, sizeof(Tile)) +
+                         carve_bytes(4 * kMaxTiles, 4);
+    return mem + pat + tsort + ell + tiles;
+}
+
+} // namespace
+
+int pcg_solve_noalloc(uint32_t n, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                      const float *val, const float *cdiag, const float *b, float *x,
+                      float tol, uint32_t max_iter, ppf_pcg_result *res, hipStream_t st) {
+    if (n == 0 || !rp || !col || !val || !cdiag || !b || !x || !res || max_iter == 0 ||
+        nnz >= kMirror) {
+        set_error("ppf_pcg_solve_noalloc: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    const size_t need = noalloc_bytes(n, nnz);
+    if (need > kNoAllocScratchBytes) {
+        set_error("ppf_pcg_solve_noalloc: %u rows with %u stored blocks need %zu bytes of "
+                  "scratch, beyond the fixed %zu",
+                  n, nnz, need, (size_t)kNoAllocScratchBytes);
+        return PPF_ERR_ARG;
+    }
+    void *region = nullptr;
+    int rc = scratch_region(&region);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    Scalars status{};
+    PcgPlanImpl pl;
+    pl.arena = static_cast<char *>(region);
+    pl.arena_cap = kNoAllocScratchBytes;
+    pl.h_scal = &status;
+    rc = plan_init(&pl, n, nnz, rp, col, 1u, nullptr, st);
+    if (rc == PPF_OK) {
+        rc = pcg_solve(&pl, val, cdiag, b, x, tol, max_iter, nullptr, res, st, nullptr);
+    }
+    // The solve ends on a synchronous status read, so nothing queued still
+    // reads the region or `status`; error paths drain the stream before the
+    // next call may reuse them.
+    if (rc != PPF_OK) {
+        (void)hipStreamSynchronize(st);
+    }
+    return rc;
 }
 
 } // namespace ppf

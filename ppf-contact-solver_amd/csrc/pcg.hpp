@@ -66,5 +66,12 @@ int pcg_time_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
                   hipStream_t st);
 int pcg_spmv(PcgPlanImpl *pl, const float *val, const float *cdiag,
              const float *x, float *y, hipStream_t st);
+// Bytes of the fixed device region of the allocation-free solve.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+// Plan and solve in one call on that region (no device or pinned allocation,
+// no graph): the same kernels, tiles and results as a plan's solve.
+int pcg_solve_noalloc(uint32_t n, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                      const float *val, const float *cdiag, const float *b, float *x,
+                      float tol, uint32_t max_iter, ppf_pcg_result *res, hipStream_t st);
 
 } // namespace ppf
