@@ -181,8 +181,20 @@ uint32_t oracle_last_system(uint32_t *n, uint32_t *rp, uint32_t *ci, float *val,
     return (uint32_t)g_last.ci.size();
 }
 
+// Per Newton iteration of the last step: PCG iterations, step scale of the
+// max_dx clamp, CCD toi, largest |dx| (4 floats each), for the tests.
+static std::vector<float> g_trace;
+
+uint32_t oracle_step_trace(float *out, uint32_t cap) {
+    const uint32_t k = (uint32_t)(g_trace.size() / 4);
+    for (uint32_t i = 0; i < std::min(k, cap); ++i)
+        std::copy(g_trace.begin() + 4 * i, g_trace.begin() + 4 * i + 4, out + 4 * i);
+    return k;
+}
+
 // One step; mesh.x (curr) and prev are updated on success. Returns 0.
 int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_result *out) {
+    g_trace.clear();
     *out = oracle_step_result{};
     const uint32_t n = sc->mesh.nv;
     const size_t n3 = 3 * (size_t)n;
@@ -420,6 +432,7 @@ int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_resul
                 break;
             }
         }
+        g_trace.insert(g_trace.end(), {(float)pr.iter, recale, toi, max_dx});
         if (toi <= FLT_EPSILON) {
             out->status = 2;
             break;

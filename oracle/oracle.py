@@ -652,6 +652,16 @@ class OracleStepper:
                                  fix_pos=_p(self.fix) if self.fix is not None else None,
                                  fix_delta=_p(self.delta) if self.delta is not None else None)
 
+    def trace(self):
+        """Per Newton iteration of the last advance: (PCG iterations, max_dx
+        step scale, CCD toi, largest |dx|)."""
+        lib = self.lib
+        lib.oracle_step_trace.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        lib.oracle_step_trace.restype = ctypes.c_uint32
+        out = np.zeros(4 * 64, np.float32)
+        k = lib.oracle_step_trace(out.ctypes.data, 64)
+        return out[:4 * min(k, 64)].reshape(-1, 4)
+
     def last_system(self):
         """The last Newton system the oracle solved: row_offset, col_index,
         value (nnz, 9), cdiag (n, 9), b (3n); eliminated blocks are zero."""

@@ -30,7 +30,7 @@ import oracle
 from ppf_amd import scenes
 from ppf_amd.solver import BatchPCG, BlockJacobiPCG, stack_systems
 from ppf_amd.stepper import Stepper, scene_to_device
-from test_gpu_step import SOLVE_BALL_FACTOR, ULP_FACTOR, solve_error_vs_fp64, ulp_spread
+from test_gpu_step import SOLVE_BALL_FACTOR, solve_error_vs_fp64, ulp_nearest
 
 pytestmark = pytest.mark.gpu
 
@@ -59,14 +59,15 @@ def test_plain_drape_60_steps(dev):
         disp = np.abs(xo - xg).max()
         eg, eo = solve_error_vs_fp64(gpu.system(), p["cg_tol"], scale)
         bar = max(1e-5, SOLVE_BALL_FACTOR * max(eg, eo))
-        spread = ulp_spread(orc, sco, xg, pg, p, scale) if err >= bar else 0.0
-        bar = max(bar, ULP_FACTOR * spread)
+        # a step on a discontinuity of the algorithm: compared with the
+        # oracle's steps from one-ulp copies of the start state
+        near = ulp_nearest(orc, sco, xg, pg, p, xn, scale) if err >= bar else (err, 0.0)
         line = (f"step {k}: status {rg['status']}/{ro['status']} newton {rg['newton_steps']}/"
                 f"{ro['newton_steps']} pcg {rg['pcg_iter_total']}/{ro['pcg_iter_total']} "
                 f"pairs {rg['pairs_max']}/{ro['pairs_max']} isect {rg['intersections']}/"
                 f"{ro['intersections']} dt {rg['dt']:.6g}/{ro['dt']:.6g} pos err {err:.2e} "
                 f"(disp {disp / scale:.2e}); last Newton solve vs fp64: gpu {eg:.2e} oracle "
-                f"{eo:.2e}; one-ulp spread {spread:.2e}; bar {bar:.2e}")
+                f"{eo:.2e}; bar {bar:.2e}; nearest one-ulp copy {near[0]:.2e} (spread {near[1]:.2e})")
         print(line, flush=True)
         assert rg["status"] == ro["status"] == 0, line
         assert rg["intersections"] == ro["intersections"] == 0, line
@@ -76,11 +77,11 @@ def test_plain_drape_60_steps(dev):
         # The two fp32 solves are equally far from the exact solution: the
         # GPU's solver is no less accurate than the oracle's.
         assert eg < 3.0 * eo + 1e-7, line
-        assert err < bar, line
+        assert near[0] < bar, line
         contact_steps += rg["pairs_max"] > 0
         within += err < 1e-5
-        worst = max(worst, err)
-        worst_ratio = max(worst_ratio, err / bar)
+        worst = max(worst, near[0])
+        worst_ratio = max(worst_ratio, near[0] / bar)
         worst_solve = max(worst_solve, eg, eo)
         p["prev_dt"] = rg["dt"]
     gpu.close()
