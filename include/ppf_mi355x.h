@@ -186,6 +186,22 @@ int ppf_pcg_plan_create_batch(uint32_t n, uint32_t nnz, const uint32_t *d_row_of
 int ppf_pcg_plan_set_tiles(ppf_pcg_plan *plan, uint32_t spmv_rows, uint32_t vector_rows,
                            void *stream);
 
+/* SpMV forms of a plan. ELL: a padded per-row entry table (rows of at most 32
+ * stored + mirrored blocks). SPLIT: the pattern and its transpose table read
+ * directly. SYM (one-scene plans): the split form, except that a stored
+ * block whose mirror row lies in the same workgroup tile is read once and
+ * its transposed product handed over through LDS; results are bit-identical
+ * to SPLIT. AUTO (the default) picks per pattern: ELL when the widest row is
+ * at most twice the average, else SYM on a one-scene plan and SPLIT on a
+ * batch. */
+enum { PPF_SPMV_AUTO = 0, PPF_SPMV_ELL = 1, PPF_SPMV_SPLIT = 2, PPF_SPMV_SYM = 3 };
+
+/* Fix the SpMV form of a plan (PPF_SPMV_*) and apply it to the current
+ * pattern; *form_in_use (may be null) receives the form the plan now runs.
+ * Measurement entry point, no reference counterpart (the reference has one
+ * operator form, solver.cu:213-259). */
+int ppf_pcg_plan_set_spmv_form(ppf_pcg_plan *plan, int form, int *form_in_use, void *stream);
+
 /* Autotuner: time the same capped solve (24 iterations) of this system
  * under candidate tile sizes (SpMV 128..2048 rows, vector kernels 256..2048)
  * and keep the fastest. h_tile_rows[2] receives the chosen sizes and *h_ms

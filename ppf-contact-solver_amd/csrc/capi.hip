@@ -342,6 +342,18 @@ int ppf_pcg_plan_set_tiles(ppf_pcg_plan *plan, uint32_t spmv_rows, uint32_t vect
     return pcg_plan_set_tiles(plan->impl, spmv_rows, vector_rows, (hipStream_t)stream);
 }
 
+int ppf_pcg_plan_set_spmv_form(ppf_pcg_plan *plan, int form, int *form_in_use, void *stream) {
+    if (!plan) {
+        set_error("ppf_pcg_plan_set_spmv_form: plan is null");
+        return PPF_ERR_ARG;
+    }
+    const int rc = pcg_plan_set_spmv_form(plan->impl, form, (hipStream_t)stream);
+    if (form_in_use) {
+        *form_in_use = pcg_plan_spmv_form(plan->impl);
+    }
+    return rc;
+}
+
 int ppf_pcg_plan_autotune(ppf_pcg_plan *plan, const float *d_value, const float *d_cdiag,
                           const float *d_b, float tol, uint32_t *h_tile_rows, float *h_ms,
                           void *stream) {

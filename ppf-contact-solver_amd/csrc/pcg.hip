@@ -50,6 +50,10 @@ constexpr unsigned kNone = 0xffffffffu;
 constexpr unsigned kMirror = 0x80000000u; // entry slot flag: apply transposed
 constexpr float kNoiseSlack = 256.0f;       // solver.cu:69
 constexpr float kFloatEps = 1.19209290e-7f; // solver.cu:70
+// In-tile mirrors the symmetric SpMV form parks in LDS per workgroup (3
+// floats each: 24 KB); a tile's mirrors beyond it are read as in the split
+// form.
+constexpr uint32_t kSymCap = 2048;
 
 // Device-resident scalars of one solve. Written by exactly one thread of one
 // kernel and read only by later kernels (stream order), except `done`, whose
@@ -88,6 +92,9 @@ struct Args {
     const uint32_t *rp, *col, *tp, *tcol, *tslot, *dslot;
     const uint2 *ent;  // fixed-width entry table (ELL form), or null
     uint32_t ell_m;    // entries per lane of the ELL kernel (row width 4*m)
+    // symmetric form: per stored slot / per mirror entry, the entry's place
+    // among its tile's in-tile mirrors (LDS), or kNone
+    const uint32_t *mq, *tq;
     const float *val, *cdiag, *b;
     float *x, *P, *r, *z, *p, *Ap;
     float *part; // [G1 pAp][G1 |terms|][G2 rz][G2 err], one per tile
@@ -486,6 +493,232 @@ __global__ __launch_bounds__(kNT) void k_spmv_split(const Args *__restrict__ A,
                 if (sl[m] == kNone) {
                     t = mk3(0, 0, 0);
                 }
+                sum = sum + t;
+                as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+            }
+        }
+        if (q == 3) {
+            const f3 t = mv(cd + 9 * (size_t)ii, vi);
+            sum = sum + t;
+            as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+        }
+#pragma unroll
+        for (int off = 2; off > 0; off >>= 1) {
+            sum.x += __shfl_xor(sum.x, off, 4);
+            sum.y += __shfl_xor(sum.y, off, 4);
+            sum.z += __shfl_xor(sum.z, off, 4);
+            if (DOT) {
+                as.x += __shfl_xor(as.x, off, 4);
+                as.y += __shfl_xor(as.y, off, 4);
+                as.z += __shfl_xor(as.z, off, 4);
+            }
+        }
+        if (valid && q == 0) {
+            store3(y + 3 * (size_t)i, sum);
+            if (DOT) {
+                dot = dot + vi.x * sum.x;
+                dot = dot + vi.y * sum.y;
+                dot = dot + vi.z * sum.z;
+                adot = adot + fabsf(vi.x) * as.x;
+                adot = adot + fabsf(vi.y) * as.y;
+                adot = adot + fabsf(vi.z) * as.z;
+            }
+        }
+    }
+    if (DOT) {
+        dot = block_sum<kNT>(dot, lds);
+        adot = block_sum<kNT>(adot, lds);
+        if (threadIdx.x == 0) {
+            gptr<float> part = as_global(A->part);
+            part[L] = dot;
+            part[A->G1 + L] = adot;
+        }
+    }
+}
+
+// Tables of the symmetric form for the tiles of tile1 (one workgroup per
+// tile). A tile's mirror entries are contiguous in the transpose table (it is
+// ordered by destination row); an entry whose source row lies in the tile too
+// gets the next place among the tile's in-tile mirrors, while places last
+// (kSymCap), and both the entry (tq) and the stored slot it mirrors (mq) hold
+// it. Every other entry, and every stored slot not mirrored in its tile, holds
+// kNone (mq is preset to kNone). *max_inner receives the largest count.
+__global__ __launch_bounds__(kNT) void k_sym_prep(const Args *__restrict__ A,
+                                                  uint32_t *__restrict__ mq,
+                                                  uint32_t *__restrict__ tq,
+                                                  uint32_t *__restrict__ max_inner) {
+    __shared__ uint32_t wsum[kNT / kWave];
+    const Tile T = load_tile(A->tile1, blockIdx.x);
+    if (T.lo >= T.hi) {
+        return;
+    }
+    gptr<const uint32_t> tp = as_global(A->tp);
+    gptr<const uint32_t> tcol = as_global(A->tcol);
+    gptr<const uint32_t> tslot = as_global(A->tslot);
+    const uint32_t tb = tp[T.lo], te = tp[T.hi];
+    const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    uint32_t running = 0;
+    for (uint32_t c = tb; c < te; c += kNT) {
+        const uint32_t t = c + threadIdx.x;
+        const bool in = t < te && tcol[t] >= T.lo;
+        const uint64_t ball = __ballot(in);
+        const uint32_t below = (uint32_t)__popcll(ball & ((1ull << lane) - 1ull));
+        if (lane == 0) {
+            wsum[w] = (uint32_t)__popcll(ball);
+        }
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < kNT / kWave; ++k) {
+            before += k < (int)w ? wsum[k] : 0u;
+            total += wsum[k];
+        }
+        __syncthreads();
+        if (t < te) {
+            const uint32_t pos = running + before + below;
+            const uint32_t v = in && pos < kSymCap ? pos : kNone;
+            tq[t] = v;
+            if (v != kNone) {
+                mq[tslot[t]] = v;
+            }
+        }
+        running += total;
+    }
+    if (threadIdx.x == 0) {
+        atomicMax(max_inner, running);
+    }
+}
+
+// Symmetric form of k_spmv_split for one-scene plans. A stored block (i, j)
+// whose mirror row j lies in the same tile is read once: row i's lanes apply
+// it to v_j for row i and, transposed, to v_i for row j, and park the second
+// product in LDS (mq), where row j's lanes take it (tq) instead of reading the
+// block and v_i again. Rows go in chunks of 64; a chunk's stored entries come
+// before a barrier and its mirrored entries after it (mirrors come from rows
+// above, so from this chunk or an earlier one). Every product is the split
+// form's float expression and every lane sums its entries in the split form's
+// order (lane q takes entries q, q+4, ... of the row's stored-then-mirrored
+// list), so y is bit-identical to k_spmv_split.
+template <bool DOT, int M>
+__global__ __launch_bounds__(kNT) void k_spmv_sym(const Args *__restrict__ A,
+                                                  const float *__restrict__ vin,
+                                                  float *__restrict__ yout) {
+    extern __shared__ float mir[]; // 3 floats per in-tile mirror (kSymCap)
+    constexpr uint32_t kRows = kNT / 4;
+    __shared__ float lds[kNT / kWave];
+    const uint32_t L = xcd_logical_block();
+    const Tile T = load_tile(A->tile1, L);
+    if (DOT && as_global(A->s + T.scene)->done) {
+        return;
+    }
+    const uint32_t q = threadIdx.x & 3u, sub = threadIdx.x >> 2;
+    gptr<const uint32_t> rp = as_global(A->rp);
+    gptr<const uint32_t> col = as_global(A->col);
+    gptr<const uint32_t> tp = as_global(A->tp);
+    gptr<const uint32_t> tcol = as_global(A->tcol);
+    gptr<const uint32_t> tslot = as_global(A->tslot);
+    gptr<const uint32_t> mq = as_global(A->mq);
+    gptr<const uint32_t> tq = as_global(A->tq);
+    gptr<const float> val = as_global(A->val);
+    gptr<const float> cd = as_global(A->cdiag);
+    gptr<const float> v = as_global(vin);
+    gptr<float> y = as_global(yout);
+    const uint32_t lo = T.lo, hi = T.hi;
+    float dot = 0.0f, adot = 0.0f;
+    for (uint32_t base = lo; base < hi; base += kRows) {
+        const uint32_t i = base + sub;
+        const bool valid = i < hi;
+        const uint32_t ii = valid ? i : lo;
+        const uint32_t r0 = rp[ii], t0 = tp[ii];
+        const uint32_t ns = valid ? rp[ii + 1] - r0 : 0u;
+        const uint32_t nm = valid ? tp[ii + 1] - t0 : 0u;
+        const f3 vi = load3(v + 3 * (size_t)ii);
+        f3 sum = mk3(0, 0, 0), as = mk3(0, 0, 0);
+        // stored entries: the row's own products, and the transposed ones of
+        // in-tile mirrors into LDS
+        for (uint32_t k0 = 0; k0 < ns; k0 += 4 * M) {
+            uint32_t cj[M], sl[M], mp[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const uint32_t k = k0 + q + 4 * m;
+                const bool ok = k < ns;
+                sl[m] = ok ? r0 + k : kNone;
+                cj[m] = ok ? col[r0 + k] : 0u;
+                mp[m] = ok ? mq[r0 + k] : kNone;
+            }
+            float blk[M][9];
+            f3 vj[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                gptr<const float> a = val + 9 * (size_t)(sl[m] == kNone ? 0u : sl[m]);
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    blk[m][c] = a[c];
+                }
+                vj[m] = load3(v + 3 * (size_t)cj[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (sl[m] != kNone) {
+                    const f3 t = mv(blk[m], vj[m]);
+                    sum = sum + t;
+                    as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+                    if (mp[m] != kNone) {
+                        const f3 u = mvt(blk[m], vi);
+                        mir[3 * mp[m]] = u.x;
+                        mir[3 * mp[m] + 1] = u.y;
+                        mir[3 * mp[m] + 2] = u.z;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // mirrored entries: entry ns + u belongs to lane (ns + u) % 4
+        for (uint32_t u0 = (q + 4u - (ns & 3u)) & 3u; u0 < nm; u0 += 4 * M) {
+            uint32_t tv[M], src[M], sl[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const uint32_t u = u0 + 4 * m;
+                const bool ok = u < nm;
+                // the three loads are independent (no extra dependent round
+                // trip for the entries read from memory)
+                tv[m] = ok ? tq[t0 + u] : kNone;
+                src[m] = ok ? tcol[t0 + u] : 0u;
+                sl[m] = ok ? tslot[t0 + u] : kNone;
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (tv[m] != kNone) {
+                    sl[m] = kMirror; // taken from LDS
+                }
+            }
+            float blk[M][9];
+            f3 vj[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (sl[m] != kNone && sl[m] != kMirror) {
+                    gptr<const float> a = val + 9 * (size_t)sl[m];
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) {
+                        blk[m][c] = a[c];
+                    }
+                    vj[m] = load3(v + 3 * (size_t)src[m]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) {
+                        blk[m][c] = 0.0f;
+                    }
+                    vj[m] = mk3(0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (sl[m] == kNone) {
+                    continue;
+                }
+                const f3 t = sl[m] == kMirror
+                                 ? mk3(mir[3 * tv[m]], mir[3 * tv[m] + 1], mir[3 * tv[m] + 2])
+                                 : mvt(blk[m], vj[m]);
                 sum = sum + t;
                 as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
             }
@@ -1051,12 +1284,12 @@ uint32_t round8(uint32_t v) { return (v + 7u) & ~7u; }
 
 // Workgroups of kNT threads of `kernel` the current device holds at once
 // (CUs x resident workgroups per CU), or `fallback` if the runtime cannot say.
-template <class K> uint32_t resident_blocks(K kernel, uint32_t fallback) {
+template <class K> uint32_t resident_blocks(K kernel, uint32_t fallback, size_t lds = 0) {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
             hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kNT, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kNT, lds) != hipSuccess ||
         cus <= 0 || per_cu <= 0) {
         return fallback;
     }
@@ -1086,6 +1319,12 @@ struct PcgPlanImpl {
     uint32_t graph_key = 0xffffffffu; // SpMV form the graph was captured for
     bool split = false;        // short rows read the pattern itself (no ELL table)
     uint32_t split_m = 4;      // entries per lane per pass of the split form
+    // symmetric form (one-scene plans): in-tile mirrors through LDS
+    bool sym = false;
+    int form = PPF_SPMV_AUTO;  // the caller's choice of form
+    uint32_t *symmem = nullptr; // mq (nnz) | tq (nnz), grown with nnz
+    size_t sym_cap = 0;
+    const uint32_t *user_rp = nullptr, *user_col = nullptr; // the caller's pattern
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
     // scenes (a plain plan is one scene) and their tiles
@@ -1146,14 +1385,25 @@ hipError_t plan_free(PcgPlanImpl *pl, void *p) {
 
 // The SpMV kernel a plan launches (its graph is captured for one).
 uint32_t spmv_key(const PcgPlanImpl &pl) {
-    return (pl.h.Gd ? 1000u : 0u) + (pl.split ? 100u + pl.split_m : pl.h.ell_m);
+    return (pl.h.Gd ? 1000u : 0u) +
+           (pl.sym ? 200u + pl.split_m : pl.split ? 100u + pl.split_m : pl.h.ell_m);
 }
+
+constexpr size_t kSymLds = kSymCap * 3 * sizeof(float);
 
 // y = M v with the plan's SpMV form (ELL width 4*m, or the CSR walk).
 template <bool DOT>
 hipError_t launch_spmv(const PcgPlanImpl &pl, const float *v, float *y,
                        hipStream_t st) {
     const Args &h = pl.h;
+    if (pl.sym) {
+        switch (pl.split_m) {
+        case 2: k_spmv_sym<DOT, 2><<<h.G1, kNT, kSymLds, st>>>(pl.d_args, v, y); break;
+        case 3: k_spmv_sym<DOT, 3><<<h.G1, kNT, kSymLds, st>>>(pl.d_args, v, y); break;
+        default: k_spmv_sym<DOT, 4><<<h.G1, kNT, kSymLds, st>>>(pl.d_args, v, y); break;
+        }
+        return hipGetLastError();
+    }
     if (pl.split) {
         switch (pl.split_m) {
         case 2: k_spmv_split<DOT, 2><<<h.G1, kNT, 0, st>>>(pl.d_args, v, y); break;
@@ -1293,11 +1543,34 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
     // the split form,
     // whose pass covers the average row (stored + mirrored entries, 2 nnz - n
     // with every diagonal stored) and takes long rows in further passes.
+    // A one-scene plan on the device allocator takes the split form's
+    // symmetric variant (in-tile mirrors through LDS) wherever it would take
+    // the split form. The caller may fix the form instead.
     {
         const double avg = n ? (2.0 * nnz - n) / n : 0.0;
         const uint32_t m = (uint32_t)((avg + 1.0 + 3.999) / 4.0);
         pl->split_m = m < 2 ? 2 : (m > 4 ? 4 : m);
-        pl->split = !(h.ell_m && 4.0 * h.ell_m <= 2.0 * avg);
+        const bool sym_ok = pl->nscene == 1 && !pl->arena;
+        switch (pl->form) {
+        case PPF_SPMV_ELL:
+            if (!h.ell_m) {
+                set_error("pcg: rows of up to %u entries do not fit the ELL form (32 at most)",
+                          maxlen);
+                return PPF_ERR_ARG;
+            }
+            pl->split = false;
+            break;
+        case PPF_SPMV_SPLIT: pl->split = true; break;
+        case PPF_SPMV_SYM:
+            if (!sym_ok) {
+                set_error("pcg: the symmetric SpMV form needs a one-scene plan");
+                return PPF_ERR_ARG;
+            }
+            pl->split = true;
+            break;
+        default: pl->split = !(h.ell_m && 4.0 * h.ell_m <= 2.0 * avg); break;
+        }
+        pl->sym = pl->split && sym_ok && pl->form != PPF_SPMV_SPLIT;
     }
     h.ent = nullptr;
     if (h.ell_m && !pl->split) {
@@ -1318,13 +1591,51 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
     return PPF_OK;
 }
 
+// Tables of the symmetric form for the current pattern and tiles.
+int build_sym(PcgPlanImpl *pl, hipStream_t st) {
+    Args &h = pl->h;
+    h.mq = h.tq = nullptr;
+    if (!pl->sym || !h.tile1) {
+        return PPF_OK;
+    }
+    const size_t need = 2 * (size_t)std::max(pl->nnz, 1u);
+    if (need > pl->sym_cap) {
+        if (pl->symmem) {
+            PPF_HIP(dev_free(pl->symmem));
+            pl->symmem = nullptr;
+            pl->sym_cap = 0;
+        }
+        const size_t cap = 2 * grow_cap(std::max(pl->nnz, 1u));
+        PPF_HIP(dev_alloc(&pl->symmem, cap * sizeof(uint32_t)));
+        pl->sym_cap = cap;
+    }
+    uint32_t *mq = pl->symmem, *tq = pl->symmem + pl->sym_cap / 2;
+    PPF_HIP(hipMemsetAsync(mq, 0xff, (size_t)std::max(pl->nnz, 1u) * sizeof(uint32_t), st));
+    PPF_HIP(hipMemsetAsync(pl->flags + 3, 0, sizeof(uint32_t), st));
+    h.mq = mq;
+    h.tq = tq;
+    PPF_HIP(hipMemcpyAsync(pl->d_args, &h, sizeof(Args), hipMemcpyHostToDevice, st));
+    k_sym_prep<<<h.G1, kNT, 0, st>>>(pl->d_args, mq, tq, pl->flags + 3);
+    PPF_HIP(hipGetLastError());
+    return PPF_OK;
+}
+
 // The plan's pattern: the caller's, or the caller's rebuilt in the plan's row
 // order (the original entries are checked the same way either way).
+int set_pattern_ordered(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                        hipStream_t st);
+
 int set_pattern(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
                 hipStream_t st) {
-    if (!pl->order) {
-        return set_pattern_arrays(pl, nnz, rp, col, st);
-    }
+    pl->user_rp = rp;
+    pl->user_col = col;
+    int rc = pl->order ? set_pattern_ordered(pl, nnz, rp, col, st)
+                       : set_pattern_arrays(pl, nnz, rp, col, st);
+    return rc == PPF_OK ? build_sym(pl, st) : rc;
+}
+
+int set_pattern_ordered(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const uint32_t *col,
+                        hipStream_t st) {
     const uint32_t n = pl->n;
     if (nnz >= kMirror) {
         set_error("pcg: %u stored blocks exceed the entry index range", nnz);
@@ -1416,8 +1727,12 @@ int set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st,
         // One round of resident workgroups per kernel (a second, partly
         // filled round costs as much as the first): the SpMV's register use
         // allows 5 workgroups per CU, the vector kernels 8.
-        const uint32_t g1 =
-            round8(std::min(resident_blocks(k_spmv_split<true, 4>, 1280u), cdiv(n, 64)));
+        uint32_t res1 = resident_blocks(k_spmv_split<true, 4>, 1280u);
+        if (!pl->arena) {
+            // the symmetric form's LDS may hold fewer workgroups per CU
+            res1 = std::min(res1, resident_blocks(k_spmv_sym<true, 4>, 1280u, kSymLds));
+        }
+        const uint32_t g1 = round8(std::min(res1, cdiv(n, 64)));
         const uint32_t g2 = std::min(resident_blocks(k_update<false>, 2048u), cdiv(n, 64));
         r1 = cdiv(cdiv(n, g1), 64) * 64;
         r2 = cdiv(cdiv(n, g2), 64) * 64;
@@ -1472,7 +1787,7 @@ int set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st,
     PPF_HIP(hipMemcpyAsync(dsc2, sc2.data(), sc2.size() * 4, hipMemcpyHostToDevice, st));
     PPF_HIP(hipStreamSynchronize(st)); // the host vectors end here
     pl->graph_key = 0xffffffffu;      // grids changed: capture again
-    return PPF_OK;
+    return build_sym(pl, st);
 }
 
 // Capture kGraphIters iterations; they read every pointer through d_args, so
@@ -1731,6 +2046,9 @@ void pcg_plan_destroy(PcgPlanImpl *pl) {
     }
     if (pl->emem) {
         (void)dev_free(pl->emem);
+    }
+    if (pl->symmem) {
+        (void)dev_free(pl->symmem);
     }
     delete pl;
 }
@@ -2141,6 +2459,29 @@ int pcg_order_system(PcgPlanImpl *pl, const float *val, const float *cdiag, cons
     PPF_HIP(hipGetLastError());
     *oval = pl->pval, *ocdiag = pl->pcd, *ob = b ? pl->pb : nullptr;
     return PPF_OK;
+}
+
+int pcg_plan_set_spmv_form(PcgPlanImpl *pl, int form, hipStream_t st) {
+    if (!pl || form < PPF_SPMV_AUTO || form > PPF_SPMV_SYM) {
+        set_error("pcg_plan_set_spmv_form: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    const int old = pl->form;
+    pl->form = form;
+    int rc = set_pattern(pl, pl->nnz, pl->user_rp, pl->user_col, st);
+    if (rc != PPF_OK) {
+        pl->form = old;
+        (void)set_pattern(pl, pl->nnz, pl->user_rp, pl->user_col, st);
+        return rc;
+    }
+    if (!pl->arena && spmv_key(*pl) != pl->graph_key) {
+        rc = capture_graph(pl);
+    }
+    return rc;
+}
+
+int pcg_plan_spmv_form(const PcgPlanImpl *pl) {
+    return pl->sym ? PPF_SPMV_SYM : pl->split ? PPF_SPMV_SPLIT : PPF_SPMV_ELL;
 }
 
 int pcg_plan_set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st) {

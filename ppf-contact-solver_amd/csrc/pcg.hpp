@@ -32,6 +32,10 @@ int pcg_order_system(PcgPlanImpl *pl, const float *val, const float *cdiag, cons
 // Rows per tile of the SpMV and of the vector kernels (every scene cut into
 // tiles; 0, 0: one resident round over a one-scene plan).
 int pcg_plan_set_tiles(PcgPlanImpl *pl, uint32_t r1, uint32_t r2, hipStream_t st);
+// Fix the SpMV form (PPF_SPMV_*; AUTO: the plan's own choice per pattern) and
+// re-apply it to the current pattern; the form in use.
+int pcg_plan_set_spmv_form(PcgPlanImpl *pl, int form, hipStream_t st);
+int pcg_plan_spmv_form(const PcgPlanImpl *pl);
 // Measure candidate tile sizes on this system and keep the fastest.
 int pcg_plan_autotune(PcgPlanImpl *pl, const float *val, const float *cdiag, const float *b,
                       float tol, uint32_t *best_r1, uint32_t *best_r2, float *best_ms,

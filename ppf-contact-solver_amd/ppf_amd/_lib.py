@@ -57,6 +57,8 @@ _PROTOS = {
     "ppf_pcg_spmv": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
     "ppf_pcg_time_spmv": (ctypes.c_int, [vp, vp, vp, vp, vp, u32,
                                          ctypes.POINTER(f32), vp]),
+    "ppf_pcg_plan_set_spmv_form": (ctypes.c_int, [vp, ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_int), vp]),
     "ppf_pcg_solve_noalloc": (ctypes.c_int, [u32, u32, vp, vp, vp, vp, vp, vp, f32, u32,
                                              ctypes.POINTER(PcgResult), vp]),
     "ppf_bj_invert": (ctypes.c_int, [u32, vp, vp, vp, vp]),

@@ -68,6 +68,16 @@ class BlockJacobiPCG:
         check(self.lib.ppf_pcg_plan_set_order(self._h, _ptr(order) if order is not None
                                               else None, _stream()), "ppf_pcg_plan_set_order")
 
+    FORMS = {"auto": 0, "ell": 1, "split": 2, "sym": 3}
+
+    def set_form(self, form):
+        """Fix the SpMV form ('auto', 'ell', 'split', 'sym'); returns the form
+        the plan now runs."""
+        used = ctypes.c_int()
+        check(self.lib.ppf_pcg_plan_set_spmv_form(self._h, self.FORMS[form], ctypes.byref(used),
+                                                  _stream()), "ppf_pcg_plan_set_spmv_form")
+        return {v: k for k, v in self.FORMS.items()}[used.value]
+
     def close(self):
         if getattr(self, "_h", None):
             self.lib.ppf_pcg_plan_destroy(self._h)

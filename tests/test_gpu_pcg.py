@@ -332,3 +332,59 @@ def test_pcg_full_size_grid_converges(dev):
     bb = s["b"].astype(np.float64)
     true = np.abs(bb - M @ a["x"].astype(np.float64)).sum() / np.abs(bb).sum()
     assert true < 1.5e-3
+
+
+def _contact_system(dev):
+    """The last Newton system of a draped sheet in contact (stepper), and the
+    stepper's locality row order."""
+    from ppf_amd.stepper import Stepper, scene_to_device
+    sc = scenes.drape_scene(res=40, height=0.002)
+    st = Stepper(scene_to_device(sc, dev))
+    p = scenes.step_params()
+    for _ in range(4):
+        r = st.advance(p)
+        assert r["status"] == 0
+        p["prev_dt"] = r["dt"]
+    s = {k: v.cpu().numpy() for k, v in st.system().items()}
+    order = st.solve_order().cpu().numpy().astype(np.int32)
+    st.close()
+    return s, order
+
+
+@pytest.mark.parametrize("kind", ["contact", "contact_ordered", "longrows", "grid_split"])
+@pytest.mark.parametrize("tiles", [None, 64, 8192])
+def test_sym_form_bitwise_equals_split(dev, kind, tiles):
+    """The symmetric SpMV form (in-tile mirrors handed over through LDS) is
+    bit-identical to the split form: the product, and every iterate of a
+    whole solve. Tiles of 64 rows put most mirrors outside their tile; tiles
+    of 8,192 rows overflow the LDS places, so the tables' cap is crossed."""
+    if kind.startswith("contact"):
+        s, order = _contact_system(dev)
+    elif kind == "longrows":
+        s, order = scenes.long_row_spd_system(n=3000, hubs=20, fan=40, seed=71), None
+    else:
+        s, order = scenes.grid_spd_system(64, seed_matrix=5, seed_rhs=6), None
+    t = to_dev(s, dev)
+    plan = BlockJacobiPCG(t["row_offset"], t["col_index"])
+    if kind == "contact_ordered":
+        plan.set_order(torch.from_numpy(order).to(dev))
+        plan.repattern(t["row_offset"], t["col_index"])
+    if tiles:
+        from ppf_amd.solver import _stream
+        assert plan.lib.ppf_pcg_plan_set_tiles(plan._h, tiles, tiles, _stream()) == 0
+    out = {}
+    for form in ("split", "sym"):
+        assert plan.set_form(form) == form
+        y = plan.spmv(t["value"], t["cdiag"], t["b"]).cpu().numpy()
+        x = torch.zeros_like(t["b"])
+        r = plan.solve(t["value"], t["cdiag"], t["b"], x)
+        assert r.status == 0
+        out[form] = (y, r.iter, r.resid, x.cpu().numpy())
+    assert plan.set_form("auto") in ("ell", "sym")
+    plan.close()
+    (ya, ia, ra, xa), (yb, ib, rb, xb) = out["split"], out["sym"]
+    print(f"{kind} tiles {tiles}: n {len(s['row_offset']) - 1} nnz {len(s['col_index'])} "
+          f"iterations {ia}/{ib}")
+    assert np.array_equal(ya.view(np.uint32), yb.view(np.uint32))
+    assert ia == ib and ra == rb
+    assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32))
