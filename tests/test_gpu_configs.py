@@ -30,7 +30,8 @@ import oracle
 from ppf_amd import scenes
 from ppf_amd.solver import BatchPCG, BlockJacobiPCG, stack_systems
 from ppf_amd.stepper import Stepper, scene_to_device
-from test_gpu_step import SOLVE_BALL_FACTOR, solve_error_vs_fp64, ulp_nearest
+from test_gpu_step import (SOLVE_BALL_FACTOR, solve_error_vs_fp64, ulp_nearest,
+                           within_ulp_copies)
 
 pytestmark = pytest.mark.gpu
 
@@ -77,11 +78,11 @@ def test_plain_drape_60_steps(dev):
         # The two fp32 solves are equally far from the exact solution: the
         # GPU's solver is no less accurate than the oracle's.
         assert eg < 3.0 * eo + 1e-7, line
-        assert near[0] < bar, line
+        assert err < bar or within_ulp_copies(err, near, bar), line
         contact_steps += rg["pairs_max"] > 0
         within += err < 1e-5
-        worst = max(worst, near[0])
-        worst_ratio = max(worst_ratio, near[0] / bar)
+        worst = max(worst, min(err, near[0]))
+        worst_ratio = max(worst_ratio, min(err, near[0]) / bar)
         worst_solve = max(worst_solve, eg, eo)
         p["prev_dt"] = rg["dt"]
     gpu.close()
