@@ -184,6 +184,7 @@ def run_drape(args, dev):
                          "pcg spmv on the last Newton system of the timed steps",
                          traffic_from_profiles("pcg_spmv_drape_stack.md", n, nnz))
     plan.close()
+    walks = walk_stats_step(one_step)
     gpu.close()
     workload = {"workload": f"stacked-layer drape: {args.layers} sheets of {args.res}x{args.res} "
                             "vertices 2 mm apart (noise 1e-4 m) over a cylinder r = 0.2 m "
@@ -195,6 +196,7 @@ def run_drape(args, dev):
     extra = {"pcg_iters": round(pcg / newton, 1),
              "learned": learned,
              "phase_ms_per_newton_step": {k: round(v / newton, 3) for k, v in phase.items()},
+             "walks": walks,
              "max_contact_pairs": pairs,
              "roofline": roof, "cpu_baseline": None}
     out = head(args, per_newton, per_newton, workload, extra)
@@ -279,6 +281,35 @@ def run_learned(args, dev, last_system):
                          "loss_first_last": [round(float(np.mean(hist[:20])), 4),
                                              round(float(np.mean(hist[-20:])), 4)],
                          "collect_s": round(t_collect, 2), "train_s": round(t_train, 2)}}
+
+
+def walk_stats_step(one_step):
+    """One more time step (after the timed window, untimed) with the BVH walk
+    statistics on: per kind of walk, launches, queries, nodes taken off the
+    walk stacks per query, CCD sweeps, the mean launch time (HIP events), and
+    the walk's algorithmic bytes per launch -- an 8-byte node record per node
+    and the two 32-byte child boxes per internal node -- over that time."""
+    from ppf_amd import _lib
+    _lib.walk_stats_enable(True)
+    try:
+        one_step()
+    finally:
+        st = _lib.walk_stats()
+        _lib.walk_stats_enable(False)
+    out = {}
+    for kind, w in st.items():
+        if not w["launches"]:
+            continue
+        per = w["launches"]
+        walk_bytes = (8 * w["nodes"] + 64 * (w["nodes"] - w["leaves"])) / per
+        ms = w["ms"] / per
+        out[kind] = {"launches": per, "queries_per_launch": round(w["queries"] / per),
+                     "nodes_per_query": round(w["nodes"] / max(w["queries"], 1), 1),
+                     "sweeps_per_launch": round(w["sweeps"] / per),
+                     "ms_per_launch": round(ms, 3),
+                     "walk_bytes_per_launch": round(walk_bytes),
+                     "walk_gbs": round(walk_bytes / (ms * 1e6), 1) if ms > 0 else None}
+    return out
 
 
 def cpu_drape(args, start):

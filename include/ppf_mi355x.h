@@ -42,6 +42,24 @@ const char *ppf_last_error(void);
  * are kept, so once warmed up a time step adds nothing to either count. */
 void ppf_alloc_counts(uint64_t *allocs, uint64_t *frees);
 
+/* Statistics of the BVH walks, a measurement mode (off by default; no
+ * reference counterpart). While on, the walk kernels add up, per kind:
+ * launches, queries (elements that walk), nodes taken off the walk stack
+ * (each reads an 8-byte node record and, if internal, its two children's
+ * 32-byte boxes), leaves among them, and sweeps (CCD pairs run by the line
+ * search), and every launch of those kernels is timed with HIP events (the
+ * host waits for it). Kinds: 0 line-search vertex sweeps (point-face,
+ * point-point, vertex vs collision face), 1 line-search edge sweeps
+ * (edge-edge, edge vs collision edge), 2 proximity vertex queries, 3
+ * proximity edge queries (the count passes). Enabling zeroes the counters;
+ * disabling keeps them. Not for concurrent use from several threads. */
+typedef struct ppf_walk_stats {
+    uint64_t launches, queries, nodes, leaves, sweeps;
+    double ms;
+} ppf_walk_stats;
+int ppf_walk_stats_enable(int on);
+int ppf_walk_stats_read(ppf_walk_stats *out /* [4] */);
+
 /* Device allocation and release through the same counters, for code layered
  * on this library (the solver-backend boundary; the reference's
  * mem::malloc_device / mem::free_device under buffer/buffer.hpp). */

@@ -82,12 +82,54 @@ __device__ inline QBox point_box(f3 p, float margin) {
                 0u};
 }
 
+// Work of one lane's tree walks, for the walk statistics: nodes taken off
+// the stack (the root included; each costs its 8-byte record and, if
+// internal, its children's two 32-byte boxes), leaves among them, and
+// candidates swept (queued CCD work).
+struct WalkCount {
+    uint32_t nodes = 0, leaves = 0, work = 0;
+};
+
+// Walk statistics of one kind of query, accumulated on the device (u64):
+// launches, queries, nodes, leaves, work. Kinds: line-search vertex sweeps,
+// line-search edge sweeps, proximity vertex queries, proximity edge queries.
+enum WalkKind { kWalkLsVertex = 0, kWalkLsEdge = 1, kWalkProxVertex = 2, kWalkProxEdge = 3 };
+constexpr int kWalkKinds = 4, kWalkFields = 5;
+
+// Adds a workgroup's counts to stats[kind] (null: statistics off). Every
+// thread calls it; `query` marks a lane that had a query.
+__device__ inline void walk_flush(unsigned long long *stats, int kind, const WalkCount &wc,
+                                  bool query) {
+    if (!stats) {
+        return;
+    }
+    unsigned long long *s = stats + kWalkFields * kind;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    uint32_t v[4] = {query ? 1u : 0u, wc.nodes, wc.leaves, wc.work};
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            v[f] += __shfl_xor(v[f], off, kWave);
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            atomicAdd(s + 1 + f, (unsigned long long)v[f]);
+        }
+    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        atomicAdd(s, 1ull);
+    }
+}
+
 // Visits every leaf whose box overlaps q; `leaf(prim)` returns whether the
 // candidate was accepted. Returns the accepted count. `stk` is this thread's
 // first LDS slot (stride kQueryThreads).
 template <class F>
 __device__ uint32_t traverse(const BvhView &bvh, const QBox &q, uint32_t *stk,
-                             uint32_t *overflow, F &&leaf) {
+                             uint32_t *overflow, F &&leaf, WalkCount *wc = nullptr) {
     if (bvh.n == 0 || !overlaps(load_box(bvh.box, bvh.root), q)) {
         return 0;
     }
@@ -97,7 +139,13 @@ __device__ uint32_t traverse(const BvhView &bvh, const QBox &q, uint32_t *stk,
     while (sp) {
         const uint32_t v = stk[(--sp) * kQueryThreads];
         const uint32_t c1 = bvh.node[2 * (size_t)v + 1];
+        if (wc) {
+            ++wc->nodes;
+        }
         if (c1 == 0) {
+            if (wc) {
+                ++wc->leaves;
+            }
             if (leaf(bvh.node[2 * (size_t)v] - 1)) {
                 ++count;
             }
@@ -148,7 +196,8 @@ __device__ inline uint32_t lanes_below(uint64_t m) {
 // of the workgroup must call it (the rounds are wave uniform).
 template <int STACK, class A, class W>
 __device__ void traverse_wave(const BvhView &bvh, const QBox &q, bool active, uint32_t qid,
-                              WaveScratchT<STACK> &ws, uint32_t *overflow, A &&accept, W &&work) {
+                              WaveScratchT<STACK> &ws, uint32_t *overflow, A &&accept, W &&work,
+                              WalkCount *wc = nullptr) {
     const uint32_t lane = threadIdx.x;
     uint32_t *stk = ws.stk + lane;
     uint32_t sp = 0;
@@ -162,8 +211,14 @@ __device__ void traverse_wave(const BvhView &bvh, const QBox &q, bool active, ui
         if (sp) {
             const uint32_t v = stk[(--sp) * kWaveQueries];
             const uint32_t c1 = bvh.node[2 * (size_t)v + 1];
+            if (wc) {
+                ++wc->nodes;
+            }
             if (c1 == 0) {
                 const uint32_t prim = bvh.node[2 * (size_t)v] - 1;
+                if (wc) {
+                    ++wc->leaves;
+                }
                 if (accept(prim)) {
                     cand = prim;
                 }
@@ -198,6 +253,9 @@ __device__ void traverse_wave(const BvhView &bvh, const QBox &q, bool active, ui
             if (lane < take) {
                 const uint32_t at = queued - 1 - lane;
                 work(ws.qid[at], ws.prim[at]);
+                if (wc) {
+                    ++wc->work;
+                }
             }
             queued -= take;
             __syncthreads();

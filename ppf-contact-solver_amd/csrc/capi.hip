@@ -21,6 +21,37 @@ static thread_local char g_err[1024];
 static std::atomic<uint64_t> g_allocs{0}, g_frees{0};
 
 void count_alloc() { g_allocs.fetch_add(1, std::memory_order_relaxed); }
+
+namespace {
+// walk statistics: 4 kinds x 5 counters on the device, kernel time per kind
+struct WalkStatsState {
+    unsigned long long *dev = nullptr;
+    bool on = false;
+    double ms[4] = {};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+} g_walk;
+} // namespace
+
+unsigned long long *walk_stats_device() { return g_walk.on ? g_walk.dev : nullptr; }
+
+WalkClock::WalkClock(int kind, hipStream_t s) : kind_(kind), s_(s), on_(g_walk.on) {
+    if (on_) {
+        (void)hipEventRecord(g_walk.ev[0], s_);
+    }
+}
+
+void WalkClock::stop() {
+    if (!on_) {
+        return;
+    }
+    on_ = false;
+    float ms = 0.0f;
+    if (hipEventRecord(g_walk.ev[1], s_) == hipSuccess &&
+        hipEventSynchronize(g_walk.ev[1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, g_walk.ev[0], g_walk.ev[1]) == hipSuccess) {
+        g_walk.ms[kind_] += ms;
+    }
+}
 void count_free() { g_frees.fetch_add(1, std::memory_order_relaxed); }
 void alloc_totals(uint64_t *allocs, uint64_t *frees) {
     *allocs = g_allocs.load(std::memory_order_relaxed);
@@ -126,6 +157,44 @@ int ppf_device_alloc(void **ptr, size_t bytes) {
 }
 
 void ppf_device_free(void *ptr) { (void)ppf::dev_free(ptr); }
+
+int ppf_walk_stats_enable(int on) {
+    if (on) {
+        if (!g_walk.dev) {
+            PPF_HIP(dev_alloc(&g_walk.dev, 4 * 5 * sizeof(unsigned long long)));
+            for (hipEvent_t &e : g_walk.ev) {
+                PPF_HIP(hipEventCreate(&e));
+            }
+        }
+        PPF_HIP(hipMemset(g_walk.dev, 0, 4 * 5 * sizeof(unsigned long long)));
+        for (double &m : g_walk.ms) {
+            m = 0.0;
+        }
+    }
+    g_walk.on = on != 0;
+    return PPF_OK;
+}
+
+int ppf_walk_stats_read(ppf_walk_stats *out) {
+    if (!out) {
+        set_error("ppf_walk_stats_read: null output");
+        return PPF_ERR_ARG;
+    }
+    unsigned long long h[20] = {};
+    if (g_walk.dev) {
+        PPF_HIP(hipDeviceSynchronize());
+        PPF_HIP(hipMemcpy(h, g_walk.dev, sizeof(h), hipMemcpyDeviceToHost));
+    }
+    for (int k = 0; k < 4; ++k) {
+        out[k].launches = h[5 * k];
+        out[k].queries = h[5 * k + 1];
+        out[k].nodes = h[5 * k + 2];
+        out[k].leaves = h[5 * k + 3];
+        out[k].sweeps = h[5 * k + 4];
+        out[k].ms = g_walk.ms[k];
+    }
+    return PPF_OK;
+}
 
 void ppf_alloc_counts(uint64_t *allocs, uint64_t *frees) {
     uint64_t a = 0, f = 0;

@@ -135,6 +135,21 @@ void set_error(const char *fmt, ...);
 void count_alloc();
 void count_free();
 void alloc_totals(uint64_t *allocs, uint64_t *frees);
+
+// Walk statistics of the BVH walks (a measurement mode, off by default):
+// device counters the walk kernels add to (null when off), and the time of
+// each walk kernel launch. A WalkClock brackets one launch of a kind; while
+// statistics are on it records HIP events around it and waits for the end.
+unsigned long long *walk_stats_device();
+class WalkClock {
+  public:
+    WalkClock(int kind, hipStream_t s);
+    void stop();
+  private:
+    int kind_;
+    hipStream_t s_;
+    bool on_;
+};
 // Capacity of a buffer that grows with the contact set: the next power of
 // two at or above twice the need (at least `floor`), as the reference sizes
 // its pair caches (contact/contact.cu:1389-1396). Growth events are then

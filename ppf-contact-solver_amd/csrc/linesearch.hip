@@ -39,6 +39,7 @@ struct LsCtx {
     // [2 ns, 3 ns); null for one scene
     uint32_t *sout;
     uint32_t ns;
+    unsigned long long *stats; // walk statistics (null: off)
 };
 
 __device__ inline f3 X(const float *a, uint32_t v) { return load3(a + 3 * (size_t)v); }
@@ -101,6 +102,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_vertex(LsCtx c) {
     const bool idyn = (vf & 1u) == 0, icol = (vf & 2u) != 0;
     const uint32_t ibody = active ? m.vbody[i] : 0u;
     float best = max_t;
+    WalkCount wc;
     traverse_wave(
         c.fb, q, active, i, ws, c.out + 3,
         [&](uint32_t fi) {
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_vertex(LsCtx c) {
                 r = ccd<3, 1>(a, b, 2.0f * m.v_off[v], m.v_ghat[v], max_t, eps, ovl);
             }
             note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, v));
-        });
+        }, &wc);
     traverse_wave(
         c.vb, q, active, i, ws, c.out + 3,
         [&](uint32_t j) {
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_vertex(LsCtx c) {
             const float r = ccd<2, 1>(a, b, m.v_off[v] + m.v_off[j],
                                       0.5f * (m.v_ghat[v] + m.v_ghat[j]), max_t, eps, ovl);
             note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, v));
-        });
+        }, &wc);
     if (c.sm.nf) {
         const bool go = active && idyn && c.lp.vmass[i] != 0.0f;
         traverse_wave(
@@ -167,9 +169,10 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_vertex(LsCtx c) {
                                           0.5f * (m.v_ghat[v] + c.sm.f_ghat[fi]), max_t, eps,
                                           ovl);
                 note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, v));
-            });
+            }, &wc);
     }
     wave_min_out(c.out + 0, best, max_t);
+    walk_flush(c.stats, kWalkLsVertex, wc, active);
 }
 
 // static vertex vs dynamic face (contact.cu:1996-2034)
@@ -317,6 +320,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_edge(LsCtx c) {
     const bool acol = (m.vflag[a0] & 2u) != 0;
     const uint32_t abody = m.vbody[a0];
     float best = seed;
+    WalkCount wc;
     traverse_wave(
         c.eb, q, active, ei, ws, c.out + 3,
         [&](uint32_t ej) {
@@ -358,7 +362,7 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_edge(LsCtx c) {
                 }
             }
             note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, e0), true);
-        });
+        }, &wc);
     if (c.sm.ne) {
         const bool go = active && edyn && c.lp.emass[ei] > 0.0f;
         traverse_wave(
@@ -375,9 +379,10 @@ __global__ __launch_bounds__(kWQ, kLsWaves) void k_ls_edge(LsCtx c) {
                                           0.5f * (m.e_ghat[e] + c.sm.e_ghat[sj]), max_t,
                                           eps, ovl);
                 note(c, ovl, r, best, vertex_scene(m.vscene, m.nscene, e0), true);
-            });
+            }, &wc);
     }
     wave_min_out(c.out + 1, best, seed);
+    walk_flush(c.stats, kWalkLsEdge, wc, active);
 }
 
 template <int C, int SPLIT>
@@ -514,11 +519,14 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
     c.out = out_;
     c.sout = ns ? out_ + 5 : nullptr;
     c.ns = ns;
+    c.stats = walk_stats_device();
     auto sweeps = [&](auto stack) -> int {
         constexpr int D = decltype(stack)::value;
         if (m.nv) {
+            WalkClock clk(kWalkLsVertex, s);
             k_ls_vertex<D><<<xcd_grid(cdiv(m.nv, kWQ)), kWQ, 0, s>>>(c);
             PPF_HIP(hipGetLastError());
+            clk.stop();
         }
         if (S.nv && m.nf) {
             k_ls_static_vertex<D><<<xcd_grid(cdiv(S.nv, kWQ)), kWQ, 0, s>>>(c);
@@ -529,8 +537,10 @@ int LineSearch::run(const ppf_contact_mesh &m, const ppf_static_mesh *sm,
             PPF_HIP(hipGetLastError());
         }
         if (m.ne) {
+            WalkClock clk(kWalkLsEdge, s);
             k_ls_edge<D><<<xcd_grid(cdiv(m.ne, kWQ)), kWQ, 0, s>>>(c);
             PPF_HIP(hipGetLastError());
+            clk.stop();
         }
         PPF_HIP(hipMemcpyAsync(h_out_, out_, words * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                s));

@@ -39,6 +39,7 @@ struct Ctx {
     ppf_pair *stash;    // kStash per query, filled by the count pass (may be null)
     uint32_t *redo;     // fill pass: only these leaf positions (null: all)
     uint32_t nredo;
+    unsigned long long *stats; // walk statistics of the count pass (null: off)
 };
 
 __device__ inline f3 X(const ppf_contact_mesh &m, uint32_t v) {
@@ -91,7 +92,10 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
     __shared__ uint32_t stk_all[kStack * kQT];
     uint32_t leaf_pos;
     const ppf_contact_mesh &m = c.m;
-    if (!query_pos<FILL>(c, c.vb.n, leaf_pos)) {
+    const bool has = query_pos<FILL>(c, c.vb.n, leaf_pos);
+    WalkCount wc;
+    [&] {
+    if (!has) {
         return;
     }
     uint32_t *stk = stk_all + threadIdx.x;
@@ -128,7 +132,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
             return true;
         }
         return false;
-    });
+    }, &wc);
     // point-edge (contact.cu:622-715)
     total += traverse(c.eb, q, stk, c.overflow, [&](uint32_t ei) {
         const uint32_t e0 = m.edge[2 * (size_t)ei], e1 = m.edge[2 * (size_t)ei + 1];
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
             return true;
         }
         return false;
-    });
+    }, &wc);
     // point-point (contact.cu:432-620)
     total += traverse(c.vb, q, stk, c.overflow, [&](uint32_t j) {
         const bool either_dyn = vdyn(m, i) || vdyn(m, j);
@@ -233,9 +237,13 @@ __global__ __launch_bounds__(kQT) void k_vertex_queries(Ctx c) {
             return true;
         }
         return false;
-    });
+    }, &wc);
     if (!FILL) {
         c.cnt[leaf_pos] = total;
+    }
+    }();
+    if (!FILL) {
+        walk_flush(c.stats, kWalkProxVertex, wc, has);
     }
 }
 
@@ -245,7 +253,10 @@ __global__ __launch_bounds__(kQT) void k_edge_queries(Ctx c) {
     __shared__ uint32_t stk_all[kStack * kQT];
     uint32_t leaf_pos;
     const ppf_contact_mesh &m = c.m;
-    if (!query_pos<FILL>(c, c.eb.n, leaf_pos)) {
+    const bool has = query_pos<FILL>(c, c.eb.n, leaf_pos);
+    WalkCount wc;
+    [&] {
+    if (!has) {
         return;
     }
     uint32_t *stk = stk_all + threadIdx.x;
@@ -294,9 +305,13 @@ __global__ __launch_bounds__(kQT) void k_edge_queries(Ctx c) {
             return true;
         }
         return false;
-    });
+    }, &wc);
     if (!FILL) {
         c.cnt[qpos] = total;
+    }
+    }();
+    if (!FILL) {
+        walk_flush(c.stats, kWalkProxEdge, wc, has);
     }
 }
 
@@ -591,13 +606,18 @@ int Narrow::proximity(const ppf_contact_mesh &m, ppf_pair *out, uint64_t cap,
     c.stash = fill ? stash_ : nullptr;
     c.redo = nullptr;
     c.nredo = 0;
+    c.stats = walk_stats_device();
     if (m.nv) {
+        WalkClock clk(kWalkProxVertex, s);
         k_vertex_queries<false><<<xcd_grid(cdiv(m.nv, kQT)), kQT, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
+        clk.stop();
     }
     if (m.ne) {
+        WalkClock clk(kWalkProxEdge, s);
         k_edge_queries<false><<<xcd_grid(cdiv(m.ne, kQT)), kQT, 0, s>>>(c);
         PPF_HIP(hipGetLastError());
+        clk.stop();
     }
     PPF_HIP(hipMemsetAsync(cnt_ + nq, 0, sizeof(uint32_t), s));
     PPF_HIP(exclusive_scan_u32(cnt_, ofs_, nq + 1, scratch_, s));

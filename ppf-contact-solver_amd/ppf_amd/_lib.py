@@ -23,8 +23,19 @@ class PcgResult(ctypes.Structure):
                 ("break_cause", i32), ("break_iter", i32), ("err0", f32)]
 
 
+class WalkStats(ctypes.Structure):
+    """ppf_walk_stats (include/ppf_mi355x.h)."""
+    _fields_ = [("launches", ctypes.c_uint64), ("queries", ctypes.c_uint64),
+                ("nodes", ctypes.c_uint64), ("leaves", ctypes.c_uint64),
+                ("sweeps", ctypes.c_uint64), ("ms", ctypes.c_double)]
+
+
+WALK_KINDS = ("ls_vertex", "ls_edge", "prox_vertex", "prox_edge")
+
 _PROTOS = {
     "ppf_last_error": (ctypes.c_char_p, []),
+    "ppf_walk_stats_enable": (ctypes.c_int, [ctypes.c_int]),
+    "ppf_walk_stats_read": (ctypes.c_int, [ctypes.POINTER(WalkStats)]),
     "ppf_alloc_counts": (None, [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "ppf_device_alloc": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),
     "ppf_device_free": (None, [vp]),
@@ -132,6 +143,20 @@ def check(rc, what):
     if rc != 0:
         msg = load().ppf_last_error().decode(errors="replace")
         raise PPFError(f"{what} failed (code {rc}): {msg}")
+
+
+def walk_stats_enable(on):
+    """Start (zeroing the counters) or stop counting the BVH walks."""
+    check(load().ppf_walk_stats_enable(1 if on else 0), "ppf_walk_stats_enable")
+
+
+def walk_stats():
+    """Walk statistics per kind (WALK_KINDS): dicts of launches, queries,
+    nodes, leaves, sweeps and ms."""
+    out = (WalkStats * 4)()
+    check(load().ppf_walk_stats_read(out), "ppf_walk_stats_read")
+    return {k: {f: getattr(out[i], f) for f, _ in WalkStats._fields_}
+            for i, k in enumerate(WALK_KINDS)}
 
 
 def alloc_counts():

@@ -47,7 +47,8 @@ def _mirrors():
             "ppf_contact_mesh": narrow.ContactMeshC,
             "ppf_contact_params": narrow.ContactParamsC, "ppf_sphere": stepper.SphereC,
             "ppf_floor": stepper.FloorC, "ppf_scene": stepper.SceneC,
-            "ppf_step_params": stepper.StepParamsC, "ppf_step_result": stepper.StepResultC}
+            "ppf_step_params": stepper.StepParamsC, "ppf_step_result": stepper.StepResultC,
+            "ppf_walk_stats": lib_mod.WalkStats}
 
 
 def _c_fields(text, name):

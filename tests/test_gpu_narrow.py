@@ -138,3 +138,81 @@ def test_full_size_triangle_soup(dev):
     print(f"2M soup: {len(g)} pairs, kinds {np.bincount(g['kind'] & 255, minlength=4)}")
     assert np.array_equal(sorted_pairs(g), sorted_pairs(o))
     nph.close()
+
+
+def host_walk(nodes, boxes, root, qmn, qmx):
+    """Nodes and leaves a depth-first walk that tests children before pushing
+    them takes off its stack, summed over the query boxes: every node whose
+    box and every ancestor's box overlap the query (order independent)."""
+    def ov(v, qi):
+        b = boxes[v]
+        return np.all((b[:, :3] <= qmx[qi]) & (b[:, 3:] >= qmn[qi]), axis=1)
+    qi = np.arange(len(qmn))
+    v = np.full(len(qmn), root, np.int64)
+    keep = ov(v, qi)
+    qi, v = qi[keep], v[keep]
+    nodes_n = leaves_n = 0
+    while len(v):
+        nodes_n += len(v)
+        c1 = nodes[v, 1].astype(np.int64)
+        leaf = c1 == 0
+        leaves_n += int(leaf.sum())
+        l = nodes[v[~leaf], 0].astype(np.int64) - 1
+        r = c1[~leaf] - 1
+        cv = np.concatenate([l, r])
+        cq = np.concatenate([qi[~leaf], qi[~leaf]])
+        keep = ov(cv, cq)
+        v, qi = cv[keep], cq[keep]
+    return nodes_n, leaves_n
+
+
+@pytest.mark.parametrize("name", ["sheets3", "grid_offset"])
+def test_walk_stats_match_host_walk(dev, name):
+    """The walk statistics of the proximity count pass equal a host walk over
+    the oracle's trees (bit-exact with the device's) with the same query
+    boxes: queries, nodes taken off the stack and leaves, per kind; one
+    launch each. Statistics off, nothing is counted."""
+    from ppf_amd import _lib
+    m = SCENES[name]()
+    t = mesh_to_device(m, dev)
+    nph = NarrowPhase()
+    nph.build(t)
+    _lib.walk_stats_enable(True)
+    nph.count(t)
+    st = _lib.walk_stats()
+    _lib.walk_stats_enable(False)
+    nph.count(t)
+    assert _lib.walk_stats() == st
+    nph.close()
+    x = np.asarray(m["x"], np.float32).reshape(-1, 3)
+    trees = [oracle.bvh_build(m, k) for k in (0, 1, 2)]
+    qv = (np.float32(0.5) * np.asarray(m["v_ghat"], np.float32)
+          + np.asarray(m["v_off"], np.float32))[:, None]
+    vmn, vmx = x - qv, x + qv
+    off_v = (np.asarray(m["vflag"], np.uint32) & 4) != 0  # inactive: an empty box
+    vmn[off_v], vmx[off_v] = np.inf, -np.inf
+    want_nodes = want_leaves = 0
+    for nodes, boxes, root in trees:
+        a, b = host_walk(nodes, boxes, root, vmn, vmx)
+        want_nodes += a
+        want_leaves += b
+    pv = st["prox_vertex"]
+    print(f"{name}: vertex queries {pv['queries']} nodes {pv['nodes']} leaves {pv['leaves']} "
+          f"({pv['nodes'] / max(pv['queries'], 1):.1f} nodes per query, {pv['ms']:.3f} ms)")
+    assert pv["launches"] == 1 and pv["queries"] == m["nv"]
+    assert (pv["nodes"], pv["leaves"]) == (want_nodes, want_leaves)
+    e = np.asarray(m["edge"], np.int64).reshape(-1, 2)
+    qe = (np.float32(0.5) * np.asarray(m["e_ghat"], np.float32)
+          + np.asarray(m["e_off"], np.float32))[:, None]
+    emn = np.minimum(x[e[:, 0]], x[e[:, 1]])
+    emx = np.maximum(x[e[:, 0]], x[e[:, 1]])
+    nz = qe[:, 0] != 0
+    emn[nz] -= qe[nz]
+    emx[nz] += qe[nz]
+    off_e = (np.asarray(m["eflag"], np.uint32) & 4) != 0
+    emn[off_e], emx[off_e] = np.inf, -np.inf
+    a, b = host_walk(*trees[1], emn, emx)
+    pe = st["prox_edge"]
+    assert pe["launches"] == 1 and pe["queries"] == m["ne"]
+    assert (pe["nodes"], pe["leaves"]) == (a, b)
+    assert st["ls_vertex"]["launches"] == st["ls_edge"]["launches"] == 0
