@@ -287,8 +287,9 @@ def walk_stats_step(one_step):
     """One more time step (after the timed window, untimed) with the BVH walk
     statistics on: per kind of walk, launches, queries, nodes taken off the
     walk stacks per query, CCD sweeps, the mean launch time (HIP events), and
-    the walk's algorithmic bytes per launch -- an 8-byte node record per node
-    and the two 32-byte child boxes per internal node -- over that time."""
+    the walk's algorithmic bytes per launch -- the 64-byte walk record of
+    every internal node entered (both children's boxes, tags and codes) --
+    over that time."""
     from ppf_amd import _lib
     _lib.walk_stats_enable(True)
     try:
@@ -301,7 +302,7 @@ def walk_stats_step(one_step):
         if not w["launches"]:
             continue
         per = w["launches"]
-        walk_bytes = (8 * w["nodes"] + 64 * (w["nodes"] - w["leaves"])) / per
+        walk_bytes = 64 * (w["nodes"] - w["leaves"]) / per
         ms = w["ms"] / per
         out[kind] = {"launches": per, "queries_per_launch": round(w["queries"] / per),
                      "nodes_per_query": round(w["nodes"] / max(w["queries"], 1), 1),

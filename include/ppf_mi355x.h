@@ -45,8 +45,8 @@ void ppf_alloc_counts(uint64_t *allocs, uint64_t *frees);
 /* Statistics of the BVH walks, a measurement mode (off by default; no
  * reference counterpart). While on, the walk kernels add up, per kind:
  * launches, queries (elements that walk), nodes taken off the walk stack
- * (each reads an 8-byte node record and, if internal, its two children's
- * 32-byte boxes), leaves among them, and sweeps (CCD pairs run by the line
+ * (an internal one reads its 64-byte walk record: both children's boxes,
+ * scene tags and codes), leaves among them, and sweeps (CCD pairs run by the line
  * search), and every launch of those kernels is timed with HIP events (the
  * host waits for it). Kinds: 0 line-search vertex sweeps (point-face,
  * point-point, vertex vs collision face), 1 line-search edge sweeps

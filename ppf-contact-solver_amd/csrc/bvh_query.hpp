@@ -82,10 +82,38 @@ __device__ inline QBox point_box(f3 p, float margin) {
                 0u};
 }
 
+// The stack code of the root: its node id, or for a one-leaf tree its
+// primitive | kLeafCode.
+__device__ inline uint32_t root_code(const BvhView &bvh) {
+    return bvh.n == 1 ? (bvh.node[2 * (size_t)bvh.root] - 1) | kLeafCode : bvh.root;
+}
+
+// Reads internal node v's walk record and tests both children against q:
+// codes and verdicts (aabb::overlap on the stored boxes and scene tags).
+__device__ inline void enter(const BvhView &bvh, uint32_t v, const QBox &q, uint32_t code[2],
+                             bool ov[2]) {
+    const uint4 *r = reinterpret_cast<const uint4 *>(bvh.rec + (v - bvh.n));
+    const uint4 a = r[0], b = r[1], c = r[2], d = r[3];
+    const float b0[6] = {__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z),
+                         __uint_as_float(b.w), __uint_as_float(c.x), __uint_as_float(c.y)};
+    const float b1[6] = {__uint_as_float(c.z), __uint_as_float(c.w), __uint_as_float(d.x),
+                         __uint_as_float(d.y), __uint_as_float(d.z), __uint_as_float(d.w)};
+    code[0] = a.x, code[1] = a.y;
+    const uint32_t tag[2] = {a.z, a.w};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const float *bx = k ? b1 : b0;
+        ov[k] = (tag[k] == q.scene || tag[k] == kMixedScene) &&
+                (bx[0] <= q.mx[0] && bx[3] >= q.mn[0]) &&
+                (bx[1] <= q.mx[1] && bx[4] >= q.mn[1]) &&
+                (bx[2] <= q.mx[2] && bx[5] >= q.mn[2]);
+    }
+}
+
 // Work of one lane's tree walks, for the walk statistics: nodes taken off
-// the stack (the root included; each costs its 8-byte record and, if
-// internal, its children's two 32-byte boxes), leaves among them, and
-// candidates swept (queued CCD work).
+// the stack (the root included; an internal one costs its 64-byte walk
+// record, a leaf nothing more), leaves among them, and candidates swept
+// (queued CCD work).
 struct WalkCount {
     uint32_t nodes = 0, leaves = 0, work = 0;
 };
@@ -135,34 +163,33 @@ __device__ uint32_t traverse(const BvhView &bvh, const QBox &q, uint32_t *stk,
     }
     uint32_t count = 0;
     uint32_t sp = 0;
-    stk[(sp++) * kQueryThreads] = bvh.root;
+    stk[(sp++) * kQueryThreads] = root_code(bvh);
     while (sp) {
         const uint32_t v = stk[(--sp) * kQueryThreads];
-        const uint32_t c1 = bvh.node[2 * (size_t)v + 1];
         if (wc) {
             ++wc->nodes;
         }
-        if (c1 == 0) {
+        if (v & kLeafCode) {
             if (wc) {
                 ++wc->leaves;
             }
-            if (leaf(bvh.node[2 * (size_t)v] - 1)) {
+            if (leaf(v & ~kLeafCode)) {
                 ++count;
             }
             continue;
         }
-        const uint32_t l = bvh.node[2 * (size_t)v] - 1, r = c1 - 1;
-        const bool ol = overlaps(load_box(bvh.box, l), q);
-        const bool orr = overlaps(load_box(bvh.box, r), q);
+        uint32_t code[2];
+        bool ov[2];
+        enter(bvh, v, q, code, ov);
         if (sp + 2 > (uint32_t)kQueryStack) {
             atomicOr(overflow, 1u);
             return count;
         }
-        if (ol) {
-            stk[(sp++) * kQueryThreads] = l;
+        if (ov[0]) {
+            stk[(sp++) * kQueryThreads] = code[0];
         }
-        if (orr) {
-            stk[(sp++) * kQueryThreads] = r;
+        if (ov[1]) {
+            stk[(sp++) * kQueryThreads] = code[1];
         }
     }
     return count;
@@ -202,7 +229,7 @@ __device__ void traverse_wave(const BvhView &bvh, const QBox &q, bool active, ui
     uint32_t *stk = ws.stk + lane;
     uint32_t sp = 0;
     if (active && bvh.n != 0 && overlaps(load_box(bvh.box, bvh.root), q)) {
-        stk[0] = bvh.root;
+        stk[0] = root_code(bvh);
         sp = 1;
     }
     uint32_t queued = 0; // wave uniform
@@ -210,12 +237,11 @@ __device__ void traverse_wave(const BvhView &bvh, const QBox &q, bool active, ui
         uint32_t cand = 0xffffffffu;
         if (sp) {
             const uint32_t v = stk[(--sp) * kWaveQueries];
-            const uint32_t c1 = bvh.node[2 * (size_t)v + 1];
             if (wc) {
                 ++wc->nodes;
             }
-            if (c1 == 0) {
-                const uint32_t prim = bvh.node[2 * (size_t)v] - 1;
+            if (v & kLeafCode) {
+                const uint32_t prim = v & ~kLeafCode;
                 if (wc) {
                     ++wc->leaves;
                 }
@@ -223,18 +249,18 @@ __device__ void traverse_wave(const BvhView &bvh, const QBox &q, bool active, ui
                     cand = prim;
                 }
             } else {
-                const uint32_t l = bvh.node[2 * (size_t)v] - 1, r = c1 - 1;
-                const bool ol = overlaps(load_box(bvh.box, l), q);
-                const bool orr = overlaps(load_box(bvh.box, r), q);
+                uint32_t code[2];
+                bool ov[2];
+                enter(bvh, v, q, code, ov);
                 if (sp + 2 > (uint32_t)STACK) {
                     atomicOr(overflow, 1u);
                     sp = 0;
                 } else {
-                    if (ol) {
-                        stk[(sp++) * kWaveQueries] = l;
+                    if (ov[0]) {
+                        stk[(sp++) * kWaveQueries] = code[0];
                     }
-                    if (orr) {
-                        stk[(sp++) * kWaveQueries] = r;
+                    if (ov[1]) {
+                        stk[(sp++) * kWaveQueries] = code[1];
                     }
                 }
             }

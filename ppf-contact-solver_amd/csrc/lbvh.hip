@@ -7,6 +7,7 @@
 // single bottom-up kernel with per-node arrival counters: the second child to
 // arrive writes the union and continues upward.
 #include "lbvh.hpp"
+#include <algorithm>
 #include "ppf_mi355x.h"
 #include "sort.hpp"
 
@@ -449,6 +450,33 @@ __global__ __launch_bounds__(kNT) void k_refit_level(const uint32_t *__restrict_
     box[par] = u;
 }
 
+// The walk records of the internal nodes from the refit boxes.
+__global__ __launch_bounds__(kNT) void k_walk_records(uint32_t n,
+                                                      const uint32_t *__restrict__ node,
+                                                      const Aabb *__restrict__ box,
+                                                      WalkRec *__restrict__ rec) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i + 1 >= n) {
+        return;
+    }
+    const uint32_t v = n + i;
+    WalkRec w;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t c = node[2 * (size_t)v + k] - 1;
+        const bool leaf = node[2 * (size_t)c + 1] == 0;
+        w.code[k] = leaf ? (node[2 * (size_t)c] - 1) | kLeafCode : c;
+        const Aabb b = box[c];
+        w.tag[k] = b.active != 0.0f ? __float_as_uint(b.pad) : kInactiveTag;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            w.box[6 * k + d] = b.mn[d];
+            w.box[6 * k + 3 + d] = b.mx[d];
+        }
+    }
+    rec[i] = w;
+}
+
 uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 } // namespace
@@ -459,7 +487,7 @@ void Lbvh::release() {
     for (void *p : {(void *)node_, (void *)parent_, (void *)depth_, (void *)level_,
                     (void *)code_,
                     (void *)idx_, (void *)scratch_, (void *)cent_, (void *)part_,
-                    (void *)box_, (void *)sb_}) {
+                    (void *)box_, (void *)rec_, (void *)sb_}) {
         if (p) {
             (void)dev_free(p);
         }
@@ -467,6 +495,7 @@ void Lbvh::release() {
     node_ = parent_ = depth_ = level_ = code_ = idx_ = scratch_ = nullptr;
     cent_ = part_ = nullptr;
     box_ = nullptr;
+    rec_ = nullptr;
     sb_ = nullptr;
     sb_cap_ = 0;
     cap_ = 0;
@@ -489,6 +518,7 @@ int Lbvh::reserve(uint32_t n) {
     PPF_HIP(dev_alloc(&cent_, 3 * (size_t)n * sizeof(float)));
     PPF_HIP(dev_alloc(&part_, (6 * nb + 6) * sizeof(float)));
     PPF_HIP(dev_alloc(&box_, nodes * sizeof(Aabb)));
+    PPF_HIP(dev_alloc(&rec_, std::max<size_t>(n, 1) * sizeof(WalkRec)));
     cap_ = n;
     return PPF_OK;
 }
@@ -498,6 +528,10 @@ int Lbvh::build(const PrimSet &p, const float *x0, const float *x1, float t,
     int rc = reserve(p.n);
     if (rc != PPF_OK) {
         return rc;
+    }
+    if (p.n >= kLeafCode) {
+        set_error("Lbvh: %u primitives exceed the walk records' index range", p.n);
+        return PPF_ERR_ARG;
     }
     n_ = p.n;
     root_ = n_ == 1 ? 0 : n_;
@@ -620,6 +654,10 @@ int Lbvh::refit(const PrimSet &p, const float *x0, const float *x1, float t,
             PPF_HIP(hipGetLastError());
         }
     }
+    if (n_ > 1) {
+        k_walk_records<<<cdiv(n_ - 1, kNT), kNT, 0, s>>>(n_, node_, box_, rec_);
+        PPF_HIP(hipGetLastError());
+    }
     return PPF_OK;
 }
 
@@ -629,6 +667,7 @@ BvhView Lbvh::view() const {
     v.root = root_;
     v.node = node_;
     v.box = box_;
+    v.rec = rec_;
     return v;
 }
 

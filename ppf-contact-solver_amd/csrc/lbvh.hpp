@@ -25,11 +25,25 @@ struct alignas(32) Aabb {
 };
 constexpr uint32_t kMixedScene = 0xffffffffu;
 
+// Walk record of internal node n + i (64 B, four 16 B loads): both children's
+// codes, scene tags and boxes, so a walk reads one record per internal node
+// it enters instead of the node and then its children's boxes. A child code
+// is the child's node id, or for a leaf its primitive | kLeafCode. An
+// inactive child's tag is kInactiveTag, which matches no query's scene.
+struct alignas(64) WalkRec {
+    uint32_t code[2];
+    uint32_t tag[2];
+    float box[12]; // child 0 min xyz, max xyz, then child 1
+};
+constexpr uint32_t kLeafCode = 0x80000000u;
+constexpr uint32_t kInactiveTag = 0xfffffffeu;
+
 struct BvhView {
     uint32_t n = 0;            // leaves
     uint32_t root = 0;
     const uint32_t *node = nullptr; // 2 words per node
     const Aabb *box = nullptr;
+    const WalkRec *rec = nullptr;   // n - 1 records (internal nodes)
 };
 
 // Primitive set a tree is built over: `arity` vertex indices per primitive
@@ -71,6 +85,7 @@ class Lbvh {
     uint32_t *code_ = nullptr, *idx_ = nullptr, *scratch_ = nullptr;
     float *cent_ = nullptr, *part_ = nullptr;
     Aabb *box_ = nullptr;
+    WalkRec *rec_ = nullptr;
     uint32_t *sb_ = nullptr; // per-scene centroid bounds (many scenes)
     size_t sb_cap_ = 0;
     void release();
