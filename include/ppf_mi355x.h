@@ -526,6 +526,22 @@ typedef struct ppf_shell {
     const uint32_t *fflag;
     const float *h_rest, *h_length, *h_area, *h_bend;
     const uint32_t *hflag;
+    /* Strain limiting (strainlimiting.cu:46-100, 243-305): per face the
+     * FaceParam strainlimit (0 = off) and min(shrink_x, shrink_y) (null =
+     * 1); the barrier keeps the larger singular value of F below
+     * 1 + (strainlimit + 1) / shrink - 1. Null f_strainlimit: no face is
+     * limited. */
+    const float *f_strainlimit;
+    const float *f_shrink;
+    /* Plasticity (plasticity.cu:21-78, 145-185): per face the FaceParam
+     * plasticity rate and threshold, per hinge the HingeParam ones (null
+     * rates: none). After every committed step the rest shape creeps toward
+     * the dead zone |s - 1| <= threshold of each singular value (faces) and
+     * |angle - rest| <= threshold (hinges), by 1 - exp(-rate dt): inv_rest
+     * and h_rest are then rewritten in place, so they must point to writable
+     * device memory. */
+    const float *f_plasticity, *f_plastic_threshold;
+    const float *h_plasticity, *h_plastic_threshold;
 } ppf_shell;
 
 /* dt and the ParamSet air terms (air_density, air_friction, wind). */
@@ -560,6 +576,46 @@ int ppf_elastic_pattern(ppf_elastic *h, const uint32_t **d_row_offset,
  * index). sh must have the topology of create. */
 int ppf_elastic_assemble(ppf_elastic *h, const ppf_shell *sh, const ppf_elastic_params *p,
                          float *d_force, float *d_cdiag, float *d_value, void *stream);
+
+/* Strain-limit force and Hessian at sh->x of every face with
+ * sh->f_strainlimit > 0 (replaces
+ * strainlimiting::embed_strainlimiting_force_hessian,
+ * strainlimiting/strainlimiting.cu:46-100): the cubic barrier on
+ * strainlimit - (s - 1) of each singular value s > 1 of F, projected by the
+ * 3x2 eigen-analysis with eiganalysis_eps (ParamSet.eiganalysis_eps),
+ * scaled by w^T H_el w + fmass / d^2 (w: the face's step-start vertices
+ * sh->x0 about their centroid; H_el: the face's 9x9 blocks of d_value;
+ * d: the limit minus the larger stretch). d_value must hold the blocks
+ * ppf_elastic_assemble wrote (the reference's tmp_fixed); the terms are added
+ * to d_force and d_value per vertex and per block in face order. A shell with
+ * a null f_strainlimit is left as it is. */
+int ppf_elastic_strain_limit(ppf_elastic *h, const ppf_shell *sh, float eiganalysis_eps,
+                             float *d_force, float *d_value, void *stream);
+
+/* Strain-limit line search of the step d_x0 -> d_x1 (replaces
+ * strainlimiting::line_search, strainlimiting/strainlimiting.cu:243-305):
+ * per limited face the largest t <= line_search_max_t, by bisection, that
+ * keeps the larger singular value of F(x0) + t (F(x1) - F(x0)) below
+ * 1 + its limit; *toi = the minimum over the faces / line_search_max_t
+ * (1 when none binds). */
+int ppf_elastic_strain_limit_toi(ppf_elastic *h, const ppf_shell *sh, const float *d_x0,
+                                 const float *d_x1, float line_search_max_t, float *toi,
+                                 void *stream);
+
+/* Rest-shape creep after a committed step at positions sh->x (replaces
+ * plasticity::update_face_plasticity and update_hinge_plasticity,
+ * plasticity/plasticity.cu:21-78, 145-185, called at main.cu:1615-1624
+ * with ParamSet.dt): per face with f_plasticity > 0, each singular value s
+ * of F = dX inv_rest with |s - 1| > threshold moves by
+ * alpha (1 -/+ threshold - s), alpha = 1 - exp(-rate dt), and the new
+ * inv_rest maps the face's current in-plane frame to U diag(s') V^T; per
+ * non-excluded hinge with h_plasticity > 0, the rest angle moves by
+ * alpha (delta - sign(delta) threshold) when |delta| = |angle - rest| >
+ * threshold. d_inv_rest (4 nf) and d_h_rest (nh) hold the current rest
+ * data and receive the new one (they may be sh->inv_rest and sh->h_rest
+ * themselves); entries that do not creep are left as they are. */
+int ppf_elastic_plasticity(ppf_elastic *h, const ppf_shell *sh, float dt, float *d_inv_rest,
+                           float *d_h_rest, void *stream);
 
 /* ------------------------------------------------------------------------
  * One time step: the Newton loop of main_helper::advance.
@@ -610,6 +666,10 @@ typedef struct ppf_step_params {
     /* ParamSet.constraint_tol: the smallest gap a kinematic floor's barrier
      * sees, as a fraction of its ghat (default 0.01) */
     float constraint_tol;
+    /* ParamSet.eiganalysis_eps: below this gap of the two singular values the
+     * strain-limit Hessian's flip mode takes the mean curvature (default
+     * 1e-2) */
+    float eiganalysis_eps;
 } ppf_step_params;
 
 /* Outcome of one step (StepResult, data.hpp:570-595). status: 0 success,

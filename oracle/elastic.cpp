@@ -689,6 +689,9 @@ struct oracle_shell {
     const uint32_t *fflag;
     const float *h_rest, *h_length, *h_area, *h_bend;
     const uint32_t *hflag;
+    const float *f_strainlimit, *f_shrink;
+    const float *f_plasticity, *f_plastic_threshold;
+    const float *h_plasticity, *h_plastic_threshold;
 };
 
 struct oracle_elastic_params {

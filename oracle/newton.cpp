@@ -39,6 +39,9 @@ struct ShellView {
     const uint32_t *fflag;
     const float *h_rest, *h_length, *h_area, *h_bend;
     const uint32_t *hflag;
+    const float *f_strainlimit, *f_shrink; // null: no face is strain-limited / no shrink
+    const float *f_plasticity, *f_plastic_threshold; // null: no plasticity
+    const float *h_plasticity, *h_plastic_threshold;
 };
 struct StaticView {
     uint32_t nv, ne, nf;
@@ -116,6 +119,18 @@ uint32_t oracle_analytic_assemble(const MeshView *m, const ContactParams *cp,
                                   const oracle_sphere *sp, uint32_t nsp,
                                   const oracle_floor *fl, uint32_t nfl,
                                   float constraint_tol, float *grad, float *cdiag);
+void oracle_strainlimit_assemble(uint32_t nv, uint32_t nf, const uint32_t *face,
+                                 const float *inv_rest, const uint32_t *fflag,
+                                 const float *fmass, const float *sl, const float *shrink,
+                                 const float *x, const float *x0, const uint32_t *rp,
+                                 const uint32_t *ci, float eig_eps, float *force, float *value);
+void oracle_plasticity(uint32_t nf, uint32_t nh, const uint32_t *face, const uint32_t *hinge,
+                       const uint32_t *hflag, const float *x, const float *f_rate,
+                       const float *f_thr, const float *h_rate, const float *h_thr, float dt,
+                       float *inv_rest, float *h_rest);
+float oracle_strainlimit_toi(uint32_t nf, const uint32_t *face, const float *inv_rest,
+                             const uint32_t *fflag, const float *sl, const float *shrink,
+                             const float *x0, const float *x1, float max_t);
 void oracle_analytic_toi(uint32_t nv, const float *x0a, const float *x1a, const uint32_t *vflag,
                          const float *vmass, const oracle_sphere *sp, uint32_t nsp,
                          const oracle_floor *fl, uint32_t nfl, float max_t, float *toi,
@@ -150,6 +165,7 @@ struct oracle_step_params {
     int32_t disable_contact;
     int32_t check_intersection;
     float constraint_tol;
+    float eiganalysis_eps;
 };
 
 struct oracle_step_result {
@@ -310,6 +326,11 @@ int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_resul
         // gradient and the per-vertex diagonal
         oracle_analytic_assemble(&m, &cp, sc->spheres, sc->nsphere, sc->floors, sc->nfloor,
                                  p->constraint_tol, cgrad.data(), cdiag.data());
+        // strain limiting (main.cu:809-822), its stiffness from the elastic
+        // blocks alone, its terms then joining force and blocks
+        oracle_strainlimit_assemble(n, sh.nf, sh.face, sh.inv_rest, sh.fflag, sh.fmass,
+                                    sh.f_strainlimit, sh.f_shrink, ex.data(), curr, erp.data(),
+                                    eci.data(), p->eiganalysis_eps, force.data(), eval.data());
         for (size_t k = 0; k < n3; ++k) force[k] = force[k] + cgrad[k];
         // merged upper matrix: elastic pattern + contact blocks (A + B)
         std::vector<uint32_t> mrp(n + 1, 0), mci;
@@ -432,6 +453,14 @@ int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_resul
                 break;
             }
         }
+        if (sc->shell.f_strainlimit) {
+            // the strain-limit line search joins the minimum (main.cu:1321-1331)
+            const float t = oracle_strainlimit_toi(
+                sc->shell.nf, sc->shell.face, sc->shell.inv_rest, sc->shell.fflag,
+                sc->shell.f_strainlimit, sc->shell.f_shrink, target.data(), ex.data(),
+                p->line_search_max_t);
+            toi = std::fmin(toi, t / p->line_search_max_t);
+        }
         g_trace.insert(g_trace.end(), {(float)pr.iter, recale, toi, max_dx});
         if (toi <= FLT_EPSILON) {
             out->status = 2;
@@ -464,6 +493,12 @@ int oracle_step(oracle_scene *sc, const oracle_step_params *p, oracle_step_resul
     if (out->status == 0 || out->status == 4) {
         std::memcpy(sc->prev, curr, n3 * sizeof(float));
         std::memcpy(curr, ex.data(), n3 * sizeof(float));
+        // rest-shape creep at the committed positions with the requested dt
+        // (main.cu:1615-1624)
+        const ShellView &s = sc->shell;
+        oracle_plasticity(s.nf, s.nh, s.face, s.hinge, s.hflag, curr, s.f_plasticity,
+                          s.f_plastic_threshold, s.h_plasticity, s.h_plastic_threshold, p->dt,
+                          const_cast<float *>(s.inv_rest), const_cast<float *>(s.h_rest));
     }
     return 0;
 }

@@ -428,7 +428,9 @@ class Shell(ctypes.Structure):
                 ("vmass", vp), ("varea", vp), ("vflag", vp), ("fmass", vp),
                 ("inv_rest", vp), ("f_mu", vp), ("f_lambda", vp), ("fflag", vp),
                 ("h_rest", vp), ("h_length", vp), ("h_area", vp), ("h_bend", vp),
-                ("hflag", vp)]
+                ("hflag", vp), ("f_strainlimit", vp), ("f_shrink", vp),
+                ("f_plasticity", vp), ("f_plastic_threshold", vp), ("h_plasticity", vp),
+                ("h_plastic_threshold", vp)]
 
 
 class ElasticParams(ctypes.Structure):
@@ -437,6 +439,10 @@ class ElasticParams(ctypes.Structure):
 
 
 _SHELL_U32 = {"face", "hinge", "v2f_off", "v2f", "vflag", "fflag", "hflag"}
+# per-face / per-hinge arrays that may be absent (null): strain limits, rest
+# shrink, plasticity
+_SHELL_OPT = {"f_strainlimit", "f_shrink", "f_plasticity", "f_plastic_threshold",
+              "h_plasticity", "h_plastic_threshold"}
 
 
 def shell_struct(sh):
@@ -444,6 +450,9 @@ def shell_struct(sh):
     for name, _ in Shell._fields_:
         if name in ("nv", "nf", "nh"):
             fields[name] = int(sh[name])
+            continue
+        if name in _SHELL_OPT and sh.get(name) is None:
+            fields[name] = None
             continue
         dt = np.uint32 if name in _SHELL_U32 else np.float32
         a = np.ascontiguousarray(sh[name], dtype=dt).reshape(-1)
@@ -534,6 +543,110 @@ def elastic_assemble(sh, rp, ci, dt, air_density=1e-3, air_friction=0.2, wind=(0
     return force, cdiag, val[:9 * len(ci)].reshape(-1, 9), int(miss)
 
 
+def _load_strainlimit():
+    lib = load()
+    if not hasattr(lib, "_sl_ready"):
+        lib.oracle_svd32.argtypes = [u32, vp, vp, vp, vp]
+        lib.oracle_svd32.restype = None
+        lib.oracle_strainlimit_faces.argtypes = [u32] + [vp] * 11 + [f32] + [vp] * 5
+        lib.oracle_strainlimit_faces.restype = None
+        lib.oracle_strainlimit_assemble.argtypes = [u32, u32] + [vp] * 10 + [f32, vp, vp]
+        lib.oracle_strainlimit_assemble.restype = None
+        lib.oracle_strainlimit_toi.argtypes = [u32] + [vp] * 7 + [f32]
+        lib.oracle_strainlimit_toi.restype = f32
+        lib._sl_ready = True
+    return lib
+
+
+def svd32(F):
+    """One-rotation Jacobi SVD of 3x2 matrices F (n, 3, 2):
+    U (n, 3, 2), S (n, 2) ascending, V (n, 2, 2)."""
+    lib = _load_strainlimit()
+    F = np.asarray(F, np.float32).reshape(-1, 3, 2)
+    n = F.shape[0]
+    Fc = np.ascontiguousarray(F.transpose(0, 2, 1)).reshape(-1)  # column-major
+    U = np.zeros(6 * n, np.float32)
+    S = np.zeros(2 * n, np.float32)
+    V = np.zeros(4 * n, np.float32)
+    lib.oracle_svd32(n, _p(Fc), _p(U), _p(S), _p(V))
+    return (U.reshape(n, 2, 3).transpose(0, 2, 1), S.reshape(n, 2),
+            V.reshape(n, 2, 2).transpose(0, 2, 1))
+
+
+def _sl_arrays(sh):
+    return (_u32(sh["face"]), _f32(sh["inv_rest"]), _u32(sh["fflag"]), _f32(sh["fmass"]),
+            _f32(sh["f_strainlimit"]),
+            _f32(sh["f_shrink"]) if sh.get("f_shrink") is not None else None)
+
+
+def strainlimit_faces(sh, x, x0, rp, ci, el, eig_eps=1e-2):
+    """Per-face strain-limit terms at positions x (stiffness from x0 and the
+    elastic blocks el over the upper pattern rp / ci): forces (nf, 9), upper
+    blocks (nf, 6, 9), stiffness (nf,), singular values (nf, 2), active (nf,)."""
+    lib = _load_strainlimit()
+    nf = int(sh["nf"])
+    face, inv, ffl, fm, sl, shr = _sl_arrays(sh)
+    x, x0, rp, ci, el = _f32(x), _f32(x0), _u32(rp), _u32(ci), _f32(el)
+    fo = np.zeros(9 * nf, np.float32)
+    bo = np.zeros(54 * nf, np.float32)
+    st = np.zeros(nf, np.float32)
+    S = np.zeros(2 * nf, np.float32)
+    act = np.zeros(nf, np.uint8)
+    lib.oracle_strainlimit_faces(nf, _p(face), _p(inv), _p(ffl), _p(fm), _p(sl),
+                                 _p(shr) if shr is not None else None, _p(x), _p(x0), _p(rp),
+                                 _p(ci), _p(el), eig_eps, _p(fo), _p(bo), _p(st), _p(S),
+                                 _p(act))
+    return fo.reshape(nf, 9), bo.reshape(nf, 6, 9), st, S.reshape(nf, 2), act
+
+
+def strainlimit_assemble(sh, x, x0, rp, ci, force, value, eig_eps=1e-2):
+    """Adds the strain-limit terms to copies of force (3 nv) and value (nnz, 9)."""
+    lib = _load_strainlimit()
+    face, inv, ffl, fm, sl, shr = _sl_arrays(sh)
+    force = np.array(force, np.float32).reshape(-1)
+    value = np.array(value, np.float32).reshape(-1, 9)
+    x, x0, rp, ci = _f32(x), _f32(x0), _u32(rp), _u32(ci)
+    lib.oracle_strainlimit_assemble(int(sh["nv"]), int(sh["nf"]), _p(face), _p(inv), _p(ffl),
+                                    _p(fm), _p(sl), _p(shr) if shr is not None else None,
+                                    _p(x), _p(x0), _p(rp), _p(ci), eig_eps, _p(force),
+                                    _p(value))
+    return force, value
+
+
+def strainlimit_toi(sh, x0, x1, max_t=1.25):
+    """Largest t <= max_t keeping every limited face below its limit along
+    x0 + t (x1 - x0) (unnormalized)."""
+    lib = _load_strainlimit()
+    face, inv, ffl, _, sl, shr = _sl_arrays(sh)
+    x0, x1 = _f32(x0), _f32(x1)
+    return float(lib.oracle_strainlimit_toi(int(sh["nf"]), _p(face), _p(inv), _p(ffl), _p(sl),
+                                            _p(shr) if shr is not None else None, _p(x0),
+                                            _p(x1), max_t))
+
+
+def plasticity(sh, x, dt):
+    """Rest-shape creep at positions x: (inv_rest (nf, 4), h_rest (nh,)) after
+    one committed step (copies; sh is not modified)."""
+    lib = load()
+    if not hasattr(lib, "_plastic_ready"):
+        lib.oracle_plasticity.argtypes = [u32, u32] + [vp] * 8 + [f32, vp, vp]
+        lib.oracle_plasticity.restype = None
+        lib._plastic_ready = True
+    inv = np.array(sh["inv_rest"], np.float32).reshape(-1, 4)
+    hr = np.array(sh["h_rest"], np.float32).reshape(-1)
+    opt = {k: (_f32(sh[k]) if sh.get(k) is not None else None)
+           for k in ("f_plasticity", "f_plastic_threshold", "h_plasticity",
+                     "h_plastic_threshold")}
+    face, hinge, hflag, x = _u32(sh["face"]), _u32(sh["hinge"]), _u32(sh["hflag"]), _f32(x)
+    lib.oracle_plasticity(int(sh["nf"]), int(sh["nh"]), _p(face), _p(hinge) if hinge.size else None,
+                          _p(hflag) if hflag.size else None, _p(x),
+                          *(_p(opt[k]) if opt[k] is not None else None
+                            for k in ("f_plasticity", "f_plastic_threshold", "h_plasticity",
+                                      "h_plastic_threshold")),
+                          dt, _p(inv), _p(hr) if hr.size else None)
+    return inv, hr
+
+
 def static_proximity(m, sm, vmass, emass, fmass):
     """Collision-mesh proximity pairs (kinds 4-6), queries in index order."""
     lib = _load_narrow()
@@ -604,7 +717,8 @@ class OracleStepParams(ctypes.Structure):
                 ("friction_mode", ctypes.c_int32), ("min_newton_steps", u32),
                 ("max_newton_steps", u32), ("air_density", f32), ("air_friction", f32),
                 ("wind", f32 * 3), ("disable_contact", ctypes.c_int32),
-                ("check_intersection", ctypes.c_int32), ("constraint_tol", f32)]
+                ("check_intersection", ctypes.c_int32), ("constraint_tol", f32),
+                ("eiganalysis_eps", f32)]
 
 
 class OracleStepResult(ctypes.Structure):

@@ -117,6 +117,7 @@ struct Backend {
     // device arrays the stepper reads through its scene
     float *curr = nullptr, *prev = nullptr, *fix_pos = nullptr, *fix_delta = nullptr;
     float *inv_rest = nullptr;
+    float *h_rest = nullptr; // HingeProp.rest_angle, rewritten by plasticity
     uint32_t *vflag = nullptr, *eflag = nullptr, *fflag = nullptr;
     uint32_t *sh_vflag = nullptr, *sh_fflag = nullptr, *hflag = nullptr;
     ppf_scene sc{};
@@ -388,8 +389,6 @@ const char *unsupported(const cts_DataSet &d, const cts_ParamSet &p) {
         if (k >= d.param_arrays.face.size) return "a face parameter index out of range";
         const cts_FaceParam &fp = d.param_arrays.face.data[k];
         if (fp.model != CTS_MODEL_BARAFF_WITKIN) return "shell models other than Baraff-Witkin";
-        if (fp.strainlimit > 0.0f) return "strain limiting";
-        if (fp.plasticity > 0.0f || fp.bend_plasticity > 0.0f) return "plasticity";
         if (fp.pressure != 0.0f) return "pressure";
         if (fp.shrink_x != 1.0f || fp.shrink_y != 1.0f) return "rest-shape shrink";
         if (fp.deform_damping != 0.0f || fp.bend_damping != 0.0f) return "Rayleigh damping";
@@ -398,7 +397,6 @@ const char *unsupported(const cts_DataSet &d, const cts_ParamSet &p) {
         const uint32_t k = d.prop.hinge.data[h].param_index;
         if (k >= d.param_arrays.hinge.size) return "a hinge parameter index out of range";
         const cts_HingeParam &hp = d.param_arrays.hinge.data[k];
-        if (hp.plasticity > 0.0f) return "plasticity";
         if (hp.bend_damping != 0.0f) return "Rayleigh damping";
     }
     for (uint32_t i = 0; i < d.prop.vertex.size; ++i)
@@ -500,8 +498,10 @@ void upload_scene(Backend &b) {
 
     std::vector<float> vg(nv), vo(nv), vfr(nv), vmass(nv), varea(nv);
     std::vector<float> eg(ne), eo(ne), efr(ne), emass(ne);
-    std::vector<float> fg(nf), fo(nf), ffr(nf), fmass(nf), mu(nf), lam(nf);
-    std::vector<float> hrest(nh), hlen(nh), harea(nh), hbend(nh);
+    std::vector<float> fg(nf), fo(nf), ffr(nf), fmass(nf), mu(nf), lam(nf), fsl(nf);
+    std::vector<float> fpl(nf), fplt(nf);
+    bool any_sl = false, any_fpl = false, any_hpl = false;
+    std::vector<float> hrest(nh), hlen(nh), harea(nh), hbend(nh), hpl(nh), hplt(nh);
     std::vector<uint32_t> vbody(nv, 0);
     for (uint32_t i = 0; i < nv; ++i) {
         const cts_VertexProp &vp = d.prop.vertex.data[i];
@@ -519,11 +519,17 @@ void upload_scene(Backend &b) {
         const cts_FaceParam &p = d.param_arrays.face.data[fp.param_index];
         fg[f] = p.ghat, fo[f] = p.offset, ffr[f] = p.friction, fmass[f] = fp.mass;
         mu[f] = p.mu, lam[f] = p.lambda;
+        fsl[f] = p.strainlimit;
+        any_sl = any_sl || p.strainlimit > 0.0f;
+        fpl[f] = p.plasticity, fplt[f] = p.plasticity_threshold;
+        any_fpl = any_fpl || p.plasticity > 0.0f;
     }
     for (uint32_t h = 0; h < nh; ++h) {
         const cts_HingeProp &hp = d.prop.hinge.data[h];
         const cts_HingeParam &p = d.param_arrays.hinge.data[hp.param_index];
         hrest[h] = hp.rest_angle, hlen[h] = hp.length, harea[h] = hp.area;
+        hpl[h] = p.plasticity, hplt[h] = p.plasticity_threshold;
+        any_hpl = any_hpl || p.plasticity > 0.0f;
         // hinge_bend_directional (data.hpp:319-326)
         hbend[h] = hp.uv_edge_sin2 < 0.0f
                        ? p.bend
@@ -558,8 +564,20 @@ void upload_scene(Backend &b) {
     sh.vmass = b.put(vmass), sh.varea = b.put(varea), sh.vflag = b.sh_vflag;
     sh.fmass = b.put(fmass), sh.inv_rest = b.inv_rest, sh.f_mu = b.put(mu),
     sh.f_lambda = b.put(lam), sh.fflag = b.sh_fflag;
-    sh.h_rest = b.put(hrest), sh.h_length = b.put(hlen), sh.h_area = b.put(harea),
+    b.h_rest = b.put(hrest);
+    sh.h_rest = b.h_rest, sh.h_length = b.put(hlen), sh.h_area = b.put(harea),
     sh.h_bend = b.put(hbend), sh.hflag = b.hflag;
+    // strain limits (shrink is refused, so every face's shrink is 1)
+    sh.f_strainlimit = any_sl ? b.put(fsl) : nullptr;
+    sh.f_shrink = nullptr;
+    // plasticity (plasticity.cu:21-78, 145-185): the stepper rewrites
+    // inv_rest and h_rest in place after each committed step
+    if (any_fpl) {
+        sh.f_plasticity = b.put(fpl), sh.f_plastic_threshold = b.put(fplt);
+    }
+    if (any_hpl) {
+        sh.h_plasticity = b.put(hpl), sh.h_plastic_threshold = b.put(hplt);
+    }
 
     b.sc.prev = b.prev;
     b.sc.emass = b.put(emass);
@@ -698,6 +716,7 @@ void advance(cts_StepResult *result) {
     sp.disable_contact = p.disable_contact ? 1 : 0;
     sp.check_intersection = p.disable_contact ? 0 : 1;
     sp.constraint_tol = p.constraint_tol;
+    sp.eiganalysis_eps = p.eiganalysis_eps;
     ppf_step_result r{};
     ppf_or_die(ppf_stepper_advance(g.stepper, &g.sc, &sp, &r, g.s), "advance");
     g.carried = false; // the step's own contact pattern replaces a carried one
@@ -753,9 +772,18 @@ void fetch_inv_rest(void) {
 }
 
 void fetch_rest_angles(void) {
-    // Rest angles change only under plasticity, which this backend refuses:
-    // the host arrays already hold the device values.
+    // HingeProp.rest_angle as bend plasticity left it (main.cu:1950-1968);
+    // rods (VertexProp.rest_bend_angle) are refused, so only hinges change
     require_ready("fetch_rest_angles");
+    if (g.nh) {
+        std::vector<float> rest(g.nh);
+        hip_or_die(hipMemcpy(rest.data(), g.h_rest, g.nh * sizeof(float),
+                             hipMemcpyDeviceToHost),
+                   "fetch_rest_angles");
+        for (uint32_t h = 0; h < g.nh; ++h) {
+            g.host.prop.hinge.data[h].rest_angle = rest[h];
+        }
+    }
 }
 
 void fetch_dyn_counts(uint32_t *n_value, uint32_t *n_offset) {

@@ -779,6 +779,71 @@ int ppf_elastic_assemble(ppf_elastic *h, const ppf_shell *sh, const ppf_elastic_
     return rc;
 }
 
+static bool strain_limit_shell_ok(const ppf_elastic *h, const ppf_shell *sh) {
+    return sh->nv == h->impl.nv() && sh->nf == h->impl.nf() &&
+           (!sh->nf || (sh->face && sh->inv_rest && sh->fflag));
+}
+
+int ppf_elastic_strain_limit(ppf_elastic *h, const ppf_shell *sh, float eiganalysis_eps,
+                             float *d_force, float *d_value, void *stream) {
+    if (!h || !sh || !d_force || (h->impl.nnz() && !d_value) || !sh->x || !sh->x0 ||
+        (sh->nf && !sh->fmass) || !strain_limit_shell_ok(h, sh) || !(eiganalysis_eps >= 0.0f)) {
+        set_error("ppf_elastic_strain_limit: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int rc = h->impl.strain_limit(*sh, eiganalysis_eps, d_force, d_value, st);
+    if (rc == PPF_OK) {
+        PPF_HIP(hipStreamSynchronize(st));
+    }
+    return rc;
+}
+
+int ppf_elastic_plasticity(ppf_elastic *h, const ppf_shell *sh, float dt, float *d_inv_rest,
+                           float *d_h_rest, void *stream) {
+    if (!h || !sh || !sh->x || !(dt > 0.0f) || sh->nv != h->impl.nv() ||
+        sh->nf != h->impl.nf() || sh->nh != h->impl.nh() ||
+        (sh->f_plasticity && sh->nf && (!sh->f_plastic_threshold || !sh->face || !d_inv_rest)) ||
+        (sh->h_plasticity && sh->nh &&
+         (!sh->h_plastic_threshold || !sh->hinge || !sh->hflag || !d_h_rest))) {
+        set_error("ppf_elastic_plasticity: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int rc = h->impl.plasticity(*sh, dt, nullptr, nullptr, d_inv_rest, d_h_rest, st);
+    if (rc == PPF_OK) {
+        PPF_HIP(hipStreamSynchronize(st));
+    }
+    return rc;
+}
+
+int ppf_elastic_strain_limit_toi(ppf_elastic *h, const ppf_shell *sh, const float *d_x0,
+                                 const float *d_x1, float line_search_max_t, float *toi,
+                                 void *stream) {
+    if (!h || !sh || !d_x0 || !d_x1 || !toi || !strain_limit_shell_ok(h, sh) ||
+        !(line_search_max_t > 0.0f)) {
+        set_error("ppf_elastic_strain_limit_toi: invalid argument");
+        return PPF_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    uint32_t *word = nullptr;
+    int rc = h->impl.toi_word(&word);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    const uint32_t init = __builtin_bit_cast(uint32_t, line_search_max_t);
+    PPF_HIP(hipMemcpyAsync(word, &init, sizeof init, hipMemcpyHostToDevice, st));
+    rc = h->impl.strain_limit_toi(*sh, d_x0, d_x1, line_search_max_t, nullptr, word, st);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    uint32_t bits = init;
+    PPF_HIP(hipMemcpyAsync(&bits, word, sizeof bits, hipMemcpyDeviceToHost, st));
+    PPF_HIP(hipStreamSynchronize(st));
+    *toi = __builtin_bit_cast(float, bits) / line_search_max_t;
+    return PPF_OK;
+}
+
 int ppf_stepper_create(const ppf_scene *scene, void *stream, ppf_stepper **out) {
     if (!scene || !out || !scene->mesh.x || !scene->prev || !scene->emass ||
         !scene->v_fric || !scene->e_fric || !scene->f_fric) {

@@ -408,6 +408,522 @@ __global__ __launch_bounds__(kNT) void k_face(ppf_shell sh, float *__restrict__ 
     }
 }
 
+// ---- strain limiting --------------------------------------------------------------
+//
+// strainlimiting.cu:17-100 (the barrier on the stretch beyond rest of each
+// singular value of F, scaled by an elastic-inclusive stiffness) and 243-305
+// (its line search), with the one-rotation Jacobi SVD of utility.cu:43-130,
+// the cubic barrier of cubic.hpp:13-40 and the 3x2 eigen-analysis of
+// eigenanalysis.cu:18-78. The strain-limit curvature table is diagonal, so
+// its two scaling modes are the singular directions themselves.
+
+// F = U diag(S) V^T of a 3x2 F (column-major), singular values ascending.
+struct Svd32 {
+    float U[6]; // 3x2, column-major
+    float S[2];
+    float V[4]; // 2x2, column-major
+};
+
+__device__ inline Svd32 svd32(const float F[6]) {
+    float A[3][2] = {{F[0], F[3]}, {F[1], F[4]}, {F[2], F[5]}};
+    float W[2][2] = {{1.0f, 0.0f}, {0.0f, 1.0f}};
+    float a = 0.0f, b = 0.0f, g = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        a += A[r][0] * A[r][0];
+        b += A[r][1] * A[r][1];
+        g += A[r][0] * A[r][1];
+    }
+    // the rotation that makes the two columns orthogonal (identity if they are)
+    const float zeta = (b - a) / (2.0f * g);
+    const float t = (g == 0.0f) ? 0.0f
+                                : (zeta >= 0.0f ? 1.0f : -1.0f) /
+                                      (fabsf(zeta) + sqrtf(1.0f + zeta * zeta));
+    const float c = 1.0f / sqrtf(1.0f + t * t), sn = c * t;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const float ap = A[r][0], aq = A[r][1];
+        A[r][0] = c * ap - sn * aq;
+        A[r][1] = sn * ap + c * aq;
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const float vp = W[r][0], vq = W[r][1];
+        W[r][0] = c * vp - sn * vq;
+        W[r][1] = sn * vp + c * vq;
+    }
+    float sv[2], u[3][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        float n2 = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) n2 += A[r][k] * A[r][k];
+        const float nrm = sqrtf(n2);
+        sv[k] = nrm;
+        const float inv = nrm > 0.0f ? 1.0f / nrm : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) u[r][k] = A[r][k] * inv;
+    }
+    const int i0 = sv[1] < sv[0] ? 1 : 0, i1 = 1 - i0;
+    Svd32 out;
+    out.S[0] = sv[i0], out.S[1] = sv[i1];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        out.U[r] = u[r][i0];
+        out.U[3 + r] = u[r][i1];
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        out.V[r] = W[r][i0];
+        out.V[2 + r] = W[r][i1];
+    }
+    return out;
+}
+
+__device__ inline float sl_limit(const ppf_shell &sh, uint32_t k) {
+    const float shrink = sh.f_shrink ? sh.f_shrink[k] : 1.0f;
+    return (sh.f_strainlimit[k] + 1.0f) / shrink - 1.0f;
+}
+
+// F = dX inv_rest of face k at positions x (column-major, as k_face).
+__device__ inline void face_F(const float *x, const uint32_t v[3], const float *inv, float F[6]) {
+    const f3 x0 = X(x, v[0]), x1 = X(x, v[1]), x2 = X(x, v[2]);
+    const float i0 = inv[0], i1 = inv[1], i2 = inv[2], i3 = inv[3];
+    const f3 d0 = x1 - x0, d1 = x2 - x0;
+    F[0] = (0.0f + d0.x * i0) + d1.x * i1, F[1] = (0.0f + d0.y * i0) + d1.y * i1;
+    F[2] = (0.0f + d0.z * i0) + d1.z * i1, F[3] = (0.0f + d0.x * i2) + d1.x * i3;
+    F[4] = (0.0f + d0.y * i2) + d1.y * i3, F[5] = (0.0f + d0.z * i2) + d1.z * i3;
+}
+
+// Block (i, j) of the upper-stored elastic pattern (zero when absent),
+// transposed when i > j (FixedCSRMat's full-matrix view).
+__device__ inline void pattern_block(const uint32_t *rp, const uint32_t *ci, const float *val,
+                                     uint32_t i, uint32_t j, float blk[9]) {
+    const uint32_t r = i < j ? i : j, c = i < j ? j : i;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) blk[e] = 0.0f;
+    for (uint32_t k = rp[r]; k < rp[r + 1]; ++k) {
+        if (ci[k] == c) {
+            const float *b = val + 9 * (size_t)k;
+            if (i <= j) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) blk[e] = b[e];
+            } else {
+#pragma unroll
+                for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) blk[rr + 3 * cc] = b[cc + 3 * rr];
+            }
+            break;
+        }
+    }
+}
+
+// M = U3 Q V^T (3x2, column-major) of a 3x2 mode Q (column-major) with
+// U3 = [U0, U1, U0 x U1].
+__device__ inline void sl_mode(const float U3[9], const float Q[6], const float V[4], float M[6]) {
+    float T[6];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc = acc + U3[r + 3 * k] * Q[k + 3 * c];
+            T[r + 3 * c] = acc;
+        }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc = acc + T[r + 3 * j] * V[c + 2 * j];
+            M[r + 3 * c] = acc;
+        }
+}
+
+// Strain-limit terms of every limited face, staged like k_face's (forces per
+// face vertex, the six upper blocks per face) with an active flag.
+__global__ __launch_bounds__(kNT) void k_sl_face(ppf_shell sh, const uint32_t *__restrict__ rp,
+                                                 const uint32_t *__restrict__ ci,
+                                                 const float *__restrict__ el, float eig_eps,
+                                                 float *__restrict__ slblk,
+                                                 float *__restrict__ slf,
+                                                 uint8_t *__restrict__ active) {
+    const uint32_t k = blockIdx.x * kNT + threadIdx.x;
+    if (k >= sh.nf) {
+        return;
+    }
+    active[k] = 0;
+    const float sl = sh.f_strainlimit[k];
+    if ((sh.fflag[k] & 1u) || !(sl > 0.0f)) {
+        return;
+    }
+    const uint32_t v[3] = {sh.face[3 * (size_t)k], sh.face[3 * (size_t)k + 1],
+                           sh.face[3 * (size_t)k + 2]};
+    const float *inv = sh.inv_rest + 4 * (size_t)k;
+    float F[6];
+    face_F(sh.x, v, inv, F);
+    const Svd32 svd = svd32(F);
+    float gs[2] = {svd.S[0] - 1.0f, svd.S[1] - 1.0f}; // stretch beyond rest
+    if (!(fmaxf(gs[0], gs[1]) > 0.0f)) {
+        return;
+    }
+    active[k] = 1;
+    // barrier on sl - stretch with gap sl (cubic.hpp:22-40, offset 0)
+    float deda[2] = {0.0f, 0.0f}, h[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        if (gs[i] > 0.0f) {
+            const float y = (sl - gs[i]) - 0.0f;
+            const float yy = y - sl;
+            deda[i] = -(yy < 0.0f ? -2.0f * yy * yy / sl : 0.0f);
+            h[i] = y - sl < 0.0f ? 4.0f * (1.0f - y / sl) : 0.0f;
+        }
+    }
+    // elastic-inclusive stiffness at the step start: w^T H_el w + m / d^2
+    float stiff = 0.0f;
+    {
+        const f3 y0 = X(sh.x0, v[0]), y1 = X(sh.x0, v[1]), y2 = X(sh.x0, v[2]);
+        const float s3 = 1.0f / 3.0f;
+        const f3 cen = (s3 * y0 + s3 * y1) + s3 * y2;
+        const f3 wv[3] = {y0 - cen, y1 - cen, y2 - cen};
+        for (int ii = 0; ii < 3; ++ii) {
+            float hv[3] = {0.0f, 0.0f, 0.0f};
+            for (int jj = 0; jj < 3; ++jj) {
+                float blk[9];
+                pattern_block(rp, ci, el, v[ii], v[jj], blk);
+                const float w[3] = {wv[jj].x, wv[jj].y, wv[jj].z};
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) hv[r] = hv[r] + blk[r + 3 * c] * w[c];
+            }
+            stiff = stiff + ((wv[ii].x * hv[0] + wv[ii].y * hv[1]) + wv[ii].z * hv[2]);
+        }
+        const float d = sl_limit(sh, k) - fmaxf(gs[0], gs[1]);
+        stiff = stiff + sh.fmass[k] / (d * d);
+    }
+    const float a0 = gs[0] + 1.0f, a1 = gs[1] + 1.0f;
+    // dE/dF = U diag(deda) V^T
+    float dF[6];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            dF[r + 3 * c] = (0.0f + (svd.U[r] * deda[0]) * svd.V[c]) +
+                            (svd.U[3 + r] * deda[1]) * svd.V[c + 2];
+    // projected d2E/dF2: six modes (twist, flip, two out-of-plane, two scaling)
+    float U3[9];
+    {
+        const f3 u0 = mk3(svd.U[0], svd.U[1], svd.U[2]), u1 = mk3(svd.U[3], svd.U[4], svd.U[5]);
+        const f3 u2 = cross3(u0, u1);
+        U3[0] = u0.x, U3[1] = u0.y, U3[2] = u0.z, U3[3] = u1.x, U3[4] = u1.y, U3[5] = u1.z;
+        U3[6] = u2.x, U3[7] = u2.y, U3[8] = u2.z;
+    }
+    const float is2 = 0.7071067811865475f;
+    const float denom = a0 - a1;
+    float lam[6];
+    lam[0] = fmaxf(0.0f, (deda[0] + deda[1]) / (a0 + a1));
+    lam[1] = fmaxf(0.0f, fabsf(denom) > eig_eps ? (deda[0] - deda[1]) / denom
+                                                : 0.5f * (h[0] + h[1]) - 0.5f * (0.0f + 0.0f));
+    lam[2] = fmaxf(0.0f, deda[0] / a0);
+    lam[3] = fmaxf(0.0f, deda[1] / a1);
+    // scaling modes in ascending curvature order
+    const int lo = h[1] < h[0] ? 1 : 0;
+    lam[4] = fmaxf(0.0f, h[lo]);
+    lam[5] = fmaxf(0.0f, h[1 - lo]);
+    float H6[36];
+#pragma unroll
+    for (int e = 0; e < 36; ++e) H6[e] = 0.0f;
+    for (int i = 0; i < 6; ++i) {
+        if (lam[i] == 0.0f) {
+            continue;
+        }
+        float Q[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (i == 0) {
+            Q[3] = is2, Q[1] = -is2; // (0,1) = 1/sqrt2, (1,0) = -1/sqrt2
+        } else if (i == 1) {
+            Q[3] = is2, Q[1] = is2;
+        } else if (i == 2) {
+            Q[2] = 1.0f;             // (2,0)
+        } else if (i == 3) {
+            Q[5] = 1.0f;             // (2,1)
+        } else {
+            const int ax = i == 4 ? lo : 1 - lo;
+            Q[ax + 3 * ax] = 1.0f;   // (ax, ax)
+        }
+        float M[6];
+        sl_mode(U3, Q, svd.V, M);
+        for (int n = 0; n < 6; ++n)
+#pragma unroll
+            for (int m = 0; m < 6; ++m) H6[m + 6 * n] = H6[m + 6 * n] + (lam[i] * M[m]) * M[n];
+    }
+    const float i0 = inv[0], i1 = inv[1], i2 = inv[2], i3 = inv[3];
+    const float gg[3][2] = {{-i0 - i1, -i2 - i3}, {i0, i2}, {i1, i3}};
+    float *fo = slf + 9 * (size_t)k;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float fe = (0.0f + gg[a][0] * dF[d]) + gg[a][1] * dF[3 + d];
+            fo[3 * a + d] = 0.0f + stiff * fe;
+        }
+    float *bo = slblk + 9 * kFacePairs * (size_t)k;
+    for (uint32_t p = 0; p < kFacePairs; ++p) {
+        uint32_t a = kFP[p][0], b = kFP[p][1];
+        if (v[a] > v[b]) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        float blk[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) blk[e] = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const float sc = gg[a][d] * gg[b][e];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+                        blk[r + 3 * c] = blk[r + 3 * c] + sc * H6[(3 * d + r) + 6 * (3 * e + c)];
+            }
+#pragma unroll
+        for (int e = 0; e < 9; ++e) bo[9 * p + e] = 0.0f + stiff * blk[e];
+    }
+}
+
+// Adds the staged strain-limit blocks to the stored blocks, per block in
+// face order (the faces' share of the elastic slot table).
+__global__ __launch_bounds__(kNT) void k_sl_gather_blocks(
+    uint32_t nnz, uint32_t nf, const uint32_t *__restrict__ slot_off,
+    const uint32_t *__restrict__ src, const float *__restrict__ slblk,
+    const uint8_t *__restrict__ active, float *__restrict__ value) {
+    const uint32_t s = blockIdx.x * kNT + threadIdx.x;
+    if (s >= nnz) {
+        return;
+    }
+    const uint32_t nfp = kFacePairs * nf;
+    float *o = value + 9 * (size_t)s;
+    float acc[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) acc[e] = o[e];
+    for (uint32_t c = slot_off[s]; c < slot_off[s + 1]; ++c) {
+        const uint32_t q = src[c];
+        if (q >= nfp || !active[q / kFacePairs]) {
+            continue;
+        }
+        const float *blk = slblk + 9 * (size_t)q;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            const float y = blk[e];
+            if (y != 0.0f) {
+                acc[e] += y;
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) o[e] = acc[e];
+}
+
+// Adds the staged strain-limit forces per vertex in face order.
+__global__ __launch_bounds__(kNT) void k_sl_gather_force(
+    uint32_t nv, uint32_t nf, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ src, const float *__restrict__ slf,
+    const uint8_t *__restrict__ active, float *__restrict__ force) {
+    const uint32_t i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= nv) {
+        return;
+    }
+    float f0 = force[3 * (size_t)i], f1 = force[3 * (size_t)i + 1], f2 = force[3 * (size_t)i + 2];
+    for (uint32_t c = off[i]; c < off[i + 1]; ++c) {
+        const uint32_t q = src[c];
+        if (q >= 3 * nf || !active[q / 3]) {
+            continue;
+        }
+        const float *g = slf + 9 * (size_t)(q / 3) + 3 * (q % 3);
+        f0 += g[0];
+        f1 += g[1];
+        f2 += g[2];
+    }
+    force[3 * (size_t)i] = f0;
+    force[3 * (size_t)i + 1] = f1;
+    force[3 * (size_t)i + 2] = f2;
+}
+
+// Largest stretch beyond rest of the two singular values of F.
+__device__ inline float max_stretch(const float F[6]) {
+    const Svd32 s = svd32(F);
+    return fmaxf(s.S[0] - 1.0f, s.S[1] - 1.0f);
+}
+
+// Strain-limit line search of one face per thread (strainlimiting.cu:243-305):
+// the sweep F0 + t (F1 - F0), t in [0, max_t]; a face whose stretch reaches
+// its limit by max_t is bisected to the last t below it.
+__global__ __launch_bounds__(kNT) void k_sl_toi(ppf_shell sh, const float *__restrict__ x0,
+                                                const float *__restrict__ x1, float max_t,
+                                                const uint32_t *__restrict__ vscene,
+                                                uint32_t *__restrict__ toi_bits) {
+    const uint32_t k = blockIdx.x * kNT + threadIdx.x;
+    if (k >= sh.nf) {
+        return;
+    }
+    const float sl = sh.f_strainlimit[k];
+    if ((sh.fflag[k] & 1u) || !(sl > 0.0f)) {
+        return;
+    }
+    const uint32_t v[3] = {sh.face[3 * (size_t)k], sh.face[3 * (size_t)k + 1],
+                           sh.face[3 * (size_t)k + 2]};
+    const float *inv = sh.inv_rest + 4 * (size_t)k;
+    const float limit = sl_limit(sh, k);
+    float F0[6], F1[6], dF[6], Ft[6];
+    face_F(x0, v, inv, F0);
+    face_F(x1, v, inv, F1);
+#pragma unroll
+    for (int e = 0; e < 6; ++e) dF[e] = F1[e] - F0[e];
+    float t = max_t;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) Ft[e] = F0[e] + t * dF[e];
+    if (max_stretch(Ft) >= limit) {
+        float upper = t, lower = 0.0f;
+        float window = upper - lower;
+        while (true) {
+            t = 0.5f * (upper + lower);
+#pragma unroll
+            for (int e = 0; e < 6; ++e) Ft[e] = F0[e] + t * dF[e];
+            if (max_stretch(Ft) - limit < 0.0f) {
+                lower = t;
+            } else {
+                upper = t;
+            }
+            const float nw = upper - lower;
+            if (nw == window) {
+                break;
+            }
+            window = nw;
+        }
+        t = lower;
+    }
+    if (t < max_t) {
+        atomicMin(toi_bits + (vscene ? vscene[v[0]] : 0u), __float_as_uint(t));
+    }
+}
+
+// ---- plasticity -------------------------------------------------------------------
+//
+// Rest-shape creep after a committed step (plasticity.cu:21-78 faces, 145-185
+// hinges): the singular values of F (faces) and the dihedral angle (hinges)
+// outside the dead zone |s - 1| <= threshold, |angle - rest| <= threshold are
+// pulled toward its edge by alpha = 1 - exp(-rate dt) (the reference takes the
+// hardware exponential there; __expf is gfx950's).
+
+__device__ inline f3 unit3(f3 v) {
+    const float z = dot3(v, v);
+    if (!(z > 0.0f)) {
+        return v;
+    }
+    const float n = sqrtf(z);
+    return mk3(v.x / n, v.y / n, v.z / n);
+}
+
+// One face per thread; faces of scenes that did not commit (ok[scene] == 0)
+// are skipped.
+__global__ __launch_bounds__(kNT) void k_plastic_face(ppf_shell sh, float dt,
+                                                      const uint32_t *__restrict__ vscene,
+                                                      const uint32_t *__restrict__ ok,
+                                                      float *__restrict__ inv_rest) {
+    const uint32_t k = blockIdx.x * kNT + threadIdx.x;
+    if (k >= sh.nf) {
+        return;
+    }
+    const float rate = sh.f_plasticity[k];
+    if (!(rate > 0.0f)) {
+        return;
+    }
+    const uint32_t v[3] = {sh.face[3 * (size_t)k], sh.face[3 * (size_t)k + 1],
+                           sh.face[3 * (size_t)k + 2]};
+    if (ok && !ok[vscene ? vscene[v[0]] : 0u]) {
+        return;
+    }
+    const float thr = sh.f_plastic_threshold[k];
+    const float alpha = 1.0f - __expf(-rate * dt);
+    float *inv = inv_rest + 4 * (size_t)k;
+    float F[6];
+    face_F(sh.x, v, inv, F);
+    const Svd32 svd = svd32(F);
+    float s[2] = {svd.S[0], svd.S[1]};
+    bool changed = false;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        if (fabsf(svd.S[i] - 1.0f) > thr) {
+            const float target = svd.S[i] < 1.0f ? 1.0f - thr : 1.0f + thr;
+            s[i] = svd.S[i] + alpha * (target - svd.S[i]);
+            changed = true;
+        }
+    }
+    if (!changed) {
+        return;
+    }
+    // F' = U diag(s) V^T
+    float Fn[6];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            Fn[r + 3 * c] = (0.0f + (svd.U[r] * s[0]) * svd.V[c]) + (svd.U[3 + r] * s[1]) * svd.V[c + 2];
+    // the face's in-plane frame (t along the first edge, b = n x t) maps dX
+    // and F' to 2x2; the new rest inverse is (P dX)^-1 (P F')
+    const f3 x0 = X(sh.x, v[0]);
+    const f3 d0 = X(sh.x, v[1]) - x0, d1 = X(sh.x, v[2]) - x0;
+    const f3 n = unit3(cross3(d0, d1));
+    const f3 t = unit3(d0);
+    const f3 b = unit3(cross3(n, t));
+    const float a00 = dot3(t, d0), a01 = dot3(t, d1), a10 = dot3(b, d0), a11 = dot3(b, d1);
+    const f3 fc0 = mk3(Fn[0], Fn[1], Fn[2]), fc1 = mk3(Fn[3], Fn[4], Fn[5]);
+    const float p00 = dot3(t, fc0), p01 = dot3(t, fc1), p10 = dot3(b, fc0), p11 = dot3(b, fc1);
+    const float det = a00 * a11 - a01 * a10;
+    const float id = 1.0f / det;
+    const float i00 = a11 * id, i01 = -a01 * id, i10 = -a10 * id, i11 = a00 * id;
+    inv[0] = i00 * p00 + i01 * p10;
+    inv[1] = i10 * p00 + i11 * p10;
+    inv[2] = i00 * p01 + i01 * p11;
+    inv[3] = i10 * p01 + i11 * p11;
+}
+
+__device__ float dihedral_angle(f3 v0, f3 v1, f3 v2, f3 v3);
+
+// One hinge per thread.
+__global__ __launch_bounds__(kNT) void k_plastic_hinge(ppf_shell sh, float dt,
+                                                       const uint32_t *__restrict__ vscene,
+                                                       const uint32_t *__restrict__ ok,
+                                                       float *__restrict__ h_rest) {
+    const uint32_t k = blockIdx.x * kNT + threadIdx.x;
+    if (k >= sh.nh || (sh.hflag[k] & 1u)) {
+        return;
+    }
+    const float rate = sh.h_plasticity[k];
+    if (!(rate > 0.0f)) {
+        return;
+    }
+    const uint32_t *h = sh.hinge + 4 * (size_t)k;
+    if (ok && !ok[vscene ? vscene[h[0]] : 0u]) {
+        return;
+    }
+    const float thr = sh.h_plastic_threshold[k];
+    const float alpha = 1.0f - __expf(-rate * dt);
+    const float theta = dihedral_angle(X(sh.x, h[2]), X(sh.x, h[1]), X(sh.x, h[0]), X(sh.x, h[3]));
+    const float rest = h_rest[k];
+    const float delta = theta - rest;
+    if (fabsf(delta) > thr) {
+        const float edge = delta > 0.0f ? thr : -thr;
+        h_rest[k] = rest + alpha * (delta - edge);
+    }
+}
+
 // ---- dihedral bending -----------------------------------------------------------
 
 __device__ float dihedral_angle(f3 v0, f3 v1, f3 v2, f3 v3) {
@@ -970,13 +1486,14 @@ ElasticAssembler::~ElasticAssembler() { release(); }
 void ElasticAssembler::release() {
     for (void *p : {(void *)rp_, (void *)ci_, (void *)slot_off_, (void *)slot_src_,
                     (void *)vtx_off_, (void *)vtx_src_, (void *)fblk_, (void *)hblk_,
-                    (void *)ff_, (void *)hf_, (void *)vf_}) {
+                    (void *)ff_, (void *)hf_, (void *)vf_, (void *)slblk_, (void *)slf_,
+                    (void *)toi_word_}) {
         if (p) {
             (void)dev_free(p);
         }
     }
-    rp_ = ci_ = slot_off_ = slot_src_ = vtx_off_ = vtx_src_ = nullptr;
-    fblk_ = hblk_ = ff_ = hf_ = vf_ = nullptr;
+    rp_ = ci_ = slot_off_ = slot_src_ = vtx_off_ = vtx_src_ = toi_word_ = nullptr;
+    fblk_ = hblk_ = ff_ = hf_ = vf_ = slblk_ = slf_ = nullptr;
     nnz_ = 0;
 }
 
@@ -1060,6 +1577,69 @@ int ElasticAssembler::create(const ppf_shell &topo, hipStream_t s) {
     (void)dev_free(head);
     (void)dev_free(sid);
     (void)dev_free(scratch);
+    return PPF_OK;
+}
+
+int ElasticAssembler::strain_limit(const ppf_shell &sh, float eig_eps, float *force,
+                                   float *value, hipStream_t s) {
+    if (!sh.f_strainlimit || !nf_) {
+        return PPF_OK;
+    }
+    if (!slblk_) {
+        PPF_HIP(dev_alloc(&slblk_, 9 * (size_t)kFacePairs * nf_ * sizeof(float)));
+        PPF_HIP(dev_alloc(&slf_, (9 * (size_t)nf_ + nf_ / 4 + 1) * sizeof(float)));
+    }
+    uint8_t *active = reinterpret_cast<uint8_t *>(slf_ + 9 * (size_t)nf_);
+    k_sl_face<<<cdiv(nf_, kNT), kNT, 0, s>>>(sh, rp_, ci_, value, eig_eps, slblk_, slf_, active);
+    PPF_HIP(hipGetLastError());
+    if (nnz_) {
+        k_sl_gather_blocks<<<cdiv(nnz_, kNT), kNT, 0, s>>>(nnz_, nf_, slot_off_, slot_src_,
+                                                          slblk_, active, value);
+        PPF_HIP(hipGetLastError());
+    }
+    if (nv_) {
+        k_sl_gather_force<<<cdiv(nv_, kNT), kNT, 0, s>>>(nv_, nf_, vtx_off_, vtx_src_, slf_,
+                                                         active, force);
+        PPF_HIP(hipGetLastError());
+    }
+    return PPF_OK;
+}
+
+int ElasticAssembler::plasticity(const ppf_shell &sh, float dt, const uint32_t *vscene,
+                                 const uint32_t *ok, float *inv_rest, float *h_rest,
+                                 hipStream_t s) {
+    if ((sh.f_plasticity && !sh.f_plastic_threshold) ||
+        (sh.h_plasticity && !sh.h_plastic_threshold)) {
+        set_error("elastic: plasticity rates without thresholds");
+        return PPF_ERR_ARG;
+    }
+    if (sh.f_plasticity && nf_) {
+        k_plastic_face<<<cdiv(nf_, kNT), kNT, 0, s>>>(sh, dt, vscene, ok, inv_rest);
+        PPF_HIP(hipGetLastError());
+    }
+    if (sh.h_plasticity && nh_) {
+        k_plastic_hinge<<<cdiv(nh_, kNT), kNT, 0, s>>>(sh, dt, vscene, ok, h_rest);
+        PPF_HIP(hipGetLastError());
+    }
+    return PPF_OK;
+}
+
+int ElasticAssembler::toi_word(uint32_t **out) {
+    if (!toi_word_) {
+        PPF_HIP(dev_alloc(&toi_word_, sizeof(uint32_t)));
+    }
+    *out = toi_word_;
+    return PPF_OK;
+}
+
+int ElasticAssembler::strain_limit_toi(const ppf_shell &sh, const float *x0, const float *x1,
+                                       float max_t, const uint32_t *vscene,
+                                       uint32_t *toi_bits, hipStream_t s) {
+    if (!sh.f_strainlimit || !nf_) {
+        return PPF_OK;
+    }
+    k_sl_toi<<<cdiv(nf_, kNT), kNT, 0, s>>>(sh, x0, x1, max_t, vscene, toi_bits);
+    PPF_HIP(hipGetLastError());
     return PPF_OK;
 }
 

@@ -43,7 +43,7 @@ __device__ inline uint32_t vsc(const uint32_t *vs, uint32_t i) { return vs ? vs[
 // consecutive float arrays; max |dx| bits, active flags and pair counts as
 // consecutive word arrays. A one-scene step uses the same arrays with ns = 1.
 enum { kDt = 0, kPrevDt = 1, kRecale = 2, kToi = 3, kBack = 4, kSceneFloats = 5 };
-enum { kMaxNorm = 0, kActive = 1, kPairs = 2, kSceneWords = 3 };
+enum { kMaxNorm = 0, kActive = 1, kPairs = 2, kSlToi = 3, kSceneWords = 4 };
 
 // Inertial target (main.cu:491-511), with each scene's dt and prev_dt.
 __global__ void k_target(uint32_t n, const float *__restrict__ curr,
@@ -598,9 +598,10 @@ int Stepper::create(const ppf_scene &sc, hipStream_t s) {
         }
     }
     PPF_HIP(dev_alloc(&sf_, 5 * (size_t)ns_ * sizeof(float)));
-    PPF_HIP(dev_alloc(&su_, 3 * (size_t)ns_ * sizeof(uint32_t)));
+    PPF_HIP(dev_alloc(&su_, kSceneWords * (size_t)ns_ * sizeof(uint32_t)));
     PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_sf_), 5 * (size_t)ns_ * sizeof(float)));
-    PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_su_), 3 * (size_t)ns_ * sizeof(uint32_t)));
+    PPF_HIP(host_alloc(reinterpret_cast<void **>(&h_su_),
+                       kSceneWords * (size_t)ns_ * sizeof(uint32_t)));
     if (nv_ > 1 && ns_ == 1) {
         std::vector<float> hx(n3);
         std::vector<uint32_t> hf(3 * (size_t)nf_), order;
@@ -786,6 +787,12 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
         return rc;
     }
     rc = analytic_assemble(m, cp, sph_, nsph_, flr_, nflr_, p.constraint_tol, cgrad_, cdiag_, s);
+    if (rc != PPF_OK) {
+        return rc;
+    }
+    // strain limiting (main.cu:809-822): its stiffness reads the elastic
+    // blocks alone, as the contact's did (tmp_fixed), then its terms join
+    rc = elastic_.strain_limit(sh, p.eiganalysis_eps, force_, eval_, s);
     if (rc != PPF_OK) {
         return rc;
     }
@@ -983,8 +990,32 @@ int Stepper::newton_iteration(const ppf_scene &sc, const ppf_step_params &p, flo
             }
         }
     }
+    const bool sl = sc.shell.f_strainlimit != nullptr;
+    if (sl) {
+        // the strain-limit line search over the same step joins the minimum
+        // (main.cu:1321-1343)
+        const uint32_t tb = __builtin_bit_cast(uint32_t, p.line_search_max_t);
+        for (uint32_t c = 0; c < ns; ++c) {
+            h_su_[kSlToi * ns + c] = tb;
+        }
+        PPF_HIP(hipMemcpyAsync(su_ + kSlToi * ns, h_su_ + kSlToi * ns, ns * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, s));
+        rc = elastic_.strain_limit_toi(sc.shell, target_, ex_, p.line_search_max_t, vs,
+                                       su_ + kSlToi * ns, s);
+        if (rc != PPF_OK) {
+            return rc;
+        }
+        PPF_HIP(hipMemcpyAsync(h_su_ + kSlToi * ns, su_ + kSlToi * ns, ns * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, s));
+    }
     PPF_HIP(hipEventRecord(ev_[4], s));
     PPF_HIP(hipEventSynchronize(ev_[4]));
+    if (sl) {
+        for (uint32_t c = 0; c < ns; ++c) {
+            const float t = __builtin_bit_cast(float, h_su_[kSlToi * ns + c]);
+            toi_out[c] = fminf(toi_out[c], t / p.line_search_max_t);
+        }
+    }
     float ms;
     PPF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
     const float ms_el = ms;
@@ -1254,6 +1285,20 @@ int Stepper::advance_scenes(const ppf_scene &sc, const ppf_step_params &p, const
         k_commit<<<cdiv(n, kNT), kNT, 0, s>>>(n, vs, su_ + kActive * ns, sc.prev,
                                                const_cast<float *>(sc.mesh.x), ex_);
         PPF_HIP(hipGetLastError());
+    }
+    // rest-shape creep of the scenes that committed, at their new positions,
+    // with the step's requested dt (main.cu:1615-1624); the shell's inv_rest
+    // and h_rest are rewritten in place
+    if ((sc.shell.f_plasticity || sc.shell.h_plasticity) &&
+        (vs || out[0].status == 0 || out[0].status == 4)) {
+        ppf_shell sh = sc.shell;
+        sh.x = sc.mesh.x;
+        const int prc = elastic_.plasticity(sh, p.dt, vs, vs ? su_ + kActive * ns : nullptr,
+                                            const_cast<float *>(sc.shell.inv_rest),
+                                            const_cast<float *>(sc.shell.h_rest), s);
+        if (prc != PPF_OK) {
+            return prc;
+        }
     }
     PPF_HIP(hipEventRecord(ev_[6], s));
     PPF_HIP(hipEventSynchronize(ev_[6]));

@@ -172,6 +172,13 @@ int main(int argc, char **argv) {
     const std::vector<float> f_mass = pk.f("f_mass"), f_mu = pk.f("f_mu"),
                              f_lam = pk.f("f_lambda"), f_ghat = pk.f("f_ghat"),
                              f_off = pk.f("f_off"), f_fric = pk.f("f_fric");
+    const std::vector<float> f_sl = pk.has("f_strainlimit") ? pk.f("f_strainlimit")
+                                                            : std::vector<float>(nf, 0.0f);
+    const std::vector<float> f_pl = pk.has("f_plasticity") ? pk.f("f_plasticity")
+                                                           : std::vector<float>(nf, 0.0f);
+    const std::vector<float> f_plt = pk.has("f_plastic_threshold")
+                                         ? pk.f("f_plastic_threshold")
+                                         : std::vector<float>(nf, 0.0f);
     std::vector<cts_FaceProp> fprop(nf);
     std::vector<cts_FaceParam> fparam(nf);
     for (uint32_t f = 0; f < nf; ++f) {
@@ -184,9 +191,16 @@ int main(int argc, char **argv) {
         p.mu = f_mu[f], p.lambda = f_lam[f], p.friction = f_fric[f];
         p.ghat = f_ghat[f], p.offset = f_off[f];
         p.shrink_x = p.shrink_y = 1.0f;
+        p.strainlimit = f_sl[f];
+        p.plasticity = f_pl[f], p.plasticity_threshold = f_plt[f];
     }
     const std::vector<float> h_rest = pk.f("h_rest"), h_len = pk.f("h_length"),
                              h_area = pk.f("h_area"), h_bend = pk.f("h_bend");
+    const std::vector<float> h_pl = pk.has("h_plasticity") ? pk.f("h_plasticity")
+                                                           : std::vector<float>(nh, 0.0f);
+    const std::vector<float> h_plt = pk.has("h_plastic_threshold")
+                                         ? pk.f("h_plastic_threshold")
+                                         : std::vector<float>(nh, 0.0f);
     std::vector<cts_HingeProp> hprop(nh);
     std::vector<cts_HingeParam> hparam(nh);
     for (uint32_t h = 0; h < nh; ++h) {
@@ -196,6 +210,7 @@ int main(int argc, char **argv) {
         hprop[h].param_index = h;
         hparam[h] = cts_HingeParam{};
         hparam[h].bend = h_bend[h];
+        hparam[h].plasticity = h_pl[h], hparam[h].plasticity_threshold = h_plt[h];
     }
     set_vec(ds.prop.vertex, vprop.data(), nv);
     set_vec(ds.prop.edge, eprop.data(), ne);
@@ -327,6 +342,7 @@ int main(int argc, char **argv) {
     param.cg_tol = (float)pk.d("cg_tol");
     param.constraint_tol = pk.has("constraint_tol") ? (float)pk.d("constraint_tol") : 0.01f;
     param.cg_max_iter = (uint32_t)pk.d("cg_max_iter");
+    param.eiganalysis_eps = pk.has("eiganalysis_eps") ? (float)pk.d("eiganalysis_eps") : 1e-2f;
     param.target_toi = (float)pk.d("target_toi");
     param.line_search_max_t = (float)pk.d("line_search_max_t");
     param.ccd_eps = (float)pk.d("ccd_eps");
@@ -424,11 +440,19 @@ int main(int argc, char **argv) {
     }
     uint32_t n_value = 0, n_offset = 0;
     fetch_dyn_counts(&n_value, &n_offset);
+    // the rest shape as the run left it (plasticity), as a checkpointing host
+    // pulls it
+    fetch_inv_rest();
+    fetch_rest_angles();
+    std::vector<float> rest_angle(nh);
+    for (uint32_t h = 0; h < nh; ++h) {
+        rest_angle[h] = hprop[h].rest_angle;
+    }
     FILE *fp = fopen(argv[2], "wb");
     if (!fp) {
         return 2;
     }
-    const uint32_t count = 10;
+    const uint32_t count = 12;
     fwrite("PPFPACK1", 1, 8, fp);
     fwrite(&count, 4, 1, fp);
     write_array(fp, "curr", 'f', curr.data(), curr.size());
@@ -443,6 +467,8 @@ int main(int argc, char **argv) {
     write_array(fp, "frees", 'u', frees.data(), frees.size());
     write_array(fp, "dyn0_offset", 'u', dyn0_off.data(), dyn0_off.size());
     write_array(fp, "dyn0_index", 'u', dyn0_idx.data(), dyn0_idx.size());
+    write_array(fp, "inv_rest", 'f', inv_rest.data(), inv_rest.size());
+    write_array(fp, "rest_angle", 'f', rest_angle.data(), rest_angle.size());
     fclose(fp);
     return rc;
 }

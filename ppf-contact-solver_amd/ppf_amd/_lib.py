@@ -97,6 +97,10 @@ _PROTOS = {
     "ppf_elastic_pattern": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                            ctypes.POINTER(u32)]),
     "ppf_elastic_assemble": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "ppf_elastic_plasticity": (ctypes.c_int, [vp, vp, ctypes.c_float, vp, vp, vp]),
+    "ppf_elastic_strain_limit": (ctypes.c_int, [vp, vp, ctypes.c_float, vp, vp, vp]),
+    "ppf_elastic_strain_limit_toi": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_float,
+                                                    ctypes.POINTER(ctypes.c_float), vp]),
     "ppf_stepper_create": (ctypes.c_int, [vp, vp, ctypes.POINTER(vp)]),
     "ppf_stepper_destroy": (None, [vp]),
     "ppf_stepper_advance": (ctypes.c_int, [vp, vp, vp, vp, vp]),

@@ -82,6 +82,10 @@ def scene_arrays(sc, params, steps, fix_schedule=None, delta_schedule=None, wind
     a["f_mass"], a["f_mu"], a["f_lambda"] = f32(sh["fmass"]), f32(sh["f_mu"]), f32(sh["f_lambda"])
     a["f_ghat"], a["f_off"], a["f_fric"] = f32(m["f_ghat"]), f32(m["f_off"]), f32(sc["fric"]["f"])
     a["inv_rest"] = f32(sh["inv_rest"])
+    for k in ("f_strainlimit", "f_plasticity", "f_plastic_threshold", "h_plasticity",
+              "h_plastic_threshold"):
+        if sh.get(k) is not None:
+            a[k] = f32(sh[k])
     for k in ("h_rest", "h_length", "h_area", "h_bend"):
         a[k] = f32(sh[k])
     pins = np.nonzero(np.asarray(m["vflag"]) & 1)[0].astype(np.uint32)
@@ -115,5 +119,7 @@ def scene_arrays(sc, params, steps, fix_schedule=None, delta_schedule=None, wind
     a["wind"] = f32(params["wind"])
     for k in PARAM_KEYS:
         a[k] = np.array([params[k]], np.float64)
+    if "eiganalysis_eps" in params:
+        a["eiganalysis_eps"] = np.array([params["eiganalysis_eps"]], np.float64)
     a["steps"] = np.array([steps], np.uint32)
     return a

@@ -20,6 +20,10 @@ f32 = ctypes.c_float
 SHELL_U32 = ("face", "hinge", "v2f_off", "v2f", "vflag", "fflag", "hflag")
 SHELL_F32 = ("x", "x0", "target", "vmass", "varea", "fmass", "inv_rest", "f_mu",
              "f_lambda", "h_rest", "h_length", "h_area", "h_bend")
+# optional per-face / per-hinge arrays (absent: no strain limit, shrink 1, no
+# plasticity)
+SHELL_OPT_F32 = ("f_strainlimit", "f_shrink", "f_plasticity", "f_plastic_threshold",
+                 "h_plasticity", "h_plastic_threshold")
 
 
 class ShellC(ctypes.Structure):
@@ -29,7 +33,9 @@ class ShellC(ctypes.Structure):
                 ("vmass", vp), ("varea", vp), ("vflag", vp), ("fmass", vp),
                 ("inv_rest", vp), ("f_mu", vp), ("f_lambda", vp), ("fflag", vp),
                 ("h_rest", vp), ("h_length", vp), ("h_area", vp), ("h_bend", vp),
-                ("hflag", vp)]
+                ("hflag", vp), ("f_strainlimit", vp), ("f_shrink", vp),
+                ("f_plasticity", vp), ("f_plastic_threshold", vp), ("h_plasticity", vp),
+                ("h_plastic_threshold", vp)]
 
 
 class ElasticParamsC(ctypes.Structure):
@@ -45,12 +51,15 @@ def shell_to_device(sh, dev):
     for k in SHELL_U32:
         a = np.ascontiguousarray(sh[k], np.uint32).reshape(-1).view(np.int32)
         out[k] = torch.from_numpy(a).to(dev)
+    for k in SHELL_OPT_F32:
+        if sh.get(k) is not None:
+            out[k] = torch.from_numpy(np.ascontiguousarray(sh[k], np.float32).reshape(-1)).to(dev)
     return out
 
 
 def shell_struct(t):
     fields = {k: t[k] for k in ("nv", "nf", "nh")}
-    for k in SHELL_F32 + SHELL_U32:
+    for k in SHELL_F32 + SHELL_U32 + tuple(k for k in SHELL_OPT_F32 if t.get(k) is not None):
         ten = t[k]
         if not ten.is_cuda or not ten.is_contiguous():
             raise ValueError(f"{k}: expected a contiguous cuda tensor")
@@ -111,3 +120,36 @@ class ElasticAssembler:
                                             _ptr(force), _ptr(cdiag), _ptr(value), _stream()),
               "ppf_elastic_assemble")
         return force, cdiag, value
+
+    def strain_limit(self, t, force, value, eiganalysis_eps=1e-2):
+        """Adds the strain-limit terms at t['x'] (stiffness from t['x0'] and
+        the elastic blocks in `value`) to force and value in place
+        (strainlimiting.cu:46-100)."""
+        s = shell_struct(t)
+        check(self.lib.ppf_elastic_strain_limit(self._h, ctypes.byref(s), eiganalysis_eps,
+                                                _ptr(force), _ptr(value), _stream()),
+              "ppf_elastic_strain_limit")
+        return force, value
+
+    def plasticity(self, t, dt, inv_rest=None, h_rest=None):
+        """Rest-shape creep at t['x'] (plasticity.cu:21-78, 145-185):
+        inv_rest / h_rest (default t's own) are updated in place."""
+        s = shell_struct(t)
+        inv_rest = t["inv_rest"] if inv_rest is None else inv_rest
+        h_rest = t["h_rest"] if h_rest is None else h_rest
+        check(self.lib.ppf_elastic_plasticity(self._h, ctypes.byref(s), dt, _ptr(inv_rest),
+                                              _ptr(h_rest) if h_rest.numel() else None,
+                                              _stream()),
+              "ppf_elastic_plasticity")
+        return inv_rest, h_rest
+
+    def strain_limit_toi(self, t, x0, x1, line_search_max_t=1.25):
+        """Fraction of the step x0 -> x1 (in units of line_search_max_t) that
+        keeps every limited face below its limit (strainlimiting.cu:243-305)."""
+        s = shell_struct(t)
+        out = f32()
+        check(self.lib.ppf_elastic_strain_limit_toi(self._h, ctypes.byref(s), _ptr(x0), _ptr(x1),
+                                                    line_search_max_t, ctypes.byref(out),
+                                                    _stream()),
+              "ppf_elastic_strain_limit_toi")
+        return out.value

@@ -616,14 +616,19 @@ def icosphere(radius=0.3, subdiv=3):
 
 
 def drape_scene(res=64, size=1.0, height=0.05, radius=0.3, subdiv=3, ghat=1e-3, offset=0.0,
-                friction=0.5, density=0.2, young=1e4, poisson=0.25, bend=10.0, pins=None):
+                friction=0.5, density=0.2, young=1e4, poisson=0.25, bend=10.0, pins=None,
+                strain_limit=0.0, plasticity=(0.0, 0.0), bend_plasticity=(0.0, 0.0)):
     """Square cloth (res x res vertices, side `size`, in the xz plane, y up)
     centred `height` above a static icosphere of `radius` at the origin: the
     plain-method drape configuration (64 x 64, edge 1/63 m, 1,280-triangle
     sphere, gap 1e-3, friction 0.5, density 0.2, Young 1e4, Poisson 0.25, bend
-    10). `pins`: indices of pinned vertices (held in place). Returns a dict of
-    numpy arrays: mesh (contact mesh, x = current positions), shell, static,
-    prev, emass, fric, fix_pos."""
+    10). `pins`: indices of pinned vertices (held in place). `strain_limit`
+    > 0 limits every face's stretch (FaceParam.strainlimit; the frontend's
+    strain-limit, examples/headless.py uses 0.05). `plasticity` and
+    `bend_plasticity`: (rate, threshold) of every face's and hinge's rest-shape
+    creep (FaceParam.plasticity / HingeParam.plasticity; rate 0 = off).
+    Returns a dict of numpy arrays: mesh (contact mesh, x = current
+    positions), shell, static, prev, emass, fric, fix_pos."""
     faces = rect_faces(res, res).astype(np.uint32)
     i, j = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
     h = size / (res - 1)
@@ -646,6 +651,15 @@ def drape_scene(res=64, size=1.0, height=0.05, radius=0.3, subdiv=3, ghat=1e-3, 
         mesh["vflag"][np.asarray(pins)] |= 1
         shell["vflag"] = mesh["vflag"].copy()
         fix_pos = mesh["x"].copy()
+    if strain_limit > 0.0:
+        shell["f_strainlimit"] = np.full(len(faces), strain_limit, np.float32)
+    if plasticity[0] > 0.0:
+        shell["f_plasticity"] = np.full(len(faces), plasticity[0], np.float32)
+        shell["f_plastic_threshold"] = np.full(len(faces), plasticity[1], np.float32)
+    if bend_plasticity[0] > 0.0:
+        nh = len(shell["hinge"])
+        shell["h_plasticity"] = np.full(nh, bend_plasticity[0], np.float32)
+        shell["h_plastic_threshold"] = np.full(nh, bend_plasticity[1], np.float32)
     return dict(mesh=mesh, shell=shell, static=static, prev=mesh["x"].copy(), emass=emass,
                 fric=fric, fix_pos=fix_pos)
 
@@ -669,7 +683,7 @@ def step_params(dt=1.0 / 60.0, fps=60.0, **kw):
              target_toi=0.25, line_search_max_t=1.25, ccd_eps=1e-7, max_dx=1.0,
              friction_eps=1e-5, friction_mode=0, min_newton_steps=0, max_newton_steps=2048,
              air_density=1e-3, air_friction=0.2, wind=(0.0, 0.0, 0.0), disable_contact=0,
-             check_intersection=1, constraint_tol=0.01)
+             check_intersection=1, constraint_tol=0.01, eiganalysis_eps=1e-2)
     p.update(kw)
     return p
 

@@ -12,7 +12,7 @@ import torch
 from . import _lib
 from ._lib import check
 from .ccd import StaticMeshC, static_struct, static_to_device
-from .elastic import SHELL_F32, SHELL_U32, ShellC
+from .elastic import SHELL_F32, SHELL_OPT_F32, SHELL_U32, ShellC
 from .narrow import ContactMeshC, _ptr, _stream, mesh_struct, mesh_to_device
 
 vp = ctypes.c_void_p
@@ -69,7 +69,7 @@ class StepParamsC(ctypes.Structure):
                 ("friction_mode", i32), ("min_newton_steps", u32), ("max_newton_steps", u32),
                 ("air_density", f32), ("air_friction", f32), ("wind", f32 * 3),
                 ("disable_contact", i32), ("check_intersection", i32),
-                ("constraint_tol", f32)]
+                ("constraint_tol", f32), ("eiganalysis_eps", f32)]
 
 
 class StepResultC(ctypes.Structure):
@@ -103,6 +103,10 @@ def scene_to_device(sc, dev):
     for k in SHELL_U32:
         a = np.ascontiguousarray(sc["shell"][k], np.uint32).reshape(-1).view(np.int32)
         sh[k] = torch.from_numpy(a).to(dev)
+    for k in SHELL_OPT_F32:
+        if sc["shell"].get(k) is not None:
+            sh[k] = torch.from_numpy(np.ascontiguousarray(sc["shell"][k], np.float32)
+                                     .reshape(-1)).to(dev)
     t["shell"] = sh
     t["static"] = static_to_device(sc["static"], dev) if sc.get("static") else None
     f = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32).reshape(-1)).to(dev)
@@ -119,7 +123,7 @@ def scene_to_device(sc, dev):
 def scene_struct(t):
     sh = t["shell"]
     fields = {k: sh[k] for k in ("nv", "nf", "nh")}
-    for k in SHELL_F32 + SHELL_U32:
+    for k in SHELL_F32 + SHELL_U32 + SHELL_OPT_F32:
         if k in sh:
             fields[k] = sh[k].data_ptr() if sh[k].numel() else None
     st = static_struct(t["static"]) if t["static"] is not None else StaticMeshC(nv=0, ne=0, nf=0)

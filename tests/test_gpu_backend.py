@@ -88,6 +88,51 @@ def test_mock_host_drape_matches_stepper_and_oracle(dev, tmp_path):
     assert out["dyn_counts"][1] == sc["mesh"]["nv"] + 1
 
 
+def test_mock_host_strain_limited_drape_matches_stepper(dev, tmp_path):
+    # FaceParam.strainlimit through the host's DataSet: the backend's run is
+    # the stepper's with the same per-face limits, bit for bit.
+    sc = scenes.drape_scene(res=12, height=0.003, strain_limit=0.01)
+    params = scenes.step_params()
+    steps = 6
+    rc, out, _ = run_host(tmp_path, scene_arrays(sc, params, steps))
+    assert rc == 0
+    x, prev, dts = stepper_run(dev, copy.deepcopy(sc), params, steps)
+    assert np.array_equal(out["curr"].reshape(-1, 3).view(np.uint32), x.view(np.uint32))
+    assert np.array_equal(out["dt"], dts)
+    sc2 = scenes.drape_scene(res=12, height=0.003)
+    x2, _, _ = stepper_run(dev, sc2, params, steps)
+    assert not np.array_equal(x2, x)      # the limit changed the run
+
+
+def test_mock_host_plasticity_matches_stepper(dev, tmp_path):
+    # FaceParam / HingeParam plasticity through the host's DataSet: positions
+    # and the rest shape the host fetches afterwards (fetch_inv_rest,
+    # fetch_rest_angles) equal the stepper's run bit for bit.
+    sc = scenes.drape_scene(res=12, height=0.003, plasticity=(5.0, 0.001),
+                            bend_plasticity=(5.0, 0.01))
+    params = scenes.step_params()
+    steps = 6
+    rc, out, _ = run_host(tmp_path, scene_arrays(sc, params, steps))
+    assert rc == 0
+    sc2 = copy.deepcopy(sc)
+    t = scene_to_device(sc2, dev)
+    st = Stepper(t)
+    p = dict(params)
+    for _ in range(steps):
+        r = st.advance(p)
+        assert r["status"] == 0, r
+        p["prev_dt"] = r["dt"]
+    x = t["mesh"]["x"].cpu().numpy()
+    inv = t["shell"]["inv_rest"].cpu().numpy()
+    hr = t["shell"]["h_rest"].cpu().numpy()
+    st.close()
+    assert np.array_equal(out["curr"].view(np.uint32), x.view(np.uint32))
+    assert np.array_equal(out["inv_rest"].view(np.uint32), inv.view(np.uint32))
+    assert np.array_equal(out["rest_angle"].view(np.uint32), hr.view(np.uint32))
+    assert not np.array_equal(inv, np.asarray(sc["shell"]["inv_rest"], np.float32).reshape(-1))
+    assert not np.array_equal(hr, np.asarray(sc["shell"]["h_rest"], np.float32))
+
+
 def test_mock_host_moving_pins(dev, tmp_path):
     res = 12
     pins = [0, res - 1]
