@@ -52,8 +52,15 @@ constexpr float kNoiseSlack = 256.0f;       // solver.cu:69
 constexpr float kFloatEps = 1.19209290e-7f; // solver.cu:70
 // In-tile mirrors the symmetric SpMV form parks in LDS per workgroup (3
 // floats each: 24 KB); a tile's mirrors beyond it are read as in the split
-// form.
-constexpr uint32_t kSymCap = 2048;
+// form. Its tables pack an LDS place (kPlaceBits, kPlaceNone = none) with a
+// column into one word per stored slot, and a place (flagged kInTile) or the
+// mirrored slot into one word per mirror entry, so columns stay below
+// kSymMaxRows.
+constexpr uint32_t kPlaceBits = 11;
+constexpr uint32_t kPlaceNone = (1u << kPlaceBits) - 1u;
+constexpr uint32_t kSymCap = kPlaceNone;
+constexpr uint32_t kSymMaxRows = 1u << (32 - kPlaceBits);
+constexpr uint32_t kInTile = 0x80000000u;
 
 // Device-resident scalars of one solve. Written by exactly one thread of one
 // kernel and read only by later kernels (stream order), except `done`, whose
@@ -92,8 +99,9 @@ struct Args {
     const uint32_t *rp, *col, *tp, *tcol, *tslot, *dslot;
     const uint2 *ent;  // fixed-width entry table (ELL form), or null
     uint32_t ell_m;    // entries per lane of the ELL kernel (row width 4*m)
-    // symmetric form: per stored slot / per mirror entry, the entry's place
-    // among its tile's in-tile mirrors (LDS), or kNone
+    // symmetric form: per stored slot (column << kPlaceBits) | its mirror's
+    // place among the tile's in-tile mirrors (LDS) or kPlaceNone; per mirror
+    // entry kInTile | that place, or the stored slot it mirrors
     const uint32_t *mq, *tq;
     const float *val, *cdiag, *b;
     float *x, *P, *r, *z, *p, *Ap;
@@ -540,9 +548,18 @@ __global__ __launch_bounds__(kNT) void k_spmv_split(const Args *__restrict__ A,
 // tile). A tile's mirror entries are contiguous in the transpose table (it is
 // ordered by destination row); an entry whose source row lies in the tile too
 // gets the next place among the tile's in-tile mirrors, while places last
-// (kSymCap), and both the entry (tq) and the stored slot it mirrors (mq) hold
-// it. Every other entry, and every stored slot not mirrored in its tile, holds
-// kNone (mq is preset to kNone). *max_inner receives the largest count.
+// (kSymCap), and both the entry (tq, flagged kInTile) and the stored slot it
+// mirrors (mq, beside the slot's column) hold it. Every other entry holds the
+// slot it mirrors, and every stored slot not mirrored in its tile holds
+// kPlaceNone (k_sym_init presets mq). *max_inner receives the largest count.
+__global__ __launch_bounds__(kNT) void k_sym_init(uint32_t nnz, const uint32_t *__restrict__ col,
+                                                  uint32_t *__restrict__ mq) {
+    const uint32_t s = blockIdx.x * kNT + threadIdx.x;
+    if (s < nnz) {
+        mq[s] = (col[s] << kPlaceBits) | kPlaceNone;
+    }
+}
+
 __global__ __launch_bounds__(kNT) void k_sym_prep(const Args *__restrict__ A,
                                                   uint32_t *__restrict__ mq,
                                                   uint32_t *__restrict__ tq,
@@ -576,10 +593,12 @@ __global__ __launch_bounds__(kNT) void k_sym_prep(const Args *__restrict__ A,
         __syncthreads();
         if (t < te) {
             const uint32_t pos = running + before + below;
-            const uint32_t v = in && pos < kSymCap ? pos : kNone;
-            tq[t] = v;
-            if (v != kNone) {
-                mq[tslot[t]] = v;
+            const uint32_t slot = tslot[t];
+            if (in && pos < kSymCap) {
+                tq[t] = kInTile | pos;
+                mq[slot] = (as_global(A->col)[slot] << kPlaceBits) | pos;
+            } else {
+                tq[t] = slot;
             }
         }
         running += total;
@@ -613,10 +632,8 @@ __global__ __launch_bounds__(kNT) void k_spmv_sym(const Args *__restrict__ A,
     }
     const uint32_t q = threadIdx.x & 3u, sub = threadIdx.x >> 2;
     gptr<const uint32_t> rp = as_global(A->rp);
-    gptr<const uint32_t> col = as_global(A->col);
     gptr<const uint32_t> tp = as_global(A->tp);
     gptr<const uint32_t> tcol = as_global(A->tcol);
-    gptr<const uint32_t> tslot = as_global(A->tslot);
     gptr<const uint32_t> mq = as_global(A->mq);
     gptr<const uint32_t> tq = as_global(A->tq);
     gptr<const float> val = as_global(A->val);
@@ -642,9 +659,10 @@ __global__ __launch_bounds__(kNT) void k_spmv_sym(const Args *__restrict__ A,
             for (int m = 0; m < M; ++m) {
                 const uint32_t k = k0 + q + 4 * m;
                 const bool ok = k < ns;
+                const uint32_t w = ok ? mq[r0 + k] : kPlaceNone;
                 sl[m] = ok ? r0 + k : kNone;
-                cj[m] = ok ? col[r0 + k] : 0u;
-                mp[m] = ok ? mq[r0 + k] : kNone;
+                cj[m] = w >> kPlaceBits;
+                mp[m] = w & kPlaceNone;
             }
             float blk[M][9];
             f3 vj[M];
@@ -663,7 +681,7 @@ __global__ __launch_bounds__(kNT) void k_spmv_sym(const Args *__restrict__ A,
                     const f3 t = mv(blk[m], vj[m]);
                     sum = sum + t;
                     as = as + mk3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
-                    if (mp[m] != kNone) {
+                    if (mp[m] != kPlaceNone) {
                         const f3 u = mvt(blk[m], vi);
                         mir[3 * mp[m]] = u.x;
                         mir[3 * mp[m] + 1] = u.y;
@@ -680,17 +698,12 @@ __global__ __launch_bounds__(kNT) void k_spmv_sym(const Args *__restrict__ A,
             for (int m = 0; m < M; ++m) {
                 const uint32_t u = u0 + 4 * m;
                 const bool ok = u < nm;
-                // the three loads are independent (no extra dependent round
+                // the two loads are independent (no extra dependent round
                 // trip for the entries read from memory)
-                tv[m] = ok ? tq[t0 + u] : kNone;
+                const uint32_t w = ok ? tq[t0 + u] : 0u;
                 src[m] = ok ? tcol[t0 + u] : 0u;
-                sl[m] = ok ? tslot[t0 + u] : kNone;
-            }
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                if (tv[m] != kNone) {
-                    sl[m] = kMirror; // taken from LDS
-                }
+                tv[m] = w & kPlaceNone;
+                sl[m] = !ok ? kNone : (w & kInTile) ? kMirror /* taken from LDS */ : w;
             }
             float blk[M][9];
             f3 vj[M];
@@ -1550,7 +1563,7 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
         const double avg = n ? (2.0 * nnz - n) / n : 0.0;
         const uint32_t m = (uint32_t)((avg + 1.0 + 3.999) / 4.0);
         pl->split_m = m < 2 ? 2 : (m > 4 ? 4 : m);
-        const bool sym_ok = pl->nscene == 1 && !pl->arena;
+        const bool sym_ok = pl->nscene == 1 && !pl->arena && n <= kSymMaxRows;
         switch (pl->form) {
         case PPF_SPMV_ELL:
             if (!h.ell_m) {
@@ -1563,7 +1576,8 @@ int set_pattern_arrays(PcgPlanImpl *pl, uint32_t nnz, const uint32_t *rp, const 
         case PPF_SPMV_SPLIT: pl->split = true; break;
         case PPF_SPMV_SYM:
             if (!sym_ok) {
-                set_error("pcg: the symmetric SpMV form needs a one-scene plan");
+                set_error("pcg: the symmetric SpMV form needs a one-scene plan of at most %u rows",
+                          kSymMaxRows);
                 return PPF_ERR_ARG;
             }
             pl->split = true;
@@ -1610,7 +1624,10 @@ int build_sym(PcgPlanImpl *pl, hipStream_t st) {
         pl->sym_cap = cap;
     }
     uint32_t *mq = pl->symmem, *tq = pl->symmem + pl->sym_cap / 2;
-    PPF_HIP(hipMemsetAsync(mq, 0xff, (size_t)std::max(pl->nnz, 1u) * sizeof(uint32_t), st));
+    if (pl->nnz) {
+        k_sym_init<<<cdiv(pl->nnz, kNT), kNT, 0, st>>>(pl->nnz, h.col, mq);
+        PPF_HIP(hipGetLastError());
+    }
     PPF_HIP(hipMemsetAsync(pl->flags + 3, 0, sizeof(uint32_t), st));
     h.mq = mq;
     h.tq = tq;
