@@ -267,63 +267,83 @@ __device__ inline v4s load_frag(const Seg &s, uint32_t row, uint32_t col) {
 }
 
 // Y[m][0:16 NT] = act(sum over segments X_seg[m] B_seg + bias) on
-// v_mfma_f32_16x16x16_bf16: one wave per 16 rows, NT column tiles of 16.
-// Bt is the weight transposed, [16 NT][ktot] bf16 (row = output column).
+// v_mfma_f32_16x16x16_bf16, K = 16 KS (the segments' widths summed). Each
+// wave loads its lanes' weight fragments once and keeps them in registers,
+// then walks 16-row tiles (grid-stride), so the weights are not re-read per
+// tile. Bt is the weight transposed, [16 NT][K] bf16 (row = output column).
 // A fragment: lane l holds X[l % 16][4 (l / 16) + 0..3]; B fragment: lane l
 // holds B[4 (l / 16) + 0..3][l % 16]; D: lane l holds Y[4 (l / 16) + 0..3][l % 16].
-template <int NT>
+// The k order (segment 0, then segment 1, 16 at a time) is fixed.
+template <int NT, int KS>
 __global__ __launch_bounds__(256) void k_dense(uint32_t M, Seg s0, Seg s1, const uint16_t *Bt,
-                                               uint32_t ktot, const float *bias, int relu,
-                                               uint16_t *yb, float *yf) {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t m0 = (blockIdx.x * 4 + wave) * 16;
-    if (m0 >= M) {
-        return;
-    }
+                                               const float *bias, int relu, uint16_t *yb,
+                                               float *yf) {
+    constexpr uint32_t K = 16 * KS;
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t ar = lane & 15u, aq = lane >> 4;
-    const bool arow_ok = m0 + ar < M;
-    v4f acc[NT];
+    v4s bw[KS][NT];
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const uint2 w = *reinterpret_cast<const uint2 *>(Bt + (size_t)(16 * t + ar) * K +
+                                                             16 * kk + 4 * aq);
+            bw[kk][t][0] = (short)(w.x & 0xffffu), bw[kk][t][1] = (short)(w.x >> 16);
+            bw[kk][t][2] = (short)(w.y & 0xffffu), bw[kk][t][3] = (short)(w.y >> 16);
+        }
+    float bc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        bc[t] = bias ? bias[16 * t + ar] : 0.0f;
     }
-    uint32_t koff = 0;
-    for (int sg = 0; sg < 2; ++sg) {
-        const Seg &s = sg ? s1 : s0;
-        for (uint32_t kk = 0; kk < s.k; kk += 16) {
+    const uint32_t ntile = (M + 15u) / 16u;
+    const uint32_t nwave = gridDim.x * 4u;
+    for (uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6); tile < ntile; tile += nwave) {
+        const uint32_t m0 = tile * 16u;
+        const bool arow_ok = m0 + ar < M;
+        v4f acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const uint32_t kg = 16u * kk;
+            const bool first = kg < s0.k;
             v4s a = v4s{0, 0, 0, 0};
             if (arow_ok) {
-                a = load_frag(s, m0 + ar, kk + 4 * aq);
+                a = first ? load_frag(s0, m0 + ar, kg + 4 * aq)
+                          : load_frag(s1, m0 + ar, kg - s0.k + 4 * aq);
             }
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                const uint2 w = *reinterpret_cast<const uint2 *>(
-                    Bt + (size_t)(16 * t + ar) * ktot + koff + kk + 4 * aq);
-                v4s b;
-                b[0] = (short)(w.x & 0xffffu), b[1] = (short)(w.x >> 16);
-                b[2] = (short)(w.y & 0xffffu), b[3] = (short)(w.y >> 16);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[kk][t], acc[t], 0, 0, 0);
             }
         }
-        koff += s.k;
-    }
-    const uint32_t ld = 16 * NT;
+        constexpr uint32_t ld = 16 * NT;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
+        for (int t = 0; t < NT; ++t) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const uint32_t row = m0 + 4 * aq + r, col = 16 * t + ar;
-            if (row < M) {
-                float v = acc[t][r] + (bias ? bias[col] : 0.0f);
-                if (relu) {
-                    v = fmaxf(v, 0.0f);
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t row = m0 + 4 * aq + r, col = 16 * t + ar;
+                if (row < M) {
+                    float v = acc[t][r] + bc[t];
+                    if (relu) {
+                        v = fmaxf(v, 0.0f);
+                    }
+                    if (yb) yb[(size_t)row * ld + col] = to_bf16(v);
+                    if (yf) yf[(size_t)row * ld + col] = v;
                 }
-                if (yb) yb[(size_t)row * ld + col] = to_bf16(v);
-                if (yf) yf[(size_t)row * ld + col] = v;
             }
         }
     }
 }
+
+// Workgroups of k_dense for M rows: enough 4-wave groups to cover the
+// tiles, at most kDenseGrid (a few rounds of resident waves on 256 CUs), so
+// each wave's register-held weights serve many tiles.
+constexpr uint32_t kDenseGrid = 2048;
+inline uint32_t dense_grid(uint32_t M) { return std::max(1u, std::min(kDenseGrid, (M + 63u) / 64u)); }
 
 // Weight-gradient partials: part[chunk][k][c] = sum over the chunk's rows m
 // of X[m][k] D[m][c] (X = the layer's input segments, D = [M][64] bf16), on
@@ -974,17 +994,16 @@ int Lpc::forward(hipStream_t s) {
     const uint32_t n = n_, E = E_;
     Seg a, none;
     a.x = x_, a.k = kFN, a.ld = kFN;
-    k_dense<4><<<cdiv(n, 64), 256, 0, s>>>(n, a, none, wt_ + dense_base(0), kFN,
-                                           w_ + LpcLayout::c0, 1, h_[0], nullptr);
+    k_dense<4, kFN / 16><<<dense_grid(n), 256, 0, s>>>(n, a, none, wt_ + dense_base(0),
+                                                       w_ + LpcLayout::c0, 1, h_[0], nullptr);
     PPF_HIP(hipGetLastError());
     for (uint32_t l = 0; l < kLpcLayers; ++l) {
         if (E) {
             Seg hs, fe;
             hs.x = h_[l], hs.idx = esrc_, hs.k = kH, hs.ld = kH;
             fe.x = f_, fe.k = kFE, fe.ld = kFE;
-            k_dense<4><<<cdiv(E, 64), 256, 0, s>>>(E, hs, fe, wt_ + dense_base(1 + 2 * l),
-                                                   kH + kFE, w_ + LpcLayout::ce(l), 1, m_[l],
-                                                   nullptr);
+            k_dense<4, (kH + kFE) / 16><<<dense_grid(E), 256, 0, s>>>(
+                E, hs, fe, wt_ + dense_base(1 + 2 * l), w_ + LpcLayout::ce(l), 1, m_[l], nullptr);
             PPF_HIP(hipGetLastError());
         }
         k_aggregate<<<cdiv(n * kH, kNT), kNT, 0, s>>>(n, eoff_, m_[l], M_[l]);
@@ -992,8 +1011,8 @@ int Lpc::forward(hipStream_t s) {
         Seg hn, mn;
         hn.x = h_[l], hn.k = kH, hn.ld = kH;
         mn.x = M_[l], mn.k = kH, mn.ld = kH;
-        k_dense<4><<<cdiv(n, 64), 256, 0, s>>>(n, hn, mn, wt_ + dense_base(2 + 2 * l), 2 * kH,
-                                               w_ + LpcLayout::cn(l), 1, h_[l + 1], nullptr);
+        k_dense<4, 2 * kH / 16><<<dense_grid(n), 256, 0, s>>>(
+            n, hn, mn, wt_ + dense_base(2 + 2 * l), w_ + LpcLayout::cn(l), 1, h_[l + 1], nullptr);
         PPF_HIP(hipGetLastError());
     }
     const uint32_t m = std::max(n, nnz_);
@@ -1083,8 +1102,8 @@ int Lpc::gradient(PcgPlanImpl *plan, const float *val, const float *cdiag, const
         dz.x = dZ_, dz.k = kH, dz.ld = kH;
         // dX = dZ Wn^T: Wn row-major [128][64] is the transposed operand of
         // this product (row = output column)
-        k_dense<8><<<cdiv(n, 64), 256, 0, s>>>(n, dz, none, wr_ + dense_base(2 + 2 * l), kH,
-                                               nullptr, 0, nullptr, dXn);
+        k_dense<8, kH / 16><<<dense_grid(n), 256, 0, s>>>(n, dz, none, wr_ + dense_base(2 + 2 * l),
+                                                          nullptr, 0, nullptr, dXn);
         PPF_HIP(hipGetLastError());
         // edge layer: m = relu([h_src, f] We + ce)
         if (E) {
@@ -1102,8 +1121,8 @@ int Lpc::gradient(PcgPlanImpl *plan, const float *val, const float *cdiag, const
             Seg dy;
             dy.x = dY_, dy.k = kH, dy.ld = kH;
             // input gradient of h_src only: the first 64 rows of We
-            k_dense<4><<<cdiv(E, 64), 256, 0, s>>>(E, dy, none, wr_ + dense_base(1 + 2 * l), kH,
-                                                   nullptr, 0, nullptr, dXe);
+            k_dense<4, kH / 16><<<dense_grid(E), 256, 0, s>>>(
+                E, dy, none, wr_ + dense_base(1 + 2 * l), nullptr, 0, nullptr, dXe);
             PPF_HIP(hipGetLastError());
         }
         k_scatter_src<<<cdiv(n * kH, kNT), kNT, 0, s>>>(n, eoff_, erev_, dXn, E ? dXe : dXn,
