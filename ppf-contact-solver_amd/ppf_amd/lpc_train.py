@@ -3,21 +3,25 @@ solver itself produces (the (Hessian, residual) pairs of its Newton
 iterations, as a linear-system dump records them). All compute is on the
 device, through the library's own forward, backward and Adam kernels.
 """
+import os
 import time
 
 import numpy as np
 import torch
 
-from . import scenes
+from . import lsys, scenes
 from .learned import LearnedPreconditioner, init_weights
 from .solver import BlockJacobiPCG
 from .stepper import Stepper, scene_to_device
 
 
-def collect_drape_systems(dev, res=64, layers=8, seeds=(0, 1, 2), steps=16, every=1):
+def collect_drape_systems(dev, res=64, layers=8, seeds=(0, 1, 2), steps=16, every=1,
+                          lsys_dir=None):
     """Run the stacked drape (block-Jacobi solver) for `steps` time steps per
     seed and keep the last Newton system of every `every`-th step (device
-    tensors: row_offset, col_index, value, cdiag, b)."""
+    tensors: row_offset, col_index, value, cdiag, b). With `lsys_dir`, each
+    kept system is also written there as an LSYS snapshot
+    (linsys_<seed>_<step>.bin, the reference dumper's format)."""
     out = []
     for seed in seeds:
         sc = scenes.stacked_drape_scene(res=res, layers=layers, seed=seed)
@@ -39,6 +43,8 @@ def collect_drape_systems(dev, res=64, layers=8, seeds=(0, 1, 2), steps=16, ever
             clock += r["dt"]
             if k % every == 0:
                 out.append(st.system())
+                if lsys_dir is not None:
+                    lsys.write(os.path.join(lsys_dir, f"linsys_{seed}_{k}.bin"), out[-1])
         st.close()
     return out
 
