@@ -164,3 +164,61 @@ def test_line_search_collision_mesh_kinds(dev, feature, mass):
     assert np.float32(toi0).view(np.uint32) == np.float32(rt0).view(np.uint32)
     assert rt0 > rt
     ls.close()
+
+
+def _comb_mesh(cluster=1000, seed=0):
+    """Tiny triangles whose centroids quantize to the Morton codes
+    2^30 - 2^b (b = 0..29) -- a comb: every split peels one of them off on
+    the left and continues on the right -- plus a run of equal codes at the
+    top, and one triangle that sweeps the whole box."""
+    pts = []
+    for b in range(30):
+        q = [1024 - 2 ** int(np.ceil(max(b - a, 0) / 3)) for a in range(3)]
+        pts.append([min(v, 1023) + 0.5 for v in q])
+    pts.append([0.25, 0.25, 0.25])
+    rng = np.random.default_rng(seed)
+    pts += list(1023.0 + 0.9 * rng.random((cluster, 3)))
+    tri = np.array([[0, 0, 0], [0.01, 0, 0], [0, 0.01, 0]])
+    x = (np.asarray(pts)[:, None, :] + tri[None]).reshape(-1, 3)
+    # a target in the run of equal codes, across the diagonal the mover's
+    # first vertex travels
+    c = 1023.3
+    target = np.array([[c - 0.01, c, c], [c + 0.01, c - 0.01, c], [c, c + 0.01, c]])
+    mover = np.array([[0.5, 0.5, 0.5], [0.51, 0.5, 0.5], [0.5, 0.51, 0.5]])
+    x = np.concatenate([x, target, mover]).astype(np.float32)
+    faces = np.arange(len(x)).reshape(-1, 3).astype(np.uint32)
+    m = scenes.contact_mesh(x, faces, ghat=1e-3, offset=0.0)
+    x1 = x.copy()
+    x1[-3:] += np.float32(1030.0)
+    return m, x1
+
+
+def _walk_stack_need(nodes, root):
+    need, st = 1, [root]
+    while st:
+        v = st.pop()
+        if nodes[v][1]:
+            need = max(need, len(st) + 2)
+            st += [nodes[v][0] - 1, nodes[v][1] - 1]
+    return need
+
+
+def test_line_search_deep_trees_take_the_long_stacks(dev):
+    # The comb's trees need more than the short first pass's 36 stack
+    # entries for a query that overlaps everything, and at most the 48 of
+    # the second pass: the first pass flags the overflow, the second runs
+    # every sweep again, and the result is the oracle's bit for bit.
+    m, x1 = _comb_mesh()
+    needs = [_walk_stack_need(*oracle.bvh_build(m, k)[::2]) for k in (0, 1, 2)]
+    print(f"walk stack entries needed (face, edge, vertex trees): {needs}")
+    assert max(needs) > 36 and max(needs) <= 48
+    ones = [np.ones(m[k], np.float32) for k in ("nv", "ne", "nf")]
+    t = mesh_to_device(m, dev)
+    ls = LineSearch()
+    toi, ov = ls(t, *_to(dev, x1.reshape(-1), *ones))
+    rt, ro = oracle.line_search(m, x1, *ones)
+    ls.close()
+    print(f"deep trees: toi gpu {toi!r} oracle {rt!r} overlap {ov}/{ro}")
+    assert np.float32(toi).view(np.uint32) == np.float32(rt).view(np.uint32)
+    assert ov == ro
+    assert 0.0 < toi < 1.0
